@@ -1,0 +1,178 @@
+/*
+ * mhfem.h -- C ABI of the MI355X-native MHFEM per-time-step path.
+ *
+ * One context (mhfem_ctx) owns one mesh, one model (n equations) and every
+ * device buffer of the scheme, the linear system and the Krylov solver, on
+ * the current HIP device, and launches all its kernels on one HIP stream.
+ * The entry points replace the per-step methods of the reference's
+ * TNL::MHFEM::Solver (include/TNL/MHFEM/Solver.h:64-117) instantiated with
+ * Device = TNL::Devices::Cuda; the C++ facade in include/mhfem/ mirrors that
+ * class and control.h on top of them.
+ *
+ * Conventions.  All pointers passed in are HOST pointers unless the name says
+ * "device".  Arrays use the reference's CUDA permutations (BaseModel.h:50-124):
+ *   Z_iF, m_iE_upw : E*n + i          Z_iK, m_iK, f_iK, R_iK : i*C + K
+ *   N_ijK, D_ijK, r_ijK : (i*n+j)*C + K
+ *   u_ijKe, a_ijKe, R_ijKe : ((i*n+j)*FPC + e)*C + K
+ *   w_iKe, v_iKe : (i*FPC + e)*C + K   Z_ijE_upw : (E*n + j)*n + i
+ *   b_ijK_storage : ((i*n+j)*S + s)*C + K   (S = 9 triangles, 14 tetrahedra)
+ * Boundary-condition arrays are indexed i*F + E (BoundaryConditions.hpp:295).
+ * Every function returns 0 on success or a negative MHFEM_E_* code; the
+ * message is available from mhfem_last_error().  A context is not
+ * thread-safe; distinct contexts are independent.
+ */
+#ifndef MHFEM_H
+#define MHFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhfem_ctx mhfem_ctx;
+
+enum {
+    MHFEM_OK = 0,
+    MHFEM_E_ARGUMENT = -1,
+    MHFEM_E_HIP = -2,
+    MHFEM_E_MESH = -3,
+    MHFEM_E_STATE = -4,
+    MHFEM_E_NOT_CONVERGED = -5,
+    MHFEM_E_BREAKDOWN = -6
+};
+
+/* MassLumping (MassMatrix.h:16-19), AdvectionDiscretization (BaseModel.h:12-16) */
+enum { MHFEM_LUMPING_DISABLED = 0, MHFEM_LUMPING_ENABLED = 1 };
+enum { MHFEM_EXPLICIT_UPWIND = 0, MHFEM_IMPLICIT_UPWIND = 1, MHFEM_IMPLICIT_TRACE = 2 };
+/* BoundaryConditionsType (BoundaryConditionsType.h:7-31) */
+enum {
+    MHFEM_BC_FIXED_VALUE = 0,
+    MHFEM_BC_FIXED_FLUX = 1,
+    MHFEM_BC_ADVECTIVE_OUTFLOW = 2,
+    MHFEM_BC_FIXED_FLUX_NO_ADVECTION = 3
+};
+
+/* Array identifiers for mhfem_set_array / mhfem_get_array / mhfem_device_array. */
+enum {
+    MHFEM_Z_iF = 0, MHFEM_Z_iK, MHFEM_N_ijK, MHFEM_u_ijKe, MHFEM_m_iK, MHFEM_D_ijK,
+    MHFEM_w_iKe, MHFEM_a_ijKe, MHFEM_r_ijK, MHFEM_f_iK,
+    MHFEM_v_iKe, MHFEM_m_iE_upw, MHFEM_Z_ijE_upw, MHFEM_b_ijK_storage, MHFEM_R_ijKe, MHFEM_R_iK,
+    MHFEM_BC_VALUES, MHFEM_BC_DIRICHLET, MHFEM_BC_MOBILITY, MHFEM_RHS,
+    MHFEM_CELL_MEASURE, MHFEM_FACE_MEASURE,
+    MHFEM_NUM_ARRAYS
+};
+
+typedef struct {
+    int dim;                    /* 2: triangles, 3: tetrahedra */
+    int num_points;
+    int num_cells;
+    const double *points;       /* num_points*dim */
+    const int32_t *cell_vertices; /* num_cells*(dim+1); local face e is opposite local vertex e */
+} mhfem_mesh_desc;
+
+typedef struct {
+    int num_equations;          /* n, 1..3 */
+    int mass_lumping;           /* MHFEM_LUMPING_* */
+    int advection;              /* MHFEM_*_UPWIND / MHFEM_IMPLICIT_TRACE */
+    int do_mobility_upwind;     /* BaseModel::do_mobility_upwind (BaseModel.h:163) */
+} mhfem_model_desc;
+
+/* Linear solver selection (configSetup.h:31-62). */
+enum { MHFEM_SOLVER_BICGSTAB = 0, MHFEM_SOLVER_GMRES = 1 };
+enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2 };
+/* precision of the Krylov iteration */
+enum { MHFEM_PRECISION_FP64 = 0, MHFEM_PRECISION_FP32_REFINED = 1 };
+/* initial guess of each solve */
+enum { MHFEM_GUESS_PREVIOUS = 0, MHFEM_GUESS_LEARNED = 1 };
+
+typedef struct {
+    int solver;                 /* MHFEM_SOLVER_* ("linear-solver") */
+    int preconditioner;         /* MHFEM_PREC_*   ("preconditioner") */
+    double convergence_residue; /* "convergence-residue" */
+    int max_iterations;         /* "max-iterations" */
+    int gmres_restart;          /* "gmres-restarting-max" */
+    int precision;              /* MHFEM_PRECISION_* */
+    double inner_residue;       /* FP32 inner tolerance for MHFEM_PRECISION_FP32_REFINED */
+    int check_every;            /* host convergence poll period (iterations), >= 1 */
+} mhfem_solver_config;
+
+typedef struct {
+    int iterations;             /* Krylov iterations of the last solve (inner, for IR) */
+    int outer_iterations;       /* refinement steps (1 for plain FP64) */
+    double residue;             /* final relative (preconditioned) residue */
+    int converged;
+} mhfem_solve_stats;
+
+/* Per-phase device times (ms, accumulated) under the reference's timer names
+ * (Solver.hpp:1353-1386). */
+typedef struct {
+    double pre_iterate, update_b_R_Q, upwind, assemble, preconditioner, linear_solver,
+        post_iterate;
+    long long iterations, time_steps;
+} mhfem_timers;
+
+/* ---- lifetime (Solver::setMesh, Solver.hpp:26-81; BaseModel::allocate, BaseModel.h:168-195) */
+mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *model);
+void mhfem_destroy(mhfem_ctx *ctx);
+const char *mhfem_last_error(const mhfem_ctx *ctx); /* ctx may be NULL: error of the last create */
+
+/* ---- topology (mesh_helpers.h:49-265) */
+int mhfem_get_counts(const mhfem_ctx *ctx, int *cells, int *faces, int *boundary_faces);
+/* host copies: cell_faces[C*FPC], face_cells[F*2] (second = -1 on the boundary) */
+int mhfem_get_topology(const mhfem_ctx *ctx, int32_t *cell_faces, int32_t *face_cells);
+
+/* ---- model data (BaseModel.h:266-286, UniformCoefficient.h:7-53):
+ * count == full size uploads the array, count == 1 makes a coefficient
+ * uniform (one broadcast value). */
+int mhfem_set_array(mhfem_ctx *ctx, int which, const double *host, long count);
+int mhfem_get_array(mhfem_ctx *ctx, int which, double *host, long count);
+long mhfem_array_size(const mhfem_ctx *ctx, int which);
+/* device pointer of an array (for in-place updates by model hooks on the
+ * context's stream); NULL for uniform coefficients */
+void *mhfem_device_array(mhfem_ctx *ctx, int which);
+void *mhfem_stream(mhfem_ctx *ctx); /* the hipStream_t of the context */
+
+/* ---- boundary conditions (BoundaryConditions::init, BoundaryConditions.hpp:272-281;
+ * BoundaryConditionsStorage.h:9-15); arrays of n*F in the i*F+E layout */
+int mhfem_set_boundary_conditions(mhfem_ctx *ctx, const uint8_t *tags, const double *values,
+                                  const double *dirichlet_values);
+
+/* ---- Solver::setInitialCondition (Solver.hpp:255-344) for component i
+ * (BaseModel::setInitialCondition, BaseModel.h:316-326) */
+int mhfem_set_initial_condition(mhfem_ctx *ctx, int i, const double *Z_K, long count);
+/* tail of setInitialCondition: Z_iF := face averages of Z_iK, v_iKe := 0
+ * (Solver.hpp:319-341) */
+int mhfem_init(mhfem_ctx *ctx);
+
+/* ---- Solver::setup + setupLinearSystem (Solver.hpp:83-253, 346-485):
+ * solver/preconditioner choice and the fixed CSR pattern */
+int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg);
+
+/* ---- one time step (control.h:74-101) */
+int mhfem_pre_iterate(mhfem_ctx *ctx, double time, double tau);     /* Solver.hpp:600-892 */
+int mhfem_assemble(mhfem_ctx *ctx, double time, double tau);        /* Solver.hpp:894-962 */
+int mhfem_solve(mhfem_ctx *ctx, mhfem_solve_stats *stats);          /* Solver.hpp:979-1266 */
+int mhfem_post_iterate(mhfem_ctx *ctx, double time, double tau);    /* Solver.hpp:1268-1324 */
+/* control.h::solve(problem, t0, t1, dt) (control.h:64-102); stats of the last step */
+int mhfem_solve_interval(mhfem_ctx *ctx, double t0, double t1, double dt,
+                         mhfem_solve_stats *last);
+
+/* ---- linear system export (Solver::saveLinearSystem, Solver.hpp:964-977):
+ * rowptr[rows+1], col[nnz], val[nnz], rhs[rows]; any pointer may be NULL */
+int mhfem_system_size(const mhfem_ctx *ctx, int *rows, long *nnz);
+int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *val, double *rhs);
+/* solve an explicit right-hand side with the current matrix (testing aid) */
+int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats);
+
+/* ---- instrumentation (Solver::writeEpilog, Solver.hpp:1326-1391) */
+int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out);
+int mhfem_reset_timers(mhfem_ctx *ctx);
+int mhfem_set_timing(mhfem_ctx *ctx, int enabled); /* HIP-event phase timers (default off) */
+int mhfem_synchronize(mhfem_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

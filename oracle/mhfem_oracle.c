@@ -1,0 +1,1152 @@
+/*
+ * mhfem_oracle.c -- CPU restatement of the TNL-MHFEM host path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation and the
+ * timed CPU baseline.  Nothing in the product library links or loads it.
+ *
+ * Every function cites the reference file:line it follows (paths relative to
+ * the TNL-MHFEM tree).  Arithmetic that lives in TNL (not vendored in the
+ * reference: face numbering, LU_sequential_*, BiCGStab, GMRES, ILU(0)) is
+ * restated from the named published methods; those parts are pinned only by
+ * mathematical checks in tests/ (closed forms, quadrature, brute-force mixed
+ * systems, scipy), never by a TNL artefact.  Face numbering and the Krylov
+ * iteration sequence are "parity unpinned" with respect to TNL itself.
+ */
+#include "mhfem_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct or_problem {
+    int dim, fpc, n, S;
+    int nv, nc, nf;
+    int lumping, advection, do_mob;
+    double *pts;
+    int *cv;
+    int *cell_faces, *face_cells;
+    double *cell_meas, *face_meas;
+    /* model (BaseModel.h:266-286) */
+    double *Z_iF, *Z_iK, *N, *u, *m, *D, *w, *a, *r, *f;
+    double *v, *m_upw, *Z_upw, *b, *R_ijKe, *R_iK;
+    /* boundary conditions (BoundaryConditions.h:76-82) + model boundary mobility */
+    uint8_t *tags;
+    double *bc_values, *bc_dirichlet, *bc_mobility;
+    /* linear system */
+    int nrows;
+    long nnz;
+    int *rowptr, *colidx;
+    double *val, *rhs;
+    char err[256];
+};
+
+/* ------------------------------------------------------------------ */
+/* indexing in the reference's CUDA permutations (BaseModel.h:50-124)  */
+#define C_ (p->nc)
+#define F_ (p->nf)
+#define NN (p->n)
+#define FP (p->fpc)
+#define ZF(j, E) p->Z_iF[(long)(E) * NN + (j)]
+#define ZK(i, K) p->Z_iK[(long)(i) * C_ + (K)]
+#define IJK(i, j, K) (((long)(i) * NN + (j)) * C_ + (K))
+#define IJKE(i, j, K, e) ((((long)(i) * NN + (j)) * FP + (e)) * C_ + (K))
+#define IK(i, K) ((long)(i) * C_ + (K))
+#define IKE(i, K, e) (((long)(i) * FP + (e)) * C_ + (K))
+#define MUPW(i, E) p->m_upw[(long)(E) * NN + (i)]
+#define ZUPW(i, j, E) p->Z_upw[((long)(E) * NN + (j)) * NN + (i)]
+#define BST(i, j, K, s) p->b[(((long)(i) * NN + (j)) * p->S + (s)) * C_ + (K)]
+#define BCI(i, E) ((long)(i) * F_ + (E))
+
+/* MassMatrix::b_ijKef -- packed upper triangle (MassMatrix.h:613-616, 884-887) */
+static inline double b_ef(const or_problem *p, int i, int j, int K, int e, int f)
+{
+    if (e > f) { int t = e; e = f; f = t; }
+    return BST(i, j, K, e + (f * (f + 1)) / 2);
+}
+/* MassMatrix::b_ijKe -- row sums (MassMatrix.h:630, 901) */
+static inline double b_e(const or_problem *p, int i, int j, int K, int e)
+{
+    return BST(i, j, K, p->S - p->fpc + e);
+}
+
+/* ------------------------------------------------------------------ */
+/* Topology.  Restates TNL's mesh initializer as used through
+ * mesh_helpers.h:49-59,151-160,235-241: faces are numbered in the order in
+ * which they are first met when cells are visited in index order and the
+ * faces of each cell in local order; local face e is the face opposite local
+ * vertex e (MassMatrix.h:538,661,787,932); the cells of a face are listed in
+ * ascending order.  Parity unpinned w.r.t. TNL (the initializer is not in the
+ * reference tree); pinned in tests by invariants and closed-form counts. */
+static unsigned long hash3(const int *k, int d)
+{
+    unsigned long h = 1469598103934665603UL;
+    for (int t = 0; t < d; t++) {
+        h ^= (unsigned long)(unsigned)k[t];
+        h *= 1099511628211UL;
+    }
+    return h ^ (h >> 29);
+}
+
+int or_topology(int dim, int nv, const double *points, int nc, const int *cv, int *cell_faces,
+                int *face_cells)
+{
+    (void)points;
+    const int fpc = dim + 1, d = dim;
+    const long maxf = (long)nc * fpc;
+    long cap = 1;
+    while (cap < 2 * maxf) cap <<= 1;
+    int *table = malloc(cap * sizeof(int));
+    int *keys = malloc(maxf * d * sizeof(int));
+    if (!table || !keys) { free(table); free(keys); return -1; }
+    memset(table, 0xff, cap * sizeof(int));
+    int nf = 0, bad = 0;
+    for (int K = 0; K < nc && !bad; K++) {
+        for (int e = 0; e < fpc; e++) {
+            int key[3], t = 0;
+            for (int v = 0; v < fpc; v++)
+                if (v != e) key[t++] = cv[(long)K * fpc + v];
+            /* sort the seed (insertion sort, d <= 3) */
+            for (int a = 1; a < d; a++)
+                for (int c = a; c > 0 && key[c - 1] > key[c]; c--) {
+                    int s = key[c]; key[c] = key[c - 1]; key[c - 1] = s;
+                }
+            for (int a = 0; a < d; a++)
+                if (key[a] < 0 || key[a] >= nv) bad = 1;
+            unsigned long h = hash3(key, d) & (cap - 1);
+            int id = -1;
+            while (table[h] >= 0) {
+                const int *kk = keys + (long)table[h] * d;
+                int eq = 1;
+                for (int a = 0; a < d; a++) eq &= (kk[a] == key[a]);
+                if (eq) { id = table[h]; break; }
+                h = (h + 1) & (cap - 1);
+            }
+            if (id < 0) {
+                id = nf++;
+                memcpy(keys + (long)id * d, key, d * sizeof(int));
+                table[h] = id;
+                face_cells[2L * id] = K;
+                face_cells[2L * id + 1] = -1;
+            } else {
+                if (face_cells[2L * id + 1] != -1) bad = 1; /* more than 2 cells */
+                face_cells[2L * id + 1] = K;
+            }
+            cell_faces[(long)K * fpc + e] = id;
+        }
+    }
+    free(table);
+    free(keys);
+    return bad ? -1 : nf;
+}
+
+static double simplex_measure(int dim, const double *v /* (dim+1)*dim */)
+{
+    if (dim == 2) {
+        double ax = v[2] - v[0], ay = v[3] - v[1], bx = v[4] - v[0], by = v[5] - v[1];
+        return 0.5 * fabs(ax * by - ay * bx);
+    }
+    double a[3], b[3], c[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = v[3 + k] - v[k];
+        b[k] = v[6 + k] - v[k];
+        c[k] = v[9 + k] - v[k];
+    }
+    double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                 a[2] * (b[0] * c[1] - b[1] * c[0]);
+    return fabs(det) / 6.0;
+}
+
+static double face_measure(int dim, const double *v /* dim points of dim coords */)
+{
+    if (dim == 2) {
+        double dx = v[2] - v[0], dy = v[3] - v[1];
+        return sqrt(dx * dx + dy * dy);
+    }
+    double a[3], b[3];
+    for (int k = 0; k < 3; k++) { a[k] = v[3 + k] - v[k]; b[k] = v[6 + k] - v[k]; }
+    double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+/* ------------------------------------------------------------------ */
+/* LU without pivoting (Doolittle, in place) and the in-place solve: the
+ * published method restated for TNL's LU_sequential_factorize /
+ * LU_sequential_solve_inplace (MassMatrix.h:571-592, Solver.hpp:849-866). */
+static void lu_factorize(double *A, int n)
+{
+    for (int k = 0; k < n; k++) {
+        for (int i = k + 1; i < n; i++) {
+            A[i * n + k] /= A[k * n + k];
+            for (int j = k + 1; j < n; j++) A[i * n + j] -= A[i * n + k] * A[k * n + j];
+        }
+    }
+}
+static void lu_solve(const double *A, int n, double *x)
+{
+    for (int i = 1; i < n; i++)
+        for (int j = 0; j < i; j++) x[i] -= A[i * n + j] * x[j];
+    for (int i = n - 1; i >= 0; i--) {
+        for (int j = i + 1; j < n; j++) x[i] -= A[i * n + j] * x[j];
+        x[i] /= A[i * n + i];
+    }
+}
+
+/* Geometric local mass matrix M_K of RTN0 (MassMatrix.h:543-569 triangle,
+ * :666-695 triangle lumped, :793-831 tetrahedron, :938-978 tetrahedron
+ * lumped), its inverse by LU column solves, scaled by D, stored upper-packed
+ * column by column, followed by the row sums (:594-597, :864-868).
+ * Pinned in tests against exact quadrature of the RTN0 basis. */
+void or_local_mass_inverse(int dim, int lumping, const double *P, double measure, double D,
+                           double *out)
+{
+    double M[16];
+    const int n = dim + 1;
+    if (dim == 2) {
+        if (!lumping) {
+            double P0[2] = {P[0] - P[4], P[1] - P[5]}, P1[2] = {P[2] - P[4], P[3] - P[5]};
+            double P00 = P0[0] * P0[0] + P0[1] * P0[1];
+            double P01 = P0[0] * P1[0] + P0[1] * P1[1];
+            double P11 = P1[0] * P1[0] + P1[1] * P1[1];
+            double den = 24 * measure;
+            M[0] = (3 * P00 + P11 - 3 * P01) / den;
+            M[4] = (P00 + 3 * P11 - 3 * P01) / den;
+            M[8] = (P00 + P11 + P01) / den;
+            M[1] = (3 * P01 - P00 - P11) / den;
+            M[2] = (P11 - P00 - P01) / den;
+            M[5] = (P00 - P11 - P01) / den;
+        } else {
+            double a01[2] = {P[0] - P[2], P[1] - P[3]}, a02[2] = {P[0] - P[4], P[1] - P[5]},
+                   a12[2] = {P[2] - P[4], P[3] - P[5]};
+#define D2(x, y) ((x)[0] * (y)[0] + (x)[1] * (y)[1])
+            double den = 12 * measure;
+            M[0] = (D2(a01, a01) + D2(a02, a02)) / den;
+            M[4] = (D2(a01, a01) + D2(a12, a12)) / den;
+            M[8] = (D2(a02, a02) + D2(a12, a12)) / den;
+            M[1] = (D2(a02, a12)) / den;
+            M[2] = (-D2(a01, a12)) / den;
+            M[5] = (D2(a01, a02)) / den;
+#undef D2
+        }
+        M[3] = M[1]; M[6] = M[2]; M[7] = M[5];
+    } else {
+#define D3(x, y) ((x)[0] * (y)[0] + (x)[1] * (y)[1] + (x)[2] * (y)[2])
+        if (!lumping) {
+            double P0[3], P1[3], P2[3];
+            for (int k = 0; k < 3; k++) {
+                P0[k] = P[k] - P[9 + k];
+                P1[k] = P[3 + k] - P[9 + k];
+                P2[k] = P[6 + k] - P[9 + k];
+            }
+            double P00 = D3(P0, P0), P11 = D3(P1, P1), P22 = D3(P2, P2);
+            double P01 = D3(P0, P1), P02 = D3(P0, P2), P12 = D3(P1, P2);
+            double den = 180 * measure;
+            M[0] = (12 * P00 + 2 * P11 + 2 * P22 - 8 * P01 - 8 * P02 + 2 * P12) / den;
+            M[5] = (2 * P00 + 12 * P11 + 2 * P22 - 8 * P01 + 2 * P02 - 8 * P12) / den;
+            M[10] = (2 * P00 + 2 * P11 + 12 * P22 + 2 * P01 - 8 * P02 - 8 * P12) / den;
+            M[15] = 2 * (P00 + P11 + P22 + P01 + P02 + P12) / den;
+            M[1] = (-3 * P00 - 3 * P11 + 2 * P22 + 12 * P01 - 3 * P02 - 3 * P12) / den;
+            M[2] = (-3 * P00 + 2 * P11 - 3 * P22 - 3 * P01 + 12 * P02 - 3 * P12) / den;
+            M[6] = (2 * P00 - 3 * P11 - 3 * P22 - 3 * P01 - 3 * P02 + 12 * P12) / den;
+            M[3] = (-3 * P00 + 2 * P11 + 2 * P22 - 3 * P01 - 3 * P02 + 2 * P12) / den;
+            M[7] = (2 * P00 - 3 * P11 + 2 * P22 - 3 * P01 + 2 * P02 - 3 * P12) / den;
+            M[11] = (2 * P00 + 2 * P11 - 3 * P22 + 2 * P01 - 3 * P02 - 3 * P12) / den;
+        } else {
+            double a01[3], a02[3], a03[3], a12[3], a13[3], a23[3];
+            for (int k = 0; k < 3; k++) {
+                a01[k] = P[k] - P[3 + k];
+                a02[k] = P[k] - P[6 + k];
+                a03[k] = P[k] - P[9 + k];
+                a12[k] = P[3 + k] - P[6 + k];
+                a13[k] = P[3 + k] - P[9 + k];
+                a23[k] = P[6 + k] - P[9 + k];
+            }
+            double den = 36 * measure;
+            M[0] = (D3(a01, a01) + D3(a02, a02) + D3(a03, a03)) / den;
+            M[5] = (D3(a01, a01) + D3(a12, a12) + D3(a13, a13)) / den;
+            M[10] = (D3(a02, a02) + D3(a12, a12) + D3(a23, a23)) / den;
+            M[15] = (D3(a03, a03) + D3(a13, a13) + D3(a23, a23)) / den;
+            M[1] = (D3(a02, a12) + D3(a03, a13)) / den;
+            M[2] = (-D3(a01, a12) + D3(a03, a23)) / den;
+            M[3] = (-D3(a01, a13) - D3(a02, a23)) / den;
+            M[6] = (D3(a01, a02) + D3(a13, a23)) / den;
+            M[7] = (D3(a01, a03) - D3(a12, a23)) / den;
+            M[11] = (D3(a02, a03) + D3(a12, a13)) / den;
+        }
+#undef D3
+        M[4] = M[1]; M[8] = M[2]; M[9] = M[6]; M[12] = M[3]; M[13] = M[7]; M[14] = M[11];
+    }
+    lu_factorize(M, n);
+    int s = 0;
+    for (int c = 0; c < n; c++) {
+        double vv[4] = {0, 0, 0, 0};
+        vv[c] = 1.0;
+        lu_solve(M, n, vv);
+        for (int r = 0; r <= c; r++) out[s++] = vv[r] * D;
+    }
+    /* row sums in ascending f (MassMatrix.h:595-597, 865-868) */
+    const int base = s;
+    for (int e = 0; e < n; e++) {
+        double sum = 0;
+        for (int f = 0; f < n; f++) {
+            int a = e < f ? e : f, bb = e < f ? f : e;
+            sum = (f == 0) ? out[a + bb * (bb + 1) / 2] : sum + out[a + bb * (bb + 1) / 2];
+        }
+        out[base + e] = sum;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+or_problem *or_create(int dim, int nv, const double *points, int nc, const int *cv, int n,
+                      int lumping, int advection, int do_mob)
+{
+    if (dim != 2 && dim != 3) return NULL;
+    or_problem *p = calloc(1, sizeof(or_problem));
+    p->dim = dim;
+    p->fpc = dim + 1;
+    p->S = dim == 2 ? 9 : 14;
+    p->n = n;
+    p->nv = nv;
+    p->nc = nc;
+    p->lumping = lumping;
+    p->advection = advection;
+    p->do_mob = do_mob;
+    p->pts = malloc(sizeof(double) * nv * dim);
+    memcpy(p->pts, points, sizeof(double) * nv * dim);
+    p->cv = malloc(sizeof(int) * (long)nc * p->fpc);
+    memcpy(p->cv, cv, sizeof(int) * (long)nc * p->fpc);
+    p->cell_faces = malloc(sizeof(int) * (long)nc * p->fpc);
+    int *fc = malloc(sizeof(int) * 2L * nc * p->fpc);
+    int nf = or_topology(dim, nv, points, nc, cv, p->cell_faces, fc);
+    if (nf < 0) {
+        free(fc);
+        or_destroy(p);
+        return NULL;
+    }
+    p->nf = nf;
+    p->face_cells = realloc(fc, sizeof(int) * 2L * nf);
+    p->cell_meas = malloc(sizeof(double) * nc);
+    p->face_meas = malloc(sizeof(double) * nf);
+    const int fpc = p->fpc;
+#pragma omp parallel for
+    for (int K = 0; K < nc; K++) {
+        double vv[12];
+        for (int k = 0; k < fpc; k++)
+            for (int c = 0; c < dim; c++) vv[k * dim + c] = points[(long)cv[(long)K * fpc + k] * dim + c];
+        p->cell_meas[K] = simplex_measure(dim, vv);
+        for (int e = 0; e < fpc; e++) {
+            double fv[9];
+            int t = 0;
+            for (int k = 0; k < fpc; k++)
+                if (k != e) {
+                    for (int c = 0; c < dim; c++) fv[t * dim + c] = vv[k * dim + c];
+                    t++;
+                }
+            int E = p->cell_faces[(long)K * fpc + e];
+            if (p->face_cells[2L * E] == K) p->face_meas[E] = face_measure(dim, fv);
+        }
+    }
+    const long C = nc, F = nf;
+    p->Z_iF = calloc(n * F, sizeof(double));
+    p->Z_iK = calloc(n * C, sizeof(double));
+    p->N = calloc(n * n * C, sizeof(double));
+    p->u = calloc(n * n * fpc * C, sizeof(double));
+    p->m = calloc(n * C, sizeof(double));
+    p->D = calloc(n * n * C, sizeof(double));
+    p->w = calloc(n * fpc * C, sizeof(double));
+    p->a = calloc(n * n * fpc * C, sizeof(double));
+    p->r = calloc(n * n * C, sizeof(double));
+    p->f = calloc(n * C, sizeof(double));
+    p->v = calloc(n * fpc * C, sizeof(double));
+    p->m_upw = calloc(n * F, sizeof(double));
+    p->Z_upw = calloc(n * n * F, sizeof(double));
+    p->b = calloc(n * n * p->S * C, sizeof(double));
+    p->R_ijKe = calloc(n * n * fpc * C, sizeof(double));
+    p->R_iK = calloc(n * C, sizeof(double));
+    p->tags = calloc(n * F, 1);
+    p->bc_values = calloc(n * F, sizeof(double));
+    p->bc_dirichlet = calloc(n * F, sizeof(double));
+    p->bc_mobility = calloc(n * F, sizeof(double));
+    for (long k = 0; k < n * F; k++) p->m_upw[k] = 1.0;
+    return p;
+}
+
+void or_destroy(or_problem *p)
+{
+    if (!p) return;
+    void *ptrs[] = {p->pts, p->cv, p->cell_faces, p->face_cells, p->cell_meas, p->face_meas,
+                    p->Z_iF, p->Z_iK, p->N, p->u, p->m, p->D, p->w, p->a, p->r, p->f, p->v,
+                    p->m_upw, p->Z_upw, p->b, p->R_ijKe, p->R_iK, p->tags, p->bc_values,
+                    p->bc_dirichlet, p->bc_mobility, p->rowptr, p->colidx, p->val, p->rhs};
+    for (unsigned k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
+    free(p);
+}
+
+const char *or_error(const or_problem *p) { return p ? p->err : "null problem"; }
+int or_num_cells(const or_problem *p) { return p->nc; }
+int or_num_faces(const or_problem *p) { return p->nf; }
+
+void *or_array(or_problem *p, const char *name, long *size)
+{
+    const long C = p->nc, F = p->nf, n = p->n, fpc = p->fpc;
+    struct { const char *nm; void *ptr; long sz; } tab[] = {
+        {"Z_iF", p->Z_iF, n * F}, {"Z_iK", p->Z_iK, n * C}, {"N_ijK", p->N, n * n * C},
+        {"u_ijKe", p->u, n * n * fpc * C}, {"m_iK", p->m, n * C}, {"D_ijK", p->D, n * n * C},
+        {"w_iKe", p->w, n * fpc * C}, {"a_ijKe", p->a, n * n * fpc * C}, {"r_ijK", p->r, n * n * C},
+        {"f_iK", p->f, n * C}, {"v_iKe", p->v, n * fpc * C}, {"m_iE_upw", p->m_upw, n * F},
+        {"Z_ijE_upw", p->Z_upw, n * n * F}, {"b_ijK_storage", p->b, n * n * p->S * C},
+        {"R_ijKe", p->R_ijKe, n * n * fpc * C}, {"R_iK", p->R_iK, n * C},
+        {"cell_measure", p->cell_meas, C}, {"face_measure", p->face_meas, F},
+        {"tags", p->tags, n * F}, {"bc_values", p->bc_values, n * F},
+        {"bc_dirichlet", p->bc_dirichlet, n * F}, {"bc_mobility", p->bc_mobility, n * F},
+        {"cell_faces", p->cell_faces, C * fpc}, {"face_cells", p->face_cells, 2 * F},
+        {"rhs", p->rhs, p->rhs ? (long)p->nrows : 0},
+    };
+    for (unsigned k = 0; k < sizeof(tab) / sizeof(tab[0]); k++)
+        if (!strcmp(tab[k].nm, name)) {
+            if (size) *size = tab[k].sz;
+            return tab[k].ptr;
+        }
+    if (size) *size = 0;
+    return NULL;
+}
+
+static inline int local_index(const or_problem *p, int K, int E)
+{
+    /* getLocalIndex (mesh_helpers.h:253-265) */
+    for (int e = 0; e < p->fpc; e++)
+        if (p->cell_faces[(long)K * p->fpc + e] == E) return e;
+    return 0;
+}
+
+/* Solver::setInitialCondition, face-average kernel (Solver.hpp:319-341) */
+void or_init(or_problem *p)
+{
+    const int n = p->n;
+#pragma omp parallel for
+    for (int E = 0; E < p->nf; E++) {
+        int K0 = p->face_cells[2L * E], K1 = p->face_cells[2L * E + 1];
+        for (int i = 0; i < n; i++) {
+            if (K1 < 0) ZF(i, E) = ZK(i, K0);
+            else ZF(i, E) = 0.5 * (ZK(i, K0) + ZK(i, K1));
+        }
+    }
+    memset(p->v, 0, sizeof(double) * (long)n * p->fpc * p->nc);
+}
+
+/* ------------------------------------------------------------------ */
+/* Row lengths and CSR pattern: Solver.hpp:448-484, LinearSystem.h:26-34,
+ * BoundaryConditions.hpp:283-299.  Row index E*n+i (BaseModel.h:259-263).
+ * Column order: interior rows merge the two cells' face lists in ascending
+ * face index (LinearSystem.h:108-246), boundary rows the sorted single-cell
+ * list (BoundaryConditions.hpp:157-218); within a face, j ascending. */
+static void sorted_faces(const or_problem *p, int K, int *perm)
+{
+    const int *fi = p->cell_faces + (long)K * p->fpc;
+    for (int j = 0; j < p->fpc; j++) perm[j] = j;
+    /* bubble sort of the local permutation (LinearSystem.h:129-136) */
+    for (int k1 = p->fpc - 1; k1 > 0; k1--)
+        for (int k2 = 0; k2 < k1; k2++)
+            if (!(fi[perm[k2]] < fi[perm[k2 + 1]])) {
+                int t = perm[k2]; perm[k2] = perm[k2 + 1]; perm[k2 + 1] = t;
+            }
+}
+
+int or_setup_linear_system(or_problem *p)
+{
+    const int n = p->n, fpc = p->fpc;
+    const long F = p->nf;
+    p->nrows = (int)(n * F);
+    free(p->rowptr); free(p->colidx); free(p->val); free(p->rhs);
+    p->rowptr = malloc(sizeof(int) * (p->nrows + 1));
+    p->rhs = calloc(p->nrows, sizeof(double));
+    long nnz = 0;
+    p->rowptr[0] = 0;
+    for (long E = 0; E < F; E++)
+        for (int i = 0; i < n; i++) {
+            int len;
+            if (p->face_cells[2 * E + 1] < 0)
+                len = (p->tags[BCI(i, E)] == OR_FIXED_VALUE) ? 1 : fpc * n;
+            else
+                len = (2 * fpc - 1) * n;
+            if (len <= 0) {
+                snprintf(p->err, sizeof p->err, "MHFEM error: attempted to set invalid rowsLengths vector");
+                return -1;
+            }
+            nnz += len;
+            p->rowptr[E * n + i + 1] = (int)nnz;
+        }
+    p->nnz = nnz;
+    p->colidx = malloc(sizeof(int) * nnz);
+    p->val = calloc(nnz, sizeof(double));
+#pragma omp parallel for
+    for (long E = 0; E < F; E++) {
+        int K0 = p->face_cells[2 * E], K1 = p->face_cells[2 * E + 1];
+        for (int i = 0; i < n; i++) {
+            int *c = p->colidx + p->rowptr[E * n + i];
+            if (K1 < 0) {
+                if (p->tags[BCI(i, E)] == OR_FIXED_VALUE) { c[0] = (int)(E * n + i); continue; }
+                int perm[4];
+                sorted_faces(p, K0, perm);
+                int t = 0;
+                for (int g = 0; g < fpc; g++)
+                    for (int j = 0; j < n; j++) c[t++] = p->cell_faces[(long)K0 * fpc + perm[g]] * n + j;
+            } else {
+                int p0[4], p1[4], g0 = 0, g1 = 0, t = 0;
+                const int *f0 = p->cell_faces + (long)K0 * fpc, *f1 = p->cell_faces + (long)K1 * fpc;
+                sorted_faces(p, K0, p0);
+                sorted_faces(p, K1, p1);
+                while (g0 < fpc || g1 < fpc) {
+                    int F0 = g0 < fpc ? f0[p0[g0]] : 0x7fffffff, F1 = g1 < fpc ? f1[p1[g1]] : 0x7fffffff;
+                    int Fm = F0 < F1 ? F0 : F1;
+                    for (int j = 0; j < n; j++) c[t++] = Fm * n + j;
+                    if (F0 == Fm) g0++;
+                    if (F1 == Fm) g1++;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* SecondaryCoefficients (SecondaryCoefficients.h) */
+static double A_no_adv(const or_problem *p, int i, int j, int K, int e, int f)
+{
+    /* A_ijKEF_no_advection (SecondaryCoefficients.h:189-211) */
+    double SQinvR = 0;
+    for (int x = 0; x < p->n; x++) SQinvR += b_e(p, i, x, K, e) * p->R_ijKe[IJKE(x, j, K, f)];
+    return b_ef(p, i, j, K, e, f) - SQinvR;
+}
+
+static double A_adv(const or_problem *p, int i, int j, int K, int E, int e, int F, int f)
+{
+    /* A_ijKEF_advection (SecondaryCoefficients.h:215-259) */
+    const double mu = MUPW(i, E);
+    double SQinvR = 0;
+    for (int x = 0; x < p->n; x++) {
+        double S = mu * b_e(p, i, x, K, e);
+        if (p->advection == OR_IMPLICIT_UPWIND) {
+            double vel = p->a[IJKE(i, x, K, e)] + p->u[IJKE(i, x, K, e)];
+            if (vel >= 0) S += vel; else S -= vel;
+        }
+        SQinvR += S * p->R_ijKe[IJKE(x, j, K, f)];
+    }
+    double T = mu * b_ef(p, i, j, K, e, f);
+    if (p->advection == OR_IMPLICIT_UPWIND && E == F) {
+        double vel = p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)];
+        if (vel < 0) T -= 2 * vel;
+    }
+    if (p->advection == OR_IMPLICIT_TRACE && E == F) {
+        double vel = p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)];
+        T -= vel;
+    }
+    return T - SQinvR;
+}
+
+static double A_interior(const or_problem *p, int i, int j, int K, int E, int e, int F, int f)
+{
+    /* A_ijKEF (SecondaryCoefficients.h:261-278) */
+    if (p->advection == OR_EXPLICIT_UPWIND) return A_no_adv(p, i, j, K, e, f);
+    return A_adv(p, i, j, K, E, e, F, f);
+}
+
+static double RHS_no_adv(const or_problem *p, int i, int K, int e)
+{
+    /* RHS_iKE_no_advection (SecondaryCoefficients.h:282-297) */
+    double value = p->w[IKE(i, K, e)];
+    for (int j = 0; j < p->n; j++) value += b_e(p, i, j, K, e) * p->R_iK[IK(j, K)];
+    return value;
+}
+
+static double RHS_adv(const or_problem *p, int i, int K, int E, int e)
+{
+    /* RHS_iKE_advection (SecondaryCoefficients.h:301-333) */
+    const double mu = MUPW(i, E);
+    double value = mu * p->w[IKE(i, K, e)];
+    for (int j = 0; j < p->n; j++) {
+        double S = mu * b_e(p, i, j, K, e);
+        const double vel = p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)];
+        if (p->advection == OR_IMPLICIT_UPWIND) {
+            if (vel >= 0) S += vel; else S -= vel;
+        }
+        value += S * p->R_iK[IK(j, K)];
+        if (p->advection == OR_EXPLICIT_UPWIND) value += ZUPW(i, j, E) * vel;
+    }
+    return value;
+}
+
+static double RHS_interior(const or_problem *p, int i, int K, int E, int e)
+{
+    /* RHS_iKE (SecondaryCoefficients.h:337-349) */
+    if (p->advection == OR_EXPLICIT_UPWIND) return RHS_no_adv(p, i, K, e);
+    return RHS_adv(p, i, K, E, e);
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver::preIterate (Solver.hpp:600-892) */
+void or_pre_iterate(or_problem *p, double time, double tau)
+{
+    (void)time;
+    const int n = p->n, fpc = p->fpc, dim = p->dim;
+    const int C = p->nc, F = p->nf;
+    /* K2: MassMatrix::update over (K,i,j) (Solver.hpp:627-639) */
+#pragma omp parallel for
+    for (int K = 0; K < C; K++) {
+        double P[12], out[14];
+        for (int k = 0; k < fpc; k++)
+            for (int c = 0; c < dim; c++) P[k * dim + c] = p->pts[(long)p->cv[(long)K * fpc + k] * dim + c];
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                or_local_mass_inverse(dim, p->lumping, P, p->cell_meas[K], p->D[IJK(i, j, K)], out);
+                for (int s = 0; s < p->S; s++) BST(i, j, K, s) = out[s];
+            }
+    }
+    /* K4: upwinded mobility over (E,i) (Solver.hpp:663-727) */
+    if (p->do_mob) {
+#pragma omp parallel for
+        for (int E = 0; E < F; E++) {
+            const int K1 = p->face_cells[2L * E], K2 = p->face_cells[2L * E + 1];
+            const int e1 = local_index(p, K1, E);
+            for (int i = 0; i < n; i++) {
+                double mu;
+                if (K2 < 0) {
+                    int inflow = 0;
+                    for (int j = 0; j < n; j++)
+                        if (p->v[IKE(j, K1, e1)] < 0) { inflow = 1; break; }
+                    mu = inflow ? p->bc_mobility[BCI(i, E)] : p->m[IK(i, K1)];
+                } else {
+                    const int e2 = local_index(p, K2, E);
+                    const double vel = p->v[IKE(i, K1, e1)] - p->v[IKE(i, K2, e2)];
+                    if (vel == 0) mu = 0.5 * (p->m[IK(i, K1)] + p->m[IK(i, K2)]);
+                    else if (vel > 0) mu = p->m[IK(i, K1)];
+                    else mu = p->m[IK(i, K2)];
+                }
+                MUPW(i, E) = mu;
+            }
+        }
+    }
+    /* K5: explicit upwind of Z over (E,j,i) (Solver.hpp:729-769) */
+    if (p->advection == OR_EXPLICIT_UPWIND) {
+#pragma omp parallel for
+        for (int E = 0; E < F; E++) {
+            const int K1 = p->face_cells[2L * E], K2 = p->face_cells[2L * E + 1];
+            const int e1 = local_index(p, K1, E);
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    const double apu = p->a[IJKE(i, j, K1, e1)] + p->u[IJKE(i, j, K1, e1)];
+                    double z;
+                    if (apu > 0.0) z = ZK(j, K1);
+                    else if (apu == 0.0) z = 0;
+                    else if (K2 >= 0) z = ZK(j, K2);
+                    else z = ZF(j, E);
+                    ZUPW(i, j, E) = z;
+                }
+        }
+    }
+    /* K6: R_ijKe and R_iK over (K,i) (Solver.hpp:793-815) */
+#pragma omp parallel for
+    for (int K = 0; K < C; K++) {
+        const int *fi = p->cell_faces + (long)K * fpc;
+        const double mK = p->cell_meas[K];
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++)
+                for (int e = 0; e < fpc; e++) {
+                    /* R_ijKe (SecondaryCoefficients.h:50-86) */
+                    double R = 0;
+                    for (int f = 0; f < fpc; f++) R += MUPW(i, fi[f]) * b_ef(p, i, j, K, f, e);
+                    if (p->advection == OR_IMPLICIT_UPWIND) {
+                        double vel = p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)];
+                        if (vel < 0) R -= 2 * vel;
+                    }
+                    if (p->advection == OR_IMPLICIT_TRACE)
+                        R -= p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)];
+                    p->R_ijKe[IJKE(i, j, K, e)] = R;
+                }
+            /* R_iK (SecondaryCoefficients.h:88-127) */
+            double R = 0;
+            for (int j = 0; j < n; j++) R += p->N[IJK(i, j, K)] * ZK(j, K);
+            R *= mK / tau;
+            R += mK * p->f[IK(i, K)];
+            for (int e = 0; e < fpc; e++) R -= MUPW(i, fi[e]) * p->w[IKE(i, K, e)];
+            if (p->advection == OR_EXPLICIT_UPWIND) {
+                double aux = 0;
+                for (int j = 0; j < n; j++)
+                    for (int e = 0; e < fpc; e++)
+                        aux += (p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)]) * ZUPW(i, j, fi[e]);
+                R -= aux;
+            }
+            p->R_iK[IK(i, K)] = R;
+        }
+    }
+    /* K7: Q, singular-row fix, LU and in-place Q^{-1} solves (Solver.hpp:817-871) */
+#pragma omp parallel for
+    for (int K = 0; K < C; K++) {
+        const int *fi = p->cell_faces + (long)K * fpc;
+        const double mK = p->cell_meas[K];
+        double Q[64], rhs[8];
+        for (int i = 0; i < n; i++) {
+            int singular = 1;
+            for (int j = 0; j < n; j++) {
+                /* Q_ijK (SecondaryCoefficients.h:129-159) */
+                double q = 0;
+                for (int e = 0; e < fpc; e++) {
+                    q += MUPW(i, fi[e]) * b_e(p, i, j, K, e) - p->u[IJKE(i, j, K, e)];
+                    if (p->advection == OR_IMPLICIT_UPWIND) {
+                        double vel = p->a[IJKE(i, j, K, e)] + p->u[IJKE(i, j, K, e)];
+                        if (vel >= 0) q += vel; else q -= vel;
+                    }
+                }
+                q += mK / tau * p->N[IJK(i, j, K)] + mK * p->r[IJK(i, j, K)];
+                Q[i * n + j] = q;
+                if (q != 0.0) singular = 0;
+            }
+            if (singular) {
+                Q[i * n + i] = 1.0;
+                p->R_iK[IK(i, K)] += ZK(i, K);
+            }
+        }
+        lu_factorize(Q, n);
+        for (int i = 0; i < n; i++) rhs[i] = p->R_iK[IK(i, K)];
+        lu_solve(Q, n, rhs);
+        for (int i = 0; i < n; i++) p->R_iK[IK(i, K)] = rhs[i];
+        for (int j = 0; j < n; j++)
+            for (int e = 0; e < fpc; e++) {
+                for (int i = 0; i < n; i++) rhs[i] = p->R_ijKe[IJKE(i, j, K, e)];
+                lu_solve(Q, n, rhs);
+                for (int i = 0; i < n; i++) p->R_ijKe[IJKE(i, j, K, e)] = rhs[i];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver::assembleLinearSystem (Solver.hpp:933-959) */
+static void boundary_row(or_problem *p, int E, int i, int row)
+{
+    /* BoundaryConditions::setMatrixElements (BoundaryConditions.hpp:335-508) */
+    const int n = p->n, fpc = p->fpc;
+    const int tag = p->tags[BCI(i, E)];
+    double *vals = p->val + p->rowptr[row];
+    if (tag == OR_FIXED_VALUE) {
+        vals[0] = 1.0;
+        p->rhs[row] = p->bc_dirichlet[BCI(i, E)];
+        return;
+    }
+    const int K = p->face_cells[2L * E];
+    const int *fi = p->cell_faces + (long)K * fpc;
+    const int e = local_index(p, K, E);
+    int perm[4];
+    sorted_faces(p, K, perm);
+    /* which coefficient set (BoundaryConditions.hpp:11-130) */
+    int inflow_identity = 0, use_adv = 0;
+    if (tag == OR_FIXED_FLUX) use_adv = 1;
+    if (tag == OR_ADVECTIVE_OUTFLOW) {
+        const double vel = p->a[IJKE(i, i, K, e)] + p->u[IJKE(i, i, K, e)];
+        if (vel < 0) inflow_identity = 1;
+    }
+#define BA(jj, FF, ff) (inflow_identity ? ((i == (jj) && E == (FF)) ? 1.0 : 0.0) \
+                        : (use_adv ? A_adv(p, i, jj, K, E, e, FF, ff) : A_no_adv(p, i, jj, K, e, ff)))
+    /* RowSetter::setRow (BoundaryConditions.hpp:132-222) */
+    const double diag = BA(i, E, e);
+    int t = 0;
+    for (int g = 0; g < fpc; g++) {
+        const int f = perm[g];
+        for (int j = 0; j < n; j++) {
+            if (j == i && fi[f] == E) vals[t++] = 1.0;
+            else vals[t++] = BA(j, fi[f], f) / diag;
+        }
+    }
+#undef BA
+    double bval;
+    if (tag == OR_ADVECTIVE_OUTFLOW) {
+        bval = 0;
+        bval += inflow_identity ? ZK(i, K) : RHS_no_adv(p, i, K, e);
+    } else {
+        bval = -p->bc_values[BCI(i, E)] * p->face_meas[E];
+        bval += use_adv ? RHS_adv(p, i, K, E, e) : RHS_no_adv(p, i, K, e);
+    }
+    p->rhs[row] = bval / diag;
+}
+
+static void interior_row(or_problem *p, int E, int i, int row)
+{
+    /* LinearSystem::setMatrixElements (LinearSystem.h:66-256) and
+     * RightHandSide::getValue (RightHandSide.h:18-45) */
+    const int n = p->n, fpc = p->fpc;
+    const int K0 = p->face_cells[2L * E], K1 = p->face_cells[2L * E + 1];
+    const int *f0 = p->cell_faces + (long)K0 * fpc, *f1 = p->cell_faces + (long)K1 * fpc;
+    int p0[4], p1[4];
+    sorted_faces(p, K0, p0);
+    sorted_faces(p, K1, p1);
+    const int e0 = local_index(p, K0, E), e1 = local_index(p, K1, E);
+    const double d = A_interior(p, i, i, K0, E, e0, E, e0) + A_interior(p, i, i, K1, E, e1, E, e1);
+    double *vals = p->val + p->rowptr[row];
+    int t = 0, g0 = 0, g1 = 0;
+    while (g0 < fpc && g1 < fpc) {
+        const int a0 = p0[g0], a1 = p1[g1];
+        if (f0[a0] < f1[a1]) {
+            for (int j = 0; j < n; j++) vals[t++] = A_interior(p, i, j, K0, E, e0, f0[a0], a0) / d;
+            g0++;
+        } else if (f0[a0] == f1[a1]) {
+            for (int j = 0; j < n; j++) {
+                if (j == i) vals[t++] = 1.0;
+                else
+                    vals[t++] = (A_interior(p, i, j, K0, E, e0, f0[a0], a0) +
+                                 A_interior(p, i, j, K1, E, e1, f1[a1], a1)) / d;
+            }
+            g0++;
+            g1++;
+        } else {
+            for (int j = 0; j < n; j++) vals[t++] = A_interior(p, i, j, K1, E, e1, f1[a1], a1) / d;
+            g1++;
+        }
+    }
+    while (g0 < fpc) {
+        const int a0 = p0[g0++];
+        for (int j = 0; j < n; j++) vals[t++] = A_interior(p, i, j, K0, E, e0, f0[a0], a0) / d;
+    }
+    while (g1 < fpc) {
+        const int a1 = p1[g1++];
+        for (int j = 0; j < n; j++) vals[t++] = A_interior(p, i, j, K1, E, e1, f1[a1], a1) / d;
+    }
+    double rhs = 0.0;
+    rhs += RHS_interior(p, i, K0, E, e0);
+    rhs += RHS_interior(p, i, K1, E, e1);
+    p->rhs[row] = rhs / d;
+}
+
+void or_assemble(or_problem *p, double time, double tau)
+{
+    (void)time; (void)tau;
+    const int n = p->n;
+#pragma omp parallel for
+    for (int E = 0; E < p->nf; E++)
+        for (int i = 0; i < n; i++) {
+            const int row = E * n + i;
+            if (p->face_cells[2L * E + 1] < 0) boundary_row(p, E, i, row);
+            else interior_row(p, E, i, row);
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver::postIterate (Solver.hpp:1273-1324) */
+void or_post_iterate(or_problem *p, double time, double tau)
+{
+    (void)time; (void)tau;
+    const int n = p->n, fpc = p->fpc;
+#pragma omp parallel for
+    for (int K = 0; K < p->nc; K++) {
+        const int *fi = p->cell_faces + (long)K * fpc;
+        /* K9: Z_iK (SecondaryCoefficients.h:163-185) */
+        for (int i = 0; i < n; i++) {
+            double Z = 0;
+            for (int f = 0; f < fpc; f++)
+                for (int j = 0; j < n; j++) Z += p->R_ijKe[IJKE(i, j, K, f)] * ZF(j, fi[f]);
+            Z += p->R_iK[IK(i, K)];
+            ZK(i, K) = Z;
+        }
+        /* K10: v_iKE (SecondaryCoefficients.h:16-48), uses the new Z_iK */
+        if (p->do_mob) {
+            for (int i = 0; i < n; i++)
+                for (int e = 0; e < fpc; e++) {
+                    double sK = 0, sE = 0;
+                    for (int j = 0; j < n; j++) {
+                        const double bb = b_e(p, i, j, K, e);
+                        sK += bb * ZK(j, K);
+                        for (int f = 0; f < fpc; f++) sE += b_ef(p, i, j, K, e, f) * ZF(j, fi[f]);
+                    }
+                    p->v[IKE(i, K, e)] = sK - sE + p->w[IKE(i, K, e)];
+                }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Sparse kernels and Krylov solvers (restated published methods). */
+void or_spmv(int nrows, const int *rp, const int *col, const double *val, const double *x, double *y)
+{
+#pragma omp parallel for
+    for (int r = 0; r < nrows; r++) {
+        double s = 0;
+        for (int k = rp[r]; k < rp[r + 1]; k++) s += val[k] * x[col[k]];
+        y[r] = s;
+    }
+}
+
+/* ILU(0), IKJ variant: Saad, Iterative Methods for Sparse Linear Systems,
+ * 2nd ed., Algorithm 10.4.  Pinned in tests by (LU)_ij == A_ij on the pattern. */
+int or_ilu0_factorize(int nrows, const int *rp, const int *col, double *a)
+{
+    int *iw = malloc(sizeof(int) * nrows);
+    int *dp = malloc(sizeof(int) * nrows);
+    for (int r = 0; r < nrows; r++) iw[r] = -1;
+    int rc = 0;
+    for (int r = 0; r < nrows && !rc; r++) {
+        dp[r] = -1;
+        for (int k = rp[r]; k < rp[r + 1]; k++) {
+            iw[col[k]] = k;
+            if (col[k] == r) dp[r] = k;
+        }
+        if (dp[r] < 0) { rc = -1; break; }
+        for (int k = rp[r]; k < rp[r + 1] && col[k] < r; k++) {
+            const int c = col[k];
+            const double piv = a[k] / a[dp[c]];
+            a[k] = piv;
+            for (int kk = dp[c] + 1; kk < rp[c + 1]; kk++) {
+                const int pos = iw[col[kk]];
+                if (pos >= 0) a[pos] -= piv * a[kk];
+            }
+        }
+        if (a[dp[r]] == 0.0) rc = -1;
+        for (int k = rp[r]; k < rp[r + 1]; k++) iw[col[k]] = -1;
+    }
+    free(iw);
+    free(dp);
+    return rc;
+}
+
+void or_ilu0_apply(int nrows, const int *rp, const int *col, const double *lu, const double *x,
+                   double *y)
+{
+    /* forward: unit lower L */
+    for (int r = 0; r < nrows; r++) {
+        double s = x[r];
+        for (int k = rp[r]; k < rp[r + 1] && col[k] < r; k++) s -= lu[k] * y[col[k]];
+        y[r] = s;
+    }
+    /* backward: U with diagonal */
+    for (int r = nrows - 1; r >= 0; r--) {
+        double s = y[r], dg = 1.0;
+        for (int k = rp[r]; k < rp[r + 1]; k++) {
+            if (col[k] > r) s -= lu[k] * y[col[k]];
+            else if (col[k] == r) dg = lu[k];
+        }
+        y[r] = s / dg;
+    }
+}
+
+typedef struct {
+    int n;
+    const int *rp, *col;
+    const double *val;
+    int prec;
+    double *dinv, *lu, *tmp;
+} krylov_op;
+
+static void prec_apply(krylov_op *op, const double *x, double *y)
+{
+    const int n = op->n;
+    if (op->prec == OR_PREC_DIAGONAL) {
+#pragma omp parallel for
+        for (int r = 0; r < n; r++) y[r] = x[r] * op->dinv[r];
+    } else if (op->prec == OR_PREC_ILU0) {
+        or_ilu0_apply(n, op->rp, op->col, op->lu, x, y);
+    } else {
+        memcpy(y, x, sizeof(double) * n);
+    }
+}
+/* y = M^{-1} A x */
+static void pmatvec(krylov_op *op, const double *x, double *y)
+{
+    or_spmv(op->n, op->rp, op->col, op->val, x, op->tmp);
+    prec_apply(op, op->tmp, y);
+}
+/* r = M^{-1}(b - A x) */
+static void presidue(krylov_op *op, const double *x, const double *b, double *r)
+{
+    or_spmv(op->n, op->rp, op->col, op->val, x, op->tmp);
+#pragma omp parallel for
+    for (int k = 0; k < op->n; k++) op->tmp[k] = b[k] - op->tmp[k];
+    prec_apply(op, op->tmp, r);
+}
+static double dot(int n, const double *x, const double *y)
+{
+    double s = 0;
+#pragma omp parallel for reduction(+ : s)
+    for (int k = 0; k < n; k++) s += x[k] * y[k];
+    return s;
+}
+
+/* BiCGStab, left preconditioned (van der Vorst 1992), with the restart on
+ * rho breakdown; stopping test ||r_k|| / ||M^{-1} b|| < tol. */
+static int bicgstab(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters,
+                    double *res)
+{
+    const int n = op->n;
+    double *r = malloc(sizeof(double) * n), *r_ast = malloc(sizeof(double) * n),
+           *pp = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n),
+           *s = malloc(sizeof(double) * n), *As = malloc(sizeof(double) * n);
+    prec_apply(op, b, s);
+    double b_norm = sqrt(dot(n, s, s));
+    if (b_norm == 0.0) b_norm = 1.0;
+    presidue(op, x, b, r);
+    memcpy(pp, r, sizeof(double) * n);
+    memcpy(r_ast, r, sizeof(double) * n);
+    double rho = dot(n, r, r_ast), r_ast_sqnorm = rho;
+    const double eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
+    double resid = sqrt(rho) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && !isnan(resid)) {
+        pmatvec(op, pp, Ap);
+        const double alpha = rho / dot(n, Ap, r_ast);
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) s[k] = r[k] - alpha * Ap[k];
+        pmatvec(op, s, As);
+        double As_s = 0, As_As = 0;
+#pragma omp parallel for reduction(+ : As_s, As_As)
+        for (int k = 0; k < n; k++) { As_s += As[k] * s[k]; As_As += As[k] * As[k]; }
+        const double omega = As_As > 0 ? As_s / As_As : 0.0;
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) {
+            x[k] += alpha * pp[k] + omega * s[k];
+            r[k] = s[k] - omega * As[k];
+        }
+        const double rho_old = rho;
+        rho = dot(n, r, r_ast);
+        if (fabs(rho) < eps2 * r_ast_sqnorm) {
+            presidue(op, x, b, r);
+            memcpy(pp, r, sizeof(double) * n);
+            memcpy(r_ast, r, sizeof(double) * n);
+            rho = r_ast_sqnorm = dot(n, r, r_ast);
+        } else {
+            const double beta = (rho / rho_old) * (alpha / omega);
+#pragma omp parallel for
+            for (int k = 0; k < n; k++) pp[k] = r[k] + beta * (pp[k] - omega * Ap[k]);
+        }
+        it++;
+        resid = sqrt(dot(n, r, r)) / b_norm;
+    }
+    *iters = it;
+    *res = resid;
+    free(r); free(r_ast); free(pp); free(Ap); free(s); free(As);
+    return resid < tol;
+}
+
+/* Restarted GMRES(m), left preconditioned (Saad & Schultz 1986), classical
+ * Gram-Schmidt with one re-orthogonalisation pass, Givens rotations. */
+static int gmres(krylov_op *op, const double *b, double *x, double tol, int maxit, int m, int *iters,
+                 double *res)
+{
+    const int n = op->n;
+    if (m < 1) m = 30;
+    double *V = malloc(sizeof(double) * (long)n * (m + 1));
+    double *w = malloc(sizeof(double) * n);
+    double *H = calloc((long)(m + 1) * m, sizeof(double));
+    double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+    double *g = malloc(sizeof(double) * (m + 1)), *y = malloc(sizeof(double) * m);
+    double *h = malloc(sizeof(double) * (m + 1)), *h2 = malloc(sizeof(double) * (m + 1));
+    prec_apply(op, b, w);
+    double b_norm = sqrt(dot(n, w, w));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0;
+    double resid;
+    presidue(op, x, b, V);
+    double beta = sqrt(dot(n, V, V));
+    resid = beta / b_norm;
+    while (resid >= tol && it < maxit && !isnan(resid)) {
+        for (int k = 0; k < n; k++) V[k] /= beta;
+        for (int k = 0; k <= m; k++) g[k] = 0;
+        g[0] = beta;
+        int j;
+        for (j = 0; j < m && it < maxit; j++) {
+            pmatvec(op, V + (long)j * n, w);
+            for (int pass = 0; pass < 2; pass++) {
+                for (int k = 0; k <= j; k++) h2[k] = dot(n, V + (long)k * n, w);
+#pragma omp parallel for
+                for (int q = 0; q < n; q++) {
+                    double s = w[q];
+                    for (int k = 0; k <= j; k++) s -= h2[k] * V[(long)k * n + q];
+                    w[q] = s;
+                }
+                for (int k = 0; k <= j; k++) h[k] = pass ? h[k] + h2[k] : h2[k];
+            }
+            h[j + 1] = sqrt(dot(n, w, w));
+            if (h[j + 1] != 0.0)
+                for (int q = 0; q < n; q++) V[(long)(j + 1) * n + q] = w[q] / h[j + 1];
+            for (int k = 0; k < j; k++) {
+                double t = cs[k] * h[k] + sn[k] * h[k + 1];
+                h[k + 1] = -sn[k] * h[k] + cs[k] * h[k + 1];
+                h[k] = t;
+            }
+            double den = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+            cs[j] = den > 0 ? h[j] / den : 1.0;
+            sn[j] = den > 0 ? h[j + 1] / den : 0.0;
+            h[j] = cs[j] * h[j] + sn[j] * h[j + 1];
+            h[j + 1] = 0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int k = 0; k <= j; k++) H[(long)k * m + j] = h[k];
+            it++;
+            resid = fabs(g[j + 1]) / b_norm;
+            if (resid < tol) { j++; break; }
+        }
+        /* back substitution and update */
+        for (int k = j - 1; k >= 0; k--) {
+            double s = g[k];
+            for (int q = k + 1; q < j; q++) s -= H[(long)k * m + q] * y[q];
+            y[k] = s / H[(long)k * m + k];
+        }
+#pragma omp parallel for
+        for (int q = 0; q < n; q++) {
+            double s = 0;
+            for (int k = 0; k < j; k++) s += y[k] * V[(long)k * n + q];
+            x[q] += s;
+        }
+        presidue(op, x, b, V);
+        beta = sqrt(dot(n, V, V));
+        resid = beta / b_norm;
+    }
+    *iters = it;
+    *res = resid;
+    free(V); free(w); free(H); free(cs); free(sn); free(g); free(y); free(h); free(h2);
+    return resid < tol;
+}
+
+int or_krylov(int nrows, const int *rp, const int *col, const double *val, const double *b,
+              double *x, int solver, int precond, double tol, int max_iter, int gmres_restart,
+              int *iterations, double *residue)
+{
+    krylov_op op = {nrows, rp, col, val, precond, NULL, NULL, NULL};
+    op.tmp = malloc(sizeof(double) * nrows);
+    int rc = 0;
+    if (precond == OR_PREC_DIAGONAL) {
+        /* Diagonal preconditioner: inverse of the matrix diagonal */
+        op.dinv = malloc(sizeof(double) * nrows);
+        for (int r = 0; r < nrows; r++) {
+            double dg = 0;
+            for (int k = rp[r]; k < rp[r + 1]; k++)
+                if (col[k] == r) dg = val[k];
+            op.dinv[r] = dg != 0.0 ? 1.0 / dg : 1.0;
+        }
+    } else if (precond == OR_PREC_ILU0) {
+        const long nnz = rp[nrows];
+        op.lu = malloc(sizeof(double) * nnz);
+        memcpy(op.lu, val, sizeof(double) * nnz);
+        if (or_ilu0_factorize(nrows, rp, col, op.lu)) rc = -1;
+    }
+    int conv = 0;
+    if (!rc) {
+        if (solver == OR_GMRES) conv = gmres(&op, b, x, tol, max_iter, gmres_restart, iterations, residue);
+        else conv = bicgstab(&op, b, x, tol, max_iter, iterations, residue);
+    }
+    free(op.tmp); free(op.dinv); free(op.lu);
+    return rc ? -1 : conv;
+}
+
+/* Solver::solveLinearSystem (Solver.hpp:1230-1265): initial guess = current Z_iF */
+int or_solve(or_problem *p, int solver, int precond, double tol, int max_iter, int gmres_restart,
+             int *iterations, double *residue)
+{
+    int rc = or_krylov(p->nrows, p->rowptr, p->colidx, p->val, p->rhs, p->Z_iF, solver, precond, tol,
+                       max_iter, gmres_restart, iterations, residue);
+    if (rc != 1)
+        snprintf(p->err, sizeof p->err,
+                 "MHFEM error: the linear system solver did not converge (%d iterations).", *iterations);
+    return rc;
+}
+
+int or_rows(const or_problem *p) { return p->nrows; }
+long or_nnz(const or_problem *p) { return p->nnz; }
+int *or_rowptr(or_problem *p) { return p->rowptr; }
+int *or_colidx(or_problem *p) { return p->colidx; }
+double *or_values(or_problem *p) { return p->val; }

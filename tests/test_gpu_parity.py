@@ -1,0 +1,173 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: integer structure (face numbering, incidence, CSR rowptr/colidx)
+bit-exact; FP64 scheme arrays and matrix values within 1e-12 relative (only
+summation order and FMA contraction differ); face traces Z_iF and cell
+averages Z_iK within 1e-10 relative L2 after each time step when both sides
+solve to 1e-13 (solver-independent comparison, tolerances in each test).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from mhfem_amd import problems, capi
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def maxrel(a, b):
+    s = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / s
+
+
+CASES = {
+    "heat2d": lambda: problems.heat_2d(s=12, steps=3),
+    "heat2d_lumped": lambda: problems.heat_2d(s=12, steps=3, lumping=1),
+    "darcy3d": lambda: problems.darcy_3d(s=6, steps=3, preconditioner="diagonal"),
+    "darcy3d_lumped": lambda: dataclasses.replace(problems.darcy_3d(s=5, steps=3, preconditioner="diagonal"), lumping=1),
+    "adr3d_n2": lambda: problems.adr_3d(s=5, steps=3, preconditioner="diagonal"),
+    "adr3d_implicit_upwind": lambda: dataclasses.replace(problems.adr_3d(s=4, steps=2, preconditioner="diagonal"),
+                                                         advection=problems.IMPLICIT_UPWIND),
+    "adr3d_implicit_trace": lambda: dataclasses.replace(problems.adr_3d(s=4, steps=2, preconditioner="diagonal"),
+                                                        advection=problems.IMPLICIT_TRACE),
+}
+
+
+@pytest.fixture(params=sorted(CASES))
+def case(request):
+    print(f"\n[case] {request.param}", flush=True)
+    return request.param, CASES[request.param]()
+
+
+def test_topology_bit_exact(case):
+    name, pr = case
+    s = capi.Solver(pr, setup=False)
+    cf, fc = s.topology()
+    ocf, ofc = pyoracle.topology(pr.points, pr.cells)
+    assert np.array_equal(cf, ocf)
+    assert np.array_equal(fc, ofc)
+    assert np.array_equal(cf, pr.cell_faces)
+    s.close()
+
+
+def test_measures(case):
+    name, pr = case
+    s = capi.Solver(pr, setup=False)
+    o = pyoracle.Oracle(pr)
+    assert maxrel(s.get_array("cell_measure"), o.array("cell_measure")) < 1e-14
+    assert maxrel(s.get_array("face_measure"), o.array("face_measure")) < 1e-14
+
+
+def test_pre_iterate_arrays(case):
+    name, pr = case
+    s = capi.Solver(pr, tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-15
+    s.preIterate(0.0, pr.dt)
+    o.pre_iterate(0.0, pr.dt)
+    for nm in ("b_ijK_storage", "R_ijKe", "R_iK", "m_iE_upw", "Z_ijE_upw"):
+        a, b = s.get_array(nm), o.array(nm)
+        assert maxrel(a, b) < 1e-12, (nm, maxrel(a, b))
+
+
+def test_assembly(case):
+    name, pr = case
+    s = capi.Solver(pr, tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    s.preIterate(0.0, pr.dt)
+    o.pre_iterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    o.assemble(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    orp, ocol, oval, orhs = o.csr()
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(col, ocol)
+    assert maxrel(val, oval) < 1e-12
+    assert maxrel(rhs, orhs) < 1e-12
+    # the diagonal of every row is exactly 1 (LinearSystem.h:196-201, BoundaryConditions.hpp:200-203, :375-377)
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    assert np.all(val[col == rows] == 1.0)
+
+
+def test_time_steps(case):
+    name, pr = case
+    s = capi.Solver(pr, tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for k in range(pr.steps):
+        s.step(t, pr.dt)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+        eF = rel(s.get_array("Z_iF"), o.array("Z_iF"))
+        eK = rel(s.get_array("Z_iK"), o.array("Z_iK"))
+        assert eF < 1e-10 and eK < 1e-10, (k, eF, eK)
+        if pr.do_mobility_upwind:
+            assert rel(s.get_array("v_iKe"), o.array("v_iKe")) < 1e-9
+
+
+@pytest.mark.parametrize("solver,prec", [("bicgstab", "diagonal"), ("bicgstab", "none"), ("bicgstab", "ilu0"),
+                                         ("gmres", "diagonal"), ("gmres", "ilu0")])
+def test_krylov_iteration_parity(solver, prec):
+    """Same algorithm, same x0: iteration counts agree and the iterates match."""
+    print(f"\n[krylov] {solver} {prec}", flush=True)
+    pr = problems.darcy_3d(s=6, steps=1, preconditioner=prec)
+    s = capi.Solver(pr, solver=solver, preconditioner=prec, tol=1e-10, check_every=1)
+    o = pyoracle.Oracle(pr)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    x0 = s.get_array("Z_iF")
+    x, st = s.solve_rhs(rhs, x0)
+    xo, conv, it, res = pyoracle.krylov(rp, col, val, rhs, x0, solver, prec, 1e-10)
+    assert conv == 1 and st.converged
+    assert abs(st.iterations - it) <= 2, (st.iterations, it)
+    assert rel(x, xo) < 1e-7
+    # both are solutions to the requested residue
+    xd, _, _, _ = pyoracle.krylov(rp, col, val, rhs, x0, "bicgstab", "ilu0", 1e-14)
+    assert rel(x, xd) < 1e-6
+
+
+def test_ilu0_step_parity():
+    pr = problems.darcy_3d(s=6, steps=2, preconditioner="ilu0")
+    s = capi.Solver(pr, tol=1e-13, check_every=1)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for k in range(pr.steps):
+        s.step(t, pr.dt)
+        it, _ = o.step(t, pr.dt, tol=1e-13)
+        assert abs(s.iterations[-1] - it) <= 2
+        t += pr.dt
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["diagonal", "ilu0"])
+def test_fp32_refinement(prec):
+    """FP32 inner BiCGStab in FP64 refinement reaches the FP64 tolerance; the
+    error against the all-FP64 oracle stays at the level of the tolerance."""
+    pr = problems.adr_3d(s=5, steps=3, preconditioner=prec)
+    s = capi.Solver(pr, tol=1e-10, precision="fp32-refined", inner_tol=1e-4)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for k in range(pr.steps):
+        st = s.step(t, pr.dt)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+        assert st.converged and st.residue < 1e-10
+        assert st.outer_iterations >= 1
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-8
+
+
+def test_nonconvergence_raises():
+    pr = problems.darcy_3d(s=5, steps=1, preconditioner="diagonal")
+    s = capi.Solver(pr, tol=1e-30, max_iterations=3)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    with pytest.raises(capi.MHFEMError, match="did not converge"):
+        s.solveLinearSystem()

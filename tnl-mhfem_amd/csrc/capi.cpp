@@ -1,0 +1,753 @@
+// capi.cpp -- the C ABI of libmhfem_amd.so: context lifetime, mesh topology,
+// the fixed CSR pattern and its SELL-64 device layout, model-array transfers
+// and the per-step entry points (include/mhfem.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "common.hpp"
+
+using namespace mhfem;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Face numbering and incidence.  Faces are numbered in the order in which
+// they are first met when the cells are visited in index order and the faces
+// of each cell in local order (local face e is opposite local vertex e,
+// MassMatrix.h:538,661,787,932); the cells of a face are kept in ascending
+// order (mesh_helpers.h:49-59 returns them in that order: cellIndexes[0] is
+// the "main" element of Solver.hpp:670-671).
+struct FaceKey {
+    int v[3];
+    bool operator==(const FaceKey &o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+};
+struct FaceKeyHash {
+    size_t operator()(const FaceKey &k) const
+    {
+        uint64_t h = (uint64_t)(uint32_t)k.v[0] * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)(uint32_t)k.v[1] + 0x7F4A7C159E3779B9ull + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)k.v[2] * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+        return (size_t)(h ^ (h >> 31));
+    }
+};
+
+void build_topology(Ctx &c, const int *cv)
+{
+    const int fpc = c.fpc, C = c.C;
+    c.h_cell_faces.assign((size_t)C * fpc, -1);
+    c.h_face_cells.clear();
+    c.h_face_cells.reserve((size_t)C * fpc);
+    std::unordered_map<FaceKey, int, FaceKeyHash> seen;
+    seen.reserve((size_t)C * fpc);
+    int nf = 0;
+    for (int K = 0; K < C; K++) {
+        for (int e = 0; e < fpc; e++) {
+            FaceKey key{{-1, -1, -1}};
+            int t = 0;
+            for (int v = 0; v < fpc; v++) {
+                if (v == e) continue;
+                const int vid = cv[(size_t)K * fpc + v];
+                if (vid < 0 || vid >= c.nv) throw Error{MHFEM_E_MESH, "cell vertex index out of range"};
+                key.v[t++] = vid;
+            }
+            std::sort(key.v, key.v + t);
+            auto it = seen.find(key);
+            int id;
+            if (it == seen.end()) {
+                id = nf++;
+                seen.emplace(key, id);
+                c.h_face_cells.push_back(K);
+                c.h_face_cells.push_back(-1);
+            } else {
+                id = it->second;
+                if (c.h_face_cells[2 * (size_t)id + 1] != -1)
+                    throw Error{MHFEM_E_MESH, "non-conforming mesh: a face belongs to more than two cells"};
+                c.h_face_cells[2 * (size_t)id + 1] = K;
+            }
+            c.h_cell_faces[(size_t)K * fpc + e] = id;
+        }
+    }
+    c.F = nf;
+    c.nbnd = 0;
+    for (int E = 0; E < nf; E++) c.nbnd += c.h_face_cells[2 * (size_t)E + 1] < 0;
+}
+
+double simplex_measure(int dim, const double *P)
+{
+    if (dim == 2) {
+        const double ax = P[2] - P[0], ay = P[3] - P[1], bx = P[4] - P[0], by = P[5] - P[1];
+        return 0.5 * std::fabs(ax * by - ay * bx);
+    }
+    double a[3], b[3], d[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = P[3 + k] - P[k];
+        b[k] = P[6 + k] - P[k];
+        d[k] = P[9 + k] - P[k];
+    }
+    const double det = a[0] * (b[1] * d[2] - b[2] * d[1]) - a[1] * (b[0] * d[2] - b[2] * d[0]) +
+                       a[2] * (b[0] * d[1] - b[1] * d[0]);
+    return std::fabs(det) / 6.0;
+}
+
+double facet_measure(int dim, const double *P)
+{
+    if (dim == 2) return std::hypot(P[2] - P[0], P[3] - P[1]);
+    double a[3], b[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = P[3 + k] - P[k];
+        b[k] = P[6 + k] - P[k];
+    }
+    const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+// local faces of K sorted by global face index (LinearSystem.h:108-136)
+void sorted_local(const Ctx &c, int K, int *perm)
+{
+    const int *fi = &c.h_cell_faces[(size_t)K * c.fpc];
+    for (int j = 0; j < c.fpc; j++) perm[j] = j;
+    std::sort(perm, perm + c.fpc, [fi](int a, int b) { return fi[a] < fi[b]; });
+}
+
+int local_index(const Ctx &c, int K, int E)
+{
+    for (int e = 0; e < c.fpc; e++)
+        if (c.h_cell_faces[(size_t)K * c.fpc + e] == E) return e;
+    return 0;
+}
+
+// Per-face column order of the hybridised rows: slot t names the local face
+// of K0 (bits 6t..6t+2) and of K1 (bits 6t+3..6t+5), 7 = absent; bits 48-50
+// and 51-53 hold the local index of the face itself in K0 / K1, bits 56-59
+// the number of slots.  Interior rows merge the two sorted face lists
+// (LinearSystem.h:181-245), boundary rows list the sorted faces of the one
+// cell (BoundaryConditions.hpp:196-218).
+std::vector<uint64_t> build_face_slots(const Ctx &c)
+{
+    std::vector<uint64_t> slots(c.F);
+    const int fpc = c.fpc;
+    for (int E = 0; E < c.F; E++) {
+        const int K0 = c.h_face_cells[2 * (size_t)E], K1 = c.h_face_cells[2 * (size_t)E + 1];
+        uint64_t info = 0;
+        int p0[4], p1[4], ns = 0;
+        sorted_local(c, K0, p0);
+        const int *f0 = &c.h_cell_faces[(size_t)K0 * fpc];
+        auto put = [&](int t, int a, int b) {
+            info |= (uint64_t)(a & 7) << (6 * t);
+            info |= (uint64_t)(b & 7) << (6 * t + 3);
+        };
+        if (K1 < 0) {
+            for (int g = 0; g < fpc; g++) put(ns++, p0[g], 7);
+            info |= (uint64_t)local_index(c, K0, E) << 48;
+        } else {
+            sorted_local(c, K1, p1);
+            const int *f1 = &c.h_cell_faces[(size_t)K1 * fpc];
+            int g0 = 0, g1 = 0;
+            while (g0 < fpc || g1 < fpc) {
+                const int F0 = g0 < fpc ? f0[p0[g0]] : INT32_MAX, F1 = g1 < fpc ? f1[p1[g1]] : INT32_MAX;
+                const int Fm = std::min(F0, F1);
+                put(ns++, F0 == Fm ? p0[g0] : 7, F1 == Fm ? p1[g1] : 7);
+                if (F0 == Fm) g0++;
+                if (F1 == Fm) g1++;
+            }
+            info |= (uint64_t)local_index(c, K0, E) << 48;
+            info |= (uint64_t)local_index(c, K1, E) << 51;
+        }
+        info |= (uint64_t)ns << 56;
+        slots[E] = info;
+    }
+    return slots;
+}
+
+template <typename T>
+void upload(DBuf<T> &d, const T *h, size_t n)
+{
+    d.alloc(n);
+    if (n) MHFEM_HIP(hipMemcpy(d.p, h, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+bool is_coefficient(int which) { return which >= MHFEM_N_ijK && which <= MHFEM_f_iK; }
+
+template <typename F>
+int guarded(mhfem_ctx *ctx, F &&f)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return MHFEM_E_ARGUMENT;
+    try {
+        f(*c);
+        return MHFEM_OK;
+    } catch (const Error &e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        c->err = e.what();
+        return MHFEM_E_STATE;
+    }
+}
+
+void phase_begin(Ctx &c, int slot)
+{
+    if (c.timing) MHFEM_HIP(hipEventRecord(c.ev[slot], c.stream));
+}
+void phase_end(Ctx &c, int slot, double &acc)
+{
+    if (!c.timing) return;
+    MHFEM_HIP(hipEventRecord(c.ev[slot + 1], c.stream));
+    MHFEM_HIP(hipEventSynchronize(c.ev[slot + 1]));
+    float ms = 0;
+    MHFEM_HIP(hipEventElapsedTime(&ms, c.ev[slot], c.ev[slot + 1]));
+    acc += ms;
+}
+
+}  // namespace
+
+namespace mhfem {
+
+long Ctx::array_size(int which) const
+{
+    const long n_ = n, C_ = C, F_ = F, f = fpc;
+    switch (which) {
+    case MHFEM_Z_iF: return n_ * F_;
+    case MHFEM_Z_iK: return n_ * C_;
+    case MHFEM_N_ijK: case MHFEM_D_ijK: case MHFEM_r_ijK: return n_ * n_ * C_;
+    case MHFEM_u_ijKe: case MHFEM_a_ijKe: case MHFEM_R_ijKe: return n_ * n_ * f * C_;
+    case MHFEM_m_iK: case MHFEM_f_iK: case MHFEM_R_iK: return n_ * C_;
+    case MHFEM_w_iKe: case MHFEM_v_iKe: return n_ * f * C_;
+    case MHFEM_m_iE_upw: return n_ * F_;
+    case MHFEM_Z_ijE_upw: return n_ * n_ * F_;
+    case MHFEM_b_ijK_storage: return n_ * n_ * S * C_;
+    case MHFEM_BC_VALUES: case MHFEM_BC_DIRICHLET: case MHFEM_BC_MOBILITY: return n_ * F_;
+    case MHFEM_RHS: return rows;
+    case MHFEM_CELL_MEASURE: return C_;
+    case MHFEM_FACE_MEASURE: return F_;
+    default: return -1;
+    }
+}
+
+SchemeView Ctx::view()
+{
+    SchemeView s{};
+    s.C = C;
+    s.F = F;
+    s.dim = dim;
+    s.fpc = fpc;
+    s.S = S;
+    s.lumping = lumping;
+    s.advection = advection;
+    s.do_mob = do_mob;
+    s.cell_faces = cell_faces.p;
+    s.cell_vertices = cell_vertices.p;
+    s.points = points.p;
+    s.face_cells = face_cells.p;
+    s.face_slots = face_slots.p;
+    s.cell_meas = arr[MHFEM_CELL_MEASURE].p;
+    s.face_meas = arr[MHFEM_FACE_MEASURE].p;
+    auto coef = [&](int w) { return Coef{arr[w].p, uniform[w] ? 0L : ~0L}; };
+    s.N = coef(MHFEM_N_ijK);
+    s.u = coef(MHFEM_u_ijKe);
+    s.m = coef(MHFEM_m_iK);
+    s.D = coef(MHFEM_D_ijK);
+    s.w = coef(MHFEM_w_iKe);
+    s.a = coef(MHFEM_a_ijKe);
+    s.r = coef(MHFEM_r_ijK);
+    s.f = coef(MHFEM_f_iK);
+    s.Z_iF = arr[MHFEM_Z_iF].p;
+    s.Z_iK = arr[MHFEM_Z_iK].p;
+    s.v = arr[MHFEM_v_iKe].p;
+    s.m_upw = arr[MHFEM_m_iE_upw].p;
+    s.Z_upw = arr[MHFEM_Z_ijE_upw].p;
+    s.b = arr[MHFEM_b_ijK_storage].p;
+    s.R_ijKe = arr[MHFEM_R_ijKe].p;
+    s.R_iK = arr[MHFEM_R_iK].p;
+    s.tags = tags.p;
+    s.bc_values = arr[MHFEM_BC_VALUES].p;
+    s.bc_dirichlet = arr[MHFEM_BC_DIRICHLET].p;
+    s.bc_mobility = arr[MHFEM_BC_MOBILITY].p;
+    s.A = SellView<double>{rows, slice_ptr.p, slice_width.p, sell_col.p, sell_val.p};
+    s.rhs = arr[MHFEM_RHS].p;
+    return s;
+}
+
+}  // namespace mhfem
+
+extern "C" {
+
+mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *model)
+{
+    Ctx *c = nullptr;
+    try {
+        if (!mesh || !model || !mesh->points || !mesh->cell_vertices)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: null argument"};
+        if (mesh->dim != 2 && mesh->dim != 3) throw Error{MHFEM_E_ARGUMENT, "mhfem_create: dim must be 2 or 3"};
+        if (model->num_equations < 1 || model->num_equations > 3)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: num_equations must be 1..3"};
+        if (mesh->num_cells < 1 || mesh->num_points < mesh->dim + 1)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: empty mesh"};
+        if (model->advection < 0 || model->advection > 2)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: unknown advection discretization"};
+        c = new Ctx();
+        c->dim = mesh->dim;
+        c->fpc = mesh->dim + 1;
+        c->S = mesh->dim == 2 ? 9 : 14;
+        c->n = model->num_equations;
+        c->nv = mesh->num_points;
+        c->C = mesh->num_cells;
+        c->lumping = model->mass_lumping ? 1 : 0;
+        c->advection = model->advection;
+        c->do_mob = model->do_mobility_upwind ? 1 : 0;
+        MHFEM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev) MHFEM_HIP(hipEventCreate(&e));
+        build_topology(*c, mesh->cell_vertices);
+        const size_t Cn = c->C, fpc = c->fpc;
+        // measures
+        std::vector<double> cm(Cn), fm(c->F, 0.0);
+        for (size_t K = 0; K < Cn; K++) {
+            double P[12];
+            for (size_t k = 0; k < fpc; k++)
+                for (int d = 0; d < c->dim; d++)
+                    P[k * c->dim + d] = mesh->points[(size_t)mesh->cell_vertices[K * fpc + k] * c->dim + d];
+            cm[K] = simplex_measure(c->dim, P);
+            for (size_t e = 0; e < fpc; e++) {
+                const int E = c->h_cell_faces[K * fpc + e];
+                if (c->h_face_cells[2 * (size_t)E] != (int)K) continue;
+                double Q[9];
+                int t = 0;
+                for (size_t k = 0; k < fpc; k++) {
+                    if (k == e) continue;
+                    for (int d = 0; d < c->dim; d++) Q[t * c->dim + d] = P[k * c->dim + d];
+                    t++;
+                }
+                fm[E] = facet_measure(c->dim, Q);
+            }
+        }
+        upload(c->cell_faces, c->h_cell_faces.data(), c->h_cell_faces.size());
+        upload(c->face_cells, c->h_face_cells.data(), c->h_face_cells.size());
+        upload(c->cell_vertices, mesh->cell_vertices, Cn * fpc);
+        upload(c->points, mesh->points, (size_t)c->nv * c->dim);
+        std::vector<uint64_t> slots = build_face_slots(*c);
+        upload(c->face_slots, slots.data(), slots.size());
+        upload(c->arr[MHFEM_CELL_MEASURE], cm.data(), cm.size());
+        upload(c->arr[MHFEM_FACE_MEASURE], fm.data(), fm.size());
+        // model arrays (BaseModel::allocate, BaseModel.h:168-195)
+        for (int w = MHFEM_Z_iF; w <= MHFEM_BC_MOBILITY; w++) {
+            if (is_coefficient(w)) {
+                c->arr[w].alloc(1);
+                c->uniform[w] = true;
+                MHFEM_HIP(hipMemset(c->arr[w].p, 0, sizeof(double)));
+            } else {
+                c->arr[w].alloc(c->array_size(w));
+                MHFEM_HIP(hipMemset(c->arr[w].p, 0, c->array_size(w) * sizeof(double)));
+            }
+        }
+        // mobility upwind values default to 1 (constant-mobility models, HeatEquationModel.h:130)
+        {
+            std::vector<double> ones((size_t)c->n * c->F, 1.0);
+            MHFEM_HIP(hipMemcpy(c->arr[MHFEM_m_iE_upw].p, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice));
+            MHFEM_HIP(hipMemcpy(c->arr[MHFEM_BC_MOBILITY].p, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        // default boundary conditions: zero Dirichlet (Solver.hpp:281-294)
+        c->h_tags.assign((size_t)c->n * c->F, MHFEM_BC_FIXED_VALUE);
+        upload(c->tags, c->h_tags.data(), c->h_tags.size());
+        MHFEM_HIP(hipDeviceSynchronize());
+        return reinterpret_cast<mhfem_ctx *>(c);
+    } catch (const Error &e) {
+        g_create_error = e.msg;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+    }
+    if (c) mhfem_destroy(reinterpret_cast<mhfem_ctx *>(c));
+    return nullptr;
+}
+
+void mhfem_destroy(mhfem_ctx *ctx)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    krylov_destroy(c->krylov);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *mhfem_last_error(const mhfem_ctx *ctx)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int mhfem_get_counts(const mhfem_ctx *ctx, int *cells, int *faces, int *boundary_faces)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c) return MHFEM_E_ARGUMENT;
+    if (cells) *cells = c->C;
+    if (faces) *faces = c->F;
+    if (boundary_faces) *boundary_faces = c->nbnd;
+    return MHFEM_OK;
+}
+
+int mhfem_get_topology(const mhfem_ctx *ctx, int32_t *cell_faces, int32_t *face_cells)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c) return MHFEM_E_ARGUMENT;
+    if (cell_faces) std::memcpy(cell_faces, c->h_cell_faces.data(), c->h_cell_faces.size() * sizeof(int));
+    if (face_cells) std::memcpy(face_cells, c->h_face_cells.data(), c->h_face_cells.size() * sizeof(int));
+    return MHFEM_OK;
+}
+
+long mhfem_array_size(const mhfem_ctx *ctx, int which)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || which < 0 || which >= MHFEM_NUM_ARRAYS) return -1;
+    if (is_coefficient(which) && c->uniform[which]) return 1;
+    return c->array_size(which);
+}
+
+int mhfem_set_array(mhfem_ctx *ctx, int which, const double *host, long count)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (which < 0 || which >= MHFEM_NUM_ARRAYS || !host) throw Error{MHFEM_E_ARGUMENT, "mhfem_set_array: bad argument"};
+        if (which == MHFEM_CELL_MEASURE || which == MHFEM_FACE_MEASURE)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_set_array: measures are derived from the mesh"};
+        const long full = c.array_size(which);
+        if (which == MHFEM_RHS && !c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        if (is_coefficient(which) && count == 1) {
+            c.arr[which].alloc(1);
+            c.uniform[which] = true;
+        } else if (count == full) {
+            if (c.arr[which].n != (size_t)full) c.arr[which].alloc(full);
+            c.uniform[which] = false;
+        } else {
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_set_array: count must be the full size" +
+                                              std::string(is_coefficient(which) ? " or 1" : "") + " (expected " +
+                                              std::to_string(full) + ", got " + std::to_string(count) + ")"};
+        }
+        MHFEM_HIP(hipMemcpyAsync(c.arr[which].p, host, count * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mhfem_get_array(mhfem_ctx *ctx, int which, double *host, long count)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (which < 0 || which >= MHFEM_NUM_ARRAYS || !host) throw Error{MHFEM_E_ARGUMENT, "mhfem_get_array: bad argument"};
+        const long have = (is_coefficient(which) && c.uniform[which]) ? 1 : c.array_size(which);
+        if (count != have || !c.arr[which].p)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_get_array: count mismatch (expected " + std::to_string(have) + ")"};
+        MHFEM_HIP(hipMemcpyAsync(host, c.arr[which].p, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+void *mhfem_device_array(mhfem_ctx *ctx, int which)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || which < 0 || which >= MHFEM_NUM_ARRAYS) return nullptr;
+    if (is_coefficient(which) && c->uniform[which]) return nullptr;
+    return c->arr[which].p;
+}
+
+void *mhfem_stream(mhfem_ctx *ctx)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    return c ? (void *)c->stream : nullptr;
+}
+
+int mhfem_set_boundary_conditions(mhfem_ctx *ctx, const uint8_t *tags, const double *values, const double *dirichlet)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!tags || !values || !dirichlet) throw Error{MHFEM_E_ARGUMENT, "mhfem_set_boundary_conditions: null array"};
+        const size_t nF = (size_t)c.n * c.F;
+        for (size_t q = 0; q < nF; q++)
+            if (tags[q] > 3) throw Error{MHFEM_E_ARGUMENT, "unknown boundary condition type"};
+        std::vector<uint8_t> old = c.h_tags;
+        c.h_tags.assign(tags, tags + nF);
+        MHFEM_HIP(hipMemcpy(c.tags.p, tags, nF, hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(c.arr[MHFEM_BC_VALUES].p, values, nF * sizeof(double), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(c.arr[MHFEM_BC_DIRICHLET].p, dirichlet, nF * sizeof(double), hipMemcpyHostToDevice));
+        c.bc_set = true;
+        // the row lengths depend on the tags (BoundaryConditions.hpp:283-299)
+        if (c.system_ready && old != c.h_tags) c.system_ready = false;
+    });
+}
+
+int mhfem_set_initial_condition(mhfem_ctx *ctx, int i, const double *Z_K, long count)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (i < 0 || i >= c.n || !Z_K) throw Error{MHFEM_E_ARGUMENT, "mhfem_set_initial_condition: bad argument"};
+        if (count != c.C)
+            throw Error{MHFEM_E_ARGUMENT, "wrong vector length for the initial condition: expected " +
+                                              std::to_string(c.C) + " elements, got " + std::to_string(count)};
+        MHFEM_HIP(hipMemcpy(c.arr[MHFEM_Z_iK].p + (size_t)i * c.C, Z_K, count * sizeof(double), hipMemcpyHostToDevice));
+    });
+}
+
+int mhfem_init(mhfem_ctx *ctx)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        launch_face_average(c);
+        launch_zero_double(c, c.arr[MHFEM_v_iKe].p, c.arr[MHFEM_v_iKe].n);
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
+        if (cfg->solver < 0 || cfg->solver > 1) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
+        if (cfg->preconditioner < 0 || cfg->preconditioner > 2) throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
+        if (cfg->precision == MHFEM_PRECISION_FP32_REFINED && cfg->solver != MHFEM_SOLVER_BICGSTAB)
+            throw Error{MHFEM_E_ARGUMENT, "FP32 refinement is implemented for BiCGStab only"};
+        c.cfg = *cfg;
+        if (c.cfg.max_iterations <= 0) c.cfg.max_iterations = 100000;
+        if (c.cfg.check_every <= 0) c.cfg.check_every = 1;
+        if (c.cfg.gmres_restart <= 0) c.cfg.gmres_restart = 30;
+        // row lengths (Solver.hpp:449-474)
+        const int n = c.n, fpc = c.fpc;
+        c.rows = n * c.F;
+        c.h_rowptr.assign((size_t)c.rows + 1, 0);
+        for (int E = 0; E < c.F; E++)
+            for (int i = 0; i < n; i++) {
+                int len;
+                if (c.h_face_cells[2 * (size_t)E + 1] < 0)
+                    len = c.h_tags[(size_t)i * c.F + E] == MHFEM_BC_FIXED_VALUE ? 1 : fpc * n;
+                else
+                    len = (2 * fpc - 1) * n;
+                c.h_rowptr[(size_t)E * n + i + 1] = len;
+            }
+        for (int r = 0; r < c.rows; r++) c.h_rowptr[r + 1] += c.h_rowptr[r];
+        c.nnz = c.h_rowptr[c.rows];
+        if ((long)c.nnz >= (long)INT32_MAX) throw Error{MHFEM_E_ARGUMENT, "matrix too large for int32 row pointers"};
+        // columns (LinearSystem.h:181-245, BoundaryConditions.hpp:196-218)
+        c.h_col.assign(c.nnz, 0);
+        for (int E = 0; E < c.F; E++) {
+            const int K0 = c.h_face_cells[2 * (size_t)E], K1 = c.h_face_cells[2 * (size_t)E + 1];
+            int p0[4], p1[4];
+            sorted_local(c, K0, p0);
+            const int *f0 = &c.h_cell_faces[(size_t)K0 * fpc];
+            for (int i = 0; i < n; i++) {
+                int *cc = &c.h_col[c.h_rowptr[(size_t)E * n + i]];
+                int t = 0;
+                if (K1 < 0) {
+                    if (c.h_tags[(size_t)i * c.F + E] == MHFEM_BC_FIXED_VALUE) {
+                        cc[0] = E * n + i;
+                        continue;
+                    }
+                    for (int g = 0; g < fpc; g++)
+                        for (int j = 0; j < n; j++) cc[t++] = f0[p0[g]] * n + j;
+                } else {
+                    sorted_local(c, K1, p1);
+                    const int *f1 = &c.h_cell_faces[(size_t)K1 * fpc];
+                    int g0 = 0, g1 = 0;
+                    while (g0 < fpc || g1 < fpc) {
+                        const int F0 = g0 < fpc ? f0[p0[g0]] : INT32_MAX, F1 = g1 < fpc ? f1[p1[g1]] : INT32_MAX;
+                        const int Fm = std::min(F0, F1);
+                        for (int j = 0; j < n; j++) cc[t++] = Fm * n + j;
+                        if (F0 == Fm) g0++;
+                        if (F1 == Fm) g1++;
+                    }
+                }
+            }
+        }
+        // SELL-64 layout
+        const int nslices = (c.rows + 63) / 64;
+        c.h_slice_ptr.assign((size_t)nslices + 1, 0);
+        c.h_slice_width.assign(nslices, 0);
+        for (int s = 0; s < nslices; s++) {
+            int w = 0;
+            for (int r = s * 64; r < std::min(c.rows, s * 64 + 64); r++) w = std::max(w, c.h_rowptr[r + 1] - c.h_rowptr[r]);
+            c.h_slice_width[s] = w;
+            c.h_slice_ptr[s + 1] = c.h_slice_ptr[s] + (int64_t)w * 64;
+        }
+        c.sell_size = c.h_slice_ptr[nslices];
+        std::vector<int> scol(c.sell_size);
+        std::vector<int64_t> c2s(c.nnz);
+        for (int s = 0; s < nslices; s++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int r = s * 64 + lane;
+                for (int k = 0; k < c.h_slice_width[s]; k++) {
+                    const int64_t q = c.h_slice_ptr[s] + (int64_t)k * 64 + lane;
+                    if (r >= c.rows) {
+                        scol[q] = 0;
+                        continue;
+                    }
+                    const int len = c.h_rowptr[r + 1] - c.h_rowptr[r];
+                    if (k < len) {
+                        scol[q] = c.h_col[c.h_rowptr[r] + k];
+                        c2s[c.h_rowptr[r] + k] = q;
+                    } else {
+                        scol[q] = r;  // padding: zero value on the row's own column
+                    }
+                }
+            }
+        upload(c.slice_ptr, c.h_slice_ptr.data(), c.h_slice_ptr.size());
+        upload(c.slice_width, c.h_slice_width.data(), c.h_slice_width.size());
+        upload(c.sell_col, scol.data(), scol.size());
+        upload(c.csr_to_sell, c2s.data(), c2s.size());
+        upload(c.rowptr, c.h_rowptr.data(), c.h_rowptr.size());
+        upload(c.csr_col, c.h_col.data(), c.h_col.size());
+        c.sell_val.alloc(c.sell_size);
+        MHFEM_HIP(hipMemset(c.sell_val.p, 0, c.sell_size * sizeof(double)));
+        c.arr[MHFEM_RHS].alloc(c.rows);
+        MHFEM_HIP(hipMemset(c.arr[MHFEM_RHS].p, 0, c.rows * sizeof(double)));
+        krylov_destroy(c.krylov);
+        c.krylov = nullptr;
+        c.system_ready = true;
+        c.krylov = krylov_create(c);
+        MHFEM_HIP(hipDeviceSynchronize());
+    });
+}
+
+int mhfem_pre_iterate(mhfem_ctx *ctx, double time, double tau)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!(tau > 0)) throw Error{MHFEM_E_ARGUMENT, "time step must be positive"};
+        phase_begin(c, 6);
+        launch_pre_iterate(c, time, tau);
+        phase_end(c, 6, c.timers.pre_iterate);
+        if (c.timing) {
+            float a = 0, b = 0;
+            MHFEM_HIP(hipEventElapsedTime(&a, c.ev[0], c.ev[1]));
+            MHFEM_HIP(hipEventElapsedTime(&b, c.ev[1], c.ev[2]));
+            c.timers.upwind += a;
+            c.timers.update_b_R_Q += b;
+        }
+    });
+}
+
+int mhfem_assemble(mhfem_ctx *ctx, double time, double tau)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "mhfem_setup_linear_system must be called first"};
+        phase_begin(c, 8);
+        launch_assemble(c, time, tau);
+        phase_end(c, 8, c.timers.assemble);
+    });
+}
+
+int mhfem_solve(mhfem_ctx *ctx, mhfem_solve_stats *stats)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "mhfem_setup_linear_system must be called first"};
+        mhfem_solve_stats st{};
+        krylov_solve(c, c.arr[MHFEM_RHS].p, c.arr[MHFEM_Z_iF].p, &st);
+        c.timers.iterations += st.iterations;
+        c.timers.time_steps += 1;
+        if (stats) *stats = st;
+        if (!st.converged)
+            throw Error{MHFEM_E_NOT_CONVERGED, "MHFEM error: the linear system solver did not converge (" +
+                                                   std::to_string(st.iterations) + " iterations)."};
+    });
+}
+
+int mhfem_post_iterate(mhfem_ctx *ctx, double time, double tau)
+{
+    (void)time;
+    (void)tau;
+    return guarded(ctx, [&](Ctx &c) {
+        phase_begin(c, 10);
+        launch_post_iterate(c);
+        phase_end(c, 10, c.timers.post_iterate);
+    });
+}
+
+int mhfem_solve_interval(mhfem_ctx *ctx, double t0, double t1, double dt, mhfem_solve_stats *last)
+{
+    if (!(dt > 0)) return MHFEM_E_ARGUMENT;
+    double t = t0;
+    int rc = MHFEM_OK;
+    // control.h:74-101: ignore very small steps at the end
+    while (t1 - t > dt * 1e-6) {
+        const double tau = std::min(dt, t1 - t);
+        if ((rc = mhfem_pre_iterate(ctx, t, tau))) return rc;
+        if ((rc = mhfem_assemble(ctx, t, tau))) return rc;
+        if ((rc = mhfem_solve(ctx, last))) return rc;
+        if ((rc = mhfem_post_iterate(ctx, t, tau))) return rc;
+        t += tau;
+    }
+    return rc;
+}
+
+int mhfem_system_size(const mhfem_ctx *ctx, int *rows, long *nnz)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || !c->system_ready) return MHFEM_E_STATE;
+    if (rows) *rows = c->rows;
+    if (nnz) *nnz = c->nnz;
+    return MHFEM_OK;
+}
+
+int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *val, double *rhs)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        if (rowptr) std::memcpy(rowptr, c.h_rowptr.data(), c.h_rowptr.size() * sizeof(int));
+        if (col) std::memcpy(col, c.h_col.data(), c.h_col.size() * sizeof(int));
+        if (val) {
+            std::vector<double> sv(c.sell_size);
+            MHFEM_HIP(hipMemcpyAsync(sv.data(), c.sell_val.p, c.sell_size * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            MHFEM_HIP(hipStreamSynchronize(c.stream));
+            for (int r = 0; r < c.rows; r++)
+                for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++)
+                    val[q] = sv[sell_pos(c.h_slice_ptr.data(), r, q - c.h_rowptr[r])];
+        }
+        if (rhs) {
+            MHFEM_HIP(hipMemcpyAsync(rhs, c.arr[MHFEM_RHS].p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            MHFEM_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        if (!rhs || !x) throw Error{MHFEM_E_ARGUMENT, "null argument"};
+        DBuf<double> b, xx;
+        upload(b, rhs, c.rows);
+        upload(xx, x, c.rows);
+        mhfem_solve_stats st{};
+        krylov_solve(c, b.p, xx.p, &st);
+        MHFEM_HIP(hipMemcpyAsync(x, xx.p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+        if (stats) *stats = st;
+    });
+}
+
+int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || !out) return MHFEM_E_ARGUMENT;
+    *out = c->timers;
+    return MHFEM_OK;
+}
+
+int mhfem_reset_timers(mhfem_ctx *ctx)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return MHFEM_E_ARGUMENT;
+    c->timers = mhfem_timers{};
+    return MHFEM_OK;
+}
+
+int mhfem_set_timing(mhfem_ctx *ctx, int enabled)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return MHFEM_E_ARGUMENT;
+    c->timing = enabled != 0;
+    return MHFEM_OK;
+}
+
+int mhfem_synchronize(mhfem_ctx *ctx)
+{
+    return guarded(ctx, [&](Ctx &c) { MHFEM_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+}  // extern "C"

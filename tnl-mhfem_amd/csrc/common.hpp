@@ -1,0 +1,161 @@
+// common.hpp -- context, device views and helpers shared by the MHFEM HIP
+// translation units.  Everything here is internal to libmhfem_amd.so; the
+// public surface is include/mhfem.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mhfem.h"
+
+namespace mhfem {
+
+// ---------------------------------------------------------------------------
+// error handling
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define MHFEM_HIP(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::mhfem::Error{MHFEM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// A model coefficient: value(idx) = ptr[idx & mask]; mask = 0 for a uniform
+// coefficient (UniformCoefficient.h:7-53), mask = ~0 for a full array.
+struct Coef {
+    const double *ptr;
+    long mask;
+    __device__ __forceinline__ double operator[](long idx) const { return ptr[idx & mask]; }
+};
+
+// SELL-64 sparse matrix: rows grouped in slices of 64 (one wavefront); entry k
+// of row r lives at slice_ptr[r/64] + k*64 + r%64, so a wavefront reading
+// entry k of its 64 rows touches one contiguous 512-byte segment.  Padding
+// entries have val = 0 and col = the row itself.
+template <typename T>
+struct SellView {
+    int rows;
+    const int64_t *slice_ptr;
+    const int *slice_width;
+    const int *col;
+    T *val;
+};
+
+__host__ __device__ inline int64_t sell_pos(const int64_t *slice_ptr, int row, int k)
+{
+    return slice_ptr[row >> 6] + (int64_t)k * 64 + (row & 63);
+}
+
+// All device pointers of the scheme, passed to kernels by value.
+struct SchemeView {
+    int C, F, dim, fpc, S;
+    int lumping, advection, do_mob;
+    // topology
+    const int *cell_faces;      // C*FPC, K-major
+    const int *cell_vertices;   // C*FPC
+    const double *points;       // nv*dim
+    const int *face_cells;      // F*2
+    const uint64_t *face_slots; // per face: merged column order (see build_pattern)
+    const double *cell_meas, *face_meas;
+    // coefficients
+    Coef N, u, m, D, w, a, r, f;
+    // scheme arrays
+    double *Z_iF, *Z_iK, *v, *m_upw, *Z_upw, *b, *R_ijKe, *R_iK;
+    // boundary conditions
+    const uint8_t *tags;
+    const double *bc_values, *bc_dirichlet, *bc_mobility;
+    // linear system
+    SellView<double> A;
+    double *rhs;
+};
+
+// ---------------------------------------------------------------------------
+// device buffer
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        release();
+        if (count) MHFEM_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+struct Krylov;  // krylov.hip
+struct Ilu0;    // krylov.hip
+
+struct Ctx {
+    int dim = 0, fpc = 0, S = 0, n = 0;
+    int nv = 0, C = 0, F = 0, nbnd = 0;
+    int lumping = 0, advection = 0, do_mob = 1;
+    hipStream_t stream = nullptr;
+    // host topology
+    std::vector<int> h_cell_faces, h_face_cells;
+    // device topology
+    DBuf<int> cell_faces, cell_vertices, face_cells;
+    DBuf<uint64_t> face_slots;
+    DBuf<double> points, cell_meas, face_meas;
+    // model arrays (index by MHFEM_* ids); uniform coefficients hold 1 value
+    DBuf<double> arr[MHFEM_NUM_ARRAYS];
+    bool uniform[MHFEM_NUM_ARRAYS] = {};
+    DBuf<uint8_t> tags;
+    std::vector<uint8_t> h_tags;
+    bool bc_set = false;
+    // linear system (SELL-64 storage + CSR pattern on the host)
+    int rows = 0;
+    long nnz = 0;
+    std::vector<int> h_rowptr, h_col;
+    std::vector<int64_t> h_slice_ptr;
+    std::vector<int> h_slice_width;
+    DBuf<int64_t> slice_ptr;
+    DBuf<int> slice_width, sell_col, rowptr, csr_col;
+    DBuf<int64_t> csr_to_sell;
+    DBuf<double> sell_val;
+    long sell_size = 0;
+    bool system_ready = false;
+    mhfem_solver_config cfg{};
+    Krylov *krylov = nullptr;
+    // timers
+    bool timing = false;
+    hipEvent_t ev[16] = {};
+    mhfem_timers timers{};
+    std::string err;
+
+    long array_size(int which) const;
+    SchemeView view();
+};
+
+// scheme.hip
+void launch_face_average(Ctx &c);
+void launch_pre_iterate(Ctx &c, double time, double tau);
+void launch_assemble(Ctx &c, double time, double tau);
+void launch_post_iterate(Ctx &c);
+void launch_zero_double(Ctx &c, double *p, size_t n);
+
+// krylov.hip
+Krylov *krylov_create(Ctx &c);
+void krylov_destroy(Krylov *k);
+// solve A x = b (b, x device arrays of length rows); returns stats
+void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st);
+
+}  // namespace mhfem

@@ -1,0 +1,680 @@
+// scheme.hip -- MHFEM scheme kernels for gfx950: local mass matrices and
+// eliminations (preIterate), face-parallel hybridised assembly, recovery of
+// the cell averages and velocities (postIterate).
+//
+// Kernels are templated on the number of equations N and faces per cell FPC
+// (3 = triangles, 4 = tetrahedra); every per-element quantity of one cell
+// lives in registers of one lane.  Arrays are in the reference's CUDA
+// permutations, i.e. the cell (or face) index is the fastest one, so the
+// cell-parallel kernels read and write with unit stride across a wavefront.
+#include "common.hpp"
+
+namespace mhfem {
+
+constexpr int WG = 256;  // work-group size (4 wavefronts)
+
+template <int FPC>
+struct Simplex {
+    static constexpr int DIM = FPC - 1;
+    static constexpr int NP = FPC * (FPC + 1) / 2;  // packed upper triangle
+    static constexpr int S = NP + FPC;              // + row sums (MassMatrix::size)
+};
+
+// packed index of (e,f) (MassMatrix.h:613-616)
+__host__ __device__ constexpr int pk(int e, int f) { return e <= f ? e + f * (f + 1) / 2 : f + e * (e + 1) / 2; }
+
+// LU without pivoting and the in-place solve (the method of TNL's
+// LU_sequential_factorize / LU_sequential_solve_inplace, used at
+// MassMatrix.h:571-592 and Solver.hpp:849-866).
+template <int M>
+__device__ __forceinline__ void lu_factorize(double (&A)[M][M])
+{
+#pragma unroll
+    for (int k = 0; k < M; k++)
+#pragma unroll
+        for (int i = k + 1; i < M; i++) {
+            A[i][k] /= A[k][k];
+#pragma unroll
+            for (int j = k + 1; j < M; j++) A[i][j] -= A[i][k] * A[k][j];
+        }
+}
+template <int M>
+__device__ __forceinline__ void lu_solve(const double (&A)[M][M], double (&x)[M])
+{
+#pragma unroll
+    for (int i = 1; i < M; i++)
+#pragma unroll
+        for (int j = 0; j < i; j++) x[i] -= A[i][j] * x[j];
+#pragma unroll
+    for (int i = M - 1; i >= 0; i--) {
+#pragma unroll
+        for (int j = i + 1; j < M; j++) x[i] -= A[i][j] * x[j];
+        x[i] /= A[i][i];
+    }
+}
+
+// Geometric part of the RTN0 local mass matrix of one simplex and its inverse
+// (packed upper triangle, column by column).  The closed forms are those of
+// MassMatrix.h:543-569 (triangle), :666-695 (triangle, lumped), :793-831
+// (tetrahedron), :938-978 (tetrahedron, lumped).
+template <int FPC>
+__device__ __forceinline__ void geom_minv(const double (&P)[FPC][FPC - 1], double meas, int lumping,
+                                          double (&out)[Simplex<FPC>::NP])
+{
+    double M[FPC][FPC];
+    if constexpr (FPC == 3) {
+        if (!lumping) {
+            const double p0x = P[0][0] - P[2][0], p0y = P[0][1] - P[2][1];
+            const double p1x = P[1][0] - P[2][0], p1y = P[1][1] - P[2][1];
+            const double P00 = p0x * p0x + p0y * p0y, P01 = p0x * p1x + p0y * p1y,
+                         P11 = p1x * p1x + p1y * p1y;
+            const double den = 24 * meas;
+            M[0][0] = (3 * P00 + P11 - 3 * P01) / den;
+            M[1][1] = (P00 + 3 * P11 - 3 * P01) / den;
+            M[2][2] = (P00 + P11 + P01) / den;
+            M[0][1] = (3 * P01 - P00 - P11) / den;
+            M[0][2] = (P11 - P00 - P01) / den;
+            M[1][2] = (P00 - P11 - P01) / den;
+        } else {
+            const double a01[2] = {P[0][0] - P[1][0], P[0][1] - P[1][1]};
+            const double a02[2] = {P[0][0] - P[2][0], P[0][1] - P[2][1]};
+            const double a12[2] = {P[1][0] - P[2][0], P[1][1] - P[2][1]};
+            auto d2 = [](const double *x, const double *y) { return x[0] * y[0] + x[1] * y[1]; };
+            const double den = 12 * meas;
+            M[0][0] = (d2(a01, a01) + d2(a02, a02)) / den;
+            M[1][1] = (d2(a01, a01) + d2(a12, a12)) / den;
+            M[2][2] = (d2(a02, a02) + d2(a12, a12)) / den;
+            M[0][1] = (d2(a02, a12)) / den;
+            M[0][2] = (-d2(a01, a12)) / den;
+            M[1][2] = (d2(a01, a02)) / den;
+        }
+    } else {
+        auto d3 = [](const double *x, const double *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+        if (!lumping) {
+            double p0[3], p1[3], p2[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                p0[k] = P[0][k] - P[3][k];
+                p1[k] = P[1][k] - P[3][k];
+                p2[k] = P[2][k] - P[3][k];
+            }
+            const double P00 = d3(p0, p0), P11 = d3(p1, p1), P22 = d3(p2, p2);
+            const double P01 = d3(p0, p1), P02 = d3(p0, p2), P12 = d3(p1, p2);
+            const double den = 180 * meas;
+            M[0][0] = (12 * P00 + 2 * P11 + 2 * P22 - 8 * P01 - 8 * P02 + 2 * P12) / den;
+            M[1][1] = (2 * P00 + 12 * P11 + 2 * P22 - 8 * P01 + 2 * P02 - 8 * P12) / den;
+            M[2][2] = (2 * P00 + 2 * P11 + 12 * P22 + 2 * P01 - 8 * P02 - 8 * P12) / den;
+            M[3][3] = 2 * (P00 + P11 + P22 + P01 + P02 + P12) / den;
+            M[0][1] = (-3 * P00 - 3 * P11 + 2 * P22 + 12 * P01 - 3 * P02 - 3 * P12) / den;
+            M[0][2] = (-3 * P00 + 2 * P11 - 3 * P22 - 3 * P01 + 12 * P02 - 3 * P12) / den;
+            M[1][2] = (2 * P00 - 3 * P11 - 3 * P22 - 3 * P01 - 3 * P02 + 12 * P12) / den;
+            M[0][3] = (-3 * P00 + 2 * P11 + 2 * P22 - 3 * P01 - 3 * P02 + 2 * P12) / den;
+            M[1][3] = (2 * P00 - 3 * P11 + 2 * P22 - 3 * P01 + 2 * P02 - 3 * P12) / den;
+            M[2][3] = (2 * P00 + 2 * P11 - 3 * P22 + 2 * P01 - 3 * P02 - 3 * P12) / den;
+        } else {
+            double a01[3], a02[3], a03[3], a12[3], a13[3], a23[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                a01[k] = P[0][k] - P[1][k];
+                a02[k] = P[0][k] - P[2][k];
+                a03[k] = P[0][k] - P[3][k];
+                a12[k] = P[1][k] - P[2][k];
+                a13[k] = P[1][k] - P[3][k];
+                a23[k] = P[2][k] - P[3][k];
+            }
+            const double den = 36 * meas;
+            M[0][0] = (d3(a01, a01) + d3(a02, a02) + d3(a03, a03)) / den;
+            M[1][1] = (d3(a01, a01) + d3(a12, a12) + d3(a13, a13)) / den;
+            M[2][2] = (d3(a02, a02) + d3(a12, a12) + d3(a23, a23)) / den;
+            M[3][3] = (d3(a03, a03) + d3(a13, a13) + d3(a23, a23)) / den;
+            M[0][1] = (d3(a02, a12) + d3(a03, a13)) / den;
+            M[0][2] = (-d3(a01, a12) + d3(a03, a23)) / den;
+            M[0][3] = (-d3(a01, a13) - d3(a02, a23)) / den;
+            M[1][2] = (d3(a01, a02) + d3(a13, a23)) / den;
+            M[1][3] = (d3(a01, a03) - d3(a12, a23)) / den;
+            M[2][3] = (d3(a02, a03) + d3(a12, a13)) / den;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < FPC; i++)
+#pragma unroll
+        for (int j = 0; j < i; j++) M[i][j] = M[j][i];
+    lu_factorize<FPC>(M);
+#pragma unroll
+    for (int c = 0; c < FPC; c++) {
+        double vv[FPC];
+#pragma unroll
+        for (int k = 0; k < FPC; k++) vv[k] = (k == c) ? 1.0 : 0.0;
+        lu_solve<FPC>(M, vv);
+#pragma unroll
+        for (int r = 0; r <= c; r++) out[pk(r, c)] = vv[r];
+    }
+}
+
+// row sum of the scaled packed matrix, ascending f (MassMatrix.h:594-597, 864-868)
+template <int FPC>
+__device__ __forceinline__ double row_sum(const double (&minv)[Simplex<FPC>::NP], double D, int e)
+{
+    double s = minv[pk(e, 0)] * D;
+#pragma unroll
+    for (int f = 1; f < FPC; f++) s += minv[pk(e, f)] * D;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Solver::setInitialCondition face averages (Solver.hpp:319-341)
+template <int N>
+__global__ void __launch_bounds__(WG) k_face_average(SchemeView s)
+{
+    const int E = blockIdx.x * WG + threadIdx.x;
+    if (E >= s.F) return;
+    const int K0 = s.face_cells[2 * E], K1 = s.face_cells[2 * E + 1];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double z0 = s.Z_iK[(long)i * s.C + K0];
+        s.Z_iF[(long)E * N + i] = (K1 < 0) ? z0 : 0.5 * (z0 + s.Z_iK[(long)i * s.C + K1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Upwind coefficients over faces: K4 mobility upwind (Solver.hpp:663-727) and
+// K5 explicit upwind of Z (Solver.hpp:729-769).  One lane per face handles all
+// components.
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_upwind(SchemeView s)
+{
+    const int E = blockIdx.x * WG + threadIdx.x;
+    if (E >= s.F) return;
+    const uint64_t info = s.face_slots[E];
+    const int K1 = s.face_cells[2 * E], K2 = s.face_cells[2 * E + 1];
+    const int e1 = (info >> 48) & 7, e2 = (info >> 51) & 7;
+    const long C = s.C;
+    if (s.do_mob) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double mu;
+            if (K2 < 0) {
+                bool inflow = false;
+#pragma unroll
+                for (int j = 0; j < N; j++) inflow |= s.v[((long)j * FPC + e1) * C + K1] < 0;
+                mu = inflow ? s.bc_mobility[(long)i * s.F + E] : s.m[(long)i * C + K1];
+            } else {
+                const double vel = s.v[((long)i * FPC + e1) * C + K1] - s.v[((long)i * FPC + e2) * C + K2];
+                if (vel == 0) mu = 0.5 * (s.m[(long)i * C + K1] + s.m[(long)i * C + K2]);
+                else if (vel > 0) mu = s.m[(long)i * C + K1];
+                else mu = s.m[(long)i * C + K2];
+            }
+            s.m_upw[(long)E * N + i] = mu;
+        }
+    }
+    if (s.advection == MHFEM_EXPLICIT_UPWIND) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const long idx = (((long)i * N + j) * FPC + e1) * C + K1;
+                const double apu = s.a[idx] + s.u[idx];
+                double z;
+                if (apu > 0.0) z = s.Z_iK[(long)j * C + K1];
+                else if (apu == 0.0) z = 0;
+                else if (K2 >= 0) z = s.Z_iK[(long)j * C + K2];
+                else z = s.Z_iF[(long)E * N + j];
+                s.Z_upw[((long)E * N + j) * N + i] = z;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell part of preIterate, fused: K2 local mass matrix and its inverse in
+// registers (Solver.hpp:627-639), K6 R_ijKe / R_iK (Solver.hpp:793-815,
+// SecondaryCoefficients.h:50-127) and K7 Q assembly, singular-row fix, LU and
+// in-place Q^{-1} solves (Solver.hpp:817-871, SecondaryCoefficients.h:129-159).
+// The model hooks updateNonLinearTerms / updateVectorCoefficients run on the
+// host before this kernel; K4/K5 do not depend on b, so they run first.
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
+{
+    using SX = Simplex<FPC>;
+    constexpr int DIM = FPC - 1, NP = SX::NP, S = SX::S;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    const long C = s.C;
+    int fi[FPC];
+    double P[FPC][DIM];
+#pragma unroll
+    for (int e = 0; e < FPC; e++) {
+        fi[e] = s.cell_faces[(long)K * FPC + e];
+        const int vtx = s.cell_vertices[(long)K * FPC + e];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) P[e][c] = s.points[(long)vtx * DIM + c];
+    }
+    const double meas = s.cell_meas[K];
+    double minv[NP];
+    geom_minv<FPC>(P, meas, s.lumping, minv);
+
+    // b = M^{-1} D_ij, stored packed + row sums (b_ijK_storage)
+    double Dij[N][N], rs[N][N][FPC];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double D = s.D[((long)i * N + j) * C + K];
+            Dij[i][j] = D;
+            double *bp = s.b + ((long)(i * N + j) * S) * C + K;
+#pragma unroll
+            for (int q = 0; q < NP; q++) bp[(long)q * C] = minv[q] * D;
+#pragma unroll
+            for (int e = 0; e < FPC; e++) {
+                rs[i][j][e] = row_sum<FPC>(minv, D, e);
+                bp[(long)(NP + e) * C] = rs[i][j][e];
+            }
+        }
+
+    double mu[N][FPC];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int e = 0; e < FPC; e++) mu[i][e] = s.m_upw[(long)fi[e] * N + i];
+
+    auto vel_of = [&](int i, int j, int e) {
+        const long idx = (((long)i * N + j) * FPC + e) * C + K;
+        return s.a[idx] + s.u[idx];
+    };
+
+    // K6: R_ijKe (SecondaryCoefficients.h:50-86) and R_iK (:88-127)
+    double R[N][N][FPC], RK[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+#pragma unroll
+            for (int e = 0; e < FPC; e++) {
+                double r = 0;
+#pragma unroll
+                for (int f = 0; f < FPC; f++) r += mu[i][f] * (minv[pk(f, e)] * Dij[i][j]);
+                if (s.advection == MHFEM_IMPLICIT_UPWIND) {
+                    const double vel = vel_of(i, j, e);
+                    if (vel < 0) r -= 2 * vel;
+                }
+                if (s.advection == MHFEM_IMPLICIT_TRACE) r -= vel_of(i, j, e);
+                R[i][j][e] = r;
+            }
+        double r = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) r += s.N[((long)i * N + j) * C + K] * s.Z_iK[(long)j * C + K];
+        r *= meas / tau;
+        r += meas * s.f[(long)i * C + K];
+#pragma unroll
+        for (int e = 0; e < FPC; e++) r -= mu[i][e] * s.w[((long)i * FPC + e) * C + K];
+        if (s.advection == MHFEM_EXPLICIT_UPWIND) {
+            double aux = 0;
+#pragma unroll
+            for (int j = 0; j < N; j++)
+#pragma unroll
+                for (int e = 0; e < FPC; e++)
+                    aux += vel_of(i, j, e) * s.Z_upw[((long)fi[e] * N + j) * N + i];
+            r -= aux;
+        }
+        RK[i] = r;
+    }
+
+    // K7: Q (SecondaryCoefficients.h:129-159), singular rows (Solver.hpp:829-847)
+    double Q[N][N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        bool singular = true;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            double q = 0;
+#pragma unroll
+            for (int e = 0; e < FPC; e++) {
+                q += mu[i][e] * rs[i][j][e] - s.u[(((long)i * N + j) * FPC + e) * C + K];
+                if (s.advection == MHFEM_IMPLICIT_UPWIND) {
+                    const double vel = vel_of(i, j, e);
+                    if (vel >= 0) q += vel;
+                    else q -= vel;
+                }
+            }
+            q += meas / tau * s.N[((long)i * N + j) * C + K] + meas * s.r[((long)i * N + j) * C + K];
+            Q[i][j] = q;
+            if (q != 0.0) singular = false;
+        }
+        if (singular) {
+            Q[i][i] = 1.0;
+            RK[i] += s.Z_iK[(long)i * C + K];
+        }
+    }
+    lu_factorize<N>(Q);
+    lu_solve<N>(Q, RK);
+#pragma unroll
+    for (int i = 0; i < N; i++) s.R_iK[(long)i * C + K] = RK[i];
+#pragma unroll
+    for (int j = 0; j < N; j++)
+#pragma unroll
+        for (int e = 0; e < FPC; e++) {
+            double x[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) x[i] = R[i][j][e];
+            lu_solve<N>(Q, x);
+#pragma unroll
+            for (int i = 0; i < N; i++) s.R_ijKe[(((long)i * N + j) * FPC + e) * C + K] = x[i];
+        }
+}
+
+// a[i] for a runtime i without dynamic register indexing (which would move the
+// array to scratch memory): unrolled selects
+template <int M>
+__device__ __forceinline__ double pick(const double (&a)[M], int i)
+{
+    double v = a[0];
+#pragma unroll
+    for (int q = 1; q < M; q++) v = (q == i) ? a[q] : v;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Face-parallel hybridised assembly (Solver.hpp:933-959).
+//
+// Per-cell operands of one matrix row, loaded once into registers:
+//   bf[j][f] = b_ijKef(e, f), be[x] = b_ixKe(e), Rv[x][j][f] = R_xjKf.
+template <int N, int FPC>
+struct CellRow {
+    int K, e;
+    double bf[N][FPC], be[N], Rv[N][N][FPC];
+    double mu;        // m_iE_upw(i, E)
+    double vel[N];    // (a+u)_ijKe(e)
+
+    __device__ __forceinline__ void load(const SchemeView &s, int i, int K_, int e_, int E)
+    {
+        constexpr int NP = Simplex<FPC>::NP, S = Simplex<FPC>::S;
+        K = K_;
+        e = e_;
+        const long C = s.C;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double *bp = s.b + ((long)(i * N + j) * S) * C + K;
+#pragma unroll
+            for (int f = 0; f < FPC; f++) bf[j][f] = bp[(long)pk(e, f) * C];
+            be[j] = bp[(long)(NP + e) * C];
+        }
+#pragma unroll
+        for (int x = 0; x < N; x++)
+#pragma unroll
+            for (int j = 0; j < N; j++)
+#pragma unroll
+                for (int f = 0; f < FPC; f++) Rv[x][j][f] = s.R_ijKe[(((long)x * N + j) * FPC + f) * C + K];
+        mu = s.m_upw[(long)E * N + i];
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const long idx = (((long)i * N + j) * FPC + e) * C + K;
+            vel[j] = s.a[idx] + s.u[idx];
+        }
+    }
+
+    // A_ijKEF_no_advection (SecondaryCoefficients.h:189-211)
+    __device__ __forceinline__ double A_no_adv(int j, int f) const
+    {
+        double SQinvR = 0;
+#pragma unroll
+        for (int x = 0; x < N; x++) SQinvR += be[x] * pick<FPC>(Rv[x][j], f);
+        return pick<FPC>(bf[j], f) - SQinvR;
+    }
+    // A_ijKEF_advection (SecondaryCoefficients.h:215-259)
+    __device__ __forceinline__ double A_adv(int advection, int j, int f, bool EisF) const
+    {
+        double SQinvR = 0;
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            double S = mu * be[x];
+            if (advection == MHFEM_IMPLICIT_UPWIND) {
+                if (vel[x] >= 0) S += vel[x];
+                else S -= vel[x];
+            }
+            SQinvR += S * pick<FPC>(Rv[x][j], f);
+        }
+        double T = mu * pick<FPC>(bf[j], f);
+        if (EisF && advection == MHFEM_IMPLICIT_UPWIND && vel[j] < 0) T -= 2 * vel[j];
+        if (EisF && advection == MHFEM_IMPLICIT_TRACE) T -= vel[j];
+        return T - SQinvR;
+    }
+    // A_ijKEF (SecondaryCoefficients.h:261-278)
+    __device__ __forceinline__ double A(int advection, int j, int f, bool EisF) const
+    {
+        return advection == MHFEM_EXPLICIT_UPWIND ? A_no_adv(j, f) : A_adv(advection, j, f, EisF);
+    }
+    // RHS_iKE_no_advection (SecondaryCoefficients.h:282-297)
+    __device__ __forceinline__ double RHS_no_adv(const SchemeView &s, int i) const
+    {
+        const long C = s.C;
+        double value = s.w[((long)i * FPC + e) * C + K];
+#pragma unroll
+        for (int j = 0; j < N; j++) value += be[j] * s.R_iK[(long)j * C + K];
+        return value;
+    }
+    // RHS_iKE_advection (SecondaryCoefficients.h:301-333)
+    __device__ __forceinline__ double RHS_adv(const SchemeView &s, int advection, int i, int E) const
+    {
+        const long C = s.C;
+        double value = mu * s.w[((long)i * FPC + e) * C + K];
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            double S = mu * be[j];
+            if (advection == MHFEM_IMPLICIT_UPWIND) {
+                if (vel[j] >= 0) S += vel[j];
+                else S -= vel[j];
+            }
+            value += S * s.R_iK[(long)j * C + K];
+            if (advection == MHFEM_EXPLICIT_UPWIND) value += s.Z_upw[((long)E * N + j) * N + i] * vel[j];
+        }
+        return value;
+    }
+    __device__ __forceinline__ double RHS(const SchemeView &s, int advection, int i, int E) const
+    {
+        return advection == MHFEM_EXPLICIT_UPWIND ? RHS_no_adv(s, i) : RHS_adv(s, advection, i, E);
+    }
+};
+
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
+{
+    const int E = blockIdx.x * WG + threadIdx.x;
+    if (E >= s.F) return;
+    const uint64_t info = s.face_slots[E];
+    const int K0 = s.face_cells[2 * E], K1 = s.face_cells[2 * E + 1];
+    const int e0 = (info >> 48) & 7, e1 = (info >> 51) & 7;
+    const int nslots = (info >> 56) & 15;
+    const int adv = s.advection;
+    if (K1 >= 0) {
+        // interior face: LinearSystem::setMatrixElements (LinearSystem.h:66-256)
+        // and RightHandSide::getValue (RightHandSide.h:18-45)
+#pragma unroll 1
+        for (int i = 0; i < N; i++) {
+            const int row = E * N + i;
+            CellRow<N, FPC> c0, c1;
+            c0.load(s, i, K0, e0, E);
+            c1.load(s, i, K1, e1, E);
+            const double d = c0.A(adv, i, e0, true) + c1.A(adv, i, e1, true);
+            int k = 0;
+            for (int t = 0; t < nslots; t++) {
+                const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
+#pragma unroll
+                for (int j = 0; j < N; j++) {
+                    double val;
+                    if (f0 < 7 && f1 < 7)
+                        val = (j == i) ? 1.0 : (c0.A(adv, j, f0, true) + c1.A(adv, j, f1, true)) / d;
+                    else if (f0 < 7)
+                        val = c0.A(adv, j, f0, false) / d;
+                    else
+                        val = c1.A(adv, j, f1, false) / d;
+                    s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = val;
+                }
+            }
+            double rhs = 0.0;
+            rhs += c0.RHS(s, adv, i, E);
+            rhs += c1.RHS(s, adv, i, E);
+            s.rhs[row] = rhs / d;
+        }
+        return;
+    }
+    // boundary face: BoundaryConditions::setMatrixElements (BoundaryConditions.hpp:335-508)
+#pragma unroll 1
+    for (int i = 0; i < N; i++) {
+        const int row = E * N + i;
+        const long bi = (long)i * s.F + E;
+        const int tag = s.tags[bi];
+        if (tag == MHFEM_BC_FIXED_VALUE) {
+            s.A.val[sell_pos(s.A.slice_ptr, row, 0)] = 1.0;
+            s.rhs[row] = s.bc_dirichlet[bi];
+            continue;
+        }
+        CellRow<N, FPC> c0;
+        c0.load(s, i, K0, e0, E);
+        // BoundaryCoefficients_* (BoundaryConditions.hpp:11-130)
+        const bool use_adv = (tag == MHFEM_BC_FIXED_FLUX);
+        const bool inflow = (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) && (c0.vel[i] < 0);
+        auto BA = [&](int j, int f, bool EisF) -> double {
+            if (inflow) return (i == j && EisF) ? 1.0 : 0.0;
+            return use_adv ? c0.A_adv(adv, j, f, EisF) : c0.A_no_adv(j, f);
+        };
+        // RowSetter::setRow (BoundaryConditions.hpp:132-222)
+        const double d = BA(i, e0, true);
+        int k = 0;
+        for (int t = 0; t < nslots; t++) {
+            const int f = (info >> (6 * t)) & 7;
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                const double val = (j == i && f == e0) ? 1.0 : BA(j, f, f == e0) / d;
+                s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = val;
+            }
+        }
+        double bval;
+        if (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) {
+            bval = 0;
+            bval += inflow ? s.Z_iK[(long)i * s.C + K0] : c0.RHS_no_adv(s, i);
+        } else {
+            bval = -s.bc_values[bi] * s.face_meas[E];
+            bval += use_adv ? c0.RHS_adv(s, adv, i, E) : c0.RHS_no_adv(s, i);
+        }
+        s.rhs[row] = bval / d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// postIterate: K9 Z_iK recovery (Solver.hpp:1283-1292, SecondaryCoefficients.h:163-185)
+// and K10 conservative velocities (Solver.hpp:1298-1313, SecondaryCoefficients.h:16-48)
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_post(SchemeView s)
+{
+    constexpr int NP = Simplex<FPC>::NP, S = Simplex<FPC>::S;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    const long C = s.C;
+    int fi[FPC];
+#pragma unroll
+    for (int e = 0; e < FPC; e++) fi[e] = s.cell_faces[(long)K * FPC + e];
+    double ZF[N][FPC];
+#pragma unroll
+    for (int f = 0; f < FPC; f++)
+#pragma unroll
+        for (int j = 0; j < N; j++) ZF[j][f] = s.Z_iF[(long)fi[f] * N + j];
+    double ZK[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double Z = 0;
+#pragma unroll
+        for (int f = 0; f < FPC; f++)
+#pragma unroll
+            for (int j = 0; j < N; j++) Z += s.R_ijKe[(((long)i * N + j) * FPC + f) * C + K] * ZF[j][f];
+        Z += s.R_iK[(long)i * C + K];
+        ZK[i] = Z;
+        s.Z_iK[(long)i * C + K] = Z;
+    }
+    if (!s.do_mob) return;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int e = 0; e < FPC; e++) {
+            double sK = 0, sE = 0;
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                const double *bp = s.b + ((long)(i * N + j) * S) * C + K;
+                sK += bp[(long)(NP + e) * C] * ZK[j];
+#pragma unroll
+                for (int f = 0; f < FPC; f++) sE += bp[(long)pk(e, f) * C] * ZF[j][f];
+            }
+            s.v[((long)i * FPC + e) * C + K] = sK - sE + s.w[((long)i * FPC + e) * C + K];
+        }
+}
+
+__global__ void k_fill(double *p, size_t n, double v)
+{
+    const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static inline unsigned blocks(long n) { return (unsigned)((n + WG - 1) / WG); }
+
+template <template <int, int> class KernelTag, typename... Args>
+static void dispatch(Ctx &c, Args &&...args);
+
+#define MHFEM_DISPATCH(c, KERNEL, GRID, ...)                                                         \
+    do {                                                                                            \
+        const unsigned g_ = (GRID);                                                                 \
+        if (g_ == 0) break;                                                                         \
+        if ((c).fpc == 3) {                                                                         \
+            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 3>), g_, WG, 0, (c).stream, __VA_ARGS__); \
+            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 3>), g_, WG, 0, (c).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<3, 3>), g_, WG, 0, (c).stream, __VA_ARGS__);           \
+        } else {                                                                                    \
+            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 4>), g_, WG, 0, (c).stream, __VA_ARGS__); \
+            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 4>), g_, WG, 0, (c).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<3, 4>), g_, WG, 0, (c).stream, __VA_ARGS__);           \
+        }                                                                                           \
+        MHFEM_HIP(hipGetLastError());                                                               \
+    } while (0)
+
+void launch_face_average(Ctx &c)
+{
+    SchemeView s = c.view();
+    if (c.F == 0) return;
+    if (c.n == 1) hipLaunchKernelGGL(k_face_average<1>, blocks(c.F), WG, 0, c.stream, s);
+    else if (c.n == 2) hipLaunchKernelGGL(k_face_average<2>, blocks(c.F), WG, 0, c.stream, s);
+    else hipLaunchKernelGGL(k_face_average<3>, blocks(c.F), WG, 0, c.stream, s);
+    MHFEM_HIP(hipGetLastError());
+}
+
+void launch_pre_iterate(Ctx &c, double time, double tau)
+{
+    (void)time;
+    SchemeView s = c.view();
+    if (c.timing) MHFEM_HIP(hipEventRecord(c.ev[0], c.stream));
+    if (c.do_mob || c.advection == MHFEM_EXPLICIT_UPWIND) MHFEM_DISPATCH(c, k_upwind, blocks(c.F), s);
+    if (c.timing) MHFEM_HIP(hipEventRecord(c.ev[1], c.stream));
+    MHFEM_DISPATCH(c, k_cell, blocks(c.C), s, tau);
+    if (c.timing) MHFEM_HIP(hipEventRecord(c.ev[2], c.stream));
+}
+
+void launch_assemble(Ctx &c, double time, double tau)
+{
+    (void)time;
+    (void)tau;
+    SchemeView s = c.view();
+    MHFEM_DISPATCH(c, k_assemble, blocks(c.F), s);
+}
+
+void launch_post_iterate(Ctx &c)
+{
+    SchemeView s = c.view();
+    MHFEM_DISPATCH(c, k_post, blocks(c.C), s);
+}
+
+void launch_zero_double(Ctx &c, double *p, size_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill, (unsigned)((n + WG - 1) / WG), WG, 0, c.stream, p, n, 0.0);
+    MHFEM_HIP(hipGetLastError());
+}
+
+}  // namespace mhfem

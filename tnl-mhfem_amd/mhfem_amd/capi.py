@@ -1,0 +1,270 @@
+"""ctypes binding of libmhfem_amd.so (include/mhfem.h) and a host-side mirror
+of the reference's ``TNL::MHFEM::Solver`` (Solver.h:64-117) and
+``control.h::solve`` (control.h:64-102).
+
+There is no fallback: if the HIP library is missing or fails to load, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libmhfem_amd.so"))
+
+# array identifiers (include/mhfem.h)
+ARRAYS = ["Z_iF", "Z_iK", "N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe", "r_ijK", "f_iK",
+          "v_iKe", "m_iE_upw", "Z_ijE_upw", "b_ijK_storage", "R_ijKe", "R_iK",
+          "bc_values", "bc_dirichlet", "bc_mobility", "rhs", "cell_measure", "face_measure"]
+ARRAY_ID = {name: k for k, name in enumerate(ARRAYS)}
+
+SOLVERS = {"bicgstab": 0, "gmres": 1}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2}
+PRECISIONS = {"fp64": 0, "fp32-refined": 1}
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("num_points", C.c_int), ("num_cells", C.c_int),
+                ("points", C.c_void_p), ("cell_vertices", C.c_void_p)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("num_equations", C.c_int), ("mass_lumping", C.c_int), ("advection", C.c_int),
+                ("do_mobility_upwind", C.c_int)]
+
+
+class SolverConfig(C.Structure):
+    _fields_ = [("solver", C.c_int), ("preconditioner", C.c_int),
+                ("convergence_residue", C.c_double), ("max_iterations", C.c_int),
+                ("gmres_restart", C.c_int), ("precision", C.c_int),
+                ("inner_residue", C.c_double), ("check_every", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int),
+                ("residue", C.c_double), ("converged", C.c_int)]
+
+
+class Timers(C.Structure):
+    _fields_ = [("pre_iterate", C.c_double), ("update_b_R_Q", C.c_double), ("upwind", C.c_double),
+                ("assemble", C.c_double), ("preconditioner", C.c_double),
+                ("linear_solver", C.c_double), ("post_iterate", C.c_double),
+                ("iterations", C.c_longlong), ("time_steps", C.c_longlong)]
+
+
+# every symbol declared in include/mhfem.h with its signature
+_P, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
+SIGNATURES = {
+    "mhfem_create": (_P, [C.POINTER(MeshDesc), C.POINTER(ModelDesc)]),
+    "mhfem_destroy": (None, [_P]),
+    "mhfem_last_error": (C.c_char_p, [_P]),
+    "mhfem_get_counts": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "mhfem_get_topology": (_I, [_P, _P, _P]),
+    "mhfem_set_array": (_I, [_P, _I, _P, _L]),
+    "mhfem_get_array": (_I, [_P, _I, _P, _L]),
+    "mhfem_array_size": (_L, [_P, _I]),
+    "mhfem_device_array": (_P, [_P, _I]),
+    "mhfem_stream": (_P, [_P]),
+    "mhfem_set_boundary_conditions": (_I, [_P, _P, _P, _P]),
+    "mhfem_set_initial_condition": (_I, [_P, _I, _P, _L]),
+    "mhfem_init": (_I, [_P]),
+    "mhfem_setup_linear_system": (_I, [_P, C.POINTER(SolverConfig)]),
+    "mhfem_pre_iterate": (_I, [_P, _D, _D]),
+    "mhfem_assemble": (_I, [_P, _D, _D]),
+    "mhfem_solve": (_I, [_P, C.POINTER(SolveStats)]),
+    "mhfem_post_iterate": (_I, [_P, _D, _D]),
+    "mhfem_solve_interval": (_I, [_P, _D, _D, _D, C.POINTER(SolveStats)]),
+    "mhfem_system_size": (_I, [_P, C.POINTER(_I), C.POINTER(_L)]),
+    "mhfem_export_system": (_I, [_P, _P, _P, _P, _P]),
+    "mhfem_solve_rhs": (_I, [_P, _P, _P, C.POINTER(SolveStats)]),
+    "mhfem_get_timers": (_I, [_P, C.POINTER(Timers)]),
+    "mhfem_reset_timers": (_I, [_P]),
+    "mhfem_set_timing": (_I, [_P, _I]),
+    "mhfem_synchronize": (_I, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmhfem_amd.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmhfem_amd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class MHFEMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Solver:
+    """Host mirror of TNL::MHFEM::Solver over the C ABI.
+
+    Method names follow the reference: ``setMesh`` is the constructor,
+    ``setup``/``setupLinearSystem`` take the solver options, ``preIterate``,
+    ``assembleLinearSystem``, ``solveLinearSystem``, ``postIterate`` form one
+    time step and :meth:`solve` is control.h::solve.
+    """
+
+    def __init__(self, problem, solver=None, preconditioner=None, tol=None, precision="fp64",
+                 inner_tol=1e-4, check_every=8, max_iterations=100000, gmres_restart=30,
+                 setup=True):
+        pr = problem
+        self.problem = pr
+        self._points = np.ascontiguousarray(pr.points, np.float64)
+        self._cells = np.ascontiguousarray(pr.cells, np.int32)
+        mesh = MeshDesc(pr.dim, self._points.shape[0], self._cells.shape[0],
+                        _ptr(self._points), _ptr(self._cells))
+        model = ModelDesc(pr.n, pr.lumping, pr.advection, pr.do_mobility_upwind)
+        L = lib()
+        self.ctx = L.mhfem_create(C.byref(mesh), C.byref(model))
+        if not self.ctx:
+            raise MHFEMError(-1, L.mhfem_last_error(None).decode())
+        c, f, b = C.c_int(), C.c_int(), C.c_int()
+        L.mhfem_get_counts(self.ctx, C.byref(c), C.byref(f), C.byref(b))
+        self.num_cells, self.num_faces, self.num_boundary_faces = c.value, f.value, b.value
+        self.n = pr.n
+        self.iterations = []
+        self.config = SolverConfig(SOLVERS[solver or pr.solver],
+                                   PRECONDITIONERS[preconditioner or pr.preconditioner],
+                                   pr.tol if tol is None else tol, max_iterations, gmres_restart,
+                                   PRECISIONS[precision], inner_tol, check_every)
+        if setup:
+            self.setInitialCondition()
+            self.setupLinearSystem()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MHFEMError(rc, lib().mhfem_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().mhfem_destroy(self.ctx)
+            self.ctx = None
+
+    __del__ = close
+
+    # ---- model data
+    def set_array(self, name, values):
+        v = np.ascontiguousarray(values, np.float64).ravel()
+        self._check(lib().mhfem_set_array(self.ctx, ARRAY_ID[name], _ptr(v), v.size))
+
+    def get_array(self, name):
+        size = lib().mhfem_array_size(self.ctx, ARRAY_ID[name])
+        out = np.empty(size, np.float64)
+        self._check(lib().mhfem_get_array(self.ctx, ARRAY_ID[name], _ptr(out), size))
+        return out
+
+    def topology(self):
+        fpc = self.problem.fpc
+        cf = np.empty((self.num_cells, fpc), np.int32)
+        fc = np.empty((self.num_faces, 2), np.int32)
+        self._check(lib().mhfem_get_topology(self.ctx, _ptr(cf), _ptr(fc)))
+        return cf, fc
+
+    # ---- Solver::setInitialCondition (Solver.hpp:255-344)
+    def setInitialCondition(self):
+        pr = self.problem
+        tags = np.ascontiguousarray(pr.tags, np.uint8)
+        vals = np.ascontiguousarray(pr.bc_values, np.float64)
+        dirv = np.ascontiguousarray(pr.bc_dirichlet, np.float64)
+        self._check(lib().mhfem_set_boundary_conditions(self.ctx, _ptr(tags), _ptr(vals), _ptr(dirv)))
+        for name in ("N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe", "r_ijK", "f_iK"):
+            v = np.asarray(pr.coeffs[name], np.float64)
+            self.set_array(name, v.reshape(1) if v.ndim == 0 else v)
+        if pr.bc_mobility is not None:
+            self.set_array("bc_mobility", pr.bc_mobility)
+        z = np.ascontiguousarray(pr.Z_iK0, np.float64).reshape(self.n, -1)
+        for i in range(self.n):
+            zi = np.ascontiguousarray(z[i])
+            self._check(lib().mhfem_set_initial_condition(self.ctx, i, _ptr(zi), zi.size))
+        self._check(lib().mhfem_init(self.ctx))
+
+    def setupLinearSystem(self):
+        self._check(lib().mhfem_setup_linear_system(self.ctx, C.byref(self.config)))
+
+    # ---- one time step (control.h:83-98)
+    def preIterate(self, t, tau):
+        self._check(lib().mhfem_pre_iterate(self.ctx, t, tau))
+
+    def assembleLinearSystem(self, t, tau):
+        if self.problem.dirichlet_at is not None:
+            self.set_array("bc_dirichlet", self.problem.dirichlet_at(t + tau))
+        self._check(lib().mhfem_assemble(self.ctx, t, tau))
+
+    def solveLinearSystem(self):
+        st = SolveStats()
+        self._check(lib().mhfem_solve(self.ctx, C.byref(st)))
+        self.iterations.append(st.iterations)
+        self.last_stats = st
+        return st
+
+    def postIterate(self, t, tau):
+        self._check(lib().mhfem_post_iterate(self.ctx, t, tau))
+
+    def step(self, t, tau):
+        self.preIterate(t, tau)
+        self.assembleLinearSystem(t, tau)
+        st = self.solveLinearSystem()
+        self.postIterate(t, tau)
+        return st
+
+    def solve(self, t0, t1, dt):
+        """control.h::solve(problem, t0, t1, dt) (control.h:64-102)."""
+        if self.problem.dirichlet_at is None:
+            st = SolveStats()
+            self._check(lib().mhfem_solve_interval(self.ctx, t0, t1, dt, C.byref(st)))
+            return st
+        t = t0
+        st = None
+        while t1 - t > dt * 1e-6:
+            tau = min(dt, t1 - t)
+            st = self.step(t, tau)
+            t += tau
+        return st
+
+    # ---- linear system
+    def export_system(self):
+        rows, nnz = C.c_int(), C.c_long()
+        self._check(lib().mhfem_system_size(self.ctx, C.byref(rows), C.byref(nnz)))
+        rp = np.empty(rows.value + 1, np.int32)
+        col = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.float64)
+        rhs = np.empty(rows.value, np.float64)
+        self._check(lib().mhfem_export_system(self.ctx, _ptr(rp), _ptr(col), _ptr(val), _ptr(rhs)))
+        return rp, col, val, rhs
+
+    def solve_rhs(self, rhs, x0):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.array(x0, np.float64, copy=True)
+        st = SolveStats()
+        self._check(lib().mhfem_solve_rhs(self.ctx, _ptr(rhs), _ptr(x), C.byref(st)))
+        return x, st
+
+    def synchronize(self):
+        self._check(lib().mhfem_synchronize(self.ctx))
+
+    def set_timing(self, on=True):
+        self._check(lib().mhfem_set_timing(self.ctx, 1 if on else 0))
+
+    def timers(self):
+        t = Timers()
+        self._check(lib().mhfem_get_timers(self.ctx, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timers._fields_}

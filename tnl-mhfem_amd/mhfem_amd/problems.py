@@ -1,0 +1,237 @@
+"""The benchmark configurations as model-coefficient sets.
+
+A :class:`Problem` carries everything the reference takes from its mesh file,
+boundary-conditions file, initial condition and model ``init``:
+the mesh, the number of equations ``n``, the per-element coefficients
+``N, u, m, D, w, a, r, f`` in the reference's CUDA layouts
+(BaseModel.h:50-124; a 0-d array means a uniform coefficient,
+UniformCoefficient.h:7-53), the boundary tags/values/Dirichlet values in the
+``i*F+E`` layout (BoundaryConditionsStorage.h:9-15), the initial ``Z_iK`` and
+the time-stepping and solver settings.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import meshgen
+
+FIXED_VALUE, FIXED_FLUX, ADVECTIVE_OUTFLOW, FIXED_FLUX_NO_ADVECTION = 0, 1, 2, 3
+EXPLICIT_UPWIND, IMPLICIT_UPWIND, IMPLICIT_TRACE = 0, 1, 2
+
+COEFFICIENTS = ("N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe", "r_ijK", "f_iK")
+
+
+@dataclass
+class Problem:
+    name: str
+    points: np.ndarray
+    cells: np.ndarray
+    n: int
+    coeffs: dict
+    tags: np.ndarray
+    bc_values: np.ndarray
+    bc_dirichlet: np.ndarray
+    Z_iK0: np.ndarray
+    dt: float
+    steps: int
+    solver: str = "bicgstab"
+    preconditioner: str = "diagonal"
+    tol: float = 1e-10
+    lumping: int = 0
+    advection: int = EXPLICIT_UPWIND
+    do_mobility_upwind: int = 1
+    bc_mobility: Optional[np.ndarray] = None
+    cell_faces: Optional[np.ndarray] = None
+    face_cells: Optional[np.ndarray] = None
+    # time-dependent Dirichlet values: f(time) -> array (i*F+E layout) or None
+    dirichlet_at: Optional[Callable[[float], np.ndarray]] = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def dim(self):
+        return self.points.shape[1]
+
+    @property
+    def num_cells(self):
+        return self.cells.shape[0]
+
+    @property
+    def num_faces(self):
+        return self.face_cells.shape[0]
+
+    @property
+    def fpc(self):
+        return self.cells.shape[1]
+
+    def coefficient_array(self, name):
+        """Full (non-uniform) array of a coefficient in the CUDA layout."""
+        n, C, fpc = self.n, self.num_cells, self.fpc
+        size = {
+            "N_ijK": n * n * C, "D_ijK": n * n * C, "r_ijK": n * n * C,
+            "u_ijKe": n * n * fpc * C, "a_ijKe": n * n * fpc * C,
+            "m_iK": n * C, "f_iK": n * C, "w_iKe": n * fpc * C,
+        }[name]
+        v = np.asarray(self.coeffs[name], dtype=np.float64)
+        if v.ndim == 0:
+            return np.full(size, float(v))
+        assert v.size == size, (name, v.size, size)
+        return v.ravel()
+
+
+def _topology(points, cells):
+    cf, fc = meshgen.face_numbering(cells, points.shape[0])
+    return cf, fc
+
+
+def _boundary_mask(fc):
+    return fc[:, 1] < 0
+
+
+def _ij_blocks(n, C, blocks):
+    """Stack per-(i,j) cell arrays into the (i,j,K) layout."""
+    out = np.empty((n, n, C))
+    for i in range(n):
+        for j in range(n):
+            out[i, j] = blocks[i][j]
+    return out.ravel()
+
+
+def heat_2d(s: int = 64, dt: float = 1e-4, steps: int = 100, lumping: int = 0) -> Problem:
+    """2-D heat equation on the unit square (HeatEquationModel.h:114-133):
+    N=1, m=1, D=1, u=w=a=r=f=0, zero Dirichlet on the whole boundary
+    (init.py:38-40), Z0 = exp(-((x-.5)^2+(y-.5)^2)/0.01) at cell centroids,
+    BiCGStab + Jacobi to 1e-10."""
+    points, cells = meshgen.unit_square_triangles(s)
+    cf, fc = _topology(points, cells)
+    F = fc.shape[0]
+    coeffs = dict(N_ijK=np.array(1.0), u_ijKe=np.array(0.0), m_iK=np.array(1.0),
+                  D_ijK=np.array(1.0), w_iKe=np.array(0.0), a_ijKe=np.array(0.0),
+                  r_ijK=np.array(0.0), f_iK=np.array(0.0))
+    c = meshgen.cell_centroids(points, cells)
+    z0 = np.exp(-((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.5) ** 2) / 0.01)
+    return Problem("heat_2d", points, cells, 1, coeffs,
+                   tags=np.full(F, FIXED_VALUE, np.uint8), bc_values=np.zeros(F),
+                   bc_dirichlet=np.zeros(F), Z_iK0=z0, dt=dt, steps=steps,
+                   solver="bicgstab", preconditioner="diagonal", lumping=lumping,
+                   do_mobility_upwind=0, bc_mobility=np.ones(F), cell_faces=cf, face_cells=fc)
+
+
+def _cube_lnK(s, sigma, corr_len, seed):
+    g = meshgen.gaussian_field((s, s, s), sigma, corr_len, seed)  # indexed [ix, iy, iz]
+    # cube (ix,iy,iz) is visited with ix fastest -> transpose to [iz,iy,ix]
+    per_cube = np.transpose(g, (2, 1, 0)).ravel()
+    return np.repeat(per_cube, 6)
+
+
+def darcy_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 20,
+             preconditioner: str = "ilu0") -> Problem:
+    """3-D Darcy-type pressure on the unit cube, Kuhn tetrahedra: n=1, N=0.2,
+    m=1, D=K with ln K a seeded Gaussian field (exponential covariance,
+    sigma=1, correlation length 0.1, constant per cube); Z=1 at x=0, Z=0 at
+    x=1, no-flow (FixedFluxNoAdvection, J=0) elsewhere; Z0=0."""
+    points, cells = meshgen.unit_cube_kuhn(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    K = np.exp(_cube_lnK(s, 1.0, 0.1, seed))
+    coeffs = dict(N_ijK=np.array(0.2), u_ijKe=np.array(0.0), m_iK=np.array(1.0), D_ijK=K,
+                  w_iKe=np.array(0.0), a_ijKe=np.array(0.0), r_ijK=np.array(0.0),
+                  f_iK=np.array(0.0))
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    bnd = _boundary_mask(fc)
+    tags = np.full(F, FIXED_VALUE, np.uint8)
+    tags[bnd] = FIXED_FLUX_NO_ADVECTION
+    x0 = bnd & (np.abs(fcen[:, 0]) < 1e-12)
+    x1 = bnd & (np.abs(fcen[:, 0] - 1.0) < 1e-12)
+    tags[x0 | x1] = FIXED_VALUE
+    dirichlet = np.zeros(F)
+    dirichlet[x0] = 1.0
+    return Problem("darcy_3d", points, cells, 1, coeffs, tags=tags, bc_values=np.zeros(F),
+                   bc_dirichlet=dirichlet, Z_iK0=np.zeros(C), dt=dt, steps=steps,
+                   solver="bicgstab", preconditioner=preconditioner, do_mobility_upwind=1,
+                   bc_mobility=np.ones(F), cell_faces=cf, face_cells=fc,
+                   extra=dict(seed=seed, s=s))
+
+
+def adr_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 50,
+           preconditioner: str = "ilu0", time_dependent: bool = False) -> Problem:
+    """n=2 coupled advection-diffusion-reaction on the Darcy mesh and K field:
+    D11=D22=K, D12=D21=0.1K, N=0.2 I, m=1, constant advection a_11=a_22=(1,.5,.25)
+    projected onto the RTN0 face fluxes (a_iiKe = |E_e| a.n_{K,e}), explicit
+    upwind, r=[[1e-2,-5e-3],[-5e-3,1e-2]], f=0, Z0=0; Dirichlet Z=(1,0.5) at
+    x=0, AdvectiveOutflow at x=1, FixedFluxNoAdvection (J=0) elsewhere.
+    ``time_dependent`` makes Z_1 = sin(2 pi t / 0.5) at x=0 (evaluated at
+    time+tau, Solver.hpp:948)."""
+    points, cells = meshgen.unit_cube_kuhn(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    n = 2
+    K = np.exp(_cube_lnK(s, 1.0, 0.1, seed))
+    zero = np.zeros(C)
+    D = _ij_blocks(n, C, [[K, 0.1 * K], [0.1 * K, K]])
+    N = _ij_blocks(n, C, [[0.2 + zero, zero], [zero, 0.2 + zero]])
+    r = _ij_blocks(n, C, [[1e-2 + zero, -5e-3 + zero], [-5e-3 + zero, 1e-2 + zero]])
+    fv = meshgen.outward_face_vectors(points, cells)  # (C, 4, 3)
+    avec = np.array([1.0, 0.5, 0.25])
+    flux = (fv * avec).sum(axis=2)  # (C, 4)
+    a = np.zeros((n, n, 4, C))
+    for i in range(n):
+        a[i, i] = flux.T
+    coeffs = dict(N_ijK=N, u_ijKe=np.array(0.0), m_iK=np.array(1.0), D_ijK=D,
+                  w_iKe=np.array(0.0), a_ijKe=a.ravel(), r_ijK=r, f_iK=np.array(0.0))
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    bnd = _boundary_mask(fc)
+    x0 = bnd & (np.abs(fcen[:, 0]) < 1e-12)
+    x1 = bnd & (np.abs(fcen[:, 0] - 1.0) < 1e-12)
+    tags = np.full((n, F), FIXED_VALUE, np.uint8)
+    tags[:, bnd] = FIXED_FLUX_NO_ADVECTION
+    tags[:, x1] = ADVECTIVE_OUTFLOW
+    tags[:, x0] = FIXED_VALUE
+    dirichlet = np.zeros((n, F))
+    dirichlet[0, x0] = 1.0
+    dirichlet[1, x0] = 0.5
+    dirichlet_at = None
+    if time_dependent:
+        base = dirichlet.ravel().copy()
+        idx = np.nonzero(x0)[0]
+
+        def dirichlet_at(t, base=base, idx=idx):
+            out = base.copy()
+            out[idx] = np.sin(2 * np.pi * t / 0.5)
+            return out
+
+        dirichlet[0, x0] = 0.0
+    return Problem("adr_3d", points, cells, n, coeffs, tags=tags.ravel(),
+                   bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
+                   Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
+                   preconditioner=preconditioner, do_mobility_upwind=1,
+                   bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc,
+                   dirichlet_at=dirichlet_at, extra=dict(seed=seed, s=s))
+
+
+def ensemble_sizes(seed: int, count: int = 96, lo: int = 32, hi: int = 256):
+    rng = np.random.default_rng(seed)
+    return rng.integers(lo, hi + 1, size=count)
+
+
+def ensemble_member(s: int, seed: int, dt: float = 1e-4, steps: int = 50) -> Problem:
+    """One member of the batched ensemble: heat-type model on an s x s square
+    mesh with D = K, ln K Gaussian (sigma=1.5, correlation length 0.05)."""
+    points, cells = meshgen.unit_square_triangles(s)
+    cf, fc = _topology(points, cells)
+    F = fc.shape[0]
+    g = meshgen.gaussian_field((s, s), 1.5, 0.05, seed)  # [ix, iy]
+    per_sq = g.T.ravel()
+    K = np.exp(np.repeat(per_sq, 2))
+    coeffs = dict(N_ijK=np.array(1.0), u_ijKe=np.array(0.0), m_iK=np.array(1.0), D_ijK=K,
+                  w_iKe=np.array(0.0), a_ijKe=np.array(0.0), r_ijK=np.array(0.0),
+                  f_iK=np.array(0.0))
+    c = meshgen.cell_centroids(points, cells)
+    z0 = np.exp(-((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.5) ** 2) / 0.01)
+    return Problem(f"ensemble_{s}", points, cells, 1, coeffs,
+                   tags=np.full(F, FIXED_VALUE, np.uint8), bc_values=np.zeros(F),
+                   bc_dirichlet=np.zeros(F), Z_iK0=z0, dt=dt, steps=steps,
+                   solver="bicgstab", preconditioner="diagonal", do_mobility_upwind=0,
+                   bc_mobility=np.ones(F), cell_faces=cf, face_cells=fc, extra=dict(seed=seed))
