@@ -167,6 +167,10 @@ int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *v
 int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats);
 
 /* ---- instrumentation (Solver::writeEpilog, Solver.hpp:1326-1391) */
+/* average duration (ms) of the solver's SpMV (y = A x, current matrix) over
+ * `reps` launches on the context's stream; precision selects the FP64 or the
+ * FP32 copy of the matrix.  Measurement aid, no counterpart in the reference. */
+int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms);
 int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out);
 int mhfem_reset_timers(mhfem_ctx *ctx);
 int mhfem_set_timing(mhfem_ctx *ctx, int enabled); /* HIP-event phase timers (default off) */

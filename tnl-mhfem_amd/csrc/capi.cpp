@@ -721,6 +721,14 @@ int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_st
     });
 }
 
+int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready || !avg_ms) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        krylov_bench_spmv(c, precision, reps, avg_ms);
+    });
+}
+
 int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out)
 {
     const Ctx *c = reinterpret_cast<const Ctx *>(ctx);

@@ -157,5 +157,6 @@ Krylov *krylov_create(Ctx &c);
 void krylov_destroy(Krylov *k);
 // solve A x = b (b, x device arrays of length rows); returns stats
 void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st);
+void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms);
 
 }  // namespace mhfem

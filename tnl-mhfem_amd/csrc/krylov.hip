@@ -4,9 +4,10 @@
 //   every matrix load of a wavefront is one contiguous 512-byte segment.
 // * BiCGStab (left preconditioned, van der Vorst 1992) as five fused kernels
 //   per iteration; dot products reduce wavefront -> work-group -> grid
-//   (deterministic "last block" reduction) and the scalar recurrences
+//   (deterministic order) and the scalar recurrences
 //   (alpha, omega, beta, residue, convergence) are evaluated on the device, so
-//   the host only polls a status word every `check_every` iterations.  Kernels
+//   the host only polls a status word every `check_every` iterations.
+//   Partial sums are finished by a one-work-group kernel on the same stream.  Kernels
 //   queued after convergence see the DONE flag and return immediately.
 // * GMRES(m), left preconditioned, classical Gram-Schmidt with one
 //   re-orthogonalisation pass; Givens rotations on the host.
@@ -35,9 +36,8 @@ enum Slot {
 enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT };
 
 struct Red {
-    double *partials;
-    unsigned *counter;
-    double *sc;
+    double *partials;  // one row of per-work-group partial sums per dot product
+    double *sc;        // device scalar slots
 };
 
 __device__ __forceinline__ bool halted(const double *sc) { return sc[S_DONE] != 0.0 || sc[S_RESTART] != 0.0; }
@@ -87,13 +87,17 @@ __device__ void finalize(double *sc, int fin)
     }
 }
 
-// Reduce ND per-thread values over the grid into sc[S_D0 + d]; the last
-// work-group to arrive finishes the reduction and runs `fin`.
+// Reduce ND per-thread values to one partial sum per work-group
+// (wavefront shuffles, then LDS).  The partials are summed by k_finalize,
+// launched next on the same stream: a kernel boundary orders the writes, so
+// no work-group issues a device-scope fence (which on gfx950 writes back the
+// L2 of its XCD and, once per work-group, costs more than the kernel's own
+// streaming).
 template <int ND>
 __device__ __forceinline__ void grid_reduce(double (&v)[ND], Red red, int fin)
 {
+    (void)fin;
     __shared__ double sm[ND][KB / 64];
-    __shared__ int last;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 #pragma unroll
     for (int d = 0; d < ND; d++) {
@@ -109,24 +113,24 @@ __device__ __forceinline__ void grid_reduce(double (&v)[ND], Red red, int fin)
             for (int q = 1; q < KB / 64; q++) s += sm[d][q];
             red.partials[(long)d * gridDim.x + blockIdx.x] = s;
         }
-        __threadfence();
-        const unsigned t = atomicAdd(red.counter, 1u);
-        last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc[ND];
+}
+
+// Sum the per-work-group partials of the previous kernel into sc[S_D0 + d]
+// and evaluate the scalar recurrence `fin`.  One work-group.
+template <int ND>
+__global__ void __launch_bounds__(KB) k_finalize(int G, Red red, int fin, int check_halt)
+{
+    if (check_halt && halted(red.sc)) return;
+    __shared__ double sm[ND][KB / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 #pragma unroll
     for (int d = 0; d < ND; d++) {
         double a = 0;
-        for (int b = tid; b < (int)gridDim.x; b += KB) a += red.partials[(long)d * gridDim.x + b];
-        acc[d] = wave_sum(a);
+        for (int b = tid; b < G; b += KB) a += red.partials[(long)d * G + b];
+        a = wave_sum(a);
+        if (lane == 0) sm[d][wid] = a;
     }
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int d = 0; d < ND; d++) sm[d][wid] = acc[d];
     __syncthreads();
     if (tid == 0) {
 #pragma unroll
@@ -137,7 +141,6 @@ __device__ __forceinline__ void grid_reduce(double (&v)[ND], Red red, int fin)
             red.sc[S_D0 + d] = s;
         }
         finalize(red.sc, fin);
-        *red.counter = 0;
     }
 }
 
@@ -369,7 +372,6 @@ struct Krylov {
     DBuf<double> r, r_ast, p, Ap, s, As, tmp, tmp2, dinv;
     DBuf<int64_t> diag_pos;  // SELL position of the diagonal of each row
     DBuf<double> partials;
-    DBuf<unsigned> counter;
     DBuf<double> sc;
     double *h_sc = nullptr;  // pinned
     Ilu0 ilu;
@@ -390,8 +392,6 @@ Krylov *krylov_create(Ctx &c)
     for (DBuf<double> *b : {&k->r, &k->r_ast, &k->p, &k->Ap, &k->s, &k->As, &k->tmp, &k->tmp2, &k->dinv})
         b->alloc(n);
     k->partials.alloc(4 * MAXG);
-    k->counter.alloc(1);
-    MHFEM_HIP(hipMemsetAsync(k->counter.p, 0, sizeof(unsigned), c.stream));
     k->sc.alloc(NSLOTS);
     MHFEM_HIP(hipHostMalloc((void **)&k->h_sc, NSLOTS * sizeof(double), hipHostMallocDefault));
     // diagonal positions in SELL storage
@@ -476,7 +476,7 @@ struct Solve {
 
     Solve(Ctx &c_, Krylov &k_) : c(c_), k(k_), n(k_.n), g(grid_for(k_.n))
     {
-        red = Red{k.partials.p, k.counter.p, k.sc.p};
+        red = Red{k.partials.p, k.sc.p};
     }
 
     template <typename T>
@@ -504,6 +504,13 @@ struct Solve {
     void dots(DotArgs<T, ND> a, int fin, bool check_halt)
     {
         hipLaunchKernelGGL((k_dots<T, ND>), g, KB, 0, c.stream, n, a, red, fin, check_halt ? 1 : 0);
+        finish<ND>(fin, check_halt);
+    }
+    // sum the partials of the reducing kernel just launched (grid g)
+    template <int ND>
+    void finish(int fin, bool check_halt)
+    {
+        hipLaunchKernelGGL((k_finalize<ND>), 1, KB, 0, c.stream, (int)g, red, fin, check_halt ? 1 : 0);
         MHFEM_HIP(hipGetLastError());
     }
 
@@ -551,8 +558,10 @@ struct Solve {
             if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, true);
         } else if (pc == MHFEM_PREC_DIAGONAL) {
             hipLaunchKernelGGL((k_spmv<T, true, ND>), g, KB, 0, c.stream, A, x, dinv, y, z, red, fin);
+            if constexpr (ND > 0) finish<ND>(fin, true);
         } else {
             hipLaunchKernelGGL((k_spmv<T, false, ND>), g, KB, 0, c.stream, A, x, dinv, y, z, red, fin);
+            if constexpr (ND > 0) finish<ND>(fin, true);
         }
         MHFEM_HIP(hipGetLastError());
     }
@@ -600,6 +609,7 @@ struct Solve {
                 hipLaunchKernelGGL((k_bicg_s<T>), g, KB, 0, c.stream, n, r, Ap, s, k.sc.p);
                 pmatvec<T, 2>(s, As, tmp, dinv, s, FIN_OMEGA);
                 hipLaunchKernelGGL((k_bicg_xr<T>), g, KB, 0, c.stream, n, x, p, s, As, r, r_ast, red);
+                finish<2>(FIN_UPDATE, true);
                 hipLaunchKernelGGL((k_bicg_p<T>), g, KB, 0, c.stream, n, r, p, Ap, k.sc.p);
                 MHFEM_HIP(hipGetLastError());
             }
@@ -806,6 +816,38 @@ static void precond_update(Ctx &c, Krylov &k, Solve &S)
         MHFEM_HIP(hipGetLastError());
     }
     (void)S;
+}
+
+// Average duration of the solver's SpMV launch (y = A x on the current
+// matrix), measured with HIP events on the context's stream.
+void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
+{
+    Krylov &k = *c.krylov;
+    Solve S(c, k);
+    const unsigned g = grid_for(k.n);
+    hipEvent_t e0 = c.ev[12], e1 = c.ev[13];
+    const double zero[NSLOTS] = {};
+    MHFEM_HIP(hipMemcpyAsync(k.sc.p, zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
+    if (precision == MHFEM_PRECISION_FP32_REFINED) {
+        if (!k.val32.p) throw Error{MHFEM_E_STATE, "FP32 matrix not configured"};
+        hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)k.n, c.arr[MHFEM_Z_iF].p, k.x32.p);
+    }
+    for (int rep = -2; rep < reps; rep++) {
+        if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
+        if (precision == MHFEM_PRECISION_FP32_REFINED)
+            hipLaunchKernelGGL((k_spmv<float, false, 0>), g, KB, 0, c.stream, S.matrix<float>(), k.x32.p,
+                               (const float *)nullptr, k.tmp32.p, (const float *)nullptr, S.red, 0);
+        else
+            hipLaunchKernelGGL((k_spmv<double, false, 0>), g, KB, 0, c.stream, S.matrix<double>(),
+                               c.arr[MHFEM_Z_iF].p, (const double *)nullptr, k.tmp.p, (const double *)nullptr,
+                               S.red, 0);
+    }
+    MHFEM_HIP(hipGetLastError());
+    MHFEM_HIP(hipEventRecord(e1, c.stream));
+    MHFEM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MHFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = reps > 0 ? ms / reps : 0.0;
 }
 
 void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)

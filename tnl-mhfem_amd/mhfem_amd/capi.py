@@ -80,6 +80,7 @@ SIGNATURES = {
     "mhfem_system_size": (_I, [_P, C.POINTER(_I), C.POINTER(_L)]),
     "mhfem_export_system": (_I, [_P, _P, _P, _P, _P]),
     "mhfem_solve_rhs": (_I, [_P, _P, _P, C.POINTER(SolveStats)]),
+    "mhfem_benchmark_spmv": (_I, [_P, _I, _I, C.POINTER(_D)]),
     "mhfem_get_timers": (_I, [_P, C.POINTER(Timers)]),
     "mhfem_reset_timers": (_I, [_P]),
     "mhfem_set_timing": (_I, [_P, _I]),
@@ -257,6 +258,11 @@ class Solver:
         st = SolveStats()
         self._check(lib().mhfem_solve_rhs(self.ctx, _ptr(rhs), _ptr(x), C.byref(st)))
         return x, st
+
+    def benchmark_spmv(self, precision="fp64", reps=50):
+        ms = C.c_double()
+        self._check(lib().mhfem_benchmark_spmv(self.ctx, PRECISIONS[precision], reps, C.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         self._check(lib().mhfem_synchronize(self.ctx))
