@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libmhfem_oracle.so")
+# MHFEM_ORACLE_LIB lets a mutation check point the pins at a deliberately
+# broken copy of the oracle
+LIB_PATH = os.environ.get("MHFEM_ORACLE_LIB", os.path.join(_HERE, "_build", "libmhfem_oracle.so"))
 
 SOLVERS = {"bicgstab": 0, "gmres": 1}
 PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2}

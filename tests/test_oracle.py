@@ -1,0 +1,436 @@
+"""Pins of the CPU oracle against facts that do not come from its own code.
+
+The reference ships no tests, fixtures or golden vectors (its example meshes
+are Git-LFS stubs), so every check here comes from mathematics or from an
+independent construction:
+  * topology: Euler's relation for the face counts, incidence invariants and
+    agreement of two independent implementations of the numbering rule;
+  * local mass matrices: exact integration of the RTN0 basis (barycentric
+    moments), vertex quadrature for the lumped variant;
+  * scheme + assembly + solve + recovery: the full (non-hybridised) mixed
+    system assembled densely from the quadrature mass matrices and solved with
+    numpy, for n = 1 and n = 2, all boundary-condition types;
+  * the linear patch test and mass conservation;
+  * ILU(0): (LU)_ij == A_ij on the pattern; Krylov: scipy's direct solve.
+"""
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from mhfem_amd import meshgen, problems
+import pyoracle
+
+
+# ---------------------------------------------------------------- topology
+@pytest.mark.parametrize("s", [1, 2, 5, 17])
+def test_triangle_face_counts(s):
+    pts, cells = meshgen.unit_square_triangles(s)
+    cf, fc = pyoracle.topology(pts, cells)
+    C = cells.shape[0]
+    nb = int((fc[:, 1] < 0).sum())
+    assert C == 2 * s * s
+    assert nb == 4 * s                       # boundary edges of the square
+    assert 3 * C == 2 * (fc.shape[0] - nb) + nb  # each cell has 3 faces
+    assert fc.shape[0] == 3 * s * s + 2 * s
+
+
+@pytest.mark.parametrize("s", [1, 2, 3, 6])
+def test_tetrahedron_face_counts(s):
+    pts, cells = meshgen.unit_cube_kuhn(s)
+    cf, fc = pyoracle.topology(pts, cells)
+    C = cells.shape[0]
+    nb = int((fc[:, 1] < 0).sum())
+    assert C == 6 * s ** 3
+    assert nb == 12 * s * s                  # 2 triangles per boundary square
+    assert 4 * C == 2 * (fc.shape[0] - nb) + nb
+    assert fc.shape[0] == 12 * s ** 3 + 6 * s * s
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_topology_invariants(mesh):
+    pts, cells = meshgen.unit_square_triangles(7) if mesh == "tri" else meshgen.unit_cube_kuhn(4)
+    cf, fc = pyoracle.topology(pts, cells)
+    nc, fpc = cells.shape
+    # local face e does not contain local vertex e and contains all the others
+    fv = {}
+    for K in range(nc):
+        for e in range(fpc):
+            verts = frozenset(int(v) for q, v in enumerate(cells[K]) if q != e)
+            E = cf[K, e]
+            if E in fv:
+                assert fv[E] == verts
+            fv[E] = verts
+    # every face id appears; first appearances are in increasing order
+    flat = cf.ravel()
+    _, first = np.unique(flat, return_index=True)
+    assert np.all(np.diff(first) > 0)
+    # face -> cells and cell -> faces are inverse relations, cells ascending
+    for E in range(fc.shape[0]):
+        K0, K1 = fc[E]
+        assert E in cf[K0]
+        if K1 >= 0:
+            assert K0 < K1 and E in cf[K1]
+    # boundary faces lie on the boundary of the unit domain
+    cen = meshgen.face_centroids(pts, cells, cf, fc.shape[0])
+    on_bnd = np.any((np.abs(cen) < 1e-12) | (np.abs(cen - 1) < 1e-12), axis=1)
+    assert np.array_equal(on_bnd, fc[:, 1] < 0)
+    # the numpy implementation of the rule agrees bit for bit
+    cf2, fc2 = meshgen.face_numbering(cells, pts.shape[0])
+    assert np.array_equal(cf, cf2) and np.array_equal(fc, fc2)
+
+
+# ---------------------------------------------------------------- mass matrix
+def exact_mass(P):
+    """M_ef = int_K w_e . w_f, w_e = (x - v_e) / (d |K|), by barycentric moments
+    int_K l_k l_l = |K| (1 + delta_kl) / ((d+1)(d+2))."""
+    d = P.shape[1]
+    n = d + 1
+    T = np.ones((n, n)); T[:, 1:] = P
+    vol = abs(np.linalg.det(T)) / math.factorial(d)
+    mom = vol * (np.ones((n, n)) + np.eye(n)) / ((d + 1) * (d + 2))
+    M = np.zeros((n, n))
+    for e in range(n):
+        for f in range(n):
+            De = P - P[e]
+            Df = P - P[f]
+            M[e, f] = np.einsum("kl,kl->", mom, De @ Df.T) / (d * vol) ** 2
+    return M, vol
+
+
+def lumped_mass(P):
+    """vertex quadrature: |K|/(d+1) sum_v w_e(v) . w_f(v)"""
+    d = P.shape[1]
+    n = d + 1
+    T = np.ones((n, n)); T[:, 1:] = P
+    vol = abs(np.linalg.det(T)) / math.factorial(d)
+    M = np.zeros((n, n))
+    for e in range(n):
+        for f in range(n):
+            M[e, f] = vol / n * sum(np.dot(P[v] - P[e], P[v] - P[f]) for v in range(n)) / (d * vol) ** 2
+    return M
+
+
+def unpack(packed, n):
+    M = np.zeros((n, n))
+    s = 0
+    for c in range(n):
+        for r in range(c + 1):
+            M[r, c] = M[c, r] = packed[s]
+            s += 1
+    return M, packed[s:s + n]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("lumping", [0, 1])
+def test_local_mass_inverse_matches_quadrature(dim, lumping):
+    rng = np.random.default_rng(10 + dim + lumping)
+    for trial in range(25):
+        P = rng.standard_normal((dim + 1, dim))
+        if trial == 0:  # a sliver
+            P[-1] = P[0] + 1e-3 * (P[1] - P[0]) + 1e-3 * rng.standard_normal(dim)
+        D = rng.uniform(0.1, 10.0)
+        out = pyoracle.local_mass_inverse(P, lumping, D)
+        Minv, rowsum = unpack(out, dim + 1)
+        M = lumped_mass(P) if lumping else exact_mass(P)[0]
+        ref = D * np.linalg.inv(M)
+        # inversion error grows with the condition number (slivers)
+        tol = max(1e-12, 1e-14 * np.linalg.cond(M))
+        assert np.abs(Minv - ref).max() / np.abs(ref).max() < tol
+        assert np.allclose(rowsum, Minv.sum(axis=1), rtol=1e-12, atol=1e-12 * np.abs(Minv).max())
+
+
+def test_mass_matrix_spd_and_unit_triangle():
+    # unit right triangle: the integrals of the RTN0 basis by hand give
+    # M = [[1/2, -1/6, 1/6]... ] only through the moments; check SPD instead
+    P = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    out = pyoracle.local_mass_inverse(P, 0, 1.0)
+    Minv, _ = unpack(out, 3)
+    assert np.all(np.linalg.eigvalsh(Minv) > 0)
+    M, vol = exact_mass(P)
+    # w_0 = (x - v0)/(2|K|) integrates |x|^2/(4|K|^2): int_K x^2+y^2 = 1/6 -> M00 = (1/6)/(4/4) = 1/6
+    assert abs(M[0, 0] - 1.0 / 6.0) < 1e-14
+    assert np.allclose(np.linalg.inv(Minv), M, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- brute force
+def _mixed_reference(pr, o_pts, tau, Z_old, Z_F_old, m_upw, bc_t):
+    """Dense full mixed system (velocities, cell and face unknowns) for one
+    time step, explicit upwind advection, from quadrature mass matrices."""
+    pts, cells = pr.points, pr.cells
+    cf, fc = pr.cell_faces, pr.face_cells
+    n, C, F, fpc = pr.n, pr.num_cells, pr.num_faces, pr.fpc
+    dim = pts.shape[1]
+    co = {k: pr.coefficient_array(k) for k in problems.COEFFICIENTS}
+    Nij = co["N_ijK"].reshape(n, n, C)
+    Dij = co["D_ijK"].reshape(n, n, C)
+    rij = co["r_ijK"].reshape(n, n, C)
+    fi = co["f_iK"].reshape(n, C)
+    wi = co["w_iKe"].reshape(n, fpc, C)
+    vel = (co["a_ijKe"] + co["u_ijKe"]).reshape(n, n, fpc, C)
+    uu = co["u_ijKe"].reshape(n, n, fpc, C)
+    nv = n * C * fpc
+    iv = lambda i, K, e: (i * C + K) * fpc + e
+    ik = lambda i, K: nv + i * C + K
+    iF = lambda i, E: nv + n * C + i * F + E
+    N = nv + n * C + n * F
+    A = np.zeros((N, N))
+    b = np.zeros(N)
+    Minv = []
+    vols = []
+    for K in range(C):
+        M, vol = (exact_mass(pts[cells[K]]) if not pr.lumping else (lumped_mass(pts[cells[K]]), None))
+        if vol is None:
+            T = np.ones((fpc, fpc)); T[:, 1:] = pts[cells[K]]
+            vol = abs(np.linalg.det(T)) / math.factorial(dim)
+        Minv.append(np.linalg.inv(M))
+        vols.append(vol)
+    # upwinded Z on faces (Solver.hpp:743-759): from the old state
+    def zupw(i, j, E):
+        K1, K2 = fc[E]
+        e1 = list(cf[K1]).index(E)
+        apu = vel[i, j, e1, K1]
+        if apu > 0:
+            return Z_old[j, K1]
+        if apu == 0:
+            return 0.0
+        return Z_old[j, K2] if K2 >= 0 else Z_F_old[j, E]
+    r = 0
+    for K in range(C):
+        for i in range(n):
+            for e in range(fpc):
+                # v_iKe - sum_j D_ij Minv (Z_jK - Z_jF) = w_iKe
+                A[r, iv(i, K, e)] = 1.0
+                for j in range(n):
+                    for f in range(fpc):
+                        bb = Dij[i, j, K] * Minv[K][e, f]
+                        A[r, ik(j, K)] -= bb
+                        A[r, iF(j, cf[K, f])] += bb
+                b[r] = wi[i, e, K]
+                r += 1
+    for K in range(C):
+        for i in range(n):
+            # sum_e m v_iKe + sum_j (|K| N_ij / tau + |K| r_ij - sum_e u_ijKe) Z_jK = rhs
+            for e in range(fpc):
+                A[r, iv(i, K, e)] += m_upw[cf[K, e], i]
+            rhs = vols[K] * fi[i, K]
+            for j in range(n):
+                A[r, ik(j, K)] += vols[K] * Nij[i, j, K] / tau + vols[K] * rij[i, j, K] - uu[i, j, :, K].sum()
+                rhs += vols[K] * Nij[i, j, K] / tau * Z_old[j, K]
+                for e in range(fpc):
+                    rhs -= vel[i, j, e, K] * zupw(i, j, cf[K, e])
+            b[r] = rhs
+            r += 1
+    for E in range(F):
+        K0, K1 = fc[E]
+        e0 = list(cf[K0]).index(E)
+        meas = np.linalg.norm(np.subtract(*pts[[v for q, v in enumerate(cells[K0]) if q != e0][:2]])) if dim == 2 else None
+        if dim == 3:
+            vv = pts[[v for q, v in enumerate(cells[K0]) if q != e0]]
+            meas = 0.5 * np.linalg.norm(np.cross(vv[1] - vv[0], vv[2] - vv[0]))
+        for i in range(n):
+            if K1 >= 0:
+                e1 = list(cf[K1]).index(E)
+                A[r, iv(i, K0, e0)] = 1.0
+                A[r, iv(i, K1, e1)] = 1.0
+            else:
+                tag = pr.tags[i * F + E]
+                if tag == problems.FIXED_VALUE:
+                    A[r, iF(i, E)] = 1.0
+                    b[r] = bc_t[i * F + E]
+                elif tag == problems.FIXED_FLUX_NO_ADVECTION:
+                    A[r, iv(i, K0, e0)] = 1.0
+                    b[r] = pr.bc_values[i * F + E] * meas
+                elif tag == problems.FIXED_FLUX:
+                    A[r, iv(i, K0, e0)] = m_upw[E, i]
+                    b[r] = pr.bc_values[i * F + E] * meas - sum(vel[i, j, e0, K0] * zupw(i, j, E) for j in range(n))
+                elif tag == problems.ADVECTIVE_OUTFLOW:
+                    if vel[i, i, e0, K0] >= 0:
+                        A[r, iv(i, K0, e0)] = 1.0
+                    else:
+                        A[r, iF(i, E)] = 1.0
+                        b[r] = Z_old[i, K0]
+            r += 1
+    assert r == N
+    x = np.linalg.solve(A, b)
+    return x[nv + n * C:].reshape(n, F), x[nv:nv + n * C].reshape(n, C)
+
+
+def _tiny_problem(kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "tri_n1":
+        pr = problems.heat_2d(s=3, steps=2)
+        pts = pr.points.copy()
+        inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
+        pts[inner] += 0.05 * rng.standard_normal((inner.sum(), 2))  # unstructured interior
+        pr = dataclasses.replace(pr, points=pts)
+    elif kind == "tet_n1":
+        pr = problems.darcy_3d(s=2, steps=2, preconditioner="diagonal")
+    else:
+        pr = problems.adr_3d(s=2, steps=2, preconditioner="diagonal")
+    n, C, F, fpc = pr.n, pr.num_cells, pr.num_faces, pr.fpc
+    co = dict(pr.coeffs)
+    co["f_iK"] = rng.uniform(-1, 1, n * C)
+    co["r_ijK"] = (np.eye(n)[:, :, None] * rng.uniform(0, 0.5, (1, 1, C))).ravel()
+    co["N_ijK"] = (np.eye(n)[:, :, None] * rng.uniform(0.5, 2, (1, 1, C))).ravel()
+    if n == 2:
+        K = rng.uniform(0.5, 2, C)
+        co["D_ijK"] = np.stack([K, 0.1 * K, 0.1 * K, K]).ravel()
+    else:
+        co["D_ijK"] = rng.uniform(0.5, 2.0, C)
+    co["w_iKe"] = 0.1 * rng.standard_normal(n * fpc * C)
+    tags = np.array(pr.tags, copy=True)
+    vals = np.zeros(n * F)
+    dirv = np.zeros(n * F)
+    bnd = np.nonzero(pr.face_cells[:, 1] < 0)[0]
+    for i in range(n):
+        for q, E in enumerate(bnd):
+            tags[i * F + E] = [problems.FIXED_VALUE, problems.FIXED_FLUX_NO_ADVECTION,
+                               problems.FIXED_FLUX, problems.ADVECTIVE_OUTFLOW][(q + i) % 4]
+            vals[i * F + E] = rng.uniform(-1, 1)
+            dirv[i * F + E] = rng.uniform(-1, 1)
+    z0 = rng.uniform(0, 1, n * C)
+    return dataclasses.replace(pr, coeffs=co, tags=tags, bc_values=vals, bc_dirichlet=dirv, Z_iK0=z0,
+                               do_mobility_upwind=0, bc_mobility=np.ones(n * F))
+
+
+@pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2"])
+def test_brute_force_mixed_system(kind):
+    pr = _tiny_problem(kind, 3)
+    o = pyoracle.Oracle(pr)
+    n, C, F = pr.n, pr.num_cells, pr.num_faces
+    t = 0.0
+    for step in range(2):
+        Z_old = o.array("Z_iK").reshape(n, C).copy()
+        ZF_old = o.array("Z_iF").reshape(F, n).T.copy()
+        m_upw = o.array("m_iE_upw").reshape(F, n).copy()
+        o.step(t, pr.dt, tol=1e-14)
+        ZF, ZK = _mixed_reference(pr, None, pr.dt, Z_old, ZF_old, m_upw, pr.bc_dirichlet)
+        got_F = o.array("Z_iF").reshape(F, n).T
+        got_K = o.array("Z_iK").reshape(n, C)
+        assert np.abs(got_F - ZF).max() / np.abs(ZF).max() < 1e-9, step
+        assert np.abs(got_K - ZK).max() / np.abs(ZK).max() < 1e-9, step
+        t += pr.dt
+
+
+# ---------------------------------------------------------------- invariants
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_linear_patch_test(mesh):
+    """Steady diffusion (N=0) with Dirichlet data from a linear function: the
+    face traces equal the function at face centroids and the cell values at
+    cell centroids (RTN0 MHFEM is exact on linear solutions)."""
+    pr = problems.heat_2d(s=6) if mesh == "tri" else problems.darcy_3d(s=3, preconditioner="diagonal")
+    F = pr.num_faces
+    g = lambda x: 0.3 + x @ np.arange(1, x.shape[1] + 1) * 0.7
+    fc = meshgen.face_centroids(pr.points, pr.cells, pr.cell_faces, F)
+    tags = np.full(F, problems.FIXED_VALUE, np.uint8)
+    tags[pr.face_cells[:, 1] >= 0] = problems.FIXED_VALUE
+    co = dict(pr.coeffs)
+    co["N_ijK"] = np.array(0.0)
+    co["D_ijK"] = np.array(2.5)
+    pr = dataclasses.replace(pr, coeffs=co, tags=tags, bc_values=np.zeros(F), bc_dirichlet=g(fc),
+                             Z_iK0=np.zeros(pr.num_cells), do_mobility_upwind=0)
+    o = pyoracle.Oracle(pr)
+    o.step(0.0, 1.0, tol=1e-14)
+    assert np.abs(o.array("Z_iF") - g(fc)).max() < 1e-10
+    cc = meshgen.cell_centroids(pr.points, pr.cells)
+    assert np.abs(o.array("Z_iK") - g(cc)).max() < 1e-10
+
+
+def test_conservation_zero_flux():
+    pr = problems.darcy_3d(s=3, preconditioner="diagonal", steps=4)
+    F = pr.num_faces
+    rng = np.random.default_rng(0)
+    pr = dataclasses.replace(pr, tags=np.full(F, problems.FIXED_FLUX_NO_ADVECTION, np.uint8),
+                             Z_iK0=rng.uniform(0, 1, pr.num_cells))
+    o = pyoracle.Oracle(pr)
+    vol = o.array("cell_measure").copy()
+    mass0 = (vol * o.array("Z_iK")).sum()
+    t = 0.0
+    for _ in range(pr.steps):
+        o.step(t, pr.dt, tol=1e-14)
+        t += pr.dt
+        assert abs((vol * o.array("Z_iK")).sum() - mass0) < 1e-10 * abs(mass0)
+
+
+def test_csr_pattern_invariants():
+    for pr in (problems.adr_3d(s=2), problems.heat_2d(s=4)):
+        o = pyoracle.Oracle(pr)
+        rp, col, val, rhs = o.csr()
+        n, F, fpc = pr.n, pr.num_faces, pr.fpc
+        cf, fc = pr.cell_faces, pr.face_cells
+        for E in range(F):
+            for i in range(n):
+                r = E * n + i
+                cols = col[rp[r]:rp[r + 1]]
+                assert np.all(np.diff(cols) > 0)
+                K0, K1 = fc[E]
+                if K1 >= 0:
+                    faces = set(cf[K0]) | set(cf[K1])
+                    assert len(cols) == (2 * fpc - 1) * n
+                elif pr.tags[i * F + E] == problems.FIXED_VALUE:
+                    assert list(cols) == [r]
+                    continue
+                else:
+                    faces = set(cf[K0])
+                    assert len(cols) == fpc * n
+                assert set(cols) == {Fc * n + j for Fc in faces for j in range(n)}
+
+
+# ---------------------------------------------------------------- linear algebra
+def _mhfem_system(prec="diagonal"):
+    pr = problems.darcy_3d(s=4, preconditioner=prec)
+    o = pyoracle.Oracle(pr)
+    o.pre_iterate(0.0, pr.dt)
+    o.assemble(0.0, pr.dt)
+    rp, col, val, rhs = o.csr()
+    return np.array(rp), np.array(col), np.array(val), np.array(rhs), o
+
+
+def test_ilu0_reproduces_pattern():
+    rp, col, val, rhs, _ = _mhfem_system()
+    lu = pyoracle.ilu0(rp, col, val)
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    Lm = sp.csr_matrix((np.where(col < rows, lu, 0.0), col, rp), shape=(n, n)) + sp.identity(n)
+    Um = sp.csr_matrix((np.where(col >= rows, lu, 0.0), col, rp), shape=(n, n))
+    P = (Lm @ Um).tocsr()
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    on = np.asarray(P[rows, col]).ravel()
+    assert np.abs(on - val).max() < 1e-12 * np.abs(val).max()
+    # and the apply solves L U y = x
+    x = np.random.default_rng(1).standard_normal(n)
+    y = pyoracle.ilu0_apply(rp, col, lu, x)
+    assert np.linalg.norm(P @ y - x) < 1e-11 * np.linalg.norm(x)
+    del A
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "gmres"])
+@pytest.mark.parametrize("prec", ["none", "diagonal", "ilu0"])
+def test_krylov_against_direct_solve(solver, prec):
+    rp, col, val, rhs, o = _mhfem_system(prec)
+    n = rp.size - 1
+    A = sp.csr_matrix((val, col, rp), shape=(n, n)).tocsc()
+    xd = spla.spsolve(A, rhs)
+    x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), solver, prec, 1e-13)
+    assert conv == 1 and res < 1e-13
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+
+
+def test_rtn0_projection_of_constant_field():
+    """a_iiKe = |E| a.n_{K,e}: fluxes of a constant field sum to zero over each
+    cell (divergence-free) and are opposite on the two sides of a face."""
+    pr = problems.adr_3d(s=3)
+    C, F = pr.num_cells, pr.num_faces
+    a = np.asarray(pr.coeffs["a_ijKe"]).reshape(2, 2, 4, C)[0, 0]
+    assert np.abs(a.sum(axis=0)).max() < 1e-14
+    cf, fc = pr.cell_faces, pr.face_cells
+    inner = np.nonzero(fc[:, 1] >= 0)[0]
+    for E in inner[:200]:
+        K0, K1 = fc[E]
+        e0 = list(cf[K0]).index(E)
+        e1 = list(cf[K1]).index(E)
+        assert abs(a[e0, K0] + a[e1, K1]) < 1e-14
