@@ -166,6 +166,42 @@ int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *v
 /* solve an explicit right-hand side with the current matrix (testing aid) */
 int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats);
 
+/* ---- learned initial guess of each solve.  No counterpart in the reference:
+ * the reference starts every Krylov solve from the previous step's Z_iF
+ * (Solver.hpp:1247-1255); this replaces only that starting vector.  A per-DOF
+ * MLP (16 features -> 64 -> 64 -> 1, BF16 MFMA, FP32 accumulation) predicts
+ * the increment of Z_iF from 2-hop features of the warm-start residual and of
+ * the previous increments; the prediction is used only when it lowers the
+ * initial residual, and the Krylov solve still decides convergence. */
+typedef struct {
+    int enabled;          /* 0: previous Z_iF (reference behaviour), 1: learned guess */
+    int train_steps;      /* train online on the first train_steps solves (0: frozen) */
+    int epochs;           /* full-batch Adam steps per training solve */
+    float learning_rate;  /* Adam step size */
+    unsigned seed;        /* weight initialisation */
+} mhfem_guess_config;
+
+typedef struct {
+    int used;             /* the prediction was accepted as the starting vector */
+    double r0_norm;       /* ||b - A Z_iF(previous)|| */
+    double r_guess_norm;  /* ||b - A x_guess|| (== r0_norm when not evaluated) */
+    double loss;          /* mean squared training loss of the last epoch (normalised units) */
+    int trained_steps;    /* solves used for training so far */
+} mhfem_guess_stats;
+
+int mhfem_guess_configure(mhfem_ctx *ctx, const mhfem_guess_config *cfg);
+int mhfem_guess_stats_last(const mhfem_ctx *ctx, mhfem_guess_stats *out);
+int mhfem_guess_num_params(void);
+/* FP32 master weights (W1[16][64], b1[64], W2[64][64], b2[64], W3[64], b3) */
+int mhfem_guess_get_weights(mhfem_ctx *ctx, float *params, int count);
+int mhfem_guess_set_weights(mhfem_ctx *ctx, const float *params, int count);
+/* the network on explicit inputs (features as BF16 bit patterns [rows][16]):
+ * forward y[rows], and the gradient of mean((y - t)^2) (count+1 floats: the
+ * last one is the summed squared error) */
+int mhfem_guess_forward(mhfem_ctx *ctx, const uint16_t *features, long rows, float *y);
+int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *targets, long rows,
+                         float *grad);
+
 /* ---- instrumentation (Solver::writeEpilog, Solver.hpp:1326-1391) */
 /* average duration (ms) of the solver's SpMV (y = A x, current matrix) over
  * `reps` launches on the context's stream; precision selects the FP64 or the

@@ -1,0 +1,82 @@
+"""The learned initial guess on the GPU: forward pass and gradient of the
+hand-written BF16 MFMA kernels against the CPU oracle of the network, and the
+gain of the trained network over the warm start (previous Z_iF)."""
+import numpy as np
+import pytest
+
+from mhfem_amd import capi, problems
+import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver():
+    pr = problems.darcy_3d(s=4, steps=1, preconditioner="diagonal")
+    return capi.Solver(pr)
+
+
+@pytest.mark.parametrize("rows", [1, 16, 17, 1000, 40000])
+def test_forward_matches_oracle(rows):
+    print(f"\n[guess forward] rows={rows}", flush=True)
+    s = _solver()
+    rng = np.random.default_rng(rows)
+    P = (rng.standard_normal(M.NPARAM) * 0.2).astype(np.float32)
+    s.set_guess_weights(P)
+    X = M.bf16_bits(rng.standard_normal((rows, M.NF)).astype(np.float32))
+    y = s.guess_forward(X)
+    yo, _ = M.forward(P.astype(np.float64), X)
+    # identical BF16 rounding points; only the FP32 summation order differs,
+    # which can move a hidden activation across a BF16 rounding boundary
+    # (2^-8 relative): bound the error at the BF16 level
+    scale = max(1.0, np.abs(yo).max())
+    assert np.abs(y - yo).max() <= 2e-2 * scale
+    assert np.linalg.norm(y - yo) <= 2e-3 * max(np.linalg.norm(yo), 1.0)
+
+
+@pytest.mark.parametrize("rows", [16, 1000, 50000])
+def test_gradient_matches_oracle(rows):
+    print(f"\n[guess gradient] rows={rows}", flush=True)
+    s = _solver()
+    rng = np.random.default_rng(7 + rows)
+    P = (rng.standard_normal(M.NPARAM) * 0.2).astype(np.float32)
+    s.set_guess_weights(P)
+    X = M.bf16_bits(rng.standard_normal((rows, M.NF)).astype(np.float32))
+    t = rng.standard_normal(rows).astype(np.float32)
+    g, sse = s.guess_gradient(X, t)
+    go, sseo = M.gradient(P.astype(np.float64), X, t.astype(np.float64))
+    assert abs(sse - sseo) <= 1e-4 * sseo
+    for name, a, b in [("W1", M.P_W1, M.P_B1), ("b1", M.P_B1, M.P_W2), ("W2", M.P_W2, M.P_B2),
+                       ("b2", M.P_B2, M.P_W3), ("W3", M.P_W3, M.P_B3), ("b3", M.P_B3, M.NPARAM)]:
+        err = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-30)
+        assert err < 1e-2, (name, err)
+
+
+def _run(pr, steps, guess=None, weights=None, train_steps=0, prec="diagonal"):
+    s = capi.Solver(pr, preconditioner=prec, tol=1e-10)
+    if guess:
+        s.configure_guess(True, train_steps=train_steps, epochs=30, learning_rate=2e-3, seed=3)
+        if weights is not None:
+            s.set_guess_weights(weights)
+    its, used = [], []
+    t = 0.0
+    for k in range(steps):
+        st = s.step(t, pr.dt)
+        t += pr.dt
+        its.append(st.iterations)
+        if guess:
+            used.append(s.guess_stats()["used"])
+    return s, its, used
+
+
+def test_trained_guess_cuts_iterations():
+    """Train online on one K field, evaluate frozen weights on a held-out K
+    field: fewer Krylov iterations than the warm start of the reference."""
+    train = problems.darcy_3d(s=12, seed=11, steps=30, preconditioner="diagonal")
+    s, its_train, _ = _run(train, 30, guess=True, train_steps=30)
+    w = s.guess_weights()
+    test = problems.darcy_3d(s=12, seed=12, steps=20, preconditioner="diagonal")
+    _, base, _ = _run(test, 20)
+    _, learned, used = _run(test, 20, guess=True, weights=w, train_steps=0)
+    print(f"\nwarm start {sum(base[2:])}, learned {sum(learned[2:])}, used {sum(used)}", flush=True)
+    assert sum(used) > 0
+    assert sum(learned[2:]) < sum(base[2:])

@@ -370,6 +370,7 @@ void mhfem_destroy(mhfem_ctx *ctx)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     krylov_destroy(c->krylov);
+    guess_destroy(c->guess);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -599,6 +600,9 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         MHFEM_HIP(hipMemset(c.arr[MHFEM_RHS].p, 0, c.rows * sizeof(double)));
         krylov_destroy(c.krylov);
         c.krylov = nullptr;
+        guess_destroy(c.guess);
+        c.guess = nullptr;
+        c.guess_on_solve = false;
         c.system_ready = true;
         c.krylov = krylov_create(c);
         MHFEM_HIP(hipDeviceSynchronize());
@@ -718,6 +722,70 @@ int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_st
         MHFEM_HIP(hipMemcpyAsync(x, xx.p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
         if (stats) *stats = st;
+    });
+}
+
+int mhfem_guess_configure(mhfem_ctx *ctx, const mhfem_guess_config *cfg)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!cfg) throw Error{MHFEM_E_ARGUMENT, "null guess config"};
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "mhfem_setup_linear_system must be called first"};
+        if (!c.guess) c.guess = guess_create(c);
+        guess_configure(c, *c.guess, *cfg);
+        c.guess_on_solve = cfg->enabled != 0;
+    });
+}
+
+int mhfem_guess_stats_last(const mhfem_ctx *ctx, mhfem_guess_stats *out)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || !out) return MHFEM_E_ARGUMENT;
+    if (!c->guess) return MHFEM_E_STATE;
+    *out = guess_last_stats(*c->guess);
+    return MHFEM_OK;
+}
+
+int mhfem_guess_num_params(void) { return guess_num_params(); }
+
+static void need_guess(Ctx &c)
+{
+    if (!c.system_ready) throw Error{MHFEM_E_STATE, "mhfem_setup_linear_system must be called first"};
+    if (!c.guess) c.guess = guess_create(c);
+}
+
+int mhfem_guess_get_weights(mhfem_ctx *ctx, float *params, int count)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        need_guess(c);
+        if (!params || count != guess_num_params()) throw Error{MHFEM_E_ARGUMENT, "weight count mismatch"};
+        guess_get_weights(c, *c.guess, params);
+    });
+}
+
+int mhfem_guess_set_weights(mhfem_ctx *ctx, const float *params, int count)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        need_guess(c);
+        if (!params || count != guess_num_params()) throw Error{MHFEM_E_ARGUMENT, "weight count mismatch"};
+        guess_set_weights(c, *c.guess, params);
+    });
+}
+
+int mhfem_guess_forward(mhfem_ctx *ctx, const uint16_t *features, long rows, float *y)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        need_guess(c);
+        if (!features || !y || rows <= 0) throw Error{MHFEM_E_ARGUMENT, "bad arguments"};
+        guess_forward(c, *c.guess, features, rows, y);
+    });
+}
+
+int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *targets, long rows, float *grad)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        need_guess(c);
+        if (!features || !targets || !grad || rows <= 0) throw Error{MHFEM_E_ARGUMENT, "bad arguments"};
+        guess_gradient(c, *c.guess, features, targets, rows, grad);
     });
 }
 

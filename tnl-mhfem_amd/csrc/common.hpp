@@ -103,6 +103,7 @@ struct DBuf {
 
 struct Krylov;  // krylov.hip
 struct Ilu0;    // krylov.hip
+struct Guess;   // guess.hip
 
 struct Ctx {
     int dim = 0, fpc = 0, S = 0, n = 0;
@@ -133,8 +134,11 @@ struct Ctx {
     DBuf<double> sell_val;
     long sell_size = 0;
     bool system_ready = false;
+    bool unit_diagonal = true;  // the assembler writes 1 on every diagonal
     mhfem_solver_config cfg{};
     Krylov *krylov = nullptr;
+    Guess *guess = nullptr;
+    bool guess_on_solve = false;  // the learned guess runs in mhfem_solve
     // timers
     bool timing = false;
     hipEvent_t ev[16] = {};
@@ -158,5 +162,21 @@ void krylov_destroy(Krylov *k);
 // solve A x = b (b, x device arrays of length rows); returns stats
 void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st);
 void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms);
+void spmv64(Ctx &c, const double *x, double *y);
+void residual64(Ctx &c, const double *x, const double *b, double *r);
+double dot64(Ctx &c, const double *x, const double *y);
+
+// guess.hip
+Guess *guess_create(Ctx &c);
+void guess_destroy(Guess *g);
+void guess_configure(Ctx &c, Guess &g, const mhfem_guess_config &cfg);
+void guess_before_solve(Ctx &c, Guess &g, const double *b, double *x);
+void guess_after_solve(Ctx &c, Guess &g, const double *x);
+void guess_forward(Ctx &c, Guess &g, const uint16_t *X, long rows, float *y);
+void guess_gradient(Ctx &c, Guess &g, const uint16_t *X, const float *t, long rows, float *grad);
+void guess_get_weights(Ctx &c, Guess &g, float *out);
+void guess_set_weights(Ctx &c, Guess &g, const float *in);
+int guess_num_params();
+mhfem_guess_stats guess_last_stats(const Guess &g);
 
 }  // namespace mhfem

@@ -1,0 +1,670 @@
+// guess.hip -- learned initial guess for the Krylov solve (no counterpart in
+// the reference; it replaces only the starting vector, which the reference
+// takes as the previous step's Z_iF, Solver.hpp:1247-1255).
+//
+// A per-DOF multilayer perceptron 16 -> 64 -> 64 -> 1 (ReLU) predicts the
+// increment of the face traces over one time step from 2-hop features of the
+// warm-start residual r0 = b - A x_prev and of the previous increments:
+//   f0..2 = r0, A r0, A^2 r0 (scaled by s_r = RMS(r0))
+//   f3..5 = dx1, dx2, A dx1   (scaled by s_x = RMS(dx1))
+//   f6 = Dirichlet row, f7 = component index, f8 = log10(s_r / s_x) / 4
+// The prediction is x0 = x_prev + s_x * y.  It is used only if it lowers the
+// initial residual; the Krylov solve then judges convergence as before.
+//
+// Arithmetic: BF16 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation.  Every
+// layer is evaluated transposed, H^T = W^T X^T, so the D operand of one MFMA
+// (lane l holds rows 4(l/16)+t, column l%16) is already the B operand of the
+// next (lane l holds k = 4(l/16)+t, n = l%16): a wavefront carries a tile of
+// 16 samples through the network in registers.  The weight gradients
+// (reductions over samples) transpose their operands through LDS.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace mhfem {
+
+constexpr int GB = 256;  // work-group: 4 wavefronts
+constexpr int NF = 16, NH = 64;
+// parameter vector layout (FP32 master copy and gradients)
+constexpr int P_W1 = 0;                 // [NF][NH]
+constexpr int P_B1 = P_W1 + NF * NH;    // [NH]
+constexpr int P_W2 = P_B1 + NH;         // [NH][NH]
+constexpr int P_B2 = P_W2 + NH * NH;    // [NH]
+constexpr int P_W3 = P_B2 + NH;         // [NH]
+constexpr int P_B3 = P_W3 + NH;         // [1]
+constexpr int NPARAM = P_B3 + 1;        // 5313
+constexpr int NPARAM_PAD = 5376;
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// float -> bf16, round to nearest even (NaN kept quiet)
+__host__ __device__ inline unsigned short f2bf(float f)
+{
+    unsigned u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf2f(unsigned short h)
+{
+    unsigned u = (unsigned)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ __forceinline__ float4_t mfma(short4_t a, short4_t b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// BF16 copies of the weights in the operand layouts the kernels load:
+//   A1[mb]      lane l: W1[4q+t][16mb+i]          (i = l%16, q = l/16)
+//   A2[mb][kb]  lane l: W2[16kb+4q+t][16mb+i]
+//   A3[kb]      lane l: W3[16kb+4q+t] if i == 0 else 0
+//   A3b[mb]     lane l: W3[16mb+i]    if q == 0 && t == 0 else 0   (backward)
+//   A2b[mb][kb] lane l: W2[16mb+i][16kb+4q+t]                       (backward)
+struct PackedWeights {
+    short4_t A1[4][64];
+    short4_t A2[16][64];
+    short4_t A3[4][64];
+    short4_t A3b[4][64];
+    short4_t A2b[16][64];
+    float b1[NH], b2[NH], b3;
+};
+
+__global__ void __launch_bounds__(64) k_pack_weights(const float *__restrict__ P, PackedWeights *__restrict__ W)
+{
+    const int l = threadIdx.x, i = l & 15, q = l >> 4;
+    for (int mb = 0; mb < 4; mb++) {
+        short4_t a;
+        for (int t = 0; t < 4; t++) a[t] = (short)f2bf(P[P_W1 + (4 * q + t) * NH + 16 * mb + i]);
+        W->A1[mb][l] = a;
+        short4_t a3, a3b;
+        for (int t = 0; t < 4; t++) {
+            a3[t] = (short)(i == 0 ? f2bf(P[P_W3 + 16 * mb + 4 * q + t]) : 0);
+            a3b[t] = (short)((q == 0 && t == 0) ? f2bf(P[P_W3 + 16 * mb + i]) : 0);
+        }
+        W->A3[mb][l] = a3;
+        W->A3b[mb][l] = a3b;
+        for (int kb = 0; kb < 4; kb++) {
+            short4_t a2, a2b;
+            for (int t = 0; t < 4; t++) {
+                a2[t] = (short)f2bf(P[P_W2 + (16 * kb + 4 * q + t) * NH + 16 * mb + i]);
+                a2b[t] = (short)f2bf(P[P_W2 + (16 * mb + i) * NH + 16 * kb + 4 * q + t]);
+            }
+            W->A2[mb * 4 + kb][l] = a2;
+            W->A2b[mb * 4 + kb][l] = a2b;
+        }
+    }
+    W->b1[l] = P[P_B1 + l];
+    W->b2[l] = P[P_B2 + l];
+    if (l == 0) W->b3 = P[P_B3];
+}
+
+// forward pass of one 16-sample tile; the activations stay in registers
+struct Tile {
+    short4_t X;       // B operand of layer 1
+    short4_t H1[4];   // relu(z1) as bf16, B operand of layer 2
+    short4_t H2[4];   // relu(z2) as bf16, B operand of layer 3
+    float out;        // lanes 0..15: prediction of sample l
+};
+
+__device__ __forceinline__ short4_t relu_bias_bf16(float4_t z, const float *b, int base)
+{
+    short4_t h;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const float v = z[t] + b[base + t];
+        h[t] = (short)f2bf(v > 0.f ? v : 0.f);
+    }
+    return h;
+}
+
+__device__ __forceinline__ void tile_forward(const PackedWeights &W, const short4_t (&A1)[4],
+                                             const short4_t (&A2)[16], const short4_t (&A3)[4], Tile &T)
+{
+    const int l = threadIdx.x & 63, q = l >> 4;
+    const float4_t zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) T.H1[mb] = relu_bias_bf16(mfma(A1[mb], T.X, zero), W.b1, 16 * mb + 4 * q);
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+        float4_t acc = zero;
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) acc = mfma(A2[mb * 4 + kb], T.H1[kb], acc);
+        T.H2[mb] = relu_bias_bf16(acc, W.b2, 16 * mb + 4 * q);
+    }
+    float4_t o = zero;
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) o = mfma(A3[kb], T.H2[kb], o);
+    T.out = o[0] + W.b3;  // row 0 of the output block: valid in lanes 0..15
+}
+
+__device__ __forceinline__ short4_t load_features(const unsigned short *__restrict__ X, long rows, long s0)
+{
+    const int l = threadIdx.x & 63, i = l & 15, q = l >> 4;
+    const long s = s0 + i;
+    short4_t x = {0, 0, 0, 0};
+    if (s < rows) {
+        const short4_t *p = reinterpret_cast<const short4_t *>(X + s * NF + 4 * q);
+        x = *p;
+    }
+    return x;
+}
+
+// y[s] = MLP(X[s]) for all samples
+__global__ void __launch_bounds__(GB) k_mlp_forward(long rows, const unsigned short *__restrict__ X,
+                                                    const PackedWeights *__restrict__ Wg, float *__restrict__ y)
+{
+    const int l = threadIdx.x & 63;
+    const PackedWeights &W = *Wg;
+    short4_t A1[4], A2[16], A3[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        A1[k] = W.A1[k][l];
+        A3[k] = W.A3[k][l];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) A2[k] = W.A2[k][l];
+    const long ntiles = (rows + 15) / 16;
+    const long wave = (long)blockIdx.x * (GB / 64) + (threadIdx.x >> 6);
+    const long nwaves = (long)gridDim.x * (GB / 64);
+    for (long tile = wave; tile < ntiles; tile += nwaves) {
+        Tile T;
+        T.X = load_features(X, rows, tile * 16);
+        tile_forward(W, A1, A2, A3, T);
+        if (l < 16 && tile * 16 + l < rows) y[tile * 16 + l] = T.out;
+    }
+}
+
+// Gradient of L = (1/rows) sum_s (y_s - t_s)^2 with respect to all
+// parameters; one partial vector per work-group, plus the partial loss.
+__global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short *__restrict__ X,
+                                                 const float *__restrict__ target,
+                                                 const PackedWeights *__restrict__ Wg, float inv_rows,
+                                                 float *__restrict__ partial)
+{
+    const int l = threadIdx.x & 63, i = l & 15, q = l >> 4, wid = threadIdx.x >> 6;
+    const PackedWeights &W = *Wg;
+    __shared__ float tr[GB / 64][2][16][17];  // per-wave transpose buffers (padded)
+    short4_t A1[4], A2[16], A3[4], A3b[4], A2b[16];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        A1[k] = W.A1[k][l];
+        A3[k] = W.A3[k][l];
+        A3b[k] = W.A3b[k][l];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        A2[k] = W.A2[k][l];
+        A2b[k] = W.A2b[k][l];
+    }
+    const float4_t zero = {0.f, 0.f, 0.f, 0.f};
+    float4_t gW2[16], gW1[4];
+#pragma unroll
+    for (int k = 0; k < 16; k++) gW2[k] = zero;
+#pragma unroll
+    for (int k = 0; k < 4; k++) gW1[k] = zero;
+    float gW3[4][4] = {}, gb2[4][4] = {}, gb1[4][4] = {};
+    float gb3 = 0.f, loss = 0.f;
+
+    // every wavefront runs the same number of tiles (tiles past the end are
+    // all-zero and contribute nothing), so the work-group barriers of the
+    // LDS transposes below are reached uniformly
+    const long ntiles = (rows + 15) / 16;
+    const long wave = (long)blockIdx.x * (GB / 64) + wid;
+    const long nwaves = (long)gridDim.x * (GB / 64);
+    const long iters = (ntiles + nwaves - 1) / nwaves;
+    for (long it = 0; it < iters; it++) {
+        const long tile = wave + it * nwaves;
+        const long s0 = tile * 16;
+        Tile T;
+        T.X = load_features(X, rows, s0);
+        tile_forward(W, A1, A2, A3, T);
+        // dL/dy for the sample of lanes 0..15, broadcast to lanes with the same i
+        float g = 0.f;
+        if (l < 16 && s0 + l < rows) {
+            const float d = T.out - target[s0 + l];
+            loss += d * d;
+            g = 2.f * d * inv_rows;
+        }
+        gb3 += g;
+        const float gi = __shfl(g, i, 64);
+        short4_t Bg = {0, 0, 0, 0};
+        if (q == 0) Bg[0] = (short)f2bf(g);
+        // layer 3 backward: dW3 (matrix-vector, per lane), g_z2
+        short4_t gZ2[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const float4_t gh = mfma(A3b[mb], Bg, zero);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const float h2 = bf2f((unsigned short)T.H2[mb][t]);
+                gW3[mb][t] += h2 * gi;
+                const float gz = h2 > 0.f ? gh[t] : 0.f;
+                gb2[mb][t] += gz;
+                gZ2[mb][t] = (short)f2bf(gz);
+            }
+        }
+        // layer 2 backward: g_z1
+        short4_t gZ1[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            float4_t acc = zero;
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++) acc = mfma(A2b[mb * 4 + kb], gZ2[kb], acc);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const float h1 = bf2f((unsigned short)T.H1[mb][t]);
+                const float gz = h1 > 0.f ? acc[t] : 0.f;
+                gb1[mb][t] += gz;
+                gZ1[mb][t] = (short)f2bf(gz);
+            }
+        }
+        // dW2[k][m] = sum_s H1[s][k] gZ2[s][m]: operands transposed through LDS
+        //   A (16x16 block kb): lane: H1^T[16kb + i][sample 4q+t]
+        //   B (16x16 block mb): lane: gZ2^T[16mb + i][sample 4q+t]
+        short4_t AH1[4], BG2[4], BG1[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                tr[wid][0][4 * q + t][i] = bf2f((unsigned short)T.H1[b][t]);
+                tr[wid][1][4 * q + t][i] = bf2f((unsigned short)gZ2[b][t]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                AH1[b][t] = (short)f2bf(tr[wid][0][i][4 * q + t]);
+                BG2[b][t] = (short)f2bf(tr[wid][1][i][4 * q + t]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; t++) tr[wid][1][4 * q + t][i] = bf2f((unsigned short)gZ1[b][t]);
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; t++) BG1[b][t] = (short)f2bf(tr[wid][1][i][4 * q + t]);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) gW2[kb * 4 + mb] = mfma(AH1[kb], BG2[mb], gW2[kb * 4 + mb]);
+        // dW1[f][k] = sum_s X[s][f] gZ1[s][k]; A lane: X[sample 4q+t][feature i]
+        short4_t AX;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const long s = s0 + 4 * q + t;
+            AX[t] = (short)(s < rows ? X[s * NF + i] : 0);
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) gW1[mb] = mfma(AX, BG1[mb], gW1[mb]);
+    }
+
+    // ---- reduce the wave accumulators into one partial vector per work-group
+    // Layouts of the accumulators (lane l = 16q + i):
+    //   gW2[kb*4+mb][t]: dW2[16kb + 4q + t][16mb + i]
+    //   gW1[mb][t]:      dW1[4q + t][16mb + i]
+    //   gW3[mb][t]:      dW3[16mb + 4q + t] (to be summed over i)
+    //   gb2/gb1[mb][t]:  db[16mb + 4q + t]    (to be summed over i)
+    float *out = partial + (long)blockIdx.x * NPARAM_PAD;
+    // sum over the 16 lanes that share q (the samples of a tile)
+    auto sum16 = [](float v) {
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+        return v;
+    };
+    // direct form: every lane writes its values into a per-wave LDS image of
+    // the parameter vector, then the 4 waves are summed
+    __shared__ float img[NPARAM_PAD];
+    for (int p = threadIdx.x; p < NPARAM_PAD; p += GB) img[p] = 0.f;
+    __syncthreads();
+    for (int w = 0; w < GB / 64; w++) {
+        if (wid == w) {
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+                for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) img[P_W2 + (16 * kb + 4 * q + t) * NH + 16 * mb + i] += gW2[kb * 4 + mb][t];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) img[P_W1 + (4 * q + t) * NH + 16 * mb + i] += gW1[mb][t];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const float w3 = sum16(gW3[mb][t]);
+                    const float b2 = sum16(gb2[mb][t]);
+                    const float b1 = sum16(gb1[mb][t]);
+                    if (i == 0) {
+                        img[P_W3 + 16 * mb + 4 * q + t] += w3;
+                        img[P_B2 + 16 * mb + 4 * q + t] += b2;
+                        img[P_B1 + 16 * mb + 4 * q + t] += b1;
+                    }
+                }
+            float b3 = gb3, ls = loss;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                b3 += __shfl_xor(b3, off, 64);
+                ls += __shfl_xor(ls, off, 64);
+            }
+            if (l == 0) {
+                img[P_B3] += b3;
+                img[NPARAM] += ls;  // loss slot (inside the padding)
+            }
+        }
+        __syncthreads();
+    }
+    for (int p = threadIdx.x; p < NPARAM_PAD; p += GB) out[p] = img[p];
+}
+
+// sum the work-group partials (deterministic order) and take one Adam step
+__global__ void __launch_bounds__(GB) k_adam(int nblocks, const float *__restrict__ partial, float *__restrict__ P,
+                                             float *__restrict__ m, float *__restrict__ v, float *__restrict__ grad_out,
+                                             float lr, float b1, float b2, float eps, int step)
+{
+    const int p = blockIdx.x * GB + threadIdx.x;
+    if (p >= NPARAM_PAD) return;
+    float g = 0.f;
+    for (int b = 0; b < nblocks; b++) g += partial[(long)b * NPARAM_PAD + p];
+    if (grad_out) grad_out[p] = g;
+    if (p >= NPARAM || lr <= 0.f) return;
+    const float mm = b1 * m[p] + (1.f - b1) * g;
+    const float vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    const float mh = mm / (1.f - powf(b1, (float)step));
+    const float vh = vv / (1.f - powf(b2, (float)step));
+    P[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// features of every row, packed as BF16 [rows][16]
+__global__ void __launch_bounds__(GB) k_features(int rows, int n, const double *__restrict__ r0,
+                                                 const double *__restrict__ Ar0, const double *__restrict__ A2r0,
+                                                 const double *__restrict__ dx1, const double *__restrict__ dx2,
+                                                 const double *__restrict__ Adx1, const uint8_t *__restrict__ dirichlet,
+                                                 const double *__restrict__ scal, unsigned short *__restrict__ X)
+{
+    const int r = blockIdx.x * GB + threadIdx.x;
+    if (r >= rows) return;
+    const double isr = scal[0] > 0 ? 1.0 / scal[0] : 0.0;
+    const double isx = scal[1] > 0 ? 1.0 / scal[1] : 0.0;
+    float f[NF] = {};
+    f[0] = (float)(r0[r] * isr);
+    f[1] = (float)(Ar0[r] * isr);
+    f[2] = (float)(A2r0[r] * isr);
+    f[3] = (float)(dx1[r] * isx);
+    f[4] = (float)(dx2[r] * isx);
+    f[5] = (float)(Adx1[r] * isx);
+    f[6] = dirichlet[r] ? 1.f : 0.f;
+    f[7] = n > 1 ? (float)(r % n) / (float)(n - 1) : 0.f;
+    f[8] = (scal[0] > 0 && scal[1] > 0) ? (float)(log10(scal[0] / scal[1]) * 0.25) : 0.f;
+    unsigned short *o = X + (long)r * NF;
+#pragma unroll
+    for (int k = 0; k < NF; k++) o[k] = f2bf(f[k]);
+}
+
+// x = x_prev + s_x * y ; target = (x_new - x_prev) / s_x ; dx history
+__global__ void __launch_bounds__(GB) k_apply_guess(int rows, const double *__restrict__ xp, const float *__restrict__ y,
+                                                    const double *__restrict__ scal, double *__restrict__ x)
+{
+    const int r = blockIdx.x * GB + threadIdx.x;
+    if (r < rows) x[r] = xp[r] + scal[1] * (double)y[r];
+}
+
+__global__ void __launch_bounds__(GB) k_after_solve(int rows, const double *__restrict__ x, const double *__restrict__ xp,
+                                                    double *__restrict__ dx1, double *__restrict__ dx2,
+                                                    const double *__restrict__ scal, float *__restrict__ target)
+{
+    const int r = blockIdx.x * GB + threadIdx.x;
+    if (r >= rows) return;
+    const double d = x[r] - xp[r];
+    if (target) target[r] = scal[1] > 0 ? (float)(d / scal[1]) : 0.f;
+    dx2[r] = dx1[r];
+    dx1[r] = d;
+}
+
+}  // namespace mhfem
+
+// ===========================================================================
+// host side
+namespace mhfem {
+
+struct Guess {
+    mhfem_guess_config cfg{};
+    int rows = 0;
+    DBuf<unsigned short> X;
+    DBuf<float> y, target;
+    DBuf<double> xp, dx1, dx2, r0, Ar0, A2r0, Adx1, rtry, scal;
+    DBuf<uint8_t> dirichlet;
+    DBuf<float> P, m, v, partial, grad;
+    DBuf<PackedWeights> W;
+    int nblocks = 0;
+    int adam_step = 0;
+    int solves = 0;        // solves seen
+    int trained = 0;       // solves used for training
+    int history = 0;       // increments available (0, 1, 2)
+    bool have_weights = false;
+    mhfem_guess_stats last{};
+};
+
+namespace {
+
+inline unsigned gblocks(long n) { return (unsigned)((n + GB - 1) / GB); }
+
+void init_params(Guess &G)
+{
+    // He-normal initialisation of the hidden layers, zero output layer
+    std::vector<float> P(NPARAM_PAD, 0.f);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)G.cfg.seed;
+    auto unif = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return ((st >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    auto normal = [&]() { return std::sqrt(-2.0 * std::log(unif())) * std::cos(6.283185307179586 * unif()); };
+    for (int k = 0; k < NF * NH; k++) P[P_W1 + k] = (float)(normal() * std::sqrt(2.0 / NF));
+    for (int k = 0; k < NH * NH; k++) P[P_W2 + k] = (float)(normal() * std::sqrt(2.0 / NH));
+    MHFEM_HIP(hipMemcpy(G.P.p, P.data(), NPARAM_PAD * sizeof(float), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemset(G.m.p, 0, NPARAM_PAD * sizeof(float)));
+    MHFEM_HIP(hipMemset(G.v.p, 0, NPARAM_PAD * sizeof(float)));
+    G.adam_step = 0;
+}
+
+void pack(Ctx &c, Guess &G)
+{
+    hipLaunchKernelGGL(k_pack_weights, 1, 64, 0, c.stream, G.P.p, G.W.p);
+    MHFEM_HIP(hipGetLastError());
+}
+
+double host_dot(Ctx &c, const double *x, const double *y) { return dot64(c, x, y); }
+
+// one full-batch Adam step on (X, target); returns the mean squared loss
+double train_epoch(Ctx &c, Guess &G, long rows, float lr)
+{
+    hipLaunchKernelGGL(k_mlp_grad, G.nblocks, GB, 0, c.stream, rows, G.X.p, G.target.p, G.W.p,
+                       1.0f / (float)rows, G.partial.p);
+    G.adam_step++;
+    hipLaunchKernelGGL(k_adam, (NPARAM_PAD + GB - 1) / GB, GB, 0, c.stream, G.nblocks, G.partial.p, G.P.p, G.m.p,
+                       G.v.p, G.grad.p, lr, 0.9f, 0.999f, 1e-8f, G.adam_step);
+    MHFEM_HIP(hipGetLastError());
+    pack(c, G);
+    return -1.0;
+}
+
+}  // namespace
+
+Guess *guess_create(Ctx &c)
+{
+    Guess *G = new Guess();
+    const int n = c.rows;
+    G->rows = n;
+    G->X.alloc((size_t)n * NF);
+    G->y.alloc(n);
+    G->target.alloc(n);
+    for (DBuf<double> *b : {&G->xp, &G->dx1, &G->dx2, &G->r0, &G->Ar0, &G->A2r0, &G->Adx1, &G->rtry}) {
+        b->alloc(n);
+        MHFEM_HIP(hipMemset(b->p, 0, (size_t)n * sizeof(double)));
+    }
+    G->scal.alloc(4);
+    std::vector<uint8_t> dflag(n);
+    for (int r = 0; r < n; r++) dflag[r] = (c.h_rowptr[r + 1] - c.h_rowptr[r]) == 1;
+    G->dirichlet.alloc(n);
+    MHFEM_HIP(hipMemcpy(G->dirichlet.p, dflag.data(), n, hipMemcpyHostToDevice));
+    G->P.alloc(NPARAM_PAD);
+    G->m.alloc(NPARAM_PAD);
+    G->v.alloc(NPARAM_PAD);
+    G->grad.alloc(NPARAM_PAD);
+    G->W.alloc(1);
+    const long ntiles = ((long)n + 15) / 16;
+    G->nblocks = (int)std::max<long>(1, std::min<long>(1024, (ntiles + 3) / 4));
+    G->partial.alloc((size_t)G->nblocks * NPARAM_PAD);
+    G->cfg.enabled = 0;
+    G->cfg.train_steps = 100;
+    G->cfg.epochs = 20;
+    G->cfg.learning_rate = 1e-3f;
+    G->cfg.seed = 1;
+    init_params(*G);
+    pack(c, *G);
+    return G;
+}
+
+void guess_destroy(Guess *G) { delete G; }
+
+void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
+{
+    const bool reseed = cfg.seed != G.cfg.seed;
+    G.cfg = cfg;
+    if (G.cfg.epochs < 0) G.cfg.epochs = 0;
+    if (reseed && !G.have_weights) {
+        init_params(G);
+        pack(c, G);
+    }
+}
+
+// Before a solve: features of the warm start and, once the network has been
+// trained, the predicted starting vector (kept only if it lowers the residual).
+void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
+{
+    const int n = G.rows;
+    const unsigned g = gblocks(n);
+    G.last = mhfem_guess_stats{};
+    MHFEM_HIP(hipMemcpyAsync(G.xp.p, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    residual64(c, x, b, G.r0.p);
+    const double rr = host_dot(c, G.r0.p, G.r0.p);
+    G.last.r0_norm = std::sqrt(rr);
+    G.last.r_guess_norm = G.last.r0_norm;
+    if (!(G.cfg.enabled) || G.history < 1 || rr == 0.0) return;
+    spmv64(c, G.r0.p, G.Ar0.p);
+    spmv64(c, G.Ar0.p, G.A2r0.p);
+    spmv64(c, G.dx1.p, G.Adx1.p);
+    const double sx2 = host_dot(c, G.dx1.p, G.dx1.p) / n;
+    double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
+    MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.dx1.p, G.dx2.p,
+                       G.Adx1.p, G.dirichlet.p, G.scal.p, G.X.p);
+    MHFEM_HIP(hipGetLastError());
+    const bool trained = G.have_weights || G.trained > 0;
+    if (!trained || sx2 == 0.0) return;
+    const long ntiles = ((long)n + 15) / 16;
+    hipLaunchKernelGGL(k_mlp_forward, (unsigned)std::max<long>(1, std::min<long>(4096, (ntiles + 3) / 4)), GB, 0,
+                       c.stream, (long)n, G.X.p, G.W.p, G.y.p);
+    hipLaunchKernelGGL(k_apply_guess, g, GB, 0, c.stream, n, G.xp.p, G.y.p, G.scal.p, G.rtry.p);
+    MHFEM_HIP(hipGetLastError());
+    // residual of the candidate (rtry is overwritten with the residual below)
+    MHFEM_HIP(hipMemcpyAsync(G.Ar0.p, G.rtry.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    residual64(c, G.Ar0.p, b, G.rtry.p);
+    const double rg = std::sqrt(host_dot(c, G.rtry.p, G.rtry.p));
+    G.last.r_guess_norm = rg;
+    if (rg < G.last.r0_norm) {
+        MHFEM_HIP(hipMemcpyAsync(x, G.Ar0.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        G.last.used = 1;
+    }
+}
+
+// After a solve: the increment becomes a training target (during the training
+// phase) and enters the history used by the next features.
+void guess_after_solve(Ctx &c, Guess &G, const double *x)
+{
+    const int n = G.rows;
+    const unsigned g = gblocks(n);
+    const bool train = G.cfg.enabled && G.history >= 1 && G.trained < G.cfg.train_steps && G.cfg.epochs > 0;
+    hipLaunchKernelGGL(k_after_solve, g, GB, 0, c.stream, n, x, G.xp.p, G.dx1.p, G.dx2.p, G.scal.p,
+                       train ? G.target.p : (float *)nullptr);
+    MHFEM_HIP(hipGetLastError());
+    if (train) {
+        for (int e = 0; e < G.cfg.epochs; e++) train_epoch(c, G, n, G.cfg.learning_rate);
+        G.trained++;
+        // loss of the last epoch
+        float loss = 0.f;
+        MHFEM_HIP(hipMemcpyAsync(&loss, G.grad.p + NPARAM, sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+        G.last.loss = loss / n;
+    }
+    G.last.trained_steps = G.trained;
+    G.history = std::min(G.history + 1, 2);
+    G.solves++;
+}
+
+void guess_forward(Ctx &c, Guess &G, const uint16_t *X, long rows, float *y)
+{
+    DBuf<unsigned short> dX;
+    DBuf<float> dy;
+    dX.alloc((size_t)rows * NF);
+    dy.alloc(rows);
+    MHFEM_HIP(hipMemcpy(dX.p, X, (size_t)rows * NF * 2, hipMemcpyHostToDevice));
+    const long ntiles = (rows + 15) / 16;
+    hipLaunchKernelGGL(k_mlp_forward, (unsigned)std::max<long>(1, std::min<long>(4096, (ntiles + 3) / 4)), GB, 0,
+                       c.stream, rows, dX.p, G.W.p, dy.p);
+    MHFEM_HIP(hipGetLastError());
+    MHFEM_HIP(hipMemcpyAsync(y, dy.p, rows * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+}
+
+void guess_gradient(Ctx &c, Guess &G, const uint16_t *X, const float *t, long rows, float *grad)
+{
+    DBuf<unsigned short> dX;
+    DBuf<float> dt;
+    dX.alloc((size_t)rows * NF);
+    dt.alloc(rows);
+    MHFEM_HIP(hipMemcpy(dX.p, X, (size_t)rows * NF * 2, hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(dt.p, t, rows * sizeof(float), hipMemcpyHostToDevice));
+    const long ntiles = (rows + 15) / 16;
+    const int nb = (int)std::max<long>(1, std::min<long>(G.nblocks, (ntiles + 3) / 4));
+    hipLaunchKernelGGL(k_mlp_grad, nb, GB, 0, c.stream, rows, dX.p, dt.p, G.W.p, 1.0f / (float)rows, G.partial.p);
+    // reduce only (learning rate 0)
+    hipLaunchKernelGGL(k_adam, (NPARAM_PAD + GB - 1) / GB, GB, 0, c.stream, nb, G.partial.p, G.P.p, G.m.p, G.v.p,
+                       G.grad.p, 0.0f, 0.9f, 0.999f, 1e-8f, 1);
+    MHFEM_HIP(hipGetLastError());
+    MHFEM_HIP(hipMemcpyAsync(grad, G.grad.p, (NPARAM + 1) * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+}
+
+void guess_get_weights(Ctx &c, Guess &G, float *out)
+{
+    MHFEM_HIP(hipMemcpyAsync(out, G.P.p, NPARAM * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+}
+
+void guess_set_weights(Ctx &c, Guess &G, const float *in)
+{
+    MHFEM_HIP(hipMemcpyAsync(G.P.p, in, NPARAM * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    pack(c, G);
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+    G.have_weights = true;
+}
+
+int guess_num_params() { return NPARAM; }
+
+mhfem_guess_stats guess_last_stats(const Guess &G) { return G.last; }
+
+}  // namespace mhfem

@@ -535,12 +535,17 @@ struct Solve {
         }
         MHFEM_HIP(hipGetLastError());
     }
+    // The assembled MHFEM rows are scaled so that every diagonal entry is
+    // exactly 1 (LinearSystem.h:196-201, BoundaryConditions.hpp:200-203,
+    // :375-377): the diagonal preconditioner is then the identity and its
+    // scaling vector is not read.
+    bool jacobi_scales() const { return c.cfg.preconditioner == MHFEM_PREC_DIAGONAL && !c.unit_diagonal; }
+
     template <typename T>
     void prec(const T *x, T *y, const T *dinv)
     {
         if (c.cfg.preconditioner == MHFEM_PREC_ILU0) ilu_apply<T>(x, y);
-        else hipLaunchKernelGGL((k_scale<T>), g, KB, 0, c.stream, n, x,
-                                c.cfg.preconditioner == MHFEM_PREC_DIAGONAL ? dinv : (const T *)nullptr, y);
+        else hipLaunchKernelGGL((k_scale<T>), g, KB, 0, c.stream, n, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
         MHFEM_HIP(hipGetLastError());
     }
 
@@ -556,7 +561,7 @@ struct Solve {
             ilu_apply<T>(tmp, y, k.sc.p);
             if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, true);
             if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, true);
-        } else if (pc == MHFEM_PREC_DIAGONAL) {
+        } else if (jacobi_scales()) {
             hipLaunchKernelGGL((k_spmv<T, true, ND>), g, KB, 0, c.stream, A, x, dinv, y, z, red, fin);
             if constexpr (ND > 0) finish<ND>(fin, true);
         } else {
@@ -818,6 +823,39 @@ static void precond_update(Ctx &c, Krylov &k, Solve &S)
     (void)S;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 helpers for callers outside the Krylov loop (learned initial guess):
+// no convergence flag is consulted.
+__global__ void __launch_bounds__(KB) k_spmv_plain(SellView<double> A, const double *__restrict__ x, double *__restrict__ y)
+{
+    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB) y[row] = sell_row(A, x, row);
+}
+
+void spmv64(Ctx &c, const double *x, double *y)
+{
+    Krylov &k = *c.krylov;
+    Solve S(c, k);
+    hipLaunchKernelGGL(k_spmv_plain, grid_for(k.n), KB, 0, c.stream, S.matrix<double>(), x, y);
+    MHFEM_HIP(hipGetLastError());
+}
+
+void residual64(Ctx &c, const double *x, const double *b, double *r)
+{
+    Krylov &k = *c.krylov;
+    Solve S(c, k);
+    hipLaunchKernelGGL((k_residual<double>), grid_for(k.n), KB, 0, c.stream, S.matrix<double>(), x, b, r);
+    MHFEM_HIP(hipGetLastError());
+}
+
+double dot64(Ctx &c, const double *x, const double *y)
+{
+    Krylov &k = *c.krylov;
+    Solve S(c, k);
+    S.dots<double, 1>(DotArgs<double, 1>{{x}, {y}}, FIN_NONE, false);
+    S.poll();
+    return k.h_sc[S_D0];
+}
+
 // Average duration of the solver's SpMV launch (y = A x on the current
 // matrix), measured with HIP events on the context's stream.
 void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
@@ -863,6 +901,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
     const double tol = c.cfg.convergence_residue;
     const int maxit = c.cfg.max_iterations;
     mhfem_solve_stats out{};
+    if (c.guess && c.guess_on_solve) guess_before_solve(c, *c.guess, b, x);
     if (c.cfg.precision == MHFEM_PRECISION_FP64) {
         if (c.cfg.solver == MHFEM_SOLVER_GMRES) {
             double res = 0;
@@ -910,6 +949,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         out.residue = res;
         out.converged = res < tol;
     }
+    if (c.guess && c.guess_on_solve) guess_after_solve(c, *c.guess, x);
     if (c.timing) {
         MHFEM_HIP(hipEventRecord(e2, c.stream));
         MHFEM_HIP(hipEventSynchronize(e2));

@@ -55,6 +55,16 @@ class Timers(C.Structure):
                 ("iterations", C.c_longlong), ("time_steps", C.c_longlong)]
 
 
+class GuessConfig(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("train_steps", C.c_int), ("epochs", C.c_int),
+                ("learning_rate", C.c_float), ("seed", C.c_uint)]
+
+
+class GuessStats(C.Structure):
+    _fields_ = [("used", C.c_int), ("r0_norm", C.c_double), ("r_guess_norm", C.c_double),
+                ("loss", C.c_double), ("trained_steps", C.c_int)]
+
+
 # every symbol declared in include/mhfem.h with its signature
 _P, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
 SIGNATURES = {
@@ -81,6 +91,13 @@ SIGNATURES = {
     "mhfem_export_system": (_I, [_P, _P, _P, _P, _P]),
     "mhfem_solve_rhs": (_I, [_P, _P, _P, C.POINTER(SolveStats)]),
     "mhfem_benchmark_spmv": (_I, [_P, _I, _I, C.POINTER(_D)]),
+    "mhfem_guess_configure": (_I, [_P, C.POINTER(GuessConfig)]),
+    "mhfem_guess_stats_last": (_I, [_P, C.POINTER(GuessStats)]),
+    "mhfem_guess_num_params": (_I, []),
+    "mhfem_guess_get_weights": (_I, [_P, _P, _I]),
+    "mhfem_guess_set_weights": (_I, [_P, _P, _I]),
+    "mhfem_guess_forward": (_I, [_P, _P, _L, _P]),
+    "mhfem_guess_gradient": (_I, [_P, _P, _P, _L, _P]),
     "mhfem_get_timers": (_I, [_P, C.POINTER(Timers)]),
     "mhfem_reset_timers": (_I, [_P]),
     "mhfem_set_timing": (_I, [_P, _I]),
@@ -263,6 +280,39 @@ class Solver:
         ms = C.c_double()
         self._check(lib().mhfem_benchmark_spmv(self.ctx, PRECISIONS[precision], reps, C.byref(ms)))
         return ms.value
+
+    # ---- learned initial guess
+    def configure_guess(self, enabled=True, train_steps=100, epochs=20, learning_rate=1e-3, seed=1):
+        cfg = GuessConfig(1 if enabled else 0, train_steps, epochs, learning_rate, seed)
+        self._check(lib().mhfem_guess_configure(self.ctx, C.byref(cfg)))
+
+    def guess_stats(self):
+        st = GuessStats()
+        self._check(lib().mhfem_guess_stats_last(self.ctx, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in GuessStats._fields_}
+
+    def guess_weights(self):
+        n = lib().mhfem_guess_num_params()
+        w = np.empty(n, np.float32)
+        self._check(lib().mhfem_guess_get_weights(self.ctx, _ptr(w), n))
+        return w
+
+    def set_guess_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(lib().mhfem_guess_set_weights(self.ctx, _ptr(w), w.size))
+
+    def guess_forward(self, Xbits):
+        X = np.ascontiguousarray(Xbits, np.uint16)
+        y = np.empty(X.shape[0], np.float32)
+        self._check(lib().mhfem_guess_forward(self.ctx, _ptr(X), X.shape[0], _ptr(y)))
+        return y
+
+    def guess_gradient(self, Xbits, t):
+        X = np.ascontiguousarray(Xbits, np.uint16)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.empty(lib().mhfem_guess_num_params() + 1, np.float32)
+        self._check(lib().mhfem_guess_gradient(self.ctx, _ptr(X), _ptr(t), X.shape[0], _ptr(g)))
+        return g[:-1], float(g[-1])
 
     def synchronize(self):
         self._check(lib().mhfem_synchronize(self.ctx))
