@@ -96,6 +96,8 @@ typedef struct {
     int precision;              /* MHFEM_PRECISION_* */
     double inner_residue;       /* FP32 inner tolerance for MHFEM_PRECISION_FP32_REFINED */
     int check_every;            /* host convergence poll period (iterations), >= 1 */
+    int rows_per_block;         /* rows per work-group of the iteration kernels, 0: default
+                                   (mhfem_autotune picks it by measurement) */
 } mhfem_solver_config;
 
 typedef struct {
@@ -158,6 +160,27 @@ int mhfem_post_iterate(mhfem_ctx *ctx, double time, double tau);    /* Solver.hp
 /* control.h::solve(problem, t0, t1, dt) (control.h:64-102); stats of the last step */
 int mhfem_solve_interval(mhfem_ctx *ctx, double t0, double t1, double dt,
                          mhfem_solve_stats *last);
+
+/* One time step (preIterate, assemble, solve, postIterate) replayed from HIP
+ * graphs captured on the first call (re-captured when tau changes): FP64
+ * BiCGStab only; equivalent to the four calls above. */
+int mhfem_step(mhfem_ctx *ctx, double time, double tau, mhfem_solve_stats *stats);
+/* Measure the rows per work-group of the iteration kernels on the assembled
+ * pattern and keep the fastest; returns the choice and its kernel time. */
+int mhfem_autotune(mhfem_ctx *ctx, int *rows_per_block, double *kernel_ms);
+
+/* ---- batched ensembles (no counterpart in the reference, where each
+ * instance is its own Solver).  The meshes are concatenated into one context
+ * whose system is block diagonal; every instance (segment) keeps its own
+ * BiCGStab scalars and converges independently, and a step of the whole
+ * ensemble is one graph launch (mhfem_step).  Arrays are the concatenation of
+ * the instances' arrays in the layouts above (cells and faces numbered
+ * instance by instance). */
+mhfem_ctx *mhfem_create_batch(const mhfem_mesh_desc *meshes, int count, const mhfem_model_desc *model);
+/* cell_offsets[count+1], face_offsets[count+1] (either may be NULL) */
+int mhfem_batch_info(const mhfem_ctx *ctx, int *count, int32_t *cell_offsets, int32_t *face_offsets);
+/* iterations and final residue of every instance in the last solve */
+int mhfem_batch_stats(mhfem_ctx *ctx, int32_t *iterations, double *residues, int count);
 
 /* ---- linear system export (Solver::saveLinearSystem, Solver.hpp:964-977):
  * rowptr[rows+1], col[nnz], val[nnz], rhs[rows]; any pointer may be NULL */

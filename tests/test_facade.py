@@ -1,0 +1,39 @@
+"""The C++ facade (include/mhfem/) compiles as plain C++17 against the C ABI,
+and its heat-equation example (the reference's examples/HeatEquation on the
+MI355X path) reproduces the oracle."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from mhfem_amd import problems
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXAMPLE = os.path.join(ROOT, "tnl-mhfem_amd", "examples", "heat_equation")
+
+
+def test_facade_compiles():
+    src = os.path.join(ROOT, "tnl-mhfem_amd", "examples", "heat_equation.cpp")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                    "-I", os.path.join(ROOT, "include"), src], check=True)
+
+
+@pytest.mark.gpu
+def test_heat_equation_example_matches_oracle():
+    import pyoracle
+
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "z.bin")
+        res = subprocess.run([EXAMPLE, out, "32", "20"], capture_output=True, text=True, timeout=120)
+        print(res.stdout[-2000:], res.stderr[-2000:])
+        assert res.returncode == 0
+        assert "Total number of time steps:" in res.stdout
+        z = np.fromfile(out, np.float64)
+    pr = problems.heat_2d(s=32, steps=20)
+    o = pyoracle.Oracle(pr)
+    o.run(0.0, 20 * pr.dt, pr.dt, tol=1e-13)
+    zo = o.array("Z_iK")
+    assert z.size == zo.size
+    assert np.linalg.norm(z - zo) / np.linalg.norm(zo) < 1e-8

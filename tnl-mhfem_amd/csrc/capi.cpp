@@ -303,6 +303,8 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
         MHFEM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) MHFEM_HIP(hipEventCreate(&e));
         build_topology(*c, mesh->cell_vertices);
+        c->h_seg_cells = {0, c->C};
+        c->h_seg_faces = {0, c->F};
         const size_t Cn = c->C, fpc = c->fpc;
         // measures
         std::vector<double> cm(Cn), fm(c->F, 0.0);
@@ -362,6 +364,70 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
     }
     if (c) mhfem_destroy(reinterpret_cast<mhfem_ctx *>(c));
     return nullptr;
+}
+
+mhfem_ctx *mhfem_create_batch(const mhfem_mesh_desc *meshes, int count, const mhfem_model_desc *model)
+{
+    try {
+        if (!meshes || count < 1 || !model) throw Error{MHFEM_E_ARGUMENT, "mhfem_create_batch: bad argument"};
+        const int dim = meshes[0].dim;
+        std::vector<double> pts;
+        std::vector<int32_t> cells;
+        std::vector<int> cell_off(1, 0);
+        int voff = 0;
+        for (int k = 0; k < count; k++) {
+            const mhfem_mesh_desc &m = meshes[k];
+            if (m.dim != dim || !m.points || !m.cell_vertices || m.num_cells < 1)
+                throw Error{MHFEM_E_ARGUMENT, "mhfem_create_batch: instance " + std::to_string(k) + " is invalid"};
+            pts.insert(pts.end(), m.points, m.points + (size_t)m.num_points * dim);
+            for (size_t q = 0; q < (size_t)m.num_cells * (dim + 1); q++) {
+                const int v = m.cell_vertices[q];
+                if (v < 0 || v >= m.num_points) throw Error{MHFEM_E_MESH, "cell vertex index out of range"};
+                cells.push_back(v + voff);
+            }
+            voff += m.num_points;
+            cell_off.push_back((int)(cells.size() / (dim + 1)));
+        }
+        mhfem_mesh_desc u{dim, voff, cell_off.back(), pts.data(), cells.data()};
+        mhfem_ctx *ctx = mhfem_create(&u, model);
+        if (!ctx) return nullptr;
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        // faces are numbered by first appearance over the cells, so the faces
+        // of each instance form a contiguous range
+        std::vector<int> face_off(1, 0);
+        int fmax = -1;
+        for (int k = 0; k < count; k++) {
+            int lo = INT32_MAX;
+            for (size_t q = (size_t)cell_off[k] * c->fpc; q < (size_t)cell_off[k + 1] * c->fpc; q++) {
+                fmax = std::max(fmax, c->h_cell_faces[q]);
+                lo = std::min(lo, c->h_cell_faces[q]);
+            }
+            if (lo != face_off.back()) {
+                mhfem_destroy(ctx);
+                throw Error{MHFEM_E_MESH, "mhfem_create_batch: instances share faces"};
+            }
+            face_off.push_back(fmax + 1);
+        }
+        c->h_seg_cells = cell_off;
+        c->h_seg_faces = face_off;
+        return ctx;
+    } catch (const Error &e) {
+        g_create_error = e.msg;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+    }
+    return nullptr;
+}
+
+int mhfem_batch_info(const mhfem_ctx *ctx, int *count, int32_t *cell_offsets, int32_t *face_offsets)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c) return MHFEM_E_ARGUMENT;
+    const int nseg = (int)c->h_seg_cells.size() - 1;
+    if (count) *count = nseg;
+    if (cell_offsets) std::copy(c->h_seg_cells.begin(), c->h_seg_cells.end(), cell_offsets);
+    if (face_offsets) std::copy(c->h_seg_faces.begin(), c->h_seg_faces.end(), face_offsets);
+    return MHFEM_OK;
 }
 
 void mhfem_destroy(mhfem_ctx *ctx)
@@ -525,6 +591,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
             }
         for (int r = 0; r < c.rows; r++) c.h_rowptr[r + 1] += c.h_rowptr[r];
         c.nnz = c.h_rowptr[c.rows];
+        c.h_seg_rows.clear();
+        for (int f : c.h_seg_faces) c.h_seg_rows.push_back(f * n);
         if ((long)c.nnz >= (long)INT32_MAX) throw Error{MHFEM_E_ARGUMENT, "matrix too large for int32 row pointers"};
         // columns (LinearSystem.h:181-245, BoundaryConditions.hpp:196-218)
         c.h_col.assign(c.nnz, 0);
@@ -786,6 +854,43 @@ int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *
         need_guess(c);
         if (!features || !targets || !grad || rows <= 0) throw Error{MHFEM_E_ARGUMENT, "bad arguments"};
         guess_gradient(c, *c.guess, features, targets, rows, grad);
+    });
+}
+
+int mhfem_batch_stats(mhfem_ctx *ctx, int32_t *iterations, double *residues, int count)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        krylov_segment_stats(c, iterations, residues, count);
+    });
+}
+
+int mhfem_step(mhfem_ctx *ctx, double time, double tau, mhfem_solve_stats *stats)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "mhfem_setup_linear_system must be called first"};
+        if (!(tau > 0)) throw Error{MHFEM_E_ARGUMENT, "time step must be positive"};
+        if (c.guess_on_solve) throw Error{MHFEM_E_STATE, "graph steps do not run the learned guess; use the four calls"};
+        mhfem_solve_stats st{};
+        krylov_graph_step(c, time, tau, &st);
+        c.timers.iterations += st.iterations;
+        c.timers.time_steps += 1;
+        if (stats) *stats = st;
+        if (!st.converged)
+            throw Error{MHFEM_E_NOT_CONVERGED, "MHFEM error: the linear system solver did not converge (" +
+                                                   std::to_string(st.iterations) + " iterations)."};
+    });
+}
+
+int mhfem_autotune(mhfem_ctx *ctx, int *rows_per_block, double *kernel_ms)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        double ms = 0;
+        const int rpb = krylov_autotune(c, &ms);
+        c.cfg.rows_per_block = rpb;
+        if (rows_per_block) *rows_per_block = rpb;
+        if (kernel_ms) *kernel_ms = ms;
     });
 }
 

@@ -112,6 +112,8 @@ struct Ctx {
     hipStream_t stream = nullptr;
     // host topology
     std::vector<int> h_cell_faces, h_face_cells;
+    // segments (batched ensembles): cells, faces and rows of each instance
+    std::vector<int> h_seg_cells, h_seg_faces, h_seg_rows;
     // device topology
     DBuf<int> cell_faces, cell_vertices, face_cells;
     DBuf<uint64_t> face_slots;
@@ -162,6 +164,9 @@ void krylov_destroy(Krylov *k);
 // solve A x = b (b, x device arrays of length rows); returns stats
 void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st);
 void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms);
+int krylov_autotune(Ctx &c, double *best_ms);
+void krylov_segment_stats(Ctx &c, int *iterations, double *residues, int count);
+void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st);
 void spmv64(Ctx &c, const double *x, double *y);
 void residual64(Ctx &c, const double *x, const double *b, double *r);
 double dot64(Ctx &c, const double *x, const double *y);

@@ -2,18 +2,25 @@
 //
 // * SELL-64 SpMV: one lane per row, a wavefront covers one 64-row slice, so
 //   every matrix load of a wavefront is one contiguous 512-byte segment.
-// * BiCGStab (left preconditioned, van der Vorst 1992) as five fused kernels
-//   per iteration; dot products reduce wavefront -> work-group -> grid
-//   (deterministic order) and the scalar recurrences
-//   (alpha, omega, beta, residue, convergence) are evaluated on the device, so
-//   the host only polls a status word every `check_every` iterations.
-//   Partial sums are finished by a one-work-group kernel on the same stream.  Kernels
-//   queued after convergence see the DONE flag and return immediately.
+// * Segmented BiCGStab (left preconditioned, van der Vorst 1992).  The
+//   unknowns are split into segments -- one for an ordinary solve, one per
+//   instance for a batched ensemble (block-diagonal system).  Every work-group
+//   owns a contiguous row range inside one segment; each segment carries its
+//   own scalar recurrence (alpha, omega, beta, rho, residue, iteration count)
+//   and DONE/RESTART flags in device memory.  Dot products reduce wavefront ->
+//   work-group -> one partial per work-group, and a finalize kernel (one
+//   work-group per segment) sums the partials in a fixed order and evaluates
+//   the recurrence, so the iteration is deterministic and the host only polls
+//   the flags every `check_every` iterations.  Work queued after a segment
+//   converged sees its DONE flag and returns; a whole step can therefore be
+//   captured once as a HIP graph and replayed.
 // * GMRES(m), left preconditioned, classical Gram-Schmidt with one
-//   re-orthogonalisation pass; Givens rotations on the host.
+//   re-orthogonalisation pass; Givens rotations on the host (one segment).
 // * Jacobi (diagonal) and ILU(0) (Saad Alg. 10.4, exact, level-scheduled:
 //   rows of one dependency level are processed by one launch).
-// * FP32 inner BiCGStab inside FP64 iterative refinement.
+// * FP32 inner BiCGStab inside FP64 iterative refinement (one segment).
+// * An autotuner picks the rows per work-group of the fused iteration kernels
+//   by timing them on the assembled pattern.
 //
 // Solver call sites in the reference: Solver.hpp:1230-1265 (TNL LinearSolver +
 // Preconditioner chosen by name, configSetup.h:31-62).
@@ -28,19 +35,33 @@ namespace mhfem {
 constexpr int KB = 256;
 constexpr int MAXG = 2048;
 
-// device scalar slots
+// device scalar slots (per segment)
 enum Slot {
     S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BETA, S_RAST_SQ, S_BNORM, S_RESID, S_TOL, S_IT,
     S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, NSLOTS = 32
 };
 enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT };
+// which segments a kernel works on
+enum Mode { M_ALL = 0, M_ACTIVE = 1, M_RESTART = 2 };
 
-struct Red {
-    double *partials;  // one row of per-work-group partial sums per dot product
-    double *sc;        // device scalar slots
+// Segmented work decomposition.  Work-group b covers rows
+// [blk_row[b], blk_row[b+1]) of segment blk_seg[b]; segment s owns the
+// work-groups [seg_blk[s], seg_blk[s+1]).
+struct Seg {
+    int nseg, nblk;
+    const int *blk_seg, *blk_row, *seg_blk;
+    double *sc;        // [nseg][NSLOTS]
+    double *partials;  // [ND][nblk]
 };
 
 __device__ __forceinline__ bool halted(const double *sc) { return sc[S_DONE] != 0.0 || sc[S_RESTART] != 0.0; }
+
+__device__ __forceinline__ bool seg_selected(const double *sc, int mode)
+{
+    if (mode == M_ACTIVE) return !halted(sc);
+    if (mode == M_RESTART) return sc[S_RESTART] != 0.0;
+    return true;
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -87,16 +108,14 @@ __device__ void finalize(double *sc, int fin)
     }
 }
 
-// Reduce ND per-thread values to one partial sum per work-group
-// (wavefront shuffles, then LDS).  The partials are summed by k_finalize,
-// launched next on the same stream: a kernel boundary orders the writes, so
-// no work-group issues a device-scope fence (which on gfx950 writes back the
-// L2 of its XCD and, once per work-group, costs more than the kernel's own
-// streaming).
+// One partial sum per work-group (wavefront shuffles, then LDS).  The
+// partials are summed by k_finalize, launched next on the same stream: the
+// kernel boundary orders the writes, so no work-group issues a device-scope
+// fence (on gfx950 that writes back the L2 of its XCD, and once per
+// work-group it costs more than the kernel's own streaming).
 template <int ND>
-__device__ __forceinline__ void grid_reduce(double (&v)[ND], Red red, int fin)
+__device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg)
 {
-    (void)fin;
     __shared__ double sm[ND][KB / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 #pragma unroll
@@ -111,23 +130,26 @@ __device__ __forceinline__ void grid_reduce(double (&v)[ND], Red red, int fin)
             double s = sm[d][0];
 #pragma unroll
             for (int q = 1; q < KB / 64; q++) s += sm[d][q];
-            red.partials[(long)d * gridDim.x + blockIdx.x] = s;
+            sg.partials[(long)d * sg.nblk + blockIdx.x] = s;
         }
     }
 }
 
-// Sum the per-work-group partials of the previous kernel into sc[S_D0 + d]
-// and evaluate the scalar recurrence `fin`.  One work-group.
+// Sum the partials of each selected segment into sc[S_D0 + d] and evaluate
+// the scalar recurrence `fin`.  One work-group per segment.
 template <int ND>
-__global__ void __launch_bounds__(KB) k_finalize(int G, Red red, int fin, int check_halt)
+__global__ void __launch_bounds__(KB) k_finalize(Seg sg, int fin, int mode)
 {
-    if (check_halt && halted(red.sc)) return;
+    const int s = blockIdx.x;
+    double *sc = sg.sc + (long)s * NSLOTS;
+    if (!seg_selected(sc, mode)) return;
     __shared__ double sm[ND][KB / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int b0 = sg.seg_blk[s], b1 = sg.seg_blk[s + 1];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
         double a = 0;
-        for (int b = tid; b < G; b += KB) a += red.partials[(long)d * G + b];
+        for (int b = b0 + tid; b < b1; b += KB) a += sg.partials[(long)d * sg.nblk + b];
         a = wave_sum(a);
         if (lane == 0) sm[d][wid] = a;
     }
@@ -135,17 +157,28 @@ __global__ void __launch_bounds__(KB) k_finalize(int G, Red red, int fin, int ch
     if (tid == 0) {
 #pragma unroll
         for (int d = 0; d < ND; d++) {
-            double s = sm[d][0];
+            double v = sm[d][0];
 #pragma unroll
-            for (int q = 1; q < KB / 64; q++) s += sm[d][q];
-            red.sc[S_D0 + d] = s;
+            for (int q = 1; q < KB / 64; q++) v += sm[d][q];
+            sc[S_D0 + d] = v;
         }
-        finalize(red.sc, fin);
+        finalize(sc, fin);
     }
 }
 
+// scalar state of every segment at the start of a solve (a kernel rather
+// than a host copy, so that it can be captured in a graph)
+__global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    double *p = sc + (long)s * NSLOTS;
+    for (int k = 0; k < NSLOTS; k++) p[k] = 0.0;
+    p[S_TOL] = tol;
+    p[S_MAXIT] = maxit;
+}
+
 // ---------------------------------------------------------------------------
-// SpMV: y = D (A x) with D = dinv (Jacobi) or identity; optional dot (y, z).
 template <typename T>
 __device__ __forceinline__ T sell_row(const SellView<T> &A, const T *__restrict__ x, int row)
 {
@@ -159,38 +192,36 @@ __device__ __forceinline__ T sell_row(const SellView<T> &A, const T *__restrict_
     return sum;
 }
 
+#define SEG_BLOCK_PROLOGUE(sg, mode)                                   \
+    const int seg_ = (sg).blk_seg[blockIdx.x];                         \
+    double *sc_ = (sg).sc + (long)seg_ * NSLOTS;                       \
+    if (!seg_selected(sc_, (mode))) return;                            \
+    const int row0_ = (sg).blk_row[blockIdx.x], row1_ = (sg).blk_row[blockIdx.x + 1]
+
+// y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y)
 template <typename T, bool DIAG, int ND>
-__global__ void __launch_bounds__(KB) k_spmv(SellView<T> A, const T *__restrict__ x, const T *__restrict__ dinv,
-                                             T *__restrict__ y, const T *__restrict__ z, Red red, int fin)
+__global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
+                                              const T *__restrict__ dinv, T *__restrict__ y, const T *__restrict__ z)
 {
-    // ND = 0: no dot; 1: (y, z); 2: (y, z) and (y, y)
-    if (halted(red.sc)) return;
+    SEG_BLOCK_PROLOGUE(sg, mode);
     double acc[ND > 0 ? ND : 1] = {};
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB) {
+    for (int row = row0_ + threadIdx.x; row < row1_; row += KB) {
         T s = sell_row(A, x, row);
         if (DIAG) s *= dinv[row];
         y[row] = s;
         if constexpr (ND >= 1) acc[0] += (double)s * (double)z[row];
         if constexpr (ND >= 2) acc[1] += (double)s * (double)s;
     }
-    if constexpr (ND > 0) grid_reduce<ND>(acc, red, fin);
+    if constexpr (ND > 0) block_partial<ND>(acc, sg);
 }
 
-// y = b - A x  (residual, then preconditioned by the caller)
+// y = b - A x on the selected segments
 template <typename T>
-__global__ void __launch_bounds__(KB) k_residual(SellView<T> A, const T *__restrict__ x, const T *__restrict__ b,
-                                                 T *__restrict__ y)
+__global__ void __launch_bounds__(KB) k_sresidual(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
+                                                  const T *__restrict__ b, T *__restrict__ y)
 {
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB)
-        y[row] = b[row] - sell_row(A, x, row);
-}
-
-// FP64 residual of the refinement loop, stored in FP32: r32 = (float)(b - A x)
-__global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const double *__restrict__ x,
-                                                       const double *__restrict__ b, float *__restrict__ r32)
-{
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB)
-        r32[row] = (float)(b[row] - sell_row(A, x, row));
+    SEG_BLOCK_PROLOGUE(sg, mode);
+    for (int row = row0_ + threadIdx.x; row < row1_; row += KB) y[row] = b[row] - sell_row(A, x, row);
 }
 
 template <typename T, int ND>
@@ -200,38 +231,58 @@ struct DotArgs {
 };
 
 template <typename T, int ND>
-__global__ void __launch_bounds__(KB) k_dots(int n, DotArgs<T, ND> a, Red red, int fin, int check_halt)
+__global__ void __launch_bounds__(KB) k_sdots(Seg sg, int mode, DotArgs<T, ND> a)
 {
-    if (check_halt && halted(red.sc)) return;
+    SEG_BLOCK_PROLOGUE(sg, mode);
     double acc[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) acc[d] = 0;
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB)
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB)
 #pragma unroll
         for (int d = 0; d < ND; d++) acc[d] += (double)a.x[d][i] * (double)a.y[d][i];
-    grid_reduce<ND>(acc, red, fin);
+    block_partial<ND>(acc, sg);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KB) k_sscale(Seg sg, int mode, const T *__restrict__ x, const T *__restrict__ d,
+                                               T *__restrict__ y)
+{
+    SEG_BLOCK_PROLOGUE(sg, mode);
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) y[i] = d ? x[i] * d[i] : x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KB) k_scopy2(Seg sg, int mode, const T *__restrict__ src, T *__restrict__ a,
+                                               T *__restrict__ b)
+{
+    SEG_BLOCK_PROLOGUE(sg, mode);
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) {
+        const T v = src[i];
+        a[i] = v;
+        if (b) b[i] = v;
+    }
 }
 
 // s = r - alpha Ap
 template <typename T>
-__global__ void __launch_bounds__(KB) k_bicg_s(int n, const T *__restrict__ r, const T *__restrict__ Ap,
-                                               T *__restrict__ s, const double *sc)
+__global__ void __launch_bounds__(KB) k_bicg_s(Seg sg, const T *__restrict__ r, const T *__restrict__ Ap,
+                                               T *__restrict__ s)
 {
-    if (halted(sc)) return;
-    const T alpha = (T)sc[S_ALPHA];
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) s[i] = r[i] - alpha * Ap[i];
+    SEG_BLOCK_PROLOGUE(sg, M_ACTIVE);
+    const T alpha = (T)sc_[S_ALPHA];
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) s[i] = r[i] - alpha * Ap[i];
 }
 
 // x += alpha p + omega s; r = s - omega As; dots (r, r_ast), (r, r)
 template <typename T>
-__global__ void __launch_bounds__(KB) k_bicg_xr(int n, T *__restrict__ x, const T *__restrict__ p,
-                                                const T *__restrict__ s, const T *__restrict__ As,
-                                                T *__restrict__ r, const T *__restrict__ r_ast, Red red)
+__global__ void __launch_bounds__(KB) k_bicg_xr(Seg sg, T *__restrict__ x, const T *__restrict__ p,
+                                                const T *__restrict__ s, const T *__restrict__ As, T *__restrict__ r,
+                                                const T *__restrict__ r_ast)
 {
-    if (halted(red.sc)) return;
-    const T alpha = (T)red.sc[S_ALPHA], omega = (T)red.sc[S_OMEGA];
+    SEG_BLOCK_PROLOGUE(sg, M_ACTIVE);
+    const T alpha = (T)sc_[S_ALPHA], omega = (T)sc_[S_OMEGA];
     double acc[2] = {0, 0};
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) {
         const T si = s[i];
         x[i] += alpha * p[i] + omega * si;
         const T ri = si - omega * As[i];
@@ -239,33 +290,41 @@ __global__ void __launch_bounds__(KB) k_bicg_xr(int n, T *__restrict__ x, const 
         acc[0] += (double)ri * (double)r_ast[i];
         acc[1] += (double)ri * (double)ri;
     }
-    grid_reduce<2>(acc, red, FIN_UPDATE);
+    block_partial<2>(acc, sg);
 }
 
 // p = r + beta (p - omega Ap)
 template <typename T>
-__global__ void __launch_bounds__(KB) k_bicg_p(int n, const T *__restrict__ r, T *__restrict__ p,
-                                               const T *__restrict__ Ap, const double *sc)
+__global__ void __launch_bounds__(KB) k_bicg_p(Seg sg, const T *__restrict__ r, T *__restrict__ p,
+                                               const T *__restrict__ Ap)
 {
-    if (halted(sc)) return;
-    const T beta = (T)sc[S_BETA], omega = (T)sc[S_OMEGA];
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) p[i] = r[i] + beta * (p[i] - omega * Ap[i]);
+    SEG_BLOCK_PROLOGUE(sg, M_ACTIVE);
+    const T beta = (T)sc_[S_BETA], omega = (T)sc_[S_OMEGA];
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) p[i] = r[i] + beta * (p[i] - omega * Ap[i]);
+}
+
+// ---------------------------------------------------------------------------
+// unsegmented helpers (no flags consulted)
+template <typename T>
+__global__ void __launch_bounds__(KB) k_residual(SellView<T> A, const T *__restrict__ x, const T *__restrict__ b,
+                                                 T *__restrict__ y)
+{
+    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB)
+        y[row] = b[row] - sell_row(A, x, row);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(KB) k_copy2(int n, const T *__restrict__ src, T *__restrict__ a, T *__restrict__ b)
+__global__ void __launch_bounds__(KB) k_spmv_plain(SellView<T> A, const T *__restrict__ x, T *__restrict__ y)
 {
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
-        const T v = src[i];
-        a[i] = v;
-        if (b) b[i] = v;
-    }
+    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB) y[row] = sell_row(A, x, row);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(KB) k_scale(int n, const T *__restrict__ x, const T *__restrict__ d, T *__restrict__ y)
+// FP64 residual of the refinement loop, stored in FP32: r32 = (float)(b - A x)
+__global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const double *__restrict__ x,
+                                                       const double *__restrict__ b, float *__restrict__ r32)
 {
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) y[i] = d ? x[i] * d[i] : x[i];
+    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB)
+        r32[row] = (float)(b[row] - sell_row(A, x, row));
 }
 
 template <typename TO, typename TI>
@@ -330,6 +389,7 @@ __global__ void __launch_bounds__(KB) k_ilu_factor(const int *__restrict__ rows,
     }
 }
 
+// sc: flags of the single segment (skip once it has halted) or nullptr
 template <typename T>
 __global__ void __launch_bounds__(KB) k_ilu_lower(const int *__restrict__ rows, int count, const int *__restrict__ rp,
                                                   const int *__restrict__ col, const int *__restrict__ dg,
@@ -358,8 +418,6 @@ __global__ void __launch_bounds__(KB) k_ilu_upper(const int *__restrict__ rows, 
 }
 
 struct Ilu0 {
-    int n = 0;
-    long nnz = 0;
     std::vector<int> h_lptr, h_uptr;  // level pointers
     DBuf<int> lrows, urows, diag;     // rows in level order, diagonal positions
     DBuf<double> lu;
@@ -371,29 +429,68 @@ struct Krylov {
     int n = 0;
     DBuf<double> r, r_ast, p, Ap, s, As, tmp, tmp2, dinv;
     DBuf<int64_t> diag_pos;  // SELL position of the diagonal of each row
+    // segmented decomposition
+    int nseg = 1, rows_per_block = 2048;
+    DBuf<int> blk_seg, blk_row, seg_blk;
+    int nblk = 0;
     DBuf<double> partials;
     DBuf<double> sc;
-    double *h_sc = nullptr;  // pinned
+    double *h_sc = nullptr;  // pinned, nseg * NSLOTS
     Ilu0 ilu;
     // FP32 inner solver
     DBuf<float> val32, dinv32, b32, x32, r32, r_ast32, p32, Ap32, s32, As32, tmp32, rhs32;
     // GMRES
     DBuf<double> V;
     int V_m = 0;
+    // captured step graphs
+    hipGraphExec_t g_first = nullptr, g_more = nullptr, g_post = nullptr;
+    double g_tau = 0;
+    int g_iters = 0;
 };
 
 static inline unsigned grid_for(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + KB - 1) / KB, MAXG)); }
+
+// work-group tables for `rpb` rows per work-group, segment aligned
+static void build_blocks(Ctx &c, Krylov &k, int rpb)
+{
+    std::vector<int> bseg, brow, sblk(1, 0);
+    for (int s = 0; s < k.nseg; s++) {
+        const int r0 = c.h_seg_rows[s], r1 = c.h_seg_rows[s + 1];
+        for (int r = r0; r < r1; r += rpb) {
+            bseg.push_back(s);
+            brow.push_back(r);
+        }
+        if (r1 == r0) {  // empty segment: one empty work-group keeps the tables uniform
+            bseg.push_back(s);
+            brow.push_back(r0);
+        }
+        sblk.push_back((int)bseg.size());
+    }
+    brow.push_back(c.h_seg_rows[k.nseg]);
+    // brow[b + 1] is the end of work-group b: the next work-group starts where
+    // this one ends, and every segment's first work-group starts at its first row
+    k.nblk = (int)bseg.size();
+    k.rows_per_block = rpb;
+    k.blk_seg.alloc(k.nblk);
+    k.blk_row.alloc(k.nblk + 1);
+    k.seg_blk.alloc(k.nseg + 1);
+    MHFEM_HIP(hipMemcpy(k.blk_seg.p, bseg.data(), k.nblk * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.blk_row.p, brow.data(), (k.nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.seg_blk.p, sblk.data(), (k.nseg + 1) * sizeof(int), hipMemcpyHostToDevice));
+    k.partials.alloc((size_t)2 * k.nblk);
+}
 
 Krylov *krylov_create(Ctx &c)
 {
     Krylov *k = new Krylov();
     const int n = c.rows;
     k->n = n;
+    k->nseg = (int)c.h_seg_rows.size() - 1;
     for (DBuf<double> *b : {&k->r, &k->r_ast, &k->p, &k->Ap, &k->s, &k->As, &k->tmp, &k->tmp2, &k->dinv})
         b->alloc(n);
-    k->partials.alloc(4 * MAXG);
-    k->sc.alloc(NSLOTS);
-    MHFEM_HIP(hipHostMalloc((void **)&k->h_sc, NSLOTS * sizeof(double), hipHostMallocDefault));
+    k->sc.alloc((size_t)k->nseg * NSLOTS);
+    MHFEM_HIP(hipHostMalloc((void **)&k->h_sc, (size_t)k->nseg * NSLOTS * sizeof(double), hipHostMallocDefault));
+    build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
     // diagonal positions in SELL storage
     std::vector<int64_t> dpos(n);
     for (int r = 0; r < n; r++) {
@@ -408,8 +505,6 @@ Krylov *krylov_create(Ctx &c)
 
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
         Ilu0 &I = k->ilu;
-        I.n = n;
-        I.nnz = c.nnz;
         std::vector<int> dg(n), lev(n), ulev(n);
         for (int r = 0; r < n; r++)
             for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++)
@@ -456,15 +551,25 @@ Krylov *krylov_create(Ctx &c)
     return k;
 }
 
+static void drop_graphs(Krylov &k)
+{
+    for (hipGraphExec_t *g : {&k.g_first, &k.g_more, &k.g_post})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+}
+
 void krylov_destroy(Krylov *k)
 {
     if (!k) return;
+    drop_graphs(*k);
     if (k->h_sc) (void)hipHostFree(k->h_sc);
     delete k;
 }
 
 // ---------------------------------------------------------------------------
-// host-side helpers
+// host-side driver
 namespace {
 
 struct Solve {
@@ -472,11 +577,11 @@ struct Solve {
     Krylov &k;
     int n;
     unsigned g;
-    Red red;
+    Seg sg;
 
     Solve(Ctx &c_, Krylov &k_) : c(c_), k(k_), n(k_.n), g(grid_for(k_.n))
     {
-        red = Red{k.partials.p, k.sc.p};
+        sg = Seg{k.nseg, k.nblk, k.blk_seg.p, k.blk_row.p, k.seg_blk.p, k.sc.p, k.partials.p};
     }
 
     template <typename T>
@@ -484,145 +589,183 @@ struct Solve {
 
     void set_scalars(double tol, int maxit)
     {
-        double h[NSLOTS] = {};
-        h[S_TOL] = tol;
-        h[S_MAXIT] = maxit;
-        MHFEM_HIP(hipMemcpyAsync(k.sc.p, h, sizeof(h), hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(k_init_scalars, (k.nseg + 63) / 64, 64, 0, c.stream, k.sc.p, k.nseg, tol, (double)maxit);
+        MHFEM_HIP(hipGetLastError());
     }
     void poll()
     {
-        MHFEM_HIP(hipMemcpyAsync(k.h_sc, k.sc.p, NSLOTS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipMemcpyAsync(k.h_sc, k.sc.p, (size_t)k.nseg * NSLOTS * sizeof(double), hipMemcpyDeviceToHost,
+                                 c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
     }
-    void clear_flag(int slot)
+    const double *seg_sc(int s) const { return k.h_sc + (long)s * NSLOTS; }
+    bool all_done() const
     {
-        const double z = 0;
-        MHFEM_HIP(hipMemcpyAsync(k.sc.p + slot, &z, sizeof(double), hipMemcpyHostToDevice, c.stream));
+        for (int s = 0; s < k.nseg; s++)
+            if (seg_sc(s)[S_DONE] == 0.0) return false;
+        return true;
+    }
+    bool any_restart() const
+    {
+        for (int s = 0; s < k.nseg; s++)
+            if (seg_sc(s)[S_RESTART] != 0.0) return true;
+        return false;
     }
 
-    template <typename T, int ND>
-    void dots(DotArgs<T, ND> a, int fin, bool check_halt)
-    {
-        hipLaunchKernelGGL((k_dots<T, ND>), g, KB, 0, c.stream, n, a, red, fin, check_halt ? 1 : 0);
-        finish<ND>(fin, check_halt);
-    }
-    // sum the partials of the reducing kernel just launched (grid g)
     template <int ND>
-    void finish(int fin, bool check_halt)
+    void finish(int fin, int mode)
     {
-        hipLaunchKernelGGL((k_finalize<ND>), 1, KB, 0, c.stream, (int)g, red, fin, check_halt ? 1 : 0);
+        hipLaunchKernelGGL((k_finalize<ND>), k.nseg, KB, 0, c.stream, sg, fin, mode);
         MHFEM_HIP(hipGetLastError());
+    }
+    template <typename T, int ND>
+    void dots(DotArgs<T, ND> a, int fin, int mode)
+    {
+        hipLaunchKernelGGL((k_sdots<T, ND>), k.nblk, KB, 0, c.stream, sg, mode, a);
+        finish<ND>(fin, mode);
     }
 
-    // ---- preconditioner apply y = M^{-1} x (T = double or float)
-    template <typename T>
-    void ilu_apply(const T *x, T *y, const double *sc = nullptr)
-    {
-        Ilu0 &I = k.ilu;
-        const T *lu;
-        if constexpr (sizeof(T) == 8) lu = I.lu.p;
-        else lu = I.lu32.p;
-        const int *rp = c.rowptr.p, *col = c.csr_col.p;
-        for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
-            const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
-            hipLaunchKernelGGL((k_ilu_lower<T>), (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
-                               rp, col, I.diag.p, lu, x, y, sc);
-        }
-        for (size_t l = 0; l + 1 < I.h_uptr.size(); l++) {
-            const int cnt = I.h_uptr[l + 1] - I.h_uptr[l];
-            hipLaunchKernelGGL((k_ilu_upper<T>), (cnt + KB - 1) / KB, KB, 0, c.stream, I.urows.p + I.h_uptr[l], cnt,
-                               rp, col, I.diag.p, lu, y, sc);
-        }
-        MHFEM_HIP(hipGetLastError());
-    }
     // The assembled MHFEM rows are scaled so that every diagonal entry is
     // exactly 1 (LinearSystem.h:196-201, BoundaryConditions.hpp:200-203,
     // :375-377): the diagonal preconditioner is then the identity and its
     // scaling vector is not read.
     bool jacobi_scales() const { return c.cfg.preconditioner == MHFEM_PREC_DIAGONAL && !c.unit_diagonal; }
 
+    // ---- preconditioner apply y = M^{-1} x (T = double or float)
     template <typename T>
-    void prec(const T *x, T *y, const T *dinv)
+    void ilu_apply(const T *x, T *y, bool skip_when_halted)
     {
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) ilu_apply<T>(x, y);
-        else hipLaunchKernelGGL((k_scale<T>), g, KB, 0, c.stream, n, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
+        Ilu0 &I = k.ilu;
+        const T *lu;
+        if constexpr (sizeof(T) == 8) lu = I.lu.p;
+        else lu = I.lu32.p;
+        const double *flag = (skip_when_halted && k.nseg == 1) ? k.sc.p : nullptr;
+        const int *rp = c.rowptr.p, *col = c.csr_col.p;
+        for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
+            const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
+            hipLaunchKernelGGL((k_ilu_lower<T>), (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
+                               rp, col, I.diag.p, lu, x, y, flag);
+        }
+        for (size_t l = 0; l + 1 < I.h_uptr.size(); l++) {
+            const int cnt = I.h_uptr[l + 1] - I.h_uptr[l];
+            hipLaunchKernelGGL((k_ilu_upper<T>), (cnt + KB - 1) / KB, KB, 0, c.stream, I.urows.p + I.h_uptr[l], cnt,
+                               rp, col, I.diag.p, lu, y, flag);
+        }
+        MHFEM_HIP(hipGetLastError());
+    }
+    template <typename T>
+    void prec(const T *x, T *y, const T *dinv, int mode)
+    {
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) ilu_apply<T>(x, y, false);
+        else hipLaunchKernelGGL((k_sscale<T>), k.nblk, KB, 0, c.stream, sg, mode, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
         MHFEM_HIP(hipGetLastError());
     }
 
-    // y = M^{-1} A x; ND fused dots: 1 -> (y, z), 2 -> (y, z), (y, y); then `fin`
+    // y = M^{-1} A x on the active segments; ND fused dots: 1 -> (y, z),
+    // 2 -> (y, z), (y, y); then `fin`
     template <typename T, int ND>
     void pmatvec(const T *x, T *y, T *tmp, const T *dinv, const T *z, int fin)
     {
         SellView<T> A = matrix<T>();
-        const int pc = c.cfg.preconditioner;
-        if (pc == MHFEM_PREC_ILU0) {
-            hipLaunchKernelGGL((k_spmv<T, false, 0>), g, KB, 0, c.stream, A, x, (const T *)nullptr, tmp,
-                               (const T *)nullptr, red, 0);
-            ilu_apply<T>(tmp, y, k.sc.p);
-            if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, true);
-            if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, true);
-        } else if (jacobi_scales()) {
-            hipLaunchKernelGGL((k_spmv<T, true, ND>), g, KB, 0, c.stream, A, x, dinv, y, z, red, fin);
-            if constexpr (ND > 0) finish<ND>(fin, true);
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
+            hipLaunchKernelGGL((k_sspmv<T, false, 0>), k.nblk, KB, 0, c.stream, A, sg, (int)M_ACTIVE, x,
+                               (const T *)nullptr, tmp, (const T *)nullptr);
+            ilu_apply<T>(tmp, y, true);
+            if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, M_ACTIVE);
+            if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, M_ACTIVE);
         } else {
-            hipLaunchKernelGGL((k_spmv<T, false, ND>), g, KB, 0, c.stream, A, x, dinv, y, z, red, fin);
-            if constexpr (ND > 0) finish<ND>(fin, true);
+            if (jacobi_scales())
+                hipLaunchKernelGGL((k_sspmv<T, true, ND>), k.nblk, KB, 0, c.stream, A, sg, (int)M_ACTIVE, x, dinv, y, z);
+            else
+                hipLaunchKernelGGL((k_sspmv<T, false, ND>), k.nblk, KB, 0, c.stream, A, sg, (int)M_ACTIVE, x, dinv, y, z);
+            if constexpr (ND > 0) finish<ND>(fin, M_ACTIVE);
         }
         MHFEM_HIP(hipGetLastError());
     }
-    // r = M^{-1}(b - A x)
+    // r = M^{-1}(b - A x) on the segments selected by mode.  ILU(0) works on
+    // every row, so for a partial selection it writes into `scratch` and the
+    // selected rows are copied to r.
     template <typename T>
-    void presidue(const T *x, const T *b, T *r, T *tmp, const T *dinv)
+    void presidue(const T *x, const T *b, T *r, T *tmp, const T *dinv, int mode, T *scratch = nullptr)
     {
-        hipLaunchKernelGGL((k_residual<T>), g, KB, 0, c.stream, matrix<T>(), x, b, tmp);
+        hipLaunchKernelGGL((k_sresidual<T>), k.nblk, KB, 0, c.stream, matrix<T>(), sg, mode, x, b, tmp);
         MHFEM_HIP(hipGetLastError());
-        prec<T>(tmp, r, dinv);
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU0 && mode != M_ALL) {
+            if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0) needs a scratch vector"};
+            ilu_apply<T>(tmp, scratch, false);
+            hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, (const T *)scratch, r, (T *)nullptr);
+            MHFEM_HIP(hipGetLastError());
+            return;
+        }
+        prec<T>(tmp, r, dinv, mode);
     }
 
     struct Vecs {
         void *r, *r_ast, *p, *Ap, *s, *As, *tmp, *tmp2, *dinv;
     };
 
-    // BiCGStab on vectors of type T (left preconditioned, restart on rho
-    // breakdown, stop when ||r_k|| / ||M^{-1} b|| < tol).
+    // ---- BiCGStab pieces (left preconditioned, restart on rho breakdown,
+    // stop when ||r_k|| / ||M^{-1} b|| < tol), shared by the eager solve and
+    // the captured step graphs
+    template <typename T>
+    void bicg_start(const T *b, T *x, const Vecs &v, double tol, int maxit)
+    {
+        set_scalars(tol, maxit);
+        prec<T>(b, (T *)v.tmp2, (const T *)v.dinv, M_ALL);
+        dots<T, 1>(DotArgs<T, 1>{{(T *)v.tmp2}, {(T *)v.tmp2}}, FIN_BNORM, M_ALL);
+        bicg_init<T>(b, x, v, M_ALL);
+    }
+    template <typename T>
+    void bicg_init(const T *b, T *x, const Vecs &v, int mode)
+    {
+        T *r = (T *)v.r;
+        presidue<T>(x, b, r, (T *)v.tmp, (const T *)v.dinv, mode, (T *)v.tmp2);
+        hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, r, (T *)v.p, (T *)v.r_ast);
+        MHFEM_HIP(hipGetLastError());
+        dots<T, 1>(DotArgs<T, 1>{{r}, {(T *)v.r_ast}}, FIN_INIT, mode);
+    }
+    template <typename T>
+    void bicg_iterations(T *x, const Vecs &v, int count)
+    {
+        T *r = (T *)v.r, *r_ast = (T *)v.r_ast, *p = (T *)v.p, *Ap = (T *)v.Ap, *s = (T *)v.s, *As = (T *)v.As,
+          *tmp = (T *)v.tmp;
+        const T *dinv = (const T *)v.dinv;
+        for (int q = 0; q < count; q++) {
+            pmatvec<T, 1>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA);
+            hipLaunchKernelGGL((k_bicg_s<T>), k.nblk, KB, 0, c.stream, sg, r, Ap, s);
+            pmatvec<T, 2>(s, As, tmp, dinv, s, FIN_OMEGA);
+            hipLaunchKernelGGL((k_bicg_xr<T>), k.nblk, KB, 0, c.stream, sg, x, p, s, As, r, r_ast);
+            finish<2>(FIN_UPDATE, M_ACTIVE);
+            hipLaunchKernelGGL((k_bicg_p<T>), k.nblk, KB, 0, c.stream, sg, r, p, Ap);
+            MHFEM_HIP(hipGetLastError());
+        }
+    }
+    // eager solve; returns the largest iteration count over the segments
     template <typename T>
     int bicgstab(const T *b, T *x, const Vecs &v, double tol, int maxit)
     {
-        T *r = (T *)v.r, *r_ast = (T *)v.r_ast, *p = (T *)v.p, *Ap = (T *)v.Ap, *s = (T *)v.s, *As = (T *)v.As,
-          *tmp = (T *)v.tmp, *tmp2 = (T *)v.tmp2;
-        const T *dinv = (const T *)v.dinv;
-        set_scalars(tol, maxit);
-        prec<T>(b, tmp2, dinv);
-        dots<T, 1>(DotArgs<T, 1>{{tmp2}, {tmp2}}, FIN_BNORM, false);
-        auto init = [&]() {
-            presidue<T>(x, b, r, tmp, dinv);
-            hipLaunchKernelGGL((k_copy2<T>), g, KB, 0, c.stream, n, r, p, r_ast);
-            MHFEM_HIP(hipGetLastError());
-            dots<T, 1>(DotArgs<T, 1>{{r}, {r_ast}}, FIN_INIT, false);
-        };
-        init();
-        const int every = std::max(1, c.cfg.check_every);
+        bicg_start<T>(b, x, v, tol, maxit);
+        return bicg_drive<T>(b, x, v, [&]() { bicg_iterations<T>(x, v, std::max(1, c.cfg.check_every)); });
+    }
+    // poll the flags, restart segments that need it, queue more iterations
+    template <typename T, typename More>
+    int bicg_drive(const T *b, T *x, const Vecs &v, More &&more)
+    {
         for (;;) {
             poll();
-            if (k.h_sc[S_DONE] != 0.0) break;
-            if (k.h_sc[S_RESTART] != 0.0) {
-                init();
+            if (all_done()) break;
+            if (any_restart()) {
+                bicg_init<T>(b, x, v, M_RESTART);
                 continue;
             }
-            for (int q = 0; q < every; q++) {
-                pmatvec<T, 1>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA);
-                hipLaunchKernelGGL((k_bicg_s<T>), g, KB, 0, c.stream, n, r, Ap, s, k.sc.p);
-                pmatvec<T, 2>(s, As, tmp, dinv, s, FIN_OMEGA);
-                hipLaunchKernelGGL((k_bicg_xr<T>), g, KB, 0, c.stream, n, x, p, s, As, r, r_ast, red);
-                finish<2>(FIN_UPDATE, true);
-                hipLaunchKernelGGL((k_bicg_p<T>), g, KB, 0, c.stream, n, r, p, Ap, k.sc.p);
-                MHFEM_HIP(hipGetLastError());
-            }
+            more();
         }
-        return (int)k.h_sc[S_IT];
+        int it = 0;
+        for (int s = 0; s < k.nseg; s++) it = std::max(it, (int)seg_sc(s)[S_IT]);
+        return it;
     }
 
-    // GMRES(m), FP64 only
+    // GMRES(m), FP64, one segment
     int gmres(const double *b, double *x, double tol, int maxit, int m, double *resid_out);
 };
 
@@ -637,11 +780,10 @@ SellView<float> Solve::matrix<float>()
     return SellView<float>{c.rows, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, k.val32.p};
 }
 
-// w -= sum_k h[k] V_k  and  h2[k] = (V_k, w) for k <= j  (classical Gram-Schmidt pieces)
+// ---- GMRES pieces: partial dots of w with V_0..V_{j1-1}, one row of partials per vector
 __global__ void __launch_bounds__(KB) k_gs_dots(int n, int j1, const double *__restrict__ V, const double *__restrict__ w,
                                                 double *__restrict__ partials)
 {
-    // partial dots of w with V_0..V_{j1-1}, one row of partials per vector
     __shared__ double sm[KB / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int kk = 0; kk < j1; kk++) {
@@ -662,7 +804,6 @@ __global__ void __launch_bounds__(KB) k_gs_dots(int n, int j1, const double *__r
 __global__ void __launch_bounds__(KB) k_reduce_rows(int rows, int cols, const double *__restrict__ partials,
                                                     double *__restrict__ out)
 {
-    // one work-group per row of partials
     __shared__ double sm[KB / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int rr = blockIdx.x;
@@ -718,16 +859,16 @@ int Solve::gmres(const double *b, double *x, double tol, int maxit, int m, doubl
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), h(m + 2), h2(m + 2);
     double *hy = hbuf.p + (m + 2);
     auto norm2 = [&](const double *v) {
-        dots<double, 1>(DotArgs<double, 1>{{v}, {v}}, FIN_NONE, false);
+        dots<double, 1>(DotArgs<double, 1>{{v}, {v}}, FIN_NONE, M_ALL);
         poll();
         return std::sqrt(k.h_sc[S_D0]);
     };
     set_scalars(tol, maxit);
-    prec<double>(b, w, dinv);
+    prec<double>(b, w, dinv, M_ALL);
     double b_norm = norm2(w);
     if (b_norm == 0.0) b_norm = 1.0;
     int it = 0;
-    presidue<double>(x, b, V, tmp, dinv);
+    presidue<double>(x, b, V, tmp, dinv, M_ALL);
     double beta = norm2(V);
     double resid = beta / b_norm;
     const unsigned gg = std::min<unsigned>(g, 1024);
@@ -780,7 +921,7 @@ int Solve::gmres(const double *b, double *x, double tol, int maxit, int m, doubl
         MHFEM_HIP(hipMemcpyAsync(hy, y.data(), j * sizeof(double), hipMemcpyHostToDevice, c.stream));
         hipLaunchKernelGGL(k_combine, g, KB, 0, c.stream, n, j, V, hy, x);
         MHFEM_HIP(hipGetLastError());
-        presidue<double>(x, b, V, tmp, dinv);
+        presidue<double>(x, b, V, tmp, dinv, M_ALL);
         beta = norm2(V);
         resid = beta / b_norm;
     }
@@ -794,7 +935,7 @@ int Solve::gmres(const double *b, double *x, double tol, int maxit, int m, doubl
 // ---------------------------------------------------------------------------
 // Preconditioner update (Solver.hpp:1237-1242): every step, from the freshly
 // assembled matrix.
-static void precond_update(Ctx &c, Krylov &k, Solve &S)
+static void precond_update(Ctx &c, Krylov &k)
 {
     const int n = k.n;
     const unsigned g = grid_for(n);
@@ -820,22 +961,16 @@ static void precond_update(Ctx &c, Krylov &k, Solve &S)
             hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, I.lu.p, I.lu32.p);
         MHFEM_HIP(hipGetLastError());
     }
-    (void)S;
 }
 
 // ---------------------------------------------------------------------------
-// FP64 helpers for callers outside the Krylov loop (learned initial guess):
-// no convergence flag is consulted.
-__global__ void __launch_bounds__(KB) k_spmv_plain(SellView<double> A, const double *__restrict__ x, double *__restrict__ y)
-{
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB) y[row] = sell_row(A, x, row);
-}
-
+// FP64 helpers for callers outside the Krylov loop (learned initial guess,
+// benchmarks): no convergence flag is consulted.
 void spmv64(Ctx &c, const double *x, double *y)
 {
     Krylov &k = *c.krylov;
     Solve S(c, k);
-    hipLaunchKernelGGL(k_spmv_plain, grid_for(k.n), KB, 0, c.stream, S.matrix<double>(), x, y);
+    hipLaunchKernelGGL((k_spmv_plain<double>), grid_for(k.n), KB, 0, c.stream, S.matrix<double>(), x, y);
     MHFEM_HIP(hipGetLastError());
 }
 
@@ -850,8 +985,9 @@ void residual64(Ctx &c, const double *x, const double *b, double *r)
 double dot64(Ctx &c, const double *x, const double *y)
 {
     Krylov &k = *c.krylov;
+    if (k.nseg != 1) throw Error{MHFEM_E_STATE, "dot64: single-segment systems only"};
     Solve S(c, k);
-    S.dots<double, 1>(DotArgs<double, 1>{{x}, {y}}, FIN_NONE, false);
+    S.dots<double, 1>(DotArgs<double, 1>{{x}, {y}}, FIN_NONE, M_ALL);
     S.poll();
     return k.h_sc[S_D0];
 }
@@ -864,8 +1000,6 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
     Solve S(c, k);
     const unsigned g = grid_for(k.n);
     hipEvent_t e0 = c.ev[12], e1 = c.ev[13];
-    const double zero[NSLOTS] = {};
-    MHFEM_HIP(hipMemcpyAsync(k.sc.p, zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
     if (precision == MHFEM_PRECISION_FP32_REFINED) {
         if (!k.val32.p) throw Error{MHFEM_E_STATE, "FP32 matrix not configured"};
         hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)k.n, c.arr[MHFEM_Z_iF].p, k.x32.p);
@@ -873,12 +1007,10 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
     for (int rep = -2; rep < reps; rep++) {
         if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
         if (precision == MHFEM_PRECISION_FP32_REFINED)
-            hipLaunchKernelGGL((k_spmv<float, false, 0>), g, KB, 0, c.stream, S.matrix<float>(), k.x32.p,
-                               (const float *)nullptr, k.tmp32.p, (const float *)nullptr, S.red, 0);
+            hipLaunchKernelGGL((k_spmv_plain<float>), g, KB, 0, c.stream, S.matrix<float>(), k.x32.p, k.tmp32.p);
         else
-            hipLaunchKernelGGL((k_spmv<double, false, 0>), g, KB, 0, c.stream, S.matrix<double>(),
-                               c.arr[MHFEM_Z_iF].p, (const double *)nullptr, k.tmp.p, (const double *)nullptr,
-                               S.red, 0);
+            hipLaunchKernelGGL((k_spmv_plain<double>), g, KB, 0, c.stream, S.matrix<double>(), c.arr[MHFEM_Z_iF].p,
+                               k.tmp.p);
     }
     MHFEM_HIP(hipGetLastError());
     MHFEM_HIP(hipEventRecord(e1, c.stream));
@@ -886,6 +1018,43 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
     float ms = 0;
     MHFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
     *avg_ms = reps > 0 ? ms / reps : 0.0;
+}
+
+// Autotuner: rows per work-group of the fused iteration kernels.  Times the
+// fused SpMV + dot kernel of the iteration for each candidate on the
+// assembled pattern and keeps the fastest.
+int krylov_autotune(Ctx &c, double *best_ms)
+{
+    Krylov &k = *c.krylov;
+    const int cands[] = {256, 512, 1024, 2048, 4096, 8192};
+    hipEvent_t e0 = c.ev[12], e1 = c.ev[13];
+    int best = k.rows_per_block;
+    float bestms = 1e30f;
+    for (int rpb : cands) {
+        build_blocks(c, k, rpb);
+        Solve S(c, k);
+        S.set_scalars(0.0, 1);
+        const double *x = c.arr[MHFEM_Z_iF].p;
+        for (int rep = -2; rep < 10; rep++) {
+            if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
+            hipLaunchKernelGGL((k_sspmv<double, false, 2>), k.nblk, KB, 0, c.stream, S.matrix<double>(), S.sg,
+                               (int)M_ALL, x, (const double *)nullptr, k.tmp.p, x);
+        }
+        MHFEM_HIP(hipGetLastError());
+        MHFEM_HIP(hipEventRecord(e1, c.stream));
+        MHFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MHFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 10;
+        if (ms < bestms) {
+            bestms = ms;
+            best = rpb;
+        }
+    }
+    build_blocks(c, k, best);
+    drop_graphs(k);
+    if (best_ms) *best_ms = bestms;
+    return best;
 }
 
 void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
@@ -896,12 +1065,14 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
     const unsigned g = grid_for(n);
     hipEvent_t e0 = c.ev[3], e1 = c.ev[4], e2 = c.ev[5];
     if (c.timing) MHFEM_HIP(hipEventRecord(e0, c.stream));
-    precond_update(c, k, S);
+    precond_update(c, k);
     if (c.timing) MHFEM_HIP(hipEventRecord(e1, c.stream));
     const double tol = c.cfg.convergence_residue;
     const int maxit = c.cfg.max_iterations;
     mhfem_solve_stats out{};
     if (c.guess && c.guess_on_solve) guess_before_solve(c, *c.guess, b, x);
+    if (k.nseg > 1 && (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64))
+        throw Error{MHFEM_E_ARGUMENT, "batched systems are solved with FP64 BiCGStab"};
     if (c.cfg.precision == MHFEM_PRECISION_FP64) {
         if (c.cfg.solver == MHFEM_SOLVER_GMRES) {
             double res = 0;
@@ -911,8 +1082,12 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         } else {
             Solve::Vecs v{k.r.p, k.r_ast.p, k.p.p, k.Ap.p, k.s.p, k.As.p, k.tmp.p, k.tmp2.p, k.dinv.p};
             out.iterations = S.bicgstab<double>(b, x, v, tol, maxit);
-            out.residue = k.h_sc[S_RESID];
-            out.converged = out.residue < tol;
+            out.residue = 0;
+            out.converged = 1;
+            for (int s = 0; s < k.nseg; s++) {
+                out.residue = std::max(out.residue, S.seg_sc(s)[S_RESID]);
+                out.converged &= S.seg_sc(s)[S_RESID] < tol;
+            }
         }
         out.outer_iterations = 1;
     } else {
@@ -922,8 +1097,8 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         float *rhs32 = k.rhs32.p, *z = k.b32.p;
         Solve::Vecs v{k.r32.p, k.r_ast32.p, k.p32.p, k.Ap32.p, k.s32.p, k.As32.p, k.tmp32.p, k.b32.p, k.dinv32.p};
         hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)n, b, rhs32);
-        S.prec<float>(rhs32, z, k.dinv32.p);
-        S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, false);
+        S.prec<float>(rhs32, z, k.dinv32.p, M_ALL);
+        S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, M_ALL);
         S.poll();
         double b_norm = std::sqrt(k.h_sc[S_D0]);
         if (b_norm == 0.0) b_norm = 1.0;
@@ -932,8 +1107,8 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         double res = 0;
         for (;;) {
             hipLaunchKernelGGL(k_residual_mixed, g, KB, 0, c.stream, S.matrix<double>(), x, b, rhs32);
-            S.prec<float>(rhs32, z, k.dinv32.p);
-            S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, false);
+            S.prec<float>(rhs32, z, k.dinv32.p, M_ALL);
+            S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, M_ALL);
             S.poll();
             res = std::sqrt(k.h_sc[S_D0]) / b_norm;
             if (!(res >= tol) || total >= maxit || outer >= 100) break;
@@ -958,6 +1133,74 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         MHFEM_HIP(hipEventElapsedTime(&t2, e1, e2));
         c.timers.preconditioner += t1;
         c.timers.linear_solver += t2;
+    }
+    *st = out;
+}
+
+// per-segment results of the last BiCGStab solve
+void krylov_segment_stats(Ctx &c, int *iterations, double *residues, int count)
+{
+    Krylov &k = *c.krylov;
+    for (int s = 0; s < std::min(count, k.nseg); s++) {
+        if (iterations) iterations[s] = (int)k.h_sc[(long)s * NSLOTS + S_IT];
+        if (residues) residues[s] = k.h_sc[(long)s * NSLOTS + S_RESID];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One time step as captured HIP graphs (FP64 BiCGStab):
+//   first = preIterate + assembly + preconditioner update + BiCGStab start
+//           + `check_every` iterations,
+//   more  = `check_every` further iterations (replayed until every segment
+//           has converged),
+//   post  = postIterate.
+// Scalars are initialised by a kernel and converged segments turn the queued
+// work into no-ops, so replaying the same graphs is exact.
+void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
+{
+    Krylov &k = *c.krylov;
+    if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64)
+        throw Error{MHFEM_E_ARGUMENT, "graph steps use FP64 BiCGStab"};
+    const int every = std::max(1, c.cfg.check_every);
+    if (k.g_first && (k.g_tau != tau || k.g_iters != every)) drop_graphs(k);
+    Solve S(c, k);
+    Solve::Vecs v{k.r.p, k.r_ast.p, k.p.p, k.Ap.p, k.s.p, k.As.p, k.tmp.p, k.tmp2.p, k.dinv.p};
+    const double *b = c.arr[MHFEM_RHS].p;
+    double *x = c.arr[MHFEM_Z_iF].p;
+    auto capture = [&](auto &&body, hipGraphExec_t *out) {
+        const bool timing = c.timing;
+        c.timing = false;  // no event records inside a captured graph
+        hipGraph_t graph;
+        MHFEM_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        body();
+        MHFEM_HIP(hipStreamEndCapture(c.stream, &graph));
+        MHFEM_HIP(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
+        MHFEM_HIP(hipGraphDestroy(graph));
+        c.timing = timing;
+    };
+    if (!k.g_first) {
+        capture([&]() {
+            launch_pre_iterate(c, time, tau);
+            launch_assemble(c, time, tau);
+            precond_update(c, k);
+            S.bicg_start<double>(b, x, v, c.cfg.convergence_residue, c.cfg.max_iterations);
+            S.bicg_iterations<double>(x, v, every);
+        }, &k.g_first);
+        capture([&]() { S.bicg_iterations<double>(x, v, every); }, &k.g_more);
+        capture([&]() { launch_post_iterate(c); }, &k.g_post);
+        k.g_tau = tau;
+        k.g_iters = every;
+    }
+    MHFEM_HIP(hipGraphLaunch(k.g_first, c.stream));
+    const int it = S.bicg_drive<double>(b, x, v, [&]() { MHFEM_HIP(hipGraphLaunch(k.g_more, c.stream)); });
+    MHFEM_HIP(hipGraphLaunch(k.g_post, c.stream));
+    mhfem_solve_stats out{};
+    out.iterations = it;
+    out.outer_iterations = 1;
+    out.converged = 1;
+    for (int s = 0; s < k.nseg; s++) {
+        out.residue = std::max(out.residue, S.seg_sc(s)[S_RESID]);
+        out.converged &= S.seg_sc(s)[S_RESID] < c.cfg.convergence_residue;
     }
     *st = out;
 }

@@ -1,0 +1,108 @@
+// heat_equation.cpp -- the reference's HeatEquation example
+// (examples/HeatEquation/HeatEquationModel.h, config.ini) on the MI355X path:
+// 2-D heat equation on the unit square, s x s squares split into triangles,
+// zero Dirichlet boundary, Gaussian initial condition, BiCGStab + diagonal.
+//
+//   heat_equation [out.bin] [s] [steps]
+// writes the final cell values Z_iK (float64) to out.bin when given.
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+#include <vector>
+
+#include "mhfem/control.hpp"
+
+using namespace mhfem;
+
+// HeatEquationModel (HeatEquationModel.h:34-235): uniform coefficients, constant mobility
+class HeatEquationModel : public BaseModel<1, MassLumping::disabled> {
+public:
+    static constexpr bool do_mobility_upwind = false;
+    std::vector<double> lastSnapshot;
+
+    bool init(const ParameterContainer &p) override
+    {
+        const double diffusivity = p.getParameter<double>("diffusivity", 1.0);
+        setCoefficient(MHFEM_N_ijK, 1.0);
+        setCoefficient(MHFEM_u_ijKe, 0.0);
+        setCoefficient(MHFEM_m_iK, 1.0);
+        setCoefficient(MHFEM_D_ijK, diffusivity);
+        setCoefficient(MHFEM_w_iKe, 0.0);
+        setCoefficient(MHFEM_a_ijKe, 0.0);
+        setCoefficient(MHFEM_r_ijK, 0.0);
+        setCoefficient(MHFEM_f_iK, 0.0);
+        return true;
+    }
+    void makeSnapshot(double time, long step, const std::vector<double> &Z) const override
+    {
+        double mx = 0;
+        for (double z : Z) mx = std::max(mx, z);
+        std::printf("snapshot %ld  t = %.4e  max Z = %.10e\n", step, time, mx);
+        const_cast<HeatEquationModel *>(this)->lastSnapshot = Z;
+    }
+};
+
+static SimplexMesh unit_square(int s)
+{
+    SimplexMesh m;
+    m.dim = 2;
+    const int n1 = s + 1;
+    for (int iy = 0; iy < n1; iy++)
+        for (int ix = 0; ix < n1; ix++) {
+            m.points.push_back((double)ix / s);
+            m.points.push_back((double)iy / s);
+        }
+    for (int qy = 0; qy < s; qy++)
+        for (int qx = 0; qx < s; qx++) {
+            const int v00 = qy * n1 + qx, v10 = v00 + 1, v01 = v00 + n1, v11 = v01 + 1;
+            for (int v : {v00, v10, v11, v00, v11, v01}) m.cells.push_back(v);
+        }
+    return m;
+}
+
+int main(int argc, char **argv)
+{
+    const int s = argc > 2 ? std::atoi(argv[2]) : 64;
+    const int steps = argc > 3 ? std::atoi(argv[3]) : 100;
+    ParameterContainer p;
+    p.setParameter("linear-solver", std::string("bicgstab"));
+    p.setParameter("preconditioner", std::string("diagonal"));
+    p.setParameter("convergence-residue", 1e-10);
+    p.setParameter("time-step", 1e-4);
+    p.setParameter("initial-time", 0.0);
+    p.setParameter("final-time", 1e-4 * steps);
+    p.setParameter("snapshot-period", 1e-4 * steps);
+    p.setParameter("diffusivity", 1.0);
+    try {
+        const SimplexMesh mesh = unit_square(s);
+        Solver<HeatEquationModel> problem;
+        problem.setMesh(mesh);
+        // initial condition at cell centroids
+        std::vector<double> z0(mesh.numCells());
+        for (int K = 0; K < mesh.numCells(); K++) {
+            double cx = 0, cy = 0;
+            for (int k = 0; k < 3; k++) {
+                cx += mesh.points[2 * mesh.cells[3 * K + k]] / 3;
+                cy += mesh.points[2 * mesh.cells[3 * K + k] + 1] / 3;
+            }
+            z0[K] = std::exp(-((cx - 0.5) * (cx - 0.5) + (cy - 0.5) * (cy - 0.5)) / 0.01);
+        }
+        problem.getMeshDependentData().setInitialCondition(0, z0);
+        if (!problem.setup(p)) throw std::runtime_error("Failed to set up the MHFEM solver.");
+        if (!problem.setInitialCondition(p)) throw std::runtime_error("Failed to set up the initial condition.");
+        problem.setupLinearSystem();
+        problem.enableTimers();
+        solve(problem, p);
+        problem.writeEpilog(std::cout);
+        if (argc > 1) {
+            const auto &Z = problem.getMeshDependentData().lastSnapshot;
+            FILE *f = std::fopen(argv[1], "wb");
+            if (!f || std::fwrite(Z.data(), sizeof(double), Z.size(), f) != Z.size()) return 3;
+            std::fclose(f);
+        }
+    } catch (const std::exception &e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

@@ -40,7 +40,7 @@ class SolverConfig(C.Structure):
     _fields_ = [("solver", C.c_int), ("preconditioner", C.c_int),
                 ("convergence_residue", C.c_double), ("max_iterations", C.c_int),
                 ("gmres_restart", C.c_int), ("precision", C.c_int),
-                ("inner_residue", C.c_double), ("check_every", C.c_int)]
+                ("inner_residue", C.c_double), ("check_every", C.c_int), ("rows_per_block", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -91,6 +91,11 @@ SIGNATURES = {
     "mhfem_export_system": (_I, [_P, _P, _P, _P, _P]),
     "mhfem_solve_rhs": (_I, [_P, _P, _P, C.POINTER(SolveStats)]),
     "mhfem_benchmark_spmv": (_I, [_P, _I, _I, C.POINTER(_D)]),
+    "mhfem_step": (_I, [_P, _D, _D, C.POINTER(SolveStats)]),
+    "mhfem_autotune": (_I, [_P, C.POINTER(_I), C.POINTER(_D)]),
+    "mhfem_create_batch": (_P, [C.POINTER(MeshDesc), _I, C.POINTER(ModelDesc)]),
+    "mhfem_batch_info": (_I, [_P, C.POINTER(_I), _P, _P]),
+    "mhfem_batch_stats": (_I, [_P, _P, _P, _I]),
     "mhfem_guess_configure": (_I, [_P, C.POINTER(GuessConfig)]),
     "mhfem_guess_stats_last": (_I, [_P, C.POINTER(GuessStats)]),
     "mhfem_guess_num_params": (_I, []),
@@ -143,16 +148,25 @@ class Solver:
 
     def __init__(self, problem, solver=None, preconditioner=None, tol=None, precision="fp64",
                  inner_tol=1e-4, check_every=8, max_iterations=100000, gmres_restart=30,
-                 setup=True):
+                 setup=True, rows_per_block=0):
         pr = problem
         self.problem = pr
-        self._points = np.ascontiguousarray(pr.points, np.float64)
-        self._cells = np.ascontiguousarray(pr.cells, np.int32)
-        mesh = MeshDesc(pr.dim, self._points.shape[0], self._cells.shape[0],
-                        _ptr(self._points), _ptr(self._cells))
         model = ModelDesc(pr.n, pr.lumping, pr.advection, pr.do_mobility_upwind)
         L = lib()
-        self.ctx = L.mhfem_create(C.byref(mesh), C.byref(model))
+        instances = pr.extra.get("instances") if pr.extra else None
+        if instances:
+            # batched ensemble: one segment per instance (mhfem_create_batch)
+            self._keep = [(np.ascontiguousarray(p_, np.float64), np.ascontiguousarray(c_, np.int32))
+                          for p_, c_ in instances]
+            descs = (MeshDesc * len(instances))(*[MeshDesc(pr.dim, p_.shape[0], c_.shape[0], _ptr(p_), _ptr(c_))
+                                                  for p_, c_ in self._keep])
+            self.ctx = L.mhfem_create_batch(descs, len(instances), C.byref(model))
+        else:
+            self._points = np.ascontiguousarray(pr.points, np.float64)
+            self._cells = np.ascontiguousarray(pr.cells, np.int32)
+            mesh = MeshDesc(pr.dim, self._points.shape[0], self._cells.shape[0],
+                            _ptr(self._points), _ptr(self._cells))
+            self.ctx = L.mhfem_create(C.byref(mesh), C.byref(model))
         if not self.ctx:
             raise MHFEMError(-1, L.mhfem_last_error(None).decode())
         c, f, b = C.c_int(), C.c_int(), C.c_int()
@@ -163,7 +177,7 @@ class Solver:
         self.config = SolverConfig(SOLVERS[solver or pr.solver],
                                    PRECONDITIONERS[preconditioner or pr.preconditioner],
                                    pr.tol if tol is None else tol, max_iterations, gmres_restart,
-                                   PRECISIONS[precision], inner_tol, check_every)
+                                   PRECISIONS[precision], inner_tol, check_every, rows_per_block)
         if setup:
             self.setInitialCondition()
             self.setupLinearSystem()
@@ -243,6 +257,37 @@ class Solver:
         st = self.solveLinearSystem()
         self.postIterate(t, tau)
         return st
+
+    def step_graph(self, t, tau):
+        """one time step replayed from captured HIP graphs (mhfem_step)"""
+        if self.problem.dirichlet_at is not None:
+            self.set_array("bc_dirichlet", self.problem.dirichlet_at(t + tau))
+        st = SolveStats()
+        self._check(lib().mhfem_step(self.ctx, t, tau, C.byref(st)))
+        self.iterations.append(st.iterations)
+        self.last_stats = st
+        return st
+
+    def autotune(self):
+        rpb, ms = C.c_int(), C.c_double()
+        self._check(lib().mhfem_autotune(self.ctx, C.byref(rpb), C.byref(ms)))
+        return rpb.value, ms.value
+
+    def batch_info(self):
+        cnt = C.c_int()
+        self._check(lib().mhfem_batch_info(self.ctx, C.byref(cnt), None, None))
+        co = np.empty(cnt.value + 1, np.int32)
+        fo = np.empty(cnt.value + 1, np.int32)
+        self._check(lib().mhfem_batch_info(self.ctx, C.byref(cnt), _ptr(co), _ptr(fo)))
+        return co, fo
+
+    def batch_stats(self):
+        co, _ = self.batch_info()
+        k = co.size - 1
+        its = np.empty(k, np.int32)
+        res = np.empty(k, np.float64)
+        self._check(lib().mhfem_batch_stats(self.ctx, _ptr(its), _ptr(res), k))
+        return its, res
 
     def solve(self, t0, t1, dt):
         """control.h::solve(problem, t0, t1, dt) (control.h:64-102)."""

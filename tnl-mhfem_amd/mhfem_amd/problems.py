@@ -235,3 +235,53 @@ def ensemble_member(s: int, seed: int, dt: float = 1e-4, steps: int = 50) -> Pro
                    bc_dirichlet=np.zeros(F), Z_iK0=z0, dt=dt, steps=steps,
                    solver="bicgstab", preconditioner="diagonal", do_mobility_upwind=0,
                    bc_mobility=np.ones(F), cell_faces=cf, face_cells=fc, extra=dict(seed=seed))
+
+
+def concatenate(members):
+    """One problem over the disjoint union of the members' meshes (cells and
+    faces numbered member by member), for a batched context: every array is
+    the concatenation of the members' arrays along the cell (or face) index,
+    which is the fastest index of every layout."""
+    n = members[0].n
+    fpc = members[0].fpc
+    assert all(m.n == n and m.fpc == fpc for m in members)
+    pts, cells, voff = [], [], 0
+    for m in members:
+        pts.append(m.points)
+        cells.append(m.cells + voff)
+        voff += m.points.shape[0]
+    points = np.concatenate(pts)
+    cells = np.concatenate(cells).astype(np.int32)
+    cf, fc = meshgen.face_numbering(cells, points.shape[0])
+    lead = {"N_ijK": (n, n), "D_ijK": (n, n), "r_ijK": (n, n), "u_ijKe": (n, n, fpc), "a_ijKe": (n, n, fpc),
+            "m_iK": (n,), "f_iK": (n,), "w_iKe": (n, fpc)}
+    coeffs = {}
+    for name, shape in lead.items():
+        vals = [np.asarray(m.coeffs[name], np.float64) for m in members]
+        if all(v.ndim == 0 for v in vals) and all(float(v) == float(vals[0]) for v in vals):
+            coeffs[name] = np.array(float(vals[0]))
+        else:
+            parts = [m.coefficient_array(name).reshape(shape + (m.num_cells,)) for m in members]
+            coeffs[name] = np.concatenate(parts, axis=-1).ravel()
+
+    def cat_faces(attr, dtype):
+        return np.concatenate([np.asarray(getattr(m, attr), dtype).reshape(n, m.num_faces) for m in members],
+                              axis=-1).ravel()
+
+    z0 = np.concatenate([np.asarray(m.Z_iK0, np.float64).reshape(n, m.num_cells) for m in members], axis=-1).ravel()
+    first = members[0]
+    return Problem(f"batch_of_{len(members)}", points, cells, n, coeffs, tags=cat_faces("tags", np.uint8),
+                   bc_values=cat_faces("bc_values", np.float64), bc_dirichlet=cat_faces("bc_dirichlet", np.float64),
+                   Z_iK0=z0, dt=first.dt, steps=first.steps, solver=first.solver, preconditioner=first.preconditioner,
+                   tol=first.tol, lumping=first.lumping, advection=first.advection,
+                   do_mobility_upwind=first.do_mobility_upwind, bc_mobility=cat_faces("bc_mobility", np.float64),
+                   cell_faces=cf, face_cells=fc,
+                   extra=dict(instances=[(m.points, m.cells) for m in members],
+                              cell_offsets=np.cumsum([0] + [m.num_cells for m in members]),
+                              face_offsets=np.cumsum([0] + [m.num_faces for m in members])))
+
+
+def ensemble(seed: int = 7, count: int = 96, lo: int = 32, hi: int = 256, steps: int = 50):
+    """The batched ensemble: `count` members with sizes drawn from the seed."""
+    sizes = ensemble_sizes(seed, count, lo, hi)
+    return concatenate([ensemble_member(int(s), seed * 1000 + k, steps=steps) for k, s in enumerate(sizes)])
