@@ -196,12 +196,20 @@ int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_st
  * the increment of Z_iF from 2-hop features of the warm-start residual and of
  * the previous increments; the prediction is used only when it lowers the
  * initial residual, and the Krylov solve still decides convergence. */
+/* how the starting vector is formed: the least-residual combination of the
+ * previous Z_iF plus the network's predicted increment and the two previous
+ * increments (COMBINED), of the network's prediction alone (NETWORK), or of
+ * the previous increments alone (HISTORY: the method without the network) */
+enum { MHFEM_GUESS_COMBINED = 0, MHFEM_GUESS_NETWORK = 1, MHFEM_GUESS_HISTORY = 2 };
+
 typedef struct {
     int enabled;          /* 0: previous Z_iF (reference behaviour), 1: learned guess */
     int train_steps;      /* train online on the first train_steps solves (0: frozen) */
-    int epochs;           /* full-batch Adam steps per training solve */
+    int epochs;           /* Adam steps per training solve */
     float learning_rate;  /* Adam step size */
     unsigned seed;        /* weight initialisation */
+    int combine;          /* MHFEM_GUESS_* */
+    int sample_stride;    /* an Adam step uses every sample_stride-th tile of 16 DOFs (>= 1) */
 } mhfem_guess_config;
 
 typedef struct {
@@ -210,6 +218,7 @@ typedef struct {
     double r_guess_norm;  /* ||b - A x_guess|| (== r0_norm when not evaluated) */
     double loss;          /* mean squared training loss of the last epoch (normalised units) */
     int trained_steps;    /* solves used for training so far */
+    double coef_network;  /* weight of the network's direction in the combination */
 } mhfem_guess_stats;
 
 int mhfem_guess_configure(mhfem_ctx *ctx, const mhfem_guess_config *cfg);

@@ -92,6 +92,8 @@ public:
         guess_.epochs = p.getParameter<int>("guess-epochs", 20);
         guess_.learning_rate = (float)p.getParameter<double>("guess-learning-rate", 1e-3);
         guess_.seed = (unsigned)p.getParameter<int>("guess-seed", 1);
+        guess_.combine = p.getParameter<int>("guess-combine", MHFEM_GUESS_COMBINED);
+        guess_.sample_stride = p.getParameter<int>("guess-sample-stride", 1);
         allIterations_ = allTimeSteps_ = 0;
         return true;
     }

@@ -51,10 +51,10 @@ def test_gradient_matches_oracle(rows):
         assert err < 1e-2, (name, err)
 
 
-def _run(pr, steps, guess=None, weights=None, train_steps=0, prec="diagonal"):
+def _run(pr, steps, guess=None, weights=None, train_steps=0, combine="combined", prec="diagonal"):
     s = capi.Solver(pr, preconditioner=prec, tol=1e-10)
     if guess:
-        s.configure_guess(True, train_steps=train_steps, epochs=30, learning_rate=2e-3, seed=3)
+        s.configure_guess(True, train_steps=train_steps, epochs=30, learning_rate=2e-3, seed=3, combine=combine)
         if weights is not None:
             s.set_guess_weights(weights)
     its, used = [], []
@@ -70,13 +70,20 @@ def _run(pr, steps, guess=None, weights=None, train_steps=0, prec="diagonal"):
 
 def test_trained_guess_cuts_iterations():
     """Train online on one K field, evaluate frozen weights on a held-out K
-    field: fewer Krylov iterations than the warm start of the reference."""
+    field.  Ablation: the reference's warm start (previous Z_iF), the
+    least-residual combination of the previous increments alone (the method
+    without the network), and the same combination with the network's
+    direction added."""
     train = problems.darcy_3d(s=12, seed=11, steps=30, preconditioner="diagonal")
     s, its_train, _ = _run(train, 30, guess=True, train_steps=30)
     w = s.guess_weights()
     test = problems.darcy_3d(s=12, seed=12, steps=20, preconditioner="diagonal")
     _, base, _ = _run(test, 20)
+    _, hist, _ = _run(test, 20, guess=True, combine="history")
     _, learned, used = _run(test, 20, guess=True, weights=w, train_steps=0)
-    print(f"\nwarm start {sum(base[2:])}, learned {sum(learned[2:])}, used {sum(used)}", flush=True)
+    b, h, l = sum(base[3:]), sum(hist[3:]), sum(learned[3:])
+    print(f"\nwarm start {b}, previous increments {h}, with network {l}, used {sum(used)}", flush=True)
     assert sum(used) > 0
-    assert sum(learned[2:]) < sum(base[2:])
+    assert l < b
+    # the network's direction does not make the combination worse
+    assert l <= h * 1.02

@@ -625,36 +625,39 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
                 }
             }
         }
-        // SELL-64 layout
+        // SELL-64 layout of the off-diagonal entries (the unit diagonal is
+        // implicit); csr_to_sell maps every CSR entry to its SELL position,
+        // -1 for the diagonal
         const int nslices = (c.rows + 63) / 64;
         c.h_slice_ptr.assign((size_t)nslices + 1, 0);
         c.h_slice_width.assign(nslices, 0);
         for (int s = 0; s < nslices; s++) {
             int w = 0;
-            for (int r = s * 64; r < std::min(c.rows, s * 64 + 64); r++) w = std::max(w, c.h_rowptr[r + 1] - c.h_rowptr[r]);
+            for (int r = s * 64; r < std::min(c.rows, s * 64 + 64); r++)
+                w = std::max(w, c.h_rowptr[r + 1] - c.h_rowptr[r] - 1);
             c.h_slice_width[s] = w;
             c.h_slice_ptr[s + 1] = c.h_slice_ptr[s] + (int64_t)w * 64;
         }
         c.sell_size = c.h_slice_ptr[nslices];
-        std::vector<int> scol(c.sell_size);
+        std::vector<int> scol(std::max<int64_t>(c.sell_size, 1));
         std::vector<int64_t> c2s(c.nnz);
         for (int s = 0; s < nslices; s++)
             for (int lane = 0; lane < 64; lane++) {
                 const int r = s * 64 + lane;
-                for (int k = 0; k < c.h_slice_width[s]; k++) {
-                    const int64_t q = c.h_slice_ptr[s] + (int64_t)k * 64 + lane;
-                    if (r >= c.rows) {
-                        scol[q] = 0;
-                        continue;
+                int k = 0;
+                if (r < c.rows)
+                    for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++) {
+                        if (c.h_col[q] == r) {
+                            c2s[q] = -1;
+                            continue;
+                        }
+                        const int64_t pos = c.h_slice_ptr[s] + (int64_t)k * 64 + lane;
+                        scol[pos] = c.h_col[q];
+                        c2s[q] = pos;
+                        k++;
                     }
-                    const int len = c.h_rowptr[r + 1] - c.h_rowptr[r];
-                    if (k < len) {
-                        scol[q] = c.h_col[c.h_rowptr[r] + k];
-                        c2s[c.h_rowptr[r] + k] = q;
-                    } else {
-                        scol[q] = r;  // padding: zero value on the row's own column
-                    }
-                }
+                for (; k < c.h_slice_width[s]; k++)  // padding: zero value on the row's own column
+                    scol[c.h_slice_ptr[s] + (int64_t)k * 64 + lane] = r < c.rows ? r : 0;
             }
         upload(c.slice_ptr, c.h_slice_ptr.data(), c.h_slice_ptr.size());
         upload(c.slice_width, c.h_slice_width.data(), c.h_slice_width.size());
@@ -662,8 +665,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         upload(c.csr_to_sell, c2s.data(), c2s.size());
         upload(c.rowptr, c.h_rowptr.data(), c.h_rowptr.size());
         upload(c.csr_col, c.h_col.data(), c.h_col.size());
-        c.sell_val.alloc(c.sell_size);
-        MHFEM_HIP(hipMemset(c.sell_val.p, 0, c.sell_size * sizeof(double)));
+        c.sell_val.alloc(std::max<int64_t>(c.sell_size, 1));
+        MHFEM_HIP(hipMemset(c.sell_val.p, 0, std::max<int64_t>(c.sell_size, 1) * sizeof(double)));
         c.arr[MHFEM_RHS].alloc(c.rows);
         MHFEM_HIP(hipMemset(c.arr[MHFEM_RHS].p, 0, c.rows * sizeof(double)));
         krylov_destroy(c.krylov);
@@ -766,9 +769,11 @@ int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *v
             std::vector<double> sv(c.sell_size);
             MHFEM_HIP(hipMemcpyAsync(sv.data(), c.sell_val.p, c.sell_size * sizeof(double), hipMemcpyDeviceToHost, c.stream));
             MHFEM_HIP(hipStreamSynchronize(c.stream));
-            for (int r = 0; r < c.rows; r++)
+            for (int r = 0; r < c.rows; r++) {
+                int k = 0;
                 for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++)
-                    val[q] = sv[sell_pos(c.h_slice_ptr.data(), r, q - c.h_rowptr[r])];
+                    val[q] = c.h_col[q] == r ? 1.0 : sv[sell_pos(c.h_slice_ptr.data(), r, k++)];
+            }
         }
         if (rhs) {
             MHFEM_HIP(hipMemcpyAsync(rhs, c.arr[MHFEM_RHS].p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));

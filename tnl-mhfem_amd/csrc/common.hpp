@@ -36,10 +36,14 @@ struct Coef {
     __device__ __forceinline__ double operator[](long idx) const { return ptr[idx & mask]; }
 };
 
-// SELL-64 sparse matrix: rows grouped in slices of 64 (one wavefront); entry k
-// of row r lives at slice_ptr[r/64] + k*64 + r%64, so a wavefront reading
-// entry k of its 64 rows touches one contiguous 512-byte segment.  Padding
-// entries have val = 0 and col = the row itself.
+// SELL-64 sparse matrix of the OFF-DIAGONAL entries: the assembled MHFEM rows
+// are scaled to a unit diagonal (LinearSystem.h:196-201,
+// BoundaryConditions.hpp:200-203, :375-377), so the diagonal is implicit and
+// A x = x + (off-diagonal part) x.  Rows are grouped in slices of 64 (one
+// wavefront); off-diagonal entry k of row r lives at
+// slice_ptr[r/64] + k*64 + r%64, so a wavefront reading entry k of its 64
+// rows touches one contiguous segment.  Padding entries have val = 0 and
+// col = the row itself.
 template <typename T>
 struct SellView {
     int rows;

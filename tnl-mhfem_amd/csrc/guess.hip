@@ -8,8 +8,10 @@
 //   f0..2 = r0, A r0, A^2 r0 (scaled by s_r = RMS(r0))
 //   f3..5 = dx1, dx2, A dx1   (scaled by s_x = RMS(dx1))
 //   f6 = Dirichlet row, f7 = component index, f8 = log10(s_r / s_x) / 4
-// The prediction is x0 = x_prev + s_x * y.  It is used only if it lowers the
-// initial residual; the Krylov solve then judges convergence as before.
+// The prediction d = s_x * y is a direction: the starting vector is the
+// combination x_prev + a_0 d + a_1 dx1 + a_2 dx2 of least residual norm
+// (normal equations of three directions), used only if it lowers the initial
+// residual; the Krylov solve then judges convergence as before.
 //
 // Arithmetic: BF16 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation.  Every
 // layer is evaluated transposed, H^T = W^T X^T, so the D operand of one MFMA
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(GB) k_mlp_forward(long rows, const unsigned sh
 __global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short *__restrict__ X,
                                                  const float *__restrict__ target,
                                                  const PackedWeights *__restrict__ Wg, float inv_rows,
-                                                 float *__restrict__ partial)
+                                                 float *__restrict__ partial, long tile_stride, long tile_offset)
 {
     const int l = threadIdx.x & 63, i = l & 15, q = l >> 4, wid = threadIdx.x >> 6;
     const PackedWeights &W = *Wg;
@@ -216,13 +218,15 @@ __global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short
     // every wavefront runs the same number of tiles (tiles past the end are
     // all-zero and contribute nothing), so the work-group barriers of the
     // LDS transposes below are reached uniformly
+    // the epoch uses every tile_stride-th 16-sample tile, starting at tile_offset
     const long ntiles = (rows + 15) / 16;
+    const long nsub = tile_offset < ntiles ? (ntiles - tile_offset + tile_stride - 1) / tile_stride : 0;
     const long wave = (long)blockIdx.x * (GB / 64) + wid;
     const long nwaves = (long)gridDim.x * (GB / 64);
-    const long iters = (ntiles + nwaves - 1) / nwaves;
+    const long iters = (nsub + nwaves - 1) / nwaves;
     for (long it = 0; it < iters; it++) {
-        const long tile = wave + it * nwaves;
-        const long s0 = tile * 16;
+        const long tsub = wave + it * nwaves;
+        const long s0 = tsub < nsub ? (tsub * tile_stride + tile_offset) * 16 : rows;
         Tile T;
         T.X = load_features(X, rows, s0);
         tile_forward(W, A1, A2, A3, T);
@@ -412,12 +416,26 @@ __global__ void __launch_bounds__(GB) k_features(int rows, int n, const double *
     for (int k = 0; k < NF; k++) o[k] = f2bf(f[k]);
 }
 
-// x = x_prev + s_x * y ; target = (x_new - x_prev) / s_x ; dx history
-__global__ void __launch_bounds__(GB) k_apply_guess(int rows, const double *__restrict__ xp, const float *__restrict__ y,
-                                                    const double *__restrict__ scal, double *__restrict__ x)
+// direction predicted by the network: d = s_x * y
+__global__ void __launch_bounds__(GB) k_direction(int rows, const float *__restrict__ y, const double *__restrict__ scal,
+                                                  double *__restrict__ d)
 {
     const int r = blockIdx.x * GB + threadIdx.x;
-    if (r < rows) x[r] = xp[r] + scal[1] * (double)y[r];
+    if (r < rows) d[r] = scal[1] * (double)y[r];
+}
+
+// x = x_prev + sum_k a_k D_k (up to three directions)
+__global__ void __launch_bounds__(GB) k_combine_dirs(int rows, const double *__restrict__ xp, const double *__restrict__ d0,
+                                                     const double *__restrict__ d1, const double *__restrict__ d2,
+                                                     double a0, double a1, double a2, double *__restrict__ x)
+{
+    const int r = blockIdx.x * GB + threadIdx.x;
+    if (r >= rows) return;
+    double v = xp[r];
+    if (d0) v += a0 * d0[r];
+    if (d1) v += a1 * d1[r];
+    if (d2) v += a2 * d2[r];
+    x[r] = v;
 }
 
 __global__ void __launch_bounds__(GB) k_after_solve(int rows, const double *__restrict__ x, const double *__restrict__ xp,
@@ -491,8 +509,12 @@ double host_dot(Ctx &c, const double *x, const double *y) { return dot64(c, x, y
 // one full-batch Adam step on (X, target); returns the mean squared loss
 double train_epoch(Ctx &c, Guess &G, long rows, float lr)
 {
+    const long ntiles = (rows + 15) / 16;
+    const long stride = std::max(1, G.cfg.sample_stride);
+    const long offset = G.adam_step % stride;
+    const long nsub = (ntiles - offset + stride - 1) / stride;
     hipLaunchKernelGGL(k_mlp_grad, G.nblocks, GB, 0, c.stream, rows, G.X.p, G.target.p, G.W.p,
-                       1.0f / (float)rows, G.partial.p);
+                       1.0f / (float)(nsub * 16), G.partial.p, stride, offset);
     G.adam_step++;
     hipLaunchKernelGGL(k_adam, (NPARAM_PAD + GB - 1) / GB, GB, 0, c.stream, G.nblocks, G.partial.p, G.P.p, G.m.p,
                        G.v.p, G.grad.p, lr, 0.9f, 0.999f, 1e-8f, G.adam_step);
@@ -573,22 +595,80 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.dx1.p, G.dx2.p,
                        G.Adx1.p, G.dirichlet.p, G.scal.p, G.X.p);
     MHFEM_HIP(hipGetLastError());
+    // Directions: the network's predicted increment and (unless the network
+    // runs alone) the two previous increments.  The starting vector is the
+    // combination x_prev + sum a_k D_k of least residual norm, so it is never
+    // worse than the warm start; it is kept only if it lowers the residual.
     const bool trained = G.have_weights || G.trained > 0;
-    if (!trained || sx2 == 0.0) return;
-    const long ntiles = ((long)n + 15) / 16;
-    hipLaunchKernelGGL(k_mlp_forward, (unsigned)std::max<long>(1, std::min<long>(4096, (ntiles + 3) / 4)), GB, 0,
-                       c.stream, (long)n, G.X.p, G.W.p, G.y.p);
-    hipLaunchKernelGGL(k_apply_guess, g, GB, 0, c.stream, n, G.xp.p, G.y.p, G.scal.p, G.rtry.p);
-    MHFEM_HIP(hipGetLastError());
-    // residual of the candidate (rtry is overwritten with the residual below)
-    MHFEM_HIP(hipMemcpyAsync(G.Ar0.p, G.rtry.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    residual64(c, G.Ar0.p, b, G.rtry.p);
-    const double rg = std::sqrt(host_dot(c, G.rtry.p, G.rtry.p));
-    G.last.r_guess_norm = rg;
-    if (rg < G.last.r0_norm) {
-        MHFEM_HIP(hipMemcpyAsync(x, G.Ar0.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-        G.last.used = 1;
+    const bool use_net = G.cfg.combine != MHFEM_GUESS_HISTORY && trained && sx2 > 0.0;
+    const bool use_hist = G.cfg.combine != MHFEM_GUESS_NETWORK;
+    const double *D[3] = {nullptr, nullptr, nullptr}, *AD[3] = {nullptr, nullptr, nullptr};
+    int k = 0;
+    if (use_net) {
+        const long ntiles = ((long)n + 15) / 16;
+        hipLaunchKernelGGL(k_mlp_forward, (unsigned)std::max<long>(1, std::min<long>(4096, (ntiles + 3) / 4)), GB, 0,
+                           c.stream, (long)n, G.X.p, G.W.p, G.y.p);
+        hipLaunchKernelGGL(k_direction, g, GB, 0, c.stream, n, G.y.p, G.scal.p, G.rtry.p);
+        MHFEM_HIP(hipGetLastError());
+        spmv64(c, G.rtry.p, G.Ar0.p);  // Ar0 and A2r0 are free once the features are packed
+        D[k] = G.rtry.p;
+        AD[k++] = G.Ar0.p;
     }
+    if (use_hist && sx2 > 0.0) {
+        D[k] = G.dx1.p;
+        AD[k++] = G.Adx1.p;
+        if (G.history >= 2) {
+            spmv64(c, G.dx2.p, G.A2r0.p);
+            D[k] = G.dx2.p;
+            AD[k++] = G.A2r0.p;
+        }
+    }
+    if (k == 0) return;
+    // normal equations of min || r0 - sum a_k A D_k ||
+    double Gm[3][3] = {}, gv[3] = {}, a[3] = {};
+    for (int p = 0; p < k; p++) {
+        gv[p] = host_dot(c, G.r0.p, AD[p]);
+        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = host_dot(c, AD[p], AD[q]);
+    }
+    // Gaussian elimination with partial pivoting, tiny Tikhonov term against
+    // (nearly) parallel directions
+    double M[3][4];
+    for (int p = 0; p < k; p++) {
+        for (int q = 0; q < k; q++) M[p][q] = Gm[p][q] + (p == q ? 1e-12 * Gm[p][p] : 0.0);
+        M[p][3] = gv[p];
+    }
+    bool ok = true;
+    for (int col = 0; col < k && ok; col++) {
+        int piv = col;
+        for (int r = col + 1; r < k; r++)
+            if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+        if (M[piv][col] == 0.0) {
+            ok = false;
+            break;
+        }
+        for (int q = 0; q < 4; q++) std::swap(M[col][q], M[piv][q]);
+        for (int r = col + 1; r < k; r++) {
+            const double f = M[r][col] / M[col][col];
+            for (int q = col; q < 4; q++) M[r][q] -= f * M[col][q];
+        }
+    }
+    if (!ok) return;
+    for (int p = k - 1; p >= 0; p--) {
+        double v = M[p][3];
+        for (int q = p + 1; q < k; q++) v -= M[p][q] * a[q];
+        a[p] = v / M[p][p];
+    }
+    double pred = rr;
+    for (int p = 0; p < k; p++) {
+        pred -= 2 * a[p] * gv[p];
+        for (int q = 0; q < k; q++) pred += a[p] * Gm[p][q] * a[q];
+    }
+    G.last.r_guess_norm = std::sqrt(std::max(pred, 0.0));
+    G.last.coef_network = use_net ? a[0] : 0.0;
+    if (!(pred < rr)) return;
+    hipLaunchKernelGGL(k_combine_dirs, g, GB, 0, c.stream, n, G.xp.p, D[0], D[1], D[2], a[0], a[1], a[2], x);
+    MHFEM_HIP(hipGetLastError());
+    G.last.used = 1;
 }
 
 // After a solve: the increment becomes a training target (during the training
@@ -640,7 +720,8 @@ void guess_gradient(Ctx &c, Guess &G, const uint16_t *X, const float *t, long ro
     MHFEM_HIP(hipMemcpy(dt.p, t, rows * sizeof(float), hipMemcpyHostToDevice));
     const long ntiles = (rows + 15) / 16;
     const int nb = (int)std::max<long>(1, std::min<long>(G.nblocks, (ntiles + 3) / 4));
-    hipLaunchKernelGGL(k_mlp_grad, nb, GB, 0, c.stream, rows, dX.p, dt.p, G.W.p, 1.0f / (float)rows, G.partial.p);
+    hipLaunchKernelGGL(k_mlp_grad, nb, GB, 0, c.stream, rows, dX.p, dt.p, G.W.p, 1.0f / (float)rows, G.partial.p,
+                       1L, 0L);
     // reduce only (learning rate 0)
     hipLaunchKernelGGL(k_adam, (NPARAM_PAD + GB - 1) / GB, GB, 0, c.stream, nb, G.partial.p, G.P.p, G.m.p, G.v.p,
                        G.grad.p, 0.0f, 0.9f, 0.999f, 1e-8f, 1);

@@ -179,12 +179,13 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 }
 
 // ---------------------------------------------------------------------------
+// (A x)[row] = x[row] + sum of the stored off-diagonal entries
 template <typename T>
 __device__ __forceinline__ T sell_row(const SellView<T> &A, const T *__restrict__ x, int row)
 {
     const int64_t base = A.slice_ptr[row >> 6] + (row & 63);
     const int w = A.slice_width[row >> 6];
-    T sum = 0;
+    T sum = x[row];
     for (int k = 0; k < w; k++) {
         const int64_t q = base + (int64_t)k * 64;
         sum += A.val[q] * x[A.col[q]];
@@ -345,24 +346,17 @@ __global__ void __launch_bounds__(KB) k_fill_t(long n, T *__restrict__ x, T v)
     for (long i = (long)blockIdx.x * KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) x[i] = v;
 }
 
-// Jacobi: dinv[r] = 1 / A(r,r) (TNL Diagonal preconditioner)
-template <typename T>
-__global__ void __launch_bounds__(KB) k_diag_inv(int n, const T *__restrict__ val, const int64_t *__restrict__ dpos,
-                                                 T *__restrict__ dinv)
-{
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
-        const T d = val[dpos[i]];
-        dinv[i] = d != (T)0 ? (T)1 / d : (T)1;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // ILU(0), level-scheduled.  Factors are kept in CSR order (natural rows).
+// CSR-ordered copy of the matrix; map < 0 marks the implicit unit diagonal
 template <typename T>
 __global__ void __launch_bounds__(KB) k_gather(long n, const T *__restrict__ src, const int64_t *__restrict__ map,
                                                T *__restrict__ dst)
 {
-    for (long i = (long)blockIdx.x * KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) dst[i] = src[map[i]];
+    for (long i = (long)blockIdx.x * KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) {
+        const int64_t q = map[i];
+        dst[i] = q < 0 ? (T)1 : src[q];
+    }
 }
 
 // IKJ elimination of the rows of one level (Saad, Alg. 10.4); the rows they
@@ -428,7 +422,6 @@ struct Ilu0 {
 struct Krylov {
     int n = 0;
     DBuf<double> r, r_ast, p, Ap, s, As, tmp, tmp2, dinv;
-    DBuf<int64_t> diag_pos;  // SELL position of the diagonal of each row
     // segmented decomposition
     int nseg = 1, rows_per_block = 2048;
     DBuf<int> blk_seg, blk_row, seg_blk;
@@ -491,18 +484,6 @@ Krylov *krylov_create(Ctx &c)
     k->sc.alloc((size_t)k->nseg * NSLOTS);
     MHFEM_HIP(hipHostMalloc((void **)&k->h_sc, (size_t)k->nseg * NSLOTS * sizeof(double), hipHostMallocDefault));
     build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
-    // diagonal positions in SELL storage
-    std::vector<int64_t> dpos(n);
-    for (int r = 0; r < n; r++) {
-        int kk = -1;
-        for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++)
-            if (c.h_col[q] == r) kk = q - c.h_rowptr[r];
-        if (kk < 0) throw Error{MHFEM_E_STATE, "matrix row without a diagonal entry"};
-        dpos[r] = sell_pos(c.h_slice_ptr.data(), r, kk);
-    }
-    k->diag_pos.alloc(n);
-    MHFEM_HIP(hipMemcpy(k->diag_pos.p, dpos.data(), n * sizeof(int64_t), hipMemcpyHostToDevice));
-
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
         Ilu0 &I = k->ilu;
         std::vector<int> dg(n), lev(n), ulev(n);
@@ -939,7 +920,8 @@ static void precond_update(Ctx &c, Krylov &k)
 {
     const int n = k.n;
     const unsigned g = grid_for(n);
-    hipLaunchKernelGGL((k_diag_inv<double>), g, KB, 0, c.stream, n, c.sell_val.p, k.diag_pos.p, k.dinv.p);
+    // the diagonal is 1 by construction: the Jacobi scaling is 1 / 1
+    hipLaunchKernelGGL((k_fill_t<double>), g, KB, 0, c.stream, (long)n, k.dinv.p, 1.0);
     MHFEM_HIP(hipGetLastError());
     const bool mixed = c.cfg.precision == MHFEM_PRECISION_FP32_REFINED;
     if (mixed) {

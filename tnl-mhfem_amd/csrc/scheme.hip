@@ -494,18 +494,22 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
             c0.load(s, i, K0, e0, E);
             c1.load(s, i, K1, e1, E);
             const double d = c0.A(adv, i, e0, true) + c1.A(adv, i, e1, true);
+            // the diagonal entry is exactly 1 (LinearSystem.h:196-201) and is
+            // not stored: k counts the off-diagonal entries of the row
             int k = 0;
             for (int t = 0; t < nslots; t++) {
                 const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
 #pragma unroll
                 for (int j = 0; j < N; j++) {
                     double val;
-                    if (f0 < 7 && f1 < 7)
-                        val = (j == i) ? 1.0 : (c0.A(adv, j, f0, true) + c1.A(adv, j, f1, true)) / d;
-                    else if (f0 < 7)
+                    if (f0 < 7 && f1 < 7) {
+                        if (j == i) continue;
+                        val = (c0.A(adv, j, f0, true) + c1.A(adv, j, f1, true)) / d;
+                    } else if (f0 < 7) {
                         val = c0.A(adv, j, f0, false) / d;
-                    else
+                    } else {
                         val = c1.A(adv, j, f1, false) / d;
+                    }
                     s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = val;
                 }
             }
@@ -523,7 +527,7 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
         const long bi = (long)i * s.F + E;
         const int tag = s.tags[bi];
         if (tag == MHFEM_BC_FIXED_VALUE) {
-            s.A.val[sell_pos(s.A.slice_ptr, row, 0)] = 1.0;
+            // the row is the (implicit) unit diagonal alone (BoundaryConditions.hpp:373-380)
             s.rhs[row] = s.bc_dirichlet[bi];
             continue;
         }
@@ -543,8 +547,8 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
             const int f = (info >> (6 * t)) & 7;
 #pragma unroll
             for (int j = 0; j < N; j++) {
-                const double val = (j == i && f == e0) ? 1.0 : BA(j, f, f == e0) / d;
-                s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = val;
+                if (j == i && f == e0) continue;  // unit diagonal, not stored (BoundaryConditions.hpp:200-203)
+                s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = BA(j, f, f == e0) / d;
             }
         }
         double bval;
