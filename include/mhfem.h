@@ -82,8 +82,13 @@ typedef struct {
 /* Linear solver selection (configSetup.h:31-62). */
 enum { MHFEM_SOLVER_BICGSTAB = 0, MHFEM_SOLVER_GMRES = 1 };
 enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2 };
-/* precision of the Krylov iteration */
-enum { MHFEM_PRECISION_FP64 = 0, MHFEM_PRECISION_FP32_REFINED = 1 };
+/* precision of the Krylov iteration.  FP32_REFINED: FP64 iterative
+   refinement whose every correction is an FP32 BiCGStab solve started from
+   zero to `inner_residue`.  FP32_RELIABLE: one FP32 BiCGStab recurrence whose
+   residual is replaced by the FP64 residual (and the FP32 correction added
+   to the FP64 solution) each time it has dropped by `inner_residue`; the
+   search direction is kept across the replacements. */
+enum { MHFEM_PRECISION_FP64 = 0, MHFEM_PRECISION_FP32_REFINED = 1, MHFEM_PRECISION_FP32_RELIABLE = 2 };
 /* initial guess of each solve */
 enum { MHFEM_GUESS_PREVIOUS = 0, MHFEM_GUESS_LEARNED = 1 };
 
@@ -94,7 +99,8 @@ typedef struct {
     int max_iterations;         /* "max-iterations" */
     int gmres_restart;          /* "gmres-restarting-max" */
     int precision;              /* MHFEM_PRECISION_* */
-    double inner_residue;       /* FP32 inner tolerance for MHFEM_PRECISION_FP32_REFINED */
+    double inner_residue;       /* FP32 inner tolerance (FP32_REFINED) or residual drop per
+                                   FP64 update (FP32_RELIABLE) */
     int check_every;            /* host convergence poll period (iterations), >= 1 */
     int rows_per_block;         /* rows per work-group of the iteration kernels, 0: default
                                    (mhfem_autotune picks it by measurement) */

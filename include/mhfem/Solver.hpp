@@ -82,9 +82,11 @@ public:
         cfg_.convergence_residue = p.getParameter<double>("convergence-residue", 1e-6);
         cfg_.max_iterations = p.getParameter<int>("max-iterations", 100000);
         cfg_.gmres_restart = p.getParameter<int>("gmres-restarting-max", 30);
-        cfg_.precision = p.getParameter<std::string>("precision", std::string("fp64")) == "fp32-refined"
-                             ? MHFEM_PRECISION_FP32_REFINED
-                             : MHFEM_PRECISION_FP64;
+        const std::string precision = p.getParameter<std::string>("precision", std::string("fp64"));
+        if (precision == "fp64") cfg_.precision = MHFEM_PRECISION_FP64;
+        else if (precision == "fp32-refined") cfg_.precision = MHFEM_PRECISION_FP32_REFINED;
+        else if (precision == "fp32-reliable") cfg_.precision = MHFEM_PRECISION_FP32_RELIABLE;
+        else return false;
         cfg_.inner_residue = p.getParameter<double>("inner-residue", 1e-3);
         cfg_.check_every = p.getParameter<int>("check-every", 8);
         guess_.enabled = p.getParameter<std::string>("initial-guess", std::string("previous")) == "learned";

@@ -147,12 +147,13 @@ def test_ilu0_step_parity():
         assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10
 
 
+@pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2)])
 @pytest.mark.parametrize("prec", ["diagonal", "ilu0"])
-def test_fp32_refinement(prec):
+def test_fp32_refinement(prec, precision, inner):
     """FP32 inner BiCGStab in FP64 refinement reaches the FP64 tolerance; the
     error against the all-FP64 oracle stays at the level of the tolerance."""
     pr = problems.adr_3d(s=5, steps=3, preconditioner=prec)
-    s = capi.Solver(pr, tol=1e-10, precision="fp32-refined", inner_tol=1e-4)
+    s = capi.Solver(pr, tol=1e-10, precision=precision, inner_tol=inner)
     o = pyoracle.Oracle(pr)
     t = 0.0
     for k in range(pr.steps):
@@ -162,6 +163,24 @@ def test_fp32_refinement(prec):
         assert st.converged and st.residue < 1e-10
         assert st.outer_iterations >= 1
         assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-8
+
+
+def test_fp32_reliable_iterations():
+    """Reliable updates keep the FP32 recurrence's Krylov space across the
+    FP64 residual replacements, so the iteration count stays close to the
+    all-FP64 solve's, unlike restarted refinement."""
+    pr = problems.adr_3d(s=12, steps=3, preconditioner="diagonal")
+    s64 = capi.Solver(pr, tol=1e-10)
+    s32 = capi.Solver(pr, tol=1e-10, precision="fp32-reliable", inner_tol=1e-2)
+    t = 0.0
+    for k in range(pr.steps):
+        a = s64.step(t, pr.dt)
+        b = s32.step(t, pr.dt)
+        t += pr.dt
+        print(k, a.iterations, b.iterations, b.outer_iterations)
+        assert b.converged and b.residue < 1e-10
+        assert b.iterations <= 1.3 * a.iterations + 8
+        assert rel(s32.get_array("Z_iF"), s64.get_array("Z_iF")) < 1e-8
 
 
 def test_nonconvergence_raises():

@@ -268,7 +268,7 @@ SchemeView Ctx::view()
     s.bc_values = arr[MHFEM_BC_VALUES].p;
     s.bc_dirichlet = arr[MHFEM_BC_DIRICHLET].p;
     s.bc_mobility = arr[MHFEM_BC_MOBILITY].p;
-    s.A = SellView<double>{rows, slice_ptr.p, slice_width.p, sell_col.p, sell_val.p};
+    s.A = SellView<double>{F, slice_ptr.p, slice_width.p, sell_col.p, sell_val.p, diag_base};
     s.rhs = arr[MHFEM_RHS].p;
     return s;
 }
@@ -570,7 +570,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
         if (cfg->solver < 0 || cfg->solver > 1) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
         if (cfg->preconditioner < 0 || cfg->preconditioner > 2) throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
-        if (cfg->precision == MHFEM_PRECISION_FP32_REFINED && cfg->solver != MHFEM_SOLVER_BICGSTAB)
+        if (cfg->precision < 0 || cfg->precision > 2) throw Error{MHFEM_E_ARGUMENT, "unknown precision"};
+        if (cfg->precision != MHFEM_PRECISION_FP64 && cfg->solver != MHFEM_SOLVER_BICGSTAB)
             throw Error{MHFEM_E_ARGUMENT, "FP32 refinement is implemented for BiCGStab only"};
         c.cfg = *cfg;
         if (c.cfg.max_iterations <= 0) c.cfg.max_iterations = 100000;
@@ -625,39 +626,59 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
                 }
             }
         }
-        // SELL-64 layout of the off-diagonal entries (the unit diagonal is
-        // implicit); csr_to_sell maps every CSR entry to its SELL position,
-        // -1 for the diagonal
-        const int nslices = (c.rows + 63) / 64;
+        // block SELL-64 layout (common.hpp): per face the ascending list of
+        // the other faces its rows couple to (union over its n rows), one
+        // n x n block each; csr_to_sell maps every CSR entry to its value
+        // position, -1 for the unit diagonal
+        const int nslices = (c.F + 63) / 64;
+        std::vector<int64_t> bptr((size_t)c.F + 1, 0);
+        std::vector<int> bface;
+        bface.reserve((size_t)c.F * (2 * fpc - 2));
+        for (int E = 0; E < c.F; E++) {
+            const size_t start = bface.size();
+            for (int i = 0; i < n; i++)
+                for (int q = c.h_rowptr[(size_t)E * n + i]; q < c.h_rowptr[(size_t)E * n + i + 1]; q++) {
+                    const int Fc = c.h_col[q] / n;
+                    if (Fc != E) bface.push_back(Fc);
+                }
+            std::sort(bface.begin() + start, bface.end());
+            bface.erase(std::unique(bface.begin() + start, bface.end()), bface.end());
+            bptr[E + 1] = (int64_t)bface.size();
+        }
         c.h_slice_ptr.assign((size_t)nslices + 1, 0);
         c.h_slice_width.assign(nslices, 0);
         for (int s = 0; s < nslices; s++) {
-            int w = 0;
-            for (int r = s * 64; r < std::min(c.rows, s * 64 + 64); r++)
-                w = std::max(w, c.h_rowptr[r + 1] - c.h_rowptr[r] - 1);
-            c.h_slice_width[s] = w;
-            c.h_slice_ptr[s + 1] = c.h_slice_ptr[s] + (int64_t)w * 64;
+            int64_t w = 0;
+            for (int E = s * 64; E < std::min(c.F, s * 64 + 64); E++) w = std::max(w, bptr[E + 1] - bptr[E]);
+            c.h_slice_width[s] = (int)w;
+            c.h_slice_ptr[s + 1] = c.h_slice_ptr[s] + w * 64;
         }
-        c.sell_size = c.h_slice_ptr[nslices];
-        std::vector<int> scol(std::max<int64_t>(c.sell_size, 1));
-        std::vector<int64_t> c2s(c.nnz);
+        const int nn = n * n;
+        c.sell_blocks = c.h_slice_ptr[nslices];
+        c.diag_base = c.sell_blocks * nn;
+        c.sell_size = c.diag_base + (int64_t)c.F * n * (n - 1);
+        std::vector<int> scol(std::max<int64_t>(c.sell_blocks, 1));
         for (int s = 0; s < nslices; s++)
             for (int lane = 0; lane < 64; lane++) {
-                const int r = s * 64 + lane;
+                const int E = s * 64 + lane;
+                const int64_t w = E < c.F ? bptr[E + 1] - bptr[E] : 0;
+                for (int k = 0; k < c.h_slice_width[s]; k++)
+                    scol[sell_block(c.h_slice_ptr.data(), E, k)] = k < w ? bface[bptr[E] + k] : (E < c.F ? E : 0);
+            }
+        std::vector<int64_t> c2s(c.nnz);
+        for (int E = 0; E < c.F; E++)
+            for (int i = 0; i < n; i++) {
+                const int r = E * n + i;
                 int k = 0;
-                if (r < c.rows)
-                    for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++) {
-                        if (c.h_col[q] == r) {
-                            c2s[q] = -1;
-                            continue;
-                        }
-                        const int64_t pos = c.h_slice_ptr[s] + (int64_t)k * 64 + lane;
-                        scol[pos] = c.h_col[q];
-                        c2s[q] = pos;
-                        k++;
+                for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++) {
+                    const int Fc = c.h_col[q] / n, j = c.h_col[q] % n;
+                    if (Fc == E) {
+                        c2s[q] = j == i ? -1 : sell_diag(c.diag_base, n, E, i, j);
+                        continue;
                     }
-                for (; k < c.h_slice_width[s]; k++)  // padding: zero value on the row's own column
-                    scol[c.h_slice_ptr[s] + (int64_t)k * 64 + lane] = r < c.rows ? r : 0;
+                    while (bface[bptr[E] + k] != Fc) k++;  // the row's faces are a sorted subset
+                    c2s[q] = sell_block(c.h_slice_ptr.data(), E, k) * nn + i * n + j;
+                }
             }
         upload(c.slice_ptr, c.h_slice_ptr.data(), c.h_slice_ptr.size());
         upload(c.slice_width, c.h_slice_width.data(), c.h_slice_width.size());
@@ -767,13 +788,11 @@ int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *v
         if (col) std::memcpy(col, c.h_col.data(), c.h_col.size() * sizeof(int));
         if (val) {
             std::vector<double> sv(c.sell_size);
+            std::vector<int64_t> c2s(c.nnz);
             MHFEM_HIP(hipMemcpyAsync(sv.data(), c.sell_val.p, c.sell_size * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            MHFEM_HIP(hipMemcpyAsync(c2s.data(), c.csr_to_sell.p, c.nnz * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
             MHFEM_HIP(hipStreamSynchronize(c.stream));
-            for (int r = 0; r < c.rows; r++) {
-                int k = 0;
-                for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++)
-                    val[q] = c.h_col[q] == r ? 1.0 : sv[sell_pos(c.h_slice_ptr.data(), r, k++)];
-            }
+            for (long q = 0; q < c.nnz; q++) val[q] = c2s[q] < 0 ? 1.0 : sv[c2s[q]];
         }
         if (rhs) {
             MHFEM_HIP(hipMemcpyAsync(rhs, c.arr[MHFEM_RHS].p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));

@@ -36,26 +36,41 @@ struct Coef {
     __device__ __forceinline__ double operator[](long idx) const { return ptr[idx & mask]; }
 };
 
-// SELL-64 sparse matrix of the OFF-DIAGONAL entries: the assembled MHFEM rows
-// are scaled to a unit diagonal (LinearSystem.h:196-201,
-// BoundaryConditions.hpp:200-203, :375-377), so the diagonal is implicit and
-// A x = x + (off-diagonal part) x.  Rows are grouped in slices of 64 (one
-// wavefront); off-diagonal entry k of row r lives at
-// slice_ptr[r/64] + k*64 + r%64, so a wavefront reading entry k of its 64
-// rows touches one contiguous segment.  Padding entries have val = 0 and
-// col = the row itself.
+// Block SELL-64 storage of the hybridised system.  The unknowns of face E
+// are the rows E*n+i, i < n (Solver.hpp:560-569); row (E,i) couples to the
+// unknowns (F,j) of every face F of the cells of E, so the matrix is made of
+// n x n blocks on a face-to-face pattern.  Faces are grouped in slices of 64
+// (one wavefront, one lane per face).  Off-diagonal block k of face E sits at
+// block index q = slice_ptr[E/64] + k*64 + E%64 with
+//   col[q]                 = F (the column face),
+//   val[q*n*n + i*n + j]   = A[(E,i), (F,j)],
+// so a wavefront reading block k of its 64 faces touches one contiguous
+// segment.  The assembled rows are scaled to a unit diagonal
+// (LinearSystem.h:196-201, BoundaryConditions.hpp:200-203, :375-377): the
+// block on the face's own column is stored without its diagonal, its n(n-1)
+// other entries at val[diag_base + E*n*(n-1) + i*(n-1) + (j < i ? j : j-1)],
+// and A x = x + (stored part) x.  Padding blocks have col = the face itself
+// and zero values; so do the entries of a face's block rows that are not in
+// that row's pattern (rows of one face may differ, e.g. a Dirichlet row).
 template <typename T>
 struct SellView {
-    int rows;
-    const int64_t *slice_ptr;
-    const int *slice_width;
-    const int *col;
+    int faces;
+    const int64_t *slice_ptr;  // in blocks
+    const int *slice_width;    // blocks per face in the slice
+    const int *col;            // column face of each block
     T *val;
+    int64_t diag_base;
 };
 
-__host__ __device__ inline int64_t sell_pos(const int64_t *slice_ptr, int row, int k)
+__host__ __device__ inline int64_t sell_block(const int64_t *slice_ptr, int face, int k)
 {
-    return slice_ptr[row >> 6] + (int64_t)k * 64 + (row & 63);
+    return slice_ptr[face >> 6] + (int64_t)k * 64 + (face & 63);
+}
+
+// position of entry (i, j) of the diagonal block of `face` (i != j)
+__host__ __device__ inline int64_t sell_diag(int64_t diag_base, int n, int face, int i, int j)
+{
+    return diag_base + (int64_t)face * n * (n - 1) + i * (n - 1) + (j < i ? j : j - 1);
 }
 
 // All device pointers of the scheme, passed to kernels by value.
@@ -128,7 +143,7 @@ struct Ctx {
     DBuf<uint8_t> tags;
     std::vector<uint8_t> h_tags;
     bool bc_set = false;
-    // linear system (SELL-64 storage + CSR pattern on the host)
+    // linear system (block SELL-64 storage + CSR pattern on the host)
     int rows = 0;
     long nnz = 0;
     std::vector<int> h_rowptr, h_col;
@@ -136,9 +151,11 @@ struct Ctx {
     std::vector<int> h_slice_width;
     DBuf<int64_t> slice_ptr;
     DBuf<int> slice_width, sell_col, rowptr, csr_col;
-    DBuf<int64_t> csr_to_sell;
+    DBuf<int64_t> csr_to_sell;  // CSR entry -> position in sell_val, -1 for the unit diagonal
     DBuf<double> sell_val;
-    long sell_size = 0;
+    long sell_size = 0;         // values in sell_val (blocks * n * n + F * n * (n - 1))
+    int64_t sell_blocks = 0;    // block slots, padding included
+    int64_t diag_base = 0;
     bool system_ready = false;
     bool unit_diagonal = true;  // the assembler writes 1 on every diagonal
     mhfem_solver_config cfg{};

@@ -38,9 +38,9 @@ constexpr int MAXG = 2048;
 // device scalar slots (per segment)
 enum Slot {
     S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BETA, S_RAST_SQ, S_BNORM, S_RESID, S_TOL, S_IT,
-    S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, NSLOTS = 32
+    S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, S_TOLABS, S_DELTA, NSLOTS = 32
 };
-enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT };
+enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT, FIN_REPLACE };
 // which segments a kernel works on
 enum Mode { M_ALL = 0, M_ACTIVE = 1, M_RESTART = 2 };
 
@@ -102,6 +102,23 @@ __device__ void finalize(double *sc, int fin)
         sc[S_RESID] = res;
         sc[S_RESTART] = 0;
         sc[S_DONE] = (!(res >= sc[S_TOL]) || sc[S_IT] >= sc[S_MAXIT]) ? 1 : 0;
+        break;
+    }
+    case FIN_REPLACE: {
+        // reliable update: the recursive residual was replaced by the true
+        // one (D0 = (r, r), D1 = (r, r_ast)); the recurrence continues with
+        // the same search direction, so only rho, beta and the reference
+        // norm of the next stage change
+        const double rn = sqrt(sc[S_D0]);
+        sc[S_RHO] = sc[S_D1];
+        sc[S_BETA] = (sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]);
+        sc[S_BNORM] = rn;
+        sc[S_TOL] = rn > 0 ? fmax(sc[S_DELTA], sc[S_TOLABS] / rn) : 1.0;
+        sc[S_RESID] = 1.0;
+        sc[S_DONE] = rn == 0.0 ? 1 : 0;
+        // breakdown of the continued recurrence: start afresh from the
+        // replaced residual
+        sc[S_RESTART] = (rn > 0.0 && !(fabs(sc[S_RHO]) >= eps2 * sc[S_RAST_SQ])) ? 1 : 0;
         break;
     }
     default: break;
@@ -179,18 +196,63 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 }
 
 // ---------------------------------------------------------------------------
-// (A x)[row] = x[row] + sum of the stored off-diagonal entries
-template <typename T>
-__device__ __forceinline__ T sell_row(const SellView<T> &A, const T *__restrict__ x, int row)
+// M consecutive values moved as one vector access (16-byte aligned where the
+// size allows; the storage offsets keep that alignment)
+template <typename T, int M>
+struct alignas((sizeof(T) * M) % 16 == 0 ? 16 : ((sizeof(T) * M) % 8 == 0 ? 8 : sizeof(T))) Pack {
+    T v[M];
+};
+template <typename T, int M>
+__device__ __forceinline__ Pack<T, M> ld(const T *p)
 {
-    const int64_t base = A.slice_ptr[row >> 6] + (row & 63);
-    const int w = A.slice_width[row >> 6];
-    T sum = x[row];
+    return *reinterpret_cast<const Pack<T, M> *>(p);
+}
+template <typename T, int M>
+__device__ __forceinline__ void st(T *p, const Pack<T, M> &v)
+{
+    *reinterpret_cast<Pack<T, M> *>(p) = v;
+}
+
+// The N rows of face E: y = x_E + (diagonal block without its unit
+// diagonal) x_E + sum over the stored blocks of B_k x_{col_k}.  One block
+// costs one column index, one N*N vector load and one N-wide gather of x.
+template <int N, typename T>
+__device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
+{
+    const Pack<T, N> xe = ld<T, N>(x + (int64_t)E * N);
+    Pack<T, N> y = xe;
+    if constexpr (N > 1) {
+        const Pack<T, N *(N - 1)> dg = ld<T, N *(N - 1)>(A.val + A.diag_base + (int64_t)E * N * (N - 1));
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++)
+                if (j != i) y.v[i] += dg.v[i * (N - 1) + (j < i ? j : j - 1)] * xe.v[j];
+    }
+    const int64_t base = A.slice_ptr[E >> 6] + (E & 63);
+    const int w = A.slice_width[E >> 6];
     for (int k = 0; k < w; k++) {
         const int64_t q = base + (int64_t)k * 64;
-        sum += A.val[q] * x[A.col[q]];
+        const Pack<T, N * N> b = ld<T, N * N>(A.val + q * (N * N));
+        const Pack<T, N> xf = ld<T, N>(x + (int64_t)A.col[q] * N);
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++) y.v[i] += b.v[i * N + j] * xf.v[j];
     }
-    return sum;
+    return y;
+}
+
+// launch helper: f(std::integral_constant<int, n>) for the supported n
+template <typename F>
+static void with_n(int n, F &&f)
+{
+    switch (n) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    default: throw Error{MHFEM_E_ARGUMENT, "unsupported number of equations"};
+    }
 }
 
 #define SEG_BLOCK_PROLOGUE(sg, mode)                                   \
@@ -199,30 +261,48 @@ __device__ __forceinline__ T sell_row(const SellView<T> &A, const T *__restrict_
     if (!seg_selected(sc_, (mode))) return;                            \
     const int row0_ = (sg).blk_row[blockIdx.x], row1_ = (sg).blk_row[blockIdx.x + 1]
 
-// y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y)
-template <typename T, bool DIAG, int ND>
+// y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y).  A
+// work-group's row range holds whole faces (build_blocks).
+template <typename T, int N, bool DIAG, int ND>
 __global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
                                               const T *__restrict__ dinv, T *__restrict__ y, const T *__restrict__ z)
 {
     SEG_BLOCK_PROLOGUE(sg, mode);
     double acc[ND > 0 ? ND : 1] = {};
-    for (int row = row0_ + threadIdx.x; row < row1_; row += KB) {
-        T s = sell_row(A, x, row);
-        if (DIAG) s *= dinv[row];
-        y[row] = s;
-        if constexpr (ND >= 1) acc[0] += (double)s * (double)z[row];
-        if constexpr (ND >= 2) acc[1] += (double)s * (double)s;
+    for (int f = row0_ / N + threadIdx.x; f < row1_ / N; f += KB) {
+        Pack<T, N> v = sell_face<N>(A, x, f);
+        if constexpr (DIAG) {
+            const Pack<T, N> d = ld<T, N>(dinv + (int64_t)f * N);
+#pragma unroll
+            for (int i = 0; i < N; i++) v.v[i] *= d.v[i];
+        }
+        st<T, N>(y + (int64_t)f * N, v);
+        if constexpr (ND >= 1) {
+            const Pack<T, N> zz = ld<T, N>(z + (int64_t)f * N);
+#pragma unroll
+            for (int i = 0; i < N; i++) acc[0] += (double)v.v[i] * (double)zz.v[i];
+        }
+        if constexpr (ND >= 2) {
+#pragma unroll
+            for (int i = 0; i < N; i++) acc[1] += (double)v.v[i] * (double)v.v[i];
+        }
     }
     if constexpr (ND > 0) block_partial<ND>(acc, sg);
 }
 
 // y = b - A x on the selected segments
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(KB) k_sresidual(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
                                                   const T *__restrict__ b, T *__restrict__ y)
 {
     SEG_BLOCK_PROLOGUE(sg, mode);
-    for (int row = row0_ + threadIdx.x; row < row1_; row += KB) y[row] = b[row] - sell_row(A, x, row);
+    for (int f = row0_ / N + threadIdx.x; f < row1_ / N; f += KB) {
+        Pack<T, N> v = sell_face<N>(A, x, f);
+        const Pack<T, N> bb = ld<T, N>(b + (int64_t)f * N);
+#pragma unroll
+        for (int i = 0; i < N; i++) v.v[i] = bb.v[i] - v.v[i];
+        st<T, N>(y + (int64_t)f * N, v);
+    }
 }
 
 template <typename T, int ND>
@@ -306,26 +386,39 @@ __global__ void __launch_bounds__(KB) k_bicg_p(Seg sg, const T *__restrict__ r, 
 
 // ---------------------------------------------------------------------------
 // unsegmented helpers (no flags consulted)
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(KB) k_residual(SellView<T> A, const T *__restrict__ x, const T *__restrict__ b,
                                                  T *__restrict__ y)
 {
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB)
-        y[row] = b[row] - sell_row(A, x, row);
+    for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
+        Pack<T, N> v = sell_face<N>(A, x, f);
+        const Pack<T, N> bb = ld<T, N>(b + (int64_t)f * N);
+#pragma unroll
+        for (int i = 0; i < N; i++) v.v[i] = bb.v[i] - v.v[i];
+        st<T, N>(y + (int64_t)f * N, v);
+    }
 }
 
-template <typename T>
+template <typename T, int N>
 __global__ void __launch_bounds__(KB) k_spmv_plain(SellView<T> A, const T *__restrict__ x, T *__restrict__ y)
 {
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB) y[row] = sell_row(A, x, row);
+    for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB)
+        st<T, N>(y + (int64_t)f * N, sell_face<N>(A, x, f));
 }
 
 // FP64 residual of the refinement loop, stored in FP32: r32 = (float)(b - A x)
+template <int N>
 __global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const double *__restrict__ x,
                                                        const double *__restrict__ b, float *__restrict__ r32)
 {
-    for (int row = blockIdx.x * KB + threadIdx.x; row < A.rows; row += gridDim.x * KB)
-        r32[row] = (float)(b[row] - sell_row(A, x, row));
+    for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
+        const Pack<double, N> v = sell_face<N>(A, x, f);
+        const Pack<double, N> bb = ld<double, N>(b + (int64_t)f * N);
+        Pack<float, N> r;
+#pragma unroll
+        for (int i = 0; i < N; i++) r.v[i] = (float)(bb.v[i] - v.v[i]);
+        st<float, N>(r32 + (int64_t)f * N, r);
+    }
 }
 
 template <typename TO, typename TI>
@@ -334,10 +427,21 @@ __global__ void __launch_bounds__(KB) k_convert(long n, const TI *__restrict__ x
     for (long i = (long)blockIdx.x * KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) y[i] = (TO)x[i];
 }
 
-// x64 += (double) d32
-__global__ void __launch_bounds__(KB) k_add_correction(int n, double *__restrict__ x, const float *__restrict__ d)
+// x64 += (double) d32; d32 = 0
+__global__ void __launch_bounds__(KB) k_add_correction(int n, double *__restrict__ x, float *__restrict__ d)
 {
-    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) x[i] += (double)d[i];
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        x[i] += (double)d[i];
+        d[i] = 0.0f;
+    }
+}
+
+__global__ void k_set_slots(double *sc, double tolabs, double delta)
+{
+    if (threadIdx.x == 0) {
+        sc[S_TOLABS] = tolabs;
+        sc[S_DELTA] = delta;
+    }
 }
 
 template <typename T>
@@ -446,6 +550,8 @@ static inline unsigned grid_for(long n) { return (unsigned)std::max<long>(1, std
 // work-group tables for `rpb` rows per work-group, segment aligned
 static void build_blocks(Ctx &c, Krylov &k, int rpb)
 {
+    // whole faces per work-group: the matrix kernels run one lane per face
+    rpb = std::max(c.n, rpb / c.n * c.n);
     std::vector<int> bseg, brow, sblk(1, 0);
     for (int s = 0; s < k.nseg; s++) {
         const int r0 = c.h_seg_rows[s], r1 = c.h_seg_rows[s + 1];
@@ -521,9 +627,9 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipMemcpy(I.urows.p, urows.data(), n * sizeof(int), hipMemcpyHostToDevice));
         MHFEM_HIP(hipMemcpy(I.diag.p, dg.data(), n * sizeof(int), hipMemcpyHostToDevice));
         I.lu.alloc(c.nnz);
-        if (c.cfg.precision == MHFEM_PRECISION_FP32_REFINED) I.lu32.alloc(c.nnz);
+        if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
     }
-    if (c.cfg.precision == MHFEM_PRECISION_FP32_REFINED) {
+    if (c.cfg.precision != MHFEM_PRECISION_FP64) {
         k->val32.alloc(c.sell_size);
         for (DBuf<float> *b : {&k->dinv32, &k->b32, &k->x32, &k->r32, &k->r_ast32, &k->p32, &k->Ap32, &k->s32,
                                &k->As32, &k->tmp32, &k->rhs32})
@@ -566,7 +672,60 @@ struct Solve {
     }
 
     template <typename T>
-    SellView<T> matrix();
+    SellView<T> matrix()
+    {
+        if constexpr (sizeof(T) == 8)
+            return SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, c.sell_val.p, c.diag_base};
+        else
+            return SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, k.val32.p, c.diag_base};
+    }
+
+    // launches of the matrix kernels, dispatched on the number of equations
+    template <typename T, bool DIAG, int ND>
+    void sspmv(const T *x, const T *dinv, T *y, const T *z, int mode)
+    {
+        SellView<T> A = matrix<T>();
+        with_n(c.n, [&](auto NC) {
+            hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND>), k.nblk, KB, 0, c.stream, A, sg, mode, x,
+                               dinv, y, z);
+        });
+        MHFEM_HIP(hipGetLastError());
+    }
+    template <typename T>
+    void sresidual(const T *x, const T *b, T *y, int mode)
+    {
+        SellView<T> A = matrix<T>();
+        with_n(c.n, [&](auto NC) {
+            hipLaunchKernelGGL((k_sresidual<T, decltype(NC)::value>), k.nblk, KB, 0, c.stream, A, sg, mode, x, b, y);
+        });
+        MHFEM_HIP(hipGetLastError());
+    }
+    template <typename T>
+    void spmv_plain(const T *x, T *y)
+    {
+        SellView<T> A = matrix<T>();
+        with_n(c.n, [&](auto NC) {
+            hipLaunchKernelGGL((k_spmv_plain<T, decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, A, x, y);
+        });
+        MHFEM_HIP(hipGetLastError());
+    }
+    template <typename T>
+    void residual(const T *x, const T *b, T *y)
+    {
+        SellView<T> A = matrix<T>();
+        with_n(c.n, [&](auto NC) {
+            hipLaunchKernelGGL((k_residual<T, decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, A, x, b, y);
+        });
+        MHFEM_HIP(hipGetLastError());
+    }
+    void residual_mixed(const double *x, const double *b, float *r32)
+    {
+        SellView<double> A = matrix<double>();
+        with_n(c.n, [&](auto NC) {
+            hipLaunchKernelGGL((k_residual_mixed<decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, A, x, b, r32);
+        });
+        MHFEM_HIP(hipGetLastError());
+    }
 
     void set_scalars(double tol, int maxit)
     {
@@ -647,18 +806,16 @@ struct Solve {
     template <typename T, int ND>
     void pmatvec(const T *x, T *y, T *tmp, const T *dinv, const T *z, int fin)
     {
-        SellView<T> A = matrix<T>();
         if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
-            hipLaunchKernelGGL((k_sspmv<T, false, 0>), k.nblk, KB, 0, c.stream, A, sg, (int)M_ACTIVE, x,
-                               (const T *)nullptr, tmp, (const T *)nullptr);
+            sspmv<T, false, 0>(x, (const T *)nullptr, tmp, (const T *)nullptr, M_ACTIVE);
             ilu_apply<T>(tmp, y, true);
             if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, M_ACTIVE);
             if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, M_ACTIVE);
         } else {
             if (jacobi_scales())
-                hipLaunchKernelGGL((k_sspmv<T, true, ND>), k.nblk, KB, 0, c.stream, A, sg, (int)M_ACTIVE, x, dinv, y, z);
+                sspmv<T, true, ND>(x, dinv, y, z, M_ACTIVE);
             else
-                hipLaunchKernelGGL((k_sspmv<T, false, ND>), k.nblk, KB, 0, c.stream, A, sg, (int)M_ACTIVE, x, dinv, y, z);
+                sspmv<T, false, ND>(x, dinv, y, z, M_ACTIVE);
             if constexpr (ND > 0) finish<ND>(fin, M_ACTIVE);
         }
         MHFEM_HIP(hipGetLastError());
@@ -669,8 +826,7 @@ struct Solve {
     template <typename T>
     void presidue(const T *x, const T *b, T *r, T *tmp, const T *dinv, int mode, T *scratch = nullptr)
     {
-        hipLaunchKernelGGL((k_sresidual<T>), k.nblk, KB, 0, c.stream, matrix<T>(), sg, mode, x, b, tmp);
-        MHFEM_HIP(hipGetLastError());
+        sresidual<T>(x, b, tmp, mode);
         if (c.cfg.preconditioner == MHFEM_PREC_ILU0 && mode != M_ALL) {
             if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0) needs a scratch vector"};
             ilu_apply<T>(tmp, scratch, false);
@@ -749,17 +905,6 @@ struct Solve {
     // GMRES(m), FP64, one segment
     int gmres(const double *b, double *x, double tol, int maxit, int m, double *resid_out);
 };
-
-template <>
-SellView<double> Solve::matrix<double>()
-{
-    return SellView<double>{c.rows, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, c.sell_val.p};
-}
-template <>
-SellView<float> Solve::matrix<float>()
-{
-    return SellView<float>{c.rows, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, k.val32.p};
-}
 
 // ---- GMRES pieces: partial dots of w with V_0..V_{j1-1}, one row of partials per vector
 __global__ void __launch_bounds__(KB) k_gs_dots(int n, int j1, const double *__restrict__ V, const double *__restrict__ w,
@@ -923,7 +1068,7 @@ static void precond_update(Ctx &c, Krylov &k)
     // the diagonal is 1 by construction: the Jacobi scaling is 1 / 1
     hipLaunchKernelGGL((k_fill_t<double>), g, KB, 0, c.stream, (long)n, k.dinv.p, 1.0);
     MHFEM_HIP(hipGetLastError());
-    const bool mixed = c.cfg.precision == MHFEM_PRECISION_FP32_REFINED;
+    const bool mixed = c.cfg.precision != MHFEM_PRECISION_FP64;
     if (mixed) {
         hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.sell_size), KB, 0, c.stream, c.sell_size,
                            c.sell_val.p, k.val32.p);
@@ -952,16 +1097,14 @@ void spmv64(Ctx &c, const double *x, double *y)
 {
     Krylov &k = *c.krylov;
     Solve S(c, k);
-    hipLaunchKernelGGL((k_spmv_plain<double>), grid_for(k.n), KB, 0, c.stream, S.matrix<double>(), x, y);
-    MHFEM_HIP(hipGetLastError());
+    S.spmv_plain<double>(x, y);
 }
 
 void residual64(Ctx &c, const double *x, const double *b, double *r)
 {
     Krylov &k = *c.krylov;
     Solve S(c, k);
-    hipLaunchKernelGGL((k_residual<double>), grid_for(k.n), KB, 0, c.stream, S.matrix<double>(), x, b, r);
-    MHFEM_HIP(hipGetLastError());
+    S.residual<double>(x, b, r);
 }
 
 double dot64(Ctx &c, const double *x, const double *y)
@@ -982,17 +1125,16 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
     Solve S(c, k);
     const unsigned g = grid_for(k.n);
     hipEvent_t e0 = c.ev[12], e1 = c.ev[13];
-    if (precision == MHFEM_PRECISION_FP32_REFINED) {
+    if (precision != MHFEM_PRECISION_FP64) {
         if (!k.val32.p) throw Error{MHFEM_E_STATE, "FP32 matrix not configured"};
         hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)k.n, c.arr[MHFEM_Z_iF].p, k.x32.p);
     }
     for (int rep = -2; rep < reps; rep++) {
         if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
-        if (precision == MHFEM_PRECISION_FP32_REFINED)
-            hipLaunchKernelGGL((k_spmv_plain<float>), g, KB, 0, c.stream, S.matrix<float>(), k.x32.p, k.tmp32.p);
+        if (precision != MHFEM_PRECISION_FP64)
+            S.spmv_plain<float>(k.x32.p, k.tmp32.p);
         else
-            hipLaunchKernelGGL((k_spmv_plain<double>), g, KB, 0, c.stream, S.matrix<double>(), c.arr[MHFEM_Z_iF].p,
-                               k.tmp.p);
+            S.spmv_plain<double>(c.arr[MHFEM_Z_iF].p, k.tmp.p);
     }
     MHFEM_HIP(hipGetLastError());
     MHFEM_HIP(hipEventRecord(e1, c.stream));
@@ -1019,10 +1161,8 @@ int krylov_autotune(Ctx &c, double *best_ms)
         const double *x = c.arr[MHFEM_Z_iF].p;
         for (int rep = -2; rep < 10; rep++) {
             if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
-            hipLaunchKernelGGL((k_sspmv<double, false, 2>), k.nblk, KB, 0, c.stream, S.matrix<double>(), S.sg,
-                               (int)M_ALL, x, (const double *)nullptr, k.tmp.p, x);
+            S.sspmv<double, false, 2>(x, (const double *)nullptr, k.tmp.p, x, M_ALL);
         }
-        MHFEM_HIP(hipGetLastError());
         MHFEM_HIP(hipEventRecord(e1, c.stream));
         MHFEM_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -1084,11 +1224,48 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         S.poll();
         double b_norm = std::sqrt(k.h_sc[S_D0]);
         if (b_norm == 0.0) b_norm = 1.0;
-        const double inner_tol = c.cfg.inner_residue > 0 ? c.cfg.inner_residue : 1e-4;
+        const bool reliable = c.cfg.precision == MHFEM_PRECISION_FP32_RELIABLE;
+        const double inner_tol = c.cfg.inner_residue > 0 ? c.cfg.inner_residue : (reliable ? 1e-2 : 1e-4);
         int total = 0, outer = 0;
         double res = 0;
-        for (;;) {
-            hipLaunchKernelGGL(k_residual_mixed, g, KB, 0, c.stream, S.matrix<double>(), x, b, rhs32);
+        for (; reliable;) {
+            // One FP32 recurrence with reliable updates (Sleijpen & van der
+            // Vorst 1996): each time its residual has dropped by inner_tol,
+            // the FP32 correction goes into the FP64 solution, the FP64
+            // residual replaces the recursive one and the recurrence goes on
+            // with its search direction and shadow residual.
+            float *r = k.r32.p;
+            S.residual_mixed(x, b, rhs32);
+            S.prec<float>(rhs32, z, k.dinv32.p, M_ALL);
+            S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, M_ALL);
+            S.poll();
+            res = std::sqrt(k.h_sc[S_D0]) / b_norm;
+            if (!(res >= tol)) break;
+            hipLaunchKernelGGL((k_fill_t<float>), g, KB, 0, c.stream, (long)n, k.x32.p, 0.0f);
+            S.bicg_start<float>(rhs32, k.x32.p, v, std::max(inner_tol, tol / res), maxit);
+            hipLaunchKernelGGL(k_set_slots, 1, 64, 0, c.stream, k.sc.p, tol * b_norm, inner_tol);
+            MHFEM_HIP(hipGetLastError());
+            const int every = std::max(1, c.cfg.check_every);
+            for (;;) {
+                total = S.bicg_drive<float>(rhs32, k.x32.p, v,
+                                            [&]() { S.bicg_iterations<float>(k.x32.p, v, every); });
+                hipLaunchKernelGGL(k_add_correction, g, KB, 0, c.stream, n, x, k.x32.p);
+                S.residual_mixed(x, b, rhs32);
+                MHFEM_HIP(hipGetLastError());
+                S.prec<float>(rhs32, r, k.dinv32.p, M_ALL);
+                S.dots<float, 2>(DotArgs<float, 2>{{r, r}, {r, k.r_ast32.p}}, FIN_REPLACE, M_ALL);
+                S.poll();
+                outer++;
+                res = std::sqrt(k.h_sc[S_D0]) / b_norm;
+                if (!(res >= tol) || total >= maxit) break;
+                hipLaunchKernelGGL((k_bicg_p<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)r, k.p32.p,
+                                   (const float *)k.Ap32.p);
+                MHFEM_HIP(hipGetLastError());
+            }
+            break;
+        }
+        for (; !reliable;) {
+            S.residual_mixed(x, b, rhs32);
             S.prec<float>(rhs32, z, k.dinv32.p, M_ALL);
             S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, M_ALL);
             S.poll();

@@ -495,23 +495,21 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
             c1.load(s, i, K1, e1, E);
             const double d = c0.A(adv, i, e0, true) + c1.A(adv, i, e1, true);
             // the diagonal entry is exactly 1 (LinearSystem.h:196-201) and is
-            // not stored: k counts the off-diagonal entries of the row
+            // not stored; k counts the blocks of the other faces
             int k = 0;
             for (int t = 0; t < nslots; t++) {
                 const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
+                if (f0 < 7 && f1 < 7) {  // the face E itself (the only face both cells share)
 #pragma unroll
-                for (int j = 0; j < N; j++) {
-                    double val;
-                    if (f0 < 7 && f1 < 7) {
-                        if (j == i) continue;
-                        val = (c0.A(adv, j, f0, true) + c1.A(adv, j, f1, true)) / d;
-                    } else if (f0 < 7) {
-                        val = c0.A(adv, j, f0, false) / d;
-                    } else {
-                        val = c1.A(adv, j, f1, false) / d;
-                    }
-                    s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = val;
+                    for (int j = 0; j < N; j++)
+                        if (j != i)
+                            s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] =
+                                (c0.A(adv, j, f0, true) + c1.A(adv, j, f1, true)) / d;
+                    continue;
                 }
+                double *blk = s.A.val + sell_block(s.A.slice_ptr, E, k++) * (N * N) + i * N;
+#pragma unroll
+                for (int j = 0; j < N; j++) blk[j] = (f0 < 7 ? c0.A(adv, j, f0, false) : c1.A(adv, j, f1, false)) / d;
             }
             double rhs = 0.0;
             rhs += c0.RHS(s, adv, i, E);
@@ -545,11 +543,15 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
         int k = 0;
         for (int t = 0; t < nslots; t++) {
             const int f = (info >> (6 * t)) & 7;
+            if (f == e0) {  // unit diagonal, not stored (BoundaryConditions.hpp:200-203)
 #pragma unroll
-            for (int j = 0; j < N; j++) {
-                if (j == i && f == e0) continue;  // unit diagonal, not stored (BoundaryConditions.hpp:200-203)
-                s.A.val[sell_pos(s.A.slice_ptr, row, k++)] = BA(j, f, f == e0) / d;
+                for (int j = 0; j < N; j++)
+                    if (j != i) s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = BA(j, f, true) / d;
+                continue;
             }
+            double *blk = s.A.val + sell_block(s.A.slice_ptr, E, k++) * (N * N) + i * N;
+#pragma unroll
+            for (int j = 0; j < N; j++) blk[j] = BA(j, f, false) / d;
         }
         double bval;
         if (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) {

@@ -23,7 +23,7 @@ ARRAY_ID = {name: k for k, name in enumerate(ARRAYS)}
 
 SOLVERS = {"bicgstab": 0, "gmres": 1}
 PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2}
-PRECISIONS = {"fp64": 0, "fp32-refined": 1}
+PRECISIONS = {"fp64": 0, "fp32-refined": 1, "fp32-reliable": 2}
 
 
 class MeshDesc(C.Structure):
