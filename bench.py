@@ -1,25 +1,40 @@
 #!/usr/bin/env python3
 """Time steps/s of the MHFEM per-time-step path on one MI355X.
 
-Workload: 3-D Darcy-type pressure on the unit cube, 64^3 cubes x 6 Kuhn
-tetrahedra (1,572,864 cells, 3,170,304 faces = DOFs), n=1, N=0.2, m=1,
-D = K with ln K a seeded Gaussian field (exponential covariance, sigma=1,
-correlation length 0.1), dt=1e-3, BiCGStab to 1e-10 relative residual.
-A "step" is one time step: preIterate + assembleLinearSystem +
-solveLinearSystem + postIterate (control.h:74-101), all inputs resident in HBM.
+Workload: n=2 coupled advection-diffusion-reaction on the unit cube, 64^3
+cubes x 6 Kuhn tetrahedra (1,572,864 cells, 3,170,304 faces, 6,340,608
+DOFs), D11=D22=K, D12=D21=0.1K with ln K a seeded Gaussian field
+(exponential covariance, sigma=1, correlation length 0.1), a_ii=(1,.5,.25)
+explicit upwind, r=[[1e-2,-5e-3],[-5e-3,1e-2]], Dirichlet (1, 0.5) at x=0,
+dt=1e-3, every solve to 1e-10 relative residual.  A "step" is one time step:
+preIterate + assembleLinearSystem + solveLinearSystem + postIterate
+(control.h:74-101), all inputs resident in HBM.  The trajectory is the
+workload's 50 steps: `warmup` untimed steps, then `steps` timed ones.
 
-Two forms are timed on the same workload from the same initial state:
-  plain  -- FP64 BiCGStab + ILU(0), the configured reference method;
-  chosen -- the fastest form that still stops at 1e-10 relative residual in
-            FP64 (FP32 inner BiCGStab inside FP64 iterative refinement).
-`value` is the chosen form's steps/s; the plain form is reported beside it.
+Forms, each run from the same initial state on a fresh solver:
+  chosen     -- learned initial guess (per-face MLP trained online during the
+                first steps, combined with the previous increments by least
+                residual) + FP32 BiCGStab whose residual is replaced by the
+                FP64 one every 10x drop (FP64 iterative refinement with
+                reliable updates), Jacobi (the identity on the unit-diagonal
+                rows).  `value` is its steps/s.
+  plain      -- FP64 BiCGStab + Jacobi, previous solution as initial guess:
+                the fastest plain FP64 form on the GPU.
+  plain_ilu0 -- FP64 BiCGStab + ILU(0), the reference's configured method
+                (bounded sample: its level-scheduled triangular solves make a
+                step take seconds).
+  chosen_without_network -- the chosen form with the network switched off
+                (previous increments only): what the trained network adds.
 
-roofline: the SpMV (the kernel that dominates the chosen form) is timed live
-with HIP events on the library's stream; achieved = algorithmic bytes per
-launch / average launch time.  cpu_baseline: the C oracle (a restatement of
-the reference host path) on a bounded sample of the same workload.
+roofline: the dominant kernel is the FP32 block-SELL SpMV of the chosen
+form's iteration.  Every 8th launch during the chosen form's timed steps is
+bracketed by HIP events on the library's stream (timing level 2);
+achieved = algorithmic bytes per launch / average launch duration.
+cpu_baseline: the C oracle (a restatement of the reference host path), FP64
+BiCGStab + ILU(0), on a bounded sample of one step of the same workload.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -37,22 +52,36 @@ def log(*a):
 
 
 def spmv_bytes(rows, nnz, fp32):
-    # algorithmic bytes of one CSR SpMV with int32 columns: values + columns,
-    # row pointers, x read once, y written once
+    """Algorithmic bytes of one SpMV: CSR with int32 columns -- values and
+    columns, row pointers, x read once, y written once."""
     if fp32:
         return 8 * nnz + 4 * (rows + 1) + 8 * rows
     return 12 * nnz + 4 * (rows + 1) + 16 * rows
 
 
-def time_form(pr, form, steps, warmup):
+def block_sell_bytes(n, faces, blocks_per_face, fp32):
+    """Bytes the block-SELL SpMV must move for the same product: per stored
+    off-diagonal block n*n values and one column index, per face the n(n-1)
+    off-diagonal entries of its own block, x read once, y written once."""
+    w = 4 if fp32 else 8
+    rows = n * faces
+    return blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
+
+
+def run_form(pr, form, steps, warmup, timing=0):
     from mhfem_amd import capi
 
     s = capi.Solver(pr, **form["solver"])
+    if form.get("guess"):
+        s.configure_guess(True, **form["guess"])
     t = 0.0
     for _ in range(warmup):
         s.step(t, pr.dt)
         t += pr.dt
     s.synchronize()
+    if timing:
+        s.set_timing(timing)
+        s.reset_timers()
     its0 = len(s.iterations)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -60,116 +89,183 @@ def time_form(pr, form, steps, warmup):
         t += pr.dt
     s.synchronize()
     dt = time.perf_counter() - t0
-    its = s.iterations[its0:]
-    return s, dt, its
+    return s, dt, s.iterations[its0:]
 
 
-def cpu_baseline(pr, sample_steps):
+def summary(name, steps, dt, its):
+    return {
+        "form": name,
+        "steps_per_s": round(steps / dt, 4),
+        "ms_per_step": round(1e3 * dt / steps, 3),
+        "iterations_per_step": round(sum(its) / max(1, len(its)), 2),
+    }
+
+
+def cpu_baseline(pr, iters_per_step, sample_iters):
+    """Oracle step time = preIterate + assembly + ILU(0) update +
+    iterations_per_step x (time of one FP64 BiCGStab+ILU(0) iteration) +
+    postIterate, the iteration time measured over `sample_iters` iterations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     threads = min(16, len(os.sched_getaffinity(0)))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = pyoracle.Oracle(pr)
-    t = 0.0
+    L = pyoracle.lib()
+    it, res = C.c_int(0), C.c_double(0)
+
+    def solve(maxit):
+        t0 = time.perf_counter()
+        L.or_solve(o.p, pyoracle.SOLVERS["bicgstab"], pyoracle.PRECONDITIONERS["ilu0"], 1e-10, maxit, 30,
+                   C.byref(it), C.byref(res))
+        return time.perf_counter() - t0
+
     t0 = time.perf_counter()
-    its = []
-    for _ in range(sample_steps):
-        it, _ = o.step(t, pr.dt)
-        its.append(it)
-        t += pr.dt
-    dt = time.perf_counter() - t0
+    o.pre_iterate(0.0, pr.dt)
+    o.assemble(0.0, pr.dt)
+    t_pre_asm = time.perf_counter() - t0
+    t1 = solve(1)                   # ILU(0) factorization + 1 iteration
+    t2 = solve(1 + sample_iters)    # ILU(0) factorization + 1 + sample_iters iterations
+    t_it = (t2 - t1) / sample_iters
+    t_factor = max(0.0, t1 - t_it)
+    t0 = time.perf_counter()
+    o.post_iterate(0.0, pr.dt)
+    t_post = time.perf_counter() - t0
+    step_s = t_pre_asm + t_factor + iters_per_step * t_it + t_post
     return {
-        "value": round(sample_steps / dt, 5),
+        "value": round(1.0 / step_s, 6),
         "unit": "time steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {sample_steps} time step(s) of the same 64^3 Darcy workload, FP64 BiCGStab+ILU(0) "
-                  f"to 1e-10 ({its} iterations), C oracle with OpenMP on {threads} host threads "
-                  f"(ILU(0) triangular solves sequential)",
+        "sample": (f"first step of the same workload on the C oracle (OpenMP, {threads} host threads; "
+                   f"ILU(0) factorization and triangular solves sequential): preIterate+assembly "
+                   f"{t_pre_asm:.2f} s, ILU(0) update {t_factor:.2f} s, {sample_iters} FP64 BiCGStab+ILU(0) "
+                   f"iterations timed at {t_it:.3f} s each and scaled to {iters_per_step:.0f} iterations/step "
+                   f"(the GPU's count for the same method), postIterate {t_post:.2f} s"),
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=64, help="cubes per edge")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--delta", type=float, default=0.1, help="FP32 residual drop per FP64 replacement")
+    ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--ilu-steps", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-ilu", action="store_true")
+    ap.add_argument("--skip-ablation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--chosen-prec", default="diagonal")
-    ap.add_argument("--inner-tol", type=float, default=1e-3)
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
-                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it "
+                                   "(DESIGN.md describes how the path would shard)", "n_gpus": args.gpus}))
         return 2
 
+    import numpy as np
     from mhfem_amd import problems
 
-    log(f"[bench] building the {args.size}^3 Darcy workload")
-    pr = problems.darcy_3d(s=args.size, seed=args.seed, preconditioner="ilu0")
+    log(f"[bench] building the n=2 {args.size}^3 workload")
+    pr = problems.adr_3d(s=args.size, seed=args.seed, preconditioner="diagonal")
     rows = pr.num_faces * pr.n
-
-    chosen = {"name": f"FP32 inner BiCGStab+{args.chosen_prec} in FP64 refinement",
-              "solver": dict(preconditioner=args.chosen_prec, precision="fp32-refined",
-                             inner_tol=args.inner_tol, check_every=8, tol=1e-10)}
-    plain = {"name": "FP64 BiCGStab+ILU(0)",
-             "solver": dict(preconditioner="ilu0", precision="fp64", check_every=4, tol=1e-10)}
+    guess = dict(train_steps=args.train_steps, epochs=10, learning_rate=2e-3, sample_stride=8, combine="combined")
+    chosen = {"name": "learned guess + FP32 BiCGStab with FP64 reliable updates (Jacobi)",
+              "solver": dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=args.delta,
+                             check_every=8, tol=1e-10),
+              "guess": guess}
+    plain = {"name": "FP64 BiCGStab + Jacobi",
+             "solver": dict(preconditioner="diagonal", precision="fp64", check_every=8, tol=1e-10)}
+    plain_ilu = {"name": "FP64 BiCGStab + ILU(0)",
+                 "solver": dict(preconditioner="ilu0", precision="fp64", check_every=4, tol=1e-10)}
+    ablation = {"name": "chosen form without the network (previous increments only)",
+                "solver": chosen["solver"], "guess": dict(guess, train_steps=0, combine="history")}
 
     log(f"[bench] chosen form: {chosen['name']}")
-    s, dt, its = time_form(pr, chosen, args.steps, args.warmup)
-    ms_step = 1e3 * dt / args.steps
-    rp, col, val, rhs = s.export_system()
+    s, dt, its = run_form(pr, chosen, args.steps, args.warmup, timing=2)
+    tm = s.timers()
+    spmv_ms = tm["spmv_ms"] / max(1, tm["spmv_timed"])
+    gs = s.guess_stats()
+    rp, col, _, _ = s.export_system()
     nnz = int(rp[-1])
-    spmv_ms = s.benchmark_spmv("fp32-refined", reps=50)
-    spmv64_ms = s.benchmark_spmv("fp64", reps=50)
-    log(f"[bench] chosen: {ms_step:.2f} ms/step, iterations {its}; spmv fp32 {spmv_ms*1e3:.1f} us, "
-        f"fp64 {spmv64_ms*1e3:.1f} us")
+    # stored off-diagonal blocks per face: the distinct other faces in each face's rows
+    faces = pr.num_faces
+    fc = col.astype(np.int64) // pr.n
+    rowface = np.repeat(np.arange(rows, dtype=np.int64) // pr.n, np.diff(rp))
+    pairs = np.unique(rowface * faces + fc)
+    blocks_per_face = (pairs.size - faces) / faces
+    z_chosen = s.get_array("Z_iF").copy()
     s.close()
+    res_chosen = summary(chosen["name"], args.steps, dt, its)
+    log(f"[bench] chosen: {res_chosen}; SpMV {spmv_ms*1e3:.1f} us over {tm['spmv_timed']} timed launches; "
+        f"guess {gs}")
 
-    result_plain = None
+    res_plain = z_plain = None
     if not args.skip_plain:
         log(f"[bench] plain form: {plain['name']}")
-        sp, dtp, itsp = time_form(pr, plain, args.steps, args.warmup)
-        result_plain = {
-            "form": plain["name"],
-            "steps_per_s": round(args.steps / dtp, 4),
-            "ms_per_step": round(1e3 * dtp / args.steps, 3),
-            "iterations_per_step": round(sum(itsp) / len(itsp), 2),
-        }
-        log(f"[bench] plain: {result_plain}")
+        sp, dtp, itsp = run_form(pr, plain, args.steps, args.warmup)
+        z_plain = sp.get_array("Z_iF").copy()
         sp.close()
+        res_plain = summary(plain["name"], args.steps, dtp, itsp)
+        log(f"[bench] plain: {res_plain}")
 
-    achieved = spmv_bytes(rows, nnz, True) / (spmv_ms * 1e-3) / 1e9
+    res_ilu = None
+    if not args.skip_ilu:
+        log(f"[bench] plain ILU(0) form (sample of {args.ilu_steps} steps after 1 warmup step)")
+        si, dti, itsi = run_form(pr, plain_ilu, args.ilu_steps, 1)
+        si.close()
+        res_ilu = summary(plain_ilu["name"], args.ilu_steps, dti, itsi)
+        res_ilu["sample"] = f"steps 2-{1 + args.ilu_steps} of the trajectory"
+        log(f"[bench] plain ILU(0): {res_ilu}")
+
+    res_abl = None
+    if not args.skip_ablation:
+        log(f"[bench] ablation: {ablation['name']}")
+        sa, dta, itsa = run_form(pr, ablation, args.steps, args.warmup)
+        sa.close()
+        res_abl = summary(ablation["name"], args.steps, dta, itsa)
+        log(f"[bench] ablation: {res_abl}")
+
+    value = args.steps / dt
+    alg = spmv_bytes(rows, nnz, True)
+    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True)
+    achieved = alg / (spmv_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(args.steps / dt, 4),
+        "value": round(value, 4),
         "unit": "time steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 refinement",
-        "data": "synthetic (seeded Gaussian ln K field, structured Kuhn mesh)",
+        "dtype": "f32 Krylov iteration / f64 residual, correction and scheme",
+        "data": "synthetic (seeded Gaussian ln K field on a structured Kuhn mesh)",
         "config": {
-            "workload": f"3-D Darcy pressure, {args.size}^3 cubes x 6 Kuhn tets, n=1, dt=1e-3, 1e-10 rel. residual",
-            "cells": pr.num_cells, "faces": pr.num_faces, "dofs": rows, "nnz": nnz,
+            "workload": f"n=2 advection-diffusion-reaction, {args.size}^3 cubes x 6 Kuhn tets, dt=1e-3, "
+                        f"{args.warmup + args.steps}-step trajectory, 1e-10 rel. residual",
+            "cells": pr.num_cells, "faces": faces, "dofs": rows, "nnz": nnz,
             "form": chosen["name"],
         },
-        "iterations_per_step": round(sum(its) / len(its), 2),
-        "dof_per_s": round(rows * args.steps / dt, 1),
-        "plain": result_plain,
-        "chosen_vs_plain": (round((args.steps / dt) / result_plain["steps_per_s"], 3) if result_plain else None),
+        "iterations_per_step": res_chosen["iterations_per_step"],
+        "dof_per_s": round(rows * value, 1),
+        "plain": res_plain,
+        "plain_ilu0": res_ilu,
+        "chosen_without_network": res_abl,
+        "chosen_vs_plain": round(value / res_plain["steps_per_s"], 3) if res_plain else None,
+        "chosen_vs_plain_ilu0": round(value / res_ilu["steps_per_s"], 3) if res_ilu else None,
+        "chosen_vs_fp64_rel_l2": (float(np.linalg.norm(z_chosen - z_plain) / np.linalg.norm(z_plain))
+                                  if z_plain is not None else None),
+        "guess": {"network_coefficient": round(gs["coef_network"], 4), "trained_steps": gs["trained_steps"],
+                  "used": gs["used"]},
         "roofline": {
-            "kernel": "k_spmv<float> (SELL-64 SpMV of the FP32 inner solve)",
+            "kernel": f"k_sspmv<float, {pr.n}, ...> (block-SELL SpMV of the FP32 iteration)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -177,14 +273,18 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "avg_launch_us": round(spmv_ms * 1e3, 2),
-            "fp64_spmv_GBs": round(spmv_bytes(rows, nnz, False) / (spmv64_ms * 1e-3) / 1e9, 1),
+            "timed_launches": tm["spmv_timed"],
+            "algorithmic_bytes": alg,
+            "format_bytes": int(fmt),
+            "format_GBs": round(fmt / (spmv_ms * 1e-3) / 1e9, 1),
         },
         "cpu_baseline": None,
     }
     if not args.skip_cpu:
         log("[bench] cpu baseline (oracle)")
-        out["cpu_baseline"] = cpu_baseline(problems.darcy_3d(s=args.size, seed=args.seed, preconditioner="ilu0"),
-                                           args.cpu_steps)
+        per_step = res_ilu["iterations_per_step"] if res_ilu else res_chosen["iterations_per_step"]
+        out["cpu_baseline"] = cpu_baseline(problems.adr_3d(s=args.size, seed=args.seed, preconditioner="ilu0"),
+                                           per_step, args.cpu_iters)
     print(json.dumps(out))
     return 0
 

@@ -119,6 +119,10 @@ typedef struct {
     double pre_iterate, update_b_R_Q, upwind, assemble, preconditioner, linear_solver,
         post_iterate;
     long long iterations, time_steps;
+    /* timing level 2: every 8th SpMV launch of the Krylov iteration is
+       bracketed by HIP events on the context's stream */
+    double spmv_ms;       /* summed duration of the timed SpMV launches */
+    long long spmv_timed; /* number of timed SpMV launches */
 } mhfem_timers;
 
 /* ---- lifetime (Solver::setMesh, Solver.hpp:26-81; BaseModel::allocate, BaseModel.h:168-195) */
@@ -247,7 +251,8 @@ int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *
 int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms);
 int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out);
 int mhfem_reset_timers(mhfem_ctx *ctx);
-int mhfem_set_timing(mhfem_ctx *ctx, int enabled); /* HIP-event phase timers (default off) */
+int mhfem_set_timing(mhfem_ctx *ctx, int level); /* 0 off (default), 1 HIP-event phase timers,
+                                                    2 phase timers + sampled SpMV launch timing */
 int mhfem_synchronize(mhfem_ctx *ctx);
 
 #ifdef __cplusplus

@@ -245,6 +245,8 @@ SchemeView Ctx::view()
     s.points = points.p;
     s.face_cells = face_cells.p;
     s.face_slots = face_slots.p;
+    s.bnd_faces = bnd_faces.p;
+    s.nbnd = nbnd;
     s.cell_meas = arr[MHFEM_CELL_MEASURE].p;
     s.face_meas = arr[MHFEM_FACE_MEASURE].p;
     auto coef = [&](int w) { return Coef{arr[w].p, uniform[w] ? 0L : ~0L}; };
@@ -333,6 +335,10 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
         upload(c->points, mesh->points, (size_t)c->nv * c->dim);
         std::vector<uint64_t> slots = build_face_slots(*c);
         upload(c->face_slots, slots.data(), slots.size());
+        std::vector<int> bfaces;
+        for (int E = 0; E < c->F; E++)
+            if (c->h_face_cells[2 * (size_t)E + 1] < 0) bfaces.push_back(E);
+        upload(c->bnd_faces, bfaces.data(), bfaces.size());
         upload(c->arr[MHFEM_CELL_MEASURE], cm.data(), cm.size());
         upload(c->arr[MHFEM_FACE_MEASURE], fm.data(), fm.size());
         // model arrays (BaseModel::allocate, BaseModel.h:168-195)
@@ -439,6 +445,7 @@ void mhfem_destroy(mhfem_ctx *ctx)
     guess_destroy(c->guess);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->kt_ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -928,8 +935,14 @@ int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms
 
 int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out)
 {
-    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    Ctx *c = const_cast<Ctx *>(reinterpret_cast<const Ctx *>(ctx));
     if (!c || !out) return MHFEM_E_ARGUMENT;
+    try {
+        kernel_timing_flush(*c);
+    } catch (const Error &e) {
+        c->err = e.msg;
+        return e.code;
+    }
     *out = c->timers;
     return MHFEM_OK;
 }
@@ -938,6 +951,7 @@ int mhfem_reset_timers(mhfem_ctx *ctx)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return MHFEM_E_ARGUMENT;
+    c->kt_used = 0;
     c->timers = mhfem_timers{};
     return MHFEM_OK;
 }
@@ -946,7 +960,9 @@ int mhfem_set_timing(mhfem_ctx *ctx, int enabled)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return MHFEM_E_ARGUMENT;
+    if (enabled < 0 || enabled > 2) return MHFEM_E_ARGUMENT;
     c->timing = enabled != 0;
+    c->timing_level = enabled;
     return MHFEM_OK;
 }
 

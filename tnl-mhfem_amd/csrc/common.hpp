@@ -73,6 +73,23 @@ __host__ __device__ inline int64_t sell_diag(int64_t diag_base, int n, int face,
     return diag_base + (int64_t)face * n * (n - 1) + i * (n - 1) + (j < i ? j : j - 1);
 }
 
+// M consecutive values moved as one vector access (16-byte aligned where the
+// size allows; the storage offsets keep that alignment)
+template <typename T, int M>
+struct alignas((sizeof(T) * M) % 16 == 0 ? 16 : ((sizeof(T) * M) % 8 == 0 ? 8 : sizeof(T))) Pack {
+    T v[M];
+};
+template <typename T, int M>
+__device__ __forceinline__ Pack<T, M> ld(const T *p)
+{
+    return *reinterpret_cast<const Pack<T, M> *>(p);
+}
+template <typename T, int M>
+__device__ __forceinline__ void st(T *p, const Pack<T, M> &v)
+{
+    *reinterpret_cast<Pack<T, M> *>(p) = v;
+}
+
 // All device pointers of the scheme, passed to kernels by value.
 struct SchemeView {
     int C, F, dim, fpc, S;
@@ -83,6 +100,8 @@ struct SchemeView {
     const double *points;       // nv*dim
     const int *face_cells;      // F*2
     const uint64_t *face_slots; // per face: merged column order (see build_pattern)
+    const int *bnd_faces;       // the boundary faces, ascending
+    int nbnd;
     const double *cell_meas, *face_meas;
     // coefficients
     Coef N, u, m, D, w, a, r, f;
@@ -134,7 +153,7 @@ struct Ctx {
     // segments (batched ensembles): cells, faces and rows of each instance
     std::vector<int> h_seg_cells, h_seg_faces, h_seg_rows;
     // device topology
-    DBuf<int> cell_faces, cell_vertices, face_cells;
+    DBuf<int> cell_faces, cell_vertices, face_cells, bnd_faces;
     DBuf<uint64_t> face_slots;
     DBuf<double> points, cell_meas, face_meas;
     // model arrays (index by MHFEM_* ids); uniform coefficients hold 1 value
@@ -164,13 +183,52 @@ struct Ctx {
     bool guess_on_solve = false;  // the learned guess runs in mhfem_solve
     // timers
     bool timing = false;
+    int timing_level = 0;
     hipEvent_t ev[16] = {};
+    // sampled SpMV launch timing (level 2): event pairs awaiting readout
+    std::vector<hipEvent_t> kt_ev;
+    size_t kt_used = 0;
+    long kt_count = 0;
     mhfem_timers timers{};
     std::string err;
 
     long array_size(int which) const;
     SchemeView view();
 };
+
+// Sampled kernel timing: kernel_timing_begin returns true when this launch
+// is to be timed (then it has recorded the opening event), and
+// kernel_timing_end records the closing one; the durations are read out by
+// kernel_timing_flush.
+inline void kernel_timing_flush(Ctx &c)
+{
+    for (size_t q = 0; q + 1 < c.kt_used; q += 2) {
+        MHFEM_HIP(hipEventSynchronize(c.kt_ev[q + 1]));
+        float ms = 0;
+        MHFEM_HIP(hipEventElapsedTime(&ms, c.kt_ev[q], c.kt_ev[q + 1]));
+        c.timers.spmv_ms += ms;
+        c.timers.spmv_timed++;
+    }
+    c.kt_used = 0;
+}
+inline bool kernel_timing_begin(Ctx &c)
+{
+    if (!c.timing || c.timing_level < 2 || (c.kt_count++ & 7) != 0) return false;
+    if (c.kt_used + 2 > c.kt_ev.size()) {
+        if (c.kt_ev.size() < 512) {
+            for (int q = 0; q < 64; q++) {
+                hipEvent_t e;
+                MHFEM_HIP(hipEventCreate(&e));
+                c.kt_ev.push_back(e);
+            }
+        } else {
+            kernel_timing_flush(c);
+        }
+    }
+    MHFEM_HIP(hipEventRecord(c.kt_ev[c.kt_used++], c.stream));
+    return true;
+}
+inline void kernel_timing_end(Ctx &c) { MHFEM_HIP(hipEventRecord(c.kt_ev[c.kt_used++], c.stream)); }
 
 // scheme.hip
 void launch_face_average(Ctx &c);

@@ -196,23 +196,6 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 }
 
 // ---------------------------------------------------------------------------
-// M consecutive values moved as one vector access (16-byte aligned where the
-// size allows; the storage offsets keep that alignment)
-template <typename T, int M>
-struct alignas((sizeof(T) * M) % 16 == 0 ? 16 : ((sizeof(T) * M) % 8 == 0 ? 8 : sizeof(T))) Pack {
-    T v[M];
-};
-template <typename T, int M>
-__device__ __forceinline__ Pack<T, M> ld(const T *p)
-{
-    return *reinterpret_cast<const Pack<T, M> *>(p);
-}
-template <typename T, int M>
-__device__ __forceinline__ void st(T *p, const Pack<T, M> &v)
-{
-    *reinterpret_cast<Pack<T, M> *>(p) = v;
-}
-
 // The N rows of face E: y = x_E + (diagonal block without its unit
 // diagonal) x_E + sum over the stored blocks of B_k x_{col_k}.  One block
 // costs one column index, one N*N vector load and one N-wide gather of x.
@@ -685,11 +668,13 @@ struct Solve {
     void sspmv(const T *x, const T *dinv, T *y, const T *z, int mode)
     {
         SellView<T> A = matrix<T>();
+        const bool timed = kernel_timing_begin(c);
         with_n(c.n, [&](auto NC) {
             hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND>), k.nblk, KB, 0, c.stream, A, sg, mode, x,
                                dinv, y, z);
         });
         MHFEM_HIP(hipGetLastError());
+        if (timed) kernel_timing_end(c);
     }
     template <typename T>
     void sresidual(const T *x, const T *b, T *y, int mode)

@@ -375,193 +375,263 @@ __device__ __forceinline__ double pick(const double (&a)[M], int i)
 // ---------------------------------------------------------------------------
 // Face-parallel hybridised assembly (Solver.hpp:933-959).
 //
-// Per-cell operands of one matrix row, loaded once into registers:
-//   bf[j][f] = b_ijKef(e, f), be[x] = b_ixKe(e), Rv[x][j][f] = R_xjKf.
+// CellRows<N, FPC>::compute evaluates, for one cell K and its local face e
+// (the face E whose rows are assembled), the cell's contributions to all N
+// rows (E, i):
+//   A[i][j][f] = A_ijKEF against unknown j of local face f,
+//   rhs[i]     = RHS_iKE.
+// Every register array is indexed by compile-time constants (the runtime
+// local indices only enter load and store addresses), so nothing is demoted
+// to scratch memory; R_xjKf is loaded once for all N rows.  Ae is selected
+// while the values are computed: a select over an array after the fact is
+// folded by the compiler into a load from a selected address, which keeps
+// the whole array in scratch memory.
 template <int N, int FPC>
-struct CellRow {
-    int K, e;
-    double bf[N][FPC], be[N], Rv[N][N][FPC];
-    double mu;        // m_iE_upw(i, E)
-    double vel[N];    // (a+u)_ijKe(e)
+struct CellRows {
+    double A[N][N][FPC];
+    double Ae[N][N];  // A[i][j][e]: the column of the face E itself
+    double rhs[N];
 
-    __device__ __forceinline__ void load(const SchemeView &s, int i, int K_, int e_, int E)
+    // use_adv: A_ijKEF_advection / RHS_iKE_advection (SecondaryCoefficients.h:215-259,
+    // 301-333); otherwise the no-advection forms (:189-211, 282-297)
+    __device__ __forceinline__ void compute(const SchemeView &s, int adv, bool use_adv, int K, int e, int E)
     {
         constexpr int NP = Simplex<FPC>::NP, S = Simplex<FPC>::S;
-        K = K_;
-        e = e_;
         const long C = s.C;
-#pragma unroll
-        for (int j = 0; j < N; j++) {
-            const double *bp = s.b + ((long)(i * N + j) * S) * C + K;
-#pragma unroll
-            for (int f = 0; f < FPC; f++) bf[j][f] = bp[(long)pk(e, f) * C];
-            be[j] = bp[(long)(NP + e) * C];
-        }
+        double Rv[N][N][FPC];
 #pragma unroll
         for (int x = 0; x < N; x++)
 #pragma unroll
             for (int j = 0; j < N; j++)
 #pragma unroll
                 for (int f = 0; f < FPC; f++) Rv[x][j][f] = s.R_ijKe[(((long)x * N + j) * FPC + f) * C + K];
-        mu = s.m_upw[(long)E * N + i];
+        double RK[N];
 #pragma unroll
-        for (int j = 0; j < N; j++) {
-            const long idx = (((long)i * N + j) * FPC + e) * C + K;
-            vel[j] = s.a[idx] + s.u[idx];
-        }
-    }
-
-    // A_ijKEF_no_advection (SecondaryCoefficients.h:189-211)
-    __device__ __forceinline__ double A_no_adv(int j, int f) const
-    {
-        double SQinvR = 0;
+        for (int j = 0; j < N; j++) RK[j] = s.R_iK[(long)j * C + K];
 #pragma unroll
-        for (int x = 0; x < N; x++) SQinvR += be[x] * pick<FPC>(Rv[x][j], f);
-        return pick<FPC>(bf[j], f) - SQinvR;
-    }
-    // A_ijKEF_advection (SecondaryCoefficients.h:215-259)
-    __device__ __forceinline__ double A_adv(int advection, int j, int f, bool EisF) const
-    {
-        double SQinvR = 0;
+        for (int i = 0; i < N; i++) {
+            double bf[N][FPC], be[N];
 #pragma unroll
-        for (int x = 0; x < N; x++) {
-            double S = mu * be[x];
-            if (advection == MHFEM_IMPLICIT_UPWIND) {
-                if (vel[x] >= 0) S += vel[x];
-                else S -= vel[x];
+            for (int j = 0; j < N; j++) {
+                const double *bp = s.b + ((long)(i * N + j) * S) * C + K;
+#pragma unroll
+                for (int f = 0; f < FPC; f++) bf[j][f] = bp[(long)pk(e, f) * C];
+                be[j] = bp[(long)(NP + e) * C];
             }
-            SQinvR += S * pick<FPC>(Rv[x][j], f);
-        }
-        double T = mu * pick<FPC>(bf[j], f);
-        if (EisF && advection == MHFEM_IMPLICIT_UPWIND && vel[j] < 0) T -= 2 * vel[j];
-        if (EisF && advection == MHFEM_IMPLICIT_TRACE) T -= vel[j];
-        return T - SQinvR;
-    }
-    // A_ijKEF (SecondaryCoefficients.h:261-278)
-    __device__ __forceinline__ double A(int advection, int j, int f, bool EisF) const
-    {
-        return advection == MHFEM_EXPLICIT_UPWIND ? A_no_adv(j, f) : A_adv(advection, j, f, EisF);
-    }
-    // RHS_iKE_no_advection (SecondaryCoefficients.h:282-297)
-    __device__ __forceinline__ double RHS_no_adv(const SchemeView &s, int i) const
-    {
-        const long C = s.C;
-        double value = s.w[((long)i * FPC + e) * C + K];
+            const double wv = s.w[((long)i * FPC + e) * C + K];
+            if (!use_adv) {
 #pragma unroll
-        for (int j = 0; j < N; j++) value += be[j] * s.R_iK[(long)j * C + K];
-        return value;
-    }
-    // RHS_iKE_advection (SecondaryCoefficients.h:301-333)
-    __device__ __forceinline__ double RHS_adv(const SchemeView &s, int advection, int i, int E) const
-    {
-        const long C = s.C;
-        double value = mu * s.w[((long)i * FPC + e) * C + K];
+                for (int j = 0; j < N; j++)
 #pragma unroll
-        for (int j = 0; j < N; j++) {
-            double S = mu * be[j];
-            if (advection == MHFEM_IMPLICIT_UPWIND) {
-                if (vel[j] >= 0) S += vel[j];
-                else S -= vel[j];
+                    for (int f = 0; f < FPC; f++) {
+                        double SQinvR = 0;
+#pragma unroll
+                        for (int x = 0; x < N; x++) SQinvR += be[x] * Rv[x][j][f];
+                        A[i][j][f] = bf[j][f] - SQinvR;
+                        if (f == e) Ae[i][j] = A[i][j][f];
+                    }
+                double v = wv;
+#pragma unroll
+                for (int j = 0; j < N; j++) v += be[j] * RK[j];
+                rhs[i] = v;
+            } else {
+                const double mu = s.m_upw[(long)E * N + i];
+                double vel[N], Sx[N];
+#pragma unroll
+                for (int j = 0; j < N; j++) {
+                    const long idx = (((long)i * N + j) * FPC + e) * C + K;
+                    vel[j] = s.a[idx] + s.u[idx];
+                }
+#pragma unroll
+                for (int x = 0; x < N; x++) {
+                    double Sv = mu * be[x];
+                    if (adv == MHFEM_IMPLICIT_UPWIND) {
+                        if (vel[x] >= 0) Sv += vel[x];
+                        else Sv -= vel[x];
+                    }
+                    Sx[x] = Sv;
+                }
+#pragma unroll
+                for (int j = 0; j < N; j++)
+#pragma unroll
+                    for (int f = 0; f < FPC; f++) {
+                        double SQinvR = 0;
+#pragma unroll
+                        for (int x = 0; x < N; x++) SQinvR += Sx[x] * Rv[x][j][f];
+                        double T = mu * bf[j][f];
+                        if (f == e && adv == MHFEM_IMPLICIT_UPWIND && vel[j] < 0) T -= 2 * vel[j];
+                        if (f == e && adv == MHFEM_IMPLICIT_TRACE) T -= vel[j];
+                        A[i][j][f] = T - SQinvR;
+                        if (f == e) Ae[i][j] = A[i][j][f];
+                    }
+                double v = mu * wv;
+#pragma unroll
+                for (int j = 0; j < N; j++) {
+                    v += Sx[j] * RK[j];
+                    if (adv == MHFEM_EXPLICIT_UPWIND) v += s.Z_upw[((long)E * N + j) * N + i] * vel[j];
+                }
+                rhs[i] = v;
             }
-            value += S * s.R_iK[(long)j * C + K];
-            if (advection == MHFEM_EXPLICIT_UPWIND) value += s.Z_upw[((long)E * N + j) * N + i] * vel[j];
         }
-        return value;
-    }
-    __device__ __forceinline__ double RHS(const SchemeView &s, int advection, int i, int E) const
-    {
-        return advection == MHFEM_EXPLICIT_UPWIND ? RHS_no_adv(s, i) : RHS_adv(s, advection, i, E);
     }
 };
 
+// Block index, in the SELL row of face E, of each local face of K0 (kp0)
+// and K1 (kp1).  The face_slots word lists the merged ascending faces of the
+// row (3 bits per cell and slot: the local index, 7 = not a face of that
+// cell); the face E itself (local index e0 in K0) has no block, so k counts
+// the other slots and stays below the face's stored width.
+template <int FPC>
+__device__ __forceinline__ void slot_blocks(uint64_t info, int nslots, int e0, int (&kp0)[FPC], int (&kp1)[FPC])
+{
+#pragma unroll
+    for (int f = 0; f < FPC; f++) kp0[f] = kp1[f] = 0;
+    int k = 0;
+    for (int t = 0; t < nslots; t++) {
+        const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
+        if (f0 == e0) continue;
+#pragma unroll
+        for (int f = 0; f < FPC; f++) {
+            if (f0 == f) kp0[f] = k;
+            if (f1 == f) kp1[f] = k;
+        }
+        k++;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void store_block_row(const SchemeView &s, int E, int k, int i, const double (&v)[N])
+{
+    Pack<double, N> p;
+#pragma unroll
+    for (int j = 0; j < N; j++) p.v[j] = v[j];
+    st<double, N>(s.A.val + sell_block(s.A.slice_ptr, E, k) * (N * N) + i * N, p);
+}
+
+// Interior faces: LinearSystem::setMatrixElements (LinearSystem.h:66-256)
+// and RightHandSide::getValue (RightHandSide.h:18-45).  Every column face
+// other than E belongs to exactly one of the two cells; the diagonal block
+// and the right-hand side add both cells' contributions (K0 first).
 template <int N, int FPC>
 __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
 {
     const int E = blockIdx.x * WG + threadIdx.x;
     if (E >= s.F) return;
+    const int K1 = s.face_cells[2 * E + 1];
+    if (K1 < 0) return;  // boundary faces: k_assemble_boundary
+    const int K0 = s.face_cells[2 * E];
     const uint64_t info = s.face_slots[E];
-    const int K0 = s.face_cells[2 * E], K1 = s.face_cells[2 * E + 1];
     const int e0 = (info >> 48) & 7, e1 = (info >> 51) & 7;
     const int nslots = (info >> 56) & 15;
+    int kp0[FPC], kp1[FPC];
+    slot_blocks<FPC>(info, nslots, e0, kp0, kp1);
     const int adv = s.advection;
-    if (K1 >= 0) {
-        // interior face: LinearSystem::setMatrixElements (LinearSystem.h:66-256)
-        // and RightHandSide::getValue (RightHandSide.h:18-45)
-#pragma unroll 1
-        for (int i = 0; i < N; i++) {
-            const int row = E * N + i;
-            CellRow<N, FPC> c0, c1;
-            c0.load(s, i, K0, e0, E);
-            c1.load(s, i, K1, e1, E);
-            const double d = c0.A(adv, i, e0, true) + c1.A(adv, i, e1, true);
-            // the diagonal entry is exactly 1 (LinearSystem.h:196-201) and is
-            // not stored; k counts the blocks of the other faces
-            int k = 0;
-            for (int t = 0; t < nslots; t++) {
-                const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
-                if (f0 < 7 && f1 < 7) {  // the face E itself (the only face both cells share)
+    const bool use_adv = adv != MHFEM_EXPLICIT_UPWIND;  // SecondaryCoefficients.h:261-278
+    CellRows<N, FPC> c0, c1;
+    c0.compute(s, adv, use_adv, K0, e0, E);
+    c1.compute(s, adv, use_adv, K1, e1, E);
 #pragma unroll
-                    for (int j = 0; j < N; j++)
-                        if (j != i)
-                            s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] =
-                                (c0.A(adv, j, f0, true) + c1.A(adv, j, f1, true)) / d;
-                    continue;
-                }
-                double *blk = s.A.val + sell_block(s.A.slice_ptr, E, k++) * (N * N) + i * N;
-#pragma unroll
-                for (int j = 0; j < N; j++) blk[j] = (f0 < 7 ? c0.A(adv, j, f0, false) : c1.A(adv, j, f1, false)) / d;
-            }
-            double rhs = 0.0;
-            rhs += c0.RHS(s, adv, i, E);
-            rhs += c1.RHS(s, adv, i, E);
-            s.rhs[row] = rhs / d;
-        }
-        return;
-    }
-    // boundary face: BoundaryConditions::setMatrixElements (BoundaryConditions.hpp:335-508)
-#pragma unroll 1
     for (int i = 0; i < N; i++) {
-        const int row = E * N + i;
+        const double d = c0.Ae[i][i] + c1.Ae[i][i];
+#pragma unroll
+        for (int f = 0; f < FPC; f++) {
+            double v[N];
+            if (f != e0) {
+#pragma unroll
+                for (int j = 0; j < N; j++) v[j] = c0.A[i][j][f] / d;
+                store_block_row<N>(s, E, kp0[f], i, v);
+            }
+            if (f != e1) {
+#pragma unroll
+                for (int j = 0; j < N; j++) v[j] = c1.A[i][j][f] / d;
+                store_block_row<N>(s, E, kp1[f], i, v);
+            }
+        }
+        // the diagonal entry itself is exactly 1 (LinearSystem.h:196-201) and is not stored
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (j != i)
+                s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = (c0.Ae[i][j] + c1.Ae[i][j]) / d;
+        double rhs = 0.0;
+        rhs += c0.rhs[i];
+        rhs += c1.rhs[i];
+        s.rhs[(long)E * N + i] = rhs / d;
+    }
+}
+
+// Boundary faces: BoundaryConditions::setMatrixElements
+// (BoundaryConditions.hpp:335-508), one thread per boundary face.
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_assemble_boundary(SchemeView s)
+{
+    const int q = blockIdx.x * WG + threadIdx.x;
+    if (q >= s.nbnd) return;
+    const int E = s.bnd_faces[q];
+    const int K0 = s.face_cells[2 * E];
+    const uint64_t info = s.face_slots[E];
+    const int e0 = (info >> 48) & 7;
+    const int nslots = (info >> 56) & 15;
+    int kp0[FPC], kp1[FPC];
+    slot_blocks<FPC>(info, nslots, e0, kp0, kp1);
+    const int adv = s.advection;
+    const long C = s.C;
+    // rows of one face may carry different tags: evaluate the forms needed
+    bool any_flux = false, any_plain = false;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int tag = s.tags[(long)i * s.F + E];
+        if (tag == MHFEM_BC_FIXED_FLUX) any_flux = true;
+        else if (tag != MHFEM_BC_FIXED_VALUE) any_plain = true;
+    }
+    CellRows<N, FPC> ca, cn;
+    if (any_flux) ca.compute(s, adv, true, K0, e0, E);
+    if (any_plain) cn.compute(s, adv, false, K0, e0, E);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
         const long bi = (long)i * s.F + E;
         const int tag = s.tags[bi];
         if (tag == MHFEM_BC_FIXED_VALUE) {
             // the row is the (implicit) unit diagonal alone (BoundaryConditions.hpp:373-380)
-            s.rhs[row] = s.bc_dirichlet[bi];
+            s.rhs[(long)E * N + i] = s.bc_dirichlet[bi];
             continue;
         }
-        CellRow<N, FPC> c0;
-        c0.load(s, i, K0, e0, E);
         // BoundaryCoefficients_* (BoundaryConditions.hpp:11-130)
-        const bool use_adv = (tag == MHFEM_BC_FIXED_FLUX);
-        const bool inflow = (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) && (c0.vel[i] < 0);
-        auto BA = [&](int j, int f, bool EisF) -> double {
-            if (inflow) return (i == j && EisF) ? 1.0 : 0.0;
-            return use_adv ? c0.A_adv(adv, j, f, EisF) : c0.A_no_adv(j, f);
-        };
-        // RowSetter::setRow (BoundaryConditions.hpp:132-222)
-        const double d = BA(i, e0, true);
-        int k = 0;
-        for (int t = 0; t < nslots; t++) {
-            const int f = (info >> (6 * t)) & 7;
-            if (f == e0) {  // unit diagonal, not stored (BoundaryConditions.hpp:200-203)
+        const bool use_adv = tag == MHFEM_BC_FIXED_FLUX;
+        const long iidx = (((long)i * N + i) * FPC + e0) * C + K0;
+        const bool inflow = tag == MHFEM_BC_ADVECTIVE_OUTFLOW && (s.a[iidx] + s.u[iidx]) < 0;
+        double A[N][FPC], Ae[N];
 #pragma unroll
-                for (int j = 0; j < N; j++)
-                    if (j != i) s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = BA(j, f, true) / d;
-                continue;
+        for (int j = 0; j < N; j++) {
+#pragma unroll
+            for (int f = 0; f < FPC; f++) {
+                const double v = use_adv ? ca.A[i][j][f] : cn.A[i][j][f];
+                A[j][f] = inflow ? ((i == j && f == e0) ? 1.0 : 0.0) : v;
             }
-            double *blk = s.A.val + sell_block(s.A.slice_ptr, E, k++) * (N * N) + i * N;
-#pragma unroll
-            for (int j = 0; j < N; j++) blk[j] = BA(j, f, false) / d;
+            const double ve = use_adv ? ca.Ae[i][j] : cn.Ae[i][j];
+            Ae[j] = inflow ? (i == j ? 1.0 : 0.0) : ve;
         }
+        // RowSetter::setRow (BoundaryConditions.hpp:132-222); unit diagonal not stored (:200-203)
+        const double d = Ae[i];
+#pragma unroll
+        for (int f = 0; f < FPC; f++) {
+            if (f == e0) continue;
+            double v[N];
+#pragma unroll
+            for (int j = 0; j < N; j++) v[j] = A[j][f] / d;
+            store_block_row<N>(s, E, kp0[f], i, v);
+        }
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (j != i) s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = Ae[j] / d;
         double bval;
         if (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) {
             bval = 0;
-            bval += inflow ? s.Z_iK[(long)i * s.C + K0] : c0.RHS_no_adv(s, i);
+            bval += inflow ? s.Z_iK[(long)i * C + K0] : cn.rhs[i];
         } else {
             bval = -s.bc_values[bi] * s.face_meas[E];
-            bval += use_adv ? c0.RHS_adv(s, adv, i, E) : c0.RHS_no_adv(s, i);
+            bval += use_adv ? ca.rhs[i] : cn.rhs[i];
         }
-        s.rhs[row] = bval / d;
+        s.rhs[(long)E * N + i] = bval / d;
     }
 }
 
@@ -668,6 +738,7 @@ void launch_assemble(Ctx &c, double time, double tau)
     (void)tau;
     SchemeView s = c.view();
     MHFEM_DISPATCH(c, k_assemble, blocks(c.F), s);
+    MHFEM_DISPATCH(c, k_assemble_boundary, blocks(c.nbnd), s);
 }
 
 void launch_post_iterate(Ctx &c)

@@ -52,7 +52,8 @@ class Timers(C.Structure):
     _fields_ = [("pre_iterate", C.c_double), ("update_b_R_Q", C.c_double), ("upwind", C.c_double),
                 ("assemble", C.c_double), ("preconditioner", C.c_double),
                 ("linear_solver", C.c_double), ("post_iterate", C.c_double),
-                ("iterations", C.c_longlong), ("time_steps", C.c_longlong)]
+                ("iterations", C.c_longlong), ("time_steps", C.c_longlong),
+                ("spmv_ms", C.c_double), ("spmv_timed", C.c_longlong)]
 
 
 GUESS_COMBINE = {"combined": 0, "network": 1, "history": 2}
@@ -369,7 +370,12 @@ class Solver:
         self._check(lib().mhfem_synchronize(self.ctx))
 
     def set_timing(self, on=True):
-        self._check(lib().mhfem_set_timing(self.ctx, 1 if on else 0))
+        """False/0: off; True/1: phase timers; 2: phase timers + sampled
+        SpMV launch timing (timers()['spmv_ms'] / ['spmv_timed'])"""
+        self._check(lib().mhfem_set_timing(self.ctx, int(on)))
+
+    def reset_timers(self):
+        self._check(lib().mhfem_reset_timers(self.ctx))
 
     def timers(self):
         t = Timers()
