@@ -25,6 +25,9 @@ Forms, each run from the same initial state on a fresh solver:
                 step take seconds).
   chosen_without_network -- the chosen form with the network switched off
                 (previous increments only): what the trained network adds.
+  fp64_history -- FP64 BiCGStab + Jacobi from the previous increments: what
+                the FP32 iteration adds, and (its distance to `plain`) the
+                spread between two FP64 solutions that both meet 1e-10.
 
 roofline: the dominant kernel is the FP32 block-SELL SpMV of the chosen
 form's iteration.  Every 8th launch during the chosen form's timed steps is
@@ -173,7 +176,8 @@ def main():
     log(f"[bench] building the n=2 {args.size}^3 workload")
     pr = problems.adr_3d(s=args.size, seed=args.seed, preconditioner="diagonal")
     rows = pr.num_faces * pr.n
-    guess = dict(train_steps=args.train_steps, epochs=10, learning_rate=2e-3, sample_stride=8, combine="combined")
+    guess = dict(train_steps=args.train_steps, epochs=10, learning_rate=2e-3, sample_stride=8, combine="combined",
+                 reject_after=3)
     chosen = {"name": "learned guess + FP32 BiCGStab with FP64 reliable updates (Jacobi)",
               "solver": dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=args.delta,
                              check_every=8, tol=1e-10),
@@ -184,6 +188,8 @@ def main():
                  "solver": dict(preconditioner="ilu0", precision="fp64", check_every=4, tol=1e-10)}
     ablation = {"name": "chosen form without the network (previous increments only)",
                 "solver": chosen["solver"], "guess": dict(guess, train_steps=0, combine="history")}
+    fp64_hist = {"name": "FP64 BiCGStab + Jacobi from the previous increments",
+                 "solver": plain["solver"], "guess": dict(guess, train_steps=0, combine="history")}
 
     log(f"[bench] chosen form: {chosen['name']}")
     s, dt, its = run_form(pr, chosen, args.steps, args.warmup, timing=2)
@@ -222,13 +228,19 @@ def main():
         res_ilu["sample"] = f"steps 2-{1 + args.ilu_steps} of the trajectory"
         log(f"[bench] plain ILU(0): {res_ilu}")
 
-    res_abl = None
+    res_abl = res_h64 = z_h64 = None
     if not args.skip_ablation:
         log(f"[bench] ablation: {ablation['name']}")
         sa, dta, itsa = run_form(pr, ablation, args.steps, args.warmup)
         sa.close()
         res_abl = summary(ablation["name"], args.steps, dta, itsa)
         log(f"[bench] ablation: {res_abl}")
+        log(f"[bench] {fp64_hist['name']}")
+        sh, dth, itsh = run_form(pr, fp64_hist, args.steps, args.warmup)
+        z_h64 = sh.get_array("Z_iF").copy()
+        sh.close()
+        res_h64 = summary(fp64_hist["name"], args.steps, dth, itsh)
+        log(f"[bench] fp64 history: {res_h64}")
 
     value = args.steps / dt
     alg = spmv_bytes(rows, nnz, True)
@@ -258,12 +270,16 @@ def main():
         "plain": res_plain,
         "plain_ilu0": res_ilu,
         "chosen_without_network": res_abl,
+        "fp64_history": res_h64,
         "chosen_vs_plain": round(value / res_plain["steps_per_s"], 3) if res_plain else None,
         "chosen_vs_plain_ilu0": round(value / res_ilu["steps_per_s"], 3) if res_ilu else None,
         "chosen_vs_fp64_rel_l2": (float(np.linalg.norm(z_chosen - z_plain) / np.linalg.norm(z_plain))
                                   if z_plain is not None else None),
+        "fp64_history_vs_fp64_rel_l2": (float(np.linalg.norm(z_h64 - z_plain) / np.linalg.norm(z_plain))
+                                        if z_plain is not None and z_h64 is not None else None),
         "guess": {"network_coefficient": round(gs["coef_network"], 4), "trained_steps": gs["trained_steps"],
-                  "used": gs["used"]},
+                  "used": gs["used"], "network_active_at_end": gs["network_active"],
+                  "network_gain_at_end": round(gs["network_gain"], 5)},
         "roofline": {
             "kernel": f"k_sspmv<float, {pr.n}, ...> (block-SELL SpMV of the FP32 iteration)",
             "bound": "hbm",

@@ -220,6 +220,10 @@ typedef struct {
     unsigned seed;        /* weight initialisation */
     int combine;          /* MHFEM_GUESS_* */
     int sample_stride;    /* an Adam step uses every sample_stride-th tile of 16 DOFs (>= 1) */
+    int reject_after;     /* once training is over, the network is switched off after this many
+                             consecutive solves in which its direction lowers the squared
+                             initial residual by less than 1 % beyond the previous increments
+                             (0: never) */
 } mhfem_guess_config;
 
 typedef struct {
@@ -229,6 +233,8 @@ typedef struct {
     double loss;          /* mean squared training loss of the last epoch (normalised units) */
     int trained_steps;    /* solves used for training so far */
     double coef_network;  /* weight of the network's direction in the combination */
+    double network_gain;  /* relative drop of the squared initial residual due to the network */
+    int network_active;   /* the network is evaluated (not switched off by reject_after) */
 } mhfem_guess_stats;
 
 int mhfem_guess_configure(mhfem_ctx *ctx, const mhfem_guess_config *cfg);

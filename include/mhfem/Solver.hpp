@@ -96,6 +96,7 @@ public:
         guess_.seed = (unsigned)p.getParameter<int>("guess-seed", 1);
         guess_.combine = p.getParameter<int>("guess-combine", MHFEM_GUESS_COMBINED);
         guess_.sample_stride = p.getParameter<int>("guess-sample-stride", 1);
+        guess_.reject_after = p.getParameter<int>("guess-reject-after", 3);
         allIterations_ = allTimeSteps_ = 0;
         return true;
     }

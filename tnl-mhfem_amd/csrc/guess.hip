@@ -450,6 +450,68 @@ __global__ void __launch_bounds__(GB) k_after_solve(int rows, const double *__re
     dx1[r] = d;
 }
 
+// Gram matrix of up to four vectors v0..v3 (absent ones null): the ten
+// products v_p . v_q, q <= p, at index p (p + 1) / 2 + q.  One partial per
+// work-group, summed in a fixed order by k_gram_final, so the result does not
+// depend on timing.
+constexpr int NGRAM = 10;
+__global__ void __launch_bounds__(GB) k_gram(int rows, const double *__restrict__ v0, const double *__restrict__ v1,
+                                             const double *__restrict__ v2, const double *__restrict__ v3,
+                                             double *__restrict__ partial)
+{
+    double acc[NGRAM] = {};
+    for (int r = blockIdx.x * GB + threadIdx.x; r < rows; r += gridDim.x * GB) {
+        const double a0 = v0[r], a1 = v1 ? v1[r] : 0.0, a2 = v2 ? v2[r] : 0.0, a3 = v3 ? v3[r] : 0.0;
+        acc[0] += a0 * a0;
+        acc[1] += a1 * a0;
+        acc[2] += a1 * a1;
+        acc[3] += a2 * a0;
+        acc[4] += a2 * a1;
+        acc[5] += a2 * a2;
+        acc[6] += a3 * a0;
+        acc[7] += a3 * a1;
+        acc[8] += a3 * a2;
+        acc[9] += a3 * a3;
+    }
+    __shared__ double sm[NGRAM][GB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NGRAM; q++) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sm[q][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NGRAM) {
+        double v = 0;
+#pragma unroll
+        for (int w = 0; w < GB / 64; w++) v += sm[threadIdx.x][w];
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(GB) k_gram_final(int nb, const double *__restrict__ partial, double *__restrict__ out)
+{
+    __shared__ double sm[GB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int q = 0; q < NGRAM; q++) {
+        double v = 0;
+        for (int b = threadIdx.x; b < nb; b += GB) v += partial[(long)q * nb + b];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sm[wid] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+#pragma unroll
+            for (int w = 0; w < GB / 64; w++) t += sm[w];
+            out[q] = t;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace mhfem
 
 // ===========================================================================
@@ -465,12 +527,18 @@ struct Guess {
     DBuf<uint8_t> dirichlet;
     DBuf<float> P, m, v, partial, grad;
     DBuf<PackedWeights> W;
+    DBuf<double> gram_part, gram;
+    double *h_gram = nullptr;  // pinned, NGRAM values
+    int gram_blocks = 0;
     int nblocks = 0;
     int adam_step = 0;
     int solves = 0;        // solves seen
     int trained = 0;       // solves used for training
     int history = 0;       // increments available (0, 1, 2)
     bool have_weights = false;
+    int net_fail = 0;      // consecutive solves in which the network did not help
+    bool net_off = false;  // switched off by the reject_after rule
+    bool features = false; // X holds the features of the current solve
     mhfem_guess_stats last{};
 };
 
@@ -504,7 +572,6 @@ void pack(Ctx &c, Guess &G)
     MHFEM_HIP(hipGetLastError());
 }
 
-double host_dot(Ctx &c, const double *x, const double *y) { return dot64(c, x, y); }
 
 // one full-batch Adam step on (X, target); returns the mean squared loss
 double train_epoch(Ctx &c, Guess &G, long rows, float lr)
@@ -550,6 +617,10 @@ Guess *guess_create(Ctx &c)
     const long ntiles = ((long)n + 15) / 16;
     G->nblocks = (int)std::max<long>(1, std::min<long>(1024, (ntiles + 3) / 4));
     G->partial.alloc((size_t)G->nblocks * NPARAM_PAD);
+    G->gram_blocks = (int)std::max<long>(1, std::min<long>(1024, ((long)n + GB - 1) / GB));
+    G->gram_part.alloc((size_t)G->gram_blocks * NGRAM);
+    G->gram.alloc(NGRAM);
+    MHFEM_HIP(hipHostMalloc((void **)&G->h_gram, NGRAM * sizeof(double), hipHostMallocDefault));
     G->cfg.enabled = 0;
     G->cfg.train_steps = 100;
     G->cfg.epochs = 20;
@@ -560,18 +631,79 @@ Guess *guess_create(Ctx &c)
     return G;
 }
 
-void guess_destroy(Guess *G) { delete G; }
+void guess_destroy(Guess *G)
+{
+    if (G && G->h_gram) (void)hipHostFree(G->h_gram);
+    delete G;
+}
 
 void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
 {
     const bool reseed = cfg.seed != G.cfg.seed;
     G.cfg = cfg;
     if (G.cfg.epochs < 0) G.cfg.epochs = 0;
+    G.net_fail = 0;
+    G.net_off = false;
     if (reseed && !G.have_weights) {
         init_params(G);
         pack(c, G);
     }
 }
+
+namespace {
+
+// The ten products of v0..v3 (null entries count as zero vectors), read back
+// with one synchronisation.
+void gram(Ctx &c, Guess &G, const double *v0, const double *v1, const double *v2, const double *v3)
+{
+    hipLaunchKernelGGL(k_gram, G.gram_blocks, GB, 0, c.stream, G.rows, v0, v1, v2, v3, G.gram_part.p);
+    hipLaunchKernelGGL(k_gram_final, 1, GB, 0, c.stream, G.gram_blocks, G.gram_part.p, G.gram.p);
+    MHFEM_HIP(hipGetLastError());
+    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, NGRAM * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+}
+
+// min || r0 - sum_{p in [lo, k)} a_p A D_p || by the normal equations
+// (Gaussian elimination with partial pivoting and a 1e-12 relative Tikhonov
+// term against nearly parallel directions).  Returns the predicted squared
+// residual (rr when the system is singular; then a = 0).
+double least_residual(const double (&Gm)[3][3], const double (&gv)[3], double rr, int lo, int k, double (&a)[3])
+{
+    a[0] = a[1] = a[2] = 0.0;
+    const int m = k - lo;
+    if (m <= 0) return rr;
+    double M[3][4];
+    for (int p = 0; p < m; p++) {
+        for (int q = 0; q < m; q++) M[p][q] = Gm[lo + p][lo + q] + (p == q ? 1e-12 * Gm[lo + p][lo + p] : 0.0);
+        M[p][3] = gv[lo + p];
+    }
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        for (int r = col + 1; r < m; r++)
+            if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+        if (M[piv][col] == 0.0) return rr;
+        for (int q = 0; q < 4; q++) std::swap(M[col][q], M[piv][q]);
+        for (int r = col + 1; r < m; r++) {
+            const double f = M[r][col] / M[col][col];
+            for (int q = col; q < 4; q++) M[r][q] -= f * M[col][q];
+        }
+    }
+    double y[3] = {};
+    for (int p = m - 1; p >= 0; p--) {
+        double v = M[p][3];
+        for (int q = p + 1; q < m; q++) v -= M[p][q] * y[q];
+        y[p] = v / M[p][p];
+    }
+    double pred = rr;
+    for (int p = 0; p < m; p++) {
+        pred -= 2 * y[p] * gv[lo + p];
+        for (int q = 0; q < m; q++) pred += y[p] * Gm[lo + p][lo + q] * y[q];
+    }
+    for (int p = 0; p < m; p++) a[lo + p] = y[p];
+    return pred;
+}
+
+}  // namespace
 
 // Before a solve: features of the warm start and, once the network has been
 // trained, the predicted starting vector (kept only if it lowers the residual).
@@ -580,28 +712,43 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     const int n = G.rows;
     const unsigned g = gblocks(n);
     G.last = mhfem_guess_stats{};
+    G.last.network_active = !G.net_off;
+    G.features = false;
     MHFEM_HIP(hipMemcpyAsync(G.xp.p, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     residual64(c, x, b, G.r0.p);
-    const double rr = host_dot(c, G.r0.p, G.r0.p);
+    gram(c, G, G.r0.p, G.dx1.p, nullptr, nullptr);
+    const double rr = G.h_gram[0];
+    const double sx2 = G.h_gram[2] / n;
     G.last.r0_norm = std::sqrt(rr);
     G.last.r_guess_norm = G.last.r0_norm;
     if (!(G.cfg.enabled) || G.history < 1 || rr == 0.0) return;
-    spmv64(c, G.r0.p, G.Ar0.p);
-    spmv64(c, G.Ar0.p, G.A2r0.p);
-    spmv64(c, G.dx1.p, G.Adx1.p);
-    const double sx2 = host_dot(c, G.dx1.p, G.dx1.p) / n;
-    double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
-    MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.dx1.p, G.dx2.p,
-                       G.Adx1.p, G.dirichlet.p, G.scal.p, G.X.p);
-    MHFEM_HIP(hipGetLastError());
+    // The network is evaluated while it trains (its features pair with this
+    // solve's increment) and afterwards until the reject_after rule switches
+    // it off.
+    const bool trained = G.have_weights || G.trained > 0;
+    const bool training = G.cfg.epochs > 0 && G.trained < G.cfg.train_steps;
+    const bool want_net = G.cfg.combine != MHFEM_GUESS_HISTORY && !G.net_off && sx2 > 0.0;
+    const bool use_net = want_net && trained;
+    const bool use_hist = G.cfg.combine != MHFEM_GUESS_NETWORK;
+    if (want_net && (use_net || training)) {
+        spmv64(c, G.r0.p, G.Ar0.p);
+        spmv64(c, G.Ar0.p, G.A2r0.p);
+        spmv64(c, G.dx1.p, G.Adx1.p);
+        double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
+        MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.dx1.p, G.dx2.p,
+                           G.Adx1.p, G.dirichlet.p, G.scal.p, G.X.p);
+        MHFEM_HIP(hipGetLastError());
+        G.features = true;
+    } else if (use_hist && sx2 > 0.0) {
+        spmv64(c, G.dx1.p, G.Adx1.p);
+        double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
+        MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
+    }
     // Directions: the network's predicted increment and (unless the network
     // runs alone) the two previous increments.  The starting vector is the
     // combination x_prev + sum a_k D_k of least residual norm, so it is never
     // worse than the warm start; it is kept only if it lowers the residual.
-    const bool trained = G.have_weights || G.trained > 0;
-    const bool use_net = G.cfg.combine != MHFEM_GUESS_HISTORY && trained && sx2 > 0.0;
-    const bool use_hist = G.cfg.combine != MHFEM_GUESS_NETWORK;
     const double *D[3] = {nullptr, nullptr, nullptr}, *AD[3] = {nullptr, nullptr, nullptr};
     int k = 0;
     if (use_net) {
@@ -624,47 +771,25 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
         }
     }
     if (k == 0) return;
-    // normal equations of min || r0 - sum a_k A D_k ||
+    gram(c, G, G.r0.p, AD[0], AD[1], AD[2]);
     double Gm[3][3] = {}, gv[3] = {}, a[3] = {};
     for (int p = 0; p < k; p++) {
-        gv[p] = host_dot(c, G.r0.p, AD[p]);
-        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = host_dot(c, AD[p], AD[q]);
+        gv[p] = G.h_gram[(p + 1) * (p + 2) / 2];
+        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = G.h_gram[(p + 1) * (p + 2) / 2 + q + 1];
     }
-    // Gaussian elimination with partial pivoting, tiny Tikhonov term against
-    // (nearly) parallel directions
-    double M[3][4];
-    for (int p = 0; p < k; p++) {
-        for (int q = 0; q < k; q++) M[p][q] = Gm[p][q] + (p == q ? 1e-12 * Gm[p][p] : 0.0);
-        M[p][3] = gv[p];
-    }
-    bool ok = true;
-    for (int col = 0; col < k && ok; col++) {
-        int piv = col;
-        for (int r = col + 1; r < k; r++)
-            if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
-        if (M[piv][col] == 0.0) {
-            ok = false;
-            break;
-        }
-        for (int q = 0; q < 4; q++) std::swap(M[col][q], M[piv][q]);
-        for (int r = col + 1; r < k; r++) {
-            const double f = M[r][col] / M[col][col];
-            for (int q = col; q < 4; q++) M[r][q] -= f * M[col][q];
-        }
-    }
-    if (!ok) return;
-    for (int p = k - 1; p >= 0; p--) {
-        double v = M[p][3];
-        for (int q = p + 1; q < k; q++) v -= M[p][q] * a[q];
-        a[p] = v / M[p][p];
-    }
-    double pred = rr;
-    for (int p = 0; p < k; p++) {
-        pred -= 2 * a[p] * gv[p];
-        for (int q = 0; q < k; q++) pred += a[p] * Gm[p][q] * a[q];
-    }
+    const double pred = least_residual(Gm, gv, rr, 0, k, a);
     G.last.r_guess_norm = std::sqrt(std::max(pred, 0.0));
     G.last.coef_network = use_net ? a[0] : 0.0;
+    if (use_net && use_hist && k > 1) {
+        // what the network adds to the previous increments alone
+        double ah[3];
+        const double pred_hist = least_residual(Gm, gv, rr, 1, k, ah);
+        G.last.network_gain = pred_hist > 0 ? (pred_hist - pred) / pred_hist : 0.0;
+        if (!training && G.cfg.reject_after > 0) {
+            G.net_fail = G.last.network_gain < 0.01 ? G.net_fail + 1 : 0;
+            if (G.net_fail >= G.cfg.reject_after) G.net_off = true;
+        }
+    }
     if (!(pred < rr)) return;
     hipLaunchKernelGGL(k_combine_dirs, g, GB, 0, c.stream, n, G.xp.p, D[0], D[1], D[2], a[0], a[1], a[2], x);
     MHFEM_HIP(hipGetLastError());
@@ -677,7 +802,7 @@ void guess_after_solve(Ctx &c, Guess &G, const double *x)
 {
     const int n = G.rows;
     const unsigned g = gblocks(n);
-    const bool train = G.cfg.enabled && G.history >= 1 && G.trained < G.cfg.train_steps && G.cfg.epochs > 0;
+    const bool train = G.cfg.enabled && G.features && G.trained < G.cfg.train_steps && G.cfg.epochs > 0;
     hipLaunchKernelGGL(k_after_solve, g, GB, 0, c.stream, n, x, G.xp.p, G.dx1.p, G.dx2.p, G.scal.p,
                        train ? G.target.p : (float *)nullptr);
     MHFEM_HIP(hipGetLastError());

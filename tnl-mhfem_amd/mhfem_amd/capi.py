@@ -62,12 +62,13 @@ GUESS_COMBINE = {"combined": 0, "network": 1, "history": 2}
 class GuessConfig(C.Structure):
     _fields_ = [("enabled", C.c_int), ("train_steps", C.c_int), ("epochs", C.c_int),
                 ("learning_rate", C.c_float), ("seed", C.c_uint), ("combine", C.c_int),
-                ("sample_stride", C.c_int)]
+                ("sample_stride", C.c_int), ("reject_after", C.c_int)]
 
 
 class GuessStats(C.Structure):
     _fields_ = [("used", C.c_int), ("r0_norm", C.c_double), ("r_guess_norm", C.c_double),
-                ("loss", C.c_double), ("trained_steps", C.c_int), ("coef_network", C.c_double)]
+                ("loss", C.c_double), ("trained_steps", C.c_int), ("coef_network", C.c_double),
+                ("network_gain", C.c_double), ("network_active", C.c_int)]
 
 
 # every symbol declared in include/mhfem.h with its signature
@@ -333,9 +334,9 @@ class Solver:
 
     # ---- learned initial guess
     def configure_guess(self, enabled=True, train_steps=100, epochs=20, learning_rate=1e-3, seed=1,
-                        combine="combined", sample_stride=1):
+                        combine="combined", sample_stride=1, reject_after=0):
         cfg = GuessConfig(1 if enabled else 0, train_steps, epochs, learning_rate, seed,
-                          GUESS_COMBINE[combine], sample_stride)
+                          GUESS_COMBINE[combine], sample_stride, reject_after)
         self._check(lib().mhfem_guess_configure(self.ctx, C.byref(cfg)))
 
     def guess_stats(self):
