@@ -84,6 +84,39 @@ __device__ __forceinline__ Pack<T, M> ld(const T *p)
 {
     return *reinterpret_cast<const Pack<T, M> *>(p);
 }
+// The same access with a non-temporal (streaming) hint: for data read once
+// per sweep (matrix values and column indices), so that it does not push
+// reused data (the gathered vector) out of the caches.
+template <typename T, int M>
+__device__ __forceinline__ Pack<T, M> ld_stream(const T *p)
+{
+    constexpr int B = (int)sizeof(T) * M;
+    if constexpr (B % 16 == 0) {
+        typedef unsigned int V __attribute__((ext_vector_type(4)));
+        struct Raw {
+            V c[B / 16];
+        } r;
+        const V *q = reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int c = 0; c < B / 16; c++) r.c[c] = __builtin_nontemporal_load(q + c);
+        return __builtin_bit_cast(Pack<T, M>, r);
+    } else if constexpr (B % 8 == 0) {
+        typedef unsigned int V __attribute__((ext_vector_type(2)));
+        struct Raw {
+            V c[B / 8];
+        } r;
+        const V *q = reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int c = 0; c < B / 8; c++) r.c[c] = __builtin_nontemporal_load(q + c);
+        return __builtin_bit_cast(Pack<T, M>, r);
+    } else {
+        Pack<T, M> r;
+#pragma unroll
+        for (int c = 0; c < M; c++) r.v[c] = __builtin_nontemporal_load(p + c);
+        return r;
+    }
+}
+
 template <typename T, int M>
 __device__ __forceinline__ void st(T *p, const Pack<T, M> &v)
 {

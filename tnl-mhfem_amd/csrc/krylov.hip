@@ -130,8 +130,20 @@ __device__ void finalize(double *sc, int fin)
 // kernel boundary orders the writes, so no work-group issues a device-scope
 // fence (on gfx950 that writes back the L2 of its XCD, and once per
 // work-group it costs more than the kernel's own streaming).
+// Work-group -> work item.  Work-groups are dealt to the 8 XCDs of the chip
+// in turn (work-group b runs on XCD b % 8); renumbering them so that XCD k
+// takes the k-th contiguous eighth of the work items keeps the faces that
+// share column faces -- and therefore their x gathers -- in one L2.  Any
+// bijection is correct; this one only changes where the work runs.
+__device__ __forceinline__ int xcd_block(int nblk)
+{
+    const int b = blockIdx.x, k = b & 7, j = b >> 3;
+    const int q = nblk >> 3, r = nblk & 7;
+    return k * q + (k < r ? k : r) + j;
+}
+
 template <int ND>
-__device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg)
+__device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg, int blk)
 {
     __shared__ double sm[ND][KB / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -147,7 +159,7 @@ __device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg)
             double s = sm[d][0];
 #pragma unroll
             for (int q = 1; q < KB / 64; q++) s += sm[d][q];
-            sg.partials[(long)d * sg.nblk + blockIdx.x] = s;
+            sg.partials[(long)d * sg.nblk + blk] = s;
         }
     }
 }
@@ -199,29 +211,55 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 // The N rows of face E: y = x_E + (diagonal block without its unit
 // diagonal) x_E + sum over the stored blocks of B_k x_{col_k}.  One block
 // costs one column index, one N*N vector load and one N-wide gather of x.
+// The loads are issued in three waves -- every column index of the face,
+// then every block and gather, then the products -- so that a lane keeps
+// up to PF blocks in flight instead of waiting on each index and gather in
+// turn.  The slice width is uniform across a wavefront's 64 faces, so the
+// k < w guards do not diverge.
 template <int N, typename T>
 __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
 {
+    constexpr int PF = N <= 2 ? 8 : 3;
+    const int64_t base = A.slice_ptr[E >> 6] + (E & 63);
+    const int w = A.slice_width[E >> 6];
+    int col[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++)
+        if (k < w) col[k] = __builtin_nontemporal_load(A.col + base + (int64_t)k * 64);
     const Pack<T, N> xe = ld<T, N>(x + (int64_t)E * N);
+    Pack<T, N * N> b[PF];
+    Pack<T, N> xf[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++)
+        if (k < w) {
+            b[k] = ld_stream<T, N * N>(A.val + (base + (int64_t)k * 64) * (N * N));
+            xf[k] = ld<T, N>(x + (int64_t)col[k] * N);
+        }
     Pack<T, N> y = xe;
     if constexpr (N > 1) {
-        const Pack<T, N *(N - 1)> dg = ld<T, N *(N - 1)>(A.val + A.diag_base + (int64_t)E * N * (N - 1));
+        const Pack<T, N *(N - 1)> dg = ld_stream<T, N *(N - 1)>(A.val + A.diag_base + (int64_t)E * N * (N - 1));
 #pragma unroll
         for (int i = 0; i < N; i++)
 #pragma unroll
             for (int j = 0; j < N; j++)
                 if (j != i) y.v[i] += dg.v[i * (N - 1) + (j < i ? j : j - 1)] * xe.v[j];
     }
-    const int64_t base = A.slice_ptr[E >> 6] + (E & 63);
-    const int w = A.slice_width[E >> 6];
-    for (int k = 0; k < w; k++) {
+#pragma unroll
+    for (int k = 0; k < PF; k++)
+        if (k < w) {
+#pragma unroll
+            for (int i = 0; i < N; i++)
+#pragma unroll
+                for (int j = 0; j < N; j++) y.v[i] += b[k].v[i * N + j] * xf[k].v[j];
+        }
+    for (int k = PF; k < w; k++) {
         const int64_t q = base + (int64_t)k * 64;
-        const Pack<T, N * N> b = ld<T, N * N>(A.val + q * (N * N));
-        const Pack<T, N> xf = ld<T, N>(x + (int64_t)A.col[q] * N);
+        const Pack<T, N * N> bk = ld<T, N * N>(A.val + q * (N * N));
+        const Pack<T, N> xk = ld<T, N>(x + (int64_t)A.col[q] * N);
 #pragma unroll
         for (int i = 0; i < N; i++)
 #pragma unroll
-            for (int j = 0; j < N; j++) y.v[i] += b.v[i * N + j] * xf.v[j];
+            for (int j = 0; j < N; j++) y.v[i] += bk.v[i * N + j] * xk.v[j];
     }
     return y;
 }
@@ -239,10 +277,11 @@ static void with_n(int n, F &&f)
 }
 
 #define SEG_BLOCK_PROLOGUE(sg, mode)                                   \
-    const int seg_ = (sg).blk_seg[blockIdx.x];                         \
+    const int blk_ = xcd_block((sg).nblk);                             \
+    const int seg_ = (sg).blk_seg[blk_];                               \
     double *sc_ = (sg).sc + (long)seg_ * NSLOTS;                       \
     if (!seg_selected(sc_, (mode))) return;                            \
-    const int row0_ = (sg).blk_row[blockIdx.x], row1_ = (sg).blk_row[blockIdx.x + 1]
+    const int row0_ = (sg).blk_row[blk_], row1_ = (sg).blk_row[blk_ + 1]
 
 // y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y).  A
 // work-group's row range holds whole faces (build_blocks).
@@ -270,7 +309,7 @@ __global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, c
             for (int i = 0; i < N; i++) acc[1] += (double)v.v[i] * (double)v.v[i];
         }
     }
-    if constexpr (ND > 0) block_partial<ND>(acc, sg);
+    if constexpr (ND > 0) block_partial<ND>(acc, sg, blk_);
 }
 
 // y = b - A x on the selected segments
@@ -304,7 +343,7 @@ __global__ void __launch_bounds__(KB) k_sdots(Seg sg, int mode, DotArgs<T, ND> a
     for (int i = row0_ + threadIdx.x; i < row1_; i += KB)
 #pragma unroll
         for (int d = 0; d < ND; d++) acc[d] += (double)a.x[d][i] * (double)a.y[d][i];
-    block_partial<ND>(acc, sg);
+    block_partial<ND>(acc, sg, blk_);
 }
 
 template <typename T>
@@ -354,7 +393,7 @@ __global__ void __launch_bounds__(KB) k_bicg_xr(Seg sg, T *__restrict__ x, const
         acc[0] += (double)ri * (double)r_ast[i];
         acc[1] += (double)ri * (double)ri;
     }
-    block_partial<2>(acc, sg);
+    block_partial<2>(acc, sg, blk_);
 }
 
 // p = r + beta (p - omega Ap)
