@@ -47,6 +47,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tnl-mhfem_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per SpMV launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain copy's reading); the summary holds the commands
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_fp32_n2_cube64_hbm_counters.md")
 METRIC = "time steps/s at 1e-10 rel. residual, chosen vs plain GPU form; HBM GB/s"
 
 
@@ -69,6 +72,19 @@ def block_sell_bytes(n, faces, blocks_per_face, fp32):
     w = 4 if fp32 else 8
     rows = n * faces
     return blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
+
+
+def profiled_traffic():
+    """bytes per launch from the line 'traffic_bytes_per_launch: N' of the
+    counter summary, None when it is absent"""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1].strip().split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def run_form(pr, form, steps, warmup, timing=0):
@@ -163,7 +179,21 @@ def main():
     ap.add_argument("--skip-ilu", action="store_true")
     ap.add_argument("--skip-ablation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 1 GiB device buffer 3 times (reference reading for the HBM counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = 1 << 28  # 2^28 floats = 1 GiB
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it "
@@ -287,7 +317,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic() if pr.n == 2 and args.size == 64 else None,
             "avg_launch_us": round(spmv_ms * 1e3, 2),
             "timed_launches": tm["spmv_timed"],
             "algorithmic_bytes": alg,
