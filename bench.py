@@ -29,6 +29,13 @@ Forms, each run from the same initial state on a fresh solver:
                 the FP32 iteration adds, and (its distance to `plain`) the
                 spread between two FP64 solutions that both meet 1e-10.
 
+ensemble: the batched form -- 96 independent 2-D heat-type instances with
+side s drawn from [32, 256] squares (log-normal K, sigma=1.5, correlation
+length 0.05), FP64 BiCGStab + Jacobi to 1e-10, solved as one segmented system
+with per-instance convergence; eager steps against steps replayed from
+captured HIP graphs, after the rows-per-work-group autotuner, and against the
+same instances stepped one after another with one context each.
+
 roofline: the dominant kernel is the FP32 block-SELL SpMV of the chosen
 form's iteration.  Every 8th launch during the chosen form's timed steps is
 bracketed by HIP events on the library's stream (timing level 2);
@@ -120,6 +127,60 @@ def summary(name, steps, dt, its):
     }
 
 
+def ensemble_leg(steps, warmup, seed=7):
+    from mhfem_amd import capi, problems
+
+    sizes = problems.ensemble_sizes(seed, 96)
+    members = [problems.ensemble_member(int(s_), seed * 1000 + k) for k, s_ in enumerate(sizes)]
+    pr = problems.concatenate(members)
+    out = {"instances": 96, "cells": pr.num_cells, "dofs": pr.num_faces * pr.n,
+           "form": "FP64 BiCGStab + Jacobi, one segmented system"}
+    # the alternative: one context per instance, stepped one after another
+    solvers = [capi.Solver(m, preconditioner="diagonal", tol=1e-10, check_every=8) for m in members]
+    t = 0.0
+    for s in solvers:
+        s.step(t, pr.dt)
+    t += pr.dt
+    for s in solvers:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in solvers:
+            s.step(t, pr.dt)
+        t += pr.dt
+    for s in solvers:
+        s.synchronize()
+    dt = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    out["one_context_per_instance"] = {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 3)}
+    for mode in ("eager", "graph"):
+        s = capi.Solver(pr, preconditioner="diagonal", tol=1e-10, check_every=8)
+        rpb, _ = s.autotune()
+        step = s.step if mode == "eager" else s.step_graph
+        t = 0.0
+        for _ in range(warmup):
+            step(t, pr.dt)
+            t += pr.dt
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(t, pr.dt)
+            t += pr.dt
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        its, res = s.batch_stats()
+        out[mode] = {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 3),
+                     "rows_per_block": rpb, "max_instance_iterations_last_step": int(its.max()),
+                     "min_instance_iterations_last_step": int(its.min()),
+                     "max_residue_last_step": float(res.max())}
+        s.close()
+    out["graph_vs_eager"] = round(out["graph"]["steps_per_s"] / out["eager"]["steps_per_s"], 3)
+    out["graph_vs_one_context_per_instance"] = round(
+        out["graph"]["steps_per_s"] / out["one_context_per_instance"]["steps_per_s"], 3)
+    return out
+
+
 def cpu_baseline(pr, iters_per_step, sample_iters):
     """Oracle step time = preIterate + assembly + ILU(0) update +
     iterations_per_step x (time of one FP64 BiCGStab+ILU(0) iteration) +
@@ -179,6 +240,7 @@ def main():
     ap.add_argument("--skip-ilu", action="store_true")
     ap.add_argument("--skip-ablation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the ensemble leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (reference reading for the HBM counters)")
     args = ap.parse_args()
@@ -324,8 +386,13 @@ def main():
             "format_bytes": int(fmt),
             "format_GBs": round(fmt / (spmv_ms * 1e-3) / 1e9, 1),
         },
+        "ensemble": None,
         "cpu_baseline": None,
     }
+    if args.ensemble_steps > 0:
+        log("[bench] batched ensemble")
+        out["ensemble"] = ensemble_leg(args.ensemble_steps, 2)
+        log(f"[bench] ensemble: {out['ensemble']}")
     if not args.skip_cpu:
         log("[bench] cpu baseline (oracle)")
         per_step = res_ilu["iterations_per_step"] if res_ilu else res_chosen["iterations_per_step"]

@@ -89,8 +89,7 @@ enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2 };
    to the FP64 solution) each time it has dropped by `inner_residue`; the
    search direction is kept across the replacements. */
 enum { MHFEM_PRECISION_FP64 = 0, MHFEM_PRECISION_FP32_REFINED = 1, MHFEM_PRECISION_FP32_RELIABLE = 2 };
-/* initial guess of each solve */
-enum { MHFEM_GUESS_PREVIOUS = 0, MHFEM_GUESS_LEARNED = 1 };
+
 
 typedef struct {
     int solver;                 /* MHFEM_SOLVER_* ("linear-solver") */

@@ -968,7 +968,9 @@ static double dot(int n, const double *x, const double *y)
 }
 
 /* BiCGStab, left preconditioned (van der Vorst 1992), with the restart on
- * rho breakdown; stopping test ||r_k|| / ||M^{-1} b|| < tol. */
+ * rho breakdown; stopping test ||r_k|| / ||M^{-1} b|| < tol.  The two other
+ * textbook breakdowns, (r*, A p) = 0 and omega = 0, also restart from the
+ * current x (where the plain recurrence would continue with inf/NaN scalars). */
 static int bicgstab(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters,
                     double *res)
 {
@@ -988,7 +990,17 @@ static int bicgstab(krylov_op *op, const double *b, double *x, double tol, int m
     int it = 0;
     while (resid >= tol && it < maxit && !isnan(resid)) {
         pmatvec(op, pp, Ap);
-        const double alpha = rho / dot(n, Ap, r_ast);
+        const double den = dot(n, Ap, r_ast), alpha = rho / den;
+        if (!(fabs(den) > 0.0) || !isfinite(alpha)) {
+            /* breakdown (r*, A p) = 0: restart from the current x */
+            presidue(op, x, b, r);
+            memcpy(pp, r, sizeof(double) * n);
+            memcpy(r_ast, r, sizeof(double) * n);
+            rho = r_ast_sqnorm = dot(n, r, r_ast);
+            it++;
+            resid = sqrt(dot(n, r, r)) / b_norm;
+            continue;
+        }
 #pragma omp parallel for
         for (int k = 0; k < n; k++) s[k] = r[k] - alpha * Ap[k];
         pmatvec(op, s, As);
@@ -1003,7 +1015,7 @@ static int bicgstab(krylov_op *op, const double *b, double *x, double tol, int m
         }
         const double rho_old = rho;
         rho = dot(n, r, r_ast);
-        if (fabs(rho) < eps2 * r_ast_sqnorm) {
+        if (!(fabs(rho) >= eps2 * r_ast_sqnorm) || omega == 0.0) {
             presidue(op, x, b, r);
             memcpy(pp, r, sizeof(double) * n);
             memcpy(r_ast, r, sizeof(double) * n);

@@ -74,14 +74,29 @@ __device__ void finalize(double *sc, int fin)
 {
     const double eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
     switch (fin) {
-    case FIN_ALPHA: sc[S_ALPHA] = sc[S_RHO] / sc[S_D0]; break;
+    case FIN_ALPHA: {
+        // breakdown (r*, A p) = 0: the segment restarts from its current x
+        // (the rest of the iteration skips it); counted as an iteration so
+        // that a segment that keeps breaking down still ends at max-iterations
+        const double a = sc[S_RHO] / sc[S_D0];
+        if (!(fabs(sc[S_D0]) > 0.0) || !isfinite(a)) {
+            sc[S_RESTART] = 1;
+            sc[S_IT] += 1;
+            if (sc[S_IT] >= sc[S_MAXIT]) sc[S_DONE] = 1;
+        } else {
+            sc[S_ALPHA] = a;
+        }
+        break;
+    }
     case FIN_OMEGA: sc[S_OMEGA] = sc[S_D1] > 0 ? sc[S_D0] / sc[S_D1] : 0.0; break;
     case FIN_UPDATE: {
         sc[S_RHO_OLD] = sc[S_RHO];
         sc[S_RHO] = sc[S_D0];
         sc[S_IT] += 1;
-        if (fabs(sc[S_RHO]) < eps2 * sc[S_RAST_SQ]) {
-            sc[S_RESTART] = 1;  // the host recomputes the residual (restart)
+        if (!(fabs(sc[S_RHO]) >= eps2 * sc[S_RAST_SQ]) || sc[S_OMEGA] == 0.0) {
+            // rho breakdown, or omega = 0 (the next beta would divide by it):
+            // the host recomputes the residual and restarts the recurrence
+            sc[S_RESTART] = 1;
         } else {
             sc[S_BETA] = (sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]);
             const double res = sqrt(sc[S_D1]) / sc[S_BNORM];
