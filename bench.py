@@ -13,7 +13,7 @@ workload's 50 steps: `warmup` untimed steps, then `steps` timed ones.
 
 Forms, each run from the same initial state on a fresh solver:
   chosen     -- learned initial guess (per-face MLP trained online during the
-                first steps, combined with the previous increments by least
+                first steps, combined with the last 8 increments by least
                 residual) + FP32 BiCGStab whose residual is replaced by the
                 FP64 one every 10x drop (FP64 iterative refinement with
                 reliable updates), Jacobi (the identity on the unit-diagonal
@@ -234,6 +234,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--delta", type=float, default=0.1, help="FP32 residual drop per FP64 replacement")
     ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--history-depth", type=int, default=8, help="previous increments in the learned guess")
     ap.add_argument("--ilu-steps", type=int, default=2)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
@@ -269,7 +270,7 @@ def main():
     pr = problems.adr_3d(s=args.size, seed=args.seed, preconditioner="diagonal")
     rows = pr.num_faces * pr.n
     guess = dict(train_steps=args.train_steps, epochs=10, learning_rate=2e-3, sample_stride=8, combine="combined",
-                 reject_after=3)
+                 reject_after=3, history_depth=args.history_depth)
     chosen = {"name": "learned guess + FP32 BiCGStab with FP64 reliable updates (Jacobi)",
               "solver": dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=args.delta,
                              check_every=8, tol=1e-10),

@@ -206,9 +206,10 @@ int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_st
  * the previous increments; the prediction is used only when it lowers the
  * initial residual, and the Krylov solve still decides convergence. */
 /* how the starting vector is formed: the least-residual combination of the
- * previous Z_iF plus the network's predicted increment and the two previous
- * increments (COMBINED), of the network's prediction alone (NETWORK), or of
- * the previous increments alone (HISTORY: the method without the network) */
+ * previous Z_iF plus the network's predicted increment and the last
+ * history_depth increments (COMBINED), of the network's prediction alone
+ * (NETWORK), or of the previous increments alone (HISTORY: the method without
+ * the network) */
 enum { MHFEM_GUESS_COMBINED = 0, MHFEM_GUESS_NETWORK = 1, MHFEM_GUESS_HISTORY = 2 };
 
 typedef struct {
@@ -223,6 +224,7 @@ typedef struct {
                              consecutive solves in which its direction lowers the squared
                              initial residual by less than 1 % beyond the previous increments
                              (0: never) */
+    int history_depth;    /* previous increments in the combination, 1..8 (0: 2) */
 } mhfem_guess_config;
 
 typedef struct {

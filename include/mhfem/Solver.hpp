@@ -97,6 +97,7 @@ public:
         guess_.combine = p.getParameter<int>("guess-combine", MHFEM_GUESS_COMBINED);
         guess_.sample_stride = p.getParameter<int>("guess-sample-stride", 1);
         guess_.reject_after = p.getParameter<int>("guess-reject-after", 3);
+        guess_.history_depth = p.getParameter<int>("guess-history-depth", 8);
         allIterations_ = allTimeSteps_ = 0;
         return true;
     }

@@ -280,6 +280,7 @@ int krylov_autotune(Ctx &c, double *best_ms);
 void krylov_segment_stats(Ctx &c, int *iterations, double *residues, int count);
 void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st);
 void spmv64(Ctx &c, const double *x, double *y);
+void spmm64(Ctx &c, const double *const *X, double *const *Y, int m);
 void residual64(Ctx &c, const double *x, const double *b, double *r);
 double dot64(Ctx &c, const double *x, const double *y);
 

@@ -9,9 +9,11 @@
 //   f3..5 = dx1, dx2, A dx1   (scaled by s_x = RMS(dx1))
 //   f6 = Dirichlet row, f7 = component index, f8 = log10(s_r / s_x) / 4
 // The prediction d = s_x * y is a direction: the starting vector is the
-// combination x_prev + a_0 d + a_1 dx1 + a_2 dx2 of least residual norm
-// (normal equations of three directions), used only if it lowers the initial
-// residual; the Krylov solve then judges convergence as before.
+// combination x_prev + a_0 d + sum_j a_j dx_j over the network's direction and
+// the last `history_depth` increments dx_1 (newest) .. dx_h, of least residual
+// norm (normal equations solved by pivoted Cholesky with truncation), used
+// only if it lowers the initial residual; the Krylov solve then judges
+// convergence as before.
 //
 // Arithmetic: BF16 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation.  Every
 // layer is evaluated transposed, H^T = W^T X^T, so the D operand of one MFMA
@@ -425,77 +427,83 @@ __global__ void __launch_bounds__(GB) k_direction(int rows, const float *__restr
 }
 
 // x = x_prev + sum_k a_k D_k (up to three directions)
-__global__ void __launch_bounds__(GB) k_combine_dirs(int rows, const double *__restrict__ xp, const double *__restrict__ d0,
-                                                     const double *__restrict__ d1, const double *__restrict__ d2,
-                                                     double a0, double a1, double a2, double *__restrict__ x)
+// x = xp + sum_k a_k d_k over up to MAXDIR directions
+constexpr int HMAX = 8;            // previous increments kept
+constexpr int MAXDIR = HMAX + 1;   // + the network's direction
+struct Dirs {
+    const double *d[MAXDIR];
+    double a[MAXDIR];
+    int k;
+};
+__global__ void __launch_bounds__(GB) k_combine_dirs(int rows, const double *__restrict__ xp, Dirs D,
+                                                     double *__restrict__ x)
 {
     const int r = blockIdx.x * GB + threadIdx.x;
     if (r >= rows) return;
     double v = xp[r];
-    if (d0) v += a0 * d0[r];
-    if (d1) v += a1 * d1[r];
-    if (d2) v += a2 * d2[r];
+    for (int k = 0; k < D.k; k++) v += D.a[k] * D.d[k][r];
     x[r] = v;
 }
 
+// the increment of this solve becomes the newest history entry (and, while
+// the network trains, its normalised target)
 __global__ void __launch_bounds__(GB) k_after_solve(int rows, const double *__restrict__ x, const double *__restrict__ xp,
-                                                    double *__restrict__ dx1, double *__restrict__ dx2,
-                                                    const double *__restrict__ scal, float *__restrict__ target)
+                                                    double *__restrict__ newest, const double *__restrict__ scal,
+                                                    float *__restrict__ target)
 {
     const int r = blockIdx.x * GB + threadIdx.x;
     if (r >= rows) return;
     const double d = x[r] - xp[r];
     if (target) target[r] = scal[1] > 0 ? (float)(d / scal[1]) : 0.f;
-    dx2[r] = dx1[r];
-    dx1[r] = d;
+    newest[r] = d;
 }
 
-// Gram matrix of up to four vectors v0..v3 (absent ones null): the ten
-// products v_p . v_q, q <= p, at index p (p + 1) / 2 + q.  One partial per
-// work-group, summed in a fixed order by k_gram_final, so the result does not
-// depend on timing.
-constexpr int NGRAM = 10;
-__global__ void __launch_bounds__(GB) k_gram(int rows, const double *__restrict__ v0, const double *__restrict__ v1,
-                                             const double *__restrict__ v2, const double *__restrict__ v3,
-                                             double *__restrict__ partial)
+// Gram matrix of NV vectors v_0..v_{NV-1}: the products v_p . v_q, q <= p,
+// at index p (p + 1) / 2 + q.  One partial per work-group, summed in a fixed
+// order by k_gram_final, so the result does not depend on timing.
+constexpr int MAXVEC = MAXDIR + 1;  // the residual + every direction
+constexpr int MAXPROD = MAXVEC * (MAXVEC + 1) / 2;
+struct Vecs {
+    const double *v[MAXVEC];
+};
+template <int NV>
+__global__ void __launch_bounds__(GB) k_gram(int rows, Vecs V, double *__restrict__ partial)
 {
-    double acc[NGRAM] = {};
+    constexpr int NP = NV * (NV + 1) / 2;
+    double acc[NP] = {};
     for (int r = blockIdx.x * GB + threadIdx.x; r < rows; r += gridDim.x * GB) {
-        const double a0 = v0[r], a1 = v1 ? v1[r] : 0.0, a2 = v2 ? v2[r] : 0.0, a3 = v3 ? v3[r] : 0.0;
-        acc[0] += a0 * a0;
-        acc[1] += a1 * a0;
-        acc[2] += a1 * a1;
-        acc[3] += a2 * a0;
-        acc[4] += a2 * a1;
-        acc[5] += a2 * a2;
-        acc[6] += a3 * a0;
-        acc[7] += a3 * a1;
-        acc[8] += a3 * a2;
-        acc[9] += a3 * a3;
+        double a[NV];
+#pragma unroll
+        for (int p = 0; p < NV; p++) a[p] = V.v[p][r];
+#pragma unroll
+        for (int p = 0; p < NV; p++)
+#pragma unroll
+            for (int q = 0; q <= p; q++) acc[p * (p + 1) / 2 + q] += a[p] * a[q];
     }
-    __shared__ double sm[NGRAM][GB / 64];
+    __shared__ double sm[NP][GB / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < NGRAM; q++) {
+    for (int q = 0; q < NP; q++) {
         double v = acc[q];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if (lane == 0) sm[q][wid] = v;
     }
     __syncthreads();
-    if (threadIdx.x < NGRAM) {
+    for (int q = threadIdx.x; q < NP; q += GB) {
         double v = 0;
 #pragma unroll
-        for (int w = 0; w < GB / 64; w++) v += sm[threadIdx.x][w];
-        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = v;
+        for (int w = 0; w < GB / 64; w++) v += sm[q][w];
+        partial[(long)q * gridDim.x + blockIdx.x] = v;
     }
 }
 
-__global__ void __launch_bounds__(GB) k_gram_final(int nb, const double *__restrict__ partial, double *__restrict__ out)
+__global__ void __launch_bounds__(GB) k_gram_final(int nb, int np, const double *__restrict__ partial,
+                                                   double *__restrict__ out)
 {
     __shared__ double sm[GB / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int q = 0; q < NGRAM; q++) {
+    for (int q = 0; q < np; q++) {
         double v = 0;
         for (int b = threadIdx.x; b < nb; b += GB) v += partial[(long)q * nb + b];
 #pragma unroll
@@ -523,18 +531,23 @@ struct Guess {
     int rows = 0;
     DBuf<unsigned short> X;
     DBuf<float> y, target;
-    DBuf<double> xp, dx1, dx2, r0, Ar0, A2r0, Adx1, rtry, scal;
+    DBuf<double> xp, r0, Ar0, A2r0, rtry, scal;
+    DBuf<double> hist[HMAX], Ahist[HMAX];  // previous increments (ring) and A times them
+    int head = 0;                          // ring slot of the newest increment
+    double *inc(int j) { return hist[(head + j) % depth].p; }   // j-th newest
+    double *Ainc(int j) { return Ahist[(head + j) % depth].p; }
     DBuf<uint8_t> dirichlet;
     DBuf<float> P, m, v, partial, grad;
     DBuf<PackedWeights> W;
     DBuf<double> gram_part, gram;
-    double *h_gram = nullptr;  // pinned, NGRAM values
+    double *h_gram = nullptr;  // pinned, MAXPROD values
     int gram_blocks = 0;
     int nblocks = 0;
     int adam_step = 0;
     int solves = 0;        // solves seen
     int trained = 0;       // solves used for training
-    int history = 0;       // increments available (0, 1, 2)
+    int history = 0;       // increments available (0 .. depth)
+    int depth = 2;         // increments used (history_depth)
     bool have_weights = false;
     int net_fail = 0;      // consecutive solves in which the network did not help
     bool net_off = false;  // switched off by the reject_after rule
@@ -600,9 +613,15 @@ Guess *guess_create(Ctx &c)
     G->X.alloc((size_t)n * NF);
     G->y.alloc(n);
     G->target.alloc(n);
-    for (DBuf<double> *b : {&G->xp, &G->dx1, &G->dx2, &G->r0, &G->Ar0, &G->A2r0, &G->Adx1, &G->rtry}) {
+    for (DBuf<double> *b : {&G->xp, &G->r0, &G->Ar0, &G->A2r0, &G->rtry}) {
         b->alloc(n);
         MHFEM_HIP(hipMemset(b->p, 0, (size_t)n * sizeof(double)));
+    }
+    for (int j = 0; j < 2; j++) {  // the default depth; guess_configure grows it
+        for (DBuf<double> *b : {&G->hist[j], &G->Ahist[j]}) {
+            b->alloc(n);
+            MHFEM_HIP(hipMemset(b->p, 0, (size_t)n * sizeof(double)));
+        }
     }
     G->scal.alloc(4);
     std::vector<uint8_t> dflag(n);
@@ -618,9 +637,9 @@ Guess *guess_create(Ctx &c)
     G->nblocks = (int)std::max<long>(1, std::min<long>(1024, (ntiles + 3) / 4));
     G->partial.alloc((size_t)G->nblocks * NPARAM_PAD);
     G->gram_blocks = (int)std::max<long>(1, std::min<long>(1024, ((long)n + GB - 1) / GB));
-    G->gram_part.alloc((size_t)G->gram_blocks * NGRAM);
-    G->gram.alloc(NGRAM);
-    MHFEM_HIP(hipHostMalloc((void **)&G->h_gram, NGRAM * sizeof(double), hipHostMallocDefault));
+    G->gram_part.alloc((size_t)G->gram_blocks * MAXPROD);
+    G->gram.alloc(MAXPROD);
+    MHFEM_HIP(hipHostMalloc((void **)&G->h_gram, MAXPROD * sizeof(double), hipHostMallocDefault));
     G->cfg.enabled = 0;
     G->cfg.train_steps = 100;
     G->cfg.epochs = 20;
@@ -644,6 +663,23 @@ void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
     if (G.cfg.epochs < 0) G.cfg.epochs = 0;
     G.net_fail = 0;
     G.net_off = false;
+    const int depth = cfg.history_depth > 0 ? std::min(cfg.history_depth, HMAX) : 2;
+    if (depth != G.depth || !G.hist[depth - 1].p) {
+        // a new depth starts an empty history in a linear ring of `depth` slots
+        G.depth = depth;
+        G.history = 0;
+        G.head = 0;
+        for (int j = 0; j < HMAX; j++) {
+            for (DBuf<double> *b : {&G.hist[j], &G.Ahist[j]}) {
+                if (j < depth) {
+                    if (!b->p) b->alloc(G.rows);
+                    MHFEM_HIP(hipMemset(b->p, 0, (size_t)G.rows * sizeof(double)));
+                } else {
+                    b->release();
+                }
+            }
+        }
+    }
     if (reseed && !G.have_weights) {
         init_params(G);
         pack(c, G);
@@ -652,54 +688,91 @@ void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
 
 namespace {
 
-// The ten products of v0..v3 (null entries count as zero vectors), read back
-// with one synchronisation.
-void gram(Ctx &c, Guess &G, const double *v0, const double *v1, const double *v2, const double *v3)
+// The products of the vectors v[0..nv) (Gram matrix, packed lower triangle at
+// p (p + 1) / 2 + q), read back with one synchronisation.
+template <int NV>
+void gram_launch(Ctx &c, Guess &G, const Vecs &V)
 {
-    hipLaunchKernelGGL(k_gram, G.gram_blocks, GB, 0, c.stream, G.rows, v0, v1, v2, v3, G.gram_part.p);
-    hipLaunchKernelGGL(k_gram_final, 1, GB, 0, c.stream, G.gram_blocks, G.gram_part.p, G.gram.p);
+    hipLaunchKernelGGL((k_gram<NV>), G.gram_blocks, GB, 0, c.stream, G.rows, V, G.gram_part.p);
+}
+void gram(Ctx &c, Guess &G, const double *const *v, int nv)
+{
+    Vecs V{};
+    for (int p = 0; p < nv; p++) V.v[p] = v[p];
+    switch (nv) {
+    case 2: gram_launch<2>(c, G, V); break;
+    case 3: gram_launch<3>(c, G, V); break;
+    case 4: gram_launch<4>(c, G, V); break;
+    case 5: gram_launch<5>(c, G, V); break;
+    case 6: gram_launch<6>(c, G, V); break;
+    case 7: gram_launch<7>(c, G, V); break;
+    case 8: gram_launch<8>(c, G, V); break;
+    case 9: gram_launch<9>(c, G, V); break;
+    case 10: gram_launch<10>(c, G, V); break;
+    default: throw Error{MHFEM_E_STATE, "gram: unsupported vector count"};
+    }
+    const int np = nv * (nv + 1) / 2;
+    hipLaunchKernelGGL(k_gram_final, 1, GB, 0, c.stream, G.gram_blocks, np, G.gram_part.p, G.gram.p);
     MHFEM_HIP(hipGetLastError());
-    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, NGRAM * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, np * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     MHFEM_HIP(hipStreamSynchronize(c.stream));
 }
 
 // min || r0 - sum_{p in [lo, k)} a_p A D_p || by the normal equations
-// (Gaussian elimination with partial pivoting and a 1e-12 relative Tikhonov
-// term against nearly parallel directions).  Returns the predicted squared
-// residual (rr when the system is singular; then a = 0).
-double least_residual(const double (&Gm)[3][3], const double (&gv)[3], double rr, int lo, int k, double (&a)[3])
+// Gm a = gv.  The directions are scaled to unit norm and the Gram matrix is
+// factorised by Cholesky with diagonal pivoting, stopping at pivots below
+// 1e-13: directions (nearly) in the span of the chosen ones get a = 0, so
+// that nearly parallel previous increments do not blow the coefficients up.
+// Returns the predicted squared residual rr - 2 a.gv + a.Gm.a.
+double least_residual(const double (&Gm)[MAXDIR][MAXDIR], const double (&gv)[MAXDIR], double rr, int lo, int k,
+                      double (&a)[MAXDIR])
 {
-    a[0] = a[1] = a[2] = 0.0;
+    for (int p = 0; p < MAXDIR; p++) a[p] = 0.0;
     const int m = k - lo;
     if (m <= 0) return rr;
-    double M[3][4];
-    for (int p = 0; p < m; p++) {
-        for (int q = 0; q < m; q++) M[p][q] = Gm[lo + p][lo + q] + (p == q ? 1e-12 * Gm[lo + p][lo + p] : 0.0);
-        M[p][3] = gv[lo + p];
-    }
-    for (int col = 0; col < m; col++) {
-        int piv = col;
-        for (int r = col + 1; r < m; r++)
-            if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
-        if (M[piv][col] == 0.0) return rr;
-        for (int q = 0; q < 4; q++) std::swap(M[col][q], M[piv][q]);
-        for (int r = col + 1; r < m; r++) {
-            const double f = M[r][col] / M[col][col];
-            for (int q = col; q < 4; q++) M[r][q] -= f * M[col][q];
+    double sc[MAXDIR], S[MAXDIR][MAXDIR], L[MAXDIR][MAXDIR] = {}, dg[MAXDIR];
+    for (int p = 0; p < m; p++) sc[p] = Gm[lo + p][lo + p] > 0 ? 1.0 / std::sqrt(Gm[lo + p][lo + p]) : 0.0;
+    for (int p = 0; p < m; p++)
+        for (int q = 0; q < m; q++) S[p][q] = Gm[lo + p][lo + q] * sc[p] * sc[q];
+    for (int p = 0; p < m; p++) dg[p] = S[p][p];
+    int piv[MAXDIR], used = 0;
+    bool taken[MAXDIR] = {};
+    for (int step = 0; step < m; step++) {
+        int j = -1;
+        for (int p = 0; p < m; p++)
+            if (!taken[p] && (j < 0 || dg[p] > dg[j])) j = p;
+        if (j < 0 || !(dg[j] > 1e-13)) break;
+        taken[j] = true;
+        piv[used] = j;
+        L[j][used] = std::sqrt(dg[j]);
+        for (int p = 0; p < m; p++) {
+            if (taken[p]) continue;
+            double v = S[p][j];
+            for (int t = 0; t < used; t++) v -= L[p][t] * L[j][t];
+            L[p][used] = v / L[j][used];
+            dg[p] -= L[p][used] * L[p][used];
         }
+        used++;
     }
-    double y[3] = {};
-    for (int p = m - 1; p >= 0; p--) {
-        double v = M[p][3];
-        for (int q = p + 1; q < m; q++) v -= M[p][q] * y[q];
-        y[p] = v / M[p][p];
+    if (used == 0) return rr;
+    // solve (L_pp L_pp^T) z = gs on the chosen directions
+    double y[MAXDIR], z[MAXDIR];
+    for (int t = 0; t < used; t++) {
+        double v = gv[lo + piv[t]] * sc[piv[t]];
+        for (int u = 0; u < t; u++) v -= L[piv[t]][u] * y[u];
+        y[t] = v / L[piv[t]][t];
     }
+    for (int t = used - 1; t >= 0; t--) {
+        double v = y[t];
+        for (int u = t + 1; u < used; u++) v -= L[piv[u]][t] * z[u];
+        z[t] = v / L[piv[t]][t];
+    }
+    for (int t = 0; t < used; t++) a[lo + piv[t]] = z[t] * sc[piv[t]];
     double pred = rr;
-    for (int p = 0; p < m; p++) {
-        pred -= 2 * y[p] * gv[lo + p];
-        for (int q = 0; q < m; q++) pred += y[p] * Gm[lo + p][lo + q] * y[q];
+    for (int p = lo; p < k; p++) {
+        pred -= 2 * a[p] * gv[p];
+        for (int q = lo; q < k; q++) pred += a[p] * Gm[p][q] * a[q];
     }
-    for (int p = 0; p < m; p++) a[lo + p] = y[p];
     return pred;
 }
 
@@ -716,7 +789,8 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     G.features = false;
     MHFEM_HIP(hipMemcpyAsync(G.xp.p, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     residual64(c, x, b, G.r0.p);
-    gram(c, G, G.r0.p, G.dx1.p, nullptr, nullptr);
+    const double *v2[2] = {G.r0.p, G.inc(0)};
+    gram(c, G, v2, 2);
     const double rr = G.h_gram[0];
     const double sx2 = G.h_gram[2] / n;
     G.last.r0_norm = std::sqrt(rr);
@@ -729,27 +803,36 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     const bool training = G.cfg.epochs > 0 && G.trained < G.cfg.train_steps;
     const bool want_net = G.cfg.combine != MHFEM_GUESS_HISTORY && !G.net_off && sx2 > 0.0;
     const bool use_net = want_net && trained;
-    const bool use_hist = G.cfg.combine != MHFEM_GUESS_NETWORK;
+    const bool use_hist = G.cfg.combine != MHFEM_GUESS_NETWORK && sx2 > 0.0;
+    const int h = std::min(G.history, G.depth);
+    if (sx2 > 0.0) {
+        double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
+        MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
+        // A times every stored increment, one pass over the matrix
+        const int m = use_hist ? h : 1;
+        const double *X[HMAX];
+        double *Y[HMAX];
+        for (int j = 0; j < m; j++) {
+            X[j] = G.inc(j);
+            Y[j] = G.Ainc(j);
+        }
+        spmm64(c, X, Y, m);
+    }
     if (want_net && (use_net || training)) {
         spmv64(c, G.r0.p, G.Ar0.p);
         spmv64(c, G.Ar0.p, G.A2r0.p);
-        spmv64(c, G.dx1.p, G.Adx1.p);
-        double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
-        MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.dx1.p, G.dx2.p,
-                           G.Adx1.p, G.dirichlet.p, G.scal.p, G.X.p);
+        const double *dx2 = G.depth > 1 ? G.inc(1) : G.inc(0);
+        hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.inc(0), dx2,
+                           G.Ainc(0), G.dirichlet.p, G.scal.p, G.X.p);
         MHFEM_HIP(hipGetLastError());
         G.features = true;
-    } else if (use_hist && sx2 > 0.0) {
-        spmv64(c, G.dx1.p, G.Adx1.p);
-        double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
-        MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
     }
     // Directions: the network's predicted increment and (unless the network
-    // runs alone) the two previous increments.  The starting vector is the
+    // runs alone) the previous increments.  The starting vector is the
     // combination x_prev + sum a_k D_k of least residual norm, so it is never
     // worse than the warm start; it is kept only if it lowers the residual.
-    const double *D[3] = {nullptr, nullptr, nullptr}, *AD[3] = {nullptr, nullptr, nullptr};
+    Dirs D{};
+    const double *AD[MAXDIR] = {};
     int k = 0;
     if (use_net) {
         const long ntiles = ((long)n + 15) / 16;
@@ -757,22 +840,22 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
                            c.stream, (long)n, G.X.p, G.W.p, G.y.p);
         hipLaunchKernelGGL(k_direction, g, GB, 0, c.stream, n, G.y.p, G.scal.p, G.rtry.p);
         MHFEM_HIP(hipGetLastError());
-        spmv64(c, G.rtry.p, G.Ar0.p);  // Ar0 and A2r0 are free once the features are packed
-        D[k] = G.rtry.p;
+        spmv64(c, G.rtry.p, G.Ar0.p);  // Ar0 is free once the features are packed
+        D.d[k] = G.rtry.p;
         AD[k++] = G.Ar0.p;
     }
-    if (use_hist && sx2 > 0.0) {
-        D[k] = G.dx1.p;
-        AD[k++] = G.Adx1.p;
-        if (G.history >= 2) {
-            spmv64(c, G.dx2.p, G.A2r0.p);
-            D[k] = G.dx2.p;
-            AD[k++] = G.A2r0.p;
+    if (use_hist) {
+        for (int j = 0; j < h; j++) {
+            D.d[k] = G.inc(j);
+            AD[k++] = G.Ainc(j);
         }
     }
     if (k == 0) return;
-    gram(c, G, G.r0.p, AD[0], AD[1], AD[2]);
-    double Gm[3][3] = {}, gv[3] = {}, a[3] = {};
+    const double *vv[MAXVEC];
+    vv[0] = G.r0.p;
+    for (int p = 0; p < k; p++) vv[p + 1] = AD[p];
+    gram(c, G, vv, k + 1);
+    double Gm[MAXDIR][MAXDIR] = {}, gv[MAXDIR] = {}, a[MAXDIR] = {};
     for (int p = 0; p < k; p++) {
         gv[p] = G.h_gram[(p + 1) * (p + 2) / 2];
         for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = G.h_gram[(p + 1) * (p + 2) / 2 + q + 1];
@@ -782,7 +865,7 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     G.last.coef_network = use_net ? a[0] : 0.0;
     if (use_net && use_hist && k > 1) {
         // what the network adds to the previous increments alone
-        double ah[3];
+        double ah[MAXDIR];
         const double pred_hist = least_residual(Gm, gv, rr, 1, k, ah);
         G.last.network_gain = pred_hist > 0 ? (pred_hist - pred) / pred_hist : 0.0;
         if (!training && G.cfg.reject_after > 0) {
@@ -791,19 +874,24 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
         }
     }
     if (!(pred < rr)) return;
-    hipLaunchKernelGGL(k_combine_dirs, g, GB, 0, c.stream, n, G.xp.p, D[0], D[1], D[2], a[0], a[1], a[2], x);
+    for (int p = 0; p < k; p++) D.a[p] = a[p];
+    D.k = k;
+    hipLaunchKernelGGL(k_combine_dirs, g, GB, 0, c.stream, n, G.xp.p, D, x);
     MHFEM_HIP(hipGetLastError());
     G.last.used = 1;
 }
 
-// After a solve: the increment becomes a training target (during the training
-// phase) and enters the history used by the next features.
+// After a solve: the increment becomes the newest history entry and, during
+// the training phase, a training target.
 void guess_after_solve(Ctx &c, Guess &G, const double *x)
 {
     const int n = G.rows;
     const unsigned g = gblocks(n);
     const bool train = G.cfg.enabled && G.features && G.trained < G.cfg.train_steps && G.cfg.epochs > 0;
-    hipLaunchKernelGGL(k_after_solve, g, GB, 0, c.stream, n, x, G.xp.p, G.dx1.p, G.dx2.p, G.scal.p,
+    // the newest increment takes the slot of the oldest (this solve's
+    // features are already packed in X)
+    G.head = (G.head + G.depth - 1) % G.depth;
+    hipLaunchKernelGGL(k_after_solve, g, GB, 0, c.stream, n, x, G.xp.p, G.inc(0), G.scal.p,
                        train ? G.target.p : (float *)nullptr);
     MHFEM_HIP(hipGetLastError());
     if (train) {
@@ -816,7 +904,7 @@ void guess_after_solve(Ctx &c, Guess &G, const double *x)
         G.last.loss = loss / n;
     }
     G.last.trained_steps = G.trained;
-    G.history = std::min(G.history + 1, 2);
+    G.history = std::min(G.history + 1, G.depth);
     G.solves++;
 }
 

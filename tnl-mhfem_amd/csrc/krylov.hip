@@ -443,6 +443,53 @@ __global__ void __launch_bounds__(KB) k_spmv_plain(SellView<T> A, const T *__res
         st<T, N>(y + (int64_t)f * N, sell_face<N>(A, x, f));
 }
 
+// Y_v = A X_v for M vectors at once: the matrix is streamed once and each
+// block multiplies the M gathered vectors
+template <int M>
+struct MultiVec {
+    const double *x[M];
+    double *y[M];
+};
+template <int N, int M>
+__global__ void __launch_bounds__(KB) k_spmm_plain(SellView<double> A, MultiVec<M> V)
+{
+    for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
+        Pack<double, N> y[M];
+#pragma unroll
+        for (int v = 0; v < M; v++) y[v] = ld<double, N>(V.x[v] + (int64_t)f * N);
+        if constexpr (N > 1) {
+            const Pack<double, N *(N - 1)> dg =
+                ld_stream<double, N *(N - 1)>(A.val + A.diag_base + (int64_t)f * N * (N - 1));
+#pragma unroll
+            for (int v = 0; v < M; v++) {
+                const Pack<double, N> xe = y[v];
+#pragma unroll
+                for (int i = 0; i < N; i++)
+#pragma unroll
+                    for (int j = 0; j < N; j++)
+                        if (j != i) y[v].v[i] += dg.v[i * (N - 1) + (j < i ? j : j - 1)] * xe.v[j];
+            }
+        }
+        const int64_t base = A.slice_ptr[f >> 6] + (f & 63);
+        const int w = A.slice_width[f >> 6];
+        for (int k = 0; k < w; k++) {
+            const int64_t q = base + (int64_t)k * 64;
+            const int col = __builtin_nontemporal_load(A.col + q);
+            const Pack<double, N * N> b = ld_stream<double, N * N>(A.val + q * (N * N));
+#pragma unroll
+            for (int v = 0; v < M; v++) {
+                const Pack<double, N> xf = ld<double, N>(V.x[v] + (int64_t)col * N);
+#pragma unroll
+                for (int i = 0; i < N; i++)
+#pragma unroll
+                    for (int j = 0; j < N; j++) y[v].v[i] += b.v[i * N + j] * xf.v[j];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < M; v++) st<double, N>(V.y[v] + (int64_t)f * N, y[v]);
+    }
+}
+
 // FP64 residual of the refinement loop, stored in FP32: r32 = (float)(b - A x)
 template <int N>
 __global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const double *__restrict__ x,
@@ -1137,6 +1184,39 @@ void spmv64(Ctx &c, const double *x, double *y)
     Krylov &k = *c.krylov;
     Solve S(c, k);
     S.spmv_plain<double>(x, y);
+}
+
+// Y_v = A X_v, v < m (m <= 8), one pass over the matrix
+template <int M>
+static void spmm_launch(Ctx &c, Solve &S, const double *const *X, double *const *Y)
+{
+    MultiVec<M> V{};
+    for (int v = 0; v < M; v++) {
+        V.x[v] = X[v];
+        V.y[v] = Y[v];
+    }
+    SellView<double> A = S.matrix<double>();
+    with_n(c.n, [&](auto NC) {
+        hipLaunchKernelGGL((k_spmm_plain<decltype(NC)::value, M>), grid_for(c.F), KB, 0, c.stream, A, V);
+    });
+    MHFEM_HIP(hipGetLastError());
+}
+void spmm64(Ctx &c, const double *const *X, double *const *Y, int m)
+{
+    Krylov &k = *c.krylov;
+    Solve S(c, k);
+    switch (m) {
+    case 0: break;
+    case 1: spmm_launch<1>(c, S, X, Y); break;
+    case 2: spmm_launch<2>(c, S, X, Y); break;
+    case 3: spmm_launch<3>(c, S, X, Y); break;
+    case 4: spmm_launch<4>(c, S, X, Y); break;
+    case 5: spmm_launch<5>(c, S, X, Y); break;
+    case 6: spmm_launch<6>(c, S, X, Y); break;
+    case 7: spmm_launch<7>(c, S, X, Y); break;
+    case 8: spmm_launch<8>(c, S, X, Y); break;
+    default: throw Error{MHFEM_E_ARGUMENT, "spmm64: at most 8 vectors"};
+    }
 }
 
 void residual64(Ctx &c, const double *x, const double *b, double *r)

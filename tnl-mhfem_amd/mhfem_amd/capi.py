@@ -62,7 +62,7 @@ GUESS_COMBINE = {"combined": 0, "network": 1, "history": 2}
 class GuessConfig(C.Structure):
     _fields_ = [("enabled", C.c_int), ("train_steps", C.c_int), ("epochs", C.c_int),
                 ("learning_rate", C.c_float), ("seed", C.c_uint), ("combine", C.c_int),
-                ("sample_stride", C.c_int), ("reject_after", C.c_int)]
+                ("sample_stride", C.c_int), ("reject_after", C.c_int), ("history_depth", C.c_int)]
 
 
 class GuessStats(C.Structure):
@@ -334,9 +334,9 @@ class Solver:
 
     # ---- learned initial guess
     def configure_guess(self, enabled=True, train_steps=100, epochs=20, learning_rate=1e-3, seed=1,
-                        combine="combined", sample_stride=1, reject_after=0):
+                        combine="combined", sample_stride=1, reject_after=0, history_depth=0):
         cfg = GuessConfig(1 if enabled else 0, train_steps, epochs, learning_rate, seed,
-                          GUESS_COMBINE[combine], sample_stride, reject_after)
+                          GUESS_COMBINE[combine], sample_stride, reject_after, history_depth)
         self._check(lib().mhfem_guess_configure(self.ctx, C.byref(cfg)))
 
     def guess_stats(self):
