@@ -13,7 +13,7 @@ workload's 50 steps: `warmup` untimed steps, then `steps` timed ones.
 
 Forms, each run from the same initial state on a fresh solver:
   chosen     -- learned initial guess (per-face MLP trained online during the
-                first steps, combined with the last 8 increments by least
+                first steps, combined with the last 16 increments by least
                 residual) + FP32 BiCGStab whose residual is replaced by the
                 FP64 one every 10x drop (FP64 iterative refinement with
                 reliable updates), Jacobi (the identity on the unit-diagonal
@@ -105,6 +105,8 @@ def run_form(pr, form, steps, warmup, timing=0):
         s.step(t, pr.dt)
         t += pr.dt
     s.synchronize()
+    if form.get("autotune"):
+        form["rows_per_block"] = s.autotune()[0]
     if timing:
         s.set_timing(timing)
         s.reset_timers()
@@ -118,9 +120,10 @@ def run_form(pr, form, steps, warmup, timing=0):
     return s, dt, s.iterations[its0:]
 
 
-def summary(name, steps, dt, its):
+def summary(name, steps, dt, its, rpb=None):
     return {
         "form": name,
+        "rows_per_block": rpb,
         "steps_per_s": round(steps / dt, 4),
         "ms_per_step": round(1e3 * dt / steps, 3),
         "iterations_per_step": round(sum(its) / max(1, len(its)), 2),
@@ -234,7 +237,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--delta", type=float, default=0.1, help="FP32 residual drop per FP64 replacement")
     ap.add_argument("--train-steps", type=int, default=10)
-    ap.add_argument("--history-depth", type=int, default=8, help="previous increments in the learned guess")
+    ap.add_argument("--history-depth", type=int, default=16, help="previous increments in the learned guess")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default rows per work-group")
     ap.add_argument("--ilu-steps", type=int, default=2)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
@@ -283,6 +287,8 @@ def main():
                 "solver": chosen["solver"], "guess": dict(guess, train_steps=0, combine="history")}
     fp64_hist = {"name": "FP64 BiCGStab + Jacobi from the previous increments",
                  "solver": plain["solver"], "guess": dict(guess, train_steps=0, combine="history")}
+    for f in (chosen, plain, ablation, fp64_hist):
+        f["autotune"] = not args.no_autotune
 
     log(f"[bench] chosen form: {chosen['name']}")
     s, dt, its = run_form(pr, chosen, args.steps, args.warmup, timing=2)
@@ -299,7 +305,7 @@ def main():
     blocks_per_face = (pairs.size - faces) / faces
     z_chosen = s.get_array("Z_iF").copy()
     s.close()
-    res_chosen = summary(chosen["name"], args.steps, dt, its)
+    res_chosen = summary(chosen["name"], args.steps, dt, its, chosen.get("rows_per_block"))
     log(f"[bench] chosen: {res_chosen}; SpMV {spmv_ms*1e3:.1f} us over {tm['spmv_timed']} timed launches; "
         f"guess {gs}")
 
@@ -309,7 +315,7 @@ def main():
         sp, dtp, itsp = run_form(pr, plain, args.steps, args.warmup)
         z_plain = sp.get_array("Z_iF").copy()
         sp.close()
-        res_plain = summary(plain["name"], args.steps, dtp, itsp)
+        res_plain = summary(plain["name"], args.steps, dtp, itsp, plain.get("rows_per_block"))
         log(f"[bench] plain: {res_plain}")
 
     res_ilu = None
@@ -326,13 +332,13 @@ def main():
         log(f"[bench] ablation: {ablation['name']}")
         sa, dta, itsa = run_form(pr, ablation, args.steps, args.warmup)
         sa.close()
-        res_abl = summary(ablation["name"], args.steps, dta, itsa)
+        res_abl = summary(ablation["name"], args.steps, dta, itsa, ablation.get("rows_per_block"))
         log(f"[bench] ablation: {res_abl}")
         log(f"[bench] {fp64_hist['name']}")
         sh, dth, itsh = run_form(pr, fp64_hist, args.steps, args.warmup)
         z_h64 = sh.get_array("Z_iF").copy()
         sh.close()
-        res_h64 = summary(fp64_hist["name"], args.steps, dth, itsh)
+        res_h64 = summary(fp64_hist["name"], args.steps, dth, itsh, fp64_hist.get("rows_per_block"))
         log(f"[bench] fp64 history: {res_h64}")
 
     value = args.steps / dt

@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(GB) k_direction(int rows, const float *__restr
 
 // x = x_prev + sum_k a_k D_k (up to three directions)
 // x = xp + sum_k a_k d_k over up to MAXDIR directions
-constexpr int HMAX = 8;            // previous increments kept
+constexpr int HMAX = 16;           // previous increments kept
 constexpr int MAXDIR = HMAX + 1;   // + the network's direction
 struct Dirs {
     const double *d[MAXDIR];
@@ -458,27 +458,41 @@ __global__ void __launch_bounds__(GB) k_after_solve(int rows, const double *__re
     newest[r] = d;
 }
 
-// Gram matrix of NV vectors v_0..v_{NV-1}: the products v_p . v_q, q <= p,
-// at index p (p + 1) / 2 + q.  One partial per work-group, summed in a fixed
-// order by k_gram_final, so the result does not depend on timing.
+// Gram matrix of up to MAXVEC vectors, computed in tiles of GT vectors: a
+// symmetric tile (the products of one chunk with itself, q <= p, packed at
+// p (p + 1) / 2 + q) or a cross tile (chunk a against chunk b, at
+// pa * GT + pb).  Absent vectors (null) count as zero.  Every tile writes one
+// partial per work-group and product at its offset in one buffer; one final
+// kernel sums them in a fixed order, so the result does not depend on timing.
 constexpr int MAXVEC = MAXDIR + 1;  // the residual + every direction
-constexpr int MAXPROD = MAXVEC * (MAXVEC + 1) / 2;
-struct Vecs {
-    const double *v[MAXVEC];
+constexpr int GT = 9;               // vectors per tile
+constexpr int NCHUNK = (MAXVEC + GT - 1) / GT;
+constexpr int MAXPROD = NCHUNK * GT * (GT + 1) / 2 + NCHUNK * (NCHUNK - 1) / 2 * GT * GT;
+struct Tile9 {
+    const double *v[GT];
 };
-template <int NV>
-__global__ void __launch_bounds__(GB) k_gram(int rows, Vecs V, double *__restrict__ partial)
+template <bool SYM>
+__global__ void __launch_bounds__(GB) k_gram_tile(int rows, Tile9 A, Tile9 B, double *__restrict__ partial)
 {
-    constexpr int NP = NV * (NV + 1) / 2;
+    constexpr int NP = SYM ? GT * (GT + 1) / 2 : GT * GT;
     double acc[NP] = {};
     for (int r = blockIdx.x * GB + threadIdx.x; r < rows; r += gridDim.x * GB) {
-        double a[NV];
+        double a[GT], b[GT];
 #pragma unroll
-        for (int p = 0; p < NV; p++) a[p] = V.v[p][r];
+        for (int p = 0; p < GT; p++) a[p] = A.v[p] ? A.v[p][r] : 0.0;
+        if constexpr (SYM) {
 #pragma unroll
-        for (int p = 0; p < NV; p++)
+            for (int p = 0; p < GT; p++)
 #pragma unroll
-            for (int q = 0; q <= p; q++) acc[p * (p + 1) / 2 + q] += a[p] * a[q];
+                for (int q = 0; q <= p; q++) acc[p * (p + 1) / 2 + q] += a[p] * a[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < GT; q++) b[q] = B.v[q] ? B.v[q][r] : 0.0;
+#pragma unroll
+            for (int p = 0; p < GT; p++)
+#pragma unroll
+                for (int q = 0; q < GT; q++) acc[p * GT + q] += a[p] * b[q];
+        }
     }
     __shared__ double sm[NP][GB / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -542,6 +556,7 @@ struct Guess {
     DBuf<double> gram_part, gram;
     double *h_gram = nullptr;  // pinned, MAXPROD values
     int gram_blocks = 0;
+    int gram_chunks = 0;
     int nblocks = 0;
     int adam_step = 0;
     int solves = 0;        // solves seen
@@ -688,34 +703,49 @@ void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
 
 namespace {
 
-// The products of the vectors v[0..nv) (Gram matrix, packed lower triangle at
-// p (p + 1) / 2 + q), read back with one synchronisation.
-template <int NV>
-void gram_launch(Ctx &c, Guess &G, const Vecs &V)
-{
-    hipLaunchKernelGGL((k_gram<NV>), G.gram_blocks, GB, 0, c.stream, G.rows, V, G.gram_part.p);
-}
+// The products of the vectors v[0..nv), read back with one synchronisation
+// into G.h_gram and unpacked by gram_at(p, q).
 void gram(Ctx &c, Guess &G, const double *const *v, int nv)
 {
-    Vecs V{};
-    for (int p = 0; p < nv; p++) V.v[p] = v[p];
-    switch (nv) {
-    case 2: gram_launch<2>(c, G, V); break;
-    case 3: gram_launch<3>(c, G, V); break;
-    case 4: gram_launch<4>(c, G, V); break;
-    case 5: gram_launch<5>(c, G, V); break;
-    case 6: gram_launch<6>(c, G, V); break;
-    case 7: gram_launch<7>(c, G, V); break;
-    case 8: gram_launch<8>(c, G, V); break;
-    case 9: gram_launch<9>(c, G, V); break;
-    case 10: gram_launch<10>(c, G, V); break;
-    default: throw Error{MHFEM_E_STATE, "gram: unsupported vector count"};
-    }
-    const int np = nv * (nv + 1) / 2;
-    hipLaunchKernelGGL(k_gram_final, 1, GB, 0, c.stream, G.gram_blocks, np, G.gram_part.p, G.gram.p);
+    if (nv < 1 || nv > MAXVEC) throw Error{MHFEM_E_STATE, "gram: unsupported vector count"};
+    const int nch = (nv + GT - 1) / GT;
+    long off = 0;
+    for (int a = 0; a < nch; a++)
+        for (int b = a; b < nch; b++) {
+            Tile9 A{}, B{};
+            for (int t = 0; t < GT; t++) {
+                A.v[t] = a * GT + t < nv ? v[a * GT + t] : nullptr;
+                B.v[t] = b * GT + t < nv ? v[b * GT + t] : nullptr;
+            }
+            double *part = G.gram_part.p + off * G.gram_blocks;
+            if (a == b) {
+                hipLaunchKernelGGL((k_gram_tile<true>), G.gram_blocks, GB, 0, c.stream, G.rows, A, B, part);
+                off += GT * (GT + 1) / 2;
+            } else {
+                hipLaunchKernelGGL((k_gram_tile<false>), G.gram_blocks, GB, 0, c.stream, G.rows, A, B, part);
+                off += GT * GT;
+            }
+        }
+    G.gram_chunks = nch;
+    hipLaunchKernelGGL(k_gram_final, 1, GB, 0, c.stream, G.gram_blocks, (int)off, G.gram_part.p, G.gram.p);
     MHFEM_HIP(hipGetLastError());
-    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, np * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, off * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     MHFEM_HIP(hipStreamSynchronize(c.stream));
+}
+
+// v_p . v_q from the tiles read back by gram()
+double gram_at(const Guess &G, int p, int q)
+{
+    if (q > p) std::swap(p, q);
+    const int cp = p / GT, cq = q / GT, lp = p % GT, lq = q % GT;
+    long off = 0;
+    for (int a = 0; a < G.gram_chunks; a++)
+        for (int b = a; b < G.gram_chunks; b++) {
+            if (a == cq && b == cp)
+                return a == b ? G.h_gram[off + lp * (lp + 1) / 2 + lq] : G.h_gram[off + lq * GT + lp];
+            off += a == b ? GT * (GT + 1) / 2 : GT * GT;
+        }
+    return 0.0;
 }
 
 // min || r0 - sum_{p in [lo, k)} a_p A D_p || by the normal equations
@@ -791,8 +821,8 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     residual64(c, x, b, G.r0.p);
     const double *v2[2] = {G.r0.p, G.inc(0)};
     gram(c, G, v2, 2);
-    const double rr = G.h_gram[0];
-    const double sx2 = G.h_gram[2] / n;
+    const double rr = gram_at(G, 0, 0);
+    const double sx2 = gram_at(G, 1, 1) / n;
     G.last.r0_norm = std::sqrt(rr);
     G.last.r_guess_norm = G.last.r0_norm;
     if (!(G.cfg.enabled) || G.history < 1 || rr == 0.0) return;
@@ -816,7 +846,7 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
             X[j] = G.inc(j);
             Y[j] = G.Ainc(j);
         }
-        spmm64(c, X, Y, m);
+        for (int j0 = 0; j0 < m; j0 += 8) spmm64(c, X + j0, Y + j0, std::min(8, m - j0));
     }
     if (want_net && (use_net || training)) {
         spmv64(c, G.r0.p, G.Ar0.p);
@@ -857,8 +887,8 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     gram(c, G, vv, k + 1);
     double Gm[MAXDIR][MAXDIR] = {}, gv[MAXDIR] = {}, a[MAXDIR] = {};
     for (int p = 0; p < k; p++) {
-        gv[p] = G.h_gram[(p + 1) * (p + 2) / 2];
-        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = G.h_gram[(p + 1) * (p + 2) / 2 + q + 1];
+        gv[p] = gram_at(G, p + 1, 0);
+        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = gram_at(G, p + 1, q + 1);
     }
     const double pred = least_residual(Gm, gv, rr, 0, k, a);
     G.last.r_guess_norm = std::sqrt(std::max(pred, 0.0));

@@ -192,9 +192,19 @@ __global__ void __launch_bounds__(KB) k_finalize(Seg sg, int fin, int mode)
     const int b0 = sg.seg_blk[s], b1 = sg.seg_blk[s + 1];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-        double a = 0;
-        for (int b = b0 + tid; b < b1; b += KB) a += sg.partials[(long)d * sg.nblk + b];
-        a = wave_sum(a);
+        // four independent loads in flight per thread (a fixed order: the
+        // sum is the same for a given decomposition)
+        const double *P = sg.partials + (long)d * sg.nblk;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        int b = b0 + tid;
+        for (; b + 3 * KB < b1; b += 4 * KB) {
+            a0 += P[b];
+            a1 += P[b + KB];
+            a2 += P[b + 2 * KB];
+            a3 += P[b + 3 * KB];
+        }
+        for (; b < b1; b += KB) a0 += P[b];
+        double a = wave_sum((a0 + a1) + (a2 + a3));
         if (lane == 0) sm[d][wid] = a;
     }
     __syncthreads();
@@ -1264,8 +1274,8 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
 }
 
 // Autotuner: rows per work-group of the fused iteration kernels.  Times the
-// fused SpMV + dot kernel of the iteration for each candidate on the
-// assembled pattern and keeps the fastest.
+// fused SpMV + dot kernel of the iteration and the reduction of its partials
+// for each candidate on the assembled pattern and keeps the fastest.
 int krylov_autotune(Ctx &c, double *best_ms)
 {
     Krylov &k = *c.krylov;
@@ -1273,6 +1283,14 @@ int krylov_autotune(Ctx &c, double *best_ms)
     hipEvent_t e0 = c.ev[12], e1 = c.ev[13];
     int best = k.rows_per_block;
     float bestms = 1e30f;
+    // the SpMV of the precision the solve iterates in
+    const bool fp32 = c.cfg.precision != MHFEM_PRECISION_FP64 && k.val32.p;
+    if (fp32) {
+        precond_update(c, k);  // the FP32 matrix copy
+        hipLaunchKernelGGL((k_convert<float, double>), grid_for(k.n), KB, 0, c.stream, (long)k.n,
+                           c.arr[MHFEM_Z_iF].p, k.x32.p);
+        MHFEM_HIP(hipGetLastError());
+    }
     for (int rpb : cands) {
         build_blocks(c, k, rpb);
         Solve S(c, k);
@@ -1280,7 +1298,12 @@ int krylov_autotune(Ctx &c, double *best_ms)
         const double *x = c.arr[MHFEM_Z_iF].p;
         for (int rep = -2; rep < 10; rep++) {
             if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
-            S.sspmv<double, false, 2>(x, (const double *)nullptr, k.tmp.p, x, M_ALL);
+            // the fused SpMV + dots and their reduction, as in an iteration
+            if (fp32)
+                S.sspmv<float, false, 2>(k.x32.p, (const float *)nullptr, k.tmp32.p, k.x32.p, M_ALL);
+            else
+                S.sspmv<double, false, 2>(x, (const double *)nullptr, k.tmp.p, x, M_ALL);
+            S.finish<2>(FIN_NONE, M_ALL);
         }
         MHFEM_HIP(hipEventRecord(e1, c.stream));
         MHFEM_HIP(hipEventSynchronize(e1));
