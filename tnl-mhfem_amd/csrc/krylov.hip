@@ -463,7 +463,14 @@ struct MultiVec {
 template <int N, int M>
 __global__ void __launch_bounds__(KB) k_spmm_plain(SellView<double> A, MultiVec<M> V)
 {
+    constexpr int PF = 8;  // column indices of a face loaded ahead (wave-uniform k < w guards)
     for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
+        const int64_t base = A.slice_ptr[f >> 6] + (f & 63);
+        const int w = A.slice_width[f >> 6];
+        int col[PF];
+#pragma unroll
+        for (int k = 0; k < PF; k++)
+            if (k < w) col[k] = __builtin_nontemporal_load(A.col + base + (int64_t)k * 64);
         Pack<double, N> y[M];
 #pragma unroll
         for (int v = 0; v < M; v++) y[v] = ld<double, N>(V.x[v] + (int64_t)f * N);
@@ -480,20 +487,24 @@ __global__ void __launch_bounds__(KB) k_spmm_plain(SellView<double> A, MultiVec<
                         if (j != i) y[v].v[i] += dg.v[i * (N - 1) + (j < i ? j : j - 1)] * xe.v[j];
             }
         }
-        const int64_t base = A.slice_ptr[f >> 6] + (f & 63);
-        const int w = A.slice_width[f >> 6];
-        for (int k = 0; k < w; k++) {
-            const int64_t q = base + (int64_t)k * 64;
-            const int col = __builtin_nontemporal_load(A.col + q);
+        auto block = [&](int64_t q, int c) {
             const Pack<double, N * N> b = ld_stream<double, N * N>(A.val + q * (N * N));
+            Pack<double, N> xf[M];
 #pragma unroll
-            for (int v = 0; v < M; v++) {
-                const Pack<double, N> xf = ld<double, N>(V.x[v] + (int64_t)col * N);
+            for (int v = 0; v < M; v++) xf[v] = ld<double, N>(V.x[v] + (int64_t)c * N);
+#pragma unroll
+            for (int v = 0; v < M; v++)
 #pragma unroll
                 for (int i = 0; i < N; i++)
 #pragma unroll
-                    for (int j = 0; j < N; j++) y[v].v[i] += b.v[i * N + j] * xf.v[j];
-            }
+                    for (int j = 0; j < N; j++) y[v].v[i] += b.v[i * N + j] * xf[v].v[j];
+        };
+#pragma unroll
+        for (int k = 0; k < PF; k++)
+            if (k < w) block(base + (int64_t)k * 64, col[k]);
+        for (int k = PF; k < w; k++) {
+            const int64_t q = base + (int64_t)k * 64;
+            block(q, __builtin_nontemporal_load(A.col + q));
         }
 #pragma unroll
         for (int v = 0; v < M; v++) st<double, N>(V.y[v] + (int64_t)f * N, y[v]);
