@@ -224,7 +224,7 @@ typedef struct {
                              consecutive solves in which its direction lowers the squared
                              initial residual by less than 1 % beyond the previous increments
                              (0: never) */
-    int history_depth;    /* previous increments in the combination, 1..8 (0: 2) */
+    int history_depth;    /* previous increments in the combination, 1..16 (0: 2) */
 } mhfem_guess_config;
 
 typedef struct {

@@ -236,28 +236,38 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 // The N rows of face E: y = x_E + (diagonal block without its unit
 // diagonal) x_E + sum over the stored blocks of B_k x_{col_k}.  One block
 // costs one column index, one N*N vector load and one N-wide gather of x.
-// The loads are issued in three waves -- every column index of the face,
-// then every block and gather, then the products -- so that a lane keeps
-// up to PF blocks in flight instead of waiting on each index and gather in
-// turn.  The slice width is uniform across a wavefront's 64 faces, so the
-// k < w guards do not diverge.
+// The loads are issued in three waves -- the column indices of the face's
+// first PF blocks, then those blocks and their gathers, then the products --
+// so that a lane keeps PF blocks in flight instead of waiting on each index
+// and gather in turn.  The slice width is uniform across a wavefront's 64
+// faces, so the k < w guards do not diverge.  Measured at n = 2 on the 64^3
+// Kuhn mesh: FP32 runs best with PF = 4 and streaming (non-temporal) matrix
+// loads, which leave the caches to the gathered x (whole time step of the
+// mixed-precision solve 36.9 ms; 37.9 ms at PF = 6, 38.7 ms at PF = 8,
+// 39.6 ms at PF = 3); FP64 with PF = 8 and ordinary loads (isolated product
+// 169 us; 183 us with streaming loads).
 template <int N, typename T>
 __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
 {
-    constexpr int PF = N <= 2 ? 8 : 3;
+    constexpr bool fp32 = sizeof(T) == 4;
+    constexpr int PF = N <= 2 ? (fp32 ? 4 : 8) : 3;
     const int64_t base = A.slice_ptr[E >> 6] + (E & 63);
     const int w = A.slice_width[E >> 6];
     int col[PF];
 #pragma unroll
     for (int k = 0; k < PF; k++)
-        if (k < w) col[k] = __builtin_nontemporal_load(A.col + base + (int64_t)k * 64);
+        if (k < w) {
+            if constexpr (fp32) col[k] = __builtin_nontemporal_load(A.col + base + (int64_t)k * 64);
+            else col[k] = A.col[base + (int64_t)k * 64];
+        }
     const Pack<T, N> xe = ld<T, N>(x + (int64_t)E * N);
     Pack<T, N * N> b[PF];
     Pack<T, N> xf[PF];
 #pragma unroll
     for (int k = 0; k < PF; k++)
         if (k < w) {
-            b[k] = ld_stream<T, N * N>(A.val + (base + (int64_t)k * 64) * (N * N));
+            if constexpr (fp32) b[k] = ld_stream<T, N * N>(A.val + (base + (int64_t)k * 64) * (N * N));
+            else b[k] = ld<T, N * N>(A.val + (base + (int64_t)k * 64) * (N * N));
             xf[k] = ld<T, N>(x + (int64_t)col[k] * N);
         }
     Pack<T, N> y = xe;
@@ -1284,9 +1294,11 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
     *avg_ms = reps > 0 ? ms / reps : 0.0;
 }
 
-// Autotuner: rows per work-group of the fused iteration kernels.  Times the
-// fused SpMV + dot kernel of the iteration and the reduction of its partials
-// for each candidate on the assembled pattern and keeps the fastest.
+// Autotuner: rows per work-group of the segmented iteration kernels.  For
+// each candidate it times a few real BiCGStab iterations -- both products,
+// the vector updates and the reductions, in the precision the solve iterates
+// in -- on the assembled system, from a scratch iterate (the solution is not
+// touched; every solve starts its recurrence afresh), and keeps the fastest.
 int krylov_autotune(Ctx &c, double *best_ms)
 {
     Krylov &k = *c.krylov;
@@ -1294,33 +1306,52 @@ int krylov_autotune(Ctx &c, double *best_ms)
     hipEvent_t e0 = c.ev[12], e1 = c.ev[13];
     int best = k.rows_per_block;
     float bestms = 1e30f;
-    // the SpMV of the precision the solve iterates in
     const bool fp32 = c.cfg.precision != MHFEM_PRECISION_FP64 && k.val32.p;
+    const int iters = c.cfg.preconditioner == MHFEM_PREC_ILU0 ? 2 : 8;
+    const unsigned g = grid_for(k.n);
+    precond_update(c, k);
+    DBuf<double> xs;  // scratch FP64 iterate
     if (fp32) {
-        precond_update(c, k);  // the FP32 matrix copy
-        hipLaunchKernelGGL((k_convert<float, double>), grid_for(k.n), KB, 0, c.stream, (long)k.n,
-                           c.arr[MHFEM_Z_iF].p, k.x32.p);
-        MHFEM_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)k.n, c.arr[MHFEM_RHS].p, k.rhs32.p);
+    } else {
+        xs.alloc(k.n);
     }
+    MHFEM_HIP(hipGetLastError());
     for (int rpb : cands) {
         build_blocks(c, k, rpb);
         Solve S(c, k);
-        S.set_scalars(0.0, 1);
-        const double *x = c.arr[MHFEM_Z_iF].p;
-        for (int rep = -2; rep < 10; rep++) {
-            if (rep == 0) MHFEM_HIP(hipEventRecord(e0, c.stream));
-            // the fused SpMV + dots and their reduction, as in an iteration
-            if (fp32)
-                S.sspmv<float, false, 2>(k.x32.p, (const float *)nullptr, k.tmp32.p, k.x32.p, M_ALL);
-            else
-                S.sspmv<double, false, 2>(x, (const double *)nullptr, k.tmp.p, x, M_ALL);
-            S.finish<2>(FIN_NONE, M_ALL);
-        }
-        MHFEM_HIP(hipEventRecord(e1, c.stream));
+        auto run = [&](auto zero) {
+            using T = decltype(zero);
+            T *x;
+            const T *b;
+            Solve::Vecs v;
+            if constexpr (sizeof(T) == 4) {
+                x = k.x32.p;
+                b = k.rhs32.p;
+                v = Solve::Vecs{k.r32.p, k.r_ast32.p, k.p32.p, k.Ap32.p, k.s32.p, k.As32.p, k.tmp32.p, k.b32.p,
+                                k.dinv32.p};
+                hipLaunchKernelGGL((k_fill_t<float>), g, KB, 0, c.stream, (long)k.n, k.x32.p, 0.0f);
+            } else {
+                x = xs.p;
+                b = c.arr[MHFEM_RHS].p;
+                v = Solve::Vecs{k.r.p, k.r_ast.p, k.p.p, k.Ap.p, k.s.p, k.As.p, k.tmp.p, k.tmp2.p, k.dinv.p};
+                MHFEM_HIP(hipMemcpyAsync(xs.p, c.arr[MHFEM_Z_iF].p, (size_t)k.n * sizeof(double),
+                                         hipMemcpyDeviceToDevice, c.stream));
+            }
+            MHFEM_HIP(hipGetLastError());
+            // tolerance 0: no segment stops during the measurement
+            S.bicg_start<T>(b, x, v, 0.0, 1 << 30);
+            S.bicg_iterations<T>(x, v, 2);
+            MHFEM_HIP(hipEventRecord(e0, c.stream));
+            S.bicg_iterations<T>(x, v, iters);
+            MHFEM_HIP(hipEventRecord(e1, c.stream));
+        };
+        if (fp32) run(0.0f);
+        else run(0.0);
         MHFEM_HIP(hipEventSynchronize(e1));
         float ms = 0;
         MHFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= 10;
+        ms /= iters;
         if (ms < bestms) {
             bestms = ms;
             best = rpb;
