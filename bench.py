@@ -111,19 +111,21 @@ def run_form(pr, form, steps, warmup, timing=0):
         s.set_timing(timing)
         s.reset_timers()
     its0 = len(s.iterations)
+    form["fallback_steps"] = 0
     t0 = time.perf_counter()
     for _ in range(steps):
-        s.step(t, pr.dt)
+        form["fallback_steps"] += s.step(t, pr.dt).fallback_iterations > 0
         t += pr.dt
     s.synchronize()
     dt = time.perf_counter() - t0
     return s, dt, s.iterations[its0:]
 
 
-def summary(name, steps, dt, its, rpb=None):
+def summary(name, steps, dt, its, rpb=None, fallback_steps=0):
     return {
         "form": name,
         "rows_per_block": rpb,
+        "steps_with_fp64_fallback": fallback_steps,
         "steps_per_s": round(steps / dt, 4),
         "ms_per_step": round(1e3 * dt / steps, 3),
         "iterations_per_step": round(sum(its) / max(1, len(its)), 2),
@@ -181,6 +183,80 @@ def ensemble_leg(steps, warmup, seed=7):
     out["graph_vs_eager"] = round(out["graph"]["steps_per_s"] / out["eager"]["steps_per_s"], 3)
     out["graph_vs_one_context_per_instance"] = round(
         out["graph"]["steps_per_s"] / out["one_context_per_instance"]["steps_per_s"], 3)
+    return out
+
+
+def study96(size, train_seeds, train_steps, eval_seed, eval_steps, history_depth):
+    """The learned-guess study on the large configuration: n = 2 on size^3
+    cubes x 6 tets with the time-dependent Dirichlet value
+    Z_1 = sin(2 pi t / 0.5) at x = 0.  The network is trained online on the
+    first `train_steps` steps of each training K field, then evaluated with
+    frozen weights on a held-out K field against the plain FP64 form and the
+    method without the network."""
+    from mhfem_amd import capi, problems
+
+    mixed = dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=0.1, check_every=8, tol=1e-10)
+    out = {"workload": f"n=2, {size}^3 cubes x 6 tets, Z_1 = sin(2 pi t/0.5) at x=0, dt=1e-3, 1e-10",
+           "train_seeds": train_seeds, "train_steps": train_steps, "eval_seed": eval_seed,
+           "eval_steps": eval_steps}
+    w = None
+    for sd in train_seeds:
+        pr = problems.adr_3d(s=size, seed=sd, preconditioner="diagonal", time_dependent=True)
+        s = capi.Solver(pr, **mixed)
+        s.configure_guess(True, train_steps=train_steps, epochs=10, learning_rate=2e-3, seed=3, sample_stride=8,
+                          history_depth=history_depth)
+        if w is not None:
+            s.set_guess_weights(w)
+        t = 0.0
+        for _ in range(train_steps):
+            s.step(t, pr.dt)
+            t += pr.dt
+        w = s.guess_weights()
+        s.close()
+        log(f"[study] trained on seed {sd}")
+    pr = problems.adr_3d(s=size, seed=eval_seed, preconditioner="diagonal", time_dependent=True)
+    out["dofs"] = pr.num_faces * pr.n
+    forms = {
+        "plain_fp64": (dict(preconditioner="diagonal", check_every=8, tol=1e-10), None),
+        "frozen_network_and_history": (mixed, dict(train_steps=0, combine="combined", history_depth=history_depth)),
+        "history_only": (mixed, dict(train_steps=0, combine="history", history_depth=history_depth)),
+        "fp64_history": (dict(preconditioner="diagonal", check_every=8, tol=1e-10),
+                         dict(train_steps=0, combine="history", history_depth=history_depth)),
+    }
+    for name, (kw, guess) in forms.items():
+        s = capi.Solver(pr, **kw)
+        if guess:
+            s.configure_guess(True, **guess)
+            if guess["combine"] == "combined":
+                s.set_guess_weights(w)
+        t = 0.0
+        s.step(t, pr.dt)
+        t += pr.dt
+        s.synchronize()
+        s.autotune()
+        t0 = time.perf_counter()
+        its = []
+        coefs = []
+        fallbacks = 0
+        for _ in range(eval_steps):
+            st = s.step(t, pr.dt)
+            its.append(st.iterations)
+            fallbacks += st.fallback_iterations > 0
+            if guess:
+                coefs.append(s.guess_stats()["coef_network"])
+            t += pr.dt
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"steps_per_s": round(eval_steps / dt, 4), "ms_per_step": round(1e3 * dt / eval_steps, 2),
+                     "iterations_per_step": round(sum(its) / len(its), 1), "steps_with_fp64_fallback": fallbacks,
+                     "iterations": its}
+        if guess and guess["combine"] == "combined":
+            out[name]["mean_abs_network_coefficient"] = round(sum(abs(c) for c in coefs) / len(coefs), 5)
+        log(f"[study] {name}: {out[name]}")
+        s.close()
+    for name in forms:
+        if name != "plain_fp64":
+            out[name]["vs_plain"] = round(out[name]["steps_per_s"] / out["plain_fp64"]["steps_per_s"], 3)
     return out
 
 
@@ -246,9 +322,16 @@ def main():
     ap.add_argument("--skip-ablation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the ensemble leg")
+    ap.add_argument("--study96", action="store_true",
+                    help="only run the large learned-guess study (96^3, time-dependent boundary value)")
+    ap.add_argument("--study-size", type=int, default=96)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (reference reading for the HBM counters)")
     args = ap.parse_args()
+
+    if args.study96:
+        print(json.dumps({"study": study96(args.study_size, [11, 12], 10, 13, 30, args.history_depth)}))
+        return 0
 
     if args.copy_calibration:
         import torch
@@ -305,7 +388,7 @@ def main():
     blocks_per_face = (pairs.size - faces) / faces
     z_chosen = s.get_array("Z_iF").copy()
     s.close()
-    res_chosen = summary(chosen["name"], args.steps, dt, its, chosen.get("rows_per_block"))
+    res_chosen = summary(chosen["name"], args.steps, dt, its, chosen.get("rows_per_block"), chosen["fallback_steps"])
     log(f"[bench] chosen: {res_chosen}; SpMV {spmv_ms*1e3:.1f} us over {tm['spmv_timed']} timed launches; "
         f"guess {gs}")
 
@@ -332,7 +415,7 @@ def main():
         log(f"[bench] ablation: {ablation['name']}")
         sa, dta, itsa = run_form(pr, ablation, args.steps, args.warmup)
         sa.close()
-        res_abl = summary(ablation["name"], args.steps, dta, itsa, ablation.get("rows_per_block"))
+        res_abl = summary(ablation["name"], args.steps, dta, itsa, ablation.get("rows_per_block"), ablation["fallback_steps"])
         log(f"[bench] ablation: {res_abl}")
         log(f"[bench] {fp64_hist['name']}")
         sh, dth, itsh = run_form(pr, fp64_hist, args.steps, args.warmup)

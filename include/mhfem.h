@@ -110,6 +110,8 @@ typedef struct {
     int outer_iterations;       /* refinement steps (1 for plain FP64) */
     double residue;             /* final relative (preconditioned) residue */
     int converged;
+    int fallback_iterations;    /* FP64 BiCGStab iterations after the FP32 iteration stopped
+                                   reducing the FP64 residual (mixed precision only) */
 } mhfem_solve_stats;
 
 /* Per-phase device times (ms, accumulated) under the reference's timer names

@@ -559,6 +559,11 @@ __global__ void k_set_slots(double *sc, double tolabs, double delta)
     }
 }
 
+__global__ void k_set_slot(double *sc, int slot, double v)
+{
+    if (threadIdx.x == 0) sc[slot] = v;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(KB) k_fill_t(long n, T *__restrict__ x, T v)
 {
@@ -654,6 +659,8 @@ struct Krylov {
     // GMRES
     DBuf<double> V;
     int V_m = 0;
+    // mixed precision: solves left that go straight to FP64 after a fallback
+    int fp64_cooldown = 0;
     // captured step graphs
     hipGraphExec_t g_first = nullptr, g_more = nullptr, g_post = nullptr;
     double g_tau = 0;
@@ -1299,6 +1306,9 @@ void krylov_bench_spmv(Ctx &c, int precision, int reps, double *avg_ms)
 // the vector updates and the reductions, in the precision the solve iterates
 // in -- on the assembled system, from a scratch iterate (the solution is not
 // touched; every solve starts its recurrence afresh), and keeps the fastest.
+// A batched solve ends with its largest instance iterating alone, so the
+// candidates are limited to those that still give that segment about two
+// work-groups per CU (2 x 256).
 int krylov_autotune(Ctx &c, double *best_ms)
 {
     Krylov &k = *c.krylov;
@@ -1308,6 +1318,9 @@ int krylov_autotune(Ctx &c, double *best_ms)
     float bestms = 1e30f;
     const bool fp32 = c.cfg.precision != MHFEM_PRECISION_FP64 && k.val32.p;
     const int iters = c.cfg.preconditioner == MHFEM_PREC_ILU0 ? 2 : 8;
+    int largest = 0;
+    for (int sg = 0; sg < k.nseg; sg++) largest = std::max(largest, c.h_seg_rows[sg + 1] - c.h_seg_rows[sg]);
+    const int limit = std::max(cands[0], largest / 512);
     const unsigned g = grid_for(k.n);
     precond_update(c, k);
     DBuf<double> xs;  // scratch FP64 iterate
@@ -1318,6 +1331,7 @@ int krylov_autotune(Ctx &c, double *best_ms)
     }
     MHFEM_HIP(hipGetLastError());
     for (int rpb : cands) {
+        if (rpb > limit && rpb != cands[0]) continue;
         build_blocks(c, k, rpb);
         Solve S(c, k);
         auto run = [&](auto zero) {
@@ -1409,10 +1423,16 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         double b_norm = std::sqrt(k.h_sc[S_D0]);
         if (b_norm == 0.0) b_norm = 1.0;
         const bool reliable = c.cfg.precision == MHFEM_PRECISION_FP32_RELIABLE;
+        // After a solve whose FP32 iteration stalled, the next solves of the
+        // trajectory (nearby systems) go straight to FP64 for a while.
+        constexpr int cooldown = 8;
+        bool fallback = k.fp64_cooldown > 0;
+        const bool skip_fp32 = fallback;
+        if (skip_fp32) k.fp64_cooldown--;
         const double inner_tol = c.cfg.inner_residue > 0 ? c.cfg.inner_residue : (reliable ? 1e-2 : 1e-4);
         int total = 0, outer = 0;
         double res = 0;
-        for (; reliable;) {
+        for (; reliable && !skip_fp32;) {
             // One FP32 recurrence with reliable updates (Sleijpen & van der
             // Vorst 1996): each time its residual has dropped by inner_tol,
             // the FP32 correction goes into the FP64 solution, the FP64
@@ -1428,11 +1448,21 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
             hipLaunchKernelGGL((k_fill_t<float>), g, KB, 0, c.stream, (long)n, k.x32.p, 0.0f);
             S.bicg_start<float>(rhs32, k.x32.p, v, std::max(inner_tol, tol / res), maxit);
             hipLaunchKernelGGL(k_set_slots, 1, 64, 0, c.stream, k.sc.p, tol * b_norm, inner_tol);
+            // each round between two replacements gets at most `round_cap`
+            // iterations, so that a recurrence that cannot reach the next
+            // replacement still reaches the stall test below
+            const int round_cap = 300;
+            hipLaunchKernelGGL(k_set_slot, 1, 64, 0, c.stream, k.sc.p, (int)S_MAXIT, (double)std::min(maxit, round_cap));
             MHFEM_HIP(hipGetLastError());
             const int every = std::max(1, c.cfg.check_every);
+            int stalls = 0;
             for (;;) {
                 total = S.bicg_drive<float>(rhs32, k.x32.p, v,
                                             [&]() { S.bicg_iterations<float>(k.x32.p, v, every); });
+                if (!std::isfinite(k.h_sc[S_RESID])) {  // a broken FP32 recurrence is not flushed
+                    fallback = true;
+                    break;
+                }
                 hipLaunchKernelGGL(k_add_correction, g, KB, 0, c.stream, n, x, k.x32.p);
                 S.residual_mixed(x, b, rhs32);
                 MHFEM_HIP(hipGetLastError());
@@ -1440,27 +1470,58 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                 S.dots<float, 2>(DotArgs<float, 2>{{r, r}, {r, k.r_ast32.p}}, FIN_REPLACE, M_ALL);
                 S.poll();
                 outer++;
+                const double prev = res;
                 res = std::sqrt(k.h_sc[S_D0]) / b_norm;
                 if (!(res >= tol) || total >= maxit) break;
+                // FP32 iterations only refine while kappa * eps_fp32 << 1: two
+                // replacements in a row that do not halve the FP64 residual
+                // hand the rest of the solve to FP64 BiCGStab
+                stalls = res > 0.5 * prev ? stalls + 1 : 0;
+                if (stalls >= 2) {
+                    fallback = true;
+                    break;
+                }
+                hipLaunchKernelGGL(k_set_slot, 1, 64, 0, c.stream, k.sc.p, (int)S_MAXIT,
+                                   (double)std::min(maxit, total + round_cap));
                 hipLaunchKernelGGL((k_bicg_p<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)r, k.p32.p,
                                    (const float *)k.Ap32.p);
                 MHFEM_HIP(hipGetLastError());
             }
             break;
         }
-        for (; !reliable;) {
+        double res_before = 0.0;
+        for (; !reliable && !skip_fp32;) {
             S.residual_mixed(x, b, rhs32);
             S.prec<float>(rhs32, z, k.dinv32.p, M_ALL);
             S.dots<float, 1>(DotArgs<float, 1>{{z}, {z}}, FIN_NONE, M_ALL);
             S.poll();
             res = std::sqrt(k.h_sc[S_D0]) / b_norm;
-            if (!(res >= tol) || total >= maxit || outer >= 100) break;
+            if (!(res >= tol) || total >= maxit) break;
+            if (outer >= 2 && res > 0.5 * res_before) {  // refinement no longer converging: FP64 takes over
+                fallback = true;
+                break;
+            }
+            res_before = res;
             hipLaunchKernelGGL((k_fill_t<float>), g, KB, 0, c.stream, (long)n, k.x32.p, 0.0f);
             MHFEM_HIP(hipGetLastError());
             total += S.bicgstab<float>(rhs32, k.x32.p, v, inner_tol, std::max(1, maxit - total));
+            if (!std::isfinite(k.h_sc[S_RESID])) {
+                fallback = true;
+                break;
+            }
             hipLaunchKernelGGL(k_add_correction, g, KB, 0, c.stream, n, x, k.x32.p);
             MHFEM_HIP(hipGetLastError());
             outer++;
+        }
+        if (fallback && !skip_fp32) k.fp64_cooldown = cooldown;
+        if (fallback && total < maxit) {
+            // FP64 BiCGStab from the current iterate; its stopping test is the
+            // plain method's
+            Solve::Vecs v64{k.r.p, k.r_ast.p, k.p.p, k.Ap.p, k.s.p, k.As.p, k.tmp.p, k.tmp2.p, k.dinv.p};
+            const int it64 = S.bicgstab<double>(b, x, v64, tol, std::max(1, maxit - total));
+            total += it64;
+            out.fallback_iterations = it64;
+            res = S.seg_sc(0)[S_RESID];
         }
         out.iterations = total;
         out.outer_iterations = outer;

@@ -45,7 +45,7 @@ class SolverConfig(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int),
-                ("residue", C.c_double), ("converged", C.c_int)]
+                ("residue", C.c_double), ("converged", C.c_int), ("fallback_iterations", C.c_int)]
 
 
 class Timers(C.Structure):
