@@ -37,6 +37,12 @@ CASES = {
                                                          advection=problems.IMPLICIT_UPWIND),
     "adr3d_implicit_trace": lambda: dataclasses.replace(problems.adr_3d(s=4, steps=2, preconditioner="diagonal"),
                                                         advection=problems.IMPLICIT_TRACE),
+    "adr3d_n3": lambda: problems.adr_3d(s=3, steps=2, preconditioner="diagonal", n=3),
+    "adr3d_n3_implicit_upwind": lambda: dataclasses.replace(
+        problems.adr_3d(s=3, steps=2, preconditioner="diagonal", n=3), advection=problems.IMPLICIT_UPWIND),
+    "adr2d_n2": lambda: problems.adr_2d(s=8, n=2),
+    "adr2d_n3_implicit_trace": lambda: dataclasses.replace(problems.adr_2d(s=6, n=3),
+                                                           advection=problems.IMPLICIT_TRACE),
 }
 
 

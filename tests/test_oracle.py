@@ -269,6 +269,14 @@ def _tiny_problem(kind, seed):
         pr = dataclasses.replace(pr, points=pts)
     elif kind == "tet_n1":
         pr = problems.darcy_3d(s=2, steps=2, preconditioner="diagonal")
+    elif kind == "adr_n3":
+        pr = problems.adr_3d(s=2, steps=2, preconditioner="diagonal", n=3)
+    elif kind == "tri_adr_n2":
+        pr = problems.adr_2d(s=3, n=2, steps=2)
+        pts = pr.points.copy()
+        inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
+        pts[inner] += 0.05 * rng.standard_normal((inner.sum(), 2))
+        pr = dataclasses.replace(pr, points=pts)
     else:
         pr = problems.adr_3d(s=2, steps=2, preconditioner="diagonal")
     n, C, F, fpc = pr.n, pr.num_cells, pr.num_faces, pr.fpc
@@ -276,9 +284,9 @@ def _tiny_problem(kind, seed):
     co["f_iK"] = rng.uniform(-1, 1, n * C)
     co["r_ijK"] = (np.eye(n)[:, :, None] * rng.uniform(0, 0.5, (1, 1, C))).ravel()
     co["N_ijK"] = (np.eye(n)[:, :, None] * rng.uniform(0.5, 2, (1, 1, C))).ravel()
-    if n == 2:
+    if n >= 2:
         K = rng.uniform(0.5, 2, C)
-        co["D_ijK"] = np.stack([K, 0.1 * K, 0.1 * K, K]).ravel()
+        co["D_ijK"] = np.stack([K if i == j else 0.1 * K for i in range(n) for j in range(n)]).ravel()
     else:
         co["D_ijK"] = rng.uniform(0.5, 2.0, C)
     co["w_iKe"] = 0.1 * rng.standard_normal(n * fpc * C)
@@ -297,7 +305,7 @@ def _tiny_problem(kind, seed):
                                do_mobility_upwind=0, bc_mobility=np.ones(n * F))
 
 
-@pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2"])
+@pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2", "adr_n3", "tri_adr_n2"])
 def test_brute_force_mixed_system(kind):
     pr = _tiny_problem(kind, 3)
     o = pyoracle.Oracle(pr)

@@ -164,6 +164,32 @@ std::vector<uint64_t> build_face_slots(const Ctx &c)
     return slots;
 }
 
+// Store map of the interior assembly, per face E: the block index, in the
+// SELL row of E, of the column face at local index f of K0 (bits 3f..3f+2)
+// and of K1 (bits 12+3f..), found by the slot walk of the device-side
+// slot_blocks (the face E itself has no block: its field is 0, unused); the
+// local index of E in K0 (bits 24-26) and K1 (bits 27-29).  Boundary faces
+// use the slot word directly.
+std::vector<uint32_t> build_face_blocks(const std::vector<uint64_t> &slots)
+{
+    std::vector<uint32_t> out(slots.size());
+    for (size_t E = 0; E < slots.size(); E++) {
+        const uint64_t info = slots[E];
+        const int e0 = (info >> 48) & 7, e1 = (info >> 51) & 7, ns = (info >> 56) & 15;
+        uint32_t w = (uint32_t)e0 << 24 | (uint32_t)e1 << 27;
+        int k = 0;
+        for (int t = 0; t < ns; t++) {
+            const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
+            if (f0 == e0) continue;
+            if (f0 < 4) w |= (uint32_t)k << (3 * f0);
+            if (f1 < 4) w |= (uint32_t)k << (12 + 3 * f1);
+            k++;
+        }
+        out[E] = w;
+    }
+    return out;
+}
+
 template <typename T>
 void upload(DBuf<T> &d, const T *h, size_t n)
 {
@@ -245,6 +271,7 @@ SchemeView Ctx::view()
     s.points = points.p;
     s.face_cells = face_cells.p;
     s.face_slots = face_slots.p;
+    s.face_blocks = face_blocks.p;
     s.bnd_faces = bnd_faces.p;
     s.nbnd = nbnd;
     s.cell_meas = arr[MHFEM_CELL_MEASURE].p;
@@ -263,7 +290,7 @@ SchemeView Ctx::view()
     s.v = arr[MHFEM_v_iKe].p;
     s.m_upw = arr[MHFEM_m_iE_upw].p;
     s.Z_upw = arr[MHFEM_Z_ijE_upw].p;
-    s.b = arr[MHFEM_b_ijK_storage].p;
+    s.minv = minv.p;
     s.R_ijKe = arr[MHFEM_R_ijKe].p;
     s.R_iK = arr[MHFEM_R_iK].p;
     s.tags = tags.p;
@@ -335,6 +362,8 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
         upload(c->points, mesh->points, (size_t)c->nv * c->dim);
         std::vector<uint64_t> slots = build_face_slots(*c);
         upload(c->face_slots, slots.data(), slots.size());
+        const std::vector<uint32_t> fblocks = build_face_blocks(slots);
+        upload(c->face_blocks, fblocks.data(), fblocks.size());
         std::vector<int> bfaces;
         for (int E = 0; E < c->F; E++)
             if (c->h_face_cells[2 * (size_t)E + 1] < 0) bfaces.push_back(E);
@@ -362,6 +391,10 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
         c->h_tags.assign((size_t)c->n * c->F, MHFEM_BC_FIXED_VALUE);
         upload(c->tags, c->h_tags.data(), c->h_tags.size());
         MHFEM_HIP(hipDeviceSynchronize());
+        // fixed geometry: M^{-1} of every cell, once
+        c->minv.alloc((size_t)(c->fpc * (c->fpc + 1) / 2) * Cn);
+        launch_geometry(*c);
+        MHFEM_HIP(hipStreamSynchronize(c->stream));
         return reinterpret_cast<mhfem_ctx *>(c);
     } catch (const Error &e) {
         g_create_error = e.msg;
@@ -514,6 +547,7 @@ int mhfem_get_array(mhfem_ctx *ctx, int which, double *host, long count)
         const long have = (is_coefficient(which) && c.uniform[which]) ? 1 : c.array_size(which);
         if (count != have || !c.arr[which].p)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_get_array: count mismatch (expected " + std::to_string(have) + ")"};
+        if (which == MHFEM_b_ijK_storage) launch_fill_b(c);  // formed on request from M^{-1} and D
         MHFEM_HIP(hipMemcpyAsync(host, c.arr[which].p, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
     });
@@ -524,6 +558,15 @@ void *mhfem_device_array(mhfem_ctx *ctx, int which)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || which < 0 || which >= MHFEM_NUM_ARRAYS) return nullptr;
     if (is_coefficient(which) && c->uniform[which]) return nullptr;
+    if (which == MHFEM_b_ijK_storage) {
+        // formed on request from M^{-1} and the current D (a snapshot)
+        try {
+            launch_fill_b(*c);
+        } catch (const Error &e) {
+            c->err = e.msg;
+            return nullptr;
+        }
+    }
     return c->arr[which].p;
 }
 

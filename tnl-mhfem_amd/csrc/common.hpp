@@ -133,13 +133,15 @@ struct SchemeView {
     const double *points;       // nv*dim
     const int *face_cells;      // F*2
     const uint64_t *face_slots; // per face: merged column order (see build_pattern)
+    const uint32_t *face_blocks; // per face: block index of each cell's local faces (build_face_blocks)
     const int *bnd_faces;       // the boundary faces, ascending
     int nbnd;
     const double *cell_meas, *face_meas;
     // coefficients
     Coef N, u, m, D, w, a, r, f;
     // scheme arrays
-    double *Z_iF, *Z_iK, *v, *m_upw, *Z_upw, *b, *R_ijKe, *R_iK;
+    double *Z_iF, *Z_iK, *v, *m_upw, *Z_upw, *R_ijKe, *R_iK;
+    const double *minv;         // packed M^{-1} per cell, NP planes of C (fixed geometry)
     // boundary conditions
     const uint8_t *tags;
     const double *bc_values, *bc_dirichlet, *bc_mobility;
@@ -188,7 +190,9 @@ struct Ctx {
     // device topology
     DBuf<int> cell_faces, cell_vertices, face_cells, bnd_faces;
     DBuf<uint64_t> face_slots;
+    DBuf<uint32_t> face_blocks;
     DBuf<double> points, cell_meas, face_meas;
+    DBuf<double> minv;  // packed inverse geometric mass matrices (k_geometry)
     // model arrays (index by MHFEM_* ids); uniform coefficients hold 1 value
     DBuf<double> arr[MHFEM_NUM_ARRAYS];
     bool uniform[MHFEM_NUM_ARRAYS] = {};
@@ -268,6 +272,8 @@ void launch_face_average(Ctx &c);
 void launch_pre_iterate(Ctx &c, double time, double tau);
 void launch_assemble(Ctx &c, double time, double tau);
 void launch_post_iterate(Ctx &c);
+void launch_geometry(Ctx &c);
+void launch_fill_b(Ctx &c);
 void launch_zero_double(Ctx &c, double *p, size_t n);
 
 // krylov.hip

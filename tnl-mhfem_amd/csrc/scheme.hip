@@ -151,14 +151,94 @@ __device__ __forceinline__ void geom_minv(const double (&P)[FPC][FPC - 1], doubl
     }
 }
 
-// row sum of the scaled packed matrix, ascending f (MassMatrix.h:594-597, 864-868)
-template <int FPC>
-__device__ __forceinline__ double row_sum(const double (&minv)[Simplex<FPC>::NP], double D, int e)
+// One entry of b_ijK = M^{-1} D_ij (MassMatrix.h:586-592), rounded once as a
+// stored b_ijK_storage value is: contraction is off, so the product never
+// fuses with the operation that consumes it.
+__device__ __forceinline__ double b_entry(double minv_ef, double D)
 {
-    double s = minv[pk(e, 0)] * D;
+#pragma clang fp contract(off)
+    return minv_ef * D;
+}
+
+// Row sum of row e of b_ijK, ascending f (MassMatrix.h:594-597, 864-868), from
+// mrow[f] = M^{-1}_ef.  Contraction is off: every kernel that forms the sum
+// gets the same bits.
+template <int FPC>
+__device__ __forceinline__ double row_sum(const double (&mrow)[FPC], double D)
+{
+#pragma clang fp contract(off)
+    double s = mrow[0] * D;
 #pragma unroll
-    for (int f = 1; f < FPC; f++) s += minv[pk(e, f)] * D;
+    for (int f = 1; f < FPC; f++) s += mrow[f] * D;
     return s;
+}
+
+// row e of the packed M^{-1} held in registers (e a compile-time index)
+template <int FPC>
+__device__ __forceinline__ void minv_row(const double (&minv)[Simplex<FPC>::NP], int e, double (&mrow)[FPC])
+{
+#pragma unroll
+    for (int f = 0; f < FPC; f++) mrow[f] = minv[pk(e, f)];
+}
+
+// M^{-1} of cell K from its packed planes (minv[q*C + K])
+template <int FPC>
+__device__ __forceinline__ void load_minv(const SchemeView &s, int K, double (&minv)[Simplex<FPC>::NP])
+{
+#pragma unroll
+    for (int q = 0; q < Simplex<FPC>::NP; q++) minv[q] = s.minv[(long)q * s.C + K];
+}
+
+// ---------------------------------------------------------------------------
+// Inverse of the geometric RTN0 mass matrix of every cell (Solver.hpp:627-639,
+// MassMatrix::update), packed in NP planes of C values.  It depends only on
+// the mesh and the lumping, both fixed when the context is created, so it is
+// computed once; b_ijK = M^{-1} D_ij is formed from it where it is used.
+template <int FPC>
+__global__ void __launch_bounds__(WG) k_geometry(SchemeView s, double *minv_out)
+{
+    constexpr int DIM = FPC - 1, NP = Simplex<FPC>::NP;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    double P[FPC][DIM];
+#pragma unroll
+    for (int e = 0; e < FPC; e++) {
+        const int vtx = s.cell_vertices[(long)K * FPC + e];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) P[e][c] = s.points[(long)vtx * DIM + c];
+    }
+    double minv[NP];
+    geom_minv<FPC>(P, s.cell_meas[K], s.lumping, minv);
+#pragma unroll
+    for (int q = 0; q < NP; q++) minv_out[(long)q * s.C + K] = minv[q];
+}
+
+// b_ijK_storage ((i*n+j)*S + q)*C + K: the packed b_ijK followed by its row
+// sums (MassMatrix.h:586-597), written only when a caller asks for the array.
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_fill_b(SchemeView s, double *b)
+{
+    constexpr int NP = Simplex<FPC>::NP, S = Simplex<FPC>::S;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    const long C = s.C;
+    double minv[NP];
+    load_minv<FPC>(s, K, minv);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double D = s.D[((long)i * N + j) * C + K];
+            double *bp = b + ((long)(i * N + j) * S) * C + K;
+#pragma unroll
+            for (int q = 0; q < NP; q++) bp[(long)q * C] = b_entry(minv[q], D);
+#pragma unroll
+            for (int e = 0; e < FPC; e++) {
+                double mrow[FPC];
+                minv_row<FPC>(minv, e, mrow);
+                bp[(long)(NP + e) * C] = row_sum<FPC>(mrow, D);
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -234,25 +314,18 @@ __global__ void __launch_bounds__(WG) k_upwind(SchemeView s)
 template <int N, int FPC>
 __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
 {
-    using SX = Simplex<FPC>;
-    constexpr int DIM = FPC - 1, NP = SX::NP, S = SX::S;
+    constexpr int NP = Simplex<FPC>::NP;
     const int K = blockIdx.x * WG + threadIdx.x;
     if (K >= s.C) return;
     const long C = s.C;
     int fi[FPC];
-    double P[FPC][DIM];
 #pragma unroll
-    for (int e = 0; e < FPC; e++) {
-        fi[e] = s.cell_faces[(long)K * FPC + e];
-        const int vtx = s.cell_vertices[(long)K * FPC + e];
-#pragma unroll
-        for (int c = 0; c < DIM; c++) P[e][c] = s.points[(long)vtx * DIM + c];
-    }
+    for (int e = 0; e < FPC; e++) fi[e] = s.cell_faces[(long)K * FPC + e];
     const double meas = s.cell_meas[K];
     double minv[NP];
-    geom_minv<FPC>(P, meas, s.lumping, minv);
+    load_minv<FPC>(s, K, minv);
 
-    // b = M^{-1} D_ij, stored packed + row sums (b_ijK_storage)
+    // b = M^{-1} D_ij and its row sums (b_ijK_storage, formed in registers)
     double Dij[N][N], rs[N][N][FPC];
 #pragma unroll
     for (int i = 0; i < N; i++)
@@ -260,13 +333,11 @@ __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
         for (int j = 0; j < N; j++) {
             const double D = s.D[((long)i * N + j) * C + K];
             Dij[i][j] = D;
-            double *bp = s.b + ((long)(i * N + j) * S) * C + K;
-#pragma unroll
-            for (int q = 0; q < NP; q++) bp[(long)q * C] = minv[q] * D;
 #pragma unroll
             for (int e = 0; e < FPC; e++) {
-                rs[i][j][e] = row_sum<FPC>(minv, D, e);
-                bp[(long)(NP + e) * C] = rs[i][j][e];
+                double mrow[FPC];
+                minv_row<FPC>(minv, e, mrow);
+                rs[i][j][e] = row_sum<FPC>(mrow, D);
             }
         }
 
@@ -291,7 +362,7 @@ __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
             for (int e = 0; e < FPC; e++) {
                 double r = 0;
 #pragma unroll
-                for (int f = 0; f < FPC; f++) r += mu[i][f] * (minv[pk(f, e)] * Dij[i][j]);
+                for (int f = 0; f < FPC; f++) r += mu[i][f] * b_entry(minv[pk(f, e)], Dij[i][j]);
                 if (s.advection == MHFEM_IMPLICIT_UPWIND) {
                     const double vel = vel_of(i, j, e);
                     if (vel < 0) r -= 2 * vel;
@@ -375,103 +446,140 @@ __device__ __forceinline__ double pick(const double (&a)[M], int i)
 // ---------------------------------------------------------------------------
 // Face-parallel hybridised assembly (Solver.hpp:933-959).
 //
-// CellRows<N, FPC>::compute evaluates, for one cell K and its local face e
-// (the face E whose rows are assembled), the cell's contributions to all N
-// rows (E, i):
-//   A[i][j][f] = A_ijKEF against unknown j of local face f,
-//   rhs[i]     = RHS_iKE.
+// CellRow<N, FPC>::compute evaluates, for one cell K, its local face e (the
+// face E whose rows are assembled) and one row i, the cell's contribution to
+// row (E, i):
+//   A[j][f] = A_ijKEF against unknown j of local face f,  Ae[j] = A[j][e],
+//   rhs     = RHS_iKE.
 // Every register array is indexed by compile-time constants (the runtime
-// local indices only enter load and store addresses), so nothing is demoted
-// to scratch memory; R_xjKf is loaded once for all N rows.  Ae is selected
-// while the values are computed: a select over an array after the fact is
-// folded by the compiler into a load from a selected address, which keeps
-// the whole array in scratch memory.
-template <int N, int FPC>
-struct CellRows {
-    double A[N][N][FPC];
-    double Ae[N][N];  // A[i][j][e]: the column of the face E itself
-    double rhs[N];
+// local indices and the row only enter load and store addresses), so nothing
+// is demoted to scratch memory.  Ae is selected while the values are
+// computed: a select over an array after the fact is folded by the compiler
+// into a load from a selected address, which keeps the whole array in
+// scratch memory.  The entries of b_ijK are formed from row e of M^{-1} and
+// D_ij (b_entry, row_sum), rounded as the stored values would be.
 
-    // use_adv: A_ijKEF_advection / RHS_iKE_advection (SecondaryCoefficients.h:215-259,
-    // 301-333); otherwise the no-advection forms (:189-211, 282-297)
-    __device__ __forceinline__ void compute(const SchemeView &s, int adv, bool use_adv, int K, int e, int E)
+// Values of cell K that every row of its local face e uses: R_xjKf, R_jK and
+// row e of M^{-1}.
+template <int N, int FPC>
+struct CellData {
+    double Rv[N][N][FPC], RK[N], mrow[FPC];
+    __device__ __forceinline__ void load(const SchemeView &s, int K, int e)
     {
-        constexpr int NP = Simplex<FPC>::NP, S = Simplex<FPC>::S;
         const long C = s.C;
-        double Rv[N][N][FPC];
 #pragma unroll
         for (int x = 0; x < N; x++)
 #pragma unroll
             for (int j = 0; j < N; j++)
 #pragma unroll
                 for (int f = 0; f < FPC; f++) Rv[x][j][f] = s.R_ijKe[(((long)x * N + j) * FPC + f) * C + K];
-        double RK[N];
 #pragma unroll
         for (int j = 0; j < N; j++) RK[j] = s.R_iK[(long)j * C + K];
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            double bf[N][FPC], be[N];
+        for (int f = 0; f < FPC; f++) mrow[f] = s.minv[(long)pk(e, f) * C + K];
+    }
+};
+
+template <int N, int FPC>
+struct CellRow {
+    double A[N][FPC];
+    double Ae[N];
+    double Aii;  // A[i][e], the diagonal entry before scaling
+    double rhs;
+
+    // Row inputs: Di[j] = D_ijK, wv = w_iKe.
+    __device__ __forceinline__ static void load_row(const SchemeView &s, int K, int e, int i, double (&Di)[N], double &wv)
+    {
+        const long C = s.C;
+#pragma unroll
+        for (int j = 0; j < N; j++) Di[j] = s.D[((long)i * N + j) * C + K];
+        wv = s.w[((long)i * FPC + e) * C + K];
+    }
+
+    // use_adv: A_ijKEF_advection / RHS_iKE_advection (SecondaryCoefficients.h:215-259,
+    // 301-333); otherwise the no-advection forms (:189-211, 282-297)
+    __device__ __forceinline__ void compute(const SchemeView &s, const CellData<N, FPC> &cd, const double (&Di)[N],
+                                            double wv, int adv, bool use_adv, int K, int e, int E, int i)
+    {
+        const long C = s.C;
+        double bf[N][FPC], be[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+#pragma unroll
+            for (int f = 0; f < FPC; f++) bf[j][f] = b_entry(cd.mrow[f], Di[j]);
+            be[j] = row_sum<FPC>(cd.mrow, Di[j]);
+        }
+        if (!use_adv) {
+#pragma unroll
+            for (int j = 0; j < N; j++)
+#pragma unroll
+                for (int f = 0; f < FPC; f++) {
+                    double SQinvR = 0;
+#pragma unroll
+                    for (int x = 0; x < N; x++) SQinvR += be[x] * cd.Rv[x][j][f];
+                    A[j][f] = bf[j][f] - SQinvR;
+                    if (f == e) Ae[j] = A[j][f];
+                    if (f == e && j == i) Aii = A[j][f];
+                }
+            double v = wv;
+#pragma unroll
+            for (int j = 0; j < N; j++) v += be[j] * cd.RK[j];
+            rhs = v;
+        } else {
+            const double mu = s.m_upw[(long)E * N + i];
+            double vel[N], Sx[N];
 #pragma unroll
             for (int j = 0; j < N; j++) {
-                const double *bp = s.b + ((long)(i * N + j) * S) * C + K;
-#pragma unroll
-                for (int f = 0; f < FPC; f++) bf[j][f] = bp[(long)pk(e, f) * C];
-                be[j] = bp[(long)(NP + e) * C];
+                const long idx = (((long)i * N + j) * FPC + e) * C + K;
+                vel[j] = s.a[idx] + s.u[idx];
             }
-            const double wv = s.w[((long)i * FPC + e) * C + K];
-            if (!use_adv) {
 #pragma unroll
-                for (int j = 0; j < N; j++)
-#pragma unroll
-                    for (int f = 0; f < FPC; f++) {
-                        double SQinvR = 0;
-#pragma unroll
-                        for (int x = 0; x < N; x++) SQinvR += be[x] * Rv[x][j][f];
-                        A[i][j][f] = bf[j][f] - SQinvR;
-                        if (f == e) Ae[i][j] = A[i][j][f];
-                    }
-                double v = wv;
-#pragma unroll
-                for (int j = 0; j < N; j++) v += be[j] * RK[j];
-                rhs[i] = v;
-            } else {
-                const double mu = s.m_upw[(long)E * N + i];
-                double vel[N], Sx[N];
-#pragma unroll
-                for (int j = 0; j < N; j++) {
-                    const long idx = (((long)i * N + j) * FPC + e) * C + K;
-                    vel[j] = s.a[idx] + s.u[idx];
+            for (int x = 0; x < N; x++) {
+                double Sv = mu * be[x];
+                if (adv == MHFEM_IMPLICIT_UPWIND) {
+                    if (vel[x] >= 0) Sv += vel[x];
+                    else Sv -= vel[x];
                 }
-#pragma unroll
-                for (int x = 0; x < N; x++) {
-                    double Sv = mu * be[x];
-                    if (adv == MHFEM_IMPLICIT_UPWIND) {
-                        if (vel[x] >= 0) Sv += vel[x];
-                        else Sv -= vel[x];
-                    }
-                    Sx[x] = Sv;
-                }
-#pragma unroll
-                for (int j = 0; j < N; j++)
-#pragma unroll
-                    for (int f = 0; f < FPC; f++) {
-                        double SQinvR = 0;
-#pragma unroll
-                        for (int x = 0; x < N; x++) SQinvR += Sx[x] * Rv[x][j][f];
-                        double T = mu * bf[j][f];
-                        if (f == e && adv == MHFEM_IMPLICIT_UPWIND && vel[j] < 0) T -= 2 * vel[j];
-                        if (f == e && adv == MHFEM_IMPLICIT_TRACE) T -= vel[j];
-                        A[i][j][f] = T - SQinvR;
-                        if (f == e) Ae[i][j] = A[i][j][f];
-                    }
-                double v = mu * wv;
-#pragma unroll
-                for (int j = 0; j < N; j++) {
-                    v += Sx[j] * RK[j];
-                    if (adv == MHFEM_EXPLICIT_UPWIND) v += s.Z_upw[((long)E * N + j) * N + i] * vel[j];
-                }
-                rhs[i] = v;
+                Sx[x] = Sv;
             }
+#pragma unroll
+            for (int j = 0; j < N; j++)
+#pragma unroll
+                for (int f = 0; f < FPC; f++) {
+                    double SQinvR = 0;
+#pragma unroll
+                    for (int x = 0; x < N; x++) SQinvR += Sx[x] * cd.Rv[x][j][f];
+                    double T = mu * bf[j][f];
+                    if (f == e && adv == MHFEM_IMPLICIT_UPWIND && vel[j] < 0) T -= 2 * vel[j];
+                    if (f == e && adv == MHFEM_IMPLICIT_TRACE) T -= vel[j];
+                    A[j][f] = T - SQinvR;
+                    if (f == e) Ae[j] = A[j][f];
+                    if (f == e && j == i) Aii = A[j][f];
+                }
+            double v = mu * wv;
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                v += Sx[j] * cd.RK[j];
+                if (adv == MHFEM_EXPLICIT_UPWIND) v += s.Z_upw[((long)E * N + j) * N + i] * vel[j];
+            }
+            rhs = v;
+        }
+    }
+};
+
+// All N rows of cell K for its local face e (the boundary rows).
+template <int N, int FPC>
+struct CellRows {
+    CellRow<N, FPC> row[N];
+    __device__ __forceinline__ void compute(const SchemeView &s, int adv, bool use_adv, int K, int e, int E)
+    {
+        CellData<N, FPC> cd;
+        cd.load(s, K, e);
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double Di[N], wv;
+            CellRow<N, FPC>::load_row(s, K, e, i, Di, wv);
+            row[i].compute(s, cd, Di, wv, adv, use_adv, K, e, E, i);
         }
     }
 };
@@ -512,49 +620,63 @@ __device__ __forceinline__ void store_block_row(const SchemeView &s, int E, int 
 // and RightHandSide::getValue (RightHandSide.h:18-45).  Every column face
 // other than E belongs to exactly one of the two cells; the diagonal block
 // and the right-hand side add both cells' contributions (K0 first).
+//
+// 2*NR lanes per face, NR = N rounded up to a power of two: lane
+// (E, c, i) evaluates row i of cell K_c and stores that row of the blocks of
+// K_c's faces, so the two rows of a block leave as one contiguous write.  The
+// lanes of one row in the two cells swap their diagonal columns and
+// right-hand sides with a lane shuffle; a sum of two terms is the same in
+// either order, so both form the identical scale d.  Few registers per lane
+// keep many waves in flight to cover the gathers of the cell data.
+// face_blocks[E] gives the block index of every local face of K0 (bits
+// 3f..3f+2) and K1 (12+3f..) in the SELL row of E, and the local index of E
+// in K0 (bits 24-26) and K1 (27-29).
 template <int N, int FPC>
 __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
 {
-    const int E = blockIdx.x * WG + threadIdx.x;
-    if (E >= s.F) return;
-    const int K1 = s.face_cells[2 * E + 1];
-    if (K1 < 0) return;  // boundary faces: k_assemble_boundary
-    const int K0 = s.face_cells[2 * E];
-    const uint64_t info = s.face_slots[E];
-    const int e0 = (info >> 48) & 7, e1 = (info >> 51) & 7;
-    const int nslots = (info >> 56) & 15;
-    int kp0[FPC], kp1[FPC];
-    slot_blocks<FPC>(info, nslots, e0, kp0, kp1);
+    constexpr int NR = N == 3 ? 4 : N;
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    const int E = (int)(t / (2 * NR)), c = (int)(t / NR) & 1, i = (int)(t % NR);
+    if (E >= s.F || i >= N) return;  // the partner lane (same E and i) returns too
+    // Both cells (one 8-byte load) and the store map in one round trip: the
+    // exit test reads the map too (its bit 31 is always clear), so neither
+    // load waits behind the branch.
+    const uint64_t cells = reinterpret_cast<const uint64_t *>(s.face_cells)[E];
+    const uint32_t map = s.face_blocks[E];
+    const int K0 = (int)(uint32_t)cells, K1 = (int)(uint32_t)(cells >> 32);
+    if ((K1 | (int)(map & 0x80000000u)) < 0) return;  // boundary faces: k_assemble_boundary
+    const int K = c ? K1 : K0;
+    const int e = (map >> (24 + 3 * c)) & 7;
     const int adv = s.advection;
     const bool use_adv = adv != MHFEM_EXPLICIT_UPWIND;  // SecondaryCoefficients.h:261-278
-    CellRows<N, FPC> c0, c1;
-    c0.compute(s, adv, use_adv, K0, e0, E);
-    c1.compute(s, adv, use_adv, K1, e1, E);
+    CellData<N, FPC> cd;
+    cd.load(s, K, e);
+    double Di[N], wv;
+    CellRow<N, FPC>::load_row(s, K, e, i, Di, wv);
+    CellRow<N, FPC> r;
+    r.compute(s, cd, Di, wv, adv, use_adv, K, e, E, i);
+    double other[N];
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        const double d = c0.Ae[i][i] + c1.Ae[i][i];
+    for (int j = 0; j < N; j++) other[j] = __shfl_xor(r.Ae[j], NR);
+    const double other_rhs = __shfl_xor(r.rhs, NR);
+    const double d = r.Aii + __shfl_xor(r.Aii, NR);
 #pragma unroll
-        for (int f = 0; f < FPC; f++) {
-            double v[N];
-            if (f != e0) {
+    for (int f = 0; f < FPC; f++) {
+        if (f == e) continue;
+        double v[N];
 #pragma unroll
-                for (int j = 0; j < N; j++) v[j] = c0.A[i][j][f] / d;
-                store_block_row<N>(s, E, kp0[f], i, v);
-            }
-            if (f != e1) {
-#pragma unroll
-                for (int j = 0; j < N; j++) v[j] = c1.A[i][j][f] / d;
-                store_block_row<N>(s, E, kp1[f], i, v);
-            }
-        }
+        for (int j = 0; j < N; j++) v[j] = r.A[j][f] / d;
+        store_block_row<N>(s, E, (map >> (12 * c + 3 * f)) & 7, i, v);
+    }
+    if (c == 0) {
         // the diagonal entry itself is exactly 1 (LinearSystem.h:196-201) and is not stored
 #pragma unroll
         for (int j = 0; j < N; j++)
-            if (j != i)
-                s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = (c0.Ae[i][j] + c1.Ae[i][j]) / d;
+            if (j != i) s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = (r.Ae[j] + other[j]) / d;
+    } else {
         double rhs = 0.0;
-        rhs += c0.rhs[i];
-        rhs += c1.rhs[i];
+        rhs += other_rhs;  // K0
+        rhs += r.rhs;      // K1
         s.rhs[(long)E * N + i] = rhs / d;
     }
 }
@@ -604,10 +726,10 @@ __global__ void __launch_bounds__(WG) k_assemble_boundary(SchemeView s)
         for (int j = 0; j < N; j++) {
 #pragma unroll
             for (int f = 0; f < FPC; f++) {
-                const double v = use_adv ? ca.A[i][j][f] : cn.A[i][j][f];
+                const double v = use_adv ? ca.row[i].A[j][f] : cn.row[i].A[j][f];
                 A[j][f] = inflow ? ((i == j && f == e0) ? 1.0 : 0.0) : v;
             }
-            const double ve = use_adv ? ca.Ae[i][j] : cn.Ae[i][j];
+            const double ve = use_adv ? ca.row[i].Ae[j] : cn.row[i].Ae[j];
             Ae[j] = inflow ? (i == j ? 1.0 : 0.0) : ve;
         }
         // RowSetter::setRow (BoundaryConditions.hpp:132-222); unit diagonal not stored (:200-203)
@@ -626,10 +748,10 @@ __global__ void __launch_bounds__(WG) k_assemble_boundary(SchemeView s)
         double bval;
         if (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) {
             bval = 0;
-            bval += inflow ? s.Z_iK[(long)i * C + K0] : cn.rhs[i];
+            bval += inflow ? s.Z_iK[(long)i * C + K0] : cn.row[i].rhs;
         } else {
             bval = -s.bc_values[bi] * s.face_meas[E];
-            bval += use_adv ? ca.rhs[i] : cn.rhs[i];
+            bval += use_adv ? ca.row[i].rhs : cn.row[i].rhs;
         }
         s.rhs[(long)E * N + i] = bval / d;
     }
@@ -641,7 +763,7 @@ __global__ void __launch_bounds__(WG) k_assemble_boundary(SchemeView s)
 template <int N, int FPC>
 __global__ void __launch_bounds__(WG) k_post(SchemeView s)
 {
-    constexpr int NP = Simplex<FPC>::NP, S = Simplex<FPC>::S;
+    constexpr int NP = Simplex<FPC>::NP;
     const int K = blockIdx.x * WG + threadIdx.x;
     if (K >= s.C) return;
     const long C = s.C;
@@ -666,17 +788,21 @@ __global__ void __launch_bounds__(WG) k_post(SchemeView s)
         s.Z_iK[(long)i * C + K] = Z;
     }
     if (!s.do_mob) return;
+    double minv[NP];
+    load_minv<FPC>(s, K, minv);
 #pragma unroll
     for (int i = 0; i < N; i++)
 #pragma unroll
         for (int e = 0; e < FPC; e++) {
+            double mrow[FPC];
+            minv_row<FPC>(minv, e, mrow);
             double sK = 0, sE = 0;
 #pragma unroll
             for (int j = 0; j < N; j++) {
-                const double *bp = s.b + ((long)(i * N + j) * S) * C + K;
-                sK += bp[(long)(NP + e) * C] * ZK[j];
+                const double D = s.D[((long)i * N + j) * C + K];
+                sK += row_sum<FPC>(mrow, D) * ZK[j];
 #pragma unroll
-                for (int f = 0; f < FPC; f++) sE += bp[(long)pk(e, f) * C] * ZF[j][f];
+                for (int f = 0; f < FPC; f++) sE += b_entry(mrow[f], D) * ZF[j][f];
             }
             s.v[((long)i * FPC + e) * C + K] = sK - sE + s.w[((long)i * FPC + e) * C + K];
         }
@@ -737,8 +863,23 @@ void launch_assemble(Ctx &c, double time, double tau)
     (void)time;
     (void)tau;
     SchemeView s = c.view();
-    MHFEM_DISPATCH(c, k_assemble, blocks(c.F), s);
+    const long lanes_per_face = 2L * (c.n == 3 ? 4 : c.n);
+    MHFEM_DISPATCH(c, k_assemble, blocks(lanes_per_face * c.F), s);
     MHFEM_DISPATCH(c, k_assemble_boundary, blocks(c.nbnd), s);
+}
+
+void launch_geometry(Ctx &c)
+{
+    SchemeView s = c.view();
+    if (c.fpc == 3) hipLaunchKernelGGL(k_geometry<3>, blocks(c.C), WG, 0, c.stream, s, c.minv.p);
+    else hipLaunchKernelGGL(k_geometry<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p);
+    MHFEM_HIP(hipGetLastError());
+}
+
+void launch_fill_b(Ctx &c)
+{
+    SchemeView s = c.view();
+    MHFEM_DISPATCH(c, k_fill_b, blocks(c.C), s, c.arr[MHFEM_b_ijK_storage].p);
 }
 
 void launch_post_iterate(Ctx &c)

@@ -155,33 +155,24 @@ def darcy_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 20,
                    extra=dict(seed=seed, s=s))
 
 
-def adr_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 50,
-           preconditioner: str = "ilu0", time_dependent: bool = False) -> Problem:
-    """n=2 coupled advection-diffusion-reaction on the Darcy mesh and K field:
-    D11=D22=K, D12=D21=0.1K, N=0.2 I, m=1, constant advection a_11=a_22=(1,.5,.25)
-    projected onto the RTN0 face fluxes (a_iiKe = |E_e| a.n_{K,e}), explicit
-    upwind, r=[[1e-2,-5e-3],[-5e-3,1e-2]], f=0, Z0=0; Dirichlet Z=(1,0.5) at
-    x=0, AdvectiveOutflow at x=1, FixedFluxNoAdvection (J=0) elsewhere.
-    ``time_dependent`` makes Z_1 = sin(2 pi t / 0.5) at x=0 (evaluated at
-    time+tau, Solver.hpp:948)."""
-    points, cells = meshgen.unit_cube_kuhn(s)
-    cf, fc = _topology(points, cells)
-    F, C = fc.shape[0], cells.shape[0]
-    n = 2
-    K = np.exp(_cube_lnK(s, 1.0, 0.1, seed))
+def _adr_coefficients(n, C, K, flux):
+    """Coupled ADR coefficients of n species: D_ij = K (i = j) or 0.1 K,
+    N = 0.2 I, m = 1, r_ij = 1e-2 (i = j) or -5e-3, a_iiKe = flux, u = w = f = 0."""
     zero = np.zeros(C)
-    D = _ij_blocks(n, C, [[K, 0.1 * K], [0.1 * K, K]])
-    N = _ij_blocks(n, C, [[0.2 + zero, zero], [zero, 0.2 + zero]])
-    r = _ij_blocks(n, C, [[1e-2 + zero, -5e-3 + zero], [-5e-3 + zero, 1e-2 + zero]])
-    fv = meshgen.outward_face_vectors(points, cells)  # (C, 4, 3)
-    avec = np.array([1.0, 0.5, 0.25])
-    flux = (fv * avec).sum(axis=2)  # (C, 4)
-    a = np.zeros((n, n, 4, C))
+    D = _ij_blocks(n, C, [[K if i == j else 0.1 * K for j in range(n)] for i in range(n)])
+    N = _ij_blocks(n, C, [[0.2 + zero if i == j else zero for j in range(n)] for i in range(n)])
+    r = _ij_blocks(n, C, [[(1e-2 if i == j else -5e-3) + zero for j in range(n)] for i in range(n)])
+    a = np.zeros((n, n, flux.shape[1], C))
     for i in range(n):
         a[i, i] = flux.T
-    coeffs = dict(N_ijK=N, u_ijKe=np.array(0.0), m_iK=np.array(1.0), D_ijK=D,
-                  w_iKe=np.array(0.0), a_ijKe=a.ravel(), r_ijK=r, f_iK=np.array(0.0))
-    fcen = meshgen.face_centroids(points, cells, cf, F)
+    return dict(N_ijK=N, u_ijKe=np.array(0.0), m_iK=np.array(1.0), D_ijK=D,
+                w_iKe=np.array(0.0), a_ijKe=a.ravel(), r_ijK=r, f_iK=np.array(0.0))
+
+
+def _adr_boundary(n, fc, fcen):
+    """Dirichlet Z_i = 2^-i at x = 0, AdvectiveOutflow at x = 1,
+    FixedFluxNoAdvection (J = 0) elsewhere; interior tags are unused."""
+    F = fc.shape[0]
     bnd = _boundary_mask(fc)
     x0 = bnd & (np.abs(fcen[:, 0]) < 1e-12)
     x1 = bnd & (np.abs(fcen[:, 0] - 1.0) < 1e-12)
@@ -190,8 +181,31 @@ def adr_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 50,
     tags[:, x1] = ADVECTIVE_OUTFLOW
     tags[:, x0] = FIXED_VALUE
     dirichlet = np.zeros((n, F))
-    dirichlet[0, x0] = 1.0
-    dirichlet[1, x0] = 0.5
+    for i in range(n):
+        dirichlet[i, x0] = 0.5 ** i
+    return tags, dirichlet, x0
+
+
+def adr_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 50,
+           preconditioner: str = "ilu0", time_dependent: bool = False, n: int = 2) -> Problem:
+    """n coupled species (default 2) of advection-diffusion-reaction on the
+    Darcy mesh and K field: D11=D22=K, D12=D21=0.1K, N=0.2 I, m=1, constant
+    advection a_11=a_22=(1,.5,.25) projected onto the RTN0 face fluxes
+    (a_iiKe = |E_e| a.n_{K,e}), explicit upwind,
+    r=[[1e-2,-5e-3],[-5e-3,1e-2]], f=0, Z0=0; Dirichlet Z=(1,0.5) at x=0,
+    AdvectiveOutflow at x=1, FixedFluxNoAdvection (J=0) elsewhere (for n != 2
+    the same pattern: D_ij = 0.1 K and r_ij = -5e-3 off the diagonal,
+    Z_i = 2^-i at x=0).  ``time_dependent`` makes Z_1 = sin(2 pi t / 0.5)
+    at x=0 (evaluated at time+tau, Solver.hpp:948)."""
+    points, cells = meshgen.unit_cube_kuhn(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    K = np.exp(_cube_lnK(s, 1.0, 0.1, seed))
+    fv = meshgen.outward_face_vectors(points, cells)  # (C, 4, 3)
+    flux = (fv * np.array([1.0, 0.5, 0.25])).sum(axis=2)  # (C, 4)
+    coeffs = _adr_coefficients(n, C, K, flux)
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    tags, dirichlet, x0 = _adr_boundary(n, fc, fcen)
     dirichlet_at = None
     if time_dependent:
         base = dirichlet.ravel().copy()
@@ -209,6 +223,28 @@ def adr_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 50,
                    preconditioner=preconditioner, do_mobility_upwind=1,
                    bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc,
                    dirichlet_at=dirichlet_at, extra=dict(seed=seed, s=s))
+
+
+def adr_2d(s: int = 16, n: int = 2, dt: float = 1e-3, steps: int = 3,
+           preconditioner: str = "diagonal") -> Problem:
+    """The coupled ADR model of adr_3d on the triangulated unit square:
+    K = 1 + 0.5 sin(2 pi x) sin(2 pi y) at cell centroids, advection
+    (1, 0.5) projected onto the face fluxes, the same boundary pattern."""
+    points, cells = meshgen.unit_square_triangles(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    c = meshgen.cell_centroids(points, cells)
+    K = 1.0 + 0.5 * np.sin(2 * np.pi * c[:, 0]) * np.sin(2 * np.pi * c[:, 1])
+    fv = meshgen.outward_face_vectors(points, cells)  # (C, 3, 2)
+    flux = (fv * np.array([1.0, 0.5])).sum(axis=2)
+    coeffs = _adr_coefficients(n, C, K, flux)
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    tags, dirichlet, _ = _adr_boundary(n, fc, fcen)
+    return Problem("adr_2d", points, cells, n, coeffs, tags=tags.ravel(),
+                   bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
+                   Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
+                   preconditioner=preconditioner, do_mobility_upwind=1,
+                   bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc)
 
 
 def ensemble_sizes(seed: int, count: int = 96, lo: int = 32, hi: int = 256):
