@@ -16,10 +16,12 @@
  *   u_ijKe, a_ijKe, R_ijKe : ((i*n+j)*FPC + e)*C + K
  *   w_iKe, v_iKe : (i*FPC + e)*C + K   Z_ijE_upw : (E*n + j)*n + i
  *   b_ijK_storage : ((i*n+j)*S + s)*C + K   (S = 9 triangles, 14 tetrahedra)
- * b_ijK_storage (= M^{-1} D_ij, MassMatrix.h:586-597) is not kept between
- * steps: the kernels form its entries from the per-cell M^{-1} of the fixed
- * geometry and the current D_ijK, and mhfem_get_array / mhfem_device_array
- * write the array from them when it is asked for.
+ * Three derived arrays are not kept in these layouts between calls, and
+ * mhfem_get_array / mhfem_device_array write them when asked (a snapshot):
+ * b_ijK_storage (= M^{-1} D_ij, MassMatrix.h:586-597) from the per-cell
+ * M^{-1} of the fixed geometry and the current D_ijK; R_ijKe and R_iK from
+ * the per-cell records the last mhfem_pre_iterate wrote, which keep them in
+ * the order the face assembly reads them.
  * Boundary-condition arrays are indexed i*F + E (BoundaryConditions.hpp:295).
  * Every function returns 0 on success or a negative MHFEM_E_* code; the
  * message is available from mhfem_last_error().  A context is not

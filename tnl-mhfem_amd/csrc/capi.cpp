@@ -291,8 +291,8 @@ SchemeView Ctx::view()
     s.m_upw = arr[MHFEM_m_iE_upw].p;
     s.Z_upw = arr[MHFEM_Z_ijE_upw].p;
     s.minv = minv.p;
-    s.R_ijKe = arr[MHFEM_R_ijKe].p;
-    s.R_iK = arr[MHFEM_R_iK].p;
+    s.minv_rows = minv_rows.p;
+    s.cell_rec = cell_rec.p;
     s.tags = tags.p;
     s.bc_values = arr[MHFEM_BC_VALUES].p;
     s.bc_dirichlet = arr[MHFEM_BC_DIRICHLET].p;
@@ -391,8 +391,14 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
         c->h_tags.assign((size_t)c->n * c->F, MHFEM_BC_FIXED_VALUE);
         upload(c->tags, c->h_tags.data(), c->h_tags.size());
         MHFEM_HIP(hipDeviceSynchronize());
-        // fixed geometry: M^{-1} of every cell, once
+        // fixed geometry: M^{-1} of every cell, once (packed planes, and
+        // row-expanded with FPC*FPC values rounded up to an even count per cell)
         c->minv.alloc((size_t)(c->fpc * (c->fpc + 1) / 2) * Cn);
+        c->minv_rows.alloc((size_t)((c->fpc * c->fpc + 1) & ~1) * Cn);
+        // per-cell assembly records: R_ijKe, R_iK, D_ijK, padded to an even count
+        const size_t rec = (size_t)((c->n * c->n * c->fpc + c->n + c->n * c->n + 1) & ~1);
+        c->cell_rec.alloc(rec * Cn);
+        MHFEM_HIP(hipMemset(c->cell_rec.p, 0, rec * Cn * sizeof(double)));
         launch_geometry(*c);
         MHFEM_HIP(hipStreamSynchronize(c->stream));
         return reinterpret_cast<mhfem_ctx *>(c);
@@ -547,7 +553,9 @@ int mhfem_get_array(mhfem_ctx *ctx, int which, double *host, long count)
         const long have = (is_coefficient(which) && c.uniform[which]) ? 1 : c.array_size(which);
         if (count != have || !c.arr[which].p)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_get_array: count mismatch (expected " + std::to_string(have) + ")"};
-        if (which == MHFEM_b_ijK_storage) launch_fill_b(c);  // formed on request from M^{-1} and D
+        // derived arrays the step does not keep in this layout: formed on request
+        if (which == MHFEM_b_ijK_storage) launch_fill_b(c);
+        if (which == MHFEM_R_ijKe || which == MHFEM_R_iK) launch_fill_R(c);
         MHFEM_HIP(hipMemcpyAsync(host, c.arr[which].p, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
     });
@@ -558,10 +566,12 @@ void *mhfem_device_array(mhfem_ctx *ctx, int which)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || which < 0 || which >= MHFEM_NUM_ARRAYS) return nullptr;
     if (is_coefficient(which) && c->uniform[which]) return nullptr;
-    if (which == MHFEM_b_ijK_storage) {
-        // formed on request from M^{-1} and the current D (a snapshot)
+    if (which == MHFEM_b_ijK_storage || which == MHFEM_R_ijKe || which == MHFEM_R_iK) {
+        // formed on request (a snapshot): b from M^{-1} and the current D,
+        // R_ijKe / R_iK from the per-cell records of the last preIterate
         try {
-            launch_fill_b(*c);
+            if (which == MHFEM_b_ijK_storage) launch_fill_b(*c);
+            else launch_fill_R(*c);
         } catch (const Error &e) {
             c->err = e.msg;
             return nullptr;

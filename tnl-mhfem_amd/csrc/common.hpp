@@ -140,8 +140,10 @@ struct SchemeView {
     // coefficients
     Coef N, u, m, D, w, a, r, f;
     // scheme arrays
-    double *Z_iF, *Z_iK, *v, *m_upw, *Z_upw, *R_ijKe, *R_iK;
-    const double *minv;         // packed M^{-1} per cell, NP planes of C (fixed geometry)
+    double *Z_iF, *Z_iK, *v, *m_upw, *Z_upw;
+    const double *minv;         // M^{-1} per cell, packed planes (fixed geometry)
+    const double *minv_rows;    // M^{-1} per cell, row-expanded (fixed geometry)
+    double *cell_rec;           // per-cell assembly inputs written by k_cell
     // boundary conditions
     const uint8_t *tags;
     const double *bc_values, *bc_dirichlet, *bc_mobility;
@@ -192,7 +194,8 @@ struct Ctx {
     DBuf<uint64_t> face_slots;
     DBuf<uint32_t> face_blocks;
     DBuf<double> points, cell_meas, face_meas;
-    DBuf<double> minv;  // packed inverse geometric mass matrices (k_geometry)
+    DBuf<double> minv, minv_rows;  // inverse geometric mass matrices: packed planes, row-expanded
+    DBuf<double> cell_rec;  // per-cell records of the assembly inputs (k_cell)
     // model arrays (index by MHFEM_* ids); uniform coefficients hold 1 value
     DBuf<double> arr[MHFEM_NUM_ARRAYS];
     bool uniform[MHFEM_NUM_ARRAYS] = {};
@@ -274,6 +277,7 @@ void launch_assemble(Ctx &c, double time, double tau);
 void launch_post_iterate(Ctx &c);
 void launch_geometry(Ctx &c);
 void launch_fill_b(Ctx &c);
+void launch_fill_R(Ctx &c);
 void launch_zero_double(Ctx &c, double *p, size_t n);
 
 // krylov.hip

@@ -173,31 +173,91 @@ __device__ __forceinline__ double row_sum(const double (&mrow)[FPC], double D)
     return s;
 }
 
-// row e of the packed M^{-1} held in registers (e a compile-time index)
+// M^{-1} of every cell is kept twice (both fixed, written once):
+//  - s.minv: the packed upper triangle in NP planes, s.minv[q*C + K], for
+//    the cell-parallel kernels (unit stride across a wavefront);
+//  - s.minv_rows: row-expanded, s.minv_rows[K*MS + e*FPC + f] = M^{-1}_ef
+//    (MS = FPC*FPC rounded up to an even count, 16-byte aligned cells), for
+//    the face kernels, which gather row e of one cell as one piece.
 template <int FPC>
-__device__ __forceinline__ void minv_row(const double (&minv)[Simplex<FPC>::NP], int e, double (&mrow)[FPC])
+struct MinvLayout {
+    static constexpr int MS = (FPC * FPC + 1) & ~1;
+};
+
+template <int FPC>
+__device__ __forceinline__ void load_minv(const SchemeView &s, int K, double (&m)[FPC][FPC])
 {
 #pragma unroll
-    for (int f = 0; f < FPC; f++) mrow[f] = minv[pk(e, f)];
+    for (int f = 0; f < FPC; f++)
+#pragma unroll
+        for (int e = 0; e <= f; e++) {
+            const double v = s.minv[(long)pk(e, f) * s.C + K];
+            m[e][f] = v;
+            m[f][e] = v;
+        }
 }
 
-// M^{-1} of cell K from its packed planes (minv[q*C + K])
+// row e (a runtime index) of cell K's M^{-1}
 template <int FPC>
-__device__ __forceinline__ void load_minv(const SchemeView &s, int K, double (&minv)[Simplex<FPC>::NP])
+__device__ __forceinline__ void load_minv_row(const SchemeView &s, int K, int e, double (&mrow)[FPC])
 {
+    const double *p = s.minv_rows + (long)K * MinvLayout<FPC>::MS + e * FPC;
+    if constexpr (FPC == 4) {
+        const Pack<double, 2> a = ld<double, 2>(p), b = ld<double, 2>(p + 2);
+        mrow[0] = a.v[0];
+        mrow[1] = a.v[1];
+        mrow[2] = b.v[0];
+        mrow[3] = b.v[1];
+    } else {
 #pragma unroll
-    for (int q = 0; q < Simplex<FPC>::NP; q++) minv[q] = s.minv[(long)q * s.C + K];
+        for (int f = 0; f < FPC; f++) mrow[f] = p[f];
+    }
+}
+
+// Inputs of the face assembly that k_cell produces for cell K, in the order
+// the face kernels read them, one contiguous 16-byte aligned record per cell
+// (reading them from the reference-layout arrays, one array per index
+// combination, would fetch a partial cache line per value):
+//   [(j*FPC + f)*N + x]  R_xjKf
+//   [RK_OFF + j]         R_jK
+//   [D_OFF + i*N + j]    D_ijK
+template <int N, int FPC>
+struct CellRec {
+    static constexpr int RK_OFF = N * N * FPC, D_OFF = RK_OFF + N;
+    static constexpr int SIZE = (D_OFF + N * N + 1) & ~1;
+    // The cell-parallel kernels move a work-group's records through LDS
+    // (lane t's record at t*LS, one double of padding against bank
+    // conflicts) so that global memory sees unit-stride lanes; a record per
+    // lane straight from global memory would touch a cache line per lane
+    // and instruction.  Only where the block fits in 64 KB of LDS.
+    static constexpr int LS = SIZE + 1;
+    static constexpr bool STAGE = LS * 256 * 8 <= 65536;
+};
+
+// Copies between the records of cells [first, first + count) in global
+// memory (contiguous) and their LDS slots; called by the whole work-group.
+template <int SIZE, int LS>
+__device__ __forceinline__ void records_to_global(double *g, const double *lds, int count)
+{
+    const int total = count * SIZE;
+    for (int q = threadIdx.x; q < total; q += blockDim.x) g[q] = lds[(q / SIZE) * LS + q % SIZE];
+}
+template <int SIZE, int LS>
+__device__ __forceinline__ void records_from_global(const double *g, double *lds, int count)
+{
+    const int total = count * SIZE;
+    for (int q = threadIdx.x; q < total; q += blockDim.x) lds[(q / SIZE) * LS + q % SIZE] = g[q];
 }
 
 // ---------------------------------------------------------------------------
 // Inverse of the geometric RTN0 mass matrix of every cell (Solver.hpp:627-639,
-// MassMatrix::update), packed in NP planes of C values.  It depends only on
-// the mesh and the lumping, both fixed when the context is created, so it is
-// computed once; b_ijK = M^{-1} D_ij is formed from it where it is used.
+// MassMatrix::update).  It depends only on the mesh and the lumping, both
+// fixed when the context is created, so it is computed once; b_ijK =
+// M^{-1} D_ij is formed from it where it is used.
 template <int FPC>
-__global__ void __launch_bounds__(WG) k_geometry(SchemeView s, double *minv_out)
+__global__ void __launch_bounds__(WG) k_geometry(SchemeView s, double *minv_out, double *rows_out)
 {
-    constexpr int DIM = FPC - 1, NP = Simplex<FPC>::NP;
+    constexpr int DIM = FPC - 1, NP = Simplex<FPC>::NP, MS = MinvLayout<FPC>::MS;
     const int K = blockIdx.x * WG + threadIdx.x;
     if (K >= s.C) return;
     double P[FPC][DIM];
@@ -211,6 +271,12 @@ __global__ void __launch_bounds__(WG) k_geometry(SchemeView s, double *minv_out)
     geom_minv<FPC>(P, s.cell_meas[K], s.lumping, minv);
 #pragma unroll
     for (int q = 0; q < NP; q++) minv_out[(long)q * s.C + K] = minv[q];
+    double *o = rows_out + (long)K * MS;
+#pragma unroll
+    for (int e = 0; e < FPC; e++)
+#pragma unroll
+        for (int f = 0; f < FPC; f++) o[e * FPC + f] = minv[pk(e, f)];
+    if constexpr (MS > FPC * FPC) o[FPC * FPC] = 0.0;
 }
 
 // b_ijK_storage ((i*n+j)*S + q)*C + K: the packed b_ijK followed by its row
@@ -222,8 +288,8 @@ __global__ void __launch_bounds__(WG) k_fill_b(SchemeView s, double *b)
     const int K = blockIdx.x * WG + threadIdx.x;
     if (K >= s.C) return;
     const long C = s.C;
-    double minv[NP];
-    load_minv<FPC>(s, K, minv);
+    double m[FPC][FPC];
+    load_minv<FPC>(s, K, m);
 #pragma unroll
     for (int i = 0; i < N; i++)
 #pragma unroll
@@ -231,13 +297,11 @@ __global__ void __launch_bounds__(WG) k_fill_b(SchemeView s, double *b)
             const double D = s.D[((long)i * N + j) * C + K];
             double *bp = b + ((long)(i * N + j) * S) * C + K;
 #pragma unroll
-            for (int q = 0; q < NP; q++) bp[(long)q * C] = b_entry(minv[q], D);
+            for (int f = 0; f < FPC; f++)
 #pragma unroll
-            for (int e = 0; e < FPC; e++) {
-                double mrow[FPC];
-                minv_row<FPC>(minv, e, mrow);
-                bp[(long)(NP + e) * C] = row_sum<FPC>(mrow, D);
-            }
+                for (int e = 0; e <= f; e++) bp[(long)pk(e, f) * C] = b_entry(m[e][f], D);
+#pragma unroll
+            for (int e = 0; e < FPC; e++) bp[(long)(NP + e) * C] = row_sum<FPC>(m[e], D);
         }
 }
 
@@ -311,36 +375,23 @@ __global__ void __launch_bounds__(WG) k_upwind(SchemeView s)
 // in-place Q^{-1} solves (Solver.hpp:817-871, SecondaryCoefficients.h:129-159).
 // The model hooks updateNonLinearTerms / updateVectorCoefficients run on the
 // host before this kernel; K4/K5 do not depend on b, so they run first.
+// The work of one cell; its record goes to rec (an LDS slot or global memory).
 template <int N, int FPC>
-__global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
+__device__ __forceinline__ void cell_step(const SchemeView &s, double tau, int K, double *rec)
 {
-    constexpr int NP = Simplex<FPC>::NP;
-    const int K = blockIdx.x * WG + threadIdx.x;
-    if (K >= s.C) return;
+    using Rec = CellRec<N, FPC>;
     const long C = s.C;
     int fi[FPC];
 #pragma unroll
     for (int e = 0; e < FPC; e++) fi[e] = s.cell_faces[(long)K * FPC + e];
     const double meas = s.cell_meas[K];
-    double minv[NP];
+    double minv[FPC][FPC];
     load_minv<FPC>(s, K, minv);
-
-    // b = M^{-1} D_ij and its row sums (b_ijK_storage, formed in registers)
-    double Dij[N][N], rs[N][N][FPC];
+    double Dij[N][N];
 #pragma unroll
     for (int i = 0; i < N; i++)
 #pragma unroll
-        for (int j = 0; j < N; j++) {
-            const double D = s.D[((long)i * N + j) * C + K];
-            Dij[i][j] = D;
-#pragma unroll
-            for (int e = 0; e < FPC; e++) {
-                double mrow[FPC];
-                minv_row<FPC>(minv, e, mrow);
-                rs[i][j][e] = row_sum<FPC>(mrow, D);
-            }
-        }
-
+        for (int j = 0; j < N; j++) Dij[i][j] = s.D[((long)i * N + j) * C + K];
     double mu[N][FPC];
 #pragma unroll
     for (int i = 0; i < N; i++)
@@ -352,24 +403,10 @@ __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
         return s.a[idx] + s.u[idx];
     };
 
-    // K6: R_ijKe (SecondaryCoefficients.h:50-86) and R_iK (:88-127)
-    double R[N][N][FPC], RK[N];
+    // K6: R_iK (SecondaryCoefficients.h:88-127)
+    double RK[N];
 #pragma unroll
     for (int i = 0; i < N; i++) {
-#pragma unroll
-        for (int j = 0; j < N; j++)
-#pragma unroll
-            for (int e = 0; e < FPC; e++) {
-                double r = 0;
-#pragma unroll
-                for (int f = 0; f < FPC; f++) r += mu[i][f] * b_entry(minv[pk(f, e)], Dij[i][j]);
-                if (s.advection == MHFEM_IMPLICIT_UPWIND) {
-                    const double vel = vel_of(i, j, e);
-                    if (vel < 0) r -= 2 * vel;
-                }
-                if (s.advection == MHFEM_IMPLICIT_TRACE) r -= vel_of(i, j, e);
-                R[i][j][e] = r;
-            }
         double r = 0;
 #pragma unroll
         for (int j = 0; j < N; j++) r += s.N[((long)i * N + j) * C + K] * s.Z_iK[(long)j * C + K];
@@ -389,7 +426,8 @@ __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
         RK[i] = r;
     }
 
-    // K7: Q (SecondaryCoefficients.h:129-159), singular rows (Solver.hpp:829-847)
+    // K7: Q (SecondaryCoefficients.h:129-159) with the row sums of b_ijK,
+    // singular rows (Solver.hpp:829-847)
     double Q[N][N];
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -399,7 +437,7 @@ __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
             double q = 0;
 #pragma unroll
             for (int e = 0; e < FPC; e++) {
-                q += mu[i][e] * rs[i][j][e] - s.u[(((long)i * N + j) * FPC + e) * C + K];
+                q += mu[i][e] * row_sum<FPC>(minv[e], Dij[i][j]) - s.u[(((long)i * N + j) * FPC + e) * C + K];
                 if (s.advection == MHFEM_IMPLICIT_UPWIND) {
                     const double vel = vel_of(i, j, e);
                     if (vel >= 0) q += vel;
@@ -417,19 +455,73 @@ __global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
     }
     lu_factorize<N>(Q);
     lu_solve<N>(Q, RK);
-#pragma unroll
-    for (int i = 0; i < N; i++) s.R_iK[(long)i * C + K] = RK[i];
+
+    // K6: R_ijKe (SecondaryCoefficients.h:50-86), one column (j, e) at a
+    // time, then Q^{-1} applied to it; written to the cell's record only
+    // (the reference-layout arrays are formed on request, k_fill_R)
 #pragma unroll
     for (int j = 0; j < N; j++)
 #pragma unroll
         for (int e = 0; e < FPC; e++) {
             double x[N];
 #pragma unroll
-            for (int i = 0; i < N; i++) x[i] = R[i][j][e];
+            for (int i = 0; i < N; i++) {
+                double r = 0;
+#pragma unroll
+                for (int f = 0; f < FPC; f++) r += mu[i][f] * b_entry(minv[f][e], Dij[i][j]);
+                if (s.advection == MHFEM_IMPLICIT_UPWIND) {
+                    const double vel = vel_of(i, j, e);
+                    if (vel < 0) r -= 2 * vel;
+                }
+                if (s.advection == MHFEM_IMPLICIT_TRACE) r -= vel_of(i, j, e);
+                x[i] = r;
+            }
             lu_solve<N>(Q, x);
 #pragma unroll
-            for (int i = 0; i < N; i++) s.R_ijKe[(((long)i * N + j) * FPC + e) * C + K] = x[i];
+            for (int i = 0; i < N; i++) rec[(j * FPC + e) * N + i] = x[i];
         }
+#pragma unroll
+    for (int i = 0; i < N; i++) rec[Rec::RK_OFF + i] = RK[i];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) rec[Rec::D_OFF + i * N + j] = Dij[i][j];
+    if constexpr (Rec::SIZE > Rec::D_OFF + N * N) rec[Rec::D_OFF + N * N] = 0.0;
+}
+
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_cell(SchemeView s, double tau)
+{
+    using Rec = CellRec<N, FPC>;
+    const int first = blockIdx.x * WG, K = first + threadIdx.x;
+    if constexpr (Rec::STAGE) {
+        __shared__ double stage[WG * Rec::LS];
+        if (K < s.C) cell_step<N, FPC>(s, tau, K, stage + threadIdx.x * Rec::LS);
+        __syncthreads();
+        records_to_global<Rec::SIZE, Rec::LS>(s.cell_rec + (long)first * Rec::SIZE, stage, min(WG, s.C - first));
+    } else {
+        if (K < s.C) cell_step<N, FPC>(s, tau, K, s.cell_rec + (long)K * Rec::SIZE);
+    }
+}
+
+// R_ijKe ((i*n+j)*FPC + e)*C + K and R_iK i*C + K in the reference's layout,
+// from the cell records; written only when a caller asks for the arrays.
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_fill_R(SchemeView s, double *R_ijKe, double *R_iK)
+{
+    using Rec = CellRec<N, FPC>;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    const long C = s.C;
+    const double *rec = s.cell_rec + (long)K * Rec::SIZE;
+#pragma unroll
+    for (int x = 0; x < N; x++)
+#pragma unroll
+        for (int j = 0; j < N; j++)
+#pragma unroll
+            for (int f = 0; f < FPC; f++) R_ijKe[(((long)x * N + j) * FPC + f) * C + K] = rec[(j * FPC + f) * N + x];
+#pragma unroll
+    for (int j = 0; j < N; j++) R_iK[(long)j * C + K] = rec[Rec::RK_OFF + j];
 }
 
 // a[i] for a runtime i without dynamic register indexing (which would move the
@@ -459,24 +551,29 @@ __device__ __forceinline__ double pick(const double (&a)[M], int i)
 // scratch memory.  The entries of b_ijK are formed from row e of M^{-1} and
 // D_ij (b_entry, row_sum), rounded as the stored values would be.
 
-// Values of cell K that every row of its local face e uses: R_xjKf, R_jK and
-// row e of M^{-1}.
+// Values of cell K that every row of its local face e uses: R_xjKf, R_jK
+// (from the cell's record) and row e of M^{-1}.
 template <int N, int FPC>
 struct CellData {
     double Rv[N][N][FPC], RK[N], mrow[FPC];
     __device__ __forceinline__ void load(const SchemeView &s, int K, int e)
     {
-        const long C = s.C;
+        using Rec = CellRec<N, FPC>;
+        const double *rec = s.cell_rec + (long)K * Rec::SIZE;
+        if constexpr (Rec::RK_OFF % 2 == 0) {
 #pragma unroll
-        for (int x = 0; x < N; x++)
+            for (int q = 0; q < Rec::RK_OFF; q += 2) {
+                const Pack<double, 2> v = ld<double, 2>(rec + q);
+                Rv[q % N][q / (N * FPC)][(q / N) % FPC] = v.v[0];
+                Rv[(q + 1) % N][(q + 1) / (N * FPC)][((q + 1) / N) % FPC] = v.v[1];
+            }
+        } else {
 #pragma unroll
-            for (int j = 0; j < N; j++)
+            for (int q = 0; q < Rec::RK_OFF; q++) Rv[q % N][q / (N * FPC)][(q / N) % FPC] = rec[q];
+        }
 #pragma unroll
-                for (int f = 0; f < FPC; f++) Rv[x][j][f] = s.R_ijKe[(((long)x * N + j) * FPC + f) * C + K];
-#pragma unroll
-        for (int j = 0; j < N; j++) RK[j] = s.R_iK[(long)j * C + K];
-#pragma unroll
-        for (int f = 0; f < FPC; f++) mrow[f] = s.minv[(long)pk(e, f) * C + K];
+        for (int j = 0; j < N; j++) RK[j] = rec[Rec::RK_OFF + j];
+        load_minv_row<FPC>(s, K, e, mrow);
     }
 };
 
@@ -487,13 +584,13 @@ struct CellRow {
     double Aii;  // A[i][e], the diagonal entry before scaling
     double rhs;
 
-    // Row inputs: Di[j] = D_ijK, wv = w_iKe.
+    // Row inputs: Di[j] = D_ijK (from the cell's record), wv = w_iKe.
     __device__ __forceinline__ static void load_row(const SchemeView &s, int K, int e, int i, double (&Di)[N], double &wv)
     {
-        const long C = s.C;
+        const double *rec = s.cell_rec + (long)K * CellRec<N, FPC>::SIZE + CellRec<N, FPC>::D_OFF + i * N;
 #pragma unroll
-        for (int j = 0; j < N; j++) Di[j] = s.D[((long)i * N + j) * C + K];
-        wv = s.w[((long)i * FPC + e) * C + K];
+        for (int j = 0; j < N; j++) Di[j] = rec[j];
+        wv = s.w[((long)i * FPC + e) * s.C + K];
     }
 
     // use_adv: A_ijKEF_advection / RHS_iKE_advection (SecondaryCoefficients.h:215-259,
@@ -761,11 +858,9 @@ __global__ void __launch_bounds__(WG) k_assemble_boundary(SchemeView s)
 // postIterate: K9 Z_iK recovery (Solver.hpp:1283-1292, SecondaryCoefficients.h:163-185)
 // and K10 conservative velocities (Solver.hpp:1298-1313, SecondaryCoefficients.h:16-48)
 template <int N, int FPC>
-__global__ void __launch_bounds__(WG) k_post(SchemeView s)
+__device__ __forceinline__ void post_step(const SchemeView &s, int K, const double *rec)
 {
-    constexpr int NP = Simplex<FPC>::NP;
-    const int K = blockIdx.x * WG + threadIdx.x;
-    if (K >= s.C) return;
+    using Rec = CellRec<N, FPC>;
     const long C = s.C;
     int fi[FPC];
 #pragma unroll
@@ -782,30 +877,43 @@ __global__ void __launch_bounds__(WG) k_post(SchemeView s)
 #pragma unroll
         for (int f = 0; f < FPC; f++)
 #pragma unroll
-            for (int j = 0; j < N; j++) Z += s.R_ijKe[(((long)i * N + j) * FPC + f) * C + K] * ZF[j][f];
-        Z += s.R_iK[(long)i * C + K];
+            for (int j = 0; j < N; j++) Z += rec[(j * FPC + f) * N + i] * ZF[j][f];
+        Z += rec[Rec::RK_OFF + i];
         ZK[i] = Z;
         s.Z_iK[(long)i * C + K] = Z;
     }
     if (!s.do_mob) return;
-    double minv[NP];
+    double minv[FPC][FPC];
     load_minv<FPC>(s, K, minv);
 #pragma unroll
     for (int i = 0; i < N; i++)
 #pragma unroll
         for (int e = 0; e < FPC; e++) {
-            double mrow[FPC];
-            minv_row<FPC>(minv, e, mrow);
             double sK = 0, sE = 0;
 #pragma unroll
             for (int j = 0; j < N; j++) {
-                const double D = s.D[((long)i * N + j) * C + K];
-                sK += row_sum<FPC>(mrow, D) * ZK[j];
+                const double D = rec[Rec::D_OFF + i * N + j];
+                sK += row_sum<FPC>(minv[e], D) * ZK[j];
 #pragma unroll
-                for (int f = 0; f < FPC; f++) sE += b_entry(mrow[f], D) * ZF[j][f];
+                for (int f = 0; f < FPC; f++) sE += b_entry(minv[e][f], D) * ZF[j][f];
             }
             s.v[((long)i * FPC + e) * C + K] = sK - sE + s.w[((long)i * FPC + e) * C + K];
         }
+}
+
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_post(SchemeView s)
+{
+    using Rec = CellRec<N, FPC>;
+    const int first = blockIdx.x * WG, K = first + threadIdx.x;
+    if constexpr (Rec::STAGE) {
+        __shared__ double stage[WG * Rec::LS];
+        records_from_global<Rec::SIZE, Rec::LS>(s.cell_rec + (long)first * Rec::SIZE, stage, min(WG, s.C - first));
+        __syncthreads();
+        if (K < s.C) post_step<N, FPC>(s, K, stage + threadIdx.x * Rec::LS);
+    } else {
+        if (K < s.C) post_step<N, FPC>(s, K, s.cell_rec + (long)K * Rec::SIZE);
+    }
 }
 
 __global__ void k_fill(double *p, size_t n, double v)
@@ -821,21 +929,22 @@ static inline unsigned blocks(long n) { return (unsigned)((n + WG - 1) / WG); }
 template <template <int, int> class KernelTag, typename... Args>
 static void dispatch(Ctx &c, Args &&...args);
 
-#define MHFEM_DISPATCH(c, KERNEL, GRID, ...)                                                         \
+#define MHFEM_DISPATCH_BS(c, KERNEL, GRID, BS, ...)                                                 \
     do {                                                                                            \
         const unsigned g_ = (GRID);                                                                 \
         if (g_ == 0) break;                                                                         \
         if ((c).fpc == 3) {                                                                         \
-            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 3>), g_, WG, 0, (c).stream, __VA_ARGS__); \
-            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 3>), g_, WG, 0, (c).stream, __VA_ARGS__); \
-            else hipLaunchKernelGGL((KERNEL<3, 3>), g_, WG, 0, (c).stream, __VA_ARGS__);           \
+            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 3>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 3>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<3, 3>), g_, BS, 0, (c).stream, __VA_ARGS__);           \
         } else {                                                                                    \
-            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 4>), g_, WG, 0, (c).stream, __VA_ARGS__); \
-            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 4>), g_, WG, 0, (c).stream, __VA_ARGS__); \
-            else hipLaunchKernelGGL((KERNEL<3, 4>), g_, WG, 0, (c).stream, __VA_ARGS__);           \
+            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 4>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 4>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<3, 4>), g_, BS, 0, (c).stream, __VA_ARGS__);           \
         }                                                                                           \
         MHFEM_HIP(hipGetLastError());                                                               \
     } while (0)
+#define MHFEM_DISPATCH(c, KERNEL, GRID, ...) MHFEM_DISPATCH_BS(c, KERNEL, GRID, WG, __VA_ARGS__)
 
 void launch_face_average(Ctx &c)
 {
@@ -871,8 +980,8 @@ void launch_assemble(Ctx &c, double time, double tau)
 void launch_geometry(Ctx &c)
 {
     SchemeView s = c.view();
-    if (c.fpc == 3) hipLaunchKernelGGL(k_geometry<3>, blocks(c.C), WG, 0, c.stream, s, c.minv.p);
-    else hipLaunchKernelGGL(k_geometry<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p);
+    if (c.fpc == 3) hipLaunchKernelGGL(k_geometry<3>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
+    else hipLaunchKernelGGL(k_geometry<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
     MHFEM_HIP(hipGetLastError());
 }
 
@@ -880,6 +989,12 @@ void launch_fill_b(Ctx &c)
 {
     SchemeView s = c.view();
     MHFEM_DISPATCH(c, k_fill_b, blocks(c.C), s, c.arr[MHFEM_b_ijK_storage].p);
+}
+
+void launch_fill_R(Ctx &c)
+{
+    SchemeView s = c.view();
+    MHFEM_DISPATCH(c, k_fill_R, blocks(c.C), s, c.arr[MHFEM_R_ijKe].p, c.arr[MHFEM_R_iK].p);
 }
 
 void launch_post_iterate(Ctx &c)
