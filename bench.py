@@ -40,6 +40,8 @@ roofline: the dominant kernel is the FP32 block-SELL SpMV of the chosen
 form's iteration.  Every 8th launch during the chosen form's timed steps is
 bracketed by HIP events on the library's stream (timing level 2);
 achieved = algorithmic bytes per launch / average launch duration.
+assembly: the same for assembleLinearSystem (interior and boundary rows),
+timed with HIP events around every assembly of the chosen form's timed steps.
 cpu_baseline: the C oracle (a restatement of the reference host path), FP64
 BiCGStab + ILU(0), on a bounded sample of one step of the same workload.
 """
@@ -79,6 +81,16 @@ def block_sell_bytes(n, faces, blocks_per_face, fp32):
     w = 4 if fp32 else 8
     rows = n * faces
     return blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
+
+
+def assembly_bytes(n, fpc, cells, faces, nnz):
+    """Algorithmic bytes of one assembly (both the interior and the boundary
+    rows): matrix values and right-hand side written, every cell's inputs
+    read once (b_ijK_storage with its row sums, R_ijKe, R_iK, w and the
+    right-hand-side terms), the face-cell and cell-face incidence."""
+    S = 9 if fpc == 3 else 14
+    rows = n * faces
+    return 8 * nnz + 8 * rows + cells * 8 * (n * n * (S + fpc) + n * (2 + fpc)) + 4 * faces * 2 + 4 * cells * fpc
 
 
 def profiled_traffic():
@@ -428,6 +440,7 @@ def main():
     alg = spmv_bytes(rows, nnz, True)
     fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True)
     achieved = alg / (spmv_ms * 1e-3) / 1e9
+    asm_alg = assembly_bytes(pr.n, pr.fpc, pr.num_cells, faces, nnz)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -475,6 +488,18 @@ def main():
             "algorithmic_bytes": alg,
             "format_bytes": int(fmt),
             "format_GBs": round(fmt / (spmv_ms * 1e-3) / 1e9, 1),
+        },
+        "assembly": {
+            "kernels": "k_assemble + k_assemble_boundary (assembleLinearSystem)",
+            "bound": "hbm",
+            "avg_ms": round(tm["assemble"] / args.steps, 4),
+            "algorithmic_bytes": asm_alg,
+            "achieved": round(asm_alg / (tm["assemble"] / args.steps * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(asm_alg / (tm["assemble"] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pre_iterate_ms": round(tm["pre_iterate"] / args.steps, 4),
+            "post_iterate_ms": round(tm["post_iterate"] / args.steps, 4),
         },
         "ensemble": None,
         "cpu_baseline": None,

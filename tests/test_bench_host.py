@@ -33,3 +33,14 @@ def test_spmv_byte_counts():
 def test_traffic_from_profile():
     t = bench.profiled_traffic()
     assert t is not None and 4e8 < t < 6e8
+
+
+def test_assembly_byte_count():
+    # n=2 tetrahedra, 64^3 cubes: 1,572,864 cells, 3,170,304 faces, 88,064,000 nonzeros
+    C, F, nnz = 1572864, 3170304, 88064000
+    got = bench.assembly_bytes(2, 4, C, F, nnz)
+    # values + rhs written; per cell b (n^2 * 14 packed entries and row sums) and R_ijKe
+    # (n^2 * 4), R_iK and f (2n), w (n * 4); incidence
+    per_cell = 8 * (4 * 14 + 4 * 4 + 2 * 2 + 2 * 4)
+    assert got == 8 * nnz + 8 * 2 * F + C * per_cell + 8 * F + 16 * C
+    assert abs(got - 1.863e9) < 1e7

@@ -27,6 +27,24 @@ def maxrel(a, b):
     return np.abs(a - b).max() / s
 
 
+def _mixed_tags(pr, seed=5):
+    """Every boundary face cycles through the four tags (per species, offset
+    by the species), with random flux and Dirichlet values."""
+    rng = np.random.default_rng(seed)
+    n, F = pr.n, pr.num_faces
+    tags = np.array(pr.tags, copy=True)
+    vals = np.zeros(n * F)
+    dirv = np.zeros(n * F)
+    kinds = [problems.FIXED_VALUE, problems.FIXED_FLUX, problems.ADVECTIVE_OUTFLOW,
+             problems.FIXED_FLUX_NO_ADVECTION]
+    for i in range(n):
+        for q, E in enumerate(np.nonzero(pr.face_cells[:, 1] < 0)[0]):
+            tags[i * F + E] = kinds[(q + i) % 4]
+            vals[i * F + E] = rng.uniform(-1, 1)
+            dirv[i * F + E] = rng.uniform(-1, 1)
+    return dataclasses.replace(pr, tags=tags, bc_values=vals, bc_dirichlet=dirv)
+
+
 CASES = {
     "heat2d": lambda: problems.heat_2d(s=12, steps=3),
     "heat2d_lumped": lambda: problems.heat_2d(s=12, steps=3, lumping=1),
@@ -41,6 +59,9 @@ CASES = {
     "adr3d_n3_implicit_upwind": lambda: dataclasses.replace(
         problems.adr_3d(s=3, steps=2, preconditioner="diagonal", n=3), advection=problems.IMPLICIT_UPWIND),
     "adr2d_n2": lambda: problems.adr_2d(s=8, n=2),
+    "adr3d_mixed_tags": lambda: _mixed_tags(problems.adr_3d(s=4, steps=2, preconditioner="diagonal")),
+    "adr3d_n3_mixed_tags_upwind": lambda: _mixed_tags(dataclasses.replace(
+        problems.adr_3d(s=3, steps=2, preconditioner="diagonal", n=3), advection=problems.IMPLICIT_UPWIND)),
     "adr2d_n3_implicit_trace": lambda: dataclasses.replace(problems.adr_2d(s=6, n=3),
                                                            advection=problems.IMPLICIT_TRACE),
 }

@@ -164,12 +164,11 @@ std::vector<uint64_t> build_face_slots(const Ctx &c)
     return slots;
 }
 
-// Store map of the interior assembly, per face E: the block index, in the
-// SELL row of E, of the column face at local index f of K0 (bits 3f..3f+2)
-// and of K1 (bits 12+3f..), found by the slot walk of the device-side
-// slot_blocks (the face E itself has no block: its field is 0, unused); the
-// local index of E in K0 (bits 24-26) and K1 (bits 27-29).  Boundary faces
-// use the slot word directly.
+// Store map of the assembly, per face E: the block index, in the SELL row of
+// E, of the column face at local index f of K0 (bits 3f..3f+2) and of K1
+// (bits 12+3f..): the slots of the merged column order, counted without the
+// slot of E itself (E has no stored block; its field is 0, unused); the
+// local index of E in K0 (bits 24-26) and K1 (bits 27-29, 0 on the boundary).
 std::vector<uint32_t> build_face_blocks(const std::vector<uint64_t> &slots)
 {
     std::vector<uint32_t> out(slots.size());

@@ -664,46 +664,6 @@ struct CellRow {
     }
 };
 
-// All N rows of cell K for its local face e (the boundary rows).
-template <int N, int FPC>
-struct CellRows {
-    CellRow<N, FPC> row[N];
-    __device__ __forceinline__ void compute(const SchemeView &s, int adv, bool use_adv, int K, int e, int E)
-    {
-        CellData<N, FPC> cd;
-        cd.load(s, K, e);
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            double Di[N], wv;
-            CellRow<N, FPC>::load_row(s, K, e, i, Di, wv);
-            row[i].compute(s, cd, Di, wv, adv, use_adv, K, e, E, i);
-        }
-    }
-};
-
-// Block index, in the SELL row of face E, of each local face of K0 (kp0)
-// and K1 (kp1).  The face_slots word lists the merged ascending faces of the
-// row (3 bits per cell and slot: the local index, 7 = not a face of that
-// cell); the face E itself (local index e0 in K0) has no block, so k counts
-// the other slots and stays below the face's stored width.
-template <int FPC>
-__device__ __forceinline__ void slot_blocks(uint64_t info, int nslots, int e0, int (&kp0)[FPC], int (&kp1)[FPC])
-{
-#pragma unroll
-    for (int f = 0; f < FPC; f++) kp0[f] = kp1[f] = 0;
-    int k = 0;
-    for (int t = 0; t < nslots; t++) {
-        const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
-        if (f0 == e0) continue;
-#pragma unroll
-        for (int f = 0; f < FPC; f++) {
-            if (f0 == f) kp0[f] = k;
-            if (f1 == f) kp1[f] = k;
-        }
-        k++;
-    }
-}
-
 template <int N>
 __device__ __forceinline__ void store_block_row(const SchemeView &s, int E, int k, int i, const double (&v)[N])
 {
@@ -711,6 +671,61 @@ __device__ __forceinline__ void store_block_row(const SchemeView &s, int E, int 
 #pragma unroll
     for (int j = 0; j < N; j++) p.v[j] = v[j];
     st<double, N>(s.A.val + sell_block(s.A.slice_ptr, E, k) * (N * N) + i * N, p);
+}
+
+// Boundary row (E, i): BoundaryConditions::setMatrixElements
+// (BoundaryConditions.hpp:335-508), one lane per row.  The row's tag selects
+// the form of its cell's contribution (BoundaryCoefficients_*,
+// BoundaryConditions.hpp:11-130): FixedFlux the advective one, the others
+// the plain one; an inflowing AdvectiveOutflow row is the identity with the
+// cell average as right-hand side; a FixedValue row is the (implicit) unit
+// diagonal alone (:373-380).
+template <int N, int FPC>
+__device__ __forceinline__ void boundary_row(const SchemeView &s, int q, int i)
+{
+    const int E = s.bnd_faces[q];
+    const long bi = (long)i * s.F + E;
+    const int tag = s.tags[bi];
+    if (tag == MHFEM_BC_FIXED_VALUE) {
+        s.rhs[(long)E * N + i] = s.bc_dirichlet[bi];
+        return;
+    }
+    const int K0 = s.face_cells[2 * E];
+    const uint32_t map = s.face_blocks[E];
+    const int e0 = (map >> 24) & 7;
+    const int adv = s.advection;
+    const long C = s.C;
+    const bool use_adv = tag == MHFEM_BC_FIXED_FLUX;
+    CellData<N, FPC> cd;
+    cd.load(s, K0, e0);
+    double Di[N], wv;
+    CellRow<N, FPC>::load_row(s, K0, e0, i, Di, wv);
+    CellRow<N, FPC> r;
+    r.compute(s, cd, Di, wv, adv, use_adv, K0, e0, E, i);
+    const long iidx = (((long)i * N + i) * FPC + e0) * C + K0;
+    const bool inflow = tag == MHFEM_BC_ADVECTIVE_OUTFLOW && (s.a[iidx] + s.u[iidx]) < 0;
+    // RowSetter::setRow (BoundaryConditions.hpp:132-222); unit diagonal not stored (:200-203)
+    const double d = inflow ? 1.0 : r.Aii;
+#pragma unroll
+    for (int f = 0; f < FPC; f++) {
+        if (f == e0) continue;
+        double v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = (inflow ? 0.0 : r.A[j][f]) / d;
+        store_block_row<N>(s, E, (map >> (3 * f)) & 7, i, v);
+    }
+#pragma unroll
+    for (int j = 0; j < N; j++)
+        if (j != i) s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = (inflow ? 0.0 : r.Ae[j]) / d;
+    double bval;
+    if (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) {
+        bval = 0;
+        bval += inflow ? s.Z_iK[(long)i * C + K0] : r.rhs;
+    } else {
+        bval = -s.bc_values[bi] * s.face_meas[E];
+        bval += r.rhs;
+    }
+    s.rhs[(long)E * N + i] = bval / d;
 }
 
 // Interior faces: LinearSystem::setMatrixElements (LinearSystem.h:66-256)
@@ -732,6 +747,16 @@ template <int N, int FPC>
 __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
 {
     constexpr int NR = N == 3 ? 4 : N;
+    // work-groups past the interior ones take the boundary rows, NR lanes
+    // per boundary face: one launch, and the few boundary rows run beside
+    // the interior ones
+    const unsigned interior_blocks = (unsigned)(((long)s.F * 2 * NR + WG - 1) / WG);
+    if (blockIdx.x >= interior_blocks) {
+        const long tb = (long)(blockIdx.x - interior_blocks) * WG + threadIdx.x;
+        const int q = (int)(tb / NR), ib = (int)(tb % NR);
+        if (q < s.nbnd && ib < N) boundary_row<N, FPC>(s, q, ib);
+        return;
+    }
     const long t = (long)blockIdx.x * WG + threadIdx.x;
     const int E = (int)(t / (2 * NR)), c = (int)(t / NR) & 1, i = (int)(t % NR);
     if (E >= s.F || i >= N) return;  // the partner lane (same E and i) returns too
@@ -741,7 +766,7 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
     const uint64_t cells = reinterpret_cast<const uint64_t *>(s.face_cells)[E];
     const uint32_t map = s.face_blocks[E];
     const int K0 = (int)(uint32_t)cells, K1 = (int)(uint32_t)(cells >> 32);
-    if ((K1 | (int)(map & 0x80000000u)) < 0) return;  // boundary faces: k_assemble_boundary
+    if ((K1 | (int)(map & 0x80000000u)) < 0) return;  // boundary faces: the boundary rows
     const int K = c ? K1 : K0;
     const int e = (map >> (24 + 3 * c)) & 7;
     const int adv = s.advection;
@@ -775,82 +800,6 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
         rhs += other_rhs;  // K0
         rhs += r.rhs;      // K1
         s.rhs[(long)E * N + i] = rhs / d;
-    }
-}
-
-// Boundary faces: BoundaryConditions::setMatrixElements
-// (BoundaryConditions.hpp:335-508), one thread per boundary face.
-template <int N, int FPC>
-__global__ void __launch_bounds__(WG) k_assemble_boundary(SchemeView s)
-{
-    const int q = blockIdx.x * WG + threadIdx.x;
-    if (q >= s.nbnd) return;
-    const int E = s.bnd_faces[q];
-    const int K0 = s.face_cells[2 * E];
-    const uint64_t info = s.face_slots[E];
-    const int e0 = (info >> 48) & 7;
-    const int nslots = (info >> 56) & 15;
-    int kp0[FPC], kp1[FPC];
-    slot_blocks<FPC>(info, nslots, e0, kp0, kp1);
-    const int adv = s.advection;
-    const long C = s.C;
-    // rows of one face may carry different tags: evaluate the forms needed
-    bool any_flux = false, any_plain = false;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const int tag = s.tags[(long)i * s.F + E];
-        if (tag == MHFEM_BC_FIXED_FLUX) any_flux = true;
-        else if (tag != MHFEM_BC_FIXED_VALUE) any_plain = true;
-    }
-    CellRows<N, FPC> ca, cn;
-    if (any_flux) ca.compute(s, adv, true, K0, e0, E);
-    if (any_plain) cn.compute(s, adv, false, K0, e0, E);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const long bi = (long)i * s.F + E;
-        const int tag = s.tags[bi];
-        if (tag == MHFEM_BC_FIXED_VALUE) {
-            // the row is the (implicit) unit diagonal alone (BoundaryConditions.hpp:373-380)
-            s.rhs[(long)E * N + i] = s.bc_dirichlet[bi];
-            continue;
-        }
-        // BoundaryCoefficients_* (BoundaryConditions.hpp:11-130)
-        const bool use_adv = tag == MHFEM_BC_FIXED_FLUX;
-        const long iidx = (((long)i * N + i) * FPC + e0) * C + K0;
-        const bool inflow = tag == MHFEM_BC_ADVECTIVE_OUTFLOW && (s.a[iidx] + s.u[iidx]) < 0;
-        double A[N][FPC], Ae[N];
-#pragma unroll
-        for (int j = 0; j < N; j++) {
-#pragma unroll
-            for (int f = 0; f < FPC; f++) {
-                const double v = use_adv ? ca.row[i].A[j][f] : cn.row[i].A[j][f];
-                A[j][f] = inflow ? ((i == j && f == e0) ? 1.0 : 0.0) : v;
-            }
-            const double ve = use_adv ? ca.row[i].Ae[j] : cn.row[i].Ae[j];
-            Ae[j] = inflow ? (i == j ? 1.0 : 0.0) : ve;
-        }
-        // RowSetter::setRow (BoundaryConditions.hpp:132-222); unit diagonal not stored (:200-203)
-        const double d = Ae[i];
-#pragma unroll
-        for (int f = 0; f < FPC; f++) {
-            if (f == e0) continue;
-            double v[N];
-#pragma unroll
-            for (int j = 0; j < N; j++) v[j] = A[j][f] / d;
-            store_block_row<N>(s, E, kp0[f], i, v);
-        }
-#pragma unroll
-        for (int j = 0; j < N; j++)
-            if (j != i) s.A.val[sell_diag(s.A.diag_base, N, E, i, j)] = Ae[j] / d;
-        double bval;
-        if (tag == MHFEM_BC_ADVECTIVE_OUTFLOW) {
-            bval = 0;
-            bval += inflow ? s.Z_iK[(long)i * C + K0] : cn.row[i].rhs;
-        } else {
-            bval = -s.bc_values[bi] * s.face_meas[E];
-            bval += use_adv ? ca.row[i].rhs : cn.row[i].rhs;
-        }
-        s.rhs[(long)E * N + i] = bval / d;
     }
 }
 
@@ -972,9 +921,10 @@ void launch_assemble(Ctx &c, double time, double tau)
     (void)time;
     (void)tau;
     SchemeView s = c.view();
-    const long lanes_per_face = 2L * (c.n == 3 ? 4 : c.n);
-    MHFEM_DISPATCH(c, k_assemble, blocks(lanes_per_face * c.F), s);
-    MHFEM_DISPATCH(c, k_assemble_boundary, blocks(c.nbnd), s);
+    // interior rows (2*NR lanes per face), then the boundary rows (NR lanes
+    // per boundary face) in the same launch
+    const long nr = c.n == 3 ? 4 : c.n;
+    MHFEM_DISPATCH(c, k_assemble, blocks(2 * nr * c.F) + blocks(nr * c.nbnd), s);
 }
 
 void launch_geometry(Ctx &c)
