@@ -290,7 +290,9 @@ int krylov_autotune(Ctx &c, double *best_ms);
 void krylov_segment_stats(Ctx &c, int *iterations, double *residues, int count);
 void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st);
 void spmv64(Ctx &c, const double *x, double *y);
-void spmm64(Ctx &c, const double *const *X, double *const *Y, int m);
+// learned-guess history: HIST_SLOTS FP64 slots per row, kept together
+constexpr int HIST_SLOTS = 16;
+void spmm_hist(Ctx &c, const double *H, double *AH, int slots);
 void residual64(Ctx &c, const double *x, const double *b, double *r);
 double dot64(Ctx &c, const double *x, const double *y);
 

@@ -395,21 +395,24 @@ __global__ void __launch_bounds__(GB) k_adam(int nblocks, const float *__restric
 // features of every row, packed as BF16 [rows][16]
 __global__ void __launch_bounds__(GB) k_features(int rows, int n, const double *__restrict__ r0,
                                                  const double *__restrict__ Ar0, const double *__restrict__ A2r0,
-                                                 const double *__restrict__ dx1, const double *__restrict__ dx2,
-                                                 const double *__restrict__ Adx1, const uint8_t *__restrict__ dirichlet,
+                                                 const double *__restrict__ H, const double *__restrict__ AH, int s1,
+                                                 int s2, const uint8_t *__restrict__ dirichlet,
                                                  const double *__restrict__ scal, unsigned short *__restrict__ X)
 {
     const int r = blockIdx.x * GB + threadIdx.x;
     if (r >= rows) return;
+    // the newest increment (slot s1), the one before it (s2) and A times the newest
+    const double dx1 = H[(long)r * HIST_SLOTS + s1], dx2 = H[(long)r * HIST_SLOTS + s2];
+    const double Adx1 = AH[(long)r * HIST_SLOTS + s1];
     const double isr = scal[0] > 0 ? 1.0 / scal[0] : 0.0;
     const double isx = scal[1] > 0 ? 1.0 / scal[1] : 0.0;
     float f[NF] = {};
     f[0] = (float)(r0[r] * isr);
     f[1] = (float)(Ar0[r] * isr);
     f[2] = (float)(A2r0[r] * isr);
-    f[3] = (float)(dx1[r] * isx);
-    f[4] = (float)(dx2[r] * isx);
-    f[5] = (float)(Adx1[r] * isx);
+    f[3] = (float)(dx1 * isx);
+    f[4] = (float)(dx2 * isx);
+    f[5] = (float)(Adx1 * isx);
     f[6] = dirichlet[r] ? 1.f : 0.f;
     f[7] = n > 1 ? (float)(r % n) / (float)(n - 1) : 0.f;
     f[8] = (scal[0] > 0 && scal[1] > 0) ? (float)(log10(scal[0] / scal[1]) * 0.25) : 0.f;
@@ -426,112 +429,151 @@ __global__ void __launch_bounds__(GB) k_direction(int rows, const float *__restr
     if (r < rows) d[r] = scal[1] * (double)y[r];
 }
 
-// x = x_prev + sum_k a_k D_k (up to three directions)
-// x = xp + sum_k a_k d_k over up to MAXDIR directions
-constexpr int HMAX = 16;           // previous increments kept
-constexpr int MAXDIR = HMAX + 1;   // + the network's direction
-struct Dirs {
-    const double *d[MAXDIR];
-    double a[MAXDIR];
-    int k;
+constexpr int HMAX = HIST_SLOTS;    // previous increments kept
+constexpr int MAXDIR = HMAX + 1;     // + the network's direction
+
+// Work-group sum (every lane calls it); the result is returned to lane 0.
+__device__ __forceinline__ double group_sum(double v)
+{
+    __shared__ double sm[GB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int w = 0; w < GB / 64; w++) t += sm[w];
+    return t;
+}
+
+// squared norm of v, one partial per work-group
+__global__ void __launch_bounds__(GB) k_norm_part(int rows, const double *__restrict__ v, double *__restrict__ part)
+{
+    const int r = blockIdx.x * GB + threadIdx.x;
+    const double x = r < rows ? v[r] : 0.0;
+    const double t = group_sum(x * x);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// The history: H[r*HIST_SLOTS + s] holds increment s of the ring (FP64) and
+// AH the same slot of A times it; a row's slots sit together, so the kernels
+// that use every slot (the product, the Gram matrix, the combination) read a
+// row in one piece.
+//
+// x = x_prev + a_net d + sum_s a_s H[., s]; G4 groups of 4 slots in use.
+struct HistCoef {
+    double a[HIST_SLOTS];
+    double a_net;
 };
-__global__ void __launch_bounds__(GB) k_combine_dirs(int rows, const double *__restrict__ xp, Dirs D,
-                                                     double *__restrict__ x)
+template <int G4>
+__global__ void __launch_bounds__(GB) k_combine_hist(int rows, const double *__restrict__ xp,
+                                                     const double *__restrict__ H, const double *__restrict__ dnet,
+                                                     HistCoef K, double *__restrict__ x)
 {
     const int r = blockIdx.x * GB + threadIdx.x;
     if (r >= rows) return;
     double v = xp[r];
-    for (int k = 0; k < D.k; k++) v += D.a[k] * D.d[k][r];
+    if (dnet) v += K.a_net * dnet[r];
+#pragma unroll
+    for (int g = 0; g < 2 * G4; g++) {
+        const Pack<double, 2> h = ld<double, 2>(H + (long)r * HIST_SLOTS + 2 * g);
+        v += K.a[2 * g] * h.v[0];
+        v += K.a[2 * g + 1] * h.v[1];
+    }
     x[r] = v;
 }
 
 // the increment of this solve becomes the newest history entry (and, while
-// the network trains, its normalised target)
+// the network trains, its normalised target); the partials of its squared
+// norm (as stored) give the next solve's feature scale
 __global__ void __launch_bounds__(GB) k_after_solve(int rows, const double *__restrict__ x, const double *__restrict__ xp,
-                                                    double *__restrict__ newest, const double *__restrict__ scal,
-                                                    float *__restrict__ target)
+                                                    double *__restrict__ H, int slot, const double *__restrict__ scal,
+                                                    float *__restrict__ target, double *__restrict__ part)
 {
     const int r = blockIdx.x * GB + threadIdx.x;
-    if (r >= rows) return;
-    const double d = x[r] - xp[r];
-    if (target) target[r] = scal[1] > 0 ? (float)(d / scal[1]) : 0.f;
-    newest[r] = d;
+    double e = 0.0;
+    if (r < rows) {
+        const double d = x[r] - xp[r];
+        if (target) target[r] = scal[1] > 0 ? (float)(d / scal[1]) : 0.f;
+        H[(long)r * HIST_SLOTS + slot] = d;
+        e = d * d;
+    }
+    const double t = group_sum(e);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// Gram matrix of up to MAXVEC vectors, computed in tiles of GT vectors: a
-// symmetric tile (the products of one chunk with itself, q <= p, packed at
-// p (p + 1) / 2 + q) or a cross tile (chunk a against chunk b, at
-// pa * GT + pb).  Absent vectors (null) count as zero.  Every tile writes one
-// partial per work-group and product at its offset in one buffer; one final
-// kernel sums them in a fixed order, so the result does not depend on timing.
-constexpr int MAXVEC = MAXDIR + 1;  // the residual + every direction
-constexpr int GT = 9;               // vectors per tile
-constexpr int NCHUNK = (MAXVEC + GT - 1) / GT;
-constexpr int MAXPROD = NCHUNK * GT * (GT + 1) / 2 + NCHUNK * (NCHUNK - 1) / 2 * GT * GT;
-struct Tile9 {
-    const double *v[GT];
-};
-template <bool SYM>
-__global__ void __launch_bounds__(GB) k_gram_tile(int rows, Tile9 A, Tile9 B, double *__restrict__ partial)
+// Gram matrix of u = [r0, w, AH slot 0 .. HIST_SLOTS-1] (w: A times the
+// network's direction, zero when absent): all NPROD products u_p . u_q,
+// q <= p, packed at p(p+1)/2 + q, in one pass.  The four wavefronts of a
+// work-group read the same 64 rows and each forms a quarter of the products
+// (triangle rows 0-8, 9-12, 13-15, 16-17: 45, 46, 45 and 35 products), so
+// the vectors come from memory once.
+// One partial per work-group and product; k_gram_final sums them in a fixed
+// order, so the result does not depend on timing.
+constexpr int NU = 2 + HIST_SLOTS, NPROD = NU * (NU + 1) / 2;
+constexpr int GACC = 46;
+template <int P0, int P1>
+__device__ __forceinline__ void gram_rows(const double (&u)[NU], double (&acc)[GACC])
 {
-    constexpr int NP = SYM ? GT * (GT + 1) / 2 : GT * GT;
-    double acc[NP] = {};
-    for (int r = blockIdx.x * GB + threadIdx.x; r < rows; r += gridDim.x * GB) {
-        double a[GT], b[GT];
 #pragma unroll
-        for (int p = 0; p < GT; p++) a[p] = A.v[p] ? A.v[p][r] : 0.0;
-        if constexpr (SYM) {
+    for (int p = P0; p < P1; p++)
 #pragma unroll
-            for (int p = 0; p < GT; p++)
+        for (int q = 0; q <= p; q++) acc[p * (p + 1) / 2 + q - P0 * (P0 + 1) / 2] += u[p] * u[q];
+}
+template <int P0, int P1>
+__device__ __forceinline__ void gram_store(const double (&acc)[GACC], double *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-                for (int q = 0; q <= p; q++) acc[p * (p + 1) / 2 + q] += a[p] * a[q];
-        } else {
+    for (int p = P0; p < P1; p++)
 #pragma unroll
-            for (int q = 0; q < GT; q++) b[q] = B.v[q] ? B.v[q][r] : 0.0;
+        for (int q = 0; q <= p; q++) {
+            double v = acc[p * (p + 1) / 2 + q - P0 * (P0 + 1) / 2];
 #pragma unroll
-            for (int p = 0; p < GT; p++)
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) partial[(long)(p * (p + 1) / 2 + q) * gridDim.x + blockIdx.x] = v;
+        }
+}
+__global__ void __launch_bounds__(GB) k_gram_hist(int rows, const double *__restrict__ r0, const double *__restrict__ w,
+                                                  const double *__restrict__ AH, double *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc[GACC] = {};
+    for (int r = blockIdx.x * 64 + lane; r < rows; r += gridDim.x * 64) {
+        double u[NU];
+        u[0] = r0[r];
+        u[1] = w ? w[r] : 0.0;
 #pragma unroll
-                for (int q = 0; q < GT; q++) acc[p * GT + q] += a[p] * b[q];
+        for (int g = 0; g < HIST_SLOTS / 2; g++) {
+            const Pack<double, 2> h = ld<double, 2>(AH + (long)r * HIST_SLOTS + 2 * g);
+            u[2 + 2 * g] = h.v[0];
+            u[3 + 2 * g] = h.v[1];
+        }
+        switch (wid) {
+        case 0: gram_rows<0, 9>(u, acc); break;
+        case 1: gram_rows<9, 13>(u, acc); break;
+        case 2: gram_rows<13, 16>(u, acc); break;
+        default: gram_rows<16, 18>(u, acc); break;
         }
     }
-    __shared__ double sm[NP][GB / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NP; q++) {
-        double v = acc[q];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0) sm[q][wid] = v;
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < NP; q += GB) {
-        double v = 0;
-#pragma unroll
-        for (int w = 0; w < GB / 64; w++) v += sm[q][w];
-        partial[(long)q * gridDim.x + blockIdx.x] = v;
+    switch (wid) {
+    case 0: gram_store<0, 9>(acc, partial); break;
+    case 1: gram_store<9, 13>(acc, partial); break;
+    case 2: gram_store<13, 16>(acc, partial); break;
+    default: gram_store<16, 18>(acc, partial); break;
     }
 }
 
-__global__ void __launch_bounds__(GB) k_gram_final(int nb, int np, const double *__restrict__ partial,
-                                                   double *__restrict__ out)
+// One work-group per product: the nb partials summed in a fixed order.
+__global__ void __launch_bounds__(GB) k_gram_final(int nb, const double *__restrict__ partial, double *__restrict__ out)
 {
-    __shared__ double sm[GB / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int q = 0; q < np; q++) {
-        double v = 0;
-        for (int b = threadIdx.x; b < nb; b += GB) v += partial[(long)q * nb + b];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0) sm[wid] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0;
-#pragma unroll
-            for (int w = 0; w < GB / 64; w++) t += sm[w];
-            out[q] = t;
-        }
-        __syncthreads();
-    }
+    const int q = blockIdx.x;
+    double v = 0;
+    for (int b = threadIdx.x; b < nb; b += GB) v += partial[(long)q * nb + b];
+    const double t = group_sum(v);
+    if (threadIdx.x == 0) out[q] = t;
 }
 
 }  // namespace mhfem
@@ -546,17 +588,16 @@ struct Guess {
     DBuf<unsigned short> X;
     DBuf<float> y, target;
     DBuf<double> xp, r0, Ar0, A2r0, rtry, scal;
-    DBuf<double> hist[HMAX], Ahist[HMAX];  // previous increments (ring) and A times them
-    int head = 0;                          // ring slot of the newest increment
-    double *inc(int j) { return hist[(head + j) % depth].p; }   // j-th newest
-    double *Ainc(int j) { return Ahist[(head + j) % depth].p; }
+    DBuf<double> H, AH;     // previous increments (ring of `depth` slots) and A times them
+    DBuf<double> norm_part, norms;  // squared norms of r0 and of the newest increment
+    int head = 0;           // ring slot of the newest increment
+    int slot(int j) const { return (head + j) % depth; }  // slot of the j-th newest
     DBuf<uint8_t> dirichlet;
     DBuf<float> P, m, v, partial, grad;
     DBuf<PackedWeights> W;
     DBuf<double> gram_part, gram;
-    double *h_gram = nullptr;  // pinned, MAXPROD values
+    double *h_gram = nullptr;  // pinned, NPROD values
     int gram_blocks = 0;
-    int gram_chunks = 0;
     int nblocks = 0;
     int adam_step = 0;
     int solves = 0;        // solves seen
@@ -632,12 +673,13 @@ Guess *guess_create(Ctx &c)
         b->alloc(n);
         MHFEM_HIP(hipMemset(b->p, 0, (size_t)n * sizeof(double)));
     }
-    for (int j = 0; j < 2; j++) {  // the default depth; guess_configure grows it
-        for (DBuf<double> *b : {&G->hist[j], &G->Ahist[j]}) {
-            b->alloc(n);
-            MHFEM_HIP(hipMemset(b->p, 0, (size_t)n * sizeof(double)));
-        }
+    for (DBuf<double> *b : {&G->H, &G->AH}) {
+        b->alloc((size_t)n * HIST_SLOTS);
+        MHFEM_HIP(hipMemset(b->p, 0, (size_t)n * HIST_SLOTS * sizeof(double)));
     }
+    G->norm_part.alloc(2 * (size_t)gblocks(n));
+    MHFEM_HIP(hipMemset(G->norm_part.p, 0, 2 * (size_t)gblocks(n) * sizeof(double)));
+    G->norms.alloc(2);
     G->scal.alloc(4);
     std::vector<uint8_t> dflag(n);
     for (int r = 0; r < n; r++) dflag[r] = (c.h_rowptr[r + 1] - c.h_rowptr[r]) == 1;
@@ -652,9 +694,9 @@ Guess *guess_create(Ctx &c)
     G->nblocks = (int)std::max<long>(1, std::min<long>(1024, (ntiles + 3) / 4));
     G->partial.alloc((size_t)G->nblocks * NPARAM_PAD);
     G->gram_blocks = (int)std::max<long>(1, std::min<long>(1024, ((long)n + GB - 1) / GB));
-    G->gram_part.alloc((size_t)G->gram_blocks * MAXPROD);
-    G->gram.alloc(MAXPROD);
-    MHFEM_HIP(hipHostMalloc((void **)&G->h_gram, MAXPROD * sizeof(double), hipHostMallocDefault));
+    G->gram_part.alloc((size_t)G->gram_blocks * NPROD);
+    G->gram.alloc(NPROD);
+    MHFEM_HIP(hipHostMalloc((void **)&G->h_gram, NPROD * sizeof(double), hipHostMallocDefault));
     G->cfg.enabled = 0;
     G->cfg.train_steps = 100;
     G->cfg.epochs = 20;
@@ -679,21 +721,14 @@ void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
     G.net_fail = 0;
     G.net_off = false;
     const int depth = cfg.history_depth > 0 ? std::min(cfg.history_depth, HMAX) : 2;
-    if (depth != G.depth || !G.hist[depth - 1].p) {
-        // a new depth starts an empty history in a linear ring of `depth` slots
+    if (depth != G.depth) {
+        // a new depth starts an empty history in a ring of `depth` slots
         G.depth = depth;
         G.history = 0;
         G.head = 0;
-        for (int j = 0; j < HMAX; j++) {
-            for (DBuf<double> *b : {&G.hist[j], &G.Ahist[j]}) {
-                if (j < depth) {
-                    if (!b->p) b->alloc(G.rows);
-                    MHFEM_HIP(hipMemset(b->p, 0, (size_t)G.rows * sizeof(double)));
-                } else {
-                    b->release();
-                }
-            }
-        }
+        for (DBuf<double> *b : {&G.H, &G.AH})
+            MHFEM_HIP(hipMemset(b->p, 0, (size_t)G.rows * HIST_SLOTS * sizeof(double)));
+        MHFEM_HIP(hipMemset(G.norm_part.p, 0, G.norm_part.n * sizeof(double)));
     }
     if (reseed && !G.have_weights) {
         init_params(G);
@@ -703,49 +738,21 @@ void guess_configure(Ctx &c, Guess &G, const mhfem_guess_config &cfg)
 
 namespace {
 
-// The products of the vectors v[0..nv), read back with one synchronisation
-// into G.h_gram and unpacked by gram_at(p, q).
-void gram(Ctx &c, Guess &G, const double *const *v, int nv)
+// All products of u = [r0, w, AH slots] (k_gram_hist), read back with one
+// synchronisation into G.h_gram; gram_at(p, q) unpacks them.
+void gram_hist(Ctx &c, Guess &G, const double *w)
 {
-    if (nv < 1 || nv > MAXVEC) throw Error{MHFEM_E_STATE, "gram: unsupported vector count"};
-    const int nch = (nv + GT - 1) / GT;
-    long off = 0;
-    for (int a = 0; a < nch; a++)
-        for (int b = a; b < nch; b++) {
-            Tile9 A{}, B{};
-            for (int t = 0; t < GT; t++) {
-                A.v[t] = a * GT + t < nv ? v[a * GT + t] : nullptr;
-                B.v[t] = b * GT + t < nv ? v[b * GT + t] : nullptr;
-            }
-            double *part = G.gram_part.p + off * G.gram_blocks;
-            if (a == b) {
-                hipLaunchKernelGGL((k_gram_tile<true>), G.gram_blocks, GB, 0, c.stream, G.rows, A, B, part);
-                off += GT * (GT + 1) / 2;
-            } else {
-                hipLaunchKernelGGL((k_gram_tile<false>), G.gram_blocks, GB, 0, c.stream, G.rows, A, B, part);
-                off += GT * GT;
-            }
-        }
-    G.gram_chunks = nch;
-    hipLaunchKernelGGL(k_gram_final, 1, GB, 0, c.stream, G.gram_blocks, (int)off, G.gram_part.p, G.gram.p);
+    hipLaunchKernelGGL(k_gram_hist, G.gram_blocks, GB, 0, c.stream, G.rows, G.r0.p, w, G.AH.p, G.gram_part.p);
+    hipLaunchKernelGGL(k_gram_final, NPROD, GB, 0, c.stream, G.gram_blocks, G.gram_part.p, G.gram.p);
     MHFEM_HIP(hipGetLastError());
-    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, off * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.gram.p, NPROD * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     MHFEM_HIP(hipStreamSynchronize(c.stream));
 }
 
-// v_p . v_q from the tiles read back by gram()
 double gram_at(const Guess &G, int p, int q)
 {
     if (q > p) std::swap(p, q);
-    const int cp = p / GT, cq = q / GT, lp = p % GT, lq = q % GT;
-    long off = 0;
-    for (int a = 0; a < G.gram_chunks; a++)
-        for (int b = a; b < G.gram_chunks; b++) {
-            if (a == cq && b == cp)
-                return a == b ? G.h_gram[off + lp * (lp + 1) / 2 + lq] : G.h_gram[off + lq * GT + lp];
-            off += a == b ? GT * (GT + 1) / 2 : GT * GT;
-        }
-    return 0.0;
+    return G.h_gram[p * (p + 1) / 2 + q];
 }
 
 // min || r0 - sum_{p in [lo, k)} a_p A D_p || by the normal equations
@@ -819,10 +826,15 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     G.features = false;
     MHFEM_HIP(hipMemcpyAsync(G.xp.p, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     residual64(c, x, b, G.r0.p);
-    const double *v2[2] = {G.r0.p, G.inc(0)};
-    gram(c, G, v2, 2);
-    const double rr = gram_at(G, 0, 0);
-    const double sx2 = gram_at(G, 1, 1) / n;
+    // |r0|^2 and |newest increment|^2 (its partials come from the last
+    // k_after_solve), read back with one synchronisation
+    hipLaunchKernelGGL(k_norm_part, g, GB, 0, c.stream, n, G.r0.p, G.norm_part.p);
+    hipLaunchKernelGGL(k_gram_final, 2, GB, 0, c.stream, (int)g, G.norm_part.p, G.norms.p);
+    MHFEM_HIP(hipGetLastError());
+    MHFEM_HIP(hipMemcpyAsync(G.h_gram, G.norms.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+    const double rr = G.h_gram[0];
+    const double sx2 = G.h_gram[1] / n;
     G.last.r0_norm = std::sqrt(rr);
     G.last.r_guess_norm = G.last.r0_norm;
     if (!(G.cfg.enabled) || G.history < 1 || rr == 0.0) return;
@@ -838,22 +850,15 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     if (sx2 > 0.0) {
         double sc[4] = {std::sqrt(rr / n), std::sqrt(sx2), 0, 0};
         MHFEM_HIP(hipMemcpyAsync(G.scal.p, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
-        // A times every stored increment, one pass over the matrix
-        const int m = use_hist ? h : 1;
-        const double *X[HMAX];
-        double *Y[HMAX];
-        for (int j = 0; j < m; j++) {
-            X[j] = G.inc(j);
-            Y[j] = G.Ainc(j);
-        }
-        for (int j0 = 0; j0 < m; j0 += 8) spmm64(c, X + j0, Y + j0, std::min(8, m - j0));
+        // A times every slot of the ring, one pass over the matrix
+        spmm_hist(c, G.H.p, G.AH.p, G.depth);
     }
     if (want_net && (use_net || training)) {
         spmv64(c, G.r0.p, G.Ar0.p);
         spmv64(c, G.Ar0.p, G.A2r0.p);
-        const double *dx2 = G.depth > 1 ? G.inc(1) : G.inc(0);
-        hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.inc(0), dx2,
-                           G.Ainc(0), G.dirichlet.p, G.scal.p, G.X.p);
+        const int s2 = G.depth > 1 ? G.slot(1) : G.slot(0);
+        hipLaunchKernelGGL(k_features, g, GB, 0, c.stream, n, c.n, G.r0.p, G.Ar0.p, G.A2r0.p, G.H.p, G.AH.p,
+                           G.slot(0), s2, G.dirichlet.p, G.scal.p, G.X.p);
         MHFEM_HIP(hipGetLastError());
         G.features = true;
     }
@@ -861,8 +866,9 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
     // runs alone) the previous increments.  The starting vector is the
     // combination x_prev + sum a_k D_k of least residual norm, so it is never
     // worse than the warm start; it is kept only if it lowers the residual.
-    Dirs D{};
-    const double *AD[MAXDIR] = {};
+    // Direction k is vector u[k] of the Gram matrix: u1 = A d for the
+    // network's direction d, u(2 + s) = A times ring slot s.
+    int uidx[MAXDIR];
     int k = 0;
     if (use_net) {
         const long ntiles = ((long)n + 15) / 16;
@@ -871,24 +877,16 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
         hipLaunchKernelGGL(k_direction, g, GB, 0, c.stream, n, G.y.p, G.scal.p, G.rtry.p);
         MHFEM_HIP(hipGetLastError());
         spmv64(c, G.rtry.p, G.Ar0.p);  // Ar0 is free once the features are packed
-        D.d[k] = G.rtry.p;
-        AD[k++] = G.Ar0.p;
+        uidx[k++] = 1;
     }
-    if (use_hist) {
-        for (int j = 0; j < h; j++) {
-            D.d[k] = G.inc(j);
-            AD[k++] = G.Ainc(j);
-        }
-    }
+    if (use_hist)
+        for (int j = 0; j < h; j++) uidx[k++] = 2 + G.slot(j);
     if (k == 0) return;
-    const double *vv[MAXVEC];
-    vv[0] = G.r0.p;
-    for (int p = 0; p < k; p++) vv[p + 1] = AD[p];
-    gram(c, G, vv, k + 1);
+    gram_hist(c, G, use_net ? G.Ar0.p : nullptr);
     double Gm[MAXDIR][MAXDIR] = {}, gv[MAXDIR] = {}, a[MAXDIR] = {};
     for (int p = 0; p < k; p++) {
-        gv[p] = gram_at(G, p + 1, 0);
-        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = gram_at(G, p + 1, q + 1);
+        gv[p] = gram_at(G, uidx[p], 0);
+        for (int q = 0; q <= p; q++) Gm[p][q] = Gm[q][p] = gram_at(G, uidx[p], uidx[q]);
     }
     const double pred = least_residual(Gm, gv, rr, 0, k, a);
     G.last.r_guess_norm = std::sqrt(std::max(pred, 0.0));
@@ -904,9 +902,18 @@ void guess_before_solve(Ctx &c, Guess &G, const double *b, double *x)
         }
     }
     if (!(pred < rr)) return;
-    for (int p = 0; p < k; p++) D.a[p] = a[p];
-    D.k = k;
-    hipLaunchKernelGGL(k_combine_dirs, g, GB, 0, c.stream, n, G.xp.p, D, x);
+    HistCoef K{};
+    for (int p = 0; p < k; p++) {
+        if (uidx[p] == 1) K.a_net = a[p];
+        else K.a[uidx[p] - 2] = a[p];
+    }
+    const double *dnet = use_net ? G.rtry.p : nullptr;
+    switch ((G.depth + 3) / 4) {
+    case 1: hipLaunchKernelGGL(k_combine_hist<1>, g, GB, 0, c.stream, n, G.xp.p, G.H.p, dnet, K, x); break;
+    case 2: hipLaunchKernelGGL(k_combine_hist<2>, g, GB, 0, c.stream, n, G.xp.p, G.H.p, dnet, K, x); break;
+    case 3: hipLaunchKernelGGL(k_combine_hist<3>, g, GB, 0, c.stream, n, G.xp.p, G.H.p, dnet, K, x); break;
+    default: hipLaunchKernelGGL(k_combine_hist<4>, g, GB, 0, c.stream, n, G.xp.p, G.H.p, dnet, K, x); break;
+    }
     MHFEM_HIP(hipGetLastError());
     G.last.used = 1;
 }
@@ -921,8 +928,8 @@ void guess_after_solve(Ctx &c, Guess &G, const double *x)
     // the newest increment takes the slot of the oldest (this solve's
     // features are already packed in X)
     G.head = (G.head + G.depth - 1) % G.depth;
-    hipLaunchKernelGGL(k_after_solve, g, GB, 0, c.stream, n, x, G.xp.p, G.inc(0), G.scal.p,
-                       train ? G.target.p : (float *)nullptr);
+    hipLaunchKernelGGL(k_after_solve, g, GB, 0, c.stream, n, x, G.xp.p, G.H.p, G.slot(0), G.scal.p,
+                       train ? G.target.p : (float *)nullptr, G.norm_part.p + g);
     MHFEM_HIP(hipGetLastError());
     if (train) {
         for (int e = 0; e < G.cfg.epochs; e++) train_epoch(c, G, n, G.cfg.learning_rate);

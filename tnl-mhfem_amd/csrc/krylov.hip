@@ -463,61 +463,90 @@ __global__ void __launch_bounds__(KB) k_spmv_plain(SellView<T> A, const T *__res
         st<T, N>(y + (int64_t)f * N, sell_face<N>(A, x, f));
 }
 
-// Y_v = A X_v for M vectors at once: the matrix is streamed once and each
-// block multiplies the M gathered vectors
-template <int M>
-struct MultiVec {
-    const double *x[M];
-    double *y[M];
-};
-template <int N, int M>
-__global__ void __launch_bounds__(KB) k_spmm_plain(SellView<double> A, MultiVec<M> V)
+// AH = A H for the stored increments of the learned guess.  H and AH keep
+// the HIST_SLOTS slots of a row together (FP64, 128 bytes per row): the
+// gather of a column face fetches the values of every slot from two cache
+// lines, where one array per slot would touch a line per slot.  The G4
+// groups of 4 slots (those in use) are taken GP groups per sweep over the
+// face's blocks (the blocks come from the caches after the first sweep).
+template <int N, int G4, int GP>
+__global__ void __launch_bounds__(KB) k_spmm_hist(SellView<double> A, const double *__restrict__ H,
+                                                  double *__restrict__ AH)
 {
-    constexpr int PF = 8;  // column indices of a face loaded ahead (wave-uniform k < w guards)
-    for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
-        const int64_t base = A.slice_ptr[f >> 6] + (f & 63);
-        const int w = A.slice_width[f >> 6];
-        int col[PF];
+    constexpr int PF = 8, SW = 4 * GP;  // slots per sweep
+    const int f = xcd_block(gridDim.x) * KB + threadIdx.x;
+    if (f >= A.faces) return;
+    const int64_t base = A.slice_ptr[f >> 6] + (f & 63);
+    const int w = A.slice_width[f >> 6];
+    int col[PF];
 #pragma unroll
-        for (int k = 0; k < PF; k++)
-            if (k < w) col[k] = __builtin_nontemporal_load(A.col + base + (int64_t)k * 64);
-        Pack<double, N> y[M];
+    for (int k = 0; k < PF; k++)
+        if (k < w) col[k] = A.col[base + (int64_t)k * 64];
+#pragma unroll 1
+    for (int g0 = 0; g0 < G4; g0 += GP) {
+        auto load_rows = [&](int face, double (&v)[N][SW]) {
 #pragma unroll
-        for (int v = 0; v < M; v++) y[v] = ld<double, N>(V.x[v] + (int64_t)f * N);
-        if constexpr (N > 1) {
-            const Pack<double, N *(N - 1)> dg =
-                ld_stream<double, N *(N - 1)>(A.val + A.diag_base + (int64_t)f * N * (N - 1));
+            for (int j = 0; j < N; j++) {
+                const double *p = H + ((int64_t)face * N + j) * HIST_SLOTS + 4 * g0;
 #pragma unroll
-            for (int v = 0; v < M; v++) {
-                const Pack<double, N> xe = y[v];
+                for (int t = 0; t < SW; t += 2) {
+                    const Pack<double, 2> a = ld<double, 2>(p + t);
+                    v[j][t] = a.v[0];
+                    v[j][t + 1] = a.v[1];
+                }
+            }
+        };
+        double y[N][SW];
+        {
+            // the unit diagonal and the stored entries of the diagonal block
+            double xe[N][SW];
+            load_rows(f, xe);
+#pragma unroll
+            for (int i = 0; i < N; i++)
+#pragma unroll
+                for (int s = 0; s < SW; s++) y[i][s] = xe[i][s];
+            if constexpr (N > 1) {
+                const Pack<double, N *(N - 1)> dg = ld<double, N *(N - 1)>(A.val + A.diag_base + (int64_t)f * N * (N - 1));
 #pragma unroll
                 for (int i = 0; i < N; i++)
 #pragma unroll
                     for (int j = 0; j < N; j++)
-                        if (j != i) y[v].v[i] += dg.v[i * (N - 1) + (j < i ? j : j - 1)] * xe.v[j];
+                        if (j != i) {
+                            const double a = dg.v[i * (N - 1) + (j < i ? j : j - 1)];
+#pragma unroll
+                            for (int s = 0; s < SW; s++) y[i][s] += a * xe[j][s];
+                        }
             }
         }
         auto block = [&](int64_t q, int c) {
-            const Pack<double, N * N> b = ld_stream<double, N * N>(A.val + q * (N * N));
-            Pack<double, N> xf[M];
+            const Pack<double, N * N> b = ld<double, N * N>(A.val + q * (N * N));
+            double xc[N][SW];
+            load_rows(c, xc);
 #pragma unroll
-            for (int v = 0; v < M; v++) xf[v] = ld<double, N>(V.x[v] + (int64_t)c * N);
+            for (int i = 0; i < N; i++)
 #pragma unroll
-            for (int v = 0; v < M; v++)
+                for (int j = 0; j < N; j++)
 #pragma unroll
-                for (int i = 0; i < N; i++)
-#pragma unroll
-                    for (int j = 0; j < N; j++) y[v].v[i] += b.v[i * N + j] * xf[v].v[j];
+                    for (int s = 0; s < SW; s++) y[i][s] += b.v[i * N + j] * xc[j][s];
         };
 #pragma unroll
         for (int k = 0; k < PF; k++)
             if (k < w) block(base + (int64_t)k * 64, col[k]);
         for (int k = PF; k < w; k++) {
             const int64_t q = base + (int64_t)k * 64;
-            block(q, __builtin_nontemporal_load(A.col + q));
+            block(q, A.col[q]);
         }
 #pragma unroll
-        for (int v = 0; v < M; v++) st<double, N>(V.y[v] + (int64_t)f * N, y[v]);
+        for (int i = 0; i < N; i++) {
+            double *o = AH + ((int64_t)f * N + i) * HIST_SLOTS + 4 * g0;
+#pragma unroll
+            for (int t = 0; t < SW; t += 2) {
+                Pack<double, 2> a;
+                a.v[0] = y[i][t];
+                a.v[1] = y[i][t + 1];
+                st<double, 2>(o + t, a);
+            }
+        }
     }
 }
 
@@ -1224,37 +1253,29 @@ void spmv64(Ctx &c, const double *x, double *y)
     S.spmv_plain<double>(x, y);
 }
 
-// Y_v = A X_v, v < m (m <= 8), one pass over the matrix
-template <int M>
-static void spmm_launch(Ctx &c, Solve &S, const double *const *X, double *const *Y)
+// AH = A H over the first `slots` slots of every row (rounded up to a
+// multiple of 4)
+void spmm_hist(Ctx &c, const double *H, double *AH, int slots)
 {
-    MultiVec<M> V{};
-    for (int v = 0; v < M; v++) {
-        V.x[v] = X[v];
-        V.y[v] = Y[v];
-    }
-    SellView<double> A = S.matrix<double>();
-    with_n(c.n, [&](auto NC) {
-        hipLaunchKernelGGL((k_spmm_plain<decltype(NC)::value, M>), grid_for(c.F), KB, 0, c.stream, A, V);
-    });
-    MHFEM_HIP(hipGetLastError());
-}
-void spmm64(Ctx &c, const double *const *X, double *const *Y, int m)
-{
+    if (slots < 1) return;
+    if (slots > HIST_SLOTS) throw Error{MHFEM_E_ARGUMENT, "spmm_hist: too many slots"};
     Krylov &k = *c.krylov;
     Solve S(c, k);
-    switch (m) {
-    case 0: break;
-    case 1: spmm_launch<1>(c, S, X, Y); break;
-    case 2: spmm_launch<2>(c, S, X, Y); break;
-    case 3: spmm_launch<3>(c, S, X, Y); break;
-    case 4: spmm_launch<4>(c, S, X, Y); break;
-    case 5: spmm_launch<5>(c, S, X, Y); break;
-    case 6: spmm_launch<6>(c, S, X, Y); break;
-    case 7: spmm_launch<7>(c, S, X, Y); break;
-    case 8: spmm_launch<8>(c, S, X, Y); break;
-    default: throw Error{MHFEM_E_ARGUMENT, "spmm64: at most 8 vectors"};
-    }
+    SellView<double> A = S.matrix<double>();
+    const unsigned grid = (unsigned)((c.F + KB - 1) / KB);
+    const int g4 = (slots + 3) / 4;
+    with_n(c.n, [&](auto NC) {
+        constexpr int NN = decltype(NC)::value;
+        // every slot in one sweep where the registers allow (8 slots per sweep at n = 3)
+        constexpr int GPMAX = NN <= 2 ? 4 : 2;
+        switch (g4) {
+        case 1: hipLaunchKernelGGL((k_spmm_hist<NN, 1, 1>), grid, KB, 0, c.stream, A, H, AH); break;
+        case 2: hipLaunchKernelGGL((k_spmm_hist<NN, 2, 2>), grid, KB, 0, c.stream, A, H, AH); break;
+        case 3: hipLaunchKernelGGL((k_spmm_hist<NN, 3, (GPMAX < 3 ? 1 : 3)>), grid, KB, 0, c.stream, A, H, AH); break;
+        default: hipLaunchKernelGGL((k_spmm_hist<NN, 4, GPMAX>), grid, KB, 0, c.stream, A, H, AH); break;
+        }
+    });
+    MHFEM_HIP(hipGetLastError());
 }
 
 void residual64(Ctx &c, const double *x, const double *b, double *r)
