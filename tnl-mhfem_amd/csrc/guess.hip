@@ -39,7 +39,7 @@ constexpr int P_B2 = P_W2 + NH * NH;    // [NH]
 constexpr int P_W3 = P_B2 + NH;         // [NH]
 constexpr int P_B3 = P_W3 + NH;         // [1]
 constexpr int NPARAM = P_B3 + 1;        // 5313
-constexpr int NPARAM_PAD = 5376;
+constexpr int NPARAM_PAD = 5376;      // a multiple of 64 (k_adam takes 64 parameters per work-group)
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -372,14 +372,28 @@ __global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short
 }
 
 // sum the work-group partials (deterministic order) and take one Adam step
+// The gradient is the sum of the nblocks per-work-group partials: each
+// work-group takes 64 parameters, its four wavefronts a quarter of the
+// partials each (four running sums per lane, so the loads overlap), in a
+// fixed order; then the Adam update.
 __global__ void __launch_bounds__(GB) k_adam(int nblocks, const float *__restrict__ partial, float *__restrict__ P,
                                              float *__restrict__ m, float *__restrict__ v, float *__restrict__ grad_out,
                                              float lr, float b1, float b2, float eps, int step)
 {
-    const int p = blockIdx.x * GB + threadIdx.x;
-    if (p >= NPARAM_PAD) return;
-    float g = 0.f;
-    for (int b = 0; b < nblocks; b++) g += partial[(long)b * NPARAM_PAD + p];
+    __shared__ float sm[GB / 64][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    const int b0 = (int)((long)nblocks * q / (GB / 64)), b1e = (int)((long)nblocks * (q + 1) / (GB / 64));
+    float a[4] = {0.f, 0.f, 0.f, 0.f};
+    int b = b0;
+    for (; b + 4 <= b1e; b += 4)
+#pragma unroll
+        for (int t = 0; t < 4; t++) a[t] += partial[(long)(b + t) * NPARAM_PAD + p];
+    for (; b < b1e; b++) a[0] += partial[(long)b * NPARAM_PAD + p];
+    sm[q][lane] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (q != 0) return;
+    const float g = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
     if (grad_out) grad_out[p] = g;
     if (p >= NPARAM || lr <= 0.f) return;
     const float mm = b1 * m[p] + (1.f - b1) * g;
@@ -652,7 +666,7 @@ double train_epoch(Ctx &c, Guess &G, long rows, float lr)
     hipLaunchKernelGGL(k_mlp_grad, G.nblocks, GB, 0, c.stream, rows, G.X.p, G.target.p, G.W.p,
                        1.0f / (float)(nsub * 16), G.partial.p, stride, offset);
     G.adam_step++;
-    hipLaunchKernelGGL(k_adam, (NPARAM_PAD + GB - 1) / GB, GB, 0, c.stream, G.nblocks, G.partial.p, G.P.p, G.m.p,
+    hipLaunchKernelGGL(k_adam, NPARAM_PAD / 64, GB, 0, c.stream, G.nblocks, G.partial.p, G.P.p, G.m.p,
                        G.v.p, G.grad.p, lr, 0.9f, 0.999f, 1e-8f, G.adam_step);
     MHFEM_HIP(hipGetLastError());
     pack(c, G);
@@ -973,7 +987,7 @@ void guess_gradient(Ctx &c, Guess &G, const uint16_t *X, const float *t, long ro
     hipLaunchKernelGGL(k_mlp_grad, nb, GB, 0, c.stream, rows, dX.p, dt.p, G.W.p, 1.0f / (float)rows, G.partial.p,
                        1L, 0L);
     // reduce only (learning rate 0)
-    hipLaunchKernelGGL(k_adam, (NPARAM_PAD + GB - 1) / GB, GB, 0, c.stream, nb, G.partial.p, G.P.p, G.m.p, G.v.p,
+    hipLaunchKernelGGL(k_adam, NPARAM_PAD / 64, GB, 0, c.stream, nb, G.partial.p, G.P.p, G.m.p, G.v.p,
                        G.grad.p, 0.0f, 0.9f, 0.999f, 1e-8f, 1);
     MHFEM_HIP(hipGetLastError());
     MHFEM_HIP(hipMemcpyAsync(grad, G.grad.p, (NPARAM + 1) * sizeof(float), hipMemcpyDeviceToHost, c.stream));
