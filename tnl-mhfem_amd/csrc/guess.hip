@@ -188,6 +188,16 @@ __global__ void __launch_bounds__(GB) k_mlp_forward(long rows, const unsigned sh
 
 // Gradient of L = (1/rows) sum_s (y_s - t_s)^2 with respect to all
 // parameters; one partial vector per work-group, plus the partial loss.
+// The transposes of k_mlp_grad exchange values between the lanes of one
+// wavefront through that wavefront's own LDS buffer.  The LDS executes a
+// wavefront's operations in issue order, so no work-group barrier is needed:
+// a wavefront-scope fence keeps the compiler from moving the accesses.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short *__restrict__ X,
                                                  const float *__restrict__ target,
                                                  const PackedWeights *__restrict__ Wg, float inv_rows,
@@ -218,24 +228,49 @@ __global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short
     float gb3 = 0.f, loss = 0.f;
 
     // every wavefront runs the same number of tiles (tiles past the end are
-    // all-zero and contribute nothing), so the work-group barriers of the
-    // LDS transposes below are reached uniformly
+    // all-zero and contribute nothing)
     // the epoch uses every tile_stride-th 16-sample tile, starting at tile_offset
     const long ntiles = (rows + 15) / 16;
     const long nsub = tile_offset < ntiles ? (ntiles - tile_offset + tile_stride - 1) / tile_stride : 0;
     const long wave = (long)blockIdx.x * (GB / 64) + wid;
     const long nwaves = (long)gridDim.x * (GB / 64);
     const long iters = (nsub + nwaves - 1) / nwaves;
-    for (long it = 0; it < iters; it++) {
+    // The inputs of a tile (features in both operand layouts, targets) are
+    // loaded one tile ahead, so their latency hides behind the previous
+    // tile's arithmetic.
+    auto tile_start = [&](long it) {
         const long tsub = wave + it * nwaves;
-        const long s0 = tsub < nsub ? (tsub * tile_stride + tile_offset) * 16 : rows;
+        return tsub < nsub ? (tsub * tile_stride + tile_offset) * 16 : rows;
+    };
+    auto load_ax = [&](long s0) {
+        short4_t ax;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const long s = s0 + 4 * q + t;
+            ax[t] = (short)(s < rows ? X[s * NF + i] : 0);
+        }
+        return ax;
+    };
+    long s_next = tile_start(0);
+    short4_t X_next = load_features(X, rows, s_next), AX_next = load_ax(s_next);
+    float t_next = (l < 16 && s_next + l < rows) ? target[s_next + l] : 0.f;
+    for (long it = 0; it < iters; it++) {
+        const long s0 = s_next;
         Tile T;
-        T.X = load_features(X, rows, s0);
+        T.X = X_next;
+        const short4_t AX = AX_next;
+        const float tgt = t_next;
+        if (it + 1 < iters) {
+            s_next = tile_start(it + 1);
+            X_next = load_features(X, rows, s_next);
+            AX_next = load_ax(s_next);
+            t_next = (l < 16 && s_next + l < rows) ? target[s_next + l] : 0.f;
+        }
         tile_forward(W, A1, A2, A3, T);
         // dL/dy for the sample of lanes 0..15, broadcast to lanes with the same i
         float g = 0.f;
         if (l < 16 && s0 + l < rows) {
-            const float d = T.out - target[s0 + l];
+            const float d = T.out - tgt;
             loss += d * d;
             g = 2.f * d * inv_rows;
         }
@@ -283,31 +318,25 @@ __global__ void __launch_bounds__(GB) k_mlp_grad(long rows, const unsigned short
                 tr[wid][0][4 * q + t][i] = bf2f((unsigned short)T.H1[b][t]);
                 tr[wid][1][4 * q + t][i] = bf2f((unsigned short)gZ2[b][t]);
             }
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 AH1[b][t] = (short)f2bf(tr[wid][0][i][4 * q + t]);
                 BG2[b][t] = (short)f2bf(tr[wid][1][i][4 * q + t]);
             }
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
             for (int t = 0; t < 4; t++) tr[wid][1][4 * q + t][i] = bf2f((unsigned short)gZ1[b][t]);
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
             for (int t = 0; t < 4; t++) BG1[b][t] = (short)f2bf(tr[wid][1][i][4 * q + t]);
-            __syncthreads();
+            wave_lds_sync();
         }
 #pragma unroll
         for (int kb = 0; kb < 4; kb++)
 #pragma unroll
             for (int mb = 0; mb < 4; mb++) gW2[kb * 4 + mb] = mfma(AH1[kb], BG2[mb], gW2[kb * 4 + mb]);
         // dW1[f][k] = sum_s X[s][f] gZ1[s][k]; A lane: X[sample 4q+t][feature i]
-        short4_t AX;
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const long s = s0 + 4 * q + t;
-            AX[t] = (short)(s < rows ? X[s * NF + i] : 0);
-        }
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) gW1[mb] = mfma(AX, BG1[mb], gW1[mb]);
     }
