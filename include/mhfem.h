@@ -128,8 +128,10 @@ typedef struct {
     long long iterations, time_steps;
     /* timing level 2: every 8th SpMV launch of the Krylov iteration is
        bracketed by HIP events on the context's stream */
-    double spmv_ms;       /* summed duration of the timed SpMV launches */
-    long long spmv_timed; /* number of timed SpMV launches */
+    double spmv_ms;       /* summed duration of the timed SpMV launches that did the
+                             product (launches queued after convergence return at once
+                             and are left out: below a quarter of the median) */
+    long long spmv_timed; /* number of those launches */
 } mhfem_timers;
 
 /* ---- lifetime (Solver::setMesh, Solver.hpp:26-81; BaseModel::allocate, BaseModel.h:168-195) */

@@ -996,6 +996,7 @@ int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out)
         return e.code;
     }
     *out = c->timers;
+    kernel_timing_summary(*c, out->spmv_ms, out->spmv_timed);
     return MHFEM_OK;
 }
 
@@ -1004,6 +1005,7 @@ int mhfem_reset_timers(mhfem_ctx *ctx)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return MHFEM_E_ARGUMENT;
     c->kt_used = 0;
+    c->kt_samples.clear();
     c->timers = mhfem_timers{};
     return MHFEM_OK;
 }

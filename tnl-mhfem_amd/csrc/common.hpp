@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/mhfem.h"
@@ -229,6 +230,7 @@ struct Ctx {
     std::vector<hipEvent_t> kt_ev;
     size_t kt_used = 0;
     long kt_count = 0;
+    std::vector<float> kt_samples;  // durations read back (ms)
     mhfem_timers timers{};
     std::string err;
 
@@ -246,10 +248,26 @@ inline void kernel_timing_flush(Ctx &c)
         MHFEM_HIP(hipEventSynchronize(c.kt_ev[q + 1]));
         float ms = 0;
         MHFEM_HIP(hipEventElapsedTime(&ms, c.kt_ev[q], c.kt_ev[q + 1]));
-        c.timers.spmv_ms += ms;
-        c.timers.spmv_timed++;
+        c.kt_samples.push_back(ms);
     }
     c.kt_used = 0;
+}
+// Average of the sampled launches that did the product: a launch queued
+// after its segments converged returns at once (a few microseconds against
+// tens), so samples below a quarter of the median are left out.
+inline void kernel_timing_summary(const Ctx &c, double &ms_sum, long long &count)
+{
+    ms_sum = 0.0;
+    count = 0;
+    if (c.kt_samples.empty()) return;
+    std::vector<float> v(c.kt_samples);
+    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    const float cut = 0.25f * v[v.size() / 2];
+    for (float ms : c.kt_samples)
+        if (ms >= cut) {
+            ms_sum += ms;
+            count++;
+        }
 }
 inline bool kernel_timing_begin(Ctx &c)
 {

@@ -682,6 +682,9 @@ struct Krylov {
     DBuf<double> partials;
     DBuf<double> sc;
     double *h_sc = nullptr;  // pinned, nseg * NSLOTS
+    // flags of a batch of iterations read while the next batch runs
+    double *h_sc_ahead[2] = {nullptr, nullptr};  // pinned, nseg * NSLOTS each
+    hipEvent_t ev_ahead[2] = {nullptr, nullptr};
     Ilu0 ilu;
     // FP32 inner solver
     DBuf<float> val32, dinv32, b32, x32, r32, r_ast32, p32, Ap32, s32, As32, tmp32, rhs32;
@@ -740,6 +743,11 @@ Krylov *krylov_create(Ctx &c)
         b->alloc(n);
     k->sc.alloc((size_t)k->nseg * NSLOTS);
     MHFEM_HIP(hipHostMalloc((void **)&k->h_sc, (size_t)k->nseg * NSLOTS * sizeof(double), hipHostMallocDefault));
+    for (int q = 0; q < 2; q++) {
+        MHFEM_HIP(hipHostMalloc((void **)&k->h_sc_ahead[q], (size_t)k->nseg * NSLOTS * sizeof(double),
+                                hipHostMallocDefault));
+        MHFEM_HIP(hipEventCreateWithFlags(&k->ev_ahead[q], hipEventDisableTiming));
+    }
     build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
         Ilu0 &I = k->ilu;
@@ -803,6 +811,10 @@ void krylov_destroy(Krylov *k)
     if (!k) return;
     drop_graphs(*k);
     if (k->h_sc) (void)hipHostFree(k->h_sc);
+    for (int q = 0; q < 2; q++) {
+        if (k->h_sc_ahead[q]) (void)hipHostFree(k->h_sc_ahead[q]);
+        if (k->ev_ahead[q]) (void)hipEventDestroy(k->ev_ahead[q]);
+    }
     delete k;
 }
 
@@ -1039,16 +1051,65 @@ struct Solve {
     }
     // poll the flags, restart segments that need it, queue more iterations
     template <typename T, typename More>
+    // Poll, restart the segments that need it, queue more iterations.  While
+    // the residual drop of the last batch says that more than two further
+    // batches are needed, one batch is kept queued ahead of the flags the host
+    // is reading (an asynchronous copy into pinned memory and an event), so
+    // the GPU does not wait for the host between batches.  Near convergence
+    // the host waits for each batch instead: a batch queued after convergence
+    // would cost its launches (the kernels of a finished segment return at
+    // once) and short solves would lose more than they gain.
     int bicg_drive(const T *b, T *x, const Vecs &v, More &&more)
     {
+        const size_t bytes = (size_t)k.nseg * NSLOTS * sizeof(double);
+        auto snap = [&](int q) {
+            MHFEM_HIP(hipMemcpyAsync(k.h_sc_ahead[q], k.sc.p, bytes, hipMemcpyDeviceToHost, c.stream));
+            MHFEM_HIP(hipEventRecord(k.ev_ahead[q], c.stream));
+        };
+        auto take = [&](int q) {
+            MHFEM_HIP(hipEventSynchronize(k.ev_ahead[q]));
+            std::memcpy(k.h_sc, k.h_sc_ahead[q], bytes);
+        };
+        // residuals one batch earlier; 0 = unknown (no batch measured yet)
+        std::vector<double> prev(k.nseg, 0.0);
+        auto remember = [&]() {
+            for (int s = 0; s < k.nseg; s++) prev[s] = seg_sc(s)[S_RESID];
+        };
+        // batches the slowest running segment still needs, by geometric
+        // extrapolation of its last batch (0 while it is unknown)
+        auto batches_left = [&]() {
+            double worst = 0.0;
+            for (int s = 0; s < k.nseg; s++) {
+                const double *f = seg_sc(s);
+                if (f[S_DONE] != 0.0 || f[S_RESTART] != 0.0) continue;
+                const double r = f[S_RESID], t = f[S_TOL], p = prev[s];
+                if (!(r > t) || !(p > 0.0)) continue;
+                worst = std::max(worst, p > r ? std::log(t / r) / std::log(r / p) : 1e9);
+            }
+            return worst;
+        };
+        poll();
         for (;;) {
-            poll();
             if (all_done()) break;
             if (any_restart()) {
                 bicg_init<T>(b, x, v, M_RESTART);
+                std::fill(prev.begin(), prev.end(), 0.0);
+                poll();
                 continue;
             }
             more();
+            if (batches_left() > 2.0) {
+                snap(0);
+                for (int q = 0;; q ^= 1) {
+                    more();
+                    snap(q ^ 1);
+                    remember();
+                    take(q);
+                    if (all_done() || any_restart() || !(batches_left() > 2.0)) break;
+                }
+            }
+            remember();
+            poll();  // the batch still queued ends; the latest flags decide
         }
         int it = 0;
         for (int s = 0; s < k.nseg; s++) it = std::max(it, (int)seg_sc(s)[S_IT]);
