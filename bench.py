@@ -40,7 +40,7 @@ roofline: the dominant kernel is the FP32 block-SELL SpMV of the chosen
 form's iteration.  Every 8th launch during the chosen form's timed steps is
 bracketed by HIP events on the library's stream (timing level 2);
 achieved = algorithmic bytes per launch / average launch duration.
-assembly: the same for assembleLinearSystem (interior and boundary rows),
+assembly: the same for assembleLinearSystem (one launch: interior and boundary rows),
 timed with HIP events around every assembly of the chosen form's timed steps.
 cpu_baseline: the C oracle (a restatement of the reference host path), FP64
 BiCGStab + ILU(0), on a bounded sample of one step of the same workload.
@@ -490,7 +490,7 @@ def main():
             "format_GBs": round(fmt / (spmv_ms * 1e-3) / 1e9, 1),
         },
         "assembly": {
-            "kernels": "k_assemble + k_assemble_boundary (assembleLinearSystem)",
+            "kernels": "k_assemble (interior and boundary rows, assembleLinearSystem)",
             "bound": "hbm",
             "avg_ms": round(tm["assemble"] / args.steps, 4),
             "algorithmic_bytes": asm_alg,

@@ -1,0 +1,63 @@
+"""Phase timers and sampled SpMV timing (mhfem_get_timers), and the derived
+arrays formed on request (b_ijK_storage, R_ijKe, R_iK) after several full
+steps, against the oracle."""
+import numpy as np
+import pytest
+
+from mhfem_amd import problems, capi
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def maxrel(a, b):
+    s = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / s
+
+
+def test_timers_consistent():
+    pr = problems.darcy_3d(s=6, steps=3, preconditioner="diagonal")
+    s = capi.Solver(pr, tol=1e-10)
+    s.set_timing(2)
+    s.reset_timers()
+    t, its = 0.0, 0
+    for _ in range(3):
+        its += s.step(t, pr.dt).iterations
+        t += pr.dt
+    tm = s.timers()
+    assert tm["time_steps"] == 3
+    assert tm["iterations"] == its
+    for k in ("pre_iterate", "assemble", "linear_solver", "post_iterate"):
+        assert tm[k] > 0.0, k
+    # every 8th product launch is timed: two per iteration, the start
+    # residual of each solve, and at most two batches of 8 iterations queued
+    # after convergence per solve (their launches return at once; on a system
+    # this small they are not much shorter than a working product)
+    assert 0 < tm["spmv_timed"] <= (2 * its + 3 * (2 + 2 * 2 * 8)) // 8 + 2
+    mean_ms = tm["spmv_ms"] / tm["spmv_timed"]
+    assert 0.0 < mean_ms < tm["linear_solver"]
+    s.reset_timers()
+    tm = s.timers()
+    assert tm["spmv_timed"] == 0 and tm["time_steps"] == 0
+    s.close()
+
+
+def test_derived_arrays_after_steps():
+    pr = problems.adr_3d(s=4, steps=3, preconditioner="diagonal")
+    s = capi.Solver(pr, tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for _ in range(3):
+        s.step(t, pr.dt)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+    # the arrays of the last preIterate; after full steps Z differs from the
+    # oracle's by the solve tolerance, which bounds the differences here
+    for nm in ("b_ijK_storage", "R_ijKe", "R_iK"):
+        a, b = s.get_array(nm), o.array(nm)
+        assert a.shape == b.shape
+        assert maxrel(a, b) < 1e-9, (nm, maxrel(a, b))
+    # a device pointer to a derived array is a snapshot written on request
+    assert s.device_array("b_ijK_storage")
+    assert s.device_array("R_ijKe")
+    s.close()

@@ -524,17 +524,6 @@ __global__ void __launch_bounds__(WG) k_fill_R(SchemeView s, double *R_ijKe, dou
     for (int j = 0; j < N; j++) R_iK[(long)j * C + K] = rec[Rec::RK_OFF + j];
 }
 
-// a[i] for a runtime i without dynamic register indexing (which would move the
-// array to scratch memory): unrolled selects
-template <int M>
-__device__ __forceinline__ double pick(const double (&a)[M], int i)
-{
-    double v = a[0];
-#pragma unroll
-    for (int q = 1; q < M; q++) v = (q == i) ? a[q] : v;
-    return v;
-}
-
 // ---------------------------------------------------------------------------
 // Face-parallel hybridised assembly (Solver.hpp:933-959).
 //

@@ -370,6 +370,10 @@ class Solver:
     def synchronize(self):
         self._check(lib().mhfem_synchronize(self.ctx))
 
+    def device_array(self, name):
+        """device address of an array (mhfem_device_array), or None"""
+        return lib().mhfem_device_array(self.ctx, ARRAY_ID[name] if isinstance(name, str) else name)
+
     def set_timing(self, on=True):
         """False/0: off; True/1: phase timers; 2: phase timers + sampled
         SpMV launch timing (timers()['spmv_ms'] / ['spmv_timed'])"""
