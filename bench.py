@@ -15,7 +15,7 @@ Forms, each run from the same initial state on a fresh solver:
   chosen     -- learned initial guess (per-face MLP trained online during the
                 first steps, combined with the last 16 increments by least
                 residual) + FP32 BiCGStab whose residual is replaced by the
-                FP64 one every 10x drop (FP64 iterative refinement with
+                FP64 one every 33x drop (FP64 iterative refinement with
                 reliable updates), Jacobi (the identity on the unit-diagonal
                 rows).  `value` is its steps/s.
   plain      -- FP64 BiCGStab + Jacobi, previous solution as initial guess:
@@ -323,7 +323,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=64, help="cubes per edge")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--delta", type=float, default=0.1, help="FP32 residual drop per FP64 replacement")
+    ap.add_argument("--delta", type=float, default=0.03, help="FP32 residual drop per FP64 replacement")
     ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--history-depth", type=int, default=16, help="previous increments in the learned guess")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default rows per work-group")
