@@ -358,8 +358,9 @@ def main():
         return 0
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it "
-                                   "(DESIGN.md describes how the path would shard)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU; a run on several GPUs is not built (it would "
+                                   "partition the cells with one ghost layer and exchange face halos and "
+                                   "scalar reductions between ranks)", "n_gpus": args.gpus}))
         return 2
 
     import numpy as np
