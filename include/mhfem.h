@@ -46,7 +46,8 @@ enum {
     MHFEM_E_MESH = -3,
     MHFEM_E_STATE = -4,
     MHFEM_E_NOT_CONVERGED = -5,
-    MHFEM_E_BREAKDOWN = -6
+    MHFEM_E_BREAKDOWN = -6,
+    MHFEM_E_FILE = -7
 };
 
 /* MassLumping (MassMatrix.h:16-19), AdvectionDiscretization (BaseModel.h:12-16) */
@@ -260,6 +261,51 @@ int mhfem_guess_set_weights(mhfem_ctx *ctx, const float *params, int count);
 int mhfem_guess_forward(mhfem_ctx *ctx, const uint16_t *features, long rows, float *y);
 int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *targets, long rows,
                          float *grad);
+
+/* ---- files on either side of the path (host code; no GPU work).  Errors
+ * leave their message for mhfem_last_error(NULL). */
+/* Mesh files of triangles or tetrahedra with their cell data: VTK XML (.vtu:
+ * ascii, base64 binary with or without zlib compression, raw or base64
+ * appended data) and legacy VTK (.vtk: ASCII or BINARY).  Replace
+ * TNL::Meshes::loadMesh (control.h:20-48) and the reader's readCellData for
+ * "InitialCondition[Z<i>]" (Solver.hpp:296-306).  format: "auto" (by the
+ * extension, as control.h:24-31 does), "vtu" or "vtk".  NULL on error. */
+typedef struct mhfem_mesh_file mhfem_mesh_file;
+mhfem_mesh_file *mhfem_mesh_file_open(const char *path, const char *format);
+void mhfem_mesh_file_close(mhfem_mesh_file *f);
+/* dim 2 (triangles) or 3 (tetrahedra); counts of points and cells */
+int mhfem_mesh_file_info(const mhfem_mesh_file *f, int *dim, long *points, long *cells);
+/* points[points*dim], cell_vertices[cells*(dim+1)] in the file's vertex order */
+int mhfem_mesh_file_read(const mhfem_mesh_file *f, double *points, int32_t *cell_vertices);
+/* a cell-data array by name (converted to double, components interleaved);
+ * returns its length (out may be NULL to ask for it) or a negative code */
+long mhfem_mesh_file_cell_data(const mhfem_mesh_file *f, const char *name, double *out, long capacity);
+int mhfem_mesh_file_cell_data_count(const mhfem_mesh_file *f);
+const char *mhfem_mesh_file_cell_data_name(const mhfem_mesh_file *f, int index);
+
+/* VTU writer of snapshots (HeatEquationModel.h:164-231: TNL's VTUWriter,
+ * FileFormat::zlib_compressed): FieldData CYCLE (Int32) and TIME (Float64)
+ * when >= 0, the mesh, and `fields` cell arrays of num_cells doubles. */
+enum { MHFEM_VTK_ASCII = 0, MHFEM_VTK_BINARY = 1, MHFEM_VTK_ZLIB = 2 };
+int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, long cycle, double time, int fields,
+                    const char *const *names, const double *const *values, int format);
+
+/* BoundaryConditionsStorage in TNL's binary object format
+ * (BoundaryConditionsStorage.h:22-72): type "TNL::MHFEM::BoundaryConditionsStorage< double >",
+ * dofSize, then the tags, values and dirichletValues arrays (i*F + E).  Load
+ * with tags, values and dirichlet NULL to learn dof_size first. */
+int mhfem_bc_save(const char *path, int64_t dof_size, const uint8_t *tags, const double *values,
+                  const double *dirichlet);
+int mhfem_bc_load(const char *path, int64_t *dof_size, uint8_t *tags, double *values, double *dirichlet,
+                  int64_t capacity);
+/* a vector as TNL::Containers::Array< double > (Solver.hpp:974-975); load
+ * returns the length (data may be NULL) or a negative code */
+int mhfem_save_vector(const char *path, const double *data, int64_t count);
+long mhfem_load_vector(const char *path, double *data, int64_t capacity);
+/* Solver::saveLinearSystem (Solver.hpp:966-977), used when a solve fails:
+ * <directory>/matrix.mtx (Matrix Market coordinate, 1-based), dof.vec.tnl
+ * (the current Z_iF) and rhs.vec.tnl */
+int mhfem_save_linear_system(mhfem_ctx *ctx, const char *directory);
 
 /* ---- instrumentation (Solver::writeEpilog, Solver.hpp:1326-1391) */
 /* average duration (ms) of the solver's SpMV (y = A x, current matrix) over

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../mhfem.h"
+#include "Mesh.hpp"
 #include "ParameterContainer.hpp"
 
 namespace mhfem {
@@ -72,12 +73,18 @@ public:
     int cells() const { return cells_; }
     int faces() const { return faces_; }
 
+    // the mesh and the output directory, for makeSnapshot
+    const SimplexMesh &getMesh() const { return *mesh_; }
+    const std::string &outputDirectory() const { return outputDirectory_; }
+
     // set by the Solver once the context exists
-    void attach(mhfem_ctx *ctx)
+    void attach(mhfem_ctx *ctx, const SimplexMesh *mesh)
     {
         ctx_ = ctx;
+        mesh_ = mesh;
         mhfem_get_counts(ctx, &cells_, &faces_, nullptr);
     }
+    void setOutputDirectory(const std::string &dir) { outputDirectory_ = dir; }
 
 protected:
     void check(int rc) const
@@ -85,6 +92,8 @@ protected:
         if (rc != MHFEM_OK) throw std::runtime_error(mhfem_last_error(ctx_));
     }
     mhfem_ctx *ctx_ = nullptr;
+    const SimplexMesh *mesh_ = nullptr;
+    std::string outputDirectory_;
     int cells_ = 0, faces_ = 0;
 };
 

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <iomanip>
+#include <iostream>
 #include <ostream>
 #include <stdexcept>
 #include <string>
@@ -14,33 +15,10 @@
 #include "../mhfem.h"
 #include "BaseModel.hpp"
 #include "BoundaryModels.hpp"
+#include "Mesh.hpp"
 #include "ParameterContainer.hpp"
 
 namespace mhfem {
-
-// simplex mesh: points (d per vertex) and cells (d+1 vertices; local face e is
-// opposite local vertex e)
-struct SimplexMesh {
-    int dim = 2;
-    std::vector<double> points;
-    std::vector<int32_t> cells;
-    int numPoints() const { return (int)(points.size() / dim); }
-    int numCells() const { return (int)(cells.size() / (dim + 1)); }
-};
-
-// BoundaryConditionsStorage (BoundaryConditionsStorage.h:9-15): arrays of n*F in the i*F+E layout
-struct BoundaryConditionsStorage {
-    std::int64_t dofSize = 0;
-    std::vector<uint8_t> tags;
-    std::vector<double> values, dirichletValues;
-};
-
-enum class BoundaryConditionsType : uint8_t {  // BoundaryConditionsType.h:7-31
-    FixedValue = 0,
-    FixedFlux = 1,
-    AdvectiveOutflow = 2,
-    FixedFluxNoAdvection = 3
-};
 
 template <typename Model, typename BoundaryModel = BoundaryModels::Stationary>
 class Solver {
@@ -56,14 +34,15 @@ public:
     // Solver::setMesh (Solver.hpp:26-81)
     void setMesh(const SimplexMesh &mesh)
     {
-        mhfem_mesh_desc md{mesh.dim, mesh.numPoints(), mesh.numCells(), mesh.points.data(), mesh.cells.data()};
+        mesh_ = mesh;
+        const mhfem_mesh_desc md = mesh_.desc();
         mhfem_model_desc mo{n, Model::lumping == MassLumping::enabled ? 1 : 0, (int)Model::Advection,
                             Model::do_mobility_upwind ? 1 : 0};
         if (ctx_) mhfem_destroy(ctx_);
         ctx_ = mhfem_create(&md, &mo);
         if (!ctx_) throw std::runtime_error(std::string("setMesh: ") + mhfem_last_error(nullptr));
         mhfem_get_counts(ctx_, &cells_, &faces_, nullptr);
-        model_.attach(ctx_);
+        model_.attach(ctx_, &mesh_);
     }
 
     // Solver::setup (Solver.hpp:83-253): solver options by name (configSetup.h:31-62)
@@ -98,6 +77,9 @@ public:
         guess_.sample_stride = p.getParameter<int>("guess-sample-stride", 1);
         guess_.reject_after = p.getParameter<int>("guess-reject-after", 3);
         guess_.history_depth = p.getParameter<int>("guess-history-depth", 8);
+        // Solver.hpp:97: snapshots and the dump of a failed solve go there
+        outputDirectory_ = p.getParameter<std::string>("output-directory", std::string());
+        model_.setOutputDirectory(outputDirectory_);
         allIterations_ = allTimeSteps_ = 0;
         return true;
     }
@@ -105,12 +87,33 @@ public:
     // boundary conditions before setInitialCondition (BoundaryConditions::init, BoundaryConditions.hpp:272-281)
     void setBoundaryConditions(const BoundaryConditionsStorage &s) { bc_ = s; }
 
-    // Solver::setInitialCondition (Solver.hpp:255-344).  Initial cell values
-    // are set beforehand through the model (setInitialCondition(i, vector));
-    // without them Z_iK = 0 (Solver.hpp:307-309).
+    // Solver::setInitialCondition (Solver.hpp:255-344).  The boundary
+    // conditions come from "boundary-conditions-file" (TNL's binary format)
+    // or from setBoundaryConditions, else zero Dirichlet everywhere; the
+    // initial cell values from the "InitialCondition[Z<i>]" cell arrays of
+    // "initial-condition", or from the model's setInitialCondition(i, vector)
+    // called beforehand, else Z_iK = 0 (Solver.hpp:307-309).
     bool setInitialCondition(const ParameterContainer &p)
     {
         const long dofs = (long)n * faces_;
+        if (p.checkParameter("boundary-conditions-file")) {
+            const std::string file = p.getParameter<std::string>("boundary-conditions-file");
+            BoundaryConditionsStorage storage;
+            storage.load(file);
+            if (storage.dofSize != dofs) {
+                errorMessage_ = "Wrong dofSize in BoundaryConditionsStorage loaded from file " + file +
+                                ". Expected " + std::to_string(dofs) + " elements, got " +
+                                std::to_string(storage.dofSize) + ".";
+                std::cerr << errorMessage_ << std::endl;
+                return false;
+            }
+            bc_ = std::move(storage);
+        }
+        if (p.checkParameter("initial-condition")) {
+            const MeshReader reader(p.getParameter<std::string>("initial-condition"));
+            for (int i = 0; i < n; i++)
+                model_.setInitialCondition(i, reader.readCellData("InitialCondition[Z" + std::to_string(i) + "]"));
+        }
         if (bc_.dofSize == 0) {
             // default: zero Dirichlet everywhere (Solver.hpp:281-294)
             bc_.dofSize = dofs;
@@ -165,6 +168,14 @@ public:
         allTimeSteps_++;
         allIterations_ += st.iterations;
         last_ = st;
+        if (rc == MHFEM_E_NOT_CONVERGED && !outputDirectory_.empty()) {
+            // save the linear system for debugging, then throw (Solver.hpp:1259-1264)
+            const std::string msg = mhfem_last_error(ctx_);
+            if (mhfem_save_linear_system(ctx_, outputDirectory_.c_str()) == MHFEM_OK)
+                std::cerr << "The linear system has been saved to " << outputDirectory_
+                          << "/{matrix.mtx,dof.vec.tnl,rhs.vec.tnl}" << std::endl;
+            throw std::runtime_error(msg);
+        }
         if (rc != MHFEM_OK) throw std::runtime_error(mhfem_last_error(ctx_));
     }
     void postIterate(double time, double tau)
@@ -197,6 +208,8 @@ public:
     // getters (Solver.h:76-94)
     Model &getMeshDependentData() { return model_; }
     const BoundaryConditionsStorage &getBoundaryConditions() const { return bc_; }
+    const SimplexMesh &getMesh() const { return mesh_; }
+    const std::string &getOutputDirectory() const { return outputDirectory_; }
     long getDofs() const { return (long)n * faces_; }
     long getLocalDofs() const { return getDofs(); }
     long getGlobalDofs() const { return getDofs(); }
@@ -211,6 +224,8 @@ private:
         if (rc != MHFEM_OK) throw std::runtime_error(mhfem_last_error(ctx_));
     }
     mhfem_ctx *ctx_ = nullptr;
+    SimplexMesh mesh_;
+    std::string outputDirectory_;
     Model model_;
     BoundaryConditionsStorage bc_;
     mhfem_solver_config cfg_{};

@@ -20,6 +20,16 @@ void init(Problem &problem, const SimplexMesh &mesh, const ParameterContainer &p
     problem.setupLinearSystem();
 }
 
+// control.h::init (control.h:15-62): the mesh from the "mesh" file
+// ("mesh-format" auto, vtu or vtk; a .pvtu distributed mesh is not supported)
+template <typename Problem>
+void init(Problem &problem, const ParameterContainer &parameters)
+{
+    const SimplexMesh mesh = loadMesh(parameters.getParameter<std::string>("mesh"),
+                                      parameters.getParameter<std::string>("mesh-format", std::string("auto")));
+    init(problem, mesh, parameters);
+}
+
 // control.h::solve(problem, startTime, stopTime, timeStep) (control.h:64-102)
 template <typename Problem>
 void solve(Problem &problem, double startTime, double stopTime, double timeStep)

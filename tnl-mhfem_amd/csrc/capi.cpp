@@ -231,6 +231,9 @@ void phase_end(Ctx &c, int slot, double &acc)
 
 }  // namespace
 
+// message of mhfem_last_error(NULL) for calls without a context (io.cpp)
+void mhfem::set_global_error(const std::string &msg) { g_create_error = msg; }
+
 namespace mhfem {
 
 long Ctx::array_size(int which) const

@@ -21,6 +21,9 @@ struct Error {
     std::string msg;
 };
 
+// sets the message mhfem_last_error(NULL) returns (calls without a context)
+void set_global_error(const std::string &msg);
+
 #define MHFEM_HIP(call)                                                                    \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

@@ -5,9 +5,17 @@
 //
 //   heat_equation [out.bin] [s] [steps]
 // writes the final cell values Z_iK (float64) to out.bin when given.
+//
+//   heat_equation --config config.ini
+// runs the reference's file-driven example instead: the mesh (.vtk or .vtu),
+// "boundary-conditions-file" (TNL binary), "initial-condition" (.vtu with
+// InitialCondition[Z0]) and the solver options come from the INI file (the
+// format of examples/HeatEquation/config.ini), and every snapshot is written
+// to <output-directory>/data_<step>.vtu with the cell array "Z".
 #include <cmath>
 #include <cstdio>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "mhfem/control.hpp"
@@ -33,12 +41,19 @@ public:
         setCoefficient(MHFEM_f_iK, 0.0);
         return true;
     }
+    // HeatEquationModel::makeSnapshot (HeatEquationModel.h:164-231): data_<step>.vtu
+    // (zlib-compressed VTU) with the cell values as "Z"
     void makeSnapshot(double time, long step, const std::vector<double> &Z) const override
     {
         double mx = 0;
         for (double z : Z) mx = std::max(mx, z);
         std::printf("snapshot %ld  t = %.4e  max Z = %.10e\n", step, time, mx);
         const_cast<HeatEquationModel *>(this)->lastSnapshot = Z;
+        if (!outputDirectory().empty()) {
+            char name[32];
+            std::snprintf(name, sizeof name, "/data_%05ld.vtu", step);
+            writeVTU(outputDirectory() + name, getMesh(), step, time, {{"Z", &Z}});
+        }
     }
 };
 
@@ -60,8 +75,27 @@ static SimplexMesh unit_square(int s)
     return m;
 }
 
+// examples/HeatEquation driven by its INI file (control.h::init, solve, writeEpilog)
+static int run_config(const char *config)
+{
+    try {
+        ParameterContainer p;
+        p.parseINI(config);
+        Solver<HeatEquationModel> problem;
+        init(problem, p);
+        problem.enableTimers();
+        solve(problem, p);
+        problem.writeEpilog(std::cout);
+    } catch (const std::exception &e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 3 && std::string(argv[1]) == "--config") return run_config(argv[2]);
     const int s = argc > 2 ? std::atoi(argv[2]) : 64;
     const int steps = argc > 3 ? std::atoi(argv[3]) : 100;
     ParameterContainer p;

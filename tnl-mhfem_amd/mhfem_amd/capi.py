@@ -113,6 +113,20 @@ SIGNATURES = {
     "mhfem_reset_timers": (_I, [_P]),
     "mhfem_set_timing": (_I, [_P, _I]),
     "mhfem_synchronize": (_I, [_P]),
+    "mhfem_mesh_file_open": (_P, [C.c_char_p, C.c_char_p]),
+    "mhfem_mesh_file_close": (None, [_P]),
+    "mhfem_mesh_file_info": (_I, [_P, C.POINTER(_I), C.POINTER(_L), C.POINTER(_L)]),
+    "mhfem_mesh_file_read": (_I, [_P, _P, _P]),
+    "mhfem_mesh_file_cell_data": (_L, [_P, C.c_char_p, _P, _L]),
+    "mhfem_mesh_file_cell_data_count": (_I, [_P]),
+    "mhfem_mesh_file_cell_data_name": (C.c_char_p, [_P, _I]),
+    "mhfem_write_vtu": (_I, [C.c_char_p, C.POINTER(MeshDesc), _L, _D, _I, C.POINTER(C.c_char_p),
+                             C.POINTER(C.c_void_p), _I]),
+    "mhfem_bc_save": (_I, [C.c_char_p, C.c_int64, _P, _P, _P]),
+    "mhfem_bc_load": (_I, [C.c_char_p, C.POINTER(C.c_int64), _P, _P, _P, C.c_int64]),
+    "mhfem_save_vector": (_I, [C.c_char_p, _P, C.c_int64]),
+    "mhfem_load_vector": (_L, [C.c_char_p, _P, C.c_int64]),
+    "mhfem_save_linear_system": (_I, [_P, C.c_char_p]),
 }
 
 _lib = None
@@ -143,6 +157,117 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _global_check(rc):
+    if rc < 0:
+        raise MHFEMError(rc, lib().mhfem_last_error(None).decode())
+    return rc
+
+
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+# ---- files on either side of the path (host code in the library, no GPU work)
+VTK_FORMATS = {"ascii": 0, "binary": 1, "zlib": 2}
+
+
+class MeshFile:
+    """A mesh file of triangles or tetrahedra with its cell data, read by the
+    library: .vtu (VTK XML) or .vtk (legacy VTK).  Stands for
+    TNL::Meshes::loadMesh (control.h:20-48) and the reader's readCellData
+    (Solver.hpp:296-306)."""
+
+    def __init__(self, path, format="auto"):
+        L = lib()
+        h = L.mhfem_mesh_file_open(_path(path), format.encode())
+        if not h:
+            raise MHFEMError(-7, L.mhfem_last_error(None).decode())
+        try:
+            d, npts, nc = C.c_int(), C.c_long(), C.c_long()
+            _global_check(L.mhfem_mesh_file_info(h, C.byref(d), C.byref(npts), C.byref(nc)))
+            self.dim = d.value
+            self.points = np.empty((npts.value, self.dim), np.float64)
+            self.cells = np.empty((nc.value, self.dim + 1), np.int32)
+            _global_check(L.mhfem_mesh_file_read(h, _ptr(self.points), _ptr(self.cells)))
+            self._data = {}
+            for k in range(L.mhfem_mesh_file_cell_data_count(h)):
+                name = L.mhfem_mesh_file_cell_data_name(h, k)
+                size = _global_check(L.mhfem_mesh_file_cell_data(h, name, None, 0))
+                v = np.empty(size, np.float64)
+                _global_check(L.mhfem_mesh_file_cell_data(h, name, _ptr(v), size))
+                self._data[name.decode()] = v
+        finally:
+            L.mhfem_mesh_file_close(h)
+
+    def cell_data_names(self):
+        return list(self._data)
+
+    def cell_data(self, name):
+        if name not in self._data:
+            raise KeyError(f"cell data array '{name}' was not found in the mesh file")
+        return self._data[name]
+
+
+def write_vtu(path, points, cells, cell_data=None, cycle=-1, time=-1.0, format="zlib"):
+    """VTU file of a simplex mesh with float64 cell arrays (the snapshot
+    writer of HeatEquationModel.h:164-231)."""
+    pts = np.ascontiguousarray(points, np.float64)
+    cel = np.ascontiguousarray(cells, np.int32)
+    mesh = MeshDesc(pts.shape[1], pts.shape[0], cel.shape[0], _ptr(pts), _ptr(cel))
+    cell_data = cell_data or {}
+    arrays = [np.ascontiguousarray(v, np.float64).ravel() for v in cell_data.values()]
+    for name, v in zip(cell_data, arrays):
+        if v.size != cel.shape[0]:
+            raise ValueError(f"cell data '{name}' has {v.size} values for {cel.shape[0]} cells")
+    names = (C.c_char_p * max(1, len(arrays)))(*[k.encode() for k in cell_data])
+    vals = (C.c_void_p * max(1, len(arrays)))(*[v.ctypes.data for v in arrays])
+    _global_check(lib().mhfem_write_vtu(_path(path), C.byref(mesh), int(cycle), float(time), len(arrays),
+                                        names, vals, VTK_FORMATS[format]))
+
+
+class BoundaryConditionsStorage:
+    """TNL::MHFEM::BoundaryConditionsStorage (BoundaryConditionsStorage.h:9-72):
+    ``tags``, ``values`` and ``dirichletValues`` of length ``dofSize`` in the
+    i*F+E layout, saved and loaded in TNL's binary object format."""
+
+    def __init__(self, dofSize=0):
+        self.dofSize = int(dofSize)
+        self.tags = np.zeros(self.dofSize, np.uint8)
+        self.values = np.zeros(self.dofSize, np.float64)
+        self.dirichletValues = np.zeros(self.dofSize, np.float64)
+
+    def save(self, path):
+        t = np.ascontiguousarray(self.tags, np.uint8)
+        v = np.ascontiguousarray(self.values, np.float64)
+        d = np.ascontiguousarray(self.dirichletValues, np.float64)
+        if not (t.size == v.size == d.size == self.dofSize):
+            raise ValueError("tags, values and dirichletValues must have dofSize elements")
+        _global_check(lib().mhfem_bc_save(_path(path), self.dofSize, _ptr(t), _ptr(v), _ptr(d)))
+
+    @classmethod
+    def load(cls, path):
+        L = lib()
+        n = C.c_int64()
+        _global_check(L.mhfem_bc_load(_path(path), C.byref(n), None, None, None, 0))
+        s = cls(n.value)
+        _global_check(L.mhfem_bc_load(_path(path), C.byref(n), _ptr(s.tags), _ptr(s.values),
+                                      _ptr(s.dirichletValues), s.dofSize))
+        return s
+
+
+def save_vector(path, x):
+    """A float64 vector as TNL::Containers::Array< double >."""
+    v = np.ascontiguousarray(x, np.float64).ravel()
+    _global_check(lib().mhfem_save_vector(_path(path), _ptr(v), v.size))
+
+
+def load_vector(path):
+    n = _global_check(lib().mhfem_load_vector(_path(path), None, 0))
+    v = np.empty(n, np.float64)
+    _global_check(lib().mhfem_load_vector(_path(path), _ptr(v), n))
+    return v
+
+
 class Solver:
     """Host mirror of TNL::MHFEM::Solver over the C ABI.
 
@@ -154,9 +279,11 @@ class Solver:
 
     def __init__(self, problem, solver=None, preconditioner=None, tol=None, precision="fp64",
                  inner_tol=1e-4, check_every=8, max_iterations=100000, gmres_restart=30,
-                 setup=True, rows_per_block=0):
+                 setup=True, rows_per_block=0, output_directory=None):
         pr = problem
         self.problem = pr
+        # where a failed solve leaves its linear system (the reference's output-directory)
+        self.output_directory = output_directory
         model = ModelDesc(pr.n, pr.lumping, pr.advection, pr.do_mobility_upwind)
         L = lib()
         instances = pr.extra.get("instances") if pr.extra else None
@@ -238,6 +365,11 @@ class Solver:
     def setupLinearSystem(self):
         self._check(lib().mhfem_setup_linear_system(self.ctx, C.byref(self.config)))
 
+    def saveLinearSystem(self, directory):
+        """Solver::saveLinearSystem (Solver.hpp:966-977): matrix.mtx, dof.vec.tnl
+        and rhs.vec.tnl in ``directory``."""
+        _global_check(lib().mhfem_save_linear_system(self.ctx, _path(directory)))
+
     # ---- one time step (control.h:83-98)
     def preIterate(self, t, tau):
         self._check(lib().mhfem_pre_iterate(self.ctx, t, tau))
@@ -249,9 +381,15 @@ class Solver:
 
     def solveLinearSystem(self):
         st = SolveStats()
-        self._check(lib().mhfem_solve(self.ctx, C.byref(st)))
+        rc = lib().mhfem_solve(self.ctx, C.byref(st))
         self.iterations.append(st.iterations)
         self.last_stats = st
+        if rc == -5 and self.output_directory:
+            # the reference saves the system before it throws (Solver.hpp:1259-1264)
+            msg = lib().mhfem_last_error(self.ctx).decode()
+            self.saveLinearSystem(self.output_directory)
+            raise MHFEMError(rc, msg)
+        self._check(rc)
         return st
 
     def postIterate(self, t, tau):

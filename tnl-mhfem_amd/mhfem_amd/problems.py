@@ -247,6 +247,53 @@ def adr_2d(s: int = 16, n: int = 2, dt: float = 1e-3, steps: int = 3,
                    bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc)
 
 
+def from_files(mesh: str, boundary_conditions_file: Optional[str] = None,
+               initial_condition: Optional[str] = None, n: int = 1, coeffs: Optional[dict] = None,
+               dt: float = 1e-3, steps: int = 1, solver: str = "bicgstab",
+               preconditioner: str = "diagonal", tol: float = 1e-10, lumping: int = 0,
+               advection: int = EXPLICIT_UPWIND, do_mobility_upwind: int = 1,
+               mesh_format: str = "auto") -> Problem:
+    """A problem from the reference's input files, as control.h::init and
+    Solver::setInitialCondition read them (control.h:20-48,
+    Solver.hpp:262-309): the mesh (.vtu or .vtk), the boundary conditions
+    (BoundaryConditionsStorage in TNL's format; zero Dirichlet everywhere when
+    absent) and the "InitialCondition[Z<i>]" cell arrays (Z_iK = 0 when
+    absent).  ``coeffs`` are the model's coefficients (default: the heat
+    model with unit diffusivity)."""
+    from . import capi
+
+    mf = capi.MeshFile(mesh, mesh_format)
+    points, cells = mf.points, mf.cells
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    dofs = n * F
+    if boundary_conditions_file is not None:
+        bcs = capi.BoundaryConditionsStorage.load(boundary_conditions_file)
+        if bcs.dofSize != dofs:
+            raise ValueError(f"Wrong dofSize in BoundaryConditionsStorage loaded from file "
+                             f"{boundary_conditions_file}. Expected {dofs} elements, got {bcs.dofSize}.")
+        tags, values, dirichlet = bcs.tags, bcs.values, bcs.dirichletValues
+    else:
+        tags = np.full(dofs, FIXED_VALUE, np.uint8)
+        values, dirichlet = np.zeros(dofs), np.zeros(dofs)
+    if initial_condition is not None:
+        ic = capi.MeshFile(initial_condition)
+        if ic.cells.shape[0] != C:
+            raise ValueError(f"{initial_condition}: {ic.cells.shape[0]} cells, the mesh has {C}")
+        Z0 = np.concatenate([ic.cell_data(f"InitialCondition[Z{i}]") for i in range(n)])
+    else:
+        Z0 = np.zeros(n * C)
+    if coeffs is None:
+        coeffs = dict(N_ijK=np.array(1.0), u_ijKe=np.array(0.0), m_iK=np.array(1.0),
+                      D_ijK=np.array(1.0), w_iKe=np.array(0.0), a_ijKe=np.array(0.0),
+                      r_ijK=np.array(0.0), f_iK=np.array(0.0))
+    return Problem("from_files", points, cells, n, coeffs, tags=tags, bc_values=values,
+                   bc_dirichlet=dirichlet, Z_iK0=Z0, dt=dt, steps=steps, solver=solver,
+                   preconditioner=preconditioner, tol=tol, lumping=lumping, advection=advection,
+                   do_mobility_upwind=do_mobility_upwind, bc_mobility=np.ones(dofs),
+                   cell_faces=cf, face_cells=fc)
+
+
 def ensemble_sizes(seed: int, count: int = 96, lo: int = 32, hi: int = 256):
     rng = np.random.default_rng(seed)
     return rng.integers(lo, hi + 1, size=count)
