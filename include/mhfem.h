@@ -137,6 +137,15 @@ typedef struct {
 
 /* ---- lifetime (Solver::setMesh, Solver.hpp:26-81; BaseModel::allocate, BaseModel.h:168-195) */
 mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *model);
+/* Any cell shape of the reference's examples/config.h:16-20 but the
+ * hexahedron: vertices_per_cell 2 (edges, dim 1), 3 (triangles), 4 with dim 2
+ * (quadrangles: axis-aligned rectangles, vertices counter-clockwise from the
+ * lower-left one as VTK orders them, MassMatrix.h:146-303) or 4 with dim 3
+ * (tetrahedra).  cell_vertices then holds num_cells*vertices_per_cell entries;
+ * b_ijK_storage has MassMatrix::size values per (i,j,K): 9 or 14 for
+ * simplices, 1 for edges, 2 for rectangles.  mhfem_create(mesh, model) is
+ * mhfem_create_cells(mesh, mesh->dim + 1, model). */
+mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell, const mhfem_model_desc *model);
 void mhfem_destroy(mhfem_ctx *ctx);
 const char *mhfem_last_error(const mhfem_ctx *ctx); /* ctx may be NULL: error of the last create */
 

@@ -21,6 +21,7 @@
 
 struct or_problem {
     int dim, fpc, n, S;
+    int shape, nvc; /* OR_SIMPLEX or OR_BOX; vertices per cell */
     int nv, nc, nf;
     int lumping, advection, do_mob;
     double *pts;
@@ -58,15 +59,39 @@ struct or_problem {
 #define BST(i, j, K, s) p->b[(((long)(i) * NN + (j)) * p->S + (s)) * C_ + (K)]
 #define BCI(i, E) ((long)(i) * F_ + (E))
 
-/* MassMatrix::b_ijKef -- packed upper triangle (MassMatrix.h:613-616, 884-887) */
+/* Direction of local face e of a box cell, which selects its storage slot:
+ * edges one slot; quadrangles e = 0, 2 (horizontal) and e = 1, 3 (vertical);
+ * hexahedra n_x faces e = 2, 4, n_y faces e = 1, 3, n_z faces e = 0, 5
+ * (MassMatrix.h:178-184, 300-306, 430-436). */
+static inline int box_dir(int dim, int e)
+{
+    static const int hex[6] = {2, 1, 0, 1, 0, 2};
+    return dim == 1 ? 0 : dim == 2 ? e % 2 : hex[e];
+}
+
+/* MassMatrix::b_ijKef -- simplices: packed upper triangle (MassMatrix.h:613-616,
+ * 884-887); boxes: diagonal (lumped) or 2b on the diagonal and b between the
+ * two faces of one direction (MassMatrix.h:57-71, 118-132, 190-205, 250-276,
+ * 384-399, 444-481) */
 static inline double b_ef(const or_problem *p, int i, int j, int K, int e, int f)
 {
+    if (p->shape == OR_BOX) {
+        const int de = box_dir(p->dim, e), df = box_dir(p->dim, f);
+        if (p->lumping) return e == f ? BST(i, j, K, de) : 0.0;
+        if (de != df) return 0.0;
+        return e == f ? 2 * BST(i, j, K, de) : BST(i, j, K, de);
+    }
     if (e > f) { int t = e; e = f; f = t; }
     return BST(i, j, K, e + (f * (f + 1)) / 2);
 }
-/* MassMatrix::b_ijKe -- row sums (MassMatrix.h:630, 901) */
+/* MassMatrix::b_ijKe -- simplices: row sums (MassMatrix.h:630, 901); boxes:
+ * b (lumped) or 3b (MassMatrix.h:73-84, 134-145, 207-221, 278-292, 401-417, 483-499) */
 static inline double b_e(const or_problem *p, int i, int j, int K, int e)
 {
+    if (p->shape == OR_BOX) {
+        const double b = BST(i, j, K, box_dir(p->dim, e));
+        return p->lumping ? b : 3 * b;
+    }
     return BST(i, j, K, p->S - p->fpc + e);
 }
 
@@ -88,11 +113,42 @@ static unsigned long hash3(const int *k, int d)
     return h ^ (h >> 29);
 }
 
+/* Vertices (positions within the cell) of local face e.  Simplices: every
+ * vertex but e.  Boxes, in TNL's subentity order of its topologies: an edge's
+ * faces are its vertices; a quadrangle's edges (0,1), (1,2), (2,3), (3,0); a
+ * hexahedron's faces (0,1,2,3), (0,1,5,4), (1,2,6,5), (2,3,7,6), (3,0,4,7),
+ * (4,5,6,7) (vertices in VTK order).  Returns the count. */
+static int face_vertices(int dim, int shape, int e, int *lv)
+{
+    static const int quad[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    static const int hex[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                  {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
+    if (shape == OR_SIMPLEX && dim > 1) {
+        int t = 0;
+        for (int v = 0; v <= dim; v++)
+            if (v != e) lv[t++] = v;
+        return t;
+    }
+    if (dim == 1) { lv[0] = e; return 1; }
+    if (dim == 2) { lv[0] = quad[e][0]; lv[1] = quad[e][1]; return 2; }
+    for (int t = 0; t < 4; t++) lv[t] = hex[e][t];
+    return 4;
+}
+
 int or_topology(int dim, int nv, const double *points, int nc, const int *cv, int *cell_faces,
                 int *face_cells)
 {
+    return or_topology_shape(dim, dim == 1 ? OR_BOX : OR_SIMPLEX, nv, points, nc, cv, cell_faces,
+                             face_cells);
+}
+
+int or_topology_shape(int dim, int shape, int nv, const double *points, int nc, const int *cv,
+                      int *cell_faces, int *face_cells)
+{
     (void)points;
-    const int fpc = dim + 1, d = dim;
+    const int box = shape == OR_BOX || dim == 1;
+    const int fpc = box ? 2 * dim : dim + 1, nvc = box ? 1 << dim : dim + 1;
+    const int d = box ? 1 << (dim - 1) : dim; /* vertices per face */
     const long maxf = (long)nc * fpc;
     long cap = 1;
     while (cap < 2 * maxf) cap <<= 1;
@@ -103,10 +159,10 @@ int or_topology(int dim, int nv, const double *points, int nc, const int *cv, in
     int nf = 0, bad = 0;
     for (int K = 0; K < nc && !bad; K++) {
         for (int e = 0; e < fpc; e++) {
-            int key[3], t = 0;
-            for (int v = 0; v < fpc; v++)
-                if (v != e) key[t++] = cv[(long)K * fpc + v];
-            /* sort the seed (insertion sort, d <= 3) */
+            int key[4], lv[4];
+            face_vertices(dim, box ? OR_BOX : OR_SIMPLEX, e, lv);
+            for (int t = 0; t < d; t++) key[t] = cv[(long)K * nvc + lv[t]];
+            /* sort the seed (insertion sort, d <= 4) */
             for (int a = 1; a < d; a++)
                 for (int c = a; c > 0 && key[c - 1] > key[c]; c--) {
                     int s = key[c]; key[c] = key[c - 1]; key[c - 1] = s;
@@ -140,6 +196,19 @@ int or_topology(int dim, int nv, const double *points, int nc, const int *cv, in
     return bad ? -1 : nf;
 }
 
+/* cell measure (TNL's getEntityMeasure): simplices by the determinant;
+ * an edge its length; a quadrangle half the cross product of its diagonals;
+ * a hexahedron (a voxel) the product of its edge lengths along the axes */
+static double box_measure(int dim, const double *v /* 2^dim points of dim coords */)
+{
+    if (dim == 1) return fabs(v[1] - v[0]);
+    if (dim == 2) {
+        const double d1x = v[4] - v[0], d1y = v[5] - v[1], d2x = v[6] - v[2], d2y = v[7] - v[3];
+        return 0.5 * fabs(d1x * d2y - d1y * d2x);
+    }
+    return fabs((v[18] - v[0]) * (v[19] - v[1]) * (v[20] - v[2]));
+}
+
 static double simplex_measure(int dim, const double *v /* (dim+1)*dim */)
 {
     if (dim == 2) {
@@ -159,6 +228,7 @@ static double simplex_measure(int dim, const double *v /* (dim+1)*dim */)
 
 static double face_measure(int dim, const double *v /* dim points of dim coords */)
 {
+    if (dim == 1) return 1.0; /* a point: the flux through it is not scaled */
     if (dim == 2) {
         double dx = v[2] - v[0], dy = v[3] - v[1];
         return sqrt(dx * dx + dy * dy);
@@ -300,11 +370,24 @@ void or_local_mass_inverse(int dim, int lumping, const double *P, double measure
 or_problem *or_create(int dim, int nv, const double *points, int nc, const int *cv, int n,
                       int lumping, int advection, int do_mob)
 {
-    if (dim != 2 && dim != 3) return NULL;
+    return or_create_shape(dim, dim == 1 ? OR_BOX : OR_SIMPLEX, nv, points, nc, cv, n, lumping,
+                           advection, do_mob);
+}
+
+/* Cells: simplices (triangles, tetrahedra) or boxes (edges, axis-aligned
+ * rectangles as Quadrangle cells, MassMatrix.h:146-303).  b_ijK_storage holds
+ * MassMatrix::size values per (i,j,K): 9 / 14 for simplices, dim for boxes. */
+or_problem *or_create_shape(int dim, int shape, int nv, const double *points, int nc, const int *cv,
+                            int n, int lumping, int advection, int do_mob)
+{
+    if (dim == 1) shape = OR_BOX;
+    if (shape == OR_SIMPLEX ? (dim != 2 && dim != 3) : (dim != 1 && dim != 2)) return NULL;
     or_problem *p = calloc(1, sizeof(or_problem));
     p->dim = dim;
-    p->fpc = dim + 1;
-    p->S = dim == 2 ? 9 : 14;
+    p->shape = shape;
+    p->fpc = shape == OR_BOX ? 2 * dim : dim + 1;
+    p->nvc = shape == OR_BOX ? 1 << dim : dim + 1;
+    p->S = shape == OR_BOX ? dim : dim == 2 ? 9 : 14;
     p->n = n;
     p->nv = nv;
     p->nc = nc;
@@ -313,11 +396,11 @@ or_problem *or_create(int dim, int nv, const double *points, int nc, const int *
     p->do_mob = do_mob;
     p->pts = malloc(sizeof(double) * nv * dim);
     memcpy(p->pts, points, sizeof(double) * nv * dim);
-    p->cv = malloc(sizeof(int) * (long)nc * p->fpc);
-    memcpy(p->cv, cv, sizeof(int) * (long)nc * p->fpc);
+    p->cv = malloc(sizeof(int) * (long)nc * p->nvc);
+    memcpy(p->cv, cv, sizeof(int) * (long)nc * p->nvc);
     p->cell_faces = malloc(sizeof(int) * (long)nc * p->fpc);
     int *fc = malloc(sizeof(int) * 2L * nc * p->fpc);
-    int nf = or_topology(dim, nv, points, nc, cv, p->cell_faces, fc);
+    int nf = or_topology_shape(dim, shape, nv, points, nc, cv, p->cell_faces, fc);
     if (nf < 0) {
         free(fc);
         or_destroy(p);
@@ -327,21 +410,19 @@ or_problem *or_create(int dim, int nv, const double *points, int nc, const int *
     p->face_cells = realloc(fc, sizeof(int) * 2L * nf);
     p->cell_meas = malloc(sizeof(double) * nc);
     p->face_meas = malloc(sizeof(double) * nf);
-    const int fpc = p->fpc;
+    const int fpc = p->fpc, nvc = p->nvc;
 #pragma omp parallel for
     for (int K = 0; K < nc; K++) {
-        double vv[12];
-        for (int k = 0; k < fpc; k++)
-            for (int c = 0; c < dim; c++) vv[k * dim + c] = points[(long)cv[(long)K * fpc + k] * dim + c];
-        p->cell_meas[K] = simplex_measure(dim, vv);
+        double vv[24];
+        for (int k = 0; k < nvc; k++)
+            for (int c = 0; c < dim; c++) vv[k * dim + c] = points[(long)cv[(long)K * nvc + k] * dim + c];
+        p->cell_meas[K] = shape == OR_BOX ? box_measure(dim, vv) : simplex_measure(dim, vv);
         for (int e = 0; e < fpc; e++) {
-            double fv[9];
-            int t = 0;
-            for (int k = 0; k < fpc; k++)
-                if (k != e) {
-                    for (int c = 0; c < dim; c++) fv[t * dim + c] = vv[k * dim + c];
-                    t++;
-                }
+            double fv[12];
+            int lv[4];
+            const int t = face_vertices(dim, shape, e, lv);
+            for (int k = 0; k < t; k++)
+                for (int c = 0; c < dim; c++) fv[k * dim + c] = vv[lv[k] * dim + c];
             int E = p->cell_faces[(long)K * fpc + e];
             if (p->face_cells[2L * E] == K) p->face_meas[E] = face_measure(dim, fv);
         }
@@ -594,6 +675,27 @@ void or_pre_iterate(or_problem *p, double time, double tau)
 #pragma omp parallel for
     for (int K = 0; K < C; K++) {
         double P[12], out[14];
+        if (p->shape == OR_BOX) {
+            /* MassMatrix.h:46-54, 107-115 (edge: h_x = |K|), :160-176, 231-247
+             * (quadrangle: h from vertices 0 and 2, an axis-aligned rectangle) */
+            const int *cvK = p->cv + (long)K * p->nvc;
+            double hx, hy = 0;
+            if (dim == 1) hx = p->cell_meas[K];
+            else {
+                hx = p->pts[(long)cvK[2] * 2] - p->pts[(long)cvK[0] * 2];
+                hy = p->pts[(long)cvK[2] * 2 + 1] - p->pts[(long)cvK[0] * 2 + 1];
+            }
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++) {
+                    const double D = p->D[IJK(i, j, K)];
+                    if (dim == 1) BST(i, j, K, 0) = 2 * D / hx;
+                    else {
+                        BST(i, j, K, 0) = 2 * D * hx / hy;
+                        BST(i, j, K, 1) = 2 * D * hy / hx;
+                    }
+                }
+            continue;
+        }
         for (int k = 0; k < fpc; k++)
             for (int c = 0; c < dim; c++) P[k * dim + c] = p->pts[(long)p->cv[(long)K * fpc + k] * dim + c];
         for (int i = 0; i < n; i++)

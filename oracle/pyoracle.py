@@ -31,6 +31,8 @@ def lib():
         sig = {
             "or_topology": (I, [I, I, P, I, P, P, P]),
             "or_create": (P, [I, I, P, I, P, I, I, I, I]),
+            "or_topology_shape": (I, [I, I, I, P, I, P, P, P]),
+            "or_create_shape": (P, [I, I, I, P, I, P, I, I, I, I]),
             "or_destroy": (None, [P]),
             "or_error": (C.c_char_p, [P]),
             "or_num_cells": (I, [P]),
@@ -65,14 +67,22 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _shape(dim, vertices_per_cell):
+    """1 (box: edges, quadrangles) or 0 (simplex)."""
+    return 1 if dim == 1 or (dim == 2 and vertices_per_cell == 4) else 0
+
+
 def topology(points, cells):
     points = np.ascontiguousarray(points, np.float64)
     cells = np.ascontiguousarray(cells, np.int32)
-    nc, fpc = cells.shape
+    nc, nvc = cells.shape
+    dim = points.shape[1]
+    box = _shape(dim, nvc)
+    fpc = 2 * dim if box else dim + 1
     cf = np.empty((nc, fpc), np.int32)
     fc = np.empty((nc * fpc, 2), np.int32)
-    nf = lib().or_topology(points.shape[1], points.shape[0], _ptr(points), nc, _ptr(cells),
-                           _ptr(cf), _ptr(fc))
+    nf = lib().or_topology_shape(dim, box, points.shape[0], _ptr(points), nc, _ptr(cells),
+                                 _ptr(cf), _ptr(fc))
     if nf < 0:
         raise ValueError("non-conforming mesh")
     return cf, fc[:nf].copy()
@@ -137,8 +147,10 @@ class Oracle:
         lum = pr.lumping if lumping is None else lumping
         adv = pr.advection if advection is None else advection
         dmu = pr.do_mobility_upwind if do_mobility_upwind is None else do_mobility_upwind
-        self.p = lib().or_create(self.points.shape[1], self.points.shape[0], _ptr(self.points),
-                                 self.cells.shape[0], _ptr(self.cells), pr.n, lum, adv, dmu)
+        dim = self.points.shape[1]
+        self.p = lib().or_create_shape(dim, _shape(dim, self.cells.shape[1]), self.points.shape[0],
+                                       _ptr(self.points), self.cells.shape[0], _ptr(self.cells),
+                                       pr.n, lum, adv, dmu)
         if not self.p:
             raise ValueError("or_create failed")
         self.nc = lib().or_num_cells(self.p)

@@ -45,6 +45,17 @@ def _mixed_tags(pr, seed=5):
     return dataclasses.replace(pr, tags=tags, bc_values=vals, bc_dirichlet=dirv)
 
 
+def _uneven_boxes(pr, seed=7):
+    """The same box mesh with uneven, anisotropic spacing (h_x != h_y)."""
+    rng = np.random.default_rng(seed)
+    pts = np.array(pr.points, copy=True)
+    for a in range(pr.dim):
+        lines = np.unique(pts[:, a])
+        moved = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, lines.size - 2)), [1.0]]) * (1.0 - 0.3 * a)
+        pts[:, a] = moved[np.searchsorted(lines, pts[:, a])]
+    return dataclasses.replace(pr, points=pts)
+
+
 CASES = {
     "heat2d": lambda: problems.heat_2d(s=12, steps=3),
     "heat2d_lumped": lambda: problems.heat_2d(s=12, steps=3, lumping=1),
@@ -64,6 +75,17 @@ CASES = {
         problems.adr_3d(s=3, steps=2, preconditioner="diagonal", n=3), advection=problems.IMPLICIT_UPWIND)),
     "adr2d_n3_implicit_trace": lambda: dataclasses.replace(problems.adr_2d(s=6, n=3),
                                                            advection=problems.IMPLICIT_TRACE),
+    # box cells: quadrangles (axis-aligned rectangles) and edges (MassMatrix.h:28-303)
+    "quad_heat": lambda: problems.heat_quads(s=10, steps=3),
+    "quad_heat_lumped": lambda: _uneven_boxes(problems.heat_quads(s=10, steps=3, lumping=1)),
+    "quad_adr_n2": lambda: _uneven_boxes(problems.adr_quads(s=8, n=2)),
+    "quad_adr_n2_implicit_upwind": lambda: dataclasses.replace(_uneven_boxes(problems.adr_quads(s=6, n=2)),
+                                                               advection=problems.IMPLICIT_UPWIND),
+    "quad_adr_n3_lumped_mixed_tags": lambda: _mixed_tags(_uneven_boxes(problems.adr_quads(s=6, n=3, lumping=1))),
+    "edge_adr_n1": lambda: _uneven_boxes(problems.adr_1d(s=40, n=1)),
+    "edge_adr_n2_lumped": lambda: _uneven_boxes(problems.adr_1d(s=40, n=2, lumping=1)),
+    "edge_adr_n3_mixed_tags_trace": lambda: _mixed_tags(dataclasses.replace(
+        _uneven_boxes(problems.adr_1d(s=30, n=3)), advection=problems.IMPLICIT_TRACE)),
 }
 
 
@@ -217,3 +239,16 @@ def test_nonconvergence_raises():
     s.assembleLinearSystem(0.0, pr.dt)
     with pytest.raises(capi.MHFEMError, match="did not converge"):
         s.solveLinearSystem()
+
+
+def test_unsupported_cells_rejected():
+    """Hexahedra and unknown cell shapes are refused before any device work."""
+    pts = np.zeros((8, 3))
+    cells = np.arange(8, dtype=np.int32)[None, :]
+    mesh = capi.MeshDesc(3, 8, 1, pts.ctypes.data, cells.ctypes.data)
+    model = capi.ModelDesc(1, 0, 0, 1)
+    L = capi.lib()
+    assert not L.mhfem_create_cells(mesh, 8, model)
+    assert b"hexahedra are not supported" in L.mhfem_last_error(None)
+    assert not L.mhfem_create_cells(mesh, 5, model)
+    assert b"not supported" in L.mhfem_last_error(None)

@@ -14,6 +14,7 @@ independent construction:
   * ILU(0): (LU)_ij == A_ij on the pattern; Krylov: scipy's direct solve.
 """
 import dataclasses
+import itertools
 import math
 
 import numpy as np
@@ -156,7 +157,119 @@ def test_mass_matrix_spd_and_unit_triangle():
     assert np.allclose(np.linalg.inv(Minv), M, rtol=1e-12)
 
 
+# ---------------------------------------------------------------- box cells
+def box_mass(P, lumping):
+    """M_ef = int_K w_e . w_f for the RTN0 basis of an axis-aligned box (an
+    edge or a rectangle, vertices in VTK order): w_e = +-(x_a - c) / |K| e_a,
+    with a the axis of face e's normal and c the coordinate of the opposite
+    face, so that w_e . n_e = 1 / |E_e| on face e and 0 on the others.  Exact
+    2-point Gauss quadrature per axis, or vertex (trapezoidal) quadrature when
+    lumped.  Returns (M, |K|)."""
+    d = P.shape[1]
+    lo, hi = P[0], P[2] if d == 2 else P[1]
+    h = hi - lo
+    vol = float(np.prod(np.abs(h)))
+    # local faces: (axis, the face's side: +1 at hi, -1 at lo)
+    faces = [(0, -1), (0, +1)] if d == 1 else [(1, -1), (0, +1), (1, +1), (0, -1)]
+
+    def w(e, x):
+        a, side = faces[e]
+        c = lo[a] if side > 0 else hi[a]
+        out = np.zeros(d)
+        out[a] = (x[a] - c) / vol  # points out of the face at either end
+        return out
+
+    if lumping:
+        corners = [lo + h * np.array(bits) for bits in itertools.product([0, 1], repeat=d)]
+        pts, wts = corners, [vol / len(corners)] * len(corners)
+    else:
+        g = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
+        pts = [lo + h * np.array(t) for t in itertools.product(g, repeat=d)]
+        wts = [vol / len(pts)] * len(pts)
+    n = len(faces)
+    M = np.zeros((n, n))
+    for x, wt in zip(pts, wts):
+        for e in range(n):
+            for f in range(n):
+                M[e, f] += wt * w(e, x) @ w(f, x)
+    return M, vol
+
+
+def box_b_from_storage(dim, lumping, st):
+    """The full b_ijKef of one (i,j,K) from its box storage (MassMatrix.h:57-71,
+    118-132, 190-205, 250-276)."""
+    fpc = 2 * dim
+    d = [0, 0] if dim == 1 else [0, 1, 0, 1]
+    B = np.zeros((fpc, fpc))
+    for e in range(fpc):
+        for f in range(fpc):
+            if lumping:
+                B[e, f] = st[d[e]] if e == f else 0.0
+            elif d[e] == d[f]:
+                B[e, f] = 2 * st[d[e]] if e == f else st[d[e]]
+    return B
+
+
+def _box_mesh(dim, s, rng):
+    """A box mesh with uneven, anisotropic spacing (hx != hy in every cell)."""
+    if dim == 1:
+        pts, cells = meshgen.unit_interval(s)
+        xs = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, s - 1)), [1.0]])
+        return xs[:, None], cells
+    pts, cells = meshgen.unit_square_quads(s)
+    xs = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, s - 1)), [1.0]])
+    ys = np.concatenate([[0.0], np.sort(rng.uniform(0, 0.7, s - 1)), [0.7]])
+    n1 = s + 1
+    pts = np.stack([np.tile(xs, n1), np.repeat(ys, n1)], axis=1)
+    return pts, cells
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("lumping", [0, 1])
+def test_box_mass_matrix_matches_quadrature(dim, lumping):
+    rng = np.random.default_rng(40 + dim + 2 * lumping)
+    pr = problems.adr_quads(s=3, n=2, lumping=lumping) if dim == 2 else problems.adr_1d(s=4, n=2, lumping=lumping)
+    pts, cells = _box_mesh(dim, 3 if dim == 2 else 4, rng)
+    cf, fc = meshgen.face_numbering(cells, pts.shape[0], dim)
+    C = cells.shape[0]
+    D = rng.uniform(0.1, 10.0, (2, 2, C))
+    pr = dataclasses.replace(pr, points=pts, cells=cells, cell_faces=cf, face_cells=fc,
+                             coeffs=dict(pr.coeffs, D_ijK=D.ravel()))
+    o = pyoracle.Oracle(pr)
+    o.pre_iterate(0.0, pr.dt)
+    st = o.array("b_ijK_storage").reshape(2, 2, dim, C)
+    meas = o.array("cell_measure")
+    for K in range(C):
+        M, vol = box_mass(pts[cells[K]], lumping)
+        assert abs(meas[K] - vol) < 1e-14 * vol
+        for i in range(2):
+            for j in range(2):
+                ref = D[i, j, K] * np.linalg.inv(M)
+                got = box_b_from_storage(dim, lumping, st[i, j, :, K])
+                assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), (K, i, j)
+
+
 # ---------------------------------------------------------------- brute force
+def _cell_mass(pr, K):
+    """(M, |K|) of cell K by quadrature of the basis."""
+    P = pr.points[pr.cells[K]]
+    if pr.box:
+        return box_mass(P, pr.lumping)
+    if not pr.lumping:
+        return exact_mass(P)
+    T = np.ones((pr.fpc, pr.fpc)); T[:, 1:] = P
+    return lumped_mass(P), abs(np.linalg.det(T)) / math.factorial(P.shape[1])
+
+
+def _face_measure(pr, K, e):
+    verts = pr.points[pr.cells[K][meshgen.face_vertex_table(pr.dim, pr.cells.shape[1])[e]]]
+    if pr.dim == 1:
+        return 1.0
+    if pr.dim == 2:
+        return np.linalg.norm(verts[1] - verts[0])
+    return 0.5 * np.linalg.norm(np.cross(verts[1] - verts[0], verts[2] - verts[0]))
+
+
 def _mixed_reference(pr, o_pts, tau, Z_old, Z_F_old, m_upw, bc_t):
     """Dense full mixed system (velocities, cell and face unknowns) for one
     time step, explicit upwind advection, from quadrature mass matrices."""
@@ -182,10 +295,7 @@ def _mixed_reference(pr, o_pts, tau, Z_old, Z_F_old, m_upw, bc_t):
     Minv = []
     vols = []
     for K in range(C):
-        M, vol = (exact_mass(pts[cells[K]]) if not pr.lumping else (lumped_mass(pts[cells[K]]), None))
-        if vol is None:
-            T = np.ones((fpc, fpc)); T[:, 1:] = pts[cells[K]]
-            vol = abs(np.linalg.det(T)) / math.factorial(dim)
+        M, vol = _cell_mass(pr, K)
         Minv.append(np.linalg.inv(M))
         vols.append(vol)
     # upwinded Z on faces (Solver.hpp:743-759): from the old state
@@ -227,10 +337,7 @@ def _mixed_reference(pr, o_pts, tau, Z_old, Z_F_old, m_upw, bc_t):
     for E in range(F):
         K0, K1 = fc[E]
         e0 = list(cf[K0]).index(E)
-        meas = np.linalg.norm(np.subtract(*pts[[v for q, v in enumerate(cells[K0]) if q != e0][:2]])) if dim == 2 else None
-        if dim == 3:
-            vv = pts[[v for q, v in enumerate(cells[K0]) if q != e0]]
-            meas = 0.5 * np.linalg.norm(np.cross(vv[1] - vv[0], vv[2] - vv[0]))
+        meas = _face_measure(pr, K0, e0)
         for i in range(n):
             if K1 >= 0:
                 e1 = list(cf[K1]).index(E)
@@ -277,6 +384,17 @@ def _tiny_problem(kind, seed):
         inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
         pts[inner] += 0.05 * rng.standard_normal((inner.sum(), 2))
         pr = dataclasses.replace(pr, points=pts)
+    elif kind.startswith(("quad", "edge")):
+        dim = 2 if kind.startswith("quad") else 1
+        n = int(kind[-1])
+        lumping = int("lumped" in kind)
+        pr = (problems.adr_quads(s=3, n=n, lumping=lumping) if dim == 2
+              else problems.adr_1d(s=5, n=n, lumping=lumping))
+        pts, cells = _box_mesh(dim, 3 if dim == 2 else 5, rng)
+        cf, fc = meshgen.face_numbering(cells, pts.shape[0], dim)
+        pr = dataclasses.replace(pr, points=pts, cells=cells, cell_faces=cf, face_cells=fc,
+                                 steps=2)
+        pr.coeffs["a_ijKe"] = 0.3 * rng.standard_normal(n * n * pr.fpc * pr.num_cells)
     else:
         pr = problems.adr_3d(s=2, steps=2, preconditioner="diagonal")
     n, C, F, fpc = pr.n, pr.num_cells, pr.num_faces, pr.fpc
@@ -305,7 +423,9 @@ def _tiny_problem(kind, seed):
                                do_mobility_upwind=0, bc_mobility=np.ones(n * F))
 
 
-@pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2", "adr_n3", "tri_adr_n2"])
+@pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2", "adr_n3", "tri_adr_n2",
+                                  "quad_n1", "quad_n2", "quad_lumped_n3", "edge_n1", "edge_n2",
+                                  "edge_lumped_n2"])
 def test_brute_force_mixed_system(kind):
     pr = _tiny_problem(kind, 3)
     o = pyoracle.Oracle(pr)

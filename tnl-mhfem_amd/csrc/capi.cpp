@@ -18,10 +18,31 @@ thread_local std::string g_create_error;
 // ---------------------------------------------------------------------------
 // Face numbering and incidence.  Faces are numbered in the order in which
 // they are first met when the cells are visited in index order and the faces
-// of each cell in local order (local face e is opposite local vertex e,
-// MassMatrix.h:538,661,787,932); the cells of a face are kept in ascending
+// of each cell in local order; the cells of a face are kept in ascending
 // order (mesh_helpers.h:49-59 returns them in that order: cellIndexes[0] is
-// the "main" element of Solver.hpp:670-671).
+// the "main" element of Solver.hpp:670-671).  Local face e of a simplex is
+// the face opposite local vertex e (MassMatrix.h:538,661,787,932); the faces
+// of an edge are its vertices, those of a quadrangle its edges (0,1), (1,2),
+// (2,3), (3,0) (TNL's subentity order, on which the face directions of
+// MassMatrix.h:178-184 rest).
+int face_vertices(int dim, int box, int e, int *lv)
+{
+    static const int quad[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    if (!box) {
+        int t = 0;
+        for (int v = 0; v <= dim; v++)
+            if (v != e) lv[t++] = v;
+        return t;
+    }
+    if (dim == 1) {
+        lv[0] = e;
+        return 1;
+    }
+    lv[0] = quad[e][0];
+    lv[1] = quad[e][1];
+    return 2;
+}
+
 struct FaceKey {
     int v[3];
     bool operator==(const FaceKey &o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
@@ -46,15 +67,15 @@ void build_topology(Ctx &c, const int *cv)
     seen.reserve((size_t)C * fpc);
     int nf = 0;
     for (int K = 0; K < C; K++) {
+        for (int v = 0; v < c.nvc; v++) {
+            const int vid = cv[(size_t)K * c.nvc + v];
+            if (vid < 0 || vid >= c.nv) throw Error{MHFEM_E_MESH, "cell vertex index out of range"};
+        }
         for (int e = 0; e < fpc; e++) {
             FaceKey key{{-1, -1, -1}};
-            int t = 0;
-            for (int v = 0; v < fpc; v++) {
-                if (v == e) continue;
-                const int vid = cv[(size_t)K * fpc + v];
-                if (vid < 0 || vid >= c.nv) throw Error{MHFEM_E_MESH, "cell vertex index out of range"};
-                key.v[t++] = vid;
-            }
+            int lv[3];
+            const int t = face_vertices(c.dim, c.box, e, lv);
+            for (int q = 0; q < t; q++) key.v[q] = cv[(size_t)K * c.nvc + lv[q]];
             std::sort(key.v, key.v + t);
             auto it = seen.find(key);
             int id;
@@ -94,8 +115,18 @@ double simplex_measure(int dim, const double *P)
     return std::fabs(det) / 6.0;
 }
 
+// measure of a box cell: an edge's length; a quadrangle's area as half the
+// cross product of its diagonals
+double box_measure(int dim, const double *P)
+{
+    if (dim == 1) return std::fabs(P[1] - P[0]);
+    const double d1x = P[4] - P[0], d1y = P[5] - P[1], d2x = P[6] - P[2], d2y = P[7] - P[3];
+    return 0.5 * std::fabs(d1x * d2y - d1y * d2x);
+}
+
 double facet_measure(int dim, const double *P)
 {
+    if (dim == 1) return 1.0;  // a point: boundary fluxes are not scaled
     if (dim == 2) return std::hypot(P[2] - P[0], P[3] - P[1]);
     double a[3], b[3];
     for (int k = 0; k < 3; k++) {
@@ -310,21 +341,40 @@ extern "C" {
 
 mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *model)
 {
+    if (mesh && (mesh->dim == 2 || mesh->dim == 3)) return mhfem_create_cells(mesh, mesh->dim + 1, model);
+    g_create_error = mesh ? "mhfem_create: dim must be 2 or 3" : "mhfem_create: null argument";
+    return nullptr;
+}
+
+mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell, const mhfem_model_desc *model)
+{
     Ctx *c = nullptr;
     try {
         if (!mesh || !model || !mesh->points || !mesh->cell_vertices)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_create: null argument"};
-        if (mesh->dim != 2 && mesh->dim != 3) throw Error{MHFEM_E_ARGUMENT, "mhfem_create: dim must be 2 or 3"};
+        const int dim = mesh->dim, vpc = vertices_per_cell;
+        const bool simplex = (dim == 2 && vpc == 3) || (dim == 3 && vpc == 4);
+        const bool box = (dim == 1 && vpc == 2) || (dim == 2 && vpc == 4);
+        if (dim == 3 && vpc == 8)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: hexahedra are not supported (edges, triangles, "
+                                          "quadrangles and tetrahedra are)"};
+        if (!simplex && !box)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: cells of " + std::to_string(vpc) + " vertices in " +
+                                              std::to_string(dim) + "-D are not supported"};
         if (model->num_equations < 1 || model->num_equations > 3)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_create: num_equations must be 1..3"};
-        if (mesh->num_cells < 1 || mesh->num_points < mesh->dim + 1)
+        if (mesh->num_cells < 1 || mesh->num_points < vpc)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_create: empty mesh"};
         if (model->advection < 0 || model->advection > 2)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_create: unknown advection discretization"};
         c = new Ctx();
-        c->dim = mesh->dim;
-        c->fpc = mesh->dim + 1;
-        c->S = mesh->dim == 2 ? 9 : 14;
+        c->dim = dim;
+        c->box = box ? 1 : 0;
+        c->nvc = vpc;
+        c->fpc = box ? 2 * dim : dim + 1;
+        // MassMatrix::size: packed upper triangle + row sums for simplices,
+        // one value per face direction for boxes (MassMatrix.h:40, 101, 160, 224)
+        c->S = box ? dim : dim == 2 ? 9 : 14;
         c->n = model->num_equations;
         c->nv = mesh->num_points;
         c->C = mesh->num_cells;
@@ -336,31 +386,29 @@ mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *mod
         build_topology(*c, mesh->cell_vertices);
         c->h_seg_cells = {0, c->C};
         c->h_seg_faces = {0, c->F};
-        const size_t Cn = c->C, fpc = c->fpc;
+        const size_t Cn = c->C, fpc = c->fpc, nvc = c->nvc;
         // measures
         std::vector<double> cm(Cn), fm(c->F, 0.0);
         for (size_t K = 0; K < Cn; K++) {
             double P[12];
-            for (size_t k = 0; k < fpc; k++)
+            for (size_t k = 0; k < nvc; k++)
                 for (int d = 0; d < c->dim; d++)
-                    P[k * c->dim + d] = mesh->points[(size_t)mesh->cell_vertices[K * fpc + k] * c->dim + d];
-            cm[K] = simplex_measure(c->dim, P);
+                    P[k * c->dim + d] = mesh->points[(size_t)mesh->cell_vertices[K * nvc + k] * c->dim + d];
+            cm[K] = c->box ? box_measure(c->dim, P) : simplex_measure(c->dim, P);
             for (size_t e = 0; e < fpc; e++) {
                 const int E = c->h_cell_faces[K * fpc + e];
                 if (c->h_face_cells[2 * (size_t)E] != (int)K) continue;
                 double Q[9];
-                int t = 0;
-                for (size_t k = 0; k < fpc; k++) {
-                    if (k == e) continue;
-                    for (int d = 0; d < c->dim; d++) Q[t * c->dim + d] = P[k * c->dim + d];
-                    t++;
-                }
+                int lv[3];
+                const int t = face_vertices(c->dim, c->box, (int)e, lv);
+                for (int q = 0; q < t; q++)
+                    for (int d = 0; d < c->dim; d++) Q[q * c->dim + d] = P[lv[q] * c->dim + d];
                 fm[E] = facet_measure(c->dim, Q);
             }
         }
         upload(c->cell_faces, c->h_cell_faces.data(), c->h_cell_faces.size());
         upload(c->face_cells, c->h_face_cells.data(), c->h_face_cells.size());
-        upload(c->cell_vertices, mesh->cell_vertices, Cn * fpc);
+        upload(c->cell_vertices, mesh->cell_vertices, Cn * nvc);
         upload(c->points, mesh->points, (size_t)c->nv * c->dim);
         std::vector<uint64_t> slots = build_face_slots(*c);
         upload(c->face_slots, slots.data(), slots.size());

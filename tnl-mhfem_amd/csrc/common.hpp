@@ -133,7 +133,7 @@ struct SchemeView {
     int lumping, advection, do_mob;
     // topology
     const int *cell_faces;      // C*FPC, K-major
-    const int *cell_vertices;   // C*FPC
+    const int *cell_vertices;   // C*NVC (NVC = FPC for simplices, 2^dim for boxes)
     const double *points;       // nv*dim
     const int *face_cells;      // F*2
     const uint64_t *face_slots; // per face: merged column order (see build_pattern)
@@ -186,6 +186,7 @@ struct Guess;   // guess.hip
 
 struct Ctx {
     int dim = 0, fpc = 0, S = 0, n = 0;
+    int box = 0, nvc = 0;  // box cells (edges, rectangles) or simplices; vertices per cell
     int nv = 0, C = 0, F = 0, nbnd = 0;
     int lumping = 0, advection = 0, do_mob = 1;
     hipStream_t stream = nullptr;

@@ -279,6 +279,73 @@ __global__ void __launch_bounds__(WG) k_geometry(SchemeView s, double *minv_out,
     if constexpr (MS > FPC * FPC) o[FPC * FPC] = 0.0;
 }
 
+// The same for box cells (MassMatrix.h:28-303): an edge (h_x = |K|) or an
+// axis-aligned rectangle (h_x, h_y from local vertices 0 and 2).  Each face
+// direction d has one value, g_0 = 2/h_x for an edge, g_0 = 2 h_x/h_y for the
+// horizontal faces e = 0, 2 of a rectangle and g_1 = 2 h_y/h_x for its
+// vertical faces e = 1, 3; M^{-1} holds g_d on the diagonal when lumped, else
+// 2 g_d on the diagonal and g_d between the two faces of direction d, so
+// that its row sums are MassMatrix::b_ijKe (g_d or 3 g_d).
+template <int FPC>
+__global__ void __launch_bounds__(WG) k_geometry_box(SchemeView s, double *minv_out, double *rows_out)
+{
+    constexpr int DIM = FPC / 2, NP = Simplex<FPC>::NP, MS = MinvLayout<FPC>::MS;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    double g[DIM];
+    if constexpr (DIM == 1) {
+        g[0] = 2.0 / s.cell_meas[K];
+    } else {
+        const int v0 = s.cell_vertices[(long)K * 4], v2 = s.cell_vertices[(long)K * 4 + 2];
+        const double hx = s.points[(long)v2 * 2] - s.points[(long)v0 * 2];
+        const double hy = s.points[(long)v2 * 2 + 1] - s.points[(long)v0 * 2 + 1];
+        g[0] = 2.0 * hx / hy;
+        g[1] = 2.0 * hy / hx;
+    }
+    double minv[NP];
+#pragma unroll
+    for (int f = 0; f < FPC; f++)
+#pragma unroll
+        for (int e = 0; e <= f; e++) {
+            const int d = e % DIM;
+            double v = 0.0;
+            if (e == f) v = s.lumping ? g[d] : 2.0 * g[d];
+            else if (f % DIM == d && !s.lumping) v = g[d];
+            minv[pk(e, f)] = v;
+        }
+#pragma unroll
+    for (int q = 0; q < NP; q++) minv_out[(long)q * s.C + K] = minv[q];
+    double *o = rows_out + (long)K * MS;
+#pragma unroll
+    for (int e = 0; e < FPC; e++)
+#pragma unroll
+        for (int f = 0; f < FPC; f++) o[e * FPC + f] = minv[pk(e, f)];
+    if constexpr (MS > FPC * FPC) o[FPC * FPC] = 0.0;
+}
+
+// b_ijK_storage of box cells: one value per face direction,
+// b_ijK_storage(i,j,K,d) = g_d D_ijK (MassMatrix.h:52-53, 172-175).
+template <int N, int FPC>
+__global__ void __launch_bounds__(WG) k_fill_b_box(SchemeView s, double *b)
+{
+    constexpr int DIM = FPC / 2;
+    const int K = blockIdx.x * WG + threadIdx.x;
+    if (K >= s.C) return;
+    const long C = s.C;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double D = s.D[((long)i * N + j) * C + K];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                // g_d: the lumped diagonal, or the entry between faces d and d + DIM
+                const double g = s.minv[(long)(s.lumping ? pk(d, d) : pk(d, d + DIM)) * C + K];
+                b[((long)(i * N + j) * DIM + d) * C + K] = b_entry(g, D);
+            }
+        }
+}
+
 // b_ijK_storage ((i*n+j)*S + q)*C + K: the packed b_ijK followed by its row
 // sums (MassMatrix.h:586-597), written only when a caller asks for the array.
 template <int N, int FPC>
@@ -875,6 +942,10 @@ static void dispatch(Ctx &c, Args &&...args);
             if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 3>), g_, BS, 0, (c).stream, __VA_ARGS__); \
             else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 3>), g_, BS, 0, (c).stream, __VA_ARGS__); \
             else hipLaunchKernelGGL((KERNEL<3, 3>), g_, BS, 0, (c).stream, __VA_ARGS__);           \
+        } else if ((c).fpc == 2) {                                                                  \
+            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 2>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 2>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<3, 2>), g_, BS, 0, (c).stream, __VA_ARGS__);           \
         } else {                                                                                    \
             if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 4>), g_, BS, 0, (c).stream, __VA_ARGS__); \
             else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 4>), g_, BS, 0, (c).stream, __VA_ARGS__); \
@@ -919,7 +990,9 @@ void launch_assemble(Ctx &c, double time, double tau)
 void launch_geometry(Ctx &c)
 {
     SchemeView s = c.view();
-    if (c.fpc == 3) hipLaunchKernelGGL(k_geometry<3>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
+    if (c.box && c.fpc == 2) hipLaunchKernelGGL(k_geometry_box<2>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
+    else if (c.box) hipLaunchKernelGGL(k_geometry_box<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
+    else if (c.fpc == 3) hipLaunchKernelGGL(k_geometry<3>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
     else hipLaunchKernelGGL(k_geometry<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
     MHFEM_HIP(hipGetLastError());
 }
@@ -927,7 +1000,8 @@ void launch_geometry(Ctx &c)
 void launch_fill_b(Ctx &c)
 {
     SchemeView s = c.view();
-    MHFEM_DISPATCH(c, k_fill_b, blocks(c.C), s, c.arr[MHFEM_b_ijK_storage].p);
+    if (c.box) MHFEM_DISPATCH(c, k_fill_b_box, blocks(c.C), s, c.arr[MHFEM_b_ijK_storage].p);
+    else MHFEM_DISPATCH(c, k_fill_b, blocks(c.C), s, c.arr[MHFEM_b_ijK_storage].p);
 }
 
 void launch_fill_R(Ctx &c)

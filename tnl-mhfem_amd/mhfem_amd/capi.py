@@ -75,6 +75,7 @@ class GuessStats(C.Structure):
 _P, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
 SIGNATURES = {
     "mhfem_create": (_P, [C.POINTER(MeshDesc), C.POINTER(ModelDesc)]),
+    "mhfem_create_cells": (_P, [C.POINTER(MeshDesc), _I, C.POINTER(ModelDesc)]),
     "mhfem_destroy": (None, [_P]),
     "mhfem_last_error": (C.c_char_p, [_P]),
     "mhfem_get_counts": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
@@ -299,7 +300,7 @@ class Solver:
             self._cells = np.ascontiguousarray(pr.cells, np.int32)
             mesh = MeshDesc(pr.dim, self._points.shape[0], self._cells.shape[0],
                             _ptr(self._points), _ptr(self._cells))
-            self.ctx = L.mhfem_create(C.byref(mesh), C.byref(model))
+            self.ctx = L.mhfem_create_cells(C.byref(mesh), self._cells.shape[1], C.byref(model))
         if not self.ctx:
             raise MHFEMError(-1, L.mhfem_last_error(None).decode())
         c, f, b = C.c_int(), C.c_int(), C.c_int()

@@ -1,10 +1,12 @@
-"""Structured simplex meshes, face numbering and seeded random fields.
+"""Structured meshes, face numbering and seeded random fields.
 
 These generate the synthetic inputs the benchmark configurations are defined
 on (the reference reads VTK meshes through TNL; its example meshes are Git-LFS
 stubs).  Vertex order inside each cell is fixed here because the scheme
-depends on it: local face ``e`` is the face opposite local vertex ``e``
-(reference MassMatrix.h:538,661,787,932).
+depends on it: local face ``e`` of a simplex is the face opposite local
+vertex ``e`` (reference MassMatrix.h:538,661,787,932); the faces of an edge
+are its vertices and those of a quadrangle its edges (0,1), (1,2), (2,3),
+(3,0), with vertices counter-clockwise from the lower-left one.
 """
 from __future__ import annotations
 
@@ -32,6 +34,46 @@ def unit_square_triangles(s: int):
     t1 = np.stack([v00, v11, v01], axis=1)
     cells = np.stack([t0, t1], axis=1).reshape(-1, 3).astype(np.int32)
     return points, cells
+
+
+def unit_square_quads(s: int):
+    """Unit square, ``s x s`` axis-aligned squares as quadrangle cells.
+
+    Vertex ``(ix, iy)`` has index ``iy*(s+1)+ix``; cells are visited with
+    ``ix`` fastest, vertices ``(v00, v10, v11, v01)`` (VTK_QUAD order).
+    """
+    n1 = s + 1
+    iy, ix = np.meshgrid(np.arange(n1), np.arange(n1), indexing="ij")
+    points = np.stack([ix.ravel() / s, iy.ravel() / s], axis=1).astype(np.float64)
+    qy, qx = np.meshgrid(np.arange(s), np.arange(s), indexing="ij")
+    v00 = (qy * n1 + qx).ravel()
+    cells = np.stack([v00, v00 + 1, v00 + 1 + n1, v00 + n1], axis=1).astype(np.int32)
+    return points, cells
+
+
+def unit_interval(s: int):
+    """Unit interval, ``s`` edges ``(k, k+1)``; points of shape ``(s+1, 1)``."""
+    points = (np.arange(s + 1, dtype=np.float64) / s)[:, None]
+    cells = np.stack([np.arange(s), np.arange(1, s + 1)], axis=1).astype(np.int32)
+    return points, cells
+
+
+def is_box(dim: int, vertices_per_cell: int) -> bool:
+    """Edges and quadrangles are box cells; triangles and tetrahedra simplices."""
+    return dim == 1 or (dim == 2 and vertices_per_cell == 4)
+
+
+def faces_per_cell(dim: int, vertices_per_cell: int) -> int:
+    return 2 * dim if is_box(dim, vertices_per_cell) else dim + 1
+
+
+def face_vertex_table(dim: int, vertices_per_cell: int):
+    """Local vertex positions of every local face."""
+    if dim == 1:
+        return [[0], [1]]
+    if is_box(dim, vertices_per_cell):
+        return [[0, 1], [1, 2], [2, 3], [3, 0]]
+    return [[v for v in range(dim + 1) if v != e] for e in range(dim + 1)]
 
 
 KUHN_PERMUTATIONS = list(itertools.permutations(range(3)))
@@ -64,21 +106,23 @@ def unit_cube_kuhn(s: int):
     return points, cells
 
 
-def face_numbering(cells: np.ndarray, num_points: int):
+def face_numbering(cells: np.ndarray, num_points: int, dim: int = None):
     """First-appearance face numbering.
 
     Faces are numbered in the order in which they are first met when cells are
-    visited in index order and the faces of a cell in local order (local face
-    ``e`` opposite local vertex ``e``); the cells of a face are in ascending
-    order, the second one is -1 on boundary faces.
+    visited in index order and the faces of a cell in local order (see the
+    module docstring for the local faces); the cells of a face are in
+    ascending order, the second one is -1 on boundary faces.  ``dim`` is the
+    mesh dimension (default: a simplex mesh, ``vertices - 1``).
     Returns ``(cell_faces[nc,fpc] int32, face_cells[nf,2] int32)``.
     """
-    nc, fpc = cells.shape
-    d = fpc - 1
+    nc, nvc = cells.shape
+    dim = nvc - 1 if dim is None else dim
+    table = face_vertex_table(dim, nvc)
+    fpc, d = len(table), len(table[0])
     seeds = np.empty((nc, fpc, d), dtype=np.int64)
-    for e in range(fpc):
-        others = [v for v in range(fpc) if v != e]
-        seeds[:, e, :] = cells[:, others]
+    for e, verts in enumerate(table):
+        seeds[:, e, :] = cells[:, verts]
     seeds.sort(axis=2)
     bits = int(np.ceil(np.log2(max(num_points, 2))))
     if bits * d > 63:
@@ -112,13 +156,10 @@ def face_numbering(cells: np.ndarray, num_points: int):
 
 def face_centroids(points, cells, cell_faces, nf):
     """Centroid of every face (from the first cell that contains it)."""
-    nc, fpc = cells.shape
     dim = points.shape[1]
     cent = np.zeros((nf, dim))
-    csum = points[cells].sum(axis=1)  # (nc, dim)
-    for e in range(fpc):
-        c = (csum - points[cells[:, e]]) / (fpc - 1)
-        cent[cell_faces[:, e]] = c
+    for e, verts in enumerate(face_vertex_table(dim, cells.shape[1])):
+        cent[cell_faces[:, e]] = points[cells[:, verts]].mean(axis=1)
     return cent
 
 
@@ -132,9 +173,22 @@ def outward_face_vectors(points, cells):
     Shape ``(nc, fpc, dim)``.  Used to project a constant vector field onto the
     lowest-order Raviart-Thomas degrees of freedom (face fluxes).
     """
-    nc, fpc = cells.shape
+    nc, nvc = cells.shape
     dim = points.shape[1]
-    P = points[cells]  # (nc, fpc, dim)
+    P = points[cells]  # (nc, nvc, dim)
+    if is_box(dim, nvc):
+        # |E| n on the faces of an axis-aligned box: -y, +x, +y, -x (an edge: -x, +x)
+        if dim == 1:
+            return np.stack([-np.ones(nc), np.ones(nc)], axis=1)[:, :, None]
+        hx = P[:, 2, 0] - P[:, 0, 0]
+        hy = P[:, 2, 1] - P[:, 0, 1]
+        out = np.zeros((nc, 4, 2))
+        out[:, 0, 1] = -hx
+        out[:, 1, 0] = hy
+        out[:, 2, 1] = hx
+        out[:, 3, 0] = -hy
+        return out
+    fpc = nvc
     out = np.zeros((nc, fpc, dim))
     for e in range(fpc):
         others = [v for v in range(fpc) if v != e]

@@ -63,8 +63,13 @@ class Problem:
         return self.face_cells.shape[0]
 
     @property
+    def box(self):
+        """Edge or quadrangle cells (else triangles or tetrahedra)."""
+        return meshgen.is_box(self.dim, self.cells.shape[1])
+
+    @property
     def fpc(self):
-        return self.cells.shape[1]
+        return meshgen.faces_per_cell(self.dim, self.cells.shape[1])
 
     def coefficient_array(self, name):
         """Full (non-uniform) array of a coefficient in the CUDA layout."""
@@ -82,7 +87,7 @@ class Problem:
 
 
 def _topology(points, cells):
-    cf, fc = meshgen.face_numbering(cells, points.shape[0])
+    cf, fc = meshgen.face_numbering(cells, points.shape[0], points.shape[1])
     return cf, fc
 
 
@@ -244,6 +249,66 @@ def adr_2d(s: int = 16, n: int = 2, dt: float = 1e-3, steps: int = 3,
                    bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
                    Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
                    preconditioner=preconditioner, do_mobility_upwind=1,
+                   bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc)
+
+
+def heat_quads(s: int = 16, dt: float = 1e-3, steps: int = 3, lumping: int = 0) -> Problem:
+    """The heat problem of heat_2d on ``s x s`` square quadrangle cells."""
+    points, cells = meshgen.unit_square_quads(s)
+    cf, fc = _topology(points, cells)
+    F = fc.shape[0]
+    coeffs = dict(N_ijK=np.array(1.0), u_ijKe=np.array(0.0), m_iK=np.array(1.0),
+                  D_ijK=np.array(1.0), w_iKe=np.array(0.0), a_ijKe=np.array(0.0),
+                  r_ijK=np.array(0.0), f_iK=np.array(0.0))
+    c = meshgen.cell_centroids(points, cells)
+    z0 = np.exp(-((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.5) ** 2) / 0.01)
+    return Problem("heat_quads", points, cells, 1, coeffs,
+                   tags=np.full(F, FIXED_VALUE, np.uint8), bc_values=np.zeros(F),
+                   bc_dirichlet=np.zeros(F), Z_iK0=z0, dt=dt, steps=steps,
+                   solver="bicgstab", preconditioner="diagonal", lumping=lumping,
+                   do_mobility_upwind=0, bc_mobility=np.ones(F), cell_faces=cf, face_cells=fc)
+
+
+def adr_quads(s: int = 16, n: int = 2, dt: float = 1e-3, steps: int = 3,
+              preconditioner: str = "diagonal", lumping: int = 0) -> Problem:
+    """The coupled ADR model of adr_2d on ``s x s`` square quadrangle cells
+    (MassMatrix.h:146-303): the same K field, advection (1, 0.5) as face
+    fluxes, the same boundary pattern."""
+    points, cells = meshgen.unit_square_quads(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    c = meshgen.cell_centroids(points, cells)
+    K = 1.0 + 0.5 * np.sin(2 * np.pi * c[:, 0]) * np.sin(2 * np.pi * c[:, 1])
+    fv = meshgen.outward_face_vectors(points, cells)  # (C, 4, 2)
+    flux = (fv * np.array([1.0, 0.5])).sum(axis=2)
+    coeffs = _adr_coefficients(n, C, K, flux)
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    tags, dirichlet, _ = _adr_boundary(n, fc, fcen)
+    return Problem("adr_quads", points, cells, n, coeffs, tags=tags.ravel(),
+                   bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
+                   Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
+                   preconditioner=preconditioner, lumping=lumping, do_mobility_upwind=1,
+                   bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc)
+
+
+def adr_1d(s: int = 32, n: int = 2, dt: float = 1e-3, steps: int = 3,
+           preconditioner: str = "diagonal", lumping: int = 0) -> Problem:
+    """The coupled ADR model on the unit interval (edge cells, MassMatrix.h:28-145):
+    K = 1 + 0.5 sin(2 pi x), advection 1 as face fluxes (+-1), Dirichlet
+    Z_i = 2^-i at x = 0, AdvectiveOutflow at x = 1."""
+    points, cells = meshgen.unit_interval(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    c = meshgen.cell_centroids(points, cells)
+    K = 1.0 + 0.5 * np.sin(2 * np.pi * c[:, 0])
+    flux = meshgen.outward_face_vectors(points, cells)[:, :, 0]  # (C, 2)
+    coeffs = _adr_coefficients(n, C, K, flux)
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    tags, dirichlet, _ = _adr_boundary(n, fc, fcen)
+    return Problem("adr_1d", points, cells, n, coeffs, tags=tags.ravel(),
+                   bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
+                   Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
+                   preconditioner=preconditioner, lumping=lumping, do_mobility_upwind=1,
                    bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc)
 
 
