@@ -273,7 +273,8 @@ int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *
 
 /* ---- files on either side of the path (host code; no GPU work).  Errors
  * leave their message for mhfem_last_error(NULL). */
-/* Mesh files of triangles or tetrahedra with their cell data: VTK XML (.vtu:
+/* Mesh files of lines, triangles, quadrangles or tetrahedra (one shape per
+ * file) with their cell data: VTK XML (.vtu:
  * ascii, base64 binary with or without zlib compression, raw or base64
  * appended data) and legacy VTK (.vtk: ASCII or BINARY).  Replace
  * TNL::Meshes::loadMesh (control.h:20-48) and the reader's readCellData for
@@ -282,9 +283,10 @@ int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *
 typedef struct mhfem_mesh_file mhfem_mesh_file;
 mhfem_mesh_file *mhfem_mesh_file_open(const char *path, const char *format);
 void mhfem_mesh_file_close(mhfem_mesh_file *f);
-/* dim 2 (triangles) or 3 (tetrahedra); counts of points and cells */
-int mhfem_mesh_file_info(const mhfem_mesh_file *f, int *dim, long *points, long *cells);
-/* points[points*dim], cell_vertices[cells*(dim+1)] in the file's vertex order */
+/* dimension, vertices per cell (the arguments of mhfem_create_cells), counts
+ * of points and cells */
+int mhfem_mesh_file_info(const mhfem_mesh_file *f, int *dim, int *vertices_per_cell, long *points, long *cells);
+/* points[points*dim], cell_vertices[cells*vertices_per_cell] in the file's vertex order */
 int mhfem_mesh_file_read(const mhfem_mesh_file *f, double *points, int32_t *cell_vertices);
 /* a cell-data array by name (converted to double, components interleaved);
  * returns its length (out may be NULL to ask for it) or a negative code */
@@ -296,8 +298,8 @@ const char *mhfem_mesh_file_cell_data_name(const mhfem_mesh_file *f, int index);
  * FileFormat::zlib_compressed): FieldData CYCLE (Int32) and TIME (Float64)
  * when >= 0, the mesh, and `fields` cell arrays of num_cells doubles. */
 enum { MHFEM_VTK_ASCII = 0, MHFEM_VTK_BINARY = 1, MHFEM_VTK_ZLIB = 2 };
-int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, long cycle, double time, int fields,
-                    const char *const *names, const double *const *values, int format);
+int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, int vertices_per_cell, long cycle, double time,
+                    int fields, const char *const *names, const double *const *values, int format);
 
 /* BoundaryConditionsStorage in TNL's binary object format
  * (BoundaryConditionsStorage.h:22-72): type "TNL::MHFEM::BoundaryConditionsStorage< double >",

@@ -18,14 +18,18 @@
 
 namespace mhfem {
 
-// simplex mesh: points (dim per vertex) and cells (dim+1 vertices; local face e
-// is opposite local vertex e)
+// mesh: points (dim per vertex) and cells of verticesPerCell vertices:
+// simplices (dim + 1; local face e is opposite local vertex e), edges (2) or
+// axis-aligned quadrangles (4 in 2-D, counter-clockwise from the lower-left
+// vertex; faces (0,1), (1,2), (2,3), (3,0))
 struct SimplexMesh {
     int dim = 2;
+    int verticesPerCell = 0;  // 0: dim + 1
     std::vector<double> points;
     std::vector<int32_t> cells;
+    int vertices() const { return verticesPerCell ? verticesPerCell : dim + 1; }
     int numPoints() const { return (int)(points.size() / dim); }
-    int numCells() const { return (int)(cells.size() / (dim + 1)); }
+    int numCells() const { return (int)(cells.size() / vertices()); }
     mhfem_mesh_desc desc() const { return mhfem_mesh_desc{dim, numPoints(), numCells(), points.data(), cells.data()}; }
 };
 
@@ -83,13 +87,14 @@ public:
 
     SimplexMesh mesh() const
     {
-        int dim = 0;
+        int dim = 0, nvc = 0;
         long np = 0, nc = 0;
-        detail::io_check(mhfem_mesh_file_info(f_, &dim, &np, &nc));
+        detail::io_check(mhfem_mesh_file_info(f_, &dim, &nvc, &np, &nc));
         SimplexMesh m;
         m.dim = dim;
+        m.verticesPerCell = nvc;
         m.points.resize((size_t)np * dim);
-        m.cells.resize((size_t)nc * (dim + 1));
+        m.cells.resize((size_t)nc * nvc);
         detail::io_check(mhfem_mesh_file_read(f_, m.points.data(), m.cells.data()));
         return m;
     }
@@ -134,8 +139,8 @@ inline void writeVTU(const std::string &fileName, const SimplexMesh &mesh, long 
         values.push_back(c.second->data());
     }
     const mhfem_mesh_desc d = mesh.desc();
-    detail::io_check(mhfem_write_vtu(fileName.c_str(), &d, cycle, time, (int)names.size(), names.data(),
-                                     values.data(), format));
+    detail::io_check(mhfem_write_vtu(fileName.c_str(), &d, mesh.vertices(), cycle, time, (int)names.size(),
+                                     names.data(), values.data(), format));
 }
 
 }  // namespace mhfem

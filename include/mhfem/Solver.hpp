@@ -39,7 +39,7 @@ public:
         mhfem_model_desc mo{n, Model::lumping == MassLumping::enabled ? 1 : 0, (int)Model::Advection,
                             Model::do_mobility_upwind ? 1 : 0};
         if (ctx_) mhfem_destroy(ctx_);
-        ctx_ = mhfem_create(&md, &mo);
+        ctx_ = mhfem_create_cells(&md, mesh_.vertices(), &mo);
         if (!ctx_) throw std::runtime_error(std::string("setMesh: ") + mhfem_last_error(nullptr));
         mhfem_get_counts(ctx_, &cells_, &faces_, nullptr);
         model_.attach(ctx_, &mesh_);

@@ -88,11 +88,16 @@ def vtu_text(points3, conn, offsets, types, cell_data, fmt="binary", header="UIn
     return blob + b"</VTKFile>\n"
 
 
+def vtk_cell_type(dim, nv):
+    """VTK_LINE 3, VTK_TRIANGLE 5, VTK_QUAD 9, VTK_TETRA 10."""
+    return {(1, 2): 3, (2, 3): 5, (2, 4): 9, (3, 4): 10}[(dim, nv)]
+
+
 def mesh_arrays(points, cells, int_t=np.int32):
     nv = cells.shape[1]
     conn = cells.astype(int_t).ravel()
     offsets = (np.arange(1, cells.shape[0] + 1) * nv).astype(int_t)
-    types = np.full(cells.shape[0], 5 if nv == 3 else 10, np.uint8)
+    types = np.full(cells.shape[0], vtk_cell_type(points.shape[1], nv), np.uint8)
     return pad3(points), conn, offsets, types
 
 
@@ -162,7 +167,8 @@ def legacy_vtk(points3, cells, cell_data, binary=False, version="2.0", extra_poi
     else:
         lst = np.concatenate([np.full((C, 1), nv), cells], axis=1)
         out += [f"CELLS {C} {C * (nv + 1)}\n".encode(), block(lst, "i4")]
-    out += [f"CELL_TYPES {C}\n".encode(), block(np.full(C, 5 if nv == 3 else 10), "i4")]
+    dim = 1 if not points3[:, 1:].any() and nv == 2 else (2 if not points3[:, 2].any() else 3)
+    out += [f"CELL_TYPES {C}\n".encode(), block(np.full(C, vtk_cell_type(dim, nv)), "i4")]
     if extra_point_data:
         out += [f"POINT_DATA {points3.shape[0]}\nSCALARS pid int 1\nLOOKUP_TABLE default\n".encode(),
                 block(np.arange(points3.shape[0]), "i4")]

@@ -32,6 +32,10 @@ def tet_mesh():
     return points, cells
 
 
+MESHES = {"tri": tri_mesh, "tet": tet_mesh, "quad": lambda: meshgen.unit_square_quads(3),
+          "edge": lambda: meshgen.unit_interval(7)}
+
+
 VTU_CASES = {
     "ascii": dict(fmt="ascii"),
     "binary_u32_apart": dict(fmt="binary", header="UInt32"),
@@ -51,10 +55,10 @@ VTU_CASES = {
 }
 
 
-@pytest.mark.parametrize("mesh", ["tri", "tet"])
+@pytest.mark.parametrize("mesh", sorted(MESHES))
 @pytest.mark.parametrize("case", sorted(VTU_CASES))
 def test_vtu_reader_every_encoding(tmp_path, mesh, case):
-    points, cells = tri_mesh() if mesh == "tri" else tet_mesh()
+    points, cells = MESHES[mesh]()
     rng = np.random.default_rng(5)
     C = cells.shape[0]
     z0 = rng.standard_normal(C)
@@ -97,9 +101,9 @@ def test_vtu_reader_rejects_bad_meshes(tmp_path):
     points, cells = tri_mesh()
     p3, conn, offs, types = mesh_arrays(points, cells)
     p = tmp_path / "a.vtu"
-    # quadrilaterals are not simplices
-    p.write_bytes(vtu_text(p3, conn[: 4 * 4], np.array([4, 8, 12, 16], np.int32), np.full(4, 9, np.uint8), {}))
-    _expect_error(p, "unsupported cell type 9")
+    # polygons (VTK_POLYGON) are not among the cell shapes of the scheme
+    p.write_bytes(vtu_text(p3, conn[: 4 * 4], np.array([4, 8, 12, 16], np.int32), np.full(4, 7, np.uint8), {}))
+    _expect_error(p, "unsupported cell type 7")
     t2 = types.copy()
     t2[3] = 10
     p.write_bytes(vtu_text(p3, conn, offs, t2, {}))
@@ -137,9 +141,9 @@ def test_vtu_reader_rejects_bad_meshes(tmp_path):
 
 
 @pytest.mark.parametrize("fmt", ["ascii", "binary", "zlib"])
-@pytest.mark.parametrize("mesh", ["tri", "tet"])
+@pytest.mark.parametrize("mesh", sorted(MESHES))
 def test_vtu_writer_follows_the_format(tmp_path, fmt, mesh):
-    points, cells = tri_mesh() if mesh == "tri" else tet_mesh()
+    points, cells = MESHES[mesh]()
     rng = np.random.default_rng(11)
     C = cells.shape[0]
     z = rng.standard_normal(C) * 1e3
@@ -156,7 +160,8 @@ def test_vtu_writer_follows_the_format(tmp_path, fmt, mesh):
     nv = cells.shape[1]
     np.testing.assert_array_equal(cl["connectivity"], cells.ravel())
     np.testing.assert_array_equal(cl["offsets"], nv * np.arange(1, C + 1))
-    np.testing.assert_array_equal(cl["types"], np.full(C, 5 if nv == 3 else 10))
+    np.testing.assert_array_equal(cl["types"], np.full(C, {(1, 2): 3, (2, 3): 5, (2, 4): 9, (3, 4): 10}[
+        (points.shape[1], nv)]))
     np.testing.assert_array_equal(cdata["Z"], z)  # %.17g text round-trips exactly
     np.testing.assert_array_equal(cdata["w & <ratio>"], w)
     # and the reader takes it back
@@ -184,9 +189,9 @@ def test_vtu_writer_without_metadata_and_bad_input(tmp_path):
 
 @pytest.mark.parametrize("binary", [False, True])
 @pytest.mark.parametrize("version", ["2.0", "5.1"])
-@pytest.mark.parametrize("mesh", ["tri", "tet"])
+@pytest.mark.parametrize("mesh", sorted(MESHES))
 def test_legacy_vtk_reader(tmp_path, binary, version, mesh):
-    points, cells = tri_mesh() if mesh == "tri" else tet_mesh()
+    points, cells = MESHES[mesh]()
     rng = np.random.default_rng(3)
     C = cells.shape[0]
     z = rng.standard_normal(C)
@@ -326,3 +331,14 @@ def test_problem_from_files_matches_the_generated_problem(tmp_path):
     assert (d.tags == 0).all() and not d.bc_dirichlet.any() and not d.Z_iK0.any() and d.tags.size == 2 * F
     with pytest.raises(ValueError, match="Wrong dofSize in BoundaryConditionsStorage"):
         problems.from_files(str(mesh), str(tmp_path / "boundary.tnl"), n=1)
+
+
+def test_problem_from_files_with_quadrangles(tmp_path):
+    pr = problems.adr_quads(s=5, n=2)
+    mesh = tmp_path / "quads.vtu"
+    capi.write_vtu(mesh, pr.points, pr.cells, format="binary")
+    got = problems.from_files(str(mesh), n=2, coeffs=pr.coeffs)
+    assert got.box and got.fpc == 4
+    np.testing.assert_array_equal(got.cells, pr.cells)
+    np.testing.assert_array_equal(got.cell_faces, pr.cell_faces)
+    np.testing.assert_array_equal(got.face_cells, pr.face_cells)

@@ -47,9 +47,9 @@ void set_global_error(const std::string &msg);  // capi.cpp: message of mhfem_la
 }
 
 struct mhfem_mesh_file {
-    int dim = 0;
+    int dim = 0, nvc = 0;         // dimension, vertices per cell
     std::vector<double> points;   // dim per vertex
-    std::vector<int32_t> cells;   // dim + 1 per cell
+    std::vector<int32_t> cells;   // nvc per cell
     std::map<std::string, std::vector<double>> cell_data;
     std::vector<std::string> cell_data_order;
 };
@@ -584,32 +584,44 @@ struct VtuContext {
     }
 };
 
-int cell_dim(int64_t vtk_type)
+// VTK cell types of the supported shapes: dimension and vertices per cell
+struct CellShape {
+    int dim, nvc;
+};
+CellShape cell_shape(int64_t vtk_type)
 {
-    if (vtk_type == 5) return 2;   // VTK_TRIANGLE
-    if (vtk_type == 10) return 3;  // VTK_TETRA
-    fail("unsupported cell type " + std::to_string(vtk_type) + " (triangles and tetrahedra only)");
+    switch (vtk_type) {
+    case 3: return {1, 2};    // VTK_LINE
+    case 5: return {2, 3};    // VTK_TRIANGLE
+    case 9: return {2, 4};    // VTK_QUAD
+    case 10: return {3, 4};   // VTK_TETRA
+    default: break;
+    }
+    fail("unsupported cell type " + std::to_string(vtk_type) +
+         " (lines, triangles, quadrangles and tetrahedra only)");
 }
 
-// common checks once points, connectivity (cells * (dim+1)) and types are known
+// common checks once points, connectivity and types are known
 void finish_mesh(mhfem_mesh_file &m, const std::vector<double> &pts3, long np, const std::vector<int64_t> &conn,
                  const std::vector<int64_t> &offsets_end, const std::vector<int64_t> &types)
 {
     const long nc = (long)types.size();
     if (nc == 0) fail("the mesh has no cells");
     if ((long)pts3.size() != 3 * np) fail("the Points array does not hold 3 coordinates per point");
-    const int dim = cell_dim(types[0]);
+    const CellShape sh = cell_shape(types[0]);
+    const int dim = sh.dim, nvc = sh.nvc;
     for (long k = 0; k < nc; k++)
         if (types[k] != types[0]) fail("mixed cell types are not supported");
     if ((long)offsets_end.size() != nc) fail("offsets and types differ in length");
     int64_t prev = 0;
     for (long k = 0; k < nc; k++) {
-        if (offsets_end[k] - prev != dim + 1) fail("cell " + std::to_string(k) + " does not have " + std::to_string(dim + 1) + " vertices");
+        if (offsets_end[k] - prev != nvc) fail("cell " + std::to_string(k) + " does not have " + std::to_string(nvc) + " vertices");
         prev = offsets_end[k];
     }
-    if ((long)conn.size() != nc * (dim + 1)) fail("connectivity length does not match the cells");
-    if (np > INT32_MAX || nc > INT32_MAX / 4) fail("mesh too large for 32-bit indices");
+    if ((long)conn.size() != nc * nvc) fail("connectivity length does not match the cells");
+    if (np > INT32_MAX || nc > INT32_MAX / 8) fail("mesh too large for 32-bit indices");
     m.dim = dim;
+    m.nvc = nvc;
     m.cells.resize(conn.size());
     for (size_t q = 0; q < conn.size(); q++) {
         if (conn[q] < 0 || conn[q] >= np) fail("connectivity index out of range");
@@ -618,7 +630,9 @@ void finish_mesh(mhfem_mesh_file &m, const std::vector<double> &pts3, long np, c
     m.points.resize((size_t)np * dim);
     for (long v = 0; v < np; v++) {
         for (int c = 0; c < dim; c++) m.points[(size_t)v * dim + c] = pts3[(size_t)v * 3 + c];
-        if (dim == 2 && pts3[(size_t)v * 3 + 2] != 0.0) fail("a triangle mesh must lie in the z = 0 plane");
+        for (int c = dim; c < 3; c++)
+            if (pts3[(size_t)v * 3 + c] != 0.0)
+                fail(dim == 1 ? "a mesh of lines must lie on the x axis" : "a 2-D mesh must lie in the z = 0 plane");
     }
 }
 
@@ -1042,12 +1056,13 @@ mhfem_mesh_file *mhfem_mesh_file_open(const char *path, const char *format)
 
 void mhfem_mesh_file_close(mhfem_mesh_file *f) { delete f; }
 
-int mhfem_mesh_file_info(const mhfem_mesh_file *f, int *dim, long *points, long *cells)
+int mhfem_mesh_file_info(const mhfem_mesh_file *f, int *dim, int *vertices_per_cell, long *points, long *cells)
 {
     if (!f) return MHFEM_E_ARGUMENT;
     if (dim) *dim = f->dim;
+    if (vertices_per_cell) *vertices_per_cell = f->nvc;
     if (points) *points = (long)(f->points.size() / f->dim);
-    if (cells) *cells = (long)(f->cells.size() / (f->dim + 1));
+    if (cells) *cells = (long)(f->cells.size() / f->nvc);
     return MHFEM_OK;
 }
 
@@ -1089,15 +1104,21 @@ const char *mhfem_mesh_file_cell_data_name(const mhfem_mesh_file *f, int index)
     return f->cell_data_order[index].c_str();
 }
 
-int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, long cycle, double time, int fields,
-                    const char *const *names, const double *const *values, int format)
+int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, int vertices_per_cell, long cycle, double time,
+                    int fields, const char *const *names, const double *const *values, int format)
 {
     return io_guarded([&] {
         if (!path || !mesh || (fields > 0 && (!names || !values))) fail("null argument", MHFEM_E_ARGUMENT);
-        if (mesh->dim != 2 && mesh->dim != 3) fail("mesh dim must be 2 or 3", MHFEM_E_ARGUMENT);
+        const int dim = mesh->dim, nv = vertices_per_cell;
+        int vtk_type = 0;
+        if (dim == 1 && nv == 2) vtk_type = 3;
+        else if (dim == 2 && nv == 3) vtk_type = 5;
+        else if (dim == 2 && nv == 4) vtk_type = 9;
+        else if (dim == 3 && nv == 4) vtk_type = 10;
+        else fail("unsupported cells: " + std::to_string(nv) + " vertices in " + std::to_string(dim) + "-D",
+                  MHFEM_E_ARGUMENT);
         if (format < MHFEM_VTK_ASCII || format > MHFEM_VTK_ZLIB) fail("unknown VTU format", MHFEM_E_ARGUMENT);
         const long np = mesh->num_points, nc = mesh->num_cells;
-        const int nv = mesh->dim + 1;
         if (np <= 0 || nc <= 0 || !mesh->points || !mesh->cell_vertices) fail("empty mesh", MHFEM_E_ARGUMENT);
         OutFile out(path);
         VtuWriter w{out, format};
@@ -1127,7 +1148,7 @@ int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, long cycle, d
         std::vector<int64_t> offs(nc);
         for (long k = 0; k < nc; k++) offs[k] = (int64_t)(k + 1) * nv;
         w.array("Int64", "offsets", offs.data(), offs.size(), 1);
-        std::vector<uint8_t> types(nc, mesh->dim == 2 ? 5 : 10);
+        std::vector<uint8_t> types(nc, (uint8_t)vtk_type);
         w.array("UInt8", "types", types.data(), types.size(), 1);
         out.put("</Cells>\n");
         if (fields > 0) {

@@ -116,12 +116,12 @@ SIGNATURES = {
     "mhfem_synchronize": (_I, [_P]),
     "mhfem_mesh_file_open": (_P, [C.c_char_p, C.c_char_p]),
     "mhfem_mesh_file_close": (None, [_P]),
-    "mhfem_mesh_file_info": (_I, [_P, C.POINTER(_I), C.POINTER(_L), C.POINTER(_L)]),
+    "mhfem_mesh_file_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_L), C.POINTER(_L)]),
     "mhfem_mesh_file_read": (_I, [_P, _P, _P]),
     "mhfem_mesh_file_cell_data": (_L, [_P, C.c_char_p, _P, _L]),
     "mhfem_mesh_file_cell_data_count": (_I, [_P]),
     "mhfem_mesh_file_cell_data_name": (C.c_char_p, [_P, _I]),
-    "mhfem_write_vtu": (_I, [C.c_char_p, C.POINTER(MeshDesc), _L, _D, _I, C.POINTER(C.c_char_p),
+    "mhfem_write_vtu": (_I, [C.c_char_p, C.POINTER(MeshDesc), _I, _L, _D, _I, C.POINTER(C.c_char_p),
                              C.POINTER(C.c_void_p), _I]),
     "mhfem_bc_save": (_I, [C.c_char_p, C.c_int64, _P, _P, _P]),
     "mhfem_bc_load": (_I, [C.c_char_p, C.POINTER(C.c_int64), _P, _P, _P, C.c_int64]),
@@ -173,8 +173,8 @@ VTK_FORMATS = {"ascii": 0, "binary": 1, "zlib": 2}
 
 
 class MeshFile:
-    """A mesh file of triangles or tetrahedra with its cell data, read by the
-    library: .vtu (VTK XML) or .vtk (legacy VTK).  Stands for
+    """A mesh file of lines, triangles, quadrangles or tetrahedra with its cell
+    data, read by the library: .vtu (VTK XML) or .vtk (legacy VTK).  Stands for
     TNL::Meshes::loadMesh (control.h:20-48) and the reader's readCellData
     (Solver.hpp:296-306)."""
 
@@ -184,11 +184,11 @@ class MeshFile:
         if not h:
             raise MHFEMError(-7, L.mhfem_last_error(None).decode())
         try:
-            d, npts, nc = C.c_int(), C.c_long(), C.c_long()
-            _global_check(L.mhfem_mesh_file_info(h, C.byref(d), C.byref(npts), C.byref(nc)))
+            d, nvc, npts, nc = C.c_int(), C.c_int(), C.c_long(), C.c_long()
+            _global_check(L.mhfem_mesh_file_info(h, C.byref(d), C.byref(nvc), C.byref(npts), C.byref(nc)))
             self.dim = d.value
             self.points = np.empty((npts.value, self.dim), np.float64)
-            self.cells = np.empty((nc.value, self.dim + 1), np.int32)
+            self.cells = np.empty((nc.value, nvc.value), np.int32)
             _global_check(L.mhfem_mesh_file_read(h, _ptr(self.points), _ptr(self.cells)))
             self._data = {}
             for k in range(L.mhfem_mesh_file_cell_data_count(h)):
@@ -210,8 +210,8 @@ class MeshFile:
 
 
 def write_vtu(path, points, cells, cell_data=None, cycle=-1, time=-1.0, format="zlib"):
-    """VTU file of a simplex mesh with float64 cell arrays (the snapshot
-    writer of HeatEquationModel.h:164-231)."""
+    """VTU file of a mesh (cells of 2, 3 or 4 vertices) with float64 cell
+    arrays (the snapshot writer of HeatEquationModel.h:164-231)."""
     pts = np.ascontiguousarray(points, np.float64)
     cel = np.ascontiguousarray(cells, np.int32)
     mesh = MeshDesc(pts.shape[1], pts.shape[0], cel.shape[0], _ptr(pts), _ptr(cel))
@@ -222,8 +222,8 @@ def write_vtu(path, points, cells, cell_data=None, cycle=-1, time=-1.0, format="
             raise ValueError(f"cell data '{name}' has {v.size} values for {cel.shape[0]} cells")
     names = (C.c_char_p * max(1, len(arrays)))(*[k.encode() for k in cell_data])
     vals = (C.c_void_p * max(1, len(arrays)))(*[v.ctypes.data for v in arrays])
-    _global_check(lib().mhfem_write_vtu(_path(path), C.byref(mesh), int(cycle), float(time), len(arrays),
-                                        names, vals, VTK_FORMATS[format]))
+    _global_check(lib().mhfem_write_vtu(_path(path), C.byref(mesh), cel.shape[1], int(cycle), float(time),
+                                        len(arrays), names, vals, VTK_FORMATS[format]))
 
 
 class BoundaryConditionsStorage:
