@@ -137,13 +137,15 @@ typedef struct {
 
 /* ---- lifetime (Solver::setMesh, Solver.hpp:26-81; BaseModel::allocate, BaseModel.h:168-195) */
 mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *model);
-/* Any cell shape of the reference's examples/config.h:16-20 but the
- * hexahedron: vertices_per_cell 2 (edges, dim 1), 3 (triangles), 4 with dim 2
+/* Every cell shape of the reference's examples/config.h:16-20:
+ * vertices_per_cell 2 (edges, dim 1), 3 (triangles), 4 with dim 2
  * (quadrangles: axis-aligned rectangles, vertices counter-clockwise from the
- * lower-left one as VTK orders them, MassMatrix.h:146-303) or 4 with dim 3
- * (tetrahedra).  cell_vertices then holds num_cells*vertices_per_cell entries;
- * b_ijK_storage has MassMatrix::size values per (i,j,K): 9 or 14 for
- * simplices, 1 for edges, 2 for rectangles.  mhfem_create(mesh, model) is
+ * lower-left one as VTK orders them, MassMatrix.h:146-303), 4 with dim 3
+ * (tetrahedra) or 8 (hexahedra: axis-aligned voxels in VTK order,
+ * MassMatrix.h:305-508).  cell_vertices then holds
+ * num_cells*vertices_per_cell entries; b_ijK_storage has MassMatrix::size
+ * values per (i,j,K): 9 or 14 for simplices, 1, 2 or 3 for edges, rectangles
+ * and voxels.  mhfem_create(mesh, model) is
  * mhfem_create_cells(mesh, mesh->dim + 1, model). */
 mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell, const mhfem_model_desc *model);
 void mhfem_destroy(mhfem_ctx *ctx);
@@ -273,8 +275,8 @@ int mhfem_guess_gradient(mhfem_ctx *ctx, const uint16_t *features, const float *
 
 /* ---- files on either side of the path (host code; no GPU work).  Errors
  * leave their message for mhfem_last_error(NULL). */
-/* Mesh files of lines, triangles, quadrangles or tetrahedra (one shape per
- * file) with their cell data: VTK XML (.vtu:
+/* Mesh files of lines, triangles, quadrangles, tetrahedra or hexahedra (one
+ * shape per file) with their cell data: VTK XML (.vtu:
  * ascii, base64 binary with or without zlib compression, raw or base64
  * appended data) and legacy VTK (.vtk: ASCII or BINARY).  Replace
  * TNL::Meshes::loadMesh (control.h:20-48) and the reader's readCellData for

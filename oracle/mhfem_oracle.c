@@ -239,6 +239,16 @@ static double face_measure(int dim, const double *v /* dim points of dim coords 
     return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
 }
 
+/* a hexahedron's face (a planar quadrangle in 3-D): half the cross product
+ * of its diagonals */
+static double quad_face_measure(const double *v /* 4 points of 3 coords */)
+{
+    double a[3], b[3];
+    for (int k = 0; k < 3; k++) { a[k] = v[6 + k] - v[k]; b[k] = v[9 + k] - v[3 + k]; }
+    double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    return 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+}
+
 /* ------------------------------------------------------------------ */
 /* LU without pivoting (Doolittle, in place) and the in-place solve: the
  * published method restated for TNL's LU_sequential_factorize /
@@ -375,13 +385,13 @@ or_problem *or_create(int dim, int nv, const double *points, int nc, const int *
 }
 
 /* Cells: simplices (triangles, tetrahedra) or boxes (edges, axis-aligned
- * rectangles as Quadrangle cells, MassMatrix.h:146-303).  b_ijK_storage holds
+ * rectangles and voxels as Quadrangle and Hexahedron cells, MassMatrix.h:28-508).  b_ijK_storage holds
  * MassMatrix::size values per (i,j,K): 9 / 14 for simplices, dim for boxes. */
 or_problem *or_create_shape(int dim, int shape, int nv, const double *points, int nc, const int *cv,
                             int n, int lumping, int advection, int do_mob)
 {
     if (dim == 1) shape = OR_BOX;
-    if (shape == OR_SIMPLEX ? (dim != 2 && dim != 3) : (dim != 1 && dim != 2)) return NULL;
+    if (dim < 1 || dim > 3 || (shape == OR_SIMPLEX && dim == 1)) return NULL;
     or_problem *p = calloc(1, sizeof(or_problem));
     p->dim = dim;
     p->shape = shape;
@@ -424,7 +434,8 @@ or_problem *or_create_shape(int dim, int shape, int nv, const double *points, in
             for (int k = 0; k < t; k++)
                 for (int c = 0; c < dim; c++) fv[k * dim + c] = vv[lv[k] * dim + c];
             int E = p->cell_faces[(long)K * fpc + e];
-            if (p->face_cells[2L * E] == K) p->face_meas[E] = face_measure(dim, fv);
+            if (p->face_cells[2L * E] == K)
+                p->face_meas[E] = (shape == OR_BOX && dim == 3) ? quad_face_measure(fv) : face_measure(dim, fv);
         }
     }
     const long C = nc, F = nf;
@@ -567,13 +578,13 @@ int or_setup_linear_system(or_problem *p)
             int *c = p->colidx + p->rowptr[E * n + i];
             if (K1 < 0) {
                 if (p->tags[BCI(i, E)] == OR_FIXED_VALUE) { c[0] = (int)(E * n + i); continue; }
-                int perm[4];
+                int perm[6];
                 sorted_faces(p, K0, perm);
                 int t = 0;
                 for (int g = 0; g < fpc; g++)
                     for (int j = 0; j < n; j++) c[t++] = p->cell_faces[(long)K0 * fpc + perm[g]] * n + j;
             } else {
-                int p0[4], p1[4], g0 = 0, g1 = 0, t = 0;
+                int p0[6], p1[6], g0 = 0, g1 = 0, t = 0;
                 const int *f0 = p->cell_faces + (long)K0 * fpc, *f1 = p->cell_faces + (long)K1 * fpc;
                 sorted_faces(p, K0, p0);
                 sorted_faces(p, K1, p1);
@@ -679,19 +690,27 @@ void or_pre_iterate(or_problem *p, double time, double tau)
             /* MassMatrix.h:46-54, 107-115 (edge: h_x = |K|), :160-176, 231-247
              * (quadrangle: h from vertices 0 and 2, an axis-aligned rectangle) */
             const int *cvK = p->cv + (long)K * p->nvc;
-            double hx, hy = 0;
+            double hx, hy = 0, hz = 0;
             if (dim == 1) hx = p->cell_meas[K];
             else {
-                hx = p->pts[(long)cvK[2] * 2] - p->pts[(long)cvK[0] * 2];
-                hy = p->pts[(long)cvK[2] * 2 + 1] - p->pts[(long)cvK[0] * 2 + 1];
+                /* the opposite corner: vertex 2 of a quadrangle, 6 of a hexahedron
+                 * (MassMatrix.h:168-172, 298-304) */
+                const long a = cvK[0], c = cvK[dim == 2 ? 2 : 6];
+                hx = p->pts[c * dim] - p->pts[a * dim];
+                hy = p->pts[c * dim + 1] - p->pts[a * dim + 1];
+                if (dim == 3) hz = p->pts[c * dim + 2] - p->pts[a * dim + 2];
             }
             for (int i = 0; i < n; i++)
                 for (int j = 0; j < n; j++) {
                     const double D = p->D[IJK(i, j, K)];
                     if (dim == 1) BST(i, j, K, 0) = 2 * D / hx;
-                    else {
+                    else if (dim == 2) {
                         BST(i, j, K, 0) = 2 * D * hx / hy;
                         BST(i, j, K, 1) = 2 * D * hy / hx;
+                    } else { /* MassMatrix.h:306-312 */
+                        BST(i, j, K, 0) = 2 * D * hy * hz / hx;
+                        BST(i, j, K, 1) = 2 * D * hx * hz / hy;
+                        BST(i, j, K, 2) = 2 * D * hx * hy / hz;
                     }
                 }
             continue;
@@ -837,7 +856,7 @@ static void boundary_row(or_problem *p, int E, int i, int row)
     const int K = p->face_cells[2L * E];
     const int *fi = p->cell_faces + (long)K * fpc;
     const int e = local_index(p, K, E);
-    int perm[4];
+    int perm[6];
     sorted_faces(p, K, perm);
     /* which coefficient set (BoundaryConditions.hpp:11-130) */
     int inflow_identity = 0, use_adv = 0;
@@ -877,7 +896,7 @@ static void interior_row(or_problem *p, int E, int i, int row)
     const int n = p->n, fpc = p->fpc;
     const int K0 = p->face_cells[2L * E], K1 = p->face_cells[2L * E + 1];
     const int *f0 = p->cell_faces + (long)K0 * fpc, *f1 = p->cell_faces + (long)K1 * fpc;
-    int p0[4], p1[4];
+    int p0[6], p1[6];
     sorted_faces(p, K0, p0);
     sorted_faces(p, K1, p1);
     const int e0 = local_index(p, K0, E), e1 = local_index(p, K1, E);

@@ -15,7 +15,7 @@
  *   m_iE_upw    [F][n]
  *   Z_ijE_upw   [F][n(j)][n(i)]
  *   b_ijK_storage                 [n][n][S][C]   (S = 9 triangles, 14 tetrahedra,
- *                                                  1 edges, 2 rectangles)
+ *                                                  1 edges, 2 rectangles, 3 voxels)
  * Boundary-condition arrays use the (i,E) layout i*F + E
  * (BoundaryConditions.hpp:295, Stationary.h:22,36).
  *
@@ -49,7 +49,7 @@ int or_topology(int dim, int nv, const double *points, int nc, const int *cell_v
 or_problem *or_create(int dim, int nv, const double *points, int nc, const int *cell_vertices,
                       int n, int lumping, int advection, int do_mobility_upwind);
 /* cell shapes: simplices (dim+1 vertices) or boxes (2^dim vertices in VTK
- * order: edges and axis-aligned rectangles); dim 1 is always a box */
+ * order: edges, axis-aligned rectangles and voxels); dim 1 is always a box */
 enum { OR_SIMPLEX = 0, OR_BOX = 1 };
 int or_topology_shape(int dim, int shape, int nv, const double *points, int nc,
                       const int *cell_vertices, int *cell_faces, int *face_cells);

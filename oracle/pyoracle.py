@@ -68,8 +68,8 @@ def _ptr(a):
 
 
 def _shape(dim, vertices_per_cell):
-    """1 (box: edges, quadrangles) or 0 (simplex)."""
-    return 1 if dim == 1 or (dim == 2 and vertices_per_cell == 4) else 0
+    """1 (box: edges, quadrangles, hexahedra) or 0 (simplex)."""
+    return 1 if dim == 1 or vertices_per_cell == 2 ** dim else 0
 
 
 def topology(points, cells):

@@ -89,8 +89,8 @@ def vtu_text(points3, conn, offsets, types, cell_data, fmt="binary", header="UIn
 
 
 def vtk_cell_type(dim, nv):
-    """VTK_LINE 3, VTK_TRIANGLE 5, VTK_QUAD 9, VTK_TETRA 10."""
-    return {(1, 2): 3, (2, 3): 5, (2, 4): 9, (3, 4): 10}[(dim, nv)]
+    """VTK_LINE 3, VTK_TRIANGLE 5, VTK_QUAD 9, VTK_TETRA 10, VTK_HEXAHEDRON 12."""
+    return {(1, 2): 3, (2, 3): 5, (2, 4): 9, (3, 4): 10, (3, 8): 12}[(dim, nv)]
 
 
 def mesh_arrays(points, cells, int_t=np.int32):

@@ -86,6 +86,11 @@ CASES = {
     "edge_adr_n2_lumped": lambda: _uneven_boxes(problems.adr_1d(s=40, n=2, lumping=1)),
     "edge_adr_n3_mixed_tags_trace": lambda: _mixed_tags(dataclasses.replace(
         _uneven_boxes(problems.adr_1d(s=30, n=3)), advection=problems.IMPLICIT_TRACE)),
+    "hex_adr_n1_lumped": lambda: _uneven_boxes(problems.adr_hexes(s=4, n=1, lumping=1)),
+    "hex_adr_n2": lambda: _uneven_boxes(problems.adr_hexes(s=4, n=2)),
+    "hex_adr_n2_implicit_upwind": lambda: dataclasses.replace(_uneven_boxes(problems.adr_hexes(s=3, n=2)),
+                                                              advection=problems.IMPLICIT_UPWIND),
+    "hex_adr_n3_mixed_tags": lambda: _mixed_tags(_uneven_boxes(problems.adr_hexes(s=3, n=3))),
 }
 
 
@@ -242,13 +247,12 @@ def test_nonconvergence_raises():
 
 
 def test_unsupported_cells_rejected():
-    """Hexahedra and unknown cell shapes are refused before any device work."""
+    """Unknown cell shapes are refused before any device work."""
     pts = np.zeros((8, 3))
     cells = np.arange(8, dtype=np.int32)[None, :]
     mesh = capi.MeshDesc(3, 8, 1, pts.ctypes.data, cells.ctypes.data)
     model = capi.ModelDesc(1, 0, 0, 1)
     L = capi.lib()
-    assert not L.mhfem_create_cells(mesh, 8, model)
-    assert b"hexahedra are not supported" in L.mhfem_last_error(None)
-    assert not L.mhfem_create_cells(mesh, 5, model)
-    assert b"not supported" in L.mhfem_last_error(None)
+    for vpc in (5, 6, 3):
+        assert not L.mhfem_create_cells(mesh, vpc, model)
+        assert b"are not supported" in L.mhfem_last_error(None)

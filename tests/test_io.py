@@ -33,7 +33,7 @@ def tet_mesh():
 
 
 MESHES = {"tri": tri_mesh, "tet": tet_mesh, "quad": lambda: meshgen.unit_square_quads(3),
-          "edge": lambda: meshgen.unit_interval(7)}
+          "edge": lambda: meshgen.unit_interval(7), "hex": lambda: meshgen.unit_cube_hexes(2)}
 
 
 VTU_CASES = {
@@ -160,7 +160,7 @@ def test_vtu_writer_follows_the_format(tmp_path, fmt, mesh):
     nv = cells.shape[1]
     np.testing.assert_array_equal(cl["connectivity"], cells.ravel())
     np.testing.assert_array_equal(cl["offsets"], nv * np.arange(1, C + 1))
-    np.testing.assert_array_equal(cl["types"], np.full(C, {(1, 2): 3, (2, 3): 5, (2, 4): 9, (3, 4): 10}[
+    np.testing.assert_array_equal(cl["types"], np.full(C, {(1, 2): 3, (2, 3): 5, (2, 4): 9, (3, 4): 10, (3, 8): 12}[
         (points.shape[1], nv)]))
     np.testing.assert_array_equal(cdata["Z"], z)  # %.17g text round-trips exactly
     np.testing.assert_array_equal(cdata["w & <ratio>"], w)

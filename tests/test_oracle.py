@@ -166,11 +166,12 @@ def box_mass(P, lumping):
     2-point Gauss quadrature per axis, or vertex (trapezoidal) quadrature when
     lumped.  Returns (M, |K|)."""
     d = P.shape[1]
-    lo, hi = P[0], P[2] if d == 2 else P[1]
+    lo, hi = P[0], P[{1: 1, 2: 2, 3: 6}[d]]
     h = hi - lo
     vol = float(np.prod(np.abs(h)))
     # local faces: (axis, the face's side: +1 at hi, -1 at lo)
-    faces = [(0, -1), (0, +1)] if d == 1 else [(1, -1), (0, +1), (1, +1), (0, -1)]
+    faces = {1: [(0, -1), (0, +1)], 2: [(1, -1), (0, +1), (1, +1), (0, -1)],
+             3: [(2, -1), (1, -1), (0, +1), (1, +1), (0, -1), (2, +1)]}[d]
 
     def w(e, x):
         a, side = faces[e]
@@ -199,7 +200,7 @@ def box_b_from_storage(dim, lumping, st):
     """The full b_ijKef of one (i,j,K) from its box storage (MassMatrix.h:57-71,
     118-132, 190-205, 250-276)."""
     fpc = 2 * dim
-    d = [0, 0] if dim == 1 else [0, 1, 0, 1]
+    d = {1: [0, 0], 2: [0, 1, 0, 1], 3: [2, 1, 0, 1, 0, 2]}[dim]
     B = np.zeros((fpc, fpc))
     for e in range(fpc):
         for f in range(fpc):
@@ -216,20 +217,21 @@ def _box_mesh(dim, s, rng):
         pts, cells = meshgen.unit_interval(s)
         xs = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, s - 1)), [1.0]])
         return xs[:, None], cells
-    pts, cells = meshgen.unit_square_quads(s)
-    xs = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, s - 1)), [1.0]])
-    ys = np.concatenate([[0.0], np.sort(rng.uniform(0, 0.7, s - 1)), [0.7]])
-    n1 = s + 1
-    pts = np.stack([np.tile(xs, n1), np.repeat(ys, n1)], axis=1)
+    pts, cells = meshgen.unit_square_quads(s) if dim == 2 else meshgen.unit_cube_hexes(s)
+    lines = [np.concatenate([[0.0], np.sort(rng.uniform(0, L, s - 1)), [L]]) for L in (1.0, 0.7, 1.3)[:dim]]
+    idx = np.rint(pts * s).astype(int)
+    pts = np.stack([lines[a][idx[:, a]] for a in range(dim)], axis=1)
     return pts, cells
 
 
-@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("dim", [1, 2, 3])
 @pytest.mark.parametrize("lumping", [0, 1])
 def test_box_mass_matrix_matches_quadrature(dim, lumping):
     rng = np.random.default_rng(40 + dim + 2 * lumping)
-    pr = problems.adr_quads(s=3, n=2, lumping=lumping) if dim == 2 else problems.adr_1d(s=4, n=2, lumping=lumping)
-    pts, cells = _box_mesh(dim, 3 if dim == 2 else 4, rng)
+    pr = {1: lambda: problems.adr_1d(s=4, n=2, lumping=lumping),
+          2: lambda: problems.adr_quads(s=3, n=2, lumping=lumping),
+          3: lambda: problems.adr_hexes(s=2, n=2, lumping=lumping)}[dim]()
+    pts, cells = _box_mesh(dim, {1: 4, 2: 3, 3: 2}[dim], rng)
     cf, fc = meshgen.face_numbering(cells, pts.shape[0], dim)
     C = cells.shape[0]
     D = rng.uniform(0.1, 10.0, (2, 2, C))
@@ -267,6 +269,8 @@ def _face_measure(pr, K, e):
         return 1.0
     if pr.dim == 2:
         return np.linalg.norm(verts[1] - verts[0])
+    if len(verts) == 4:  # a hexahedron's rectangle
+        return np.linalg.norm(verts[1] - verts[0]) * np.linalg.norm(verts[3] - verts[0])
     return 0.5 * np.linalg.norm(np.cross(verts[1] - verts[0], verts[2] - verts[0]))
 
 
@@ -384,13 +388,14 @@ def _tiny_problem(kind, seed):
         inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
         pts[inner] += 0.05 * rng.standard_normal((inner.sum(), 2))
         pr = dataclasses.replace(pr, points=pts)
-    elif kind.startswith(("quad", "edge")):
-        dim = 2 if kind.startswith("quad") else 1
+    elif kind.startswith(("quad", "edge", "hex")):
+        dim = {"edge": 1, "quad": 2, "hex_": 3}[kind[:4]]
         n = int(kind[-1])
         lumping = int("lumped" in kind)
-        pr = (problems.adr_quads(s=3, n=n, lumping=lumping) if dim == 2
-              else problems.adr_1d(s=5, n=n, lumping=lumping))
-        pts, cells = _box_mesh(dim, 3 if dim == 2 else 5, rng)
+        pr = {1: lambda: problems.adr_1d(s=5, n=n, lumping=lumping),
+              2: lambda: problems.adr_quads(s=3, n=n, lumping=lumping),
+              3: lambda: problems.adr_hexes(s=2, n=n, lumping=lumping)}[dim]()
+        pts, cells = _box_mesh(dim, {1: 5, 2: 3, 3: 2}[dim], rng)
         cf, fc = meshgen.face_numbering(cells, pts.shape[0], dim)
         pr = dataclasses.replace(pr, points=pts, cells=cells, cell_faces=cf, face_cells=fc,
                                  steps=2)
@@ -425,7 +430,7 @@ def _tiny_problem(kind, seed):
 
 @pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2", "adr_n3", "tri_adr_n2",
                                   "quad_n1", "quad_n2", "quad_lumped_n3", "edge_n1", "edge_n2",
-                                  "edge_lumped_n2"])
+                                  "edge_lumped_n2", "hex_n1", "hex_n2", "hex_lumped_n2"])
 def test_brute_force_mixed_system(kind):
     pr = _tiny_problem(kind, 3)
     o = pyoracle.Oracle(pr)

@@ -23,11 +23,14 @@ thread_local std::string g_create_error;
 // the "main" element of Solver.hpp:670-671).  Local face e of a simplex is
 // the face opposite local vertex e (MassMatrix.h:538,661,787,932); the faces
 // of an edge are its vertices, those of a quadrangle its edges (0,1), (1,2),
-// (2,3), (3,0) (TNL's subentity order, on which the face directions of
-// MassMatrix.h:178-184 rest).
+// (2,3), (3,0), those of a hexahedron (VTK vertex order) (0,1,2,3),
+// (0,1,5,4), (1,2,6,5), (2,3,7,6), (3,0,4,7), (4,5,6,7) (TNL's subentity
+// order, on which the face directions of MassMatrix.h:178-184, 430-436 rest).
 int face_vertices(int dim, int box, int e, int *lv)
 {
     static const int quad[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    static const int hex[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                  {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
     if (!box) {
         int t = 0;
         for (int v = 0; v <= dim; v++)
@@ -38,14 +41,21 @@ int face_vertices(int dim, int box, int e, int *lv)
         lv[0] = e;
         return 1;
     }
-    lv[0] = quad[e][0];
-    lv[1] = quad[e][1];
-    return 2;
+    if (dim == 2) {
+        lv[0] = quad[e][0];
+        lv[1] = quad[e][1];
+        return 2;
+    }
+    for (int t = 0; t < 4; t++) lv[t] = hex[e][t];
+    return 4;
 }
 
 struct FaceKey {
-    int v[3];
-    bool operator==(const FaceKey &o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+    int v[4];
+    bool operator==(const FaceKey &o) const
+    {
+        return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2] && v[3] == o.v[3];
+    }
 };
 struct FaceKeyHash {
     size_t operator()(const FaceKey &k) const
@@ -53,6 +63,7 @@ struct FaceKeyHash {
         uint64_t h = (uint64_t)(uint32_t)k.v[0] * 0x9E3779B97F4A7C15ull;
         h ^= (uint64_t)(uint32_t)k.v[1] + 0x7F4A7C159E3779B9ull + (h << 6) + (h >> 2);
         h ^= (uint64_t)(uint32_t)k.v[2] * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)k.v[3] * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
         return (size_t)(h ^ (h >> 31));
     }
 };
@@ -72,8 +83,8 @@ void build_topology(Ctx &c, const int *cv)
             if (vid < 0 || vid >= c.nv) throw Error{MHFEM_E_MESH, "cell vertex index out of range"};
         }
         for (int e = 0; e < fpc; e++) {
-            FaceKey key{{-1, -1, -1}};
-            int lv[3];
+            FaceKey key{{-1, -1, -1, -1}};
+            int lv[4];
             const int t = face_vertices(c.dim, c.box, e, lv);
             for (int q = 0; q < t; q++) key.v[q] = cv[(size_t)K * c.nvc + lv[q]];
             std::sort(key.v, key.v + t);
@@ -116,12 +127,25 @@ double simplex_measure(int dim, const double *P)
 }
 
 // measure of a box cell: an edge's length; a quadrangle's area as half the
-// cross product of its diagonals
+// cross product of its diagonals; a voxel's volume from vertices 0 and 6
 double box_measure(int dim, const double *P)
 {
     if (dim == 1) return std::fabs(P[1] - P[0]);
+    if (dim == 3) return std::fabs((P[18] - P[0]) * (P[19] - P[1]) * (P[20] - P[2]));
     const double d1x = P[4] - P[0], d1y = P[5] - P[1], d2x = P[6] - P[2], d2y = P[7] - P[3];
     return 0.5 * std::fabs(d1x * d2y - d1y * d2x);
+}
+
+// a hexahedron's face (a planar quadrangle in 3-D): half the cross product of its diagonals
+double quad_facet_measure(const double *P)
+{
+    double a[3], b[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = P[6 + k] - P[k];
+        b[k] = P[9 + k] - P[3 + k];
+    }
+    const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
 }
 
 double facet_measure(int dim, const double *P)
@@ -153,28 +177,35 @@ int local_index(const Ctx &c, int K, int E)
 }
 
 // Per-face column order of the hybridised rows: slot t names the local face
-// of K0 (bits 6t..6t+2) and of K1 (bits 6t+3..6t+5), 7 = absent; bits 48-50
-// and 51-53 hold the local index of the face itself in K0 / K1, bits 56-59
-// the number of slots.  Interior rows merge the two sorted face lists
-// (LinearSystem.h:181-245), boundary rows list the sorted faces of the one
-// cell (BoundaryConditions.hpp:196-218).
-std::vector<uint64_t> build_face_slots(const Ctx &c)
+// of K0 (f0[t]) and of K1 (f1[t]), 7 = absent, and e0 / e1 are the local
+// indices of the face itself in K0 / K1.  Interior rows merge the two sorted
+// face lists (LinearSystem.h:181-245), boundary rows list the sorted faces of
+// the one cell (BoundaryConditions.hpp:196-218).
+struct FaceSlots {
+    int8_t f0[12], f1[12];
+    int ns, e0, e1;
+};
+
+std::vector<FaceSlots> build_face_slots(const Ctx &c)
 {
-    std::vector<uint64_t> slots(c.F);
+    std::vector<FaceSlots> slots(c.F);
     const int fpc = c.fpc;
     for (int E = 0; E < c.F; E++) {
         const int K0 = c.h_face_cells[2 * (size_t)E], K1 = c.h_face_cells[2 * (size_t)E + 1];
-        uint64_t info = 0;
-        int p0[4], p1[4], ns = 0;
+        FaceSlots &fs = slots[E];
+        fs.ns = 0;
+        fs.e1 = 0;
+        int p0[6], p1[6];
         sorted_local(c, K0, p0);
         const int *f0 = &c.h_cell_faces[(size_t)K0 * fpc];
-        auto put = [&](int t, int a, int b) {
-            info |= (uint64_t)(a & 7) << (6 * t);
-            info |= (uint64_t)(b & 7) << (6 * t + 3);
+        auto put = [&](int a, int b) {
+            fs.f0[fs.ns] = (int8_t)a;
+            fs.f1[fs.ns] = (int8_t)b;
+            fs.ns++;
         };
+        fs.e0 = local_index(c, K0, E);
         if (K1 < 0) {
-            for (int g = 0; g < fpc; g++) put(ns++, p0[g], 7);
-            info |= (uint64_t)local_index(c, K0, E) << 48;
+            for (int g = 0; g < fpc; g++) put(p0[g], 7);
         } else {
             sorted_local(c, K1, p1);
             const int *f1 = &c.h_cell_faces[(size_t)K1 * fpc];
@@ -182,40 +213,51 @@ std::vector<uint64_t> build_face_slots(const Ctx &c)
             while (g0 < fpc || g1 < fpc) {
                 const int F0 = g0 < fpc ? f0[p0[g0]] : INT32_MAX, F1 = g1 < fpc ? f1[p1[g1]] : INT32_MAX;
                 const int Fm = std::min(F0, F1);
-                put(ns++, F0 == Fm ? p0[g0] : 7, F1 == Fm ? p1[g1] : 7);
+                put(F0 == Fm ? p0[g0] : 7, F1 == Fm ? p1[g1] : 7);
                 if (F0 == Fm) g0++;
                 if (F1 == Fm) g1++;
             }
-            info |= (uint64_t)local_index(c, K0, E) << 48;
-            info |= (uint64_t)local_index(c, K1, E) << 51;
+            fs.e1 = local_index(c, K1, E);
         }
-        info |= (uint64_t)ns << 56;
-        slots[E] = info;
     }
     return slots;
 }
 
-// Store map of the assembly, per face E: the block index, in the SELL row of
-// E, of the column face at local index f of K0 (bits 3f..3f+2) and of K1
-// (bits 12+3f..): the slots of the merged column order, counted without the
-// slot of E itself (E has no stored block; its field is 0, unused); the
-// local index of E in K0 (bits 24-26) and K1 (bits 27-29, 0 on the boundary).
-std::vector<uint32_t> build_face_blocks(const std::vector<uint64_t> &slots)
+// The per-face word the upwind kernel reads: the local index of the face in
+// K0 (bits 48-50) and K1 (bits 51-53).
+std::vector<uint64_t> face_slot_words(const std::vector<FaceSlots> &slots)
 {
-    std::vector<uint32_t> out(slots.size());
+    std::vector<uint64_t> out(slots.size());
+    for (size_t E = 0; E < slots.size(); E++) out[E] = (uint64_t)slots[E].e0 << 48 | (uint64_t)slots[E].e1 << 51;
+    return out;
+}
+
+// Store map of the assembly, per face E: the block index, in the SELL row of
+// E, of the column face at local index f of K0 and of K1 (the slots of the
+// merged column order, counted without the slot of E itself, which has no
+// stored block; its field is 0, unused), and the local index of E in K0 and
+// K1 (0 on the boundary).  Up to four faces per cell, one 32-bit word: fields
+// of 3 bits at 3f (K0) and 12+3f (K1), local indices at bits 24 and 27.  Six
+// faces (hexahedra), one 64-bit word: fields of 4 bits at 4f and 24+4f, local
+// indices at bits 48 and 51.  The top bit is always clear.
+std::vector<uint32_t> build_face_blocks(const std::vector<FaceSlots> &slots, int fpc)
+{
+    const bool wide = fpc > 4;
+    const int fb = wide ? 4 : 3, cb = wide ? 24 : 12, eb = wide ? 48 : 24;
+    std::vector<uint32_t> out(slots.size() * (wide ? 2 : 1));
     for (size_t E = 0; E < slots.size(); E++) {
-        const uint64_t info = slots[E];
-        const int e0 = (info >> 48) & 7, e1 = (info >> 51) & 7, ns = (info >> 56) & 15;
-        uint32_t w = (uint32_t)e0 << 24 | (uint32_t)e1 << 27;
+        const FaceSlots &fs = slots[E];
+        uint64_t w = (uint64_t)fs.e0 << eb | (uint64_t)fs.e1 << (eb + 3);
         int k = 0;
-        for (int t = 0; t < ns; t++) {
-            const int f0 = (info >> (6 * t)) & 7, f1 = (info >> (6 * t + 3)) & 7;
-            if (f0 == e0) continue;
-            if (f0 < 4) w |= (uint32_t)k << (3 * f0);
-            if (f1 < 4) w |= (uint32_t)k << (12 + 3 * f1);
+        for (int t = 0; t < fs.ns; t++) {
+            const int f0 = fs.f0[t], f1 = fs.f1[t];
+            if (f0 == fs.e0) continue;
+            if (f0 < fpc) w |= (uint64_t)k << (fb * f0);
+            if (f1 < fpc) w |= (uint64_t)k << (cb + fb * f1);
             k++;
         }
-        out[E] = w;
+        if (wide) std::memcpy(&out[2 * E], &w, 8);
+        else out[E] = (uint32_t)w;
     }
     return out;
 }
@@ -354,10 +396,7 @@ mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell
             throw Error{MHFEM_E_ARGUMENT, "mhfem_create: null argument"};
         const int dim = mesh->dim, vpc = vertices_per_cell;
         const bool simplex = (dim == 2 && vpc == 3) || (dim == 3 && vpc == 4);
-        const bool box = (dim == 1 && vpc == 2) || (dim == 2 && vpc == 4);
-        if (dim == 3 && vpc == 8)
-            throw Error{MHFEM_E_ARGUMENT, "mhfem_create: hexahedra are not supported (edges, triangles, "
-                                          "quadrangles and tetrahedra are)"};
+        const bool box = (dim == 1 && vpc == 2) || (dim == 2 && vpc == 4) || (dim == 3 && vpc == 8);
         if (!simplex && !box)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_create: cells of " + std::to_string(vpc) + " vertices in " +
                                               std::to_string(dim) + "-D are not supported"};
@@ -390,7 +429,7 @@ mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell
         // measures
         std::vector<double> cm(Cn), fm(c->F, 0.0);
         for (size_t K = 0; K < Cn; K++) {
-            double P[12];
+            double P[24];
             for (size_t k = 0; k < nvc; k++)
                 for (int d = 0; d < c->dim; d++)
                     P[k * c->dim + d] = mesh->points[(size_t)mesh->cell_vertices[K * nvc + k] * c->dim + d];
@@ -398,21 +437,22 @@ mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell
             for (size_t e = 0; e < fpc; e++) {
                 const int E = c->h_cell_faces[K * fpc + e];
                 if (c->h_face_cells[2 * (size_t)E] != (int)K) continue;
-                double Q[9];
-                int lv[3];
+                double Q[12];
+                int lv[4];
                 const int t = face_vertices(c->dim, c->box, (int)e, lv);
                 for (int q = 0; q < t; q++)
                     for (int d = 0; d < c->dim; d++) Q[q * c->dim + d] = P[lv[q] * c->dim + d];
-                fm[E] = facet_measure(c->dim, Q);
+                fm[E] = t == 4 ? quad_facet_measure(Q) : facet_measure(c->dim, Q);
             }
         }
         upload(c->cell_faces, c->h_cell_faces.data(), c->h_cell_faces.size());
         upload(c->face_cells, c->h_face_cells.data(), c->h_face_cells.size());
         upload(c->cell_vertices, mesh->cell_vertices, Cn * nvc);
         upload(c->points, mesh->points, (size_t)c->nv * c->dim);
-        std::vector<uint64_t> slots = build_face_slots(*c);
-        upload(c->face_slots, slots.data(), slots.size());
-        const std::vector<uint32_t> fblocks = build_face_blocks(slots);
+        const std::vector<FaceSlots> slots = build_face_slots(*c);
+        const std::vector<uint64_t> words = face_slot_words(slots);
+        upload(c->face_slots, words.data(), words.size());
+        const std::vector<uint32_t> fblocks = build_face_blocks(slots, c->fpc);
         upload(c->face_blocks, fblocks.data(), fblocks.size());
         std::vector<int> bfaces;
         for (int E = 0; E < c->F; E++)
@@ -709,7 +749,7 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         c.h_col.assign(c.nnz, 0);
         for (int E = 0; E < c.F; E++) {
             const int K0 = c.h_face_cells[2 * (size_t)E], K1 = c.h_face_cells[2 * (size_t)E + 1];
-            int p0[4], p1[4];
+            int p0[6], p1[6];
             sorted_local(c, K0, p0);
             const int *f0 = &c.h_cell_faces[(size_t)K0 * fpc];
             for (int i = 0; i < n; i++) {

@@ -1,7 +1,7 @@
 // io.cpp -- the files on either side of the time-step path (host code only).
 //
-//  * Mesh readers for unstructured grids of triangles or tetrahedra with their
-//    cell data: VTK XML (.vtu; ascii, inline base64 with or without zlib
+//  * Mesh readers for unstructured grids of lines, triangles, quadrangles,
+//    tetrahedra or hexahedra with their cell data: VTK XML (.vtu; ascii, inline base64 with or without zlib
 //    compression, raw or base64 appended data) and legacy VTK (.vtk; ASCII or
 //    BINARY, the classic CELLS layout and the OFFSETS/CONNECTIVITY layout of
 //    file version 5).  They take the place of TNL::Meshes::loadMesh
@@ -595,10 +595,11 @@ CellShape cell_shape(int64_t vtk_type)
     case 5: return {2, 3};    // VTK_TRIANGLE
     case 9: return {2, 4};    // VTK_QUAD
     case 10: return {3, 4};   // VTK_TETRA
+    case 12: return {3, 8};   // VTK_HEXAHEDRON
     default: break;
     }
     fail("unsupported cell type " + std::to_string(vtk_type) +
-         " (lines, triangles, quadrangles and tetrahedra only)");
+         " (lines, triangles, quadrangles, tetrahedra and hexahedra only)");
 }
 
 // common checks once points, connectivity and types are known
@@ -1115,6 +1116,7 @@ int mhfem_write_vtu(const char *path, const mhfem_mesh_desc *mesh, int vertices_
         else if (dim == 2 && nv == 3) vtk_type = 5;
         else if (dim == 2 && nv == 4) vtk_type = 9;
         else if (dim == 3 && nv == 4) vtk_type = 10;
+        else if (dim == 3 && nv == 8) vtk_type = 12;
         else fail("unsupported cells: " + std::to_string(nv) + " vertices in " + std::to_string(dim) + "-D",
                   MHFEM_E_ARGUMENT);
         if (format < MHFEM_VTK_ASCII || format > MHFEM_VTK_ZLIB) fail("unknown VTU format", MHFEM_E_ARGUMENT);

@@ -279,38 +279,63 @@ __global__ void __launch_bounds__(WG) k_geometry(SchemeView s, double *minv_out,
     if constexpr (MS > FPC * FPC) o[FPC * FPC] = 0.0;
 }
 
-// The same for box cells (MassMatrix.h:28-303): an edge (h_x = |K|) or an
-// axis-aligned rectangle (h_x, h_y from local vertices 0 and 2).  Each face
-// direction d has one value, g_0 = 2/h_x for an edge, g_0 = 2 h_x/h_y for the
-// horizontal faces e = 0, 2 of a rectangle and g_1 = 2 h_y/h_x for its
-// vertical faces e = 1, 3; M^{-1} holds g_d on the diagonal when lumped, else
-// 2 g_d on the diagonal and g_d between the two faces of direction d, so
-// that its row sums are MassMatrix::b_ijKe (g_d or 3 g_d).
+// Box cells: the direction of local face e (the slot of MassMatrix::size it
+// reads) and the two faces of direction d.  Edges: one direction; quadrangles:
+// horizontal faces e = 0, 2 and vertical e = 1, 3 (MassMatrix.h:178-184);
+// hexahedra: n_x faces e = 2, 4, n_y e = 1, 3, n_z e = 0, 5 (MassMatrix.h:430-436).
+template <int FPC>
+__host__ __device__ constexpr int box_dir(int e)
+{
+    return FPC == 2 ? 0 : FPC == 4 ? e % 2 : (e == 2 || e == 4) ? 0 : (e == 1 || e == 3) ? 1 : 2;
+}
+template <int FPC>
+__host__ __device__ constexpr int box_face(int d, int second)
+{
+    return FPC == 2 ? second : FPC == 4 ? d + 2 * second : (d == 0 ? 2 + 2 * second : d == 1 ? 1 + 2 * second : 5 * second);
+}
+
+// The same for box cells (MassMatrix.h:28-508): an edge (h_x = |K|), an
+// axis-aligned rectangle (h_x, h_y from local vertices 0 and 2) or a voxel
+// (h_x, h_y, h_z from local vertices 0 and 6).  Each face direction d has one
+// value: g = 2/h_x for an edge; 2 h_x/h_y (horizontal faces) and 2 h_y/h_x
+// (vertical faces) for a rectangle; 2 h_y h_z/h_x, 2 h_x h_z/h_y and
+// 2 h_x h_y/h_z (n_x, n_y, n_z faces) for a voxel.  M^{-1} holds g_d on the
+// diagonal when lumped, else 2 g_d on the diagonal and g_d between the two
+// faces of direction d, so that its row sums are MassMatrix::b_ijKe (g_d or
+// 3 g_d).
 template <int FPC>
 __global__ void __launch_bounds__(WG) k_geometry_box(SchemeView s, double *minv_out, double *rows_out)
 {
-    constexpr int DIM = FPC / 2, NP = Simplex<FPC>::NP, MS = MinvLayout<FPC>::MS;
+    constexpr int DIM = FPC / 2, NVC = 1 << DIM, NP = Simplex<FPC>::NP, MS = MinvLayout<FPC>::MS;
     const int K = blockIdx.x * WG + threadIdx.x;
     if (K >= s.C) return;
     double g[DIM];
     if constexpr (DIM == 1) {
         g[0] = 2.0 / s.cell_meas[K];
     } else {
-        const int v0 = s.cell_vertices[(long)K * 4], v2 = s.cell_vertices[(long)K * 4 + 2];
-        const double hx = s.points[(long)v2 * 2] - s.points[(long)v0 * 2];
-        const double hy = s.points[(long)v2 * 2 + 1] - s.points[(long)v0 * 2 + 1];
-        g[0] = 2.0 * hx / hy;
-        g[1] = 2.0 * hy / hx;
+        // the opposite corner: vertex 2 of a rectangle, 6 of a voxel
+        const int v0 = s.cell_vertices[(long)K * NVC], vc = s.cell_vertices[(long)K * NVC + (DIM == 2 ? 2 : 6)];
+        double h[DIM];
+#pragma unroll
+        for (int a = 0; a < DIM; a++) h[a] = s.points[(long)vc * DIM + a] - s.points[(long)v0 * DIM + a];
+        if constexpr (DIM == 2) {
+            g[0] = 2.0 * h[0] / h[1];
+            g[1] = 2.0 * h[1] / h[0];
+        } else {
+            g[0] = 2.0 * h[1] * h[2] / h[0];
+            g[1] = 2.0 * h[0] * h[2] / h[1];
+            g[2] = 2.0 * h[0] * h[1] / h[2];
+        }
     }
     double minv[NP];
 #pragma unroll
     for (int f = 0; f < FPC; f++)
 #pragma unroll
         for (int e = 0; e <= f; e++) {
-            const int d = e % DIM;
+            const int d = box_dir<FPC>(e);
             double v = 0.0;
             if (e == f) v = s.lumping ? g[d] : 2.0 * g[d];
-            else if (f % DIM == d && !s.lumping) v = g[d];
+            else if (box_dir<FPC>(f) == d && !s.lumping) v = g[d];
             minv[pk(e, f)] = v;
         }
 #pragma unroll
@@ -339,8 +364,9 @@ __global__ void __launch_bounds__(WG) k_fill_b_box(SchemeView s, double *b)
             const double D = s.D[((long)i * N + j) * C + K];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                // g_d: the lumped diagonal, or the entry between faces d and d + DIM
-                const double g = s.minv[(long)(s.lumping ? pk(d, d) : pk(d, d + DIM)) * C + K];
+                // g_d: the lumped diagonal, or the entry between the two faces of direction d
+                const int e0 = box_face<FPC>(d, 0), e1 = box_face<FPC>(d, 1);
+                const double g = s.minv[(long)(s.lumping ? pk(e0, e0) : pk(e0, e1)) * C + K];
                 b[((long)(i * N + j) * DIM + d) * C + K] = b_entry(g, D);
             }
         }
@@ -729,6 +755,47 @@ __device__ __forceinline__ void store_block_row(const SchemeView &s, int E, int 
     st<double, N>(s.A.val + sell_block(s.A.slice_ptr, E, k) * (N * N) + i * N, p);
 }
 
+// The assembly's per-face store map (built on the host): a 32-bit word for up
+// to four faces per cell (block-index fields of 3 bits, K1's at bit 12, the
+// local indices of the face at bits 24 and 27), a 64-bit word for six
+// (fields of 4 bits, K1's at bit 24, local indices at bits 48 and 51).  The
+// top bit is always clear.
+template <int FPC>
+struct FaceMap {
+    using T = uint32_t;
+    static constexpr int FB = 3, CB = 12, EB = 24;
+};
+template <>
+struct FaceMap<6> {
+    using T = uint64_t;
+    static constexpr int FB = 4, CB = 24, EB = 48;
+};
+template <int FPC>
+__device__ __forceinline__ typename FaceMap<FPC>::T face_map(const SchemeView &s, int E)
+{
+    return reinterpret_cast<const typename FaceMap<FPC>::T *>(s.face_blocks)[E];
+}
+// block index of local face f of cell c (0: K0, 1: K1) in the SELL row of the face
+template <int FPC>
+__device__ __forceinline__ int map_block(typename FaceMap<FPC>::T map, int c, int f)
+{
+    using M = FaceMap<FPC>;
+    return (int)(map >> (M::CB * c + M::FB * f)) & ((1 << M::FB) - 1);
+}
+// local index of the face in cell c
+template <int FPC>
+__device__ __forceinline__ int map_local(typename FaceMap<FPC>::T map, int c)
+{
+    return (int)(map >> (FaceMap<FPC>::EB + 3 * c)) & 7;
+}
+// the top bit of the map as a sign bit (always clear; read so that the exit
+// test waits for the map load, see k_assemble)
+template <int FPC>
+__device__ __forceinline__ int map_top(typename FaceMap<FPC>::T map)
+{
+    return (int)((uint32_t)(map >> (8 * sizeof(typename FaceMap<FPC>::T) - 32)) & 0x80000000u);
+}
+
 // Boundary row (E, i): BoundaryConditions::setMatrixElements
 // (BoundaryConditions.hpp:335-508), one lane per row.  The row's tag selects
 // the form of its cell's contribution (BoundaryCoefficients_*,
@@ -747,8 +814,8 @@ __device__ __forceinline__ void boundary_row(const SchemeView &s, int q, int i)
         return;
     }
     const int K0 = s.face_cells[2 * E];
-    const uint32_t map = s.face_blocks[E];
-    const int e0 = (map >> 24) & 7;
+    const auto map = face_map<FPC>(s, E);
+    const int e0 = map_local<FPC>(map, 0);
     const int adv = s.advection;
     const long C = s.C;
     const bool use_adv = tag == MHFEM_BC_FIXED_FLUX;
@@ -768,7 +835,7 @@ __device__ __forceinline__ void boundary_row(const SchemeView &s, int q, int i)
         double v[N];
 #pragma unroll
         for (int j = 0; j < N; j++) v[j] = (inflow ? 0.0 : r.A[j][f]) / d;
-        store_block_row<N>(s, E, (map >> (3 * f)) & 7, i, v);
+        store_block_row<N>(s, E, map_block<FPC>(map, 0, f), i, v);
     }
 #pragma unroll
     for (int j = 0; j < N; j++)
@@ -796,9 +863,8 @@ __device__ __forceinline__ void boundary_row(const SchemeView &s, int q, int i)
 // right-hand sides with a lane shuffle; a sum of two terms is the same in
 // either order, so both form the identical scale d.  Few registers per lane
 // keep many waves in flight to cover the gathers of the cell data.
-// face_blocks[E] gives the block index of every local face of K0 (bits
-// 3f..3f+2) and K1 (12+3f..) in the SELL row of E, and the local index of E
-// in K0 (bits 24-26) and K1 (27-29).
+// The face map gives the block index of every local face of K0 and K1 in the
+// SELL row of E, and the local index of E in K0 and K1.
 template <int N, int FPC>
 __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
 {
@@ -820,11 +886,11 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
     // exit test reads the map too (its bit 31 is always clear), so neither
     // load waits behind the branch.
     const uint64_t cells = reinterpret_cast<const uint64_t *>(s.face_cells)[E];
-    const uint32_t map = s.face_blocks[E];
+    const auto map = face_map<FPC>(s, E);
     const int K0 = (int)(uint32_t)cells, K1 = (int)(uint32_t)(cells >> 32);
-    if ((K1 | (int)(map & 0x80000000u)) < 0) return;  // boundary faces: the boundary rows
+    if ((K1 | map_top<FPC>(map)) < 0) return;  // boundary faces: the boundary rows
     const int K = c ? K1 : K0;
-    const int e = (map >> (24 + 3 * c)) & 7;
+    const int e = map_local<FPC>(map, c);
     const int adv = s.advection;
     const bool use_adv = adv != MHFEM_EXPLICIT_UPWIND;  // SecondaryCoefficients.h:261-278
     CellData<N, FPC> cd;
@@ -844,7 +910,7 @@ __global__ void __launch_bounds__(WG) k_assemble(SchemeView s)
         double v[N];
 #pragma unroll
         for (int j = 0; j < N; j++) v[j] = r.A[j][f] / d;
-        store_block_row<N>(s, E, (map >> (12 * c + 3 * f)) & 7, i, v);
+        store_block_row<N>(s, E, map_block<FPC>(map, c, f), i, v);
     }
     if (c == 0) {
         // the diagonal entry itself is exactly 1 (LinearSystem.h:196-201) and is not stored
@@ -946,6 +1012,10 @@ static void dispatch(Ctx &c, Args &&...args);
             if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 2>), g_, BS, 0, (c).stream, __VA_ARGS__); \
             else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 2>), g_, BS, 0, (c).stream, __VA_ARGS__); \
             else hipLaunchKernelGGL((KERNEL<3, 2>), g_, BS, 0, (c).stream, __VA_ARGS__);           \
+        } else if ((c).fpc == 6) {                                                                  \
+            if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 6>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 6>), g_, BS, 0, (c).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<3, 6>), g_, BS, 0, (c).stream, __VA_ARGS__);           \
         } else {                                                                                    \
             if ((c).n == 1) hipLaunchKernelGGL((KERNEL<1, 4>), g_, BS, 0, (c).stream, __VA_ARGS__); \
             else if ((c).n == 2) hipLaunchKernelGGL((KERNEL<2, 4>), g_, BS, 0, (c).stream, __VA_ARGS__); \
@@ -991,6 +1061,7 @@ void launch_geometry(Ctx &c)
 {
     SchemeView s = c.view();
     if (c.box && c.fpc == 2) hipLaunchKernelGGL(k_geometry_box<2>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
+    else if (c.box && c.fpc == 6) hipLaunchKernelGGL(k_geometry_box<6>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
     else if (c.box) hipLaunchKernelGGL(k_geometry_box<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
     else if (c.fpc == 3) hipLaunchKernelGGL(k_geometry<3>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);
     else hipLaunchKernelGGL(k_geometry<4>, blocks(c.C), WG, 0, c.stream, s, c.minv.p, c.minv_rows.p);

@@ -5,8 +5,10 @@ on (the reference reads VTK meshes through TNL; its example meshes are Git-LFS
 stubs).  Vertex order inside each cell is fixed here because the scheme
 depends on it: local face ``e`` of a simplex is the face opposite local
 vertex ``e`` (reference MassMatrix.h:538,661,787,932); the faces of an edge
-are its vertices and those of a quadrangle its edges (0,1), (1,2), (2,3),
-(3,0), with vertices counter-clockwise from the lower-left one.
+are its vertices, those of a quadrangle its edges (0,1), (1,2), (2,3), (3,0),
+with vertices counter-clockwise from the lower-left one, and those of a
+hexahedron (VTK vertex order) (0,1,2,3), (0,1,5,4), (1,2,6,5), (2,3,7,6),
+(3,0,4,7), (4,5,6,7).
 """
 from __future__ import annotations
 
@@ -58,9 +60,27 @@ def unit_interval(s: int):
     return points, cells
 
 
+def unit_cube_hexes(s: int):
+    """Unit cube, ``s^3`` cubes as hexahedron cells.
+
+    Vertex ``(ix, iy, iz)`` has index ``(iz*(s+1)+iy)*(s+1)+ix``; cells are
+    visited with ``ix`` fastest, vertices in VTK_HEXAHEDRON order: the bottom
+    face counter-clockwise from the lower-left corner, then the top face.
+    """
+    n1 = s + 1
+    iz, iy, ix = np.meshgrid(np.arange(n1), np.arange(n1), np.arange(n1), indexing="ij")
+    points = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(np.float64) / s
+    qz, qy, qx = np.meshgrid(np.arange(s), np.arange(s), np.arange(s), indexing="ij")
+    v = ((qz * n1 + qy) * n1 + qx).ravel()
+    up = n1 * n1
+    bottom = [v, v + 1, v + 1 + n1, v + n1]
+    cells = np.stack(bottom + [b + up for b in bottom], axis=1).astype(np.int32)
+    return points, cells
+
+
 def is_box(dim: int, vertices_per_cell: int) -> bool:
-    """Edges and quadrangles are box cells; triangles and tetrahedra simplices."""
-    return dim == 1 or (dim == 2 and vertices_per_cell == 4)
+    """Edges, quadrangles and hexahedra are box cells; triangles and tetrahedra simplices."""
+    return dim == 1 or vertices_per_cell == 2 ** dim
 
 
 def faces_per_cell(dim: int, vertices_per_cell: int) -> int:
@@ -71,8 +91,10 @@ def face_vertex_table(dim: int, vertices_per_cell: int):
     """Local vertex positions of every local face."""
     if dim == 1:
         return [[0], [1]]
-    if is_box(dim, vertices_per_cell):
+    if is_box(dim, vertices_per_cell) and dim == 2:
         return [[0, 1], [1, 2], [2, 3], [3, 0]]
+    if is_box(dim, vertices_per_cell):
+        return [[0, 1, 2, 3], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7], [4, 5, 6, 7]]
     return [[v for v in range(dim + 1) if v != e] for e in range(dim + 1)]
 
 
@@ -180,6 +202,13 @@ def outward_face_vectors(points, cells):
         # |E| n on the faces of an axis-aligned box: -y, +x, +y, -x (an edge: -x, +x)
         if dim == 1:
             return np.stack([-np.ones(nc), np.ones(nc)], axis=1)[:, :, None]
+        if dim == 3:
+            # faces -z, -y, +x, +y, -x, +z
+            h = P[:, 6] - P[:, 0]
+            out = np.zeros((nc, 6, 3))
+            for e, (a, sign) in enumerate([(2, -1), (1, -1), (0, 1), (1, 1), (0, -1), (2, 1)]):
+                out[:, e, a] = sign * np.prod(np.delete(h, a, axis=1), axis=1)
+            return out
         hx = P[:, 2, 0] - P[:, 0, 0]
         hy = P[:, 2, 1] - P[:, 0, 1]
         out = np.zeros((nc, 4, 2))

@@ -64,7 +64,7 @@ class Problem:
 
     @property
     def box(self):
-        """Edge or quadrangle cells (else triangles or tetrahedra)."""
+        """Edge, quadrangle or hexahedron cells (else triangles or tetrahedra)."""
         return meshgen.is_box(self.dim, self.cells.shape[1])
 
     @property
@@ -285,6 +285,28 @@ def adr_quads(s: int = 16, n: int = 2, dt: float = 1e-3, steps: int = 3,
     fcen = meshgen.face_centroids(points, cells, cf, F)
     tags, dirichlet, _ = _adr_boundary(n, fc, fcen)
     return Problem("adr_quads", points, cells, n, coeffs, tags=tags.ravel(),
+                   bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
+                   Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
+                   preconditioner=preconditioner, lumping=lumping, do_mobility_upwind=1,
+                   bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc)
+
+
+def adr_hexes(s: int = 6, n: int = 2, dt: float = 1e-3, steps: int = 3,
+              preconditioner: str = "diagonal", lumping: int = 0, seed: int = 1) -> Problem:
+    """The coupled ADR model of adr_3d on ``s^3`` cube (voxel) cells
+    (MassMatrix.h:305-508): ln K the seeded Gaussian field, advection
+    (1, .5, .25) as face fluxes, the same boundary pattern."""
+    points, cells = meshgen.unit_cube_hexes(s)
+    cf, fc = _topology(points, cells)
+    F, C = fc.shape[0], cells.shape[0]
+    g = meshgen.gaussian_field((s, s, s), 1.0, 0.1, seed)  # [ix, iy, iz]
+    K = np.exp(np.transpose(g, (2, 1, 0)).ravel())
+    fv = meshgen.outward_face_vectors(points, cells)  # (C, 6, 3)
+    flux = (fv * np.array([1.0, 0.5, 0.25])).sum(axis=2)
+    coeffs = _adr_coefficients(n, C, K, flux)
+    fcen = meshgen.face_centroids(points, cells, cf, F)
+    tags, dirichlet, _ = _adr_boundary(n, fc, fcen)
+    return Problem("adr_hexes", points, cells, n, coeffs, tags=tags.ravel(),
                    bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
                    Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
                    preconditioner=preconditioner, lumping=lumping, do_mobility_upwind=1,
