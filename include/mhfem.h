@@ -88,7 +88,14 @@ typedef struct {
 
 /* Linear solver selection (configSetup.h:31-62). */
 enum { MHFEM_SOLVER_BICGSTAB = 0, MHFEM_SOLVER_GMRES = 1 };
-enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2 };
+/* MHFEM_PREC_AMG: algebraic multigrid in the configuration of the
+ * reference's Ginkgo branch (Solver.hpp:1025-1075): parallel graph matching
+ * aggregation, one V-cycle with 2 (coarsest: 4) Richardson sweeps of relaxation
+ * 0.9 preconditioned by block Jacobi (blocks of at most n rows), rebuilt when
+ * the last solve took more than min(2u, u + 5) iterations, u those right after
+ * the last rebuild (Solver.hpp:1017, 1158-1160).  FP64 solvers, single
+ * systems, the four per-step calls (not mhfem_step). */
+enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2, MHFEM_PREC_AMG = 3 };
 /* precision of the Krylov iteration.  FP32_REFINED: FP64 iterative
    refinement whose every correction is an FP32 BiCGStab solve started from
    zero to `inner_residue`.  FP32_RELIABLE: one FP32 BiCGStab recurrence whose
@@ -325,6 +332,13 @@ int mhfem_save_linear_system(mhfem_ctx *ctx, const char *directory);
  * `reps` launches on the context's stream; precision selects the FP64 or the
  * FP32 copy of the matrix.  Measurement aid, no counterpart in the reference. */
 int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms);
+/* AMG inspection (no counterpart in the reference): levels of the hierarchy
+ * and rebuilds so far; rows, nonzeros and the aggregate of every row
+ * (rows entries, NULL allowed, none on the coarsest level) of one level; one
+ * V-cycle x = M^{-1} b on host vectors of the system's length. */
+int mhfem_amg_info(const mhfem_ctx *ctx, int *levels, int *rebuilds);
+int mhfem_amg_level(const mhfem_ctx *ctx, int level, int *rows, long *nnz, int32_t *aggregates);
+int mhfem_amg_apply(mhfem_ctx *ctx, const double *b, double *x);
 int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out);
 int mhfem_reset_timers(mhfem_ctx *ctx);
 int mhfem_set_timing(mhfem_ctx *ctx, int level); /* 0 off (default), 1 HIP-event phase timers,

@@ -57,6 +57,8 @@ public:
         if (prec == "none") cfg_.preconditioner = MHFEM_PREC_NONE;
         else if (prec == "diagonal") cfg_.preconditioner = MHFEM_PREC_DIAGONAL;
         else if (prec == "ilu0") cfg_.preconditioner = MHFEM_PREC_ILU0;
+        // the reference's Ginkgo "AMGX" preconditioner (Solver.hpp:1025-1075)
+        else if (prec == "amg" || prec == "AMGX") cfg_.preconditioner = MHFEM_PREC_AMG;
         else return false;
         cfg_.convergence_residue = p.getParameter<double>("convergence-residue", 1e-6);
         cfg_.max_iterations = p.getParameter<int>("max-iterations", 100000);

@@ -719,7 +719,11 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
         if (cfg->solver < 0 || cfg->solver > 1) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
-        if (cfg->preconditioner < 0 || cfg->preconditioner > 2) throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
+        if (cfg->preconditioner < 0 || cfg->preconditioner > 3) throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
+        if (cfg->preconditioner == MHFEM_PREC_AMG && cfg->precision != MHFEM_PRECISION_FP64)
+            throw Error{MHFEM_E_ARGUMENT, "the AMG preconditioner works with the FP64 solvers"};
+        if (cfg->preconditioner == MHFEM_PREC_AMG && c.h_seg_faces.size() > 2)
+            throw Error{MHFEM_E_ARGUMENT, "the AMG preconditioner is for single systems, not batches"};
         if (cfg->precision < 0 || cfg->precision > 2) throw Error{MHFEM_E_ARGUMENT, "unknown precision"};
         if (cfg->precision != MHFEM_PRECISION_FP64 && cfg->solver != MHFEM_SOLVER_BICGSTAB)
             throw Error{MHFEM_E_ARGUMENT, "FP32 refinement is implemented for BiCGStab only"};
@@ -1073,6 +1077,42 @@ int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms
     return guarded(ctx, [&](Ctx &c) {
         if (!c.system_ready || !avg_ms) throw Error{MHFEM_E_STATE, "linear system not set up"};
         krylov_bench_spmv(c, precision, reps, avg_ms);
+    });
+}
+
+int mhfem_amg_info(const mhfem_ctx *ctx, int *levels, int *rebuilds)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || !c->krylov || !krylov_amg(*c->krylov)) return MHFEM_E_STATE;
+    const Amg &a = *krylov_amg(*c->krylov);
+    if (levels) *levels = amg_num_levels(a);
+    if (rebuilds) *rebuilds = amg_rebuilds(a);
+    return MHFEM_OK;
+}
+
+int mhfem_amg_level(const mhfem_ctx *ctx, int level, int *rows, long *nnz, int32_t *aggregates)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || !c->krylov || !krylov_amg(*c->krylov)) return MHFEM_E_STATE;
+    const Amg &a = *krylov_amg(*c->krylov);
+    if (level < 0 || level >= amg_num_levels(a)) return MHFEM_E_ARGUMENT;
+    amg_level_info(a, level, rows, nnz, aggregates);
+    return MHFEM_OK;
+}
+
+int mhfem_amg_apply(mhfem_ctx *ctx, const double *b, double *x)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.krylov || !krylov_amg(*c.krylov)) throw Error{MHFEM_E_STATE, "no AMG preconditioner"};
+        Amg &a = *krylov_amg(*c.krylov);
+        if (amg_num_levels(a) == 0) throw Error{MHFEM_E_STATE, "the AMG hierarchy is built by the first solve"};
+        DBuf<double> db, dx;
+        db.alloc(c.rows);
+        dx.alloc(c.rows);
+        MHFEM_HIP(hipMemcpyAsync(db.p, b, c.rows * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        amg_apply(c, a, db.p, dx.p, nullptr);
+        MHFEM_HIP(hipMemcpyAsync(x, dx.p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
     });
 }
 

@@ -24,6 +24,15 @@ struct Error {
 // sets the message mhfem_last_error(NULL) returns (calls without a context)
 void set_global_error(const std::string &msg);
 
+// device scalar slots of the Krylov solver (per segment)
+enum Slot {
+    S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BETA, S_RAST_SQ, S_BNORM, S_RESID, S_TOL, S_IT,
+    S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, S_TOLABS, S_DELTA, NSLOTS = 32
+};
+// a segment's solve has stopped (converged, failed or waiting for a restart):
+// kernels queued behind it return at once
+__device__ __forceinline__ bool halted(const double *sc) { return sc[S_DONE] != 0.0 || sc[S_RESTART] != 0.0; }
+
 #define MHFEM_HIP(call)                                                                    \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \
@@ -183,6 +192,7 @@ struct DBuf {
 struct Krylov;  // krylov.hip
 struct Ilu0;    // krylov.hip
 struct Guess;   // guess.hip
+struct Amg;     // amg.hip
 
 struct Ctx {
     int dim = 0, fpc = 0, S = 0, n = 0;
@@ -317,6 +327,19 @@ constexpr int HIST_SLOTS = 16;
 void spmm_hist(Ctx &c, const double *H, double *AH, int slots);
 void residual64(Ctx &c, const double *x, const double *b, double *r);
 double dot64(Ctx &c, const double *x, const double *y);
+// the assembled matrix's values in CSR order (unit diagonal included) into dst[nnz]
+void launch_csr_gather(Ctx &c, double *dst);
+
+Amg *krylov_amg(const Krylov &k);  // the AMG preconditioner of a solver, or nullptr
+// amg.hip: algebraic multigrid preconditioner (FP64)
+Amg *amg_create();
+void amg_destroy(Amg *a);
+void amg_update(Ctx &c, Amg &a);
+void amg_apply(Ctx &c, Amg &a, const double *x, double *y, const double *halt_flags);
+void amg_after_solve(Amg &a, long long iterations);
+int amg_num_levels(const Amg &a);
+int amg_rebuilds(const Amg &a);
+void amg_level_info(const Amg &a, int level, int *rows, long *nnz, int32_t *aggregates);
 
 // guess.hip
 Guess *guess_create(Ctx &c);

@@ -35,11 +35,6 @@ namespace mhfem {
 constexpr int KB = 256;
 constexpr int MAXG = 2048;
 
-// device scalar slots (per segment)
-enum Slot {
-    S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BETA, S_RAST_SQ, S_BNORM, S_RESID, S_TOL, S_IT,
-    S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, S_TOLABS, S_DELTA, NSLOTS = 32
-};
 enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT, FIN_REPLACE };
 // which segments a kernel works on
 enum Mode { M_ALL = 0, M_ACTIVE = 1, M_RESTART = 2 };
@@ -54,7 +49,6 @@ struct Seg {
     double *partials;  // [ND][nblk]
 };
 
-__device__ __forceinline__ bool halted(const double *sc) { return sc[S_DONE] != 0.0 || sc[S_RESTART] != 0.0; }
 
 __device__ __forceinline__ bool seg_selected(const double *sc, int mode)
 {
@@ -686,6 +680,7 @@ struct Krylov {
     double *h_sc_ahead[2] = {nullptr, nullptr};  // pinned, nseg * NSLOTS each
     hipEvent_t ev_ahead[2] = {nullptr, nullptr};
     Ilu0 ilu;
+    Amg *amg = nullptr;
     // FP32 inner solver
     DBuf<float> val32, dinv32, b32, x32, r32, r_ast32, p32, Ap32, s32, As32, tmp32, rhs32;
     // GMRES
@@ -788,6 +783,7 @@ Krylov *krylov_create(Ctx &c)
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
     }
+    if (c.cfg.preconditioner == MHFEM_PREC_AMG) k->amg = amg_create();
     if (c.cfg.precision != MHFEM_PRECISION_FP64) {
         k->val32.alloc(c.sell_size);
         for (DBuf<float> *b : {&k->dinv32, &k->b32, &k->x32, &k->r32, &k->r_ast32, &k->p32, &k->Ap32, &k->s32,
@@ -808,6 +804,7 @@ static void drop_graphs(Krylov &k)
 
 void krylov_destroy(Krylov *k)
 {
+    if (k) amg_destroy(k->amg);
     if (!k) return;
     drop_graphs(*k);
     if (k->h_sc) (void)hipHostFree(k->h_sc);
@@ -958,10 +955,22 @@ struct Solve {
         }
         MHFEM_HIP(hipGetLastError());
     }
+    // one V-cycle (FP64 only; the setup rejects AMG with an FP32 solver)
+    template <typename T>
+    void amg_apply_t(const T *x, T *y, bool skip_when_halted)
+    {
+        if constexpr (sizeof(T) == 8) {
+            amg_apply(c, *k.amg, x, y, (skip_when_halted && k.nseg == 1) ? k.sc.p : nullptr);
+        } else {
+            (void)x, (void)y, (void)skip_when_halted;
+            throw Error{MHFEM_E_STATE, "the AMG preconditioner works in FP64"};
+        }
+    }
     template <typename T>
     void prec(const T *x, T *y, const T *dinv, int mode)
     {
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) ilu_apply<T>(x, y, false);
+        if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(x, y, false);
+        else if (c.cfg.preconditioner == MHFEM_PREC_ILU0) ilu_apply<T>(x, y, false);
         else hipLaunchKernelGGL((k_sscale<T>), k.nblk, KB, 0, c.stream, sg, mode, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
         MHFEM_HIP(hipGetLastError());
     }
@@ -971,9 +980,10 @@ struct Solve {
     template <typename T, int ND>
     void pmatvec(const T *x, T *y, T *tmp, const T *dinv, const T *z, int fin)
     {
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || c.cfg.preconditioner == MHFEM_PREC_AMG) {
             sspmv<T, false, 0>(x, (const T *)nullptr, tmp, (const T *)nullptr, M_ACTIVE);
-            ilu_apply<T>(tmp, y, true);
+            if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(tmp, y, true);
+            else ilu_apply<T>(tmp, y, true);
             if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, M_ACTIVE);
             if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, M_ACTIVE);
         } else {
@@ -992,9 +1002,10 @@ struct Solve {
     void presidue(const T *x, const T *b, T *r, T *tmp, const T *dinv, int mode, T *scratch = nullptr)
     {
         sresidual<T>(x, b, tmp, mode);
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU0 && mode != M_ALL) {
-            if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0) needs a scratch vector"};
-            ilu_apply<T>(tmp, scratch, false);
+        if ((c.cfg.preconditioner == MHFEM_PREC_ILU0 || c.cfg.preconditioner == MHFEM_PREC_AMG) && mode != M_ALL) {
+            if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0) or AMG needs a scratch vector"};
+            if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(tmp, scratch, false);
+            else ilu_apply<T>(tmp, scratch, false);
             hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, (const T *)scratch, r, (T *)nullptr);
             MHFEM_HIP(hipGetLastError());
             return;
@@ -1289,6 +1300,7 @@ static void precond_update(Ctx &c, Krylov &k)
         hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)n, k.dinv.p, k.dinv32.p);
         MHFEM_HIP(hipGetLastError());
     }
+    if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_update(c, *k.amg);
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
         Ilu0 &I = k.ilu;
         hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p,
@@ -1302,6 +1314,15 @@ static void precond_update(Ctx &c, Krylov &k)
             hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, I.lu.p, I.lu32.p);
         MHFEM_HIP(hipGetLastError());
     }
+}
+
+Amg *krylov_amg(const Krylov &k) { return k.amg; }
+
+void launch_csr_gather(Ctx &c, double *dst)
+{
+    hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p, c.csr_to_sell.p,
+                       dst);
+    MHFEM_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -1611,6 +1632,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         out.converged = res < tol;
     }
     if (c.guess && c.guess_on_solve) guess_after_solve(c, *c.guess, x);
+    if (k.amg) amg_after_solve(*k.amg, out.iterations);
     if (c.timing) {
         MHFEM_HIP(hipEventRecord(e2, c.stream));
         MHFEM_HIP(hipEventSynchronize(e2));
@@ -1647,6 +1669,9 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
     Krylov &k = *c.krylov;
     if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64)
         throw Error{MHFEM_E_ARGUMENT, "graph steps use FP64 BiCGStab"};
+    if (c.cfg.preconditioner == MHFEM_PREC_AMG)
+        throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG preconditioner (its setup runs on the host); "
+                                      "use the four calls"};
     const int every = std::max(1, c.cfg.check_every);
     if (k.g_first && (k.g_tau != tau || k.g_iters != every)) drop_graphs(k);
     Solve S(c, k);

@@ -22,7 +22,7 @@ ARRAYS = ["Z_iF", "Z_iK", "N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe",
 ARRAY_ID = {name: k for k, name in enumerate(ARRAYS)}
 
 SOLVERS = {"bicgstab": 0, "gmres": 1}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3}
 PRECISIONS = {"fp64": 0, "fp32-refined": 1, "fp32-reliable": 2}
 
 
@@ -128,6 +128,9 @@ SIGNATURES = {
     "mhfem_save_vector": (_I, [C.c_char_p, _P, C.c_int64]),
     "mhfem_load_vector": (_L, [C.c_char_p, _P, C.c_int64]),
     "mhfem_save_linear_system": (_I, [_P, C.c_char_p]),
+    "mhfem_amg_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "mhfem_amg_level": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_L), _P]),
+    "mhfem_amg_apply": (_I, [_P, _P, _P]),
 }
 
 _lib = None
@@ -365,6 +368,27 @@ class Solver:
 
     def setupLinearSystem(self):
         self._check(lib().mhfem_setup_linear_system(self.ctx, C.byref(self.config)))
+
+    def amg_hierarchy(self):
+        """(rows, nnz, aggregates or None) of every level of the AMG preconditioner."""
+        lv, rb = C.c_int(), C.c_int()
+        self._check(lib().mhfem_amg_info(self.ctx, C.byref(lv), C.byref(rb)))
+        out = []
+        for l in range(lv.value):
+            rows, nnz = C.c_int(), C.c_long()
+            self._check(lib().mhfem_amg_level(self.ctx, l, C.byref(rows), C.byref(nnz), None))
+            agg = None
+            if l + 1 < lv.value:
+                agg = np.empty(rows.value, np.int32)
+                self._check(lib().mhfem_amg_level(self.ctx, l, None, None, _ptr(agg)))
+            out.append((rows.value, nnz.value, agg))
+        return out, rb.value
+
+    def amg_apply(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        self._check(lib().mhfem_amg_apply(self.ctx, _ptr(b), _ptr(x)))
+        return x
 
     def saveLinearSystem(self, directory):
         """Solver::saveLinearSystem (Solver.hpp:966-977): matrix.mtx, dof.vec.tnl
