@@ -331,6 +331,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-ilu", action="store_true")
+    ap.add_argument("--amg-steps", type=int, default=3,
+                    help="timed steps of the FP64 BiCGStab + AMG form (0 skips it)")
     ap.add_argument("--skip-ablation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the ensemble leg")
@@ -379,6 +381,8 @@ def main():
              "solver": dict(preconditioner="diagonal", precision="fp64", check_every=8, tol=1e-10)}
     plain_ilu = {"name": "FP64 BiCGStab + ILU(0)",
                  "solver": dict(preconditioner="ilu0", precision="fp64", check_every=4, tol=1e-10)}
+    plain_amg = {"name": "FP64 BiCGStab + AMG (PGM aggregation, block-Jacobi Richardson smoothing, V-cycle)",
+                 "solver": dict(preconditioner="amg", precision="fp64", check_every=4, tol=1e-10)}
     ablation = {"name": "chosen form without the network (previous increments only)",
                 "solver": chosen["solver"], "guess": dict(guess, train_steps=0, combine="history")}
     fp64_hist = {"name": "FP64 BiCGStab + Jacobi from the previous increments",
@@ -423,6 +427,30 @@ def main():
         res_ilu["sample"] = f"steps 2-{1 + args.ilu_steps} of the trajectory"
         log(f"[bench] plain ILU(0): {res_ilu}")
 
+    res_amg = None
+    if args.amg_steps > 0:
+        log(f"[bench] AMG form ({args.amg_steps} timed steps after 1 warmup step, which builds the hierarchy)")
+        from mhfem_amd import capi as _capi
+        sa = _capi.Solver(pr, **plain_amg["solver"])
+        t_setup = time.perf_counter()
+        sa.step(0.0, pr.dt)
+        sa.synchronize()
+        first_ms = 1e3 * (time.perf_counter() - t_setup)
+        t = pr.dt
+        its0 = len(sa.iterations)
+        t0 = time.perf_counter()
+        for _ in range(args.amg_steps):
+            sa.step(t, pr.dt)
+            t += pr.dt
+        sa.synchronize()
+        dta = time.perf_counter() - t0
+        levels, rebuilds = sa.amg_hierarchy()
+        res_amg = summary(plain_amg["name"], args.amg_steps, dta, sa.iterations[its0:])
+        res_amg.update(first_step_ms_with_setup=round(first_ms, 1), levels=len(levels),
+                       level_rows=[lv[0] for lv in levels[:6]], rebuilds=rebuilds)
+        sa.close()
+        log(f"[bench] AMG: {res_amg}")
+
     res_abl = res_h64 = z_h64 = None
     if not args.skip_ablation:
         log(f"[bench] ablation: {ablation['name']}")
@@ -465,6 +493,7 @@ def main():
         "dof_per_s": round(rows * value, 1),
         "plain": res_plain,
         "plain_ilu0": res_ilu,
+        "plain_amg": res_amg,
         "chosen_without_network": res_abl,
         "fp64_history": res_h64,
         "chosen_vs_plain": round(value / res_plain["steps_per_s"], 3) if res_plain else None,

@@ -16,12 +16,15 @@
 //     with the current matrix.
 // Ginkgo itself is not in the reference tree: the steps are restated from its
 // published PGM / block-Jacobi / multigrid algorithms.  The hierarchy is
-// built on the host (setup is rare); the V-cycle runs on the device, one
-// kernel per sweep, residual restriction and prolongation.
+// built on the host (setup is rare); the V-cycle runs on the device: per
+// sweep a residual (the solver's block-SELL product on the finest level, CSR
+// vector on the others) and a block update, then restriction and
+// prolongation kernels.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -33,7 +36,7 @@ constexpr int AB = 256;  // work-group size of the V-cycle kernels
 constexpr double OMEGA = 0.9;
 constexpr int MAX_LEVELS = 25, MIN_COARSE_ROWS = 4, PGM_ROUNDS = 15;
 constexpr double MAX_UNASSIGNED = 0.1;
-constexpr int PRE_SWEEPS = 2, COARSEST_SWEEPS = 4;
+constexpr int PRE_SWEEPS = 2, COARSEST_SWEEPS = 4;  // post-smoothing repeats the pre-smoother
 
 struct HostCsr {
     int n = 0;
@@ -283,46 +286,52 @@ __global__ void __launch_bounds__(AB) k_amg_bj_init(int nb, const int *__restric
     }
 }
 
-// x_new = x_old + omega B^{-1} (b - A x_old), block by block
-__global__ void __launch_bounds__(AB) k_amg_bj_sweep(int nb, const int *__restrict__ bptr,
-                                                     const double *__restrict__ binv, const int *__restrict__ rp,
-                                                     const int *__restrict__ col, const double *__restrict__ val,
-                                                     const double *__restrict__ b, const double *__restrict__ xo,
-                                                     double *__restrict__ xn, const double *sc)
+// r = b - A x on a coarse level: L lanes per row (CSR vector), partial sums
+// combined by a fixed butterfly
+template <int L>
+__global__ void __launch_bounds__(AB) k_amg_residual(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                                     const double *__restrict__ val, const double *__restrict__ x,
+                                                     const double *__restrict__ b, double *__restrict__ r,
+                                                     const double *sc)
+{
+    if (amg_halted(sc)) return;
+    const long t = (long)blockIdx.x * AB + threadIdx.x;
+    const int row = (int)(t / L), lane = (int)(t % L);
+    double s = 0.0;
+    if (row < n)
+        for (int q = rp[row] + lane; q < rp[row + 1]; q += L) s += val[q] * x[col[q]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+    if (row < n && lane == 0) r[row] = b[row] - s;
+}
+
+// x += omega B^{-1} r, block by block
+__global__ void __launch_bounds__(AB) k_amg_bj_update(int nb, const int *__restrict__ bptr,
+                                                      const double *__restrict__ binv, const double *__restrict__ r,
+                                                      double *__restrict__ x, const double *sc)
 {
     const int k = blockIdx.x * AB + threadIdx.x;
     if (k >= nb || amg_halted(sc)) return;
     const int r0 = bptr[k], bs = bptr[k + 1] - r0;
     const double *B = binv + (size_t)k * 9;
-    double res[3];
-    for (int r = 0; r < bs; r++) {
+    double v[3];
+    for (int i = 0; i < bs; i++) v[i] = r[r0 + i];
+    for (int i = 0; i < bs; i++) {
         double s = 0.0;
-        for (int q = rp[r0 + r]; q < rp[r0 + r + 1]; q++) s += val[q] * xo[col[q]];
-        res[r] = b[r0 + r] - s;
-    }
-    for (int r = 0; r < bs; r++) {
-        double s = 0.0;
-        for (int c = 0; c < bs; c++) s += B[r * 3 + c] * res[c];
-        xn[r0 + r] = xo[r0 + r] + OMEGA * s;
+        for (int c = 0; c < bs; c++) s += B[i * 3 + c] * v[c];
+        x[r0 + i] += OMEGA * s;
     }
 }
 
-// bc[I] = sum over the members i of I of (b - A x)_i
+// bc[I] = sum of r over the members of aggregate I
 __global__ void __launch_bounds__(AB) k_amg_restrict(int nc, const int *__restrict__ pt_ptr,
-                                                     const int *__restrict__ pt_idx, const int *__restrict__ rp,
-                                                     const int *__restrict__ col, const double *__restrict__ val,
-                                                     const double *__restrict__ b, const double *__restrict__ x,
+                                                     const int *__restrict__ pt_idx, const double *__restrict__ r,
                                                      double *__restrict__ bc, const double *sc)
 {
     const int I = blockIdx.x * AB + threadIdx.x;
     if (I >= nc || amg_halted(sc)) return;
     double acc = 0.0;
-    for (int m = pt_ptr[I]; m < pt_ptr[I + 1]; m++) {
-        const int i = pt_idx[m];
-        double s = 0.0;
-        for (int q = rp[i]; q < rp[i + 1]; q++) s += val[q] * x[col[q]];
-        acc += b[i] - s;
-    }
+    for (int m = pt_ptr[I]; m < pt_ptr[I + 1]; m++) acc += r[pt_idx[m]];
     bc[I] = acc;
 }
 
@@ -354,9 +363,8 @@ void check_level(const HostCsr &Ac, const std::vector<int> &agg, int nc, int nf)
 struct AmgLevel {
     int n = 0, nc = 0, nb = 0;
     long nnz = 0;
-    const int *rp = nullptr, *col = nullptr;  // level 0: the solver's CSR pattern
-    DBuf<int> rp_own, col_own, agg, pt_ptr, pt_idx, bptr;
-    DBuf<double> val, binv, b, x, t0, t1;
+    DBuf<int> rp, col, agg, pt_ptr, pt_idx, bptr;  // rp, col, val: coarse levels (the finest uses the solver's matrix)
+    DBuf<double> val, binv, b, x, r;
     std::vector<int> h_agg;  // for inspection
 };
 
@@ -370,31 +378,28 @@ struct Amg {
 Amg *amg_create() { return new Amg(); }
 void amg_destroy(Amg *a) { delete a; }
 
-// Fine-level values: the assembled matrix in CSR order (the diagonal is 1).
-static void gather_fine(Ctx &c, AmgLevel &f)
-{
-    const long nnz = c.nnz;
-    f.val.alloc(nnz);
-    launch_csr_gather(c, f.val.p);
-}
-
 static void build(Ctx &c, Amg &a)
 {
     a.L.clear();
     a.L.push_back(std::make_unique<AmgLevel>());
-    AmgLevel &f = *a.L[0];
-    f.n = c.rows;
-    f.nnz = c.nnz;
-    f.rp = c.rowptr.p;
-    f.col = c.csr_col.p;
-    gather_fine(c, f);
+    {
+        AmgLevel &f = *a.L[0];
+        f.n = c.rows;
+        f.nnz = c.nnz;
+    }
+    // the finest level's values in CSR order (unit diagonal included), for the host setup
     HostCsr A;
     A.n = c.rows;
     A.rp = c.h_rowptr;
     A.col = c.h_col;
     A.val.resize(c.nnz);
-    MHFEM_HIP(hipMemcpyAsync(A.val.data(), f.val.p, c.nnz * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    MHFEM_HIP(hipStreamSynchronize(c.stream));
+    {
+        DBuf<double> tmp;
+        tmp.alloc(std::max<long>(c.nnz, 1));
+        launch_csr_gather(c, tmp.p);
+        MHFEM_HIP(hipMemcpyAsync(A.val.data(), tmp.p, c.nnz * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+    }
     for (;;) {
         AmgLevel &lv = *a.L.back();
         std::vector<int> bptr;
@@ -409,8 +414,7 @@ static void build(Ctx &c, Amg &a)
         lv.binv.alloc(binv.size());
         MHFEM_HIP(hipMemcpy(lv.bptr.p, bptr.data(), bptr.size() * sizeof(int), hipMemcpyHostToDevice));
         MHFEM_HIP(hipMemcpy(lv.binv.p, binv.data(), binv.size() * sizeof(double), hipMemcpyHostToDevice));
-        lv.t0.alloc(A.n);
-        lv.t1.alloc(A.n);
+        lv.r.alloc(A.n);
         if ((int)a.L.size() >= MAX_LEVELS || A.n <= MIN_COARSE_ROWS) break;
         std::vector<int> agg;
         const int nc = pgm(A, agg);
@@ -437,14 +441,12 @@ static void build(Ctx &c, Amg &a)
         AmgLevel &cl = *a.L.back();
         cl.n = nc;
         cl.nnz = (long)Ac.col.size();
-        cl.rp_own.alloc(nc + 1);
-        cl.col_own.alloc(std::max<long>(cl.nnz, 1));
+        cl.rp.alloc(nc + 1);
+        cl.col.alloc(std::max<long>(cl.nnz, 1));
         cl.val.alloc(std::max<long>(cl.nnz, 1));
-        MHFEM_HIP(hipMemcpy(cl.rp_own.p, Ac.rp.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice));
-        MHFEM_HIP(hipMemcpy(cl.col_own.p, Ac.col.data(), cl.nnz * sizeof(int), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(cl.rp.p, Ac.rp.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(cl.col.p, Ac.col.data(), cl.nnz * sizeof(int), hipMemcpyHostToDevice));
         MHFEM_HIP(hipMemcpy(cl.val.p, Ac.val.data(), cl.nnz * sizeof(double), hipMemcpyHostToDevice));
-        cl.rp = cl.rp_own.p;
-        cl.col = cl.col_own.p;
         cl.b.alloc(nc);
         cl.x.alloc(nc);
         A = std::move(Ac);
@@ -453,13 +455,13 @@ static void build(Ctx &c, Amg &a)
 }
 
 // Preconditioner update before a solve (Solver.hpp:1012-1017): rebuild unless
-// the reuse rule holds; the finest level takes the current matrix either way.
+// the reuse rule holds.  The finest level's residuals always use the solver's
+// current matrix.
 void amg_update(Ctx &c, Amg &a)
 {
     const bool reuse = !a.L.empty() && a.last_iters <= std::min(2 * a.updated_iters, a.updated_iters + 5);
     a.rebuilt = !reuse;
-    if (reuse) launch_csr_gather(c, a.L[0]->val.p);
-    else build(c, a);
+    if (!reuse) build(c, a);
 }
 
 // after the solve (Solver.hpp:1158-1160)
@@ -469,36 +471,51 @@ void amg_after_solve(Amg &a, long long iterations)
     if (a.rebuilt || a.updated_iters == 0) a.updated_iters = iterations;
 }
 
-static void vcycle(Ctx &c, Amg &a, int l, const double *b, double *out, const double *sc)
+// r = b - A x on level l: the solver's block-SELL matrix on the finest level,
+// CSR vector on the coarse ones (lanes per row from the mean row length)
+static void level_residual(Ctx &c, AmgLevel &v, int l, const double *x, const double *b, double *r,
+                           const double *sc)
+{
+    if (l == 0) {
+        residual64_halt(c, x, b, r, sc);
+        return;
+    }
+    const long avg = v.nnz / std::max(1, v.n);
+    auto go = [&](auto LC) {
+        constexpr int L = decltype(LC)::value;
+        hipLaunchKernelGGL((k_amg_residual<L>), agrid((long)v.n * L), AB, 0, c.stream, v.n, v.rp.p, v.col.p, v.val.p,
+                           x, b, r, sc);
+    };
+    if (avg <= 6) go(std::integral_constant<int, 4>{});
+    else if (avg <= 12) go(std::integral_constant<int, 8>{});
+    else if (avg <= 24) go(std::integral_constant<int, 16>{});
+    else go(std::integral_constant<int, 32>{});
+    MHFEM_HIP(hipGetLastError());
+}
+
+// one V-cycle on level l from a zero guess: x = M_l^{-1} b (x and b distinct)
+static void vcycle(Ctx &c, Amg &a, int l, const double *b, double *x, const double *sc)
 {
     AmgLevel &v = *a.L[l];
     const bool last = l + 1 == (int)a.L.size();
-    hipLaunchKernelGGL(k_amg_bj_init, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, b, v.t0.p, sc);
+    auto sweep = [&]() {
+        level_residual(c, v, l, x, b, v.r.p, sc);
+        hipLaunchKernelGGL(k_amg_bj_update, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, v.r.p, x, sc);
+    };
+    // the first Richardson step from zero: x = omega B^{-1} b
+    hipLaunchKernelGGL(k_amg_bj_init, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, b, x, sc);
     if (last) {
-        // Richardson sweeps on the coarsest level: t0 -> t1 -> t0 -> out
-        double *bufs[2] = {v.t1.p, v.t0.p};
-        const double *cur = v.t0.p;
-        for (int s = 1; s < COARSEST_SWEEPS; s++) {
-            double *nx = s + 1 == COARSEST_SWEEPS ? out : bufs[(s - 1) & 1];
-            hipLaunchKernelGGL(k_amg_bj_sweep, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, v.rp, v.col,
-                               v.val.p, b, cur, nx, sc);
-            cur = nx;
-        }
+        for (int s = 1; s < COARSEST_SWEEPS; s++) sweep();
         MHFEM_HIP(hipGetLastError());
         return;
     }
-    static_assert(PRE_SWEEPS == 2, "the buffer rotation assumes two sweeps");
-    hipLaunchKernelGGL(k_amg_bj_sweep, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, v.rp, v.col, v.val.p,
-                       b, v.t0.p, v.t1.p, sc);
+    for (int s = 1; s < PRE_SWEEPS; s++) sweep();
     AmgLevel &w = *a.L[l + 1];
-    hipLaunchKernelGGL(k_amg_restrict, agrid(v.nc), AB, 0, c.stream, v.nc, v.pt_ptr.p, v.pt_idx.p, v.rp, v.col,
-                       v.val.p, b, v.t1.p, w.b.p, sc);
+    level_residual(c, v, l, x, b, v.r.p, sc);
+    hipLaunchKernelGGL(k_amg_restrict, agrid(v.nc), AB, 0, c.stream, v.nc, v.pt_ptr.p, v.pt_idx.p, v.r.p, w.b.p, sc);
     vcycle(c, a, l + 1, w.b.p, w.x.p, sc);
-    hipLaunchKernelGGL(k_amg_prolong, agrid(v.n), AB, 0, c.stream, v.n, v.agg.p, w.x.p, v.t1.p, sc);
-    hipLaunchKernelGGL(k_amg_bj_sweep, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, v.rp, v.col, v.val.p,
-                       b, v.t1.p, v.t0.p, sc);
-    hipLaunchKernelGGL(k_amg_bj_sweep, agrid(v.nb), AB, 0, c.stream, v.nb, v.bptr.p, v.binv.p, v.rp, v.col, v.val.p,
-                       b, v.t0.p, out, sc);
+    hipLaunchKernelGGL(k_amg_prolong, agrid(v.n), AB, 0, c.stream, v.n, v.agg.p, w.x.p, x, sc);
+    for (int s = 0; s < PRE_SWEEPS; s++) sweep();
     MHFEM_HIP(hipGetLastError());
 }
 
@@ -506,6 +523,7 @@ static void vcycle(Ctx &c, Amg &a, int l, const double *b, double *out, const do
 void amg_apply(Ctx &c, Amg &a, const double *x, double *y, const double *sc)
 {
     if (a.L.empty()) throw Error{MHFEM_E_STATE, "AMG hierarchy not built"};
+    if (x == y) throw Error{MHFEM_E_STATE, "AMG: the V-cycle needs distinct input and output vectors"};
     vcycle(c, a, 0, x, y, sc);
 }
 

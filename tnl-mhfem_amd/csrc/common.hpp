@@ -326,6 +326,8 @@ void spmv64(Ctx &c, const double *x, double *y);
 constexpr int HIST_SLOTS = 16;
 void spmm_hist(Ctx &c, const double *H, double *AH, int slots);
 void residual64(Ctx &c, const double *x, const double *b, double *r);
+// the same, skipped once the single segment's solve has halted (sc: its flags, or nullptr)
+void residual64_halt(Ctx &c, const double *x, const double *b, double *r, const double *sc);
 double dot64(Ctx &c, const double *x, const double *y);
 // the assembled matrix's values in CSR order (unit diagonal included) into dst[nnz]
 void launch_csr_gather(Ctx &c, double *dst);

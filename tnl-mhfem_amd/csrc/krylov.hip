@@ -450,6 +450,23 @@ __global__ void __launch_bounds__(KB) k_residual(SellView<T> A, const T *__restr
     }
 }
 
+// y = b - A x, skipped once the single segment has halted (the finest level
+// of the AMG V-cycle)
+template <int N>
+__global__ void __launch_bounds__(KB) k_residual_halt(SellView<double> A, const double *__restrict__ x,
+                                                      const double *__restrict__ b, double *__restrict__ y,
+                                                      const double *sc)
+{
+    if (sc && halted(sc)) return;
+    for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
+        Pack<double, N> v = sell_face<N>(A, x, f);
+        const Pack<double, N> bb = ld<double, N>(b + (int64_t)f * N);
+#pragma unroll
+        for (int i = 0; i < N; i++) v.v[i] = bb.v[i] - v.v[i];
+        st<double, N>(y + (int64_t)f * N, v);
+    }
+}
+
 template <typename T, int N>
 __global__ void __launch_bounds__(KB) k_spmv_plain(SellView<T> A, const T *__restrict__ x, T *__restrict__ y)
 {
@@ -1365,6 +1382,15 @@ void residual64(Ctx &c, const double *x, const double *b, double *r)
     Krylov &k = *c.krylov;
     Solve S(c, k);
     S.residual<double>(x, b, r);
+}
+
+void residual64_halt(Ctx &c, const double *x, const double *b, double *r, const double *sc)
+{
+    SellView<double> A{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, c.sell_val.p, c.diag_base};
+    with_n(c.n, [&](auto NC) {
+        hipLaunchKernelGGL((k_residual_halt<decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, A, x, b, r, sc);
+    });
+    MHFEM_HIP(hipGetLastError());
 }
 
 double dot64(Ctx &c, const double *x, const double *y)
