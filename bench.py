@@ -285,9 +285,9 @@ def cpu_baseline(pr, iters_per_step, sample_iters):
     L = pyoracle.lib()
     it, res = C.c_int(0), C.c_double(0)
 
-    def solve(maxit):
+    def solve(orc, maxit):
         t0 = time.perf_counter()
-        L.or_solve(o.p, pyoracle.SOLVERS["bicgstab"], pyoracle.PRECONDITIONERS["ilu0"], 1e-10, maxit, 30,
+        L.or_solve(orc.p, pyoracle.SOLVERS["bicgstab"], pyoracle.PRECONDITIONERS["ilu0"], 1e-10, maxit, 30,
                    C.byref(it), C.byref(res))
         return time.perf_counter() - t0
 
@@ -295,19 +295,46 @@ def cpu_baseline(pr, iters_per_step, sample_iters):
     o.pre_iterate(0.0, pr.dt)
     o.assemble(0.0, pr.dt)
     t_pre_asm = time.perf_counter() - t0
-    t1 = solve(1)                   # ILU(0) factorization + 1 iteration
-    t2 = solve(1 + sample_iters)    # ILU(0) factorization + 1 + sample_iters iterations
+    t1 = solve(o, 1)                   # ILU(0) factorization + 1 iteration
+    t2 = solve(o, 1 + sample_iters)    # ILU(0) factorization + 1 + sample_iters iterations
     t_it = (t2 - t1) / sample_iters
     t_factor = max(0.0, t1 - t_it)
     t0 = time.perf_counter()
     o.post_iterate(0.0, pr.dt)
     t_post = time.perf_counter() - t0
     step_s = t_pre_asm + t_factor + iters_per_step * t_it + t_post
+    # the same first step on one thread (fewer sampled iterations)
+    L.or_set_threads(1)
+    o1 = pyoracle.Oracle(pr)
+    t0 = time.perf_counter()
+    o1.pre_iterate(0.0, pr.dt)
+    o1.assemble(0.0, pr.dt)
+    t1_pre_asm = time.perf_counter() - t0
+    s1 = solve(o1, 1)
+    s2 = solve(o1, 1 + max(2, sample_iters // 4))
+    t1_it = (s2 - s1) / max(2, sample_iters // 4)
+    t1_factor = max(0.0, s1 - t1_it)
+    t0 = time.perf_counter()
+    o1.post_iterate(0.0, pr.dt)
+    t1_post = time.perf_counter() - t0
+    step1_s = t1_pre_asm + t1_factor + iters_per_step * t1_it + t1_post
+    L.or_set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(1.0 / step_s, 6),
         "unit": "time steps/s",
         "cores": threads,
         "kind": "port",
+        "cpu_model": model,
+        "one_thread": {"value": round(1.0 / step1_s, 6), "unit": "time steps/s",
+                       "sample": (f"the same first step on 1 thread: preIterate+assembly {t1_pre_asm:.2f} s, "
+                                  f"ILU(0) update {t1_factor:.2f} s, {max(2, sample_iters // 4)} iterations at "
+                                  f"{t1_it:.3f} s each, postIterate {t1_post:.2f} s")},
         "sample": (f"first step of the same workload on the C oracle (OpenMP, {threads} host threads; "
                    f"ILU(0) factorization and triangular solves sequential): preIterate+assembly "
                    f"{t_pre_asm:.2f} s, ILU(0) update {t_factor:.2f} s, {sample_iters} FP64 BiCGStab+ILU(0) "

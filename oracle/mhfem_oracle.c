@@ -15,6 +15,9 @@
 #include "mhfem_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -475,6 +478,16 @@ void or_destroy(or_problem *p)
 }
 
 const char *or_error(const or_problem *p) { return p ? p->err : "null problem"; }
+
+/* OpenMP threads of the parallel loops (the CPU baseline times 1 and many) */
+void or_set_threads(int threads)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+    (void)threads;
+#endif
+}
 int or_num_cells(const or_problem *p) { return p->nc; }
 int or_num_faces(const or_problem *p) { return p->nf; }
 

@@ -57,6 +57,8 @@ or_problem *or_create_shape(int dim, int shape, int nv, const double *points, in
                             const int *cell_vertices, int n, int lumping, int advection,
                             int do_mobility_upwind);
 void or_destroy(or_problem *p);
+/* OpenMP threads of the oracle's parallel loops */
+void or_set_threads(int threads);
 const char *or_error(const or_problem *p);
 
 int or_num_cells(const or_problem *p);

@@ -34,6 +34,7 @@ def lib():
             "or_topology_shape": (I, [I, I, I, P, I, P, P, P]),
             "or_create_shape": (P, [I, I, I, P, I, P, I, I, I, I]),
             "or_destroy": (None, [P]),
+            "or_set_threads": (None, [I]),
             "or_error": (C.c_char_p, [P]),
             "or_num_cells": (I, [P]),
             "or_num_faces": (I, [P]),
