@@ -834,6 +834,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
                     c2s[q] = sell_block(c.h_slice_ptr.data(), E, k) * nn + i * n + j;
                 }
             }
+        c.h_bptr = std::move(bptr);
+        c.h_bface = std::move(bface);
         upload(c.slice_ptr, c.h_slice_ptr.data(), c.h_slice_ptr.size());
         upload(c.slice_width, c.h_slice_width.data(), c.h_slice_width.size());
         upload(c.sell_col, scol.data(), scol.size());

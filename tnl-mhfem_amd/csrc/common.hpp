@@ -223,6 +223,8 @@ struct Ctx {
     std::vector<int> h_rowptr, h_col;
     std::vector<int64_t> h_slice_ptr;
     std::vector<int> h_slice_width;
+    std::vector<int64_t> h_bptr;  // per face: its stored blocks' column faces h_bface[h_bptr[E] ..]
+    std::vector<int> h_bface;
     DBuf<int64_t> slice_ptr;
     DBuf<int> slice_width, sell_col, rowptr, csr_col;
     DBuf<int64_t> csr_to_sell;  // CSR entry -> position in sell_val, -1 for the unit diagonal

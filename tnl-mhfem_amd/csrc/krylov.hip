@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 #include "common.hpp"
 
@@ -561,10 +562,12 @@ __global__ void __launch_bounds__(KB) k_spmm_hist(SellView<double> A, const doub
     }
 }
 
-// FP64 residual of the refinement loop, stored in FP32: r32 = (float)(b - A x)
+// FP64 residual of the refinement loop, stored in FP32: r32 = (float)(b - A x),
+// face f's rows at face iperm[f] of the FP32 iteration's order (identity if null)
 template <int N>
 __global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const double *__restrict__ x,
-                                                       const double *__restrict__ b, float *__restrict__ r32)
+                                                       const double *__restrict__ b, float *__restrict__ r32,
+                                                       const int *__restrict__ iperm)
 {
     for (int f = blockIdx.x * KB + threadIdx.x; f < A.faces; f += gridDim.x * KB) {
         const Pack<double, N> v = sell_face<N>(A, x, f);
@@ -572,7 +575,33 @@ __global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const
         Pack<float, N> r;
 #pragma unroll
         for (int i = 0; i < N; i++) r.v[i] = (float)(bb.v[i] - v.v[i]);
-        st<float, N>(r32 + (int64_t)f * N, r);
+        st<float, N>(r32 + (int64_t)(iperm ? iperm[f] : f) * N, r);
+    }
+}
+
+// The FP32 copy of the matrix in the gather-friendly face order: block b of
+// the permuted layout takes natural block map[b] (-1: padding, zero); the
+// within-face couplings of permuted face p are those of face perm[p].
+template <int N>
+__global__ void __launch_bounds__(KB) k_convert_permuted(int64_t blocks, const int64_t *__restrict__ map,
+                                                         const double *__restrict__ src, float *__restrict__ dst)
+{
+    constexpr int NN = N * N;
+    for (int64_t b = (int64_t)blockIdx.x * KB + threadIdx.x; b < blocks; b += (int64_t)gridDim.x * KB) {
+        const int64_t q = map[b];
+#pragma unroll
+        for (int t = 0; t < NN; t++) dst[b * NN + t] = q >= 0 ? (float)src[q * NN + t] : 0.0f;
+    }
+}
+template <int N>
+__global__ void __launch_bounds__(KB) k_convert_permuted_diag(int faces, const int *__restrict__ perm,
+                                                              const double *__restrict__ src, float *__restrict__ dst)
+{
+    constexpr int D = N * (N - 1);
+    for (int p = blockIdx.x * KB + threadIdx.x; p < faces; p += gridDim.x * KB) {
+        const int64_t e = perm[p];
+#pragma unroll
+        for (int t = 0; t < D; t++) dst[(int64_t)p * D + t] = (float)src[e * D + t];
     }
 }
 
@@ -588,6 +617,21 @@ __global__ void __launch_bounds__(KB) k_add_correction(int n, double *__restrict
     for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
         x[i] += (double)d[i];
         d[i] = 0.0f;
+    }
+}
+
+// the same with the FP32 iteration's face order: x[f] += d32[iperm[f]]
+template <int N>
+__global__ void __launch_bounds__(KB) k_add_correction_perm(int faces, double *__restrict__ x,
+                                                            float *__restrict__ d, const int *__restrict__ iperm)
+{
+    for (int f = blockIdx.x * KB + threadIdx.x; f < faces; f += gridDim.x * KB) {
+        const int64_t q = (int64_t)iperm[f] * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            x[(int64_t)f * N + i] += (double)d[q + i];
+            d[q + i] = 0.0f;
+        }
     }
 }
 
@@ -700,6 +744,12 @@ struct Krylov {
     Amg *amg = nullptr;
     // FP32 inner solver
     DBuf<float> val32, dinv32, b32, x32, r32, r_ast32, p32, Ap32, s32, As32, tmp32, rhs32;
+    // its matrix in a face order that groups faces with the same neighbour
+    // offsets (see build_fp32_order): perm[p] = natural face, iperm inverse
+    bool permuted = false;
+    DBuf<int> perm, iperm, slice_width32, col32;
+    DBuf<int64_t> slice_ptr32, map32;
+    int64_t blocks32 = 0, diag_base32 = 0;
     // GMRES
     DBuf<double> V;
     int V_m = 0;
@@ -743,6 +793,95 @@ static void build_blocks(Ctx &c, Krylov &k, int rpb)
     MHFEM_HIP(hipMemcpy(k.blk_row.p, brow.data(), (k.nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
     MHFEM_HIP(hipMemcpy(k.seg_blk.p, sblk.data(), (k.nseg + 1) * sizeof(int), hipMemcpyHostToDevice));
     k.partials.alloc((size_t)2 * k.nblk);
+}
+
+// Face order of the FP32 iteration's matrix copy.  The natural
+// (first-appearance) numbering interleaves the face types of a cell, so the
+// 64 faces of a slice gather x from 64 unrelated places.  Faces whose stored
+// blocks sit at the same offsets F - E (faces of one type away from the
+// boundary) are taken together, each group in natural order and the groups in
+// order of first appearance: the k-th column faces of consecutive faces of a
+// group are then consecutive faces of one other group, and a wavefront's
+// gathers fall into few cache lines.  The FP32 vectors live in this order;
+// the FP64 residual and correction kernels translate at the boundary.
+struct OffsetsHash {
+    size_t operator()(const std::vector<int> &v) const
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (int x : v) h = (h ^ (uint32_t)x) * 1099511628211ull;
+        return (size_t)h;
+    }
+};
+
+static void build_fp32_order(Ctx &c, Krylov &k)
+{
+    const int F = c.F, n = c.n, nn = n * n;
+    std::unordered_map<std::vector<int>, int, OffsetsHash> groups;
+    std::vector<int> gid(F);
+    std::vector<int> key;
+    for (int E = 0; E < F; E++) {
+        key.clear();
+        for (int64_t q = c.h_bptr[E]; q < c.h_bptr[E + 1]; q++) key.push_back(c.h_bface[q] - E);
+        auto it = groups.find(key);
+        if (it == groups.end()) it = groups.emplace(key, (int)groups.size()).first;
+        gid[E] = it->second;
+    }
+    // stable counting sort by group
+    const int ng = (int)groups.size();
+    std::vector<int> start(ng + 1, 0);
+    for (int E = 0; E < F; E++) start[gid[E] + 1]++;
+    for (int g = 0; g < ng; g++) start[g + 1] += start[g];
+    std::vector<int> perm(F), iperm(F);
+    for (int E = 0; E < F; E++) perm[start[gid[E]]++] = E;
+    for (int p = 0; p < F; p++) iperm[perm[p]] = p;
+    // block-SELL layout of the permuted matrix
+    const int nslices = (F + 63) / 64;
+    std::vector<int> w32(nslices, 0);
+    std::vector<int64_t> sp32(nslices + 1, 0);
+    for (int s = 0; s < nslices; s++) {
+        int w = 0;
+        for (int p = s * 64; p < std::min(F, s * 64 + 64); p++)
+            w = std::max<int>(w, (int)(c.h_bptr[perm[p] + 1] - c.h_bptr[perm[p]]));
+        w32[s] = w;
+        sp32[s + 1] = sp32[s] + (int64_t)w * 64;
+    }
+    const int64_t blocks = sp32[nslices];
+    std::vector<int> col(std::max<int64_t>(blocks, 1));
+    std::vector<int64_t> map(std::max<int64_t>(blocks, 1), -1);
+    for (int s = 0; s < nslices; s++)
+        for (int lane = 0; lane < 64; lane++) {
+            const int p = s * 64 + lane;
+            const int E = p < F ? perm[p] : -1;
+            const int64_t cnt = E >= 0 ? c.h_bptr[E + 1] - c.h_bptr[E] : 0;
+            for (int kk = 0; kk < w32[s]; kk++) {
+                const int64_t b = sell_block(sp32.data(), p, kk);
+                if (kk < cnt) {
+                    col[b] = iperm[c.h_bface[c.h_bptr[E] + kk]];
+                    map[b] = sell_block(c.h_slice_ptr.data(), E, kk);
+                } else {
+                    col[b] = p < F ? p : 0;
+                    map[b] = -1;
+                }
+            }
+        }
+    for (int64_t b = 0; b < blocks; b++)
+        if (col[b] < 0 || col[b] >= F || map[b] >= c.sell_blocks)
+            throw Error{MHFEM_E_STATE, "inconsistent FP32 matrix order"};
+    k.blocks32 = blocks;
+    k.diag_base32 = blocks * nn;
+    k.perm.alloc(F);
+    k.iperm.alloc(F);
+    k.slice_width32.alloc(nslices);
+    k.slice_ptr32.alloc(nslices + 1);
+    k.col32.alloc(col.size());
+    k.map32.alloc(map.size());
+    MHFEM_HIP(hipMemcpy(k.perm.p, perm.data(), F * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.iperm.p, iperm.data(), F * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.slice_width32.p, w32.data(), nslices * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.slice_ptr32.p, sp32.data(), (nslices + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.col32.p, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(k.map32.p, map.data(), map.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    k.permuted = true;
 }
 
 Krylov *krylov_create(Ctx &c)
@@ -802,7 +941,12 @@ Krylov *krylov_create(Ctx &c)
     }
     if (c.cfg.preconditioner == MHFEM_PREC_AMG) k->amg = amg_create();
     if (c.cfg.precision != MHFEM_PRECISION_FP64) {
-        k->val32.alloc(c.sell_size);
+        // the gather-friendly order needs a plain FP32 iteration: one system,
+        // the implicit unit-diagonal Jacobi (or none), the block layout
+        const bool reorder = k->nseg == 1 && c.unit_diagonal && c.cfg.fp32_natural_order == 0 &&
+                             (c.cfg.preconditioner == MHFEM_PREC_DIAGONAL || c.cfg.preconditioner == MHFEM_PREC_NONE);
+        if (reorder) build_fp32_order(c, *k);
+        k->val32.alloc(k->permuted ? k->diag_base32 + (int64_t)c.F * c.n * (c.n - 1) : c.sell_size);
         for (DBuf<float> *b : {&k->dinv32, &k->b32, &k->x32, &k->r32, &k->r_ast32, &k->p32, &k->Ap32, &k->s32,
                                &k->As32, &k->tmp32, &k->rhs32})
             b->alloc(n);
@@ -854,7 +998,8 @@ struct Solve {
         if constexpr (sizeof(T) == 8)
             return SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, c.sell_val.p, c.diag_base};
         else
-            return SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, k.val32.p, c.diag_base};
+            return k.permuted ? SellView<T>{c.F, k.slice_ptr32.p, k.slice_width32.p, k.col32.p, k.val32.p, k.diag_base32}
+                              : SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, k.val32.p, c.diag_base};
     }
 
     // launches of the matrix kernels, dispatched on the number of equations
@@ -901,8 +1046,22 @@ struct Solve {
     {
         SellView<double> A = matrix<double>();
         with_n(c.n, [&](auto NC) {
-            hipLaunchKernelGGL((k_residual_mixed<decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, A, x, b, r32);
+            hipLaunchKernelGGL((k_residual_mixed<decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, A, x, b, r32,
+                               k.permuted ? (const int *)k.iperm.p : (const int *)nullptr);
         });
+        MHFEM_HIP(hipGetLastError());
+    }
+
+    // x64 += d32 (in the FP32 iteration's face order); d32 = 0
+    void add_correction(double *x, float *d32)
+    {
+        if (k.permuted)
+            with_n(c.n, [&](auto NC) {
+                hipLaunchKernelGGL((k_add_correction_perm<decltype(NC)::value>), grid_for(c.F), KB, 0, c.stream, c.F,
+                                   x, d32, (const int *)k.iperm.p);
+            });
+        else
+            hipLaunchKernelGGL(k_add_correction, g, KB, 0, c.stream, n, x, d32);
         MHFEM_HIP(hipGetLastError());
     }
 
@@ -1311,7 +1470,18 @@ static void precond_update(Ctx &c, Krylov &k)
     hipLaunchKernelGGL((k_fill_t<double>), g, KB, 0, c.stream, (long)n, k.dinv.p, 1.0);
     MHFEM_HIP(hipGetLastError());
     const bool mixed = c.cfg.precision != MHFEM_PRECISION_FP64;
-    if (mixed) {
+    if (mixed && k.permuted) {
+        with_n(c.n, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_convert_permuted<N>), grid_for(k.blocks32), KB, 0, c.stream, k.blocks32, k.map32.p,
+                               c.sell_val.p, k.val32.p);
+            if constexpr (N > 1)
+                hipLaunchKernelGGL((k_convert_permuted_diag<N>), grid_for(c.F), KB, 0, c.stream, c.F, k.perm.p,
+                                   c.sell_val.p + c.diag_base, k.val32.p + k.diag_base32);
+        });
+        hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)n, k.dinv.p, k.dinv32.p);
+        MHFEM_HIP(hipGetLastError());
+    } else if (mixed) {
         hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.sell_size), KB, 0, c.stream, c.sell_size,
                            c.sell_val.p, k.val32.p);
         hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)n, k.dinv.p, k.dinv32.p);
@@ -1592,7 +1762,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                     fallback = true;
                     break;
                 }
-                hipLaunchKernelGGL(k_add_correction, g, KB, 0, c.stream, n, x, k.x32.p);
+                S.add_correction(x, k.x32.p);
                 S.residual_mixed(x, b, rhs32);
                 MHFEM_HIP(hipGetLastError());
                 S.prec<float>(rhs32, r, k.dinv32.p, M_ALL);
@@ -1638,7 +1808,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                 fallback = true;
                 break;
             }
-            hipLaunchKernelGGL(k_add_correction, g, KB, 0, c.stream, n, x, k.x32.p);
+            S.add_correction(x, k.x32.p);
             MHFEM_HIP(hipGetLastError());
             outer++;
         }
