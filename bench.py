@@ -358,8 +358,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-ilu", action="store_true")
-    ap.add_argument("--fp32-natural-order", action="store_true",
-                    help="keep the FP32 iteration in the natural face order (comparison)")
+    ap.add_argument("--fp32-face-groups", action="store_true",
+                    help="FP32 iteration with faces grouped by type (comparison)")
     ap.add_argument("--amg-steps", type=int, default=3,
                     help="timed steps of the FP64 BiCGStab + AMG form (0 skips it)")
     ap.add_argument("--skip-ablation", action="store_true")
@@ -404,7 +404,7 @@ def main():
                  reject_after=3, history_depth=args.history_depth)
     chosen = {"name": "learned guess + FP32 BiCGStab with FP64 reliable updates (Jacobi)",
               "solver": dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=args.delta,
-                             check_every=8, tol=1e-10, fp32_natural_order=int(args.fp32_natural_order)),
+                             check_every=8, tol=1e-10, fp32_face_groups=int(args.fp32_face_groups)),
               "guess": guess}
     plain = {"name": "FP64 BiCGStab + Jacobi",
              "solver": dict(preconditioner="diagonal", precision="fp64", check_every=8, tol=1e-10)}

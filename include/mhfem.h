@@ -117,10 +117,10 @@ typedef struct {
     int check_every;            /* host convergence poll period (iterations), >= 1 */
     int rows_per_block;         /* rows per work-group of the iteration kernels, 0: default
                                    (mhfem_autotune picks it by measurement) */
-    int fp32_natural_order;     /* FP32 iterations: 0 (default) keeps the FP32 matrix copy and
-                                   vectors in a face order that groups faces of one type, for
-                                   coalesced gathers (results unchanged up to rounding);
-                                   1 keeps the natural face order */
+    int fp32_face_groups;       /* FP32 iterations with Jacobi or no preconditioner: 1 keeps the
+                                   FP32 matrix copy and vectors in an order that groups faces of
+                                   one type inside windows of 4096 faces, for coalesced gathers
+                                   (results unchanged up to rounding); 0 (default): natural order */
 } mhfem_solver_config;
 
 typedef struct {

@@ -222,6 +222,26 @@ def test_fp32_refinement(prec, precision, inner):
         assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-8
 
 
+@pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2)])
+def test_fp32_face_groups(precision, inner):
+    """The FP32 iteration in the grouped face order (permuted FP32 matrix copy,
+    translated residuals and corrections) reaches the FP64 tolerance and the
+    oracle's solution, with iteration counts close to the natural order's."""
+    pr = problems.adr_3d(s=5, steps=3, preconditioner="diagonal")
+    s = capi.Solver(pr, tol=1e-10, precision=precision, inner_tol=inner, fp32_face_groups=1)
+    nat = capi.Solver(pr, tol=1e-10, precision=precision, inner_tol=inner)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for k in range(pr.steps):
+        st = s.step(t, pr.dt)
+        sn = nat.step(t, pr.dt)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+        assert st.converged and st.residue < 1e-10
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-8
+        assert abs(st.iterations - sn.iterations) <= max(3, 0.2 * sn.iterations), (st.iterations, sn.iterations)
+
+
 def test_fp32_reliable_iterations():
     """Reliable updates keep the FP32 recurrence's Krylov space across the
     FP64 residual replacements, so the iteration count stays close to the

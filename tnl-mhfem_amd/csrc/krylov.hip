@@ -799,10 +799,10 @@ static void build_blocks(Ctx &c, Krylov &k, int rpb)
 // (first-appearance) numbering interleaves the face types of a cell, so the
 // 64 faces of a slice gather x from 64 unrelated places.  Faces whose stored
 // blocks sit at the same offsets F - E (faces of one type away from the
-// boundary) are taken together, each group in natural order and the groups in
-// order of first appearance: the k-th column faces of consecutive faces of a
-// group are then consecutive faces of one other group, and a wavefront's
-// gathers fall into few cache lines.  The FP32 vectors live in this order;
+// boundary) are taken together inside windows of the natural order: the k-th
+// column faces of consecutive faces of a group are then mostly consecutive
+// faces of one other group, and a wavefront's gathers fall into few cache
+// lines.  The FP32 vectors live in this order;
 // the FP64 residual and correction kernels translate at the boundary.
 struct OffsetsHash {
     size_t operator()(const std::vector<int> &v) const
@@ -826,13 +826,19 @@ static void build_fp32_order(Ctx &c, Krylov &k)
         if (it == groups.end()) it = groups.emplace(key, (int)groups.size()).first;
         gid[E] = it->second;
     }
-    // stable counting sort by group
-    const int ng = (int)groups.size();
-    std::vector<int> start(ng + 1, 0);
-    for (int E = 0; E < F; E++) start[gid[E] + 1]++;
-    for (int g = 0; g < ng; g++) start[g + 1] += start[g];
-    std::vector<int> perm(F), iperm(F);
-    for (int E = 0; E < F; E++) perm[start[gid[E]]++] = E;
+    // within each window of CHUNK consecutive faces, a stable sort by group:
+    // the windows keep the natural order's locality (each XCD sweeps a
+    // contiguous range of faces and keeps its part of x in its L2), the
+    // groups inside a window give the coalesced runs
+    constexpr int CHUNK = 4096;
+    std::vector<int> perm(F), iperm(F), order;
+    for (int c0 = 0; c0 < F; c0 += CHUNK) {
+        const int c1 = std::min(F, c0 + CHUNK);
+        order.resize(c1 - c0);
+        for (int E = c0; E < c1; E++) order[E - c0] = E;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gid[a] < gid[b]; });
+        std::copy(order.begin(), order.end(), perm.begin() + c0);
+    }
     for (int p = 0; p < F; p++) iperm[perm[p]] = p;
     // block-SELL layout of the permuted matrix
     const int nslices = (F + 63) / 64;
@@ -943,7 +949,7 @@ Krylov *krylov_create(Ctx &c)
     if (c.cfg.precision != MHFEM_PRECISION_FP64) {
         // the gather-friendly order needs a plain FP32 iteration: one system,
         // the implicit unit-diagonal Jacobi (or none), the block layout
-        const bool reorder = k->nseg == 1 && c.unit_diagonal && c.cfg.fp32_natural_order == 0 &&
+        const bool reorder = k->nseg == 1 && c.unit_diagonal && c.cfg.fp32_face_groups != 0 &&
                              (c.cfg.preconditioner == MHFEM_PREC_DIAGONAL || c.cfg.preconditioner == MHFEM_PREC_NONE);
         if (reorder) build_fp32_order(c, *k);
         k->val32.alloc(k->permuted ? k->diag_base32 + (int64_t)c.F * c.n * (c.n - 1) : c.sell_size);

@@ -41,7 +41,7 @@ class SolverConfig(C.Structure):
                 ("convergence_residue", C.c_double), ("max_iterations", C.c_int),
                 ("gmres_restart", C.c_int), ("precision", C.c_int),
                 ("inner_residue", C.c_double), ("check_every", C.c_int), ("rows_per_block", C.c_int),
-                ("fp32_natural_order", C.c_int)]
+                ("fp32_face_groups", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -284,7 +284,7 @@ class Solver:
 
     def __init__(self, problem, solver=None, preconditioner=None, tol=None, precision="fp64",
                  inner_tol=1e-4, check_every=8, max_iterations=100000, gmres_restart=30,
-                 setup=True, rows_per_block=0, output_directory=None, fp32_natural_order=0):
+                 setup=True, rows_per_block=0, output_directory=None, fp32_face_groups=0):
         pr = problem
         self.problem = pr
         # where a failed solve leaves its linear system (the reference's output-directory)
@@ -316,7 +316,7 @@ class Solver:
                                    PRECONDITIONERS[preconditioner or pr.preconditioner],
                                    pr.tol if tol is None else tol, max_iterations, gmres_restart,
                                    PRECISIONS[precision], inner_tol, check_every, rows_per_block,
-                                   fp32_natural_order)
+                                   fp32_face_groups)
         if setup:
             self.setInitialCondition()
             self.setupLinearSystem()
