@@ -1248,6 +1248,150 @@ static int gmres(krylov_op *op, const double *b, double *x, double tol, int maxi
     return resid < tol;
 }
 
+/* Preconditioned CG (Hestenes & Stiefel 1952): z = M^{-1} r, stopping test
+ * ||z|| / ||M^{-1} b|| < tol (the preconditioned residual, as for the other
+ * solvers).  Meant for symmetric positive definite systems. */
+static int cg(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters, double *res)
+{
+    const int n = op->n;
+    double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n), *pp = malloc(sizeof(double) * n),
+           *q = malloc(sizeof(double) * n);
+    prec_apply(op, b, z);
+    double b_norm = sqrt(dot(n, z, z));
+    if (b_norm == 0.0) b_norm = 1.0;
+    or_spmv(n, op->rp, op->col, op->val, x, r);
+#pragma omp parallel for
+    for (int k = 0; k < n; k++) r[k] = b[k] - r[k];
+    prec_apply(op, r, z);
+    memcpy(pp, z, sizeof(double) * n);
+    double rho = dot(n, r, z), resid = sqrt(dot(n, z, z)) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && isfinite(resid)) {
+        or_spmv(n, op->rp, op->col, op->val, pp, q);
+        const double alpha = rho / dot(n, pp, q);
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) {
+            x[k] += alpha * pp[k];
+            r[k] -= alpha * q[k];
+        }
+        prec_apply(op, r, z);
+        const double rho_new = dot(n, r, z);
+        resid = sqrt(dot(n, z, z)) / b_norm;
+        const double beta = rho_new / rho;
+        rho = rho_new;
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) pp[k] = z[k] + beta * pp[k];
+        it++;
+    }
+    *iters = it;
+    *res = resid;
+    free(r); free(z); free(pp); free(q);
+    return resid < tol;
+}
+
+/* TFQMR (Freund 1993) on the left-preconditioned system M^{-1} A x = M^{-1} b,
+ * in the two-half-step form (Kelley 1995, Alg. tfqmr), shadow vector r_0.
+ * An iteration is one pair of half steps (two operator applications).  The
+ * quasi-residual bound tau sqrt(m + 1) decides when to look at the true
+ * preconditioned residual, which must then be below tol (else the iteration
+ * goes on).  On a breakdown ((r_0, v) = 0 or rho = 0, which the MHFEM systems
+ * with their identity Dirichlet rows do reach) it restarts from the current
+ * x, as BiCGStab does here. */
+static int tfqmr(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters, double *res)
+{
+    const int n = op->n;
+    double *rt = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * n), *y1 = malloc(sizeof(double) * n),
+           *y2 = malloc(sizeof(double) * n), *u1 = malloc(sizeof(double) * n), *u2 = malloc(sizeof(double) * n),
+           *v = malloc(sizeof(double) * n), *d = malloc(sizeof(double) * n), *r = malloc(sizeof(double) * n);
+    prec_apply(op, b, r);
+    double b_norm = sqrt(dot(n, r, r));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0, converged = 0;
+    double tau, theta, eta, rho, resid;
+restart:
+    presidue(op, x, b, rt);
+    memcpy(w, rt, sizeof(double) * n);
+    memcpy(y1, rt, sizeof(double) * n);
+    memset(d, 0, sizeof(double) * n);
+    pmatvec(op, y1, v);
+    memcpy(u1, v, sizeof(double) * n);
+    tau = sqrt(dot(n, rt, rt));
+    theta = eta = 0;
+    rho = tau * tau;
+    resid = tau / b_norm;
+    if (resid < tol) converged = 1;
+    while (!converged && it < maxit && isfinite(resid)) {
+        it++;
+        const double sigma = dot(n, rt, v), alpha = rho / sigma;
+        if (!(fabs(sigma) > 0.0) || !isfinite(alpha)) goto restart;
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) y2[k] = y1[k] - alpha * v[k];
+        pmatvec(op, y2, u2);
+        for (int j = 1; j <= 2 && !converged; j++) {
+            const double *u = j == 1 ? u1 : u2, *y = j == 1 ? y1 : y2;
+            const int m = 2 * it - 2 + j;
+            const double f = theta * theta * eta / alpha;
+#pragma omp parallel for
+            for (int k = 0; k < n; k++) {
+                w[k] -= alpha * u[k];
+                d[k] = y[k] + f * d[k];
+            }
+            theta = sqrt(dot(n, w, w)) / tau;
+            const double c = 1.0 / sqrt(1.0 + theta * theta);
+            tau = tau * theta * c;
+            eta = c * c * alpha;
+#pragma omp parallel for
+            for (int k = 0; k < n; k++) x[k] += eta * d[k];
+            if (tau * sqrt((double)m + 1.0) < tol * b_norm) {
+                presidue(op, x, b, r);
+                resid = sqrt(dot(n, r, r)) / b_norm;
+                converged = resid < tol;
+            }
+        }
+        if (converged) break;
+        const double rho_new = dot(n, rt, w), beta = rho_new / rho;
+        if (!(fabs(rho_new) > 0.0) || !isfinite(beta)) goto restart;
+        rho = rho_new;
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) y1[k] = w[k] + beta * y2[k];
+        pmatvec(op, y1, u1);
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) v[k] = u1[k] + beta * (u2[k] + beta * v[k]);
+    }
+    presidue(op, x, b, r);
+    resid = sqrt(dot(n, r, r)) / b_norm;
+    *iters = it;
+    *res = resid;
+    free(rt); free(w); free(y1); free(y2); free(u1); free(u2); free(v); free(d); free(r);
+    return resid < tol;
+}
+
+/* Stationary (Jacobi / Richardson) iteration x += omega M^{-1} (b - A x),
+ * omega = 1; with the "diagonal" preconditioner this is the Jacobi method */
+static int richardson(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters,
+                      double *res)
+{
+    const int n = op->n;
+    double *r = malloc(sizeof(double) * n);
+    prec_apply(op, b, r);
+    double b_norm = sqrt(dot(n, r, r));
+    if (b_norm == 0.0) b_norm = 1.0;
+    presidue(op, x, b, r);
+    double resid = sqrt(dot(n, r, r)) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && isfinite(resid)) {
+#pragma omp parallel for
+        for (int k = 0; k < n; k++) x[k] += r[k];
+        presidue(op, x, b, r);
+        resid = sqrt(dot(n, r, r)) / b_norm;
+        it++;
+    }
+    *iters = it;
+    *res = resid;
+    free(r);
+    return resid < tol;
+}
+
 int or_krylov(int nrows, const int *rp, const int *col, const double *val, const double *b,
               double *x, int solver, int precond, double tol, int max_iter, int gmres_restart,
               int *iterations, double *residue)
@@ -1273,6 +1417,9 @@ int or_krylov(int nrows, const int *rp, const int *col, const double *val, const
     int conv = 0;
     if (!rc) {
         if (solver == OR_GMRES) conv = gmres(&op, b, x, tol, max_iter, gmres_restart, iterations, residue);
+        else if (solver == OR_CG) conv = cg(&op, b, x, tol, max_iter, iterations, residue);
+        else if (solver == OR_TFQMR) conv = tfqmr(&op, b, x, tol, max_iter, iterations, residue);
+        else if (solver == OR_JACOBI) conv = richardson(&op, b, x, tol, max_iter, iterations, residue);
         else conv = bicgstab(&op, b, x, tol, max_iter, iterations, residue);
     }
     free(op.tmp); free(op.dinv); free(op.lu);

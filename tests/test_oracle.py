@@ -541,7 +541,7 @@ def test_ilu0_reproduces_pattern():
     del A
 
 
-@pytest.mark.parametrize("solver", ["bicgstab", "gmres"])
+@pytest.mark.parametrize("solver", ["bicgstab", "gmres", "tfqmr", "jacobi"])
 @pytest.mark.parametrize("prec", ["none", "diagonal", "ilu0"])
 def test_krylov_against_direct_solve(solver, prec):
     rp, col, val, rhs, o = _mhfem_system(prec)
@@ -551,6 +551,43 @@ def test_krylov_against_direct_solve(solver, prec):
     x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), solver, prec, 1e-13)
     assert conv == 1 and res < 1e-13
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["none", "diagonal", "ilu0"])
+def test_cg_on_spd_system(prec):
+    """CG on the symmetric positive definite 2-D Laplacian (5-point, Dirichlet),
+    whose solution scipy gives directly; and CG's residuals are the Lanczos
+    ones, so without a preconditioner it takes at most as many iterations as
+    the system has distinct eigenvalues (here a small grid, exact arithmetic
+    aside)."""
+    m = 12
+    T = sp.diags([-np.ones(m - 1), 4 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    A = (sp.kron(sp.identity(m), T) + sp.kron(sp.diags([-np.ones(m - 1), -np.ones(m - 1)], [-1, 1]),
+                                                 sp.identity(m))).tocsr()
+    A.sort_indices()
+    b = np.random.default_rng(2).standard_normal(A.shape[0])
+    xd = spla.spsolve(A.tocsc(), b)
+    x, conv, it, res = pyoracle.krylov(A.indptr, A.indices, A.data, b, np.zeros(A.shape[0]), "cg", prec, 1e-12)
+    assert conv == 1 and res < 1e-12
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-10
+    if prec == "none":
+        assert it <= len(np.unique(np.round(np.linalg.eigvalsh(A.toarray()), 8))) + 1
+
+
+def test_jacobi_iteration_count_matches_spectral_radius():
+    """The stationary iteration x += M^{-1}(b - A x) contracts the error by
+    rho(I - M^{-1}A) per step: on the diagonal preconditioner the number of
+    iterations to 1e-8 is close to log(1e-8)/log(rho)."""
+    rp, col, val, rhs, _ = _mhfem_system("diagonal")
+    n = rp.size - 1
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    rho = max(abs(np.linalg.eigvals(np.eye(n) - A.toarray())))
+    assert rho < 1
+    x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), "jacobi", "diagonal", 1e-8,
+                                       max_iter=100000)
+    assert conv == 1
+    expect = np.log(1e-8) / np.log(rho)
+    assert 0.3 * expect < it < 1.5 * expect, (it, expect)
 
 
 def test_rtn0_projection_of_constant_field():
