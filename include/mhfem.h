@@ -86,8 +86,20 @@ typedef struct {
     int do_mobility_upwind;     /* BaseModel::do_mobility_upwind (BaseModel.h:163) */
 } mhfem_model_desc;
 
-/* Linear solver selection (configSetup.h:31-62). */
-enum { MHFEM_SOLVER_BICGSTAB = 0, MHFEM_SOLVER_GMRES = 1 };
+/* Linear solver selection (configSetup.h:31-62, the names of
+ * TNL::Solvers::getLinearSolverOptions).  BiCGStab and GMRES(m) as in the
+ * reference; CG for symmetric positive definite systems; TFQMR, restarted
+ * from the current iterate on a breakdown; JACOBI the stationary iteration
+ * x += M^{-1}(b - A x), which is the Jacobi method on the unit-diagonal MHFEM
+ * rows.  All stop on ||M^{-1}(b - A x)|| / ||M^{-1} b|| < convergence_residue.
+ * The FP32 precisions, batches and graph steps take BiCGStab only. */
+enum {
+    MHFEM_SOLVER_BICGSTAB = 0,
+    MHFEM_SOLVER_GMRES = 1,
+    MHFEM_SOLVER_CG = 2,
+    MHFEM_SOLVER_TFQMR = 3,
+    MHFEM_SOLVER_JACOBI = 4
+};
 /* MHFEM_PREC_AMG: algebraic multigrid in the configuration of the
  * reference's Ginkgo branch (Solver.hpp:1025-1075): parallel graph matching
  * aggregation, one V-cycle with 2 (coarsest: 4) Richardson sweeps of relaxation

@@ -191,6 +191,63 @@ def test_krylov_iteration_parity(solver, prec):
     assert rel(x, xd) < 1e-6
 
 
+_SYSTEMS = {"heat2d": lambda prec: problems.heat_2d(s=12, steps=1),
+            "darcy3d": lambda prec: problems.darcy_3d(s=6, steps=1, preconditioner=prec)}
+
+
+@pytest.mark.parametrize("solver,prec,case", [
+    ("cg", "none", "heat2d"), ("cg", "diagonal", "heat2d"), ("cg", "ilu0", "darcy3d"),
+    ("tfqmr", "none", "darcy3d"), ("tfqmr", "diagonal", "darcy3d"), ("tfqmr", "ilu0", "darcy3d"),
+    ("tfqmr", "ilu0", "heat2d"), ("jacobi", "none", "heat2d"), ("jacobi", "ilu0", "darcy3d")])
+def test_other_solvers_iteration_parity(solver, prec, case):
+    """CG, TFQMR (with its restarts) and the Jacobi iteration against the
+    oracle's from the same x0: iteration counts agree and so do the iterates.
+    Unpreconditioned TFQMR on the Darcy system is sensitive to rounding: the
+    oracle's own count moves between 79 and 95 when the matrix entries are
+    perturbed by one or two ulps, so TFQMR's counts are compared within 20 %;
+    the others within 2 %."""
+    print(f"\n[krylov] {solver} {prec} {case}", flush=True)
+    pr = _SYSTEMS[case](prec)
+    s = capi.Solver(pr, solver=solver, preconditioner=prec, tol=1e-10)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    x0 = s.get_array("Z_iF")
+    x, st = s.solve_rhs(rhs, x0)
+    xo, conv, it, res = pyoracle.krylov(rp, col, val, rhs, x0, solver, prec, 1e-10, max_iter=20000)
+    assert conv == 1 and st.converged and st.residue < 1e-10
+    assert abs(st.iterations - it) <= max(2, (0.2 if solver == "tfqmr" else 0.02) * it), (st.iterations, it)
+    assert rel(x, xo) < 1e-7
+    xd, _, _, _ = pyoracle.krylov(rp, col, val, rhs, x0, "bicgstab", "ilu0", 1e-14)
+    assert rel(x, xd) < 1e-6
+    s.close()
+
+
+@pytest.mark.parametrize("solver,prec", [("cg", "diagonal"), ("tfqmr", "ilu0"), ("tfqmr", "amg"),
+                                         ("cg", "amg"), ("jacobi", "ilu0")])
+def test_other_solvers_step_parity(solver, prec):
+    """Whole time steps with each solver reach the scheme oracle's solution."""
+    print(f"\n[step] {solver} {prec}", flush=True)
+    pr = problems.heat_2d(s=12, steps=3)
+    s = capi.Solver(pr, solver=solver, preconditioner=prec, tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for k in range(pr.steps):
+        st = s.step(t, pr.dt)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+        assert st.converged
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10, k
+    s.close()
+
+
+def test_other_solvers_refuse_fp32():
+    pr = problems.heat_2d(s=6, steps=1)
+    for solver in ("cg", "tfqmr", "jacobi"):
+        with pytest.raises(capi.MHFEMError, match="BiCGStab"):
+            capi.Solver(pr, solver=solver, precision="fp32-refined")
+
+
 def test_ilu0_step_parity():
     pr = problems.darcy_3d(s=6, steps=2, preconditioner="ilu0")
     s = capi.Solver(pr, tol=1e-13, check_every=1)

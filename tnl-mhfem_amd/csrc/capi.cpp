@@ -718,7 +718,7 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
 {
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
-        if (cfg->solver < 0 || cfg->solver > 1) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
+        if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_JACOBI) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
         if (cfg->preconditioner < 0 || cfg->preconditioner > 3) throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
         if (cfg->preconditioner == MHFEM_PREC_AMG && cfg->precision != MHFEM_PRECISION_FP64)
             throw Error{MHFEM_E_ARGUMENT, "the AMG preconditioner works with the FP64 solvers"};

@@ -753,6 +753,8 @@ struct Krylov {
     // GMRES
     DBuf<double> V;
     int V_m = 0;
+    // TFQMR's two vectors beyond the BiCGStab set (allocated on first use)
+    DBuf<double> ext;
     // mixed precision: solves left that go straight to FP64 after a fallback
     int fp64_cooldown = 0;
     // captured step graphs
@@ -1311,6 +1313,15 @@ struct Solve {
 
     // GMRES(m), FP64, one segment
     int gmres(const double *b, double *x, double tol, int maxit, int m, double *resid_out);
+    int cg(const double *b, double *x, double tol, int maxit, double *resid_out);
+    int tfqmr(const double *b, double *x, double tol, int maxit, double *resid_out);
+    int richardson(const double *b, double *x, double tol, int maxit, double *resid_out);
+    // host-driven pieces of those three: y = a u + b v (y may be u or v),
+    // dot products read back to the host, device copies
+    void lin2(double a, const double *u, double b, const double *v, double *y);
+    double dot1(const double *x, const double *y);
+    void dot2(const double *x0, const double *y0, const double *x1, const double *y1, double *d0, double *d1);
+    void copy(const double *x, double *y);
 };
 
 // ---- GMRES pieces: partial dots of w with V_0..V_{j1-1}, one row of partials per vector
@@ -1459,6 +1470,183 @@ int Solve::gmres(const double *b, double *x, double tol, int maxit, int m, doubl
         resid = beta / b_norm;
     }
     MHFEM_HIP(hipStreamSynchronize(c.stream));
+    *resid_out = resid;
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// CG, TFQMR and the Jacobi/Richardson iteration, FP64, the other solvers of
+// the reference's "linear-solver" option (configSetup.h:31-37).  Their scalar
+// recurrences run on the host, one read-back per dot product; the vectors stay
+// in HBM.  Each stops on ||M^{-1}(b - A x)|| / ||M^{-1} b|| < tol like the
+// BiCGStab and GMRES here.
+
+__global__ void __launch_bounds__(KB) k_lin2(int n, double a, const double *u, double b, const double *v, double *y)
+{
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) y[i] = a * u[i] + b * v[i];
+}
+
+void Solve::lin2(double a, const double *u, double b, const double *v, double *y)
+{
+    hipLaunchKernelGGL(k_lin2, g, KB, 0, c.stream, n, a, u, b, v, y);
+    MHFEM_HIP(hipGetLastError());
+}
+double Solve::dot1(const double *x, const double *y)
+{
+    dots<double, 1>(DotArgs<double, 1>{{x}, {y}}, FIN_NONE, M_ALL);
+    poll();
+    return k.h_sc[S_D0];
+}
+void Solve::dot2(const double *x0, const double *y0, const double *x1, const double *y1, double *d0, double *d1)
+{
+    dots<double, 2>(DotArgs<double, 2>{{x0, x1}, {y0, y1}}, FIN_NONE, M_ALL);
+    poll();
+    *d0 = k.h_sc[S_D0];
+    *d1 = k.h_sc[S_D1];
+}
+void Solve::copy(const double *x, double *y)
+{
+    MHFEM_HIP(hipMemcpyAsync(y, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+}
+
+// Preconditioned CG (Hestenes & Stiefel 1952), z = M^{-1} r; meant for
+// symmetric positive definite systems (TNL::Solvers::Linear::CG)
+int Solve::cg(const double *b, double *x, double tol, int maxit, double *resid_out)
+{
+    double *r = k.r.p, *z = k.s.p, *p = k.p.p, *q = k.Ap.p;
+    const double *dinv = k.dinv.p;
+    set_scalars(tol, maxit);
+    prec<double>(b, z, dinv, M_ALL);
+    double b_norm = std::sqrt(dot1(z, z));
+    if (b_norm == 0.0) b_norm = 1.0;
+    sresidual<double>(x, b, r, M_ALL);
+    prec<double>(r, z, dinv, M_ALL);
+    copy(z, p);
+    double rho, zz;
+    dot2(r, z, z, z, &rho, &zz);
+    double resid = std::sqrt(zz) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && std::isfinite(resid)) {
+        spmv_plain<double>(p, q);
+        const double alpha = rho / dot1(p, q);
+        lin2(1.0, x, alpha, p, x);
+        lin2(1.0, r, -alpha, q, r);
+        prec<double>(r, z, dinv, M_ALL);
+        double rho_new;
+        dot2(r, z, z, z, &rho_new, &zz);
+        resid = std::sqrt(zz) / b_norm;
+        const double beta = rho_new / rho;
+        rho = rho_new;
+        lin2(1.0, z, beta, p, p);
+        it++;
+    }
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+    *resid_out = resid;
+    return it;
+}
+
+// TFQMR (Freund 1993) on M^{-1} A x = M^{-1} b in the two-half-step form
+// (Kelley 1995), shadow vector r_0 (TNL::Solvers::Linear::TFQMR).  One
+// iteration is a pair of half steps.  When the quasi-residual bound
+// tau sqrt(m + 1) falls below tol ||M^{-1} b|| the true preconditioned
+// residual is computed and must be below tol too.  A breakdown, (r_0, v) = 0
+// or rho = 0, which the identity rows of Dirichlet faces do reach, restarts
+// the recurrence from the current x.
+int Solve::tfqmr(const double *b, double *x, double tol, int maxit, double *resid_out)
+{
+    if (k.ext.n < (size_t)2 * n) k.ext.alloc((size_t)2 * n);
+    double *rt = k.r_ast.p, *w = k.r.p, *y1 = k.p.p, *y2 = k.s.p, *u1 = k.Ap.p, *u2 = k.As.p, *v = k.tmp2.p,
+           *d = k.ext.p, *r = k.ext.p + n, *tmp = k.tmp.p;
+    const double *dinv = k.dinv.p;
+    auto pmv = [&](const double *in, double *out) {  // out = M^{-1} A in
+        spmv_plain<double>(in, tmp);
+        prec<double>(tmp, out, dinv, M_ALL);
+    };
+    set_scalars(tol, maxit);
+    prec<double>(b, r, dinv, M_ALL);
+    double b_norm = std::sqrt(dot1(r, r));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0;
+    bool converged = false, restart = true;
+    double tau = 0, theta = 0, eta = 0, rho = 0, resid = 0;
+    while (restart) {
+        restart = false;
+        presidue<double>(x, b, rt, tmp, dinv, M_ALL);
+        copy(rt, w);
+        copy(rt, y1);
+        MHFEM_HIP(hipMemsetAsync(d, 0, (size_t)n * sizeof(double), c.stream));
+        pmv(y1, v);
+        copy(v, u1);
+        tau = std::sqrt(dot1(rt, rt));
+        theta = eta = 0;
+        rho = tau * tau;
+        resid = tau / b_norm;
+        if (resid < tol) converged = true;
+        while (!converged && it < maxit && std::isfinite(resid)) {
+            it++;
+            const double sigma = dot1(rt, v), alpha = rho / sigma;
+            if (!(std::fabs(sigma) > 0.0) || !std::isfinite(alpha)) {
+                restart = true;
+                break;
+            }
+            lin2(1.0, y1, -alpha, v, y2);
+            pmv(y2, u2);
+            for (int j = 1; j <= 2 && !converged; j++) {
+                const double *u = j == 1 ? u1 : u2, *y = j == 1 ? y1 : y2;
+                const int m = 2 * it - 2 + j;
+                const double f = theta * theta * eta / alpha;
+                lin2(1.0, w, -alpha, u, w);
+                lin2(1.0, y, f, d, d);
+                theta = std::sqrt(dot1(w, w)) / tau;
+                const double cth = 1.0 / std::sqrt(1.0 + theta * theta);
+                tau = tau * theta * cth;
+                eta = cth * cth * alpha;
+                lin2(1.0, x, eta, d, x);
+                if (tau * std::sqrt((double)m + 1.0) < tol * b_norm) {
+                    presidue<double>(x, b, r, tmp, dinv, M_ALL);
+                    resid = std::sqrt(dot1(r, r)) / b_norm;
+                    converged = resid < tol;
+                }
+            }
+            if (converged) break;
+            const double rho_new = dot1(rt, w), beta = rho_new / rho;
+            if (!(std::fabs(rho_new) > 0.0) || !std::isfinite(beta)) {
+                restart = true;
+                break;
+            }
+            rho = rho_new;
+            lin2(1.0, w, beta, y2, y1);
+            pmv(y1, u1);
+            lin2(1.0, u2, beta, v, v);  // v = u1 + beta (u2 + beta v)
+            lin2(1.0, u1, beta, v, v);
+        }
+    }
+    presidue<double>(x, b, r, tmp, dinv, M_ALL);
+    resid = std::sqrt(dot1(r, r)) / b_norm;
+    *resid_out = resid;
+    return it;
+}
+
+// Stationary iteration x += M^{-1}(b - A x) (relaxation 1): with the diagonal
+// preconditioner, or the unit diagonal the MHFEM rows are scaled to, it is the
+// Jacobi method (TNL::Solvers::Linear::Jacobi, omega = 1)
+int Solve::richardson(const double *b, double *x, double tol, int maxit, double *resid_out)
+{
+    double *r = k.r.p, *tmp = k.tmp.p;
+    const double *dinv = k.dinv.p;
+    set_scalars(tol, maxit);
+    prec<double>(b, r, dinv, M_ALL);
+    double b_norm = std::sqrt(dot1(r, r));
+    if (b_norm == 0.0) b_norm = 1.0;
+    presidue<double>(x, b, r, tmp, dinv, M_ALL);
+    double resid = std::sqrt(dot1(r, r)) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && std::isfinite(resid)) {
+        lin2(1.0, x, 1.0, r, x);
+        presidue<double>(x, b, r, tmp, dinv, M_ALL);
+        resid = std::sqrt(dot1(r, r)) / b_norm;
+        it++;
+    }
     *resid_out = resid;
     return it;
 }
@@ -1699,9 +1887,14 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
     if (k.nseg > 1 && (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64))
         throw Error{MHFEM_E_ARGUMENT, "batched systems are solved with FP64 BiCGStab"};
     if (c.cfg.precision == MHFEM_PRECISION_FP64) {
-        if (c.cfg.solver == MHFEM_SOLVER_GMRES) {
+        if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB) {
             double res = 0;
-            out.iterations = S.gmres(b, x, tol, maxit, c.cfg.gmres_restart, &res);
+            switch (c.cfg.solver) {
+            case MHFEM_SOLVER_GMRES: out.iterations = S.gmres(b, x, tol, maxit, c.cfg.gmres_restart, &res); break;
+            case MHFEM_SOLVER_CG: out.iterations = S.cg(b, x, tol, maxit, &res); break;
+            case MHFEM_SOLVER_TFQMR: out.iterations = S.tfqmr(b, x, tol, maxit, &res); break;
+            default: out.iterations = S.richardson(b, x, tol, maxit, &res); break;
+            }
             out.residue = res;
             out.converged = res < tol;
         } else {
