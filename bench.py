@@ -362,6 +362,8 @@ def main():
                     help="FP32 iteration with faces grouped by type (comparison)")
     ap.add_argument("--amg-steps", type=int, default=3,
                     help="timed steps of the FP64 BiCGStab + AMG form (0 skips it)")
+    ap.add_argument("--isai-steps", type=int, default=3,
+                    help="timed steps of the FP64 BiCGStab + ILU-ISAI form (0 skips it)")
     ap.add_argument("--skip-ablation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the ensemble leg")
@@ -412,6 +414,8 @@ def main():
                  "solver": dict(preconditioner="ilu0", precision="fp64", check_every=4, tol=1e-10)}
     plain_amg = {"name": "FP64 BiCGStab + AMG (PGM aggregation, block-Jacobi Richardson smoothing, V-cycle)",
                  "solver": dict(preconditioner="amg", precision="fp64", check_every=4, tol=1e-10)}
+    plain_isai = {"name": "FP64 BiCGStab + ILU-ISAI (ILU(0) factors as sparse approximate inverses, power 2)",
+                  "solver": dict(preconditioner="ilu-isai", precision="fp64", check_every=4, tol=1e-10)}
     ablation = {"name": "chosen form without the network (previous increments only)",
                 "solver": chosen["solver"], "guess": dict(guess, train_steps=0, combine="history")}
     fp64_hist = {"name": "FP64 BiCGStab + Jacobi from the previous increments",
@@ -480,6 +484,30 @@ def main():
         sa.close()
         log(f"[bench] AMG: {res_amg}")
 
+    res_isai = None
+    if args.isai_steps > 0:
+        log(f"[bench] ILU-ISAI form ({args.isai_steps} timed steps after 1 warmup step, which builds it)")
+        from mhfem_amd import capi as _capi
+        si = _capi.Solver(pr, **plain_isai["solver"])
+        t_setup = time.perf_counter()
+        si.step(0.0, pr.dt)
+        si.synchronize()
+        first_ms = 1e3 * (time.perf_counter() - t_setup)
+        t = pr.dt
+        its0 = len(si.iterations)
+        t0 = time.perf_counter()
+        for _ in range(args.isai_steps):
+            si.step(t, pr.dt)
+            t += pr.dt
+        si.synchronize()
+        dti = time.perf_counter() - t0
+        (lrp, _, _), (urp, _, _), rebuilds = si.isai_matrices()
+        res_isai = summary(plain_isai["name"], args.isai_steps, dti, si.iterations[its0:])
+        res_isai.update(first_step_ms_with_setup=round(first_ms, 1), nnz_lower=int(lrp[-1]),
+                        nnz_upper=int(urp[-1]), rebuilds=rebuilds)
+        si.close()
+        log(f"[bench] ILU-ISAI: {res_isai}")
+
     res_abl = res_h64 = z_h64 = None
     if not args.skip_ablation:
         log(f"[bench] ablation: {ablation['name']}")
@@ -523,6 +551,7 @@ def main():
         "plain": res_plain,
         "plain_ilu0": res_ilu,
         "plain_amg": res_amg,
+        "plain_ilu_isai": res_isai,
         "chosen_without_network": res_abl,
         "fp64_history": res_h64,
         "chosen_vs_plain": round(value / res_plain["steps_per_s"], 3) if res_plain else None,

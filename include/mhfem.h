@@ -107,7 +107,12 @@ enum {
  * the last solve took more than min(2u, u + 5) iterations, u those right after
  * the last rebuild (Solver.hpp:1017, 1158-1160).  FP64 solvers, single
  * systems, the four per-step calls (not mhfem_step). */
-enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2, MHFEM_PREC_AMG = 3 };
+/* MHFEM_PREC_ILU_ISAI: the Ginkgo branch's "ILU_ISAI" (Solver.hpp:1089-1130):
+ * ILU(0) factors replaced by incomplete sparse approximate inverses on the
+ * patterns of L^2 and U^2, applied as two sparse products; every row of the
+ * inverses is solved exactly.  Same reuse rule, configurations and limits as
+ * MHFEM_PREC_AMG.  (PARILU_ISAI and PARILUT_ISAI are not provided.) */
+enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2, MHFEM_PREC_AMG = 3, MHFEM_PREC_ILU_ISAI = 4 };
 /* precision of the Krylov iteration.  FP32_REFINED: FP64 iterative
    refinement whose every correction is an FP32 BiCGStab solve started from
    zero to `inner_residue`.  FP32_RELIABLE: one FP32 BiCGStab recurrence whose
@@ -355,6 +360,11 @@ int mhfem_benchmark_spmv(mhfem_ctx *ctx, int precision, int reps, double *avg_ms
 int mhfem_amg_info(const mhfem_ctx *ctx, int *levels, int *rebuilds);
 int mhfem_amg_level(const mhfem_ctx *ctx, int level, int *rows, long *nnz, int32_t *aggregates);
 int mhfem_amg_apply(mhfem_ctx *ctx, const double *b, double *x);
+/* ILU-ISAI inspection (no counterpart in the reference): nonzeros of M_L and
+ * M_U and rebuilds so far; M_L (upper = 0) or M_U (upper = 1) in CSR
+ * (rowptr: rows + 1 entries, col/val: the nonzeros; NULL outputs skipped). */
+int mhfem_isai_info(const mhfem_ctx *ctx, long *nnz_lower, long *nnz_upper, int *rebuilds);
+int mhfem_isai_matrix(mhfem_ctx *ctx, int upper, int32_t *rowptr, int32_t *col, double *val);
 int mhfem_get_timers(const mhfem_ctx *ctx, mhfem_timers *out);
 int mhfem_reset_timers(mhfem_ctx *ctx);
 int mhfem_set_timing(mhfem_ctx *ctx, int level); /* 0 off (default), 1 HIP-event phase timers,

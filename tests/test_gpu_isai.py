@@ -1,0 +1,84 @@
+"""The ILU-ISAI preconditioner on the GPU against its oracle
+(oracle/isai_oracle.py): the approximate inverses (patterns bit for bit,
+values to 1e-12), the BiCGStab iteration count from the same x0, whole time
+steps against the scheme oracle, the reuse rule over a trajectory, and the
+configurations it refuses."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import amg_oracle as amg
+import isai_oracle as isai
+import pyoracle
+from mhfem_amd import capi, problems
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "darcy3d": lambda: problems.darcy_3d(s=5, steps=4),
+    "adr3d_n2": lambda: problems.adr_3d(s=4, n=2, steps=4),
+    "heat2d": lambda: problems.heat_2d(s=12, steps=4),
+    "hex_n2": lambda: problems.adr_hexes(s=3, n=2, steps=4),
+}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_inverses_and_iterations_match_oracle(name):
+    print(f"\n[case] {name}", flush=True)
+    pr = CASES[name]()
+    s = capi.Solver(pr, preconditioner="ilu-isai", tol=1e-10)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    x0 = s.get_array("Z_iF")
+    x, st = s.solve_rhs(rhs, x0)
+    A = sp.csr_matrix((val, col, rp))
+    P = isai.IluIsai(A)
+    (lrp, lcol, lval), (urp, ucol, uval), rebuilds = s.isai_matrices()
+    assert rebuilds == 1
+    for (grp, gcol, gval), M in (((lrp, lcol, lval), P.ML), ((urp, ucol, uval), P.MU)):
+        assert np.array_equal(grp, M.indptr) and np.array_equal(gcol, M.indices)
+        assert np.abs(gval - M.data).max() <= 1e-12 * np.abs(M.data).max()
+    xo, it, res = isai.bicgstab(A, rhs, x0, P, 1e-10)
+    print("iterations", st.iterations, it, flush=True)
+    assert st.converged and abs(st.iterations - it) <= 2, (st.iterations, it)
+    assert rel(x, xo) < 1e-7
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["darcy3d", "hex_n2"])
+def test_steps_match_scheme_oracle_and_reuse_rule(name):
+    print(f"\n[case] {name}", flush=True)
+    pr = CASES[name]()
+    s = capi.Solver(pr, preconditioner="ilu-isai", tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    rule, rebuilds, its = amg.ReuseRule(), 0, []
+    t = 0.0
+    for k in range(pr.steps):
+        rebuild = rule.rebuild()
+        rebuilds += rebuild
+        st = s.step(t, pr.dt)
+        rule.after(st.iterations, rebuild)
+        its.append(st.iterations)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+        assert st.converged
+        for nm in ("Z_iF", "Z_iK"):
+            assert rel(s.get_array(nm), o.array(nm)) < 1e-10, (k, nm)
+    print("iterations", its, "rebuilds", rebuilds, flush=True)
+    assert s.isai_matrices()[2] == rebuilds
+    s.close()
+
+
+def test_ilu_isai_refuses_fp32_and_graph_steps():
+    pr = problems.heat_2d(s=6, steps=1)
+    with pytest.raises(capi.MHFEMError, match="FP64"):
+        capi.Solver(pr, preconditioner="ilu-isai", precision="fp32-reliable")
+    s = capi.Solver(pr, preconditioner="ilu-isai")
+    with pytest.raises(capi.MHFEMError, match="ILU-ISAI"):
+        s.step_graph(0.0, pr.dt)
+    s.close()

@@ -719,11 +719,14 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
         if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_JACOBI) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
-        if (cfg->preconditioner < 0 || cfg->preconditioner > 3) throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
-        if (cfg->preconditioner == MHFEM_PREC_AMG && cfg->precision != MHFEM_PRECISION_FP64)
-            throw Error{MHFEM_E_ARGUMENT, "the AMG preconditioner works with the FP64 solvers"};
-        if (cfg->preconditioner == MHFEM_PREC_AMG && c.h_seg_faces.size() > 2)
-            throw Error{MHFEM_E_ARGUMENT, "the AMG preconditioner is for single systems, not batches"};
+        if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_ILU_ISAI)
+            throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
+        const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || cfg->preconditioner == MHFEM_PREC_ILU_ISAI;
+        const char *pname = cfg->preconditioner == MHFEM_PREC_AMG ? "AMG" : "ILU-ISAI";
+        if (reused && cfg->precision != MHFEM_PRECISION_FP64)
+            throw Error{MHFEM_E_ARGUMENT, std::string("the ") + pname + " preconditioner works with the FP64 solvers"};
+        if (reused && c.h_seg_faces.size() > 2)
+            throw Error{MHFEM_E_ARGUMENT, std::string("the ") + pname + " preconditioner is for single systems, not batches"};
         if (cfg->precision < 0 || cfg->precision > 2) throw Error{MHFEM_E_ARGUMENT, "unknown precision"};
         if (cfg->precision != MHFEM_PRECISION_FP64 && cfg->solver != MHFEM_SOLVER_BICGSTAB)
             throw Error{MHFEM_E_ARGUMENT, "FP32 refinement is implemented for BiCGStab only"};
@@ -1115,6 +1118,22 @@ int mhfem_amg_apply(mhfem_ctx *ctx, const double *b, double *x)
         amg_apply(c, a, db.p, dx.p, nullptr);
         MHFEM_HIP(hipMemcpyAsync(x, dx.p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mhfem_isai_info(const mhfem_ctx *ctx, long *nnz_lower, long *nnz_upper, int *rebuilds)
+{
+    const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+    if (!c || !c->krylov || !krylov_isai_info(*c->krylov, nnz_lower, nnz_upper, rebuilds)) return MHFEM_E_STATE;
+    return MHFEM_OK;
+}
+
+int mhfem_isai_matrix(mhfem_ctx *ctx, int upper, int32_t *rowptr, int32_t *col, double *val)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.krylov || !krylov_isai_info(*c.krylov, nullptr, nullptr, nullptr))
+            throw Error{MHFEM_E_STATE, "no ILU-ISAI preconditioner"};
+        krylov_isai_export(c, upper ? 1 : 0, rowptr, col, val);
     });
 }
 

@@ -335,6 +335,10 @@ double dot64(Ctx &c, const double *x, const double *y);
 void launch_csr_gather(Ctx &c, double *dst);
 
 Amg *krylov_amg(const Krylov &k);  // the AMG preconditioner of a solver, or nullptr
+// the ILU-ISAI preconditioner's sizes and rebuild count (false: none); copy of
+// M_L (upper = 0) or M_U (upper = 1) in CSR, NULL outputs skipped
+bool krylov_isai_info(const Krylov &k, long *nnz_lower, long *nnz_upper, int *rebuilds);
+void krylov_isai_export(Ctx &c, int upper, int32_t *rowptr, int32_t *col, double *val);
 // amg.hip: algebraic multigrid preconditioner (FP64)
 Amg *amg_create();
 void amg_destroy(Amg *a);

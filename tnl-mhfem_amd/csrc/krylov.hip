@@ -727,6 +727,82 @@ struct Ilu0 {
 };
 
 // ---------------------------------------------------------------------------
+// ILU-ISAI (the reference's Ginkgo "ILU_ISAI", Solver.hpp:1089-1130): the
+// ILU(0) factors L (unit lower) and U are replaced by incomplete sparse
+// approximate inverses M_L, M_U on the patterns of L^2 and U^2 (sparsity
+// power 2, Solver.hpp:1111), so that the apply y = M_U (M_L x) is two CSR
+// products instead of two level-scheduled triangular solves.  Row i of M
+// solves (M T)_{i,j} = delta_ij for j in its pattern J (Anzt, Huckle,
+// Braeckle, Dongarra 2018), the triangular system T[J,J]^T m = e_i.
+//
+// One lane per row.  The row's slice of m first holds running sums, then the
+// solution: with J sorted, position q is final once every position on its far
+// side is (q' > q for L, q' < q for U); row J_q of the factor then adds
+// m_q T_{J_q, J_p} to each position p it reaches, found by merging two sorted
+// column lists.
+template <bool LOWER>
+__global__ void __launch_bounds__(KB) k_isai(int n, const int *__restrict__ mrp, const int *__restrict__ mcol,
+                                             const int *__restrict__ rp, const int *__restrict__ col,
+                                             const int *__restrict__ dg, const double *__restrict__ lu,
+                                             double *__restrict__ m)
+{
+    const int i = blockIdx.x * KB + threadIdx.x;
+    if (i >= n) return;
+    const int b = mrp[i], e = mrp[i + 1];
+    for (int p = b; p < e; p++) m[p] = 0.0;
+    for (int s = 0; s < e - b; s++) {
+        const int q = LOWER ? e - 1 - s : b + s;
+        const int t = mcol[q];
+        const double mq = ((t == i ? 1.0 : 0.0) - m[q]) / (LOWER ? 1.0 : lu[dg[t]]);
+        m[q] = mq;
+        if (LOWER) {  // row t of L left of its diagonal meets positions b..q-1
+            int p = b;
+            for (int kk = rp[t]; kk < dg[t]; kk++) {
+                const int cc = col[kk];
+                while (p < q && mcol[p] < cc) p++;
+                if (p < q && mcol[p] == cc) m[p] += mq * lu[kk];
+            }
+        } else {  // row t of U right of its diagonal meets positions q+1..e-1
+            int p = q + 1;
+            for (int kk = dg[t] + 1; kk < rp[t + 1]; kk++) {
+                const int cc = col[kk];
+                while (p < e && mcol[p] < cc) p++;
+                if (p < e && mcol[p] == cc) m[p] += mq * lu[kk];
+            }
+        }
+    }
+}
+
+// y = M x, CSR with L lanes per row; sc: flags of the single segment (skip
+// once it has halted) or nullptr
+template <int L>
+__global__ void __launch_bounds__(KB) k_csr_mv(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                               const double *__restrict__ val, const double *__restrict__ x,
+                                               double *__restrict__ y, const double *sc)
+{
+    if (sc && halted(sc)) return;
+    const long t = (long)blockIdx.x * KB + threadIdx.x;
+    const int row = (int)(t / L), lane = (int)(t % L);
+    double s = 0.0;
+    if (row < n)
+        for (int q = rp[row] + lane; q < rp[row + 1]; q += L) s += val[q] * x[col[q]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+    if (row < n && lane == 0) y[row] = s;
+}
+
+struct Isai {
+    DBuf<int> lrp, lcol, urp, ucol;  // patterns of L^2 and U^2
+    DBuf<double> lval, uval, t;      // M_L, M_U; t = M_L x
+    long nl = 0, nu = 0;
+    int lanes_l = 8, lanes_u = 8;
+    // reuse rule (Solver.hpp:1017, 1158-1160)
+    long long last_iters = 0, updated_iters = 0;
+    int rebuilds = 0;
+    bool rebuilt = false;
+};
+
+// ---------------------------------------------------------------------------
 struct Krylov {
     int n = 0;
     DBuf<double> r, r_ast, p, Ap, s, As, tmp, tmp2, dinv;
@@ -741,6 +817,7 @@ struct Krylov {
     double *h_sc_ahead[2] = {nullptr, nullptr};  // pinned, nseg * NSLOTS each
     hipEvent_t ev_ahead[2] = {nullptr, nullptr};
     Ilu0 ilu;
+    Isai isai;
     Amg *amg = nullptr;
     // FP32 inner solver
     DBuf<float> val32, dinv32, b32, x32, r32, r_ast32, p32, Ap32, s32, As32, tmp32, rhs32;
@@ -892,6 +969,46 @@ static void build_fp32_order(Ctx &c, Krylov &k)
     k.permuted = true;
 }
 
+// patterns of L^2 and U^2 from the factors' pattern (that of the matrix):
+// row i of L^2 is the union of the rows t of L that row i of L reaches
+static void isai_patterns(Ctx &c, Krylov &k, const std::vector<int> &dg)
+{
+    const int n = k.n;
+    const std::vector<int> &rp = c.h_rowptr, &col = c.h_col;
+    auto build = [&](bool lower, DBuf<int> &drp, DBuf<int> &dcol, long &nnz, int &lanes) {
+        std::vector<int> prp(n + 1, 0), pcol, mark(n, -1), row;
+        pcol.reserve((size_t)rp[n] * 2);
+        for (int i = 0; i < n; i++) {
+            row.clear();
+            for (int a = lower ? rp[i] : dg[i]; a < (lower ? dg[i] + 1 : rp[i + 1]); a++) {
+                const int t = col[a];
+                for (int q = lower ? rp[t] : dg[t]; q < (lower ? dg[t] + 1 : rp[t + 1]); q++)
+                    if (mark[col[q]] != i) {
+                        mark[col[q]] = i;
+                        row.push_back(col[q]);
+                    }
+            }
+            std::sort(row.begin(), row.end());
+            pcol.insert(pcol.end(), row.begin(), row.end());
+            if (pcol.size() > (size_t)INT32_MAX) throw Error{MHFEM_E_ARGUMENT, "ILU-ISAI pattern exceeds 2^31 entries"};
+            prp[i + 1] = (int)pcol.size();
+        }
+        nnz = (long)pcol.size();
+        const long avg = nnz / std::max(1, n);
+        lanes = avg <= 6 ? 4 : avg <= 12 ? 8 : avg <= 24 ? 16 : 32;
+        drp.alloc(n + 1);
+        dcol.alloc(nnz);
+        MHFEM_HIP(hipMemcpy(drp.p, prp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(dcol.p, pcol.data(), nnz * sizeof(int), hipMemcpyHostToDevice));
+    };
+    Isai &S = k.isai;
+    build(true, S.lrp, S.lcol, S.nl, S.lanes_l);
+    build(false, S.urp, S.ucol, S.nu, S.lanes_u);
+    S.lval.alloc(S.nl);
+    S.uval.alloc(S.nu);
+    S.t.alloc(n);
+}
+
 Krylov *krylov_create(Ctx &c)
 {
     Krylov *k = new Krylov();
@@ -908,7 +1025,7 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipEventCreateWithFlags(&k->ev_ahead[q], hipEventDisableTiming));
     }
     build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
-    if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
+    if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) {
         Ilu0 &I = k->ilu;
         std::vector<int> dg(n), lev(n), ulev(n);
         for (int r = 0; r < n; r++)
@@ -946,6 +1063,7 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipMemcpy(I.diag.p, dg.data(), n * sizeof(int), hipMemcpyHostToDevice));
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) isai_patterns(c, *k, dg);
     }
     if (c.cfg.preconditioner == MHFEM_PREC_AMG) k->amg = amg_create();
     if (c.cfg.precision != MHFEM_PRECISION_FP64) {
@@ -1150,11 +1268,50 @@ struct Solve {
             throw Error{MHFEM_E_STATE, "the AMG preconditioner works in FP64"};
         }
     }
+    // y = M_U (M_L x): two CSR products (FP64 only, as AMG)
+    template <typename T>
+    void isai_apply_t(const T *x, T *y, bool skip_when_halted)
+    {
+        if constexpr (sizeof(T) == 8) {
+            const Isai &S = k.isai;
+            const double *flag = (skip_when_halted && k.nseg == 1) ? k.sc.p : nullptr;
+            auto mv = [&](int lanes, const int *rp, const int *col, const double *val, const double *in, double *out) {
+                auto go = [&](auto LC) {
+                    constexpr int L = decltype(LC)::value;
+                    const long threads = (long)n * L;
+                    hipLaunchKernelGGL((k_csr_mv<L>), (unsigned)((threads + KB - 1) / KB), KB, 0, c.stream, n, rp,
+                                       col, val, in, out, flag);
+                };
+                if (lanes == 4) go(std::integral_constant<int, 4>{});
+                else if (lanes == 8) go(std::integral_constant<int, 8>{});
+                else if (lanes == 16) go(std::integral_constant<int, 16>{});
+                else go(std::integral_constant<int, 32>{});
+            };
+            mv(S.lanes_l, S.lrp.p, S.lcol.p, S.lval.p, x, S.t.p);
+            mv(S.lanes_u, S.urp.p, S.ucol.p, S.uval.p, S.t.p, y);
+            MHFEM_HIP(hipGetLastError());
+        } else {
+            (void)x, (void)y, (void)skip_when_halted;
+            throw Error{MHFEM_E_STATE, "the ILU-ISAI preconditioner works in FP64"};
+        }
+    }
+    // preconditioners applied by their own kernels after the matrix product
+    bool separate_prec() const
+    {
+        const int p = c.cfg.preconditioner;
+        return p == MHFEM_PREC_ILU0 || p == MHFEM_PREC_AMG || p == MHFEM_PREC_ILU_ISAI;
+    }
+    template <typename T>
+    void separate_apply(const T *x, T *y, bool skip_when_halted)
+    {
+        if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(x, y, skip_when_halted);
+        else if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) isai_apply_t<T>(x, y, skip_when_halted);
+        else ilu_apply<T>(x, y, skip_when_halted);
+    }
     template <typename T>
     void prec(const T *x, T *y, const T *dinv, int mode)
     {
-        if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(x, y, false);
-        else if (c.cfg.preconditioner == MHFEM_PREC_ILU0) ilu_apply<T>(x, y, false);
+        if (separate_prec()) separate_apply<T>(x, y, false);
         else hipLaunchKernelGGL((k_sscale<T>), k.nblk, KB, 0, c.stream, sg, mode, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
         MHFEM_HIP(hipGetLastError());
     }
@@ -1164,10 +1321,9 @@ struct Solve {
     template <typename T, int ND>
     void pmatvec(const T *x, T *y, T *tmp, const T *dinv, const T *z, int fin)
     {
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || c.cfg.preconditioner == MHFEM_PREC_AMG) {
+        if (separate_prec()) {
             sspmv<T, false, 0>(x, (const T *)nullptr, tmp, (const T *)nullptr, M_ACTIVE);
-            if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(tmp, y, true);
-            else ilu_apply<T>(tmp, y, true);
+            separate_apply<T>(tmp, y, true);
             if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, M_ACTIVE);
             if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, M_ACTIVE);
         } else {
@@ -1186,10 +1342,9 @@ struct Solve {
     void presidue(const T *x, const T *b, T *r, T *tmp, const T *dinv, int mode, T *scratch = nullptr)
     {
         sresidual<T>(x, b, tmp, mode);
-        if ((c.cfg.preconditioner == MHFEM_PREC_ILU0 || c.cfg.preconditioner == MHFEM_PREC_AMG) && mode != M_ALL) {
-            if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0) or AMG needs a scratch vector"};
-            if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(tmp, scratch, false);
-            else ilu_apply<T>(tmp, scratch, false);
+        if (separate_prec() && mode != M_ALL) {
+            if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0), ILU-ISAI or AMG needs a scratch vector"};
+            separate_apply<T>(tmp, scratch, false);
             hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, (const T *)scratch, r, (T *)nullptr);
             MHFEM_HIP(hipGetLastError());
             return;
@@ -1656,6 +1811,20 @@ int Solve::richardson(const double *b, double *x, double tol, int maxit, double 
 // ---------------------------------------------------------------------------
 // Preconditioner update (Solver.hpp:1237-1242): every step, from the freshly
 // assembled matrix.
+// ILU(0) of the current matrix into k.ilu.lu (CSR order), level by level
+static void ilu_factor(Ctx &c, Krylov &k)
+{
+    Ilu0 &I = k.ilu;
+    hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p, c.csr_to_sell.p,
+                       I.lu.p);
+    for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
+        const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
+        hipLaunchKernelGGL(k_ilu_factor, (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
+                           c.rowptr.p, c.csr_col.p, I.diag.p, I.lu.p);
+    }
+    MHFEM_HIP(hipGetLastError());
+}
+
 static void precond_update(Ctx &c, Krylov &k)
 {
     const int n = k.n;
@@ -1682,15 +1851,25 @@ static void precond_update(Ctx &c, Krylov &k)
         MHFEM_HIP(hipGetLastError());
     }
     if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_update(c, *k.amg);
+    if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) {
+        // rebuilt under the reuse rule of the reference's Ginkgo branch
+        Isai &S = k.isai;
+        const bool reuse = S.rebuilds > 0 && S.last_iters <= std::min(2 * S.updated_iters, S.updated_iters + 5);
+        S.rebuilt = !reuse;
+        if (!reuse) {
+            ilu_factor(c, k);
+            const Ilu0 &I = k.ilu;
+            hipLaunchKernelGGL(k_isai<true>, grid_for(n), KB, 0, c.stream, n, S.lrp.p, S.lcol.p, c.rowptr.p,
+                               c.csr_col.p, I.diag.p, I.lu.p, S.lval.p);
+            hipLaunchKernelGGL(k_isai<false>, grid_for(n), KB, 0, c.stream, n, S.urp.p, S.ucol.p, c.rowptr.p,
+                               c.csr_col.p, I.diag.p, I.lu.p, S.uval.p);
+            MHFEM_HIP(hipGetLastError());
+            S.rebuilds++;
+        }
+    }
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
         Ilu0 &I = k.ilu;
-        hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p,
-                           c.csr_to_sell.p, I.lu.p);
-        for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
-            const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
-            hipLaunchKernelGGL(k_ilu_factor, (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
-                               c.rowptr.p, c.csr_col.p, I.diag.p, I.lu.p);
-        }
+        ilu_factor(c, k);
         if (mixed)
             hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, I.lu.p, I.lu32.p);
         MHFEM_HIP(hipGetLastError());
@@ -1698,6 +1877,29 @@ static void precond_update(Ctx &c, Krylov &k)
 }
 
 Amg *krylov_amg(const Krylov &k) { return k.amg; }
+
+bool krylov_isai_info(const Krylov &k, long *nnz_lower, long *nnz_upper, int *rebuilds)
+{
+    if (!k.isai.lrp.p) return false;
+    if (nnz_lower) *nnz_lower = k.isai.nl;
+    if (nnz_upper) *nnz_upper = k.isai.nu;
+    if (rebuilds) *rebuilds = k.isai.rebuilds;
+    return true;
+}
+
+void krylov_isai_export(Ctx &c, int upper, int32_t *rowptr, int32_t *col, double *val)
+{
+    const Isai &S = c.krylov->isai;
+    const int n = c.krylov->n;
+    const long nnz = upper ? S.nu : S.nl;
+    auto get = [&](void *dst, const void *src, size_t bytes) {
+        if (dst) MHFEM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    };
+    get(rowptr, upper ? S.urp.p : S.lrp.p, (n + 1) * sizeof(int));
+    get(col, upper ? S.ucol.p : S.lcol.p, nnz * sizeof(int));
+    get(val, upper ? S.uval.p : S.lval.p, nnz * sizeof(double));
+    MHFEM_HIP(hipStreamSynchronize(c.stream));
+}
 
 void launch_csr_gather(Ctx &c, double *dst)
 {
@@ -2028,6 +2230,10 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
     }
     if (c.guess && c.guess_on_solve) guess_after_solve(c, *c.guess, x);
     if (k.amg) amg_after_solve(*k.amg, out.iterations);
+    if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) {
+        k.isai.last_iters = out.iterations;
+        if (k.isai.rebuilt || k.isai.updated_iters == 0) k.isai.updated_iters = out.iterations;
+    }
     if (c.timing) {
         MHFEM_HIP(hipEventRecord(e2, c.stream));
         MHFEM_HIP(hipEventSynchronize(e2));
@@ -2064,9 +2270,9 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
     Krylov &k = *c.krylov;
     if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64)
         throw Error{MHFEM_E_ARGUMENT, "graph steps use FP64 BiCGStab"};
-    if (c.cfg.preconditioner == MHFEM_PREC_AMG)
-        throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG preconditioner (its setup runs on the host); "
-                                      "use the four calls"};
+    if (c.cfg.preconditioner == MHFEM_PREC_AMG || c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI)
+        throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG or ILU-ISAI preconditioners (their reuse rule "
+                                      "runs on the host); use the four calls"};
     const int every = std::max(1, c.cfg.check_every);
     if (k.g_first && (k.g_tau != tau || k.g_iters != every)) drop_graphs(k);
     Solve S(c, k);

@@ -22,7 +22,7 @@ ARRAYS = ["Z_iF", "Z_iK", "N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe",
 ARRAY_ID = {name: k for k, name in enumerate(ARRAYS)}
 
 SOLVERS = {"bicgstab": 0, "gmres": 1, "cg": 2, "tfqmr": 3, "jacobi": 4}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3, "ilu-isai": 4}
 PRECISIONS = {"fp64": 0, "fp32-refined": 1, "fp32-reliable": 2}
 
 
@@ -132,6 +132,8 @@ SIGNATURES = {
     "mhfem_amg_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "mhfem_amg_level": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_L), _P]),
     "mhfem_amg_apply": (_I, [_P, _P, _P]),
+    "mhfem_isai_info": (_I, [_P, C.POINTER(_L), C.POINTER(_L), C.POINTER(_I)]),
+    "mhfem_isai_matrix": (_I, [_P, _I, _P, _P, _P]),
 }
 
 _lib = None
@@ -391,6 +393,22 @@ class Solver:
         x = np.empty_like(b)
         self._check(lib().mhfem_amg_apply(self.ctx, _ptr(b), _ptr(x)))
         return x
+
+    def isai_matrices(self):
+        """(M_L, M_U, rebuilds): the ILU-ISAI preconditioner's approximate
+        inverses as (rowptr, col, val) triples, and its rebuild count."""
+        nl, nu, rb = C.c_long(), C.c_long(), C.c_int()
+        self._check(lib().mhfem_isai_info(self.ctx, C.byref(nl), C.byref(nu), C.byref(rb)))
+        rows, nnzA = C.c_int(), C.c_long()
+        self._check(lib().mhfem_system_size(self.ctx, C.byref(rows), C.byref(nnzA)))
+        out = []
+        for upper, nnz in ((0, nl.value), (1, nu.value)):
+            rp = np.empty(rows.value + 1, np.int32)
+            col = np.empty(nnz, np.int32)
+            val = np.empty(nnz)
+            self._check(lib().mhfem_isai_matrix(self.ctx, upper, _ptr(rp), _ptr(col), _ptr(val)))
+            out.append((rp, col, val))
+        return out[0], out[1], rb.value
 
     def saveLinearSystem(self, directory):
         """Solver::saveLinearSystem (Solver.hpp:966-977): matrix.mtx, dof.vec.tnl
