@@ -74,6 +74,22 @@ def isai(T, lower, k=SPARSITY_POWER):
     return M
 
 
+def isai_rows(T, rows, lower, k=SPARSITY_POWER):
+    """Rows `rows` of the ISAI of T, each as (pattern, values); for spot checks
+    of large systems (T in CSR with sorted columns)."""
+    T = sp.csr_matrix(T)
+    out = []
+    for i in rows:
+        J = T.indices[T.indptr[i]:T.indptr[i + 1]]
+        for _ in range(k - 1):
+            J = np.unique(np.concatenate([T.indices[T.indptr[t]:T.indptr[t + 1]] for t in J]))
+        sub = T[J][:, J].toarray()
+        e = np.zeros(J.size)
+        e[np.searchsorted(J, i)] = 1.0
+        out.append((J, np.linalg.solve(sub.T, e)))
+    return out
+
+
 class IluIsai:
     """y = M_U (M_L x) with M_L, M_U the ISAIs of A's ILU(0) factors."""
 

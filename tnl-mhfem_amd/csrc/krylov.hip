@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <unordered_map>
 
 #include "common.hpp"
@@ -773,8 +774,9 @@ __global__ void __launch_bounds__(KB) k_isai(int n, const int *__restrict__ mrp,
     }
 }
 
-// y = M x, CSR with L lanes per row; sc: flags of the single segment (skip
-// once it has halted) or nullptr
+// y = M x, CSR with L lanes per row (L is measured once per matrix: the best
+// choice depends on how the row lengths spread); sc: flags of the single
+// segment (skip once it has halted) or nullptr
 template <int L>
 __global__ void __launch_bounds__(KB) k_csr_mv(int n, const int *__restrict__ rp, const int *__restrict__ col,
                                                const double *__restrict__ val, const double *__restrict__ x,
@@ -791,11 +793,16 @@ __global__ void __launch_bounds__(KB) k_csr_mv(int n, const int *__restrict__ rp
     if (row < n && lane == 0) y[row] = s;
 }
 
+struct IsaiFactor {
+    DBuf<int> rp, col;  // pattern (that of L^2 or U^2), CSR
+    DBuf<double> val;
+    long nnz = 0;
+    int lanes = 8;  // lanes per row of the product, measured at the first build
+};
+
 struct Isai {
-    DBuf<int> lrp, lcol, urp, ucol;  // patterns of L^2 and U^2
-    DBuf<double> lval, uval, t;      // M_L, M_U; t = M_L x
-    long nl = 0, nu = 0;
-    int lanes_l = 8, lanes_u = 8;
+    IsaiFactor lo, up;  // M_L, M_U
+    DBuf<double> t;     // M_L x
     // reuse rule (Solver.hpp:1017, 1158-1160)
     long long last_iters = 0, updated_iters = 0;
     int rebuilds = 0;
@@ -969,13 +976,55 @@ static void build_fp32_order(Ctx &c, Krylov &k)
     k.permuted = true;
 }
 
+static void isai_mv(Ctx &c, const IsaiFactor &F, const double *x, double *y, const double *flag, int lanes = 0)
+{
+    const int n = c.rows;
+    auto go = [&](auto LC) {
+        constexpr int L = decltype(LC)::value;
+        hipLaunchKernelGGL((k_csr_mv<L>), (unsigned)(((long)n * L + KB - 1) / KB), KB, 0, c.stream, n, F.rp.p,
+                           F.col.p, F.val.p, x, y, flag);
+    };
+    switch (lanes ? lanes : F.lanes) {
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 16: go(std::integral_constant<int, 16>{}); break;
+    default: go(std::integral_constant<int, 32>{}); break;
+    }
+    MHFEM_HIP(hipGetLastError());
+}
+
+// the fastest lanes per row for this matrix: each candidate timed over three
+// launches after one warm-up launch (x, y: any vectors of the system's length)
+static int isai_tune_lanes(Ctx &c, const IsaiFactor &F, const double *x, double *y)
+{
+    hipEvent_t e0, e1;
+    MHFEM_HIP(hipEventCreate(&e0));
+    MHFEM_HIP(hipEventCreate(&e1));
+    int best = 8;
+    float best_ms = 1e30f;
+    for (int lanes : {2, 4, 8, 16, 32}) {
+        isai_mv(c, F, x, y, nullptr, lanes);
+        MHFEM_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 3; r++) isai_mv(c, F, x, y, nullptr, lanes);
+        MHFEM_HIP(hipEventRecord(e1, c.stream));
+        MHFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MHFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) best_ms = ms, best = lanes;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
 // patterns of L^2 and U^2 from the factors' pattern (that of the matrix):
 // row i of L^2 is the union of the rows t of L that row i of L reaches
 static void isai_patterns(Ctx &c, Krylov &k, const std::vector<int> &dg)
 {
     const int n = k.n;
     const std::vector<int> &rp = c.h_rowptr, &col = c.h_col;
-    auto build = [&](bool lower, DBuf<int> &drp, DBuf<int> &dcol, long &nnz, int &lanes) {
+    auto build = [&](bool lower, IsaiFactor &F) {
         std::vector<int> prp(n + 1, 0), pcol, mark(n, -1), row;
         pcol.reserve((size_t)rp[n] * 2);
         for (int i = 0; i < n; i++) {
@@ -993,19 +1042,16 @@ static void isai_patterns(Ctx &c, Krylov &k, const std::vector<int> &dg)
             if (pcol.size() > (size_t)INT32_MAX) throw Error{MHFEM_E_ARGUMENT, "ILU-ISAI pattern exceeds 2^31 entries"};
             prp[i + 1] = (int)pcol.size();
         }
-        nnz = (long)pcol.size();
-        const long avg = nnz / std::max(1, n);
-        lanes = avg <= 6 ? 4 : avg <= 12 ? 8 : avg <= 24 ? 16 : 32;
-        drp.alloc(n + 1);
-        dcol.alloc(nnz);
-        MHFEM_HIP(hipMemcpy(drp.p, prp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-        MHFEM_HIP(hipMemcpy(dcol.p, pcol.data(), nnz * sizeof(int), hipMemcpyHostToDevice));
+        F.nnz = (long)pcol.size();
+        F.rp.alloc(n + 1);
+        F.col.alloc(F.nnz);
+        F.val.alloc(F.nnz);
+        MHFEM_HIP(hipMemcpy(F.rp.p, prp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(F.col.p, pcol.data(), F.nnz * sizeof(int), hipMemcpyHostToDevice));
     };
     Isai &S = k.isai;
-    build(true, S.lrp, S.lcol, S.nl, S.lanes_l);
-    build(false, S.urp, S.ucol, S.nu, S.lanes_u);
-    S.lval.alloc(S.nl);
-    S.uval.alloc(S.nu);
+    build(true, S.lo);
+    build(false, S.up);
     S.t.alloc(n);
 }
 
@@ -1275,20 +1321,8 @@ struct Solve {
         if constexpr (sizeof(T) == 8) {
             const Isai &S = k.isai;
             const double *flag = (skip_when_halted && k.nseg == 1) ? k.sc.p : nullptr;
-            auto mv = [&](int lanes, const int *rp, const int *col, const double *val, const double *in, double *out) {
-                auto go = [&](auto LC) {
-                    constexpr int L = decltype(LC)::value;
-                    const long threads = (long)n * L;
-                    hipLaunchKernelGGL((k_csr_mv<L>), (unsigned)((threads + KB - 1) / KB), KB, 0, c.stream, n, rp,
-                                       col, val, in, out, flag);
-                };
-                if (lanes == 4) go(std::integral_constant<int, 4>{});
-                else if (lanes == 8) go(std::integral_constant<int, 8>{});
-                else if (lanes == 16) go(std::integral_constant<int, 16>{});
-                else go(std::integral_constant<int, 32>{});
-            };
-            mv(S.lanes_l, S.lrp.p, S.lcol.p, S.lval.p, x, S.t.p);
-            mv(S.lanes_u, S.urp.p, S.ucol.p, S.uval.p, S.t.p, y);
+            isai_mv(c, S.lo, x, S.t.p, flag);
+            isai_mv(c, S.up, S.t.p, y, flag);
             MHFEM_HIP(hipGetLastError());
         } else {
             (void)x, (void)y, (void)skip_when_halted;
@@ -1859,11 +1893,14 @@ static void precond_update(Ctx &c, Krylov &k)
         if (!reuse) {
             ilu_factor(c, k);
             const Ilu0 &I = k.ilu;
-            hipLaunchKernelGGL(k_isai<true>, grid_for(n), KB, 0, c.stream, n, S.lrp.p, S.lcol.p, c.rowptr.p,
-                               c.csr_col.p, I.diag.p, I.lu.p, S.lval.p);
-            hipLaunchKernelGGL(k_isai<false>, grid_for(n), KB, 0, c.stream, n, S.urp.p, S.ucol.p, c.rowptr.p,
-                               c.csr_col.p, I.diag.p, I.lu.p, S.uval.p);
+            const unsigned gr = (unsigned)((n + KB - 1) / KB);
+            hipLaunchKernelGGL(k_isai<true>, gr, KB, 0, c.stream, n, S.lo.rp.p, S.lo.col.p, c.rowptr.p, c.csr_col.p,
+                               I.diag.p, I.lu.p, S.lo.val.p);
+            hipLaunchKernelGGL(k_isai<false>, gr, KB, 0, c.stream, n, S.up.rp.p, S.up.col.p, c.rowptr.p, c.csr_col.p,
+                               I.diag.p, I.lu.p, S.up.val.p);
             MHFEM_HIP(hipGetLastError());
+            if (S.rebuilds == 0)
+                for (IsaiFactor *F : {&S.lo, &S.up}) F->lanes = isai_tune_lanes(c, *F, k.dinv.p, S.t.p);
             S.rebuilds++;
         }
     }
@@ -1880,24 +1917,23 @@ Amg *krylov_amg(const Krylov &k) { return k.amg; }
 
 bool krylov_isai_info(const Krylov &k, long *nnz_lower, long *nnz_upper, int *rebuilds)
 {
-    if (!k.isai.lrp.p) return false;
-    if (nnz_lower) *nnz_lower = k.isai.nl;
-    if (nnz_upper) *nnz_upper = k.isai.nu;
+    if (!k.isai.lo.rp.p) return false;
+    if (nnz_lower) *nnz_lower = k.isai.lo.nnz;
+    if (nnz_upper) *nnz_upper = k.isai.up.nnz;
     if (rebuilds) *rebuilds = k.isai.rebuilds;
     return true;
 }
 
 void krylov_isai_export(Ctx &c, int upper, int32_t *rowptr, int32_t *col, double *val)
 {
-    const Isai &S = c.krylov->isai;
+    const IsaiFactor &F = upper ? c.krylov->isai.up : c.krylov->isai.lo;
     const int n = c.krylov->n;
-    const long nnz = upper ? S.nu : S.nl;
     auto get = [&](void *dst, const void *src, size_t bytes) {
         if (dst) MHFEM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
     };
-    get(rowptr, upper ? S.urp.p : S.lrp.p, (n + 1) * sizeof(int));
-    get(col, upper ? S.ucol.p : S.lcol.p, nnz * sizeof(int));
-    get(val, upper ? S.uval.p : S.lval.p, nnz * sizeof(double));
+    get(rowptr, F.rp.p, (n + 1) * sizeof(int));
+    get(col, F.col.p, F.nnz * sizeof(int));
+    get(val, F.val.p, F.nnz * sizeof(double));
     MHFEM_HIP(hipStreamSynchronize(c.stream));
 }
 
