@@ -26,7 +26,9 @@ def init_py_initial_condition(points, cells, alpha=-0.05, beta=-0.05, gamma=5.0)
 
 
 @pytest.mark.gpu
-def test_config_driven_heat_example_matches_oracle(tmp_path):
+@pytest.mark.parametrize("solver,prec", [("gmres", "diagonal"), ("cg", "diagonal"), ("tfqmr", "ILU_ISAI"),
+                                         ("bicgstabl", "AMGX")])
+def test_config_driven_heat_example_matches_oracle(tmp_path, solver, prec):
     import pyoracle
 
     points, cells = meshgen.unit_square_triangles(16)
@@ -53,15 +55,15 @@ snapshot-period = 0.1
 time-step = 0.05
 
 # MHFEM linear system solver
-linear-solver = gmres
+linear-solver = {solver}
 gmres-restarting-max = 30
-preconditioner = diagonal
+preconditioner = {prec}
 convergence-residue = 1e-11
 
 # Heat equation model parameters
 diffusivity = 1e-3
 """)
-    print("case: heat example --config, s = 16, 6 steps", flush=True)
+    print(f"case: heat example --config, s = 16, 6 steps, {solver} + {prec}", flush=True)
     res = subprocess.run([EXAMPLE, "--config", str(tmp_path / "config.ini")], capture_output=True, text=True,
                          timeout=120)
     print(res.stdout[-2000:], res.stderr[-2000:])
