@@ -110,9 +110,19 @@ enum {
 /* MHFEM_PREC_ILU_ISAI: the Ginkgo branch's "ILU_ISAI" (Solver.hpp:1089-1130):
  * ILU(0) factors replaced by incomplete sparse approximate inverses on the
  * patterns of L^2 and U^2, applied as two sparse products; every row of the
- * inverses is solved exactly.  Same reuse rule, configurations and limits as
- * MHFEM_PREC_AMG.  (PARILU_ISAI and PARILUT_ISAI are not provided.) */
-enum { MHFEM_PREC_NONE = 0, MHFEM_PREC_DIAGONAL = 1, MHFEM_PREC_ILU0 = 2, MHFEM_PREC_AMG = 3, MHFEM_PREC_ILU_ISAI = 4 };
+ * inverses is solved exactly.  MHFEM_PREC_PARILU_ISAI ("PARILU_ISAI"): the
+ * same with ParILU factors (5 synchronous fixed-point sweeps of the ILU(0)
+ * equations instead of the level-scheduled elimination).  Same reuse rule,
+ * configurations and limits as MHFEM_PREC_AMG.  (PARILUT_ISAI is not
+ * provided.) */
+enum {
+    MHFEM_PREC_NONE = 0,
+    MHFEM_PREC_DIAGONAL = 1,
+    MHFEM_PREC_ILU0 = 2,
+    MHFEM_PREC_AMG = 3,
+    MHFEM_PREC_ILU_ISAI = 4,
+    MHFEM_PREC_PARILU_ISAI = 5
+};
 /* precision of the Krylov iteration.  FP32_REFINED: FP64 iterative
    refinement whose every correction is an FP32 BiCGStab solve started from
    zero to `inner_residue`.  FP32_RELIABLE: one FP32 BiCGStab recurrence whose

@@ -65,8 +65,9 @@ public:
         else if (prec == "ilu0") cfg_.preconditioner = MHFEM_PREC_ILU0;
         // the reference's Ginkgo "AMGX" preconditioner (Solver.hpp:1025-1075)
         else if (prec == "amg" || prec == "AMGX") cfg_.preconditioner = MHFEM_PREC_AMG;
-        // its "ILU_ISAI" (Solver.hpp:1089-1130); PARILU_ISAI and PARILUT_ISAI are not built
+        // its "ILU_ISAI" and "PARILU_ISAI" (Solver.hpp:1089-1130); PARILUT_ISAI is not built
         else if (prec == "ilu-isai" || prec == "ILU_ISAI") cfg_.preconditioner = MHFEM_PREC_ILU_ISAI;
+        else if (prec == "parilu-isai" || prec == "PARILU_ISAI") cfg_.preconditioner = MHFEM_PREC_PARILU_ISAI;
         else return false;
         cfg_.convergence_residue = p.getParameter<double>("convergence-residue", 1e-6);
         cfg_.max_iterations = p.getParameter<int>("max-iterations", 100000);

@@ -1,7 +1,8 @@
-"""CPU restatement of the ILU-ISAI preconditioner (the reference's Ginkgo
-"ILU_ISAI" configuration, Solver.hpp:1089-1130): ILU(0) factors L (unit lower)
-and U, each replaced by an incomplete sparse approximate inverse whose
-pattern is that of the factor's second power, applied as y = M_U (M_L x).
+"""CPU restatement of the ILU-ISAI preconditioners (the reference's Ginkgo
+"ILU_ISAI" and "PARILU_ISAI" configurations, Solver.hpp:1089-1130): ILU(0)
+factors L (unit lower) and U, exact or by ParILU sweeps, each replaced by an
+incomplete sparse approximate inverse whose pattern is that of the factor's
+second power, applied as y = M_U (M_L x).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the HIP
 library's ILU-ISAI preconditioner.  Ginkgo is not in the reference tree; this
@@ -38,6 +39,51 @@ def ilu0(A):
                 p = pos.get(col[t])
                 if p is not None:
                     lu[p] -= lu[q] * lu[t]
+    LU = sp.csr_matrix((lu, col, rp), shape=A.shape)
+    L = sp.tril(LU, -1, format="csr") + sp.identity(n, format="csr")
+    U = sp.triu(LU, 0, format="csr")
+    L.sort_indices()
+    U.sort_indices()
+    return L, U
+
+
+PARILU_SWEEPS = 5  # synchronous sweeps of the ParILU fixed-point iteration
+
+
+def parilu(A, sweeps=PARILU_SWEEPS):
+    """ParILU (Chow & Patel 2015): the ILU(0) equations on A's pattern S,
+        l_ij u_jj + sum_{k<j} l_ik u_kj = a_ij   (i > j),
+        u_ij + sum_{k<i} l_ik u_kj = a_ij         (i <= j),
+    solved by fixed-point sweeps from l_ij = a_ij / a_jj, u_ij = a_ij.  Each
+    sweep evaluates every entry from the previous sweep's values (synchronous
+    sweeps; Ginkgo updates in place, asynchronously).  Returns (L, U) as
+    ilu0 does; the fixed point is ILU(0)."""
+    A = sp.csr_matrix(A, dtype=float, copy=True)
+    A.sort_indices()
+    rp, col, a = A.indptr, A.indices, A.data
+    n = A.shape[0]
+    diag = np.array([rp[i] + np.searchsorted(col[rp[i]:rp[i + 1]], i) for i in range(n)])
+    pos = [{col[q]: q for q in range(rp[i], rp[i + 1])} for i in range(n)]
+    lu = a.copy()
+    for i in range(n):
+        for q in range(rp[i], diag[i]):
+            lu[q] = a[q] / a[diag[col[q]]]
+    for _ in range(sweeps):
+        new = np.empty_like(lu)
+        for i in range(n):
+            for q in range(rp[i], rp[i + 1]):
+                j = col[q]
+                m = min(i, j)
+                acc = a[q]
+                for q2 in range(rp[i], rp[i + 1]):
+                    k = col[q2]
+                    if k >= m:
+                        break
+                    p = pos[k].get(j)
+                    if p is not None:
+                        acc -= lu[q2] * lu[p]
+                new[q] = acc / lu[diag[j]] if i > j else acc
+        lu = new
     LU = sp.csr_matrix((lu, col, rp), shape=A.shape)
     L = sp.tril(LU, -1, format="csr") + sp.identity(n, format="csr")
     U = sp.triu(LU, 0, format="csr")
@@ -91,10 +137,11 @@ def isai_rows(T, rows, lower, k=SPARSITY_POWER):
 
 
 class IluIsai:
-    """y = M_U (M_L x) with M_L, M_U the ISAIs of A's ILU(0) factors."""
+    """y = M_U (M_L x) with M_L, M_U the ISAIs of A's ILU(0) factors (or of
+    its ParILU factors with `sweeps` synchronous sweeps)."""
 
-    def __init__(self, A, k=SPARSITY_POWER):
-        self.L, self.U = ilu0(A)
+    def __init__(self, A, k=SPARSITY_POWER, sweeps=None):
+        self.L, self.U = ilu0(A) if sweeps is None else parilu(A, sweeps)
         self.ML = isai(self.L, True, k)
         self.MU = isai(self.U, False, k)
 

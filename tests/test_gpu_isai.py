@@ -1,8 +1,8 @@
-"""The ILU-ISAI preconditioner on the GPU against its oracle
-(oracle/isai_oracle.py): the approximate inverses (patterns bit for bit,
-values to 1e-12), the BiCGStab iteration count from the same x0, whole time
-steps against the scheme oracle, the reuse rule over a trajectory, and the
-configurations it refuses."""
+"""The ILU-ISAI and ParILU-ISAI preconditioners on the GPU against their
+oracle (oracle/isai_oracle.py): the approximate inverses (patterns bit for
+bit, values to 1e-12), the BiCGStab iteration count from the same x0, whole
+time steps against the scheme oracle, the reuse rule over a trajectory, and
+the configurations they refuse."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -26,18 +26,22 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+FACTORS = {"ilu-isai": None, "parilu-isai": isai.PARILU_SWEEPS}
+
+
+@pytest.mark.parametrize("prec", sorted(FACTORS))
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_inverses_and_iterations_match_oracle(name):
-    print(f"\n[case] {name}", flush=True)
+def test_inverses_and_iterations_match_oracle(name, prec):
+    print(f"\n[case] {name} {prec}", flush=True)
     pr = CASES[name]()
-    s = capi.Solver(pr, preconditioner="ilu-isai", tol=1e-10)
+    s = capi.Solver(pr, preconditioner=prec, tol=1e-10)
     s.preIterate(0.0, pr.dt)
     s.assembleLinearSystem(0.0, pr.dt)
     rp, col, val, rhs = s.export_system()
     x0 = s.get_array("Z_iF")
     x, st = s.solve_rhs(rhs, x0)
     A = sp.csr_matrix((val, col, rp))
-    P = isai.IluIsai(A)
+    P = isai.IluIsai(A, sweeps=FACTORS[prec])
     (lrp, lcol, lval), (urp, ucol, uval), rebuilds = s.isai_matrices()
     assert rebuilds == 1
     for (grp, gcol, gval), M in (((lrp, lcol, lval), P.ML), ((urp, ucol, uval), P.MU)):
@@ -50,11 +54,12 @@ def test_inverses_and_iterations_match_oracle(name):
     s.close()
 
 
+@pytest.mark.parametrize("prec", sorted(FACTORS))
 @pytest.mark.parametrize("name", ["darcy3d", "hex_n2"])
-def test_steps_match_scheme_oracle_and_reuse_rule(name):
-    print(f"\n[case] {name}", flush=True)
+def test_steps_match_scheme_oracle_and_reuse_rule(name, prec):
+    print(f"\n[case] {name} {prec}", flush=True)
     pr = CASES[name]()
-    s = capi.Solver(pr, preconditioner="ilu-isai", tol=1e-13)
+    s = capi.Solver(pr, preconditioner=prec, tol=1e-13)
     o = pyoracle.Oracle(pr)
     rule, rebuilds, its = amg.ReuseRule(), 0, []
     t = 0.0

@@ -120,3 +120,36 @@ def test_isai_rows_equal_the_full_construction():
         for i, (J, v) in zip(rows, isai.isai_rows(T, rows, lower)):
             assert np.array_equal(J, M.indices[M.indptr[i]:M.indptr[i + 1]])
             assert np.array_equal(v, M.data[M.indptr[i]:M.indptr[i + 1]])
+
+
+def test_parilu_fixed_point_is_ilu0():
+    """ParILU's synchronous sweeps reach ILU(0) exactly once their number
+    exceeds the depth of the factors' dependency chains (here a small system),
+    and move towards it before that."""
+    A, _ = _system("darcy3d")
+    A = A[:200, :200].tocsr()
+    L0, U0 = isai.ilu0(A)
+    ref = abs(L0).max() + abs(U0).max()
+    errs = []
+    for sweeps in (0, 1, 2, 3, 5):
+        L, U = isai.parilu(A, sweeps)
+        errs.append(max(abs(L - L0).max(), abs(U - U0).max()) / ref)
+    assert errs[0] > 1e-3 and errs[-1] < errs[0] / 10, errs
+    L, U = isai.parilu(A, 200)
+    assert max(abs(L - L0).max(), abs(U - U0).max()) <= 1e-13 * ref
+
+
+def test_parilu_initial_guess_and_one_sweep_by_hand():
+    """Zero sweeps: l_ij = a_ij / a_jj, u_ij = a_ij.  One sweep on a 3x3 matrix
+    with a full pattern, worked by hand from those values."""
+    a = np.array([[4.0, 1.0, 2.0], [1.0, 5.0, 1.0], [2.0, 1.0, 6.0]])
+    A = sp.csr_matrix(a)
+    L, U = isai.parilu(A, 0)
+    assert np.allclose(L.toarray(), [[1, 0, 0], [0.25, 1, 0], [0.5, 0.2, 1]], rtol=0, atol=1e-15)
+    assert np.allclose(U.toarray(), np.triu(a), rtol=0, atol=1e-15)
+    L, U = isai.parilu(A, 1)
+    # from l10 = .25, l20 = .5, l21 = .2, u = triu(a):
+    # u11 = 5 - l10 u01 = 4.75; u12 = 1 - l10 u02 = 0.5; u22 = 6 - l20 u02 - l21 u12 = 4.8
+    # l21 = (1 - l20 u01) / u11 = 0.5 / 5 = 0.1 (old u11 = 5)
+    assert np.allclose(U.toarray(), [[4, 1, 2], [0, 4.75, 0.5], [0, 0, 4.8]], rtol=0, atol=1e-15)
+    assert np.allclose(L.toarray(), [[1, 0, 0], [0.25, 1, 0], [0.5, 0.1, 1]], rtol=0, atol=1e-15)

@@ -719,9 +719,9 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
         if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_JACOBI) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
-        if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_ILU_ISAI)
+        if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_PARILU_ISAI)
             throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
-        const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || cfg->preconditioner == MHFEM_PREC_ILU_ISAI;
+        const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || is_isai(cfg->preconditioner);
         const char *pname = cfg->preconditioner == MHFEM_PREC_AMG ? "AMG" : "ILU-ISAI";
         if (reused && cfg->precision != MHFEM_PRECISION_FP64)
             throw Error{MHFEM_E_ARGUMENT, std::string("the ") + pname + " preconditioner works with the FP64 solvers"};

@@ -334,6 +334,7 @@ double dot64(Ctx &c, const double *x, const double *y);
 // the assembled matrix's values in CSR order (unit diagonal included) into dst[nnz]
 void launch_csr_gather(Ctx &c, double *dst);
 
+inline bool is_isai(int prec) { return prec == MHFEM_PREC_ILU_ISAI || prec == MHFEM_PREC_PARILU_ISAI; }
 Amg *krylov_amg(const Krylov &k);  // the AMG preconditioner of a solver, or nullptr
 // the ILU-ISAI preconditioner's sizes and rebuild count (false: none); copy of
 // M_L (upper = 0) or M_U (upper = 1) in CSR, NULL outputs skipped

@@ -692,6 +692,41 @@ __global__ void __launch_bounds__(KB) k_ilu_factor(const int *__restrict__ rows,
     }
 }
 
+// ParILU (Chow & Patel 2015): the ILU(0) equations on the matrix pattern,
+//   l_ij u_jj + sum_{k<j} l_ik u_kj = a_ij (i > j),  u_ij + sum_{k<i} l_ik u_kj = a_ij (i <= j),
+// solved by synchronous fixed-point sweeps: every entry of a sweep is
+// computed from the previous sweep's values (src), so sweeps are
+// deterministic (Ginkgo's ParIlu updates in place, asynchronously).  One lane
+// per row; u_kj is found in row k right of its diagonal (k < j there).
+__global__ void __launch_bounds__(KB) k_parilu_sweep(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                                     const int *__restrict__ dg, const double *__restrict__ a,
+                                                     const double *__restrict__ src, double *__restrict__ dst)
+{
+    const int i = blockIdx.x * KB + threadIdx.x;
+    if (i >= n) return;
+    const int b = rp[i], e = rp[i + 1];
+    for (int q = b; q < e; q++) {
+        const int j = col[q], m = min(i, j);
+        double acc = a[q];
+        for (int q2 = b; q2 < e && col[q2] < m; q2++) {
+            const int kk = col[q2];
+            for (int t = dg[kk]; t < rp[kk + 1] && col[t] <= j; t++)
+                if (col[t] == j) acc -= src[q2] * src[t];
+        }
+        dst[q] = i > j ? acc / src[dg[j]] : acc;
+    }
+}
+
+// the sweeps' starting point: l_ij = a_ij / a_jj, u_ij = a_ij
+__global__ void __launch_bounds__(KB) k_parilu_init(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                                    const int *__restrict__ dg, const double *__restrict__ a,
+                                                    double *__restrict__ lu)
+{
+    const int i = blockIdx.x * KB + threadIdx.x;
+    if (i >= n) return;
+    for (int q = rp[i]; q < rp[i + 1]; q++) lu[q] = q < dg[i] ? a[q] / a[dg[col[q]]] : a[q];
+}
+
 // sc: flags of the single segment (skip once it has halted) or nullptr
 template <typename T>
 __global__ void __launch_bounds__(KB) k_ilu_lower(const int *__restrict__ rows, int count, const int *__restrict__ rp,
@@ -725,6 +760,7 @@ struct Ilu0 {
     DBuf<int> lrows, urows, diag;     // rows in level order, diagonal positions
     DBuf<double> lu;
     DBuf<float> lu32;
+    DBuf<double> a0, swp;  // ParILU: the matrix in CSR order, the other sweep buffer
 };
 
 // ---------------------------------------------------------------------------
@@ -1071,7 +1107,7 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipEventCreateWithFlags(&k->ev_ahead[q], hipEventDisableTiming));
     }
     build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
-    if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) {
+    if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || is_isai(c.cfg.preconditioner)) {
         Ilu0 &I = k->ilu;
         std::vector<int> dg(n), lev(n), ulev(n);
         for (int r = 0; r < n; r++)
@@ -1109,7 +1145,11 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipMemcpy(I.diag.p, dg.data(), n * sizeof(int), hipMemcpyHostToDevice));
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) isai_patterns(c, *k, dg);
+        if (is_isai(c.cfg.preconditioner)) isai_patterns(c, *k, dg);
+        if (c.cfg.preconditioner == MHFEM_PREC_PARILU_ISAI) {
+            I.a0.alloc(c.nnz);
+            I.swp.alloc(c.nnz);
+        }
     }
     if (c.cfg.preconditioner == MHFEM_PREC_AMG) k->amg = amg_create();
     if (c.cfg.precision != MHFEM_PRECISION_FP64) {
@@ -1333,13 +1373,13 @@ struct Solve {
     bool separate_prec() const
     {
         const int p = c.cfg.preconditioner;
-        return p == MHFEM_PREC_ILU0 || p == MHFEM_PREC_AMG || p == MHFEM_PREC_ILU_ISAI;
+        return p == MHFEM_PREC_ILU0 || p == MHFEM_PREC_AMG || is_isai(p);
     }
     template <typename T>
     void separate_apply(const T *x, T *y, bool skip_when_halted)
     {
         if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_apply_t<T>(x, y, skip_when_halted);
-        else if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) isai_apply_t<T>(x, y, skip_when_halted);
+        else if (is_isai(c.cfg.preconditioner)) isai_apply_t<T>(x, y, skip_when_halted);
         else ilu_apply<T>(x, y, skip_when_halted);
     }
     template <typename T>
@@ -1859,6 +1899,27 @@ static void ilu_factor(Ctx &c, Krylov &k)
     MHFEM_HIP(hipGetLastError());
 }
 
+// ParILU factors of the current matrix into k.ilu.lu (CSR order)
+constexpr int PARILU_SWEEPS = 5;
+static void parilu_factor(Ctx &c, Krylov &k)
+{
+    Ilu0 &I = k.ilu;
+    const int n = k.n;
+    const unsigned gr = (unsigned)((n + KB - 1) / KB);
+    hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p, c.csr_to_sell.p,
+                       I.a0.p);
+    hipLaunchKernelGGL(k_parilu_init, gr, KB, 0, c.stream, n, c.rowptr.p, c.csr_col.p, I.diag.p, I.a0.p, I.lu.p);
+    double *src = I.lu.p, *dst = I.swp.p;
+    for (int s = 0; s < PARILU_SWEEPS; s++) {
+        hipLaunchKernelGGL(k_parilu_sweep, gr, KB, 0, c.stream, n, c.rowptr.p, c.csr_col.p, I.diag.p, I.a0.p, src,
+                           dst);
+        std::swap(src, dst);
+    }
+    if (src != I.lu.p)
+        MHFEM_HIP(hipMemcpyAsync(I.lu.p, src, (size_t)c.nnz * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    MHFEM_HIP(hipGetLastError());
+}
+
 static void precond_update(Ctx &c, Krylov &k)
 {
     const int n = k.n;
@@ -1885,13 +1946,14 @@ static void precond_update(Ctx &c, Krylov &k)
         MHFEM_HIP(hipGetLastError());
     }
     if (c.cfg.preconditioner == MHFEM_PREC_AMG) amg_update(c, *k.amg);
-    if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) {
+    if (is_isai(c.cfg.preconditioner)) {
         // rebuilt under the reuse rule of the reference's Ginkgo branch
         Isai &S = k.isai;
         const bool reuse = S.rebuilds > 0 && S.last_iters <= std::min(2 * S.updated_iters, S.updated_iters + 5);
         S.rebuilt = !reuse;
         if (!reuse) {
-            ilu_factor(c, k);
+            if (c.cfg.preconditioner == MHFEM_PREC_PARILU_ISAI) parilu_factor(c, k);
+            else ilu_factor(c, k);
             const Ilu0 &I = k.ilu;
             const unsigned gr = (unsigned)((n + KB - 1) / KB);
             hipLaunchKernelGGL(k_isai<true>, gr, KB, 0, c.stream, n, S.lo.rp.p, S.lo.col.p, c.rowptr.p, c.csr_col.p,
@@ -2266,7 +2328,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
     }
     if (c.guess && c.guess_on_solve) guess_after_solve(c, *c.guess, x);
     if (k.amg) amg_after_solve(*k.amg, out.iterations);
-    if (c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI) {
+    if (is_isai(c.cfg.preconditioner)) {
         k.isai.last_iters = out.iterations;
         if (k.isai.rebuilt || k.isai.updated_iters == 0) k.isai.updated_iters = out.iterations;
     }
@@ -2306,7 +2368,7 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
     Krylov &k = *c.krylov;
     if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64)
         throw Error{MHFEM_E_ARGUMENT, "graph steps use FP64 BiCGStab"};
-    if (c.cfg.preconditioner == MHFEM_PREC_AMG || c.cfg.preconditioner == MHFEM_PREC_ILU_ISAI)
+    if (c.cfg.preconditioner == MHFEM_PREC_AMG || is_isai(c.cfg.preconditioner))
         throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG or ILU-ISAI preconditioners (their reuse rule "
                                       "runs on the host); use the four calls"};
     const int every = std::max(1, c.cfg.check_every);

@@ -22,7 +22,7 @@ ARRAYS = ["Z_iF", "Z_iK", "N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe",
 ARRAY_ID = {name: k for k, name in enumerate(ARRAYS)}
 
 SOLVERS = {"bicgstab": 0, "gmres": 1, "cg": 2, "tfqmr": 3, "jacobi": 4}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3, "ilu-isai": 4}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3, "ilu-isai": 4, "parilu-isai": 5}
 PRECISIONS = {"fp64": 0, "fp32-refined": 1, "fp32-reliable": 2}
 
 
