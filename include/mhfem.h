@@ -91,14 +91,21 @@ typedef struct {
  * reference; CG for symmetric positive definite systems; TFQMR, restarted
  * from the current iterate on a breakdown; JACOBI the stationary iteration
  * x += M^{-1}(b - A x), which is the Jacobi method on the unit-diagonal MHFEM
- * rows.  All stop on ||M^{-1}(b - A x)|| / ||M^{-1} b|| < convergence_residue.
+ * rows; BICGSTABL BiCGStab(l) (Sleijpen & Fokkema), one iteration per BiCG
+ * step, convergence checked per cycle of l; IDRS IDR(s) with
+ * biorthogonalisation (van Gijzen & Sonneveld, Alg. 913), one iteration per
+ * operator application, on a fixed pseudo-random orthonormal shadow space.
+ * All are left preconditioned and stop on
+ * ||M^{-1}(b - A x)|| / ||M^{-1} b|| < convergence_residue.
  * The FP32 precisions, batches and graph steps take BiCGStab only. */
 enum {
     MHFEM_SOLVER_BICGSTAB = 0,
     MHFEM_SOLVER_GMRES = 1,
     MHFEM_SOLVER_CG = 2,
     MHFEM_SOLVER_TFQMR = 3,
-    MHFEM_SOLVER_JACOBI = 4
+    MHFEM_SOLVER_JACOBI = 4,
+    MHFEM_SOLVER_BICGSTABL = 5,
+    MHFEM_SOLVER_IDRS = 6
 };
 /* MHFEM_PREC_AMG: algebraic multigrid in the configuration of the
  * reference's Ginkgo branch (Solver.hpp:1025-1075): parallel graph matching
@@ -148,6 +155,8 @@ typedef struct {
                                    FP32 matrix copy and vectors in an order that groups faces of
                                    one type inside windows of 4096 faces, for coalesced gathers
                                    (results unchanged up to rounding); 0 (default): natural order */
+    int bicgstab_ell;           /* l of BiCGStab(l) ("bicgstab-ell"), 1..8; 0: 1 */
+    int idr_s;                  /* s of IDR(s) ("idr-s"), 1..8; 0: 4 */
 } mhfem_solver_config;
 
 typedef struct {

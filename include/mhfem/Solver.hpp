@@ -52,14 +52,15 @@ public:
         const std::string prec = p.getParameter<std::string>("preconditioner", std::string("none"));
         cfg_ = mhfem_solver_config{};
         if (solver == "bicgstab") cfg_.solver = MHFEM_SOLVER_BICGSTAB;
-        // BiCGStab(l) with l = 1 is BiCGStab; larger l, SOR and IDR(s) are not built
-        else if (solver == "bicgstabl" && p.getParameter<int>("bicgstab-ell", 1) == 1)
-            cfg_.solver = MHFEM_SOLVER_BICGSTAB;
+        else if (solver == "bicgstabl") cfg_.solver = MHFEM_SOLVER_BICGSTABL;
         else if (solver == "gmres") cfg_.solver = MHFEM_SOLVER_GMRES;
         else if (solver == "cg") cfg_.solver = MHFEM_SOLVER_CG;
         else if (solver == "tfqmr") cfg_.solver = MHFEM_SOLVER_TFQMR;
         else if (solver == "jacobi") cfg_.solver = MHFEM_SOLVER_JACOBI;
-        else return false;
+        else if (solver == "idrs") cfg_.solver = MHFEM_SOLVER_IDRS;
+        else return false;  // "sor" is not built
+        cfg_.bicgstab_ell = p.getParameter<int>("bicgstab-ell", 1);
+        cfg_.idr_s = p.getParameter<int>("idr-s", 4);
         if (prec == "none") cfg_.preconditioner = MHFEM_PREC_NONE;
         else if (prec == "diagonal") cfg_.preconditioner = MHFEM_PREC_DIAGONAL;
         else if (prec == "ilu0") cfg_.preconditioner = MHFEM_PREC_ILU0;

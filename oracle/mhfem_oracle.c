@@ -1366,6 +1366,196 @@ restart:
     return resid < tol;
 }
 
+/* BiCGStab(l) (Sleijpen & Fokkema 1993) on M^{-1} A x = M^{-1} b: l BiCG
+ * steps, then a minimal-residual polynomial of degree l over the l + 1
+ * residuals (modified Gram-Schmidt).  One iteration is one BiCG step (two
+ * operator applications, as one BiCGStab iteration); convergence is checked
+ * after each cycle of l.  On a breakdown (rho0 = 0, (u_{j+1}, r~) = 0 or
+ * sigma_j = 0) it restarts from the current x.  l = 1 is BiCGStab. */
+static void axpy(int n, double a, const double *x, double *y)
+{
+#pragma omp parallel for
+    for (int k = 0; k < n; k++) y[k] += a * x[k];
+}
+static int bicgstabl(krylov_op *op, const double *b, double *x, double tol, int maxit, int ell, int *iters,
+                     double *res)
+{
+    const int n = op->n;
+    if (ell < 1) ell = 1;
+    if (ell > 8) ell = 8;
+    double *R = malloc(sizeof(double) * n * (ell + 1)), *U = malloc(sizeof(double) * n * (ell + 1)),
+           *rt = malloc(sizeof(double) * n);
+    double tau[9][9], sigma[9], gp[9], g[9], gpp[9];
+    prec_apply(op, b, rt);
+    double b_norm = sqrt(dot(n, rt, rt));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0;
+    double resid, rho0 = 1, alpha = 0, omega = 1;
+restart:
+    presidue(op, x, b, R);
+    memcpy(rt, R, sizeof(double) * n);
+    memset(U, 0, sizeof(double) * n);
+    rho0 = 1, alpha = 0, omega = 1;
+    resid = sqrt(dot(n, R, R)) / b_norm;
+    while (resid >= tol && it < maxit && isfinite(resid)) {
+        rho0 = -omega * rho0;
+        for (int j = 0; j < ell; j++) {
+            it++;
+            const double rho1 = dot(n, R + (long)j * n, rt), beta = alpha * rho1 / rho0;
+            if (!(fabs(rho0) > 0.0) || !isfinite(beta)) goto restart;
+            rho0 = rho1;
+#pragma omp parallel for
+            for (int k = 0; k < n; k++)
+                for (int i = 0; i <= j; i++) U[(long)i * n + k] = R[(long)i * n + k] - beta * U[(long)i * n + k];
+            pmatvec(op, U + (long)j * n, U + (long)(j + 1) * n);
+            const double gam = dot(n, U + (long)(j + 1) * n, rt);
+            alpha = rho0 / gam;
+            if (!(fabs(gam) > 0.0) || !isfinite(alpha)) goto restart;
+#pragma omp parallel for
+            for (int k = 0; k < n; k++)
+                for (int i = 0; i <= j; i++) R[(long)i * n + k] -= alpha * U[(long)(i + 1) * n + k];
+            pmatvec(op, R + (long)j * n, R + (long)(j + 1) * n);
+            axpy(n, alpha, U, x);
+        }
+        for (int j = 1; j <= ell; j++) {
+            double *rj = R + (long)j * n;
+            for (int i = 1; i < j; i++) {
+                tau[i][j] = dot(n, rj, R + (long)i * n) / sigma[i];
+                axpy(n, -tau[i][j], R + (long)i * n, rj);
+            }
+            sigma[j] = dot(n, rj, rj);
+            if (!(sigma[j] > 0.0)) goto restart;
+            gp[j] = dot(n, R, rj) / sigma[j];
+        }
+        g[ell] = gp[ell];
+        omega = g[ell];
+        for (int j = ell - 1; j >= 1; j--) {
+            double t = gp[j];
+            for (int i = j + 1; i <= ell; i++) t -= tau[j][i] * g[i];
+            g[j] = t;
+        }
+        for (int j = 1; j < ell; j++) {
+            double t = g[j + 1];
+            for (int i = j + 1; i < ell; i++) t += tau[j][i] * g[i + 1];
+            gpp[j] = t;
+        }
+        axpy(n, g[1], R, x);
+        axpy(n, -gp[ell], R + (long)ell * n, R);
+        axpy(n, -g[ell], U + (long)ell * n, U);
+        for (int j = 1; j < ell; j++) {
+            axpy(n, -g[j], U + (long)j * n, U);
+            axpy(n, gpp[j], R + (long)j * n, x);
+            axpy(n, -gp[j], R + (long)j * n, R);
+        }
+        resid = sqrt(dot(n, R, R)) / b_norm;
+    }
+    *iters = it;
+    *res = resid;
+    free(R); free(U); free(rt);
+    return resid < tol;
+}
+
+/* the shadow space of IDR(s): s columns of a fixed pseudo-random matrix
+ * (an integer hash of (row, column) mapped to [-1, 1) with 24 bits, so the
+ * values are exact in any precision), orthonormalised by modified
+ * Gram-Schmidt */
+double or_idr_shadow_value(long i, int k)
+{
+    uint32_t h = (uint32_t)i * 0x9E3779B1u + (uint32_t)(k + 1) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (double)(h >> 8) * (2.0 / 16777216.0) - 1.0;
+}
+
+/* IDR(s) with biorthogonalisation (van Gijzen & Sonneveld 2011, Alg. 913) on
+ * M^{-1} A x = M^{-1} b, with the "maintaining the convergence" choice of
+ * omega (angle 0.7).  One iteration is one operator application (s + 1 per
+ * cycle); convergence is checked after each.  On a breakdown (M_kk = 0 or
+ * t = 0) it restarts from the current x. */
+static int idrs(krylov_op *op, const double *b, double *x, double tol, int maxit, int s, int *iters, double *res)
+{
+    const int n = op->n;
+    if (s < 1) s = 1;
+    if (s > 8) s = 8;
+    double *P = malloc(sizeof(double) * n * s), *G = malloc(sizeof(double) * n * s),
+           *U = malloc(sizeof(double) * n * s), *r = malloc(sizeof(double) * n), *v = malloc(sizeof(double) * n),
+           *t = malloc(sizeof(double) * n);
+    double Mm[8][8], f[8], c[8];
+    for (int k = 0; k < s; k++) {
+        double *pk = P + (long)k * n;
+        for (long i = 0; i < n; i++) pk[i] = or_idr_shadow_value(i, k);
+        for (int q = 0; q < k; q++) axpy(n, -dot(n, pk, P + (long)q * n), P + (long)q * n, pk);
+        const double nrm = sqrt(dot(n, pk, pk));
+        for (long i = 0; i < n; i++) pk[i] /= nrm;
+    }
+    prec_apply(op, b, r);
+    double b_norm = sqrt(dot(n, r, r));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0;
+    double resid, om;
+restart:
+    presidue(op, x, b, r);
+    memset(G, 0, sizeof(double) * n * s);
+    memset(U, 0, sizeof(double) * n * s);
+    for (int i = 0; i < s; i++)
+        for (int j = 0; j < s; j++) Mm[i][j] = i == j;
+    om = 1.0;
+    resid = sqrt(dot(n, r, r)) / b_norm;
+    while (resid >= tol && it < maxit && isfinite(resid)) {
+        for (int i = 0; i < s; i++) f[i] = dot(n, P + (long)i * n, r);
+        for (int k = 0; k < s && resid >= tol && it < maxit; k++) {
+            /* c = M(k:s, k:s) \ f(k:s), M lower triangular */
+            for (int i = k; i < s; i++) {
+                double acc = f[i];
+                for (int j = k; j < i; j++) acc -= Mm[i][j] * c[j];
+                c[i] = acc / Mm[i][i];
+            }
+            memcpy(v, r, sizeof(double) * n);
+            for (int i = k; i < s; i++) axpy(n, -c[i], G + (long)i * n, v);
+            double *uk = U + (long)k * n, *gk = G + (long)k * n;
+#pragma omp parallel for
+            for (long q = 0; q < n; q++) {
+                double acc = om * v[q];
+                for (int i = k; i < s; i++) acc += c[i] * U[(long)i * n + q];
+                uk[q] = acc;
+            }
+            pmatvec(op, uk, gk);
+            it++;
+            for (int i = 0; i < k; i++) {
+                const double al = dot(n, P + (long)i * n, gk) / Mm[i][i];
+                axpy(n, -al, G + (long)i * n, gk);
+                axpy(n, -al, U + (long)i * n, uk);
+            }
+            for (int i = k; i < s; i++) Mm[i][k] = dot(n, P + (long)i * n, gk);
+            if (!(fabs(Mm[k][k]) > 0.0)) goto restart;
+            const double beta = f[k] / Mm[k][k];
+            axpy(n, -beta, gk, r);
+            axpy(n, beta, uk, x);
+            resid = sqrt(dot(n, r, r)) / b_norm;
+            for (int i = k + 1; i < s; i++) f[i] -= beta * Mm[i][k];
+        }
+        if (resid < tol || it >= maxit || !isfinite(resid)) break;
+        memcpy(v, r, sizeof(double) * n);
+        pmatvec(op, v, t);
+        it++;
+        const double tt = dot(n, t, t), tr = dot(n, t, r);
+        if (!(tt > 0.0)) goto restart;
+        om = tr / tt;
+        const double rho = fabs(tr) / (sqrt(tt) * sqrt(dot(n, r, r)));
+        if (rho < 0.7) om *= 0.7 / rho;
+        axpy(n, -om, t, r);
+        axpy(n, om, v, x);
+        resid = sqrt(dot(n, r, r)) / b_norm;
+    }
+    *iters = it;
+    *res = resid;
+    free(P); free(G); free(U); free(r); free(v); free(t);
+    return resid < tol;
+}
+
 /* Stationary (Jacobi / Richardson) iteration x += omega M^{-1} (b - A x),
  * omega = 1; with the "diagonal" preconditioner this is the Jacobi method */
 static int richardson(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters,
@@ -1420,6 +1610,8 @@ int or_krylov(int nrows, const int *rp, const int *col, const double *val, const
         else if (solver == OR_CG) conv = cg(&op, b, x, tol, max_iter, iterations, residue);
         else if (solver == OR_TFQMR) conv = tfqmr(&op, b, x, tol, max_iter, iterations, residue);
         else if (solver == OR_JACOBI) conv = richardson(&op, b, x, tol, max_iter, iterations, residue);
+        else if (solver == OR_BICGSTABL) conv = bicgstabl(&op, b, x, tol, max_iter, gmres_restart, iterations, residue);
+        else if (solver == OR_IDRS) conv = idrs(&op, b, x, tol, max_iter, gmres_restart, iterations, residue);
         else conv = bicgstab(&op, b, x, tol, max_iter, iterations, residue);
     }
     free(op.tmp); free(op.dinv); free(op.lu);

@@ -223,7 +223,32 @@ def test_other_solvers_iteration_parity(solver, prec, case):
     s.close()
 
 
-@pytest.mark.parametrize("solver,prec", [("cg", "diagonal"), ("tfqmr", "ilu0"), ("tfqmr", "amg"),
+@pytest.mark.parametrize("solver,m,prec,case", [
+    ("bicgstabl", 1, "none", "darcy3d"), ("bicgstabl", 2, "diagonal", "darcy3d"), ("bicgstabl", 4, "ilu0", "darcy3d"),
+    ("bicgstabl", 2, "none", "heat2d"), ("idrs", 1, "ilu0", "darcy3d"), ("idrs", 4, "none", "darcy3d"),
+    ("idrs", 4, "ilu0", "darcy3d"), ("idrs", 8, "diagonal", "heat2d")])
+def test_bicgstabl_and_idrs_iteration_parity(solver, m, prec, case):
+    """BiCGStab(l) and IDR(s) (same shadow space, from the same integer hash)
+    against the oracle's from the same x0: iteration counts within 10 % and
+    the iterates within 1e-7."""
+    print(f"\n[krylov] {solver}({m}) {prec} {case}", flush=True)
+    pr = _SYSTEMS[case](prec)
+    kw = {"bicgstab_ell": m} if solver == "bicgstabl" else {"idr_s": m}
+    s = capi.Solver(pr, solver=solver, preconditioner=prec, tol=1e-10, **kw)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    x0 = s.get_array("Z_iF")
+    x, st = s.solve_rhs(rhs, x0)
+    xo, conv, it, res = pyoracle.krylov(rp, col, val, rhs, x0, solver, prec, 1e-10, max_iter=20000, restart=m)
+    assert conv == 1 and st.converged and st.residue < 1e-10
+    assert abs(st.iterations - it) <= max(2, 0.1 * it), (st.iterations, it)
+    assert rel(x, xo) < 1e-7
+    s.close()
+
+
+@pytest.mark.parametrize("solver,prec", [("bicgstabl", "ilu-isai"), ("idrs", "amg"),
+                                         ("cg", "diagonal"), ("tfqmr", "ilu0"), ("tfqmr", "amg"),
                                          ("cg", "amg"), ("jacobi", "ilu0")])
 def test_other_solvers_step_parity(solver, prec):
     """Whole time steps with each solver reach the scheme oracle's solution."""
@@ -243,7 +268,7 @@ def test_other_solvers_step_parity(solver, prec):
 
 def test_other_solvers_refuse_fp32():
     pr = problems.heat_2d(s=6, steps=1)
-    for solver in ("cg", "tfqmr", "jacobi"):
+    for solver in ("cg", "tfqmr", "jacobi", "bicgstabl", "idrs"):
         with pytest.raises(capi.MHFEMError, match="BiCGStab"):
             capi.Solver(pr, solver=solver, precision="fp32-refined")
 

@@ -604,3 +604,46 @@ def test_rtn0_projection_of_constant_field():
         e0 = list(cf[K0]).index(E)
         e1 = list(cf[K1]).index(E)
         assert abs(a[e0, K0] + a[e1, K1]) < 1e-14
+
+
+@pytest.mark.parametrize("prec", ["none", "ilu0"])
+def test_bicgstabl_with_l1_is_bicgstab_and_idr1_takes_twice_its_products(prec):
+    """BiCGStab(1) is BiCGStab (same iterations, same iterate).  IDR(1) with
+    the shadow vector r_0 and minimal-residual omega generates BiCGStab's
+    residuals every second operator application (Sonneveld & van Gijzen
+    2008); with a random shadow vector and the angle-corrected omega used here
+    its count of applications stays close to twice BiCGStab's iterations."""
+    rp, col, val, rhs, _ = _mhfem_system(prec)
+    x0 = np.zeros(rhs.size)
+    xb, _, itb, _ = pyoracle.krylov(rp, col, val, rhs, x0, "bicgstab", prec, 1e-10)
+    xl, cl, itl, _ = pyoracle.krylov(rp, col, val, rhs, x0, "bicgstabl", prec, 1e-10, restart=1)
+    assert cl == 1 and abs(itl - itb) <= 1
+    assert np.linalg.norm(xl - xb) / np.linalg.norm(xb) < 1e-8
+    xi, ci, iti, _ = pyoracle.krylov(rp, col, val, rhs, x0, "idrs", prec, 1e-10, restart=1)
+    assert ci == 1 and abs(iti - 2 * itb) <= 0.25 * 2 * itb, (iti, itb)
+
+
+@pytest.mark.parametrize("solver,m", [("bicgstabl", 2), ("bicgstabl", 4), ("idrs", 2), ("idrs", 4), ("idrs", 8)])
+@pytest.mark.parametrize("prec", ["none", "diagonal", "ilu0"])
+def test_bicgstabl_and_idrs_against_direct_solve(solver, m, prec):
+    rp, col, val, rhs, _ = _mhfem_system(prec)
+    n = rp.size - 1
+    xd = spla.spsolve(sp.csr_matrix((val, col, rp), shape=(n, n)).tocsc(), rhs)
+    x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), solver, prec, 1e-12, restart=m)
+    assert conv == 1 and res < 1e-12
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+
+
+def test_idrs_finite_termination():
+    """In exact arithmetic IDR(s) solves an N x N system in at most N + N/s
+    operator applications (Sonneveld & van Gijzen 2008); on a small
+    well-conditioned nonsymmetric system the count stays near that bound."""
+    rng = np.random.default_rng(8)
+    N, s = 40, 4
+    A = sp.csr_matrix(np.eye(N) * 4 + rng.uniform(-1, 1, (N, N)) * 0.3)
+    A.sort_indices()
+    b = rng.standard_normal(N)
+    x, conv, it, res = pyoracle.krylov(A.indptr, A.indices, A.data, b, np.zeros(N), "idrs", "none", 1e-12, restart=s)
+    assert conv == 1
+    assert it <= N + N // s + 2, it
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-10

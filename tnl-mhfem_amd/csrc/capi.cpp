@@ -718,7 +718,7 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
 {
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
-        if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_JACOBI) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
+        if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_IDRS) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
         if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_PARILU_ISAI)
             throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
         const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || is_isai(cfg->preconditioner);
@@ -734,6 +734,9 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         if (c.cfg.max_iterations <= 0) c.cfg.max_iterations = 100000;
         if (c.cfg.check_every <= 0) c.cfg.check_every = 1;
         if (c.cfg.gmres_restart <= 0) c.cfg.gmres_restart = 30;
+        if (c.cfg.bicgstab_ell <= 0) c.cfg.bicgstab_ell = 1;
+        if (c.cfg.idr_s <= 0) c.cfg.idr_s = 4;
+        if (c.cfg.bicgstab_ell > 8 || c.cfg.idr_s > 8) throw Error{MHFEM_E_ARGUMENT, "bicgstab_ell and idr_s are at most 8"};
         // row lengths (Solver.hpp:449-474)
         const int n = c.n, fpc = c.fpc;
         c.rows = n * c.F;

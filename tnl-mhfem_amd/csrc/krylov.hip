@@ -873,7 +873,8 @@ struct Krylov {
     // GMRES
     DBuf<double> V;
     int V_m = 0;
-    // TFQMR's two vectors beyond the BiCGStab set (allocated on first use)
+    // vectors of TFQMR, BiCGStab(l) and IDR(s) beyond the BiCGStab set
+    // (allocated on first use)
     DBuf<double> ext;
     // mixed precision: solves left that go straight to FP64 after a fallback
     int fp64_cooldown = 0;
@@ -1545,6 +1546,10 @@ struct Solve {
     int cg(const double *b, double *x, double tol, int maxit, double *resid_out);
     int tfqmr(const double *b, double *x, double tol, int maxit, double *resid_out);
     int richardson(const double *b, double *x, double tol, int maxit, double *resid_out);
+    int bicgstabl(const double *b, double *x, double tol, int maxit, int ell, double *resid_out);
+    int idrs(const double *b, double *x, double tol, int maxit, int s, double *resid_out);
+    void pmv(const double *in, double *out);  // out = M^{-1} A in (through k.tmp)
+    double *ext(size_t vectors);             // k.ext grown to hold `vectors` vectors
     // host-driven pieces of those three: y = a u + b v (y may be u or v),
     // dot products read back to the host, device copies
     void lin2(double a, const double *u, double b, const double *v, double *y);
@@ -1781,16 +1786,23 @@ int Solve::cg(const double *b, double *x, double tol, int maxit, double *resid_o
 // residual is computed and must be below tol too.  A breakdown, (r_0, v) = 0
 // or rho = 0, which the identity rows of Dirichlet faces do reach, restarts
 // the recurrence from the current x.
+double *Solve::ext(size_t vectors)
+{
+    if (k.ext.n < vectors * n) k.ext.alloc(vectors * n);
+    return k.ext.p;
+}
+void Solve::pmv(const double *in, double *out)
+{
+    spmv_plain<double>(in, k.tmp.p);
+    prec<double>(k.tmp.p, out, k.dinv.p, M_ALL);
+}
+
 int Solve::tfqmr(const double *b, double *x, double tol, int maxit, double *resid_out)
 {
-    if (k.ext.n < (size_t)2 * n) k.ext.alloc((size_t)2 * n);
+    double *e = ext(2);
     double *rt = k.r_ast.p, *w = k.r.p, *y1 = k.p.p, *y2 = k.s.p, *u1 = k.Ap.p, *u2 = k.As.p, *v = k.tmp2.p,
-           *d = k.ext.p, *r = k.ext.p + n, *tmp = k.tmp.p;
+           *d = e, *r = e + n, *tmp = k.tmp.p;
     const double *dinv = k.dinv.p;
-    auto pmv = [&](const double *in, double *out) {  // out = M^{-1} A in
-        spmv_plain<double>(in, tmp);
-        prec<double>(tmp, out, dinv, M_ALL);
-    };
     set_scalars(tol, maxit);
     prec<double>(b, r, dinv, M_ALL);
     double b_norm = std::sqrt(dot1(r, r));
@@ -1852,6 +1864,198 @@ int Solve::tfqmr(const double *b, double *x, double tol, int maxit, double *resi
     }
     presidue<double>(x, b, r, tmp, dinv, M_ALL);
     resid = std::sqrt(dot1(r, r)) / b_norm;
+    *resid_out = resid;
+    return it;
+}
+
+// BiCGStab(l) (Sleijpen & Fokkema 1993) on M^{-1} A x = M^{-1} b: l BiCG steps,
+// then a minimal-residual polynomial of degree l over the l + 1 residuals
+// (modified Gram-Schmidt) (TNL::Solvers::Linear::BICGStabL).  One iteration is
+// one BiCG step (two operator applications); convergence is checked after each
+// cycle.  A breakdown (rho0 = 0, (u_{j+1}, r~) = 0, sigma_j = 0) restarts from
+// the current x.  l = 1 is BiCGStab.
+int Solve::bicgstabl(const double *b, double *x, double tol, int maxit, int ell, double *resid_out)
+{
+    ell = std::max(1, std::min(ell, 8));
+    double *R = ext((size_t)2 * (ell + 1));
+    double *U = R + (size_t)(ell + 1) * n, *rt = k.r_ast.p;
+    auto Rv = [&](int j) { return R + (size_t)j * n; };
+    auto Uv = [&](int j) { return U + (size_t)j * n; };
+    double tau[9][9] = {}, sigma[9] = {}, gp[9] = {}, gg[9] = {}, gpp[9] = {};
+    set_scalars(tol, maxit);
+    prec<double>(b, rt, k.dinv.p, M_ALL);
+    double b_norm = std::sqrt(dot1(rt, rt));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0;
+    double resid = 0, rho0 = 1, alpha = 0, omega = 1;
+    bool restart = true;
+    while (restart) {
+        restart = false;
+        presidue<double>(x, b, R, k.tmp.p, k.dinv.p, M_ALL);
+        copy(R, rt);
+        MHFEM_HIP(hipMemsetAsync(U, 0, (size_t)n * sizeof(double), c.stream));
+        rho0 = 1, alpha = 0, omega = 1;
+        resid = std::sqrt(dot1(R, R)) / b_norm;
+        while (!restart && resid >= tol && it < maxit && std::isfinite(resid)) {
+            rho0 = -omega * rho0;
+            for (int j = 0; j < ell && !restart; j++) {
+                it++;
+                const double rho1 = dot1(Rv(j), rt), beta = alpha * rho1 / rho0;
+                if (!(std::fabs(rho0) > 0.0) || !std::isfinite(beta)) {
+                    restart = true;
+                    break;
+                }
+                rho0 = rho1;
+                for (int i = 0; i <= j; i++) lin2(1.0, Rv(i), -beta, Uv(i), Uv(i));
+                pmv(Uv(j), Uv(j + 1));
+                const double gam = dot1(Uv(j + 1), rt);
+                alpha = rho0 / gam;
+                if (!(std::fabs(gam) > 0.0) || !std::isfinite(alpha)) {
+                    restart = true;
+                    break;
+                }
+                for (int i = 0; i <= j; i++) lin2(1.0, Rv(i), -alpha, Uv(i + 1), Rv(i));
+                pmv(Rv(j), Rv(j + 1));
+                lin2(1.0, x, alpha, Uv(0), x);
+            }
+            if (restart) break;
+            for (int j = 1; j <= ell && !restart; j++) {
+                for (int i = 1; i < j; i++) {
+                    tau[i][j] = dot1(Rv(j), Rv(i)) / sigma[i];
+                    lin2(1.0, Rv(j), -tau[i][j], Rv(i), Rv(j));
+                }
+                sigma[j] = dot1(Rv(j), Rv(j));
+                if (!(sigma[j] > 0.0)) restart = true;
+                else gp[j] = dot1(Rv(0), Rv(j)) / sigma[j];
+            }
+            if (restart) break;
+            gg[ell] = gp[ell];
+            omega = gg[ell];
+            for (int j = ell - 1; j >= 1; j--) {
+                double t = gp[j];
+                for (int i = j + 1; i <= ell; i++) t -= tau[j][i] * gg[i];
+                gg[j] = t;
+            }
+            for (int j = 1; j < ell; j++) {
+                double t = gg[j + 1];
+                for (int i = j + 1; i < ell; i++) t += tau[j][i] * gg[i + 1];
+                gpp[j] = t;
+            }
+            lin2(1.0, x, gg[1], Rv(0), x);
+            lin2(1.0, Rv(0), -gp[ell], Rv(ell), Rv(0));
+            lin2(1.0, Uv(0), -gg[ell], Uv(ell), Uv(0));
+            for (int j = 1; j < ell; j++) {
+                lin2(1.0, Uv(0), -gg[j], Uv(j), Uv(0));
+                lin2(1.0, x, gpp[j], Rv(j), x);
+                lin2(1.0, Rv(0), -gp[j], Rv(j), Rv(0));
+            }
+            resid = std::sqrt(dot1(Rv(0), Rv(0))) / b_norm;
+        }
+    }
+    *resid_out = resid;
+    return it;
+}
+
+// entry (i, k) of IDR(s)'s shadow space before orthonormalisation: an integer
+// hash mapped to [-1, 1) with 24 bits, exact in any precision
+__global__ void __launch_bounds__(KB) k_idr_shadow(int n, int k, double *__restrict__ p)
+{
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        uint32_t h = (uint32_t)i * 0x9E3779B1u + (uint32_t)(k + 1) * 0x85EBCA77u;
+        h ^= h >> 15;
+        h *= 0x2C1B3C6Du;
+        h ^= h >> 12;
+        h *= 0x297A2D39u;
+        h ^= h >> 15;
+        p[i] = (double)(h >> 8) * (2.0 / 16777216.0) - 1.0;
+    }
+}
+
+// IDR(s) with biorthogonalisation (van Gijzen & Sonneveld 2011, Alg. 913) on
+// M^{-1} A x = M^{-1} b, omega by minimal residual with the angle correction
+// 0.7 (TNL::Solvers::Linear::IDRs).  One iteration is one operator
+// application (s + 1 per cycle), convergence checked after each.  The shadow
+// space is the orthonormalised hash matrix above.  A breakdown (M_kk = 0,
+// t = 0) restarts from the current x.
+int Solve::idrs(const double *b, double *x, double tol, int maxit, int s, double *resid_out)
+{
+    s = std::max(1, std::min(s, 8));
+    double *P = ext((size_t)3 * s + 2);
+    double *G = P + (size_t)s * n, *U = G + (size_t)s * n, *v = U + (size_t)s * n, *t = v + n, *r = k.r.p;
+    auto Pv = [&](int i) { return P + (size_t)i * n; };
+    auto Gv = [&](int i) { return G + (size_t)i * n; };
+    auto Uv = [&](int i) { return U + (size_t)i * n; };
+    double Mm[8][8], f[8], cc[8];
+    for (int q = 0; q < s; q++) {
+        hipLaunchKernelGGL(k_idr_shadow, g, KB, 0, c.stream, n, q, Pv(q));
+        MHFEM_HIP(hipGetLastError());
+        for (int i = 0; i < q; i++) lin2(1.0, Pv(q), -dot1(Pv(q), Pv(i)), Pv(i), Pv(q));
+        lin2(1.0 / std::sqrt(dot1(Pv(q), Pv(q))), Pv(q), 0.0, Pv(q), Pv(q));
+    }
+    set_scalars(tol, maxit);
+    prec<double>(b, r, k.dinv.p, M_ALL);
+    double b_norm = std::sqrt(dot1(r, r));
+    if (b_norm == 0.0) b_norm = 1.0;
+    int it = 0;
+    double resid = 0, om = 1;
+    bool restart = true;
+    while (restart) {
+        restart = false;
+        presidue<double>(x, b, r, k.tmp.p, k.dinv.p, M_ALL);
+        MHFEM_HIP(hipMemsetAsync(G, 0, (size_t)2 * s * n * sizeof(double), c.stream));
+        for (int i = 0; i < s; i++)
+            for (int j = 0; j < s; j++) Mm[i][j] = i == j;
+        om = 1.0;
+        resid = std::sqrt(dot1(r, r)) / b_norm;
+        while (!restart && resid >= tol && it < maxit && std::isfinite(resid)) {
+            for (int i = 0; i < s; i++) f[i] = dot1(Pv(i), r);
+            for (int kk = 0; kk < s && resid >= tol && it < maxit; kk++) {
+                for (int i = kk; i < s; i++) {  // c = M(k:s, k:s) \ f(k:s), M lower triangular
+                    double acc = f[i];
+                    for (int j = kk; j < i; j++) acc -= Mm[i][j] * cc[j];
+                    cc[i] = acc / Mm[i][i];
+                }
+                copy(r, v);
+                for (int i = kk; i < s; i++) lin2(1.0, v, -cc[i], Gv(i), v);
+                lin2(om, v, 0.0, v, v);  // u_k = om v + sum_i c_i u_i (old u_k included)
+                for (int i = kk; i < s; i++) lin2(1.0, v, cc[i], Uv(i), v);
+                copy(v, Uv(kk));
+                pmv(Uv(kk), Gv(kk));
+                it++;
+                for (int i = 0; i < kk; i++) {
+                    const double al = dot1(Pv(i), Gv(kk)) / Mm[i][i];
+                    lin2(1.0, Gv(kk), -al, Gv(i), Gv(kk));
+                    lin2(1.0, Uv(kk), -al, Uv(i), Uv(kk));
+                }
+                for (int i = kk; i < s; i++) Mm[i][kk] = dot1(Pv(i), Gv(kk));
+                if (!(std::fabs(Mm[kk][kk]) > 0.0)) {
+                    restart = true;
+                    break;
+                }
+                const double beta = f[kk] / Mm[kk][kk];
+                lin2(1.0, r, -beta, Gv(kk), r);
+                lin2(1.0, x, beta, Uv(kk), x);
+                resid = std::sqrt(dot1(r, r)) / b_norm;
+                for (int i = kk + 1; i < s; i++) f[i] -= beta * Mm[i][kk];
+            }
+            if (restart || resid < tol || it >= maxit || !std::isfinite(resid)) break;
+            copy(r, v);
+            pmv(v, t);
+            it++;
+            double tt, tr;
+            dot2(t, t, t, r, &tt, &tr);
+            if (!(tt > 0.0)) {
+                restart = true;
+                break;
+            }
+            om = tr / tt;
+            const double rho = std::fabs(tr) / (std::sqrt(tt) * std::sqrt(dot1(r, r)));
+            if (rho < 0.7) om *= 0.7 / rho;
+            lin2(1.0, r, -om, t, r);
+            lin2(1.0, x, om, v, x);
+            resid = std::sqrt(dot1(r, r)) / b_norm;
+        }
+    }
     *resid_out = resid;
     return it;
 }
@@ -2193,6 +2397,8 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
             case MHFEM_SOLVER_GMRES: out.iterations = S.gmres(b, x, tol, maxit, c.cfg.gmres_restart, &res); break;
             case MHFEM_SOLVER_CG: out.iterations = S.cg(b, x, tol, maxit, &res); break;
             case MHFEM_SOLVER_TFQMR: out.iterations = S.tfqmr(b, x, tol, maxit, &res); break;
+            case MHFEM_SOLVER_BICGSTABL: out.iterations = S.bicgstabl(b, x, tol, maxit, c.cfg.bicgstab_ell, &res); break;
+            case MHFEM_SOLVER_IDRS: out.iterations = S.idrs(b, x, tol, maxit, c.cfg.idr_s, &res); break;
             default: out.iterations = S.richardson(b, x, tol, maxit, &res); break;
             }
             out.residue = res;
