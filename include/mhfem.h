@@ -96,7 +96,9 @@ typedef struct {
  * biorthogonalisation (van Gijzen & Sonneveld, Alg. 913), one iteration per
  * operator application, on a fixed pseudo-random orthonormal shadow space.
  * All are left preconditioned and stop on
- * ||M^{-1}(b - A x)|| / ||M^{-1} b|| < convergence_residue.
+ * ||M^{-1}(b - A x)|| / ||M^{-1} b|| < convergence_residue.  SOR: forward
+ * sweeps with relaxation sor_omega (one iteration per sweep), no
+ * preconditioner, stopping on ||b - A x|| / ||b||.
  * The FP32 precisions, batches and graph steps take BiCGStab only. */
 enum {
     MHFEM_SOLVER_BICGSTAB = 0,
@@ -105,7 +107,8 @@ enum {
     MHFEM_SOLVER_TFQMR = 3,
     MHFEM_SOLVER_JACOBI = 4,
     MHFEM_SOLVER_BICGSTABL = 5,
-    MHFEM_SOLVER_IDRS = 6
+    MHFEM_SOLVER_IDRS = 6,
+    MHFEM_SOLVER_SOR = 7
 };
 /* MHFEM_PREC_AMG: algebraic multigrid in the configuration of the
  * reference's Ginkgo branch (Solver.hpp:1025-1075): parallel graph matching
@@ -157,6 +160,7 @@ typedef struct {
                                    (results unchanged up to rounding); 0 (default): natural order */
     int bicgstab_ell;           /* l of BiCGStab(l) ("bicgstab-ell"), 1..8; 0: 1 */
     int idr_s;                  /* s of IDR(s) ("idr-s"), 1..8; 0: 4 */
+    double sor_omega;           /* relaxation of SOR ("sor-omega"), in (0, 2); 0: 1 */
 } mhfem_solver_config;
 
 typedef struct {

@@ -58,9 +58,11 @@ public:
         else if (solver == "tfqmr") cfg_.solver = MHFEM_SOLVER_TFQMR;
         else if (solver == "jacobi") cfg_.solver = MHFEM_SOLVER_JACOBI;
         else if (solver == "idrs") cfg_.solver = MHFEM_SOLVER_IDRS;
-        else return false;  // "sor" is not built
+        else if (solver == "sor") cfg_.solver = MHFEM_SOLVER_SOR;
+        else return false;
         cfg_.bicgstab_ell = p.getParameter<int>("bicgstab-ell", 1);
         cfg_.idr_s = p.getParameter<int>("idr-s", 4);
+        cfg_.sor_omega = p.getParameter<double>("sor-omega", 1.0);
         if (prec == "none") cfg_.preconditioner = MHFEM_PREC_NONE;
         else if (prec == "diagonal") cfg_.preconditioner = MHFEM_PREC_DIAGONAL;
         else if (prec == "ilu0") cfg_.preconditioner = MHFEM_PREC_ILU0;

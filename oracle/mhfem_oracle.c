@@ -1556,6 +1556,45 @@ restart:
     return resid < tol;
 }
 
+/* SOR (TNL::Solvers::Linear::SOR): forward sweeps
+ *   x_i <- (1 - w) x_i + w (b_i - sum_{j != i} a_ij x_j) / a_ii
+ * in row order, each row using the values already updated in this sweep for
+ * j < i; w = `omega`.  The preconditioner is not used; the stopping test is
+ * the unpreconditioned ||b - A x|| / ||b|| < tol, checked after each sweep
+ * (one iteration). */
+static int sor(krylov_op *op, const double *b, double *x, double tol, int maxit, double omega, int *iters,
+               double *res)
+{
+    const int n = op->n;
+    const int *rp = op->rp, *col = op->col;
+    const double *val = op->val;
+    double *r = malloc(sizeof(double) * n);
+    double b_norm = sqrt(dot(n, b, b));
+    if (b_norm == 0.0) b_norm = 1.0;
+    or_spmv(n, rp, col, val, x, r);
+    for (int k = 0; k < n; k++) r[k] = b[k] - r[k];
+    double resid = sqrt(dot(n, r, r)) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && isfinite(resid)) {
+        for (int i = 0; i < n; i++) {
+            double acc = b[i], d = 0.0;
+            for (int q = rp[i]; q < rp[i + 1]; q++) {
+                if (col[q] == i) d = val[q];
+                else acc -= val[q] * x[col[q]];
+            }
+            x[i] = (1.0 - omega) * x[i] + omega * acc / d;
+        }
+        or_spmv(n, rp, col, val, x, r);
+        for (int k = 0; k < n; k++) r[k] = b[k] - r[k];
+        resid = sqrt(dot(n, r, r)) / b_norm;
+        it++;
+    }
+    *iters = it;
+    *res = resid;
+    free(r);
+    return resid < tol;
+}
+
 /* Stationary (Jacobi / Richardson) iteration x += omega M^{-1} (b - A x),
  * omega = 1; with the "diagonal" preconditioner this is the Jacobi method */
 static int richardson(krylov_op *op, const double *b, double *x, double tol, int maxit, int *iters,
@@ -1612,6 +1651,7 @@ int or_krylov(int nrows, const int *rp, const int *col, const double *val, const
         else if (solver == OR_JACOBI) conv = richardson(&op, b, x, tol, max_iter, iterations, residue);
         else if (solver == OR_BICGSTABL) conv = bicgstabl(&op, b, x, tol, max_iter, gmres_restart, iterations, residue);
         else if (solver == OR_IDRS) conv = idrs(&op, b, x, tol, max_iter, gmres_restart, iterations, residue);
+        else if (solver == OR_SOR) conv = sor(&op, b, x, tol, max_iter, gmres_restart / 100.0, iterations, residue);
         else conv = bicgstab(&op, b, x, tol, max_iter, iterations, residue);
     }
     free(op.tmp); free(op.dinv); free(op.lu);

@@ -37,10 +37,11 @@ enum { OR_EXPLICIT_UPWIND = 0, OR_IMPLICIT_UPWIND = 1, OR_IMPLICIT_TRACE = 2 };
 /* boundary condition tags (BoundaryConditionsType.h:7-31) */
 enum { OR_FIXED_VALUE = 0, OR_FIXED_FLUX = 1, OR_ADVECTIVE_OUTFLOW = 2, OR_FIXED_FLUX_NO_ADVECTION = 3 };
 /* linear solvers / preconditioners */
-enum { OR_BICGSTAB = 0, OR_GMRES = 1, OR_CG = 2, OR_TFQMR = 3, OR_JACOBI = 4, OR_BICGSTABL = 5, OR_IDRS = 6 };
-/* or_krylov's gmres_restart is the restart length of GMRES, l of BiCGStab(l)
- * and s of IDR(s); or_idr_shadow_value: entry (i, k) of IDR(s)'s shadow space
- * before orthonormalisation */
+enum { OR_BICGSTAB = 0, OR_GMRES = 1, OR_CG = 2, OR_TFQMR = 3, OR_JACOBI = 4, OR_BICGSTABL = 5, OR_IDRS = 6,
+       OR_SOR = 7 };
+/* or_krylov's gmres_restart is the restart length of GMRES, l of BiCGStab(l),
+ * s of IDR(s) and 100 w for SOR's relaxation w; or_idr_shadow_value: entry
+ * (i, k) of IDR(s)'s shadow space before orthonormalisation */
 double or_idr_shadow_value(long i, int k);
 enum { OR_PREC_NONE = 0, OR_PREC_DIAGONAL = 1, OR_PREC_ILU0 = 2 };
 

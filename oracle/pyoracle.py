@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # broken copy of the oracle
 LIB_PATH = os.environ.get("MHFEM_ORACLE_LIB", os.path.join(_HERE, "_build", "libmhfem_oracle.so"))
 
-SOLVERS = {"bicgstab": 0, "gmres": 1, "cg": 2, "tfqmr": 3, "jacobi": 4, "bicgstabl": 5, "idrs": 6}
+SOLVERS = {"bicgstab": 0, "gmres": 1, "cg": 2, "tfqmr": 3, "jacobi": 4, "bicgstabl": 5, "idrs": 6, "sor": 7}
 PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2}
 
 _lib = None

@@ -247,7 +247,27 @@ def test_bicgstabl_and_idrs_iteration_parity(solver, m, prec, case):
     s.close()
 
 
-@pytest.mark.parametrize("solver,prec", [("bicgstabl", "ilu-isai"), ("idrs", "amg"),
+@pytest.mark.parametrize("omega,case", [(1.0, "heat2d"), (1.4, "heat2d"), (1.0, "darcy3d")])
+def test_sor_iteration_parity(omega, case):
+    """SOR sweeps computed level by level equal the oracle's sequential
+    sweeps: the same number of sweeps and the same iterate."""
+    print(f"\n[krylov] sor w={omega} {case}", flush=True)
+    pr = _SYSTEMS[case]("none")
+    s = capi.Solver(pr, solver="sor", preconditioner="none", tol=1e-10, sor_omega=omega)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    x0 = s.get_array("Z_iF")
+    x, st = s.solve_rhs(rhs, x0)
+    xo, conv, it, res = pyoracle.krylov(rp, col, val, rhs, x0, "sor", "none", 1e-10, max_iter=100000,
+                                        restart=int(round(100 * omega)))
+    assert conv == 1 and st.converged
+    assert abs(st.iterations - it) <= 1, (st.iterations, it)
+    assert rel(x, xo) < 1e-9
+    s.close()
+
+
+@pytest.mark.parametrize("solver,prec", [("sor", "none"), ("bicgstabl", "ilu-isai"), ("idrs", "amg"),
                                          ("cg", "diagonal"), ("tfqmr", "ilu0"), ("tfqmr", "amg"),
                                          ("cg", "amg"), ("jacobi", "ilu0")])
 def test_other_solvers_step_parity(solver, prec):
@@ -268,7 +288,7 @@ def test_other_solvers_step_parity(solver, prec):
 
 def test_other_solvers_refuse_fp32():
     pr = problems.heat_2d(s=6, steps=1)
-    for solver in ("cg", "tfqmr", "jacobi", "bicgstabl", "idrs"):
+    for solver in ("cg", "tfqmr", "jacobi", "bicgstabl", "idrs", "sor"):
         with pytest.raises(capi.MHFEMError, match="BiCGStab"):
             capi.Solver(pr, solver=solver, precision="fp32-refined")
 

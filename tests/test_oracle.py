@@ -647,3 +647,26 @@ def test_idrs_finite_termination():
     assert conv == 1
     assert it <= N + N // s + 2, it
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_sor_by_hand_and_against_direct_solve():
+    """One SOR sweep on a 3x3 system worked by hand (w = 1.5, Gauss-Seidel
+    order), and convergence to the direct solution on the MHFEM system for
+    w = 1 (Gauss-Seidel) and w = 1.3."""
+    A = sp.csr_matrix(np.array([[4.0, -1.0, 0.0], [-1.0, 4.0, -1.0], [0.0, -1.0, 4.0]]))
+    b = np.array([3.0, 2.0, 3.0])
+    x, conv, it, res = pyoracle.krylov(A.indptr, A.indices, A.data, b, np.zeros(3), "sor", "none", 1e-30,
+                                       max_iter=1, restart=150)
+    w = 1.5
+    x0 = w * 3.0 / 4
+    x1 = w * (2.0 + x0) / 4
+    x2 = w * (3.0 + x1) / 4
+    assert it == 1 and np.allclose(x, [x0, x1, x2], rtol=1e-15)
+    rp, col, val, rhs, _ = _mhfem_system("none")
+    n = rp.size - 1
+    xd = spla.spsolve(sp.csr_matrix((val, col, rp), shape=(n, n)).tocsc(), rhs)
+    for m in (100, 130):
+        x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), "sor", "none", 1e-11, max_iter=100000,
+                                           restart=m)
+        assert conv == 1
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8

@@ -718,7 +718,7 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
 {
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
-        if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_IDRS) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
+        if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_SOR) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
         if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_PARILU_ISAI)
             throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
         const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || is_isai(cfg->preconditioner);
@@ -737,6 +737,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         if (c.cfg.bicgstab_ell <= 0) c.cfg.bicgstab_ell = 1;
         if (c.cfg.idr_s <= 0) c.cfg.idr_s = 4;
         if (c.cfg.bicgstab_ell > 8 || c.cfg.idr_s > 8) throw Error{MHFEM_E_ARGUMENT, "bicgstab_ell and idr_s are at most 8"};
+        if (c.cfg.sor_omega == 0.0) c.cfg.sor_omega = 1.0;
+        if (!(c.cfg.sor_omega > 0.0 && c.cfg.sor_omega < 2.0)) throw Error{MHFEM_E_ARGUMENT, "sor_omega must lie in (0, 2)"};
         // row lengths (Solver.hpp:449-474)
         const int n = c.n, fpc = c.fpc;
         c.rows = n * c.F;

@@ -1108,7 +1108,8 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipEventCreateWithFlags(&k->ev_ahead[q], hipEventDisableTiming));
     }
     build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
-    if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || is_isai(c.cfg.preconditioner)) {
+    // level tables: ILU(0), the ISAI factors and SOR's sweeps
+    if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || is_isai(c.cfg.preconditioner) || c.cfg.solver == MHFEM_SOLVER_SOR) {
         Ilu0 &I = k->ilu;
         std::vector<int> dg(n), lev(n), ulev(n);
         for (int r = 0; r < n; r++)
@@ -1147,7 +1148,7 @@ Krylov *krylov_create(Ctx &c)
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
         if (is_isai(c.cfg.preconditioner)) isai_patterns(c, *k, dg);
-        if (c.cfg.preconditioner == MHFEM_PREC_PARILU_ISAI) {
+        if (c.cfg.preconditioner == MHFEM_PREC_PARILU_ISAI || c.cfg.solver == MHFEM_SOLVER_SOR) {
             I.a0.alloc(c.nnz);
             I.swp.alloc(c.nnz);
         }
@@ -1548,6 +1549,7 @@ struct Solve {
     int richardson(const double *b, double *x, double tol, int maxit, double *resid_out);
     int bicgstabl(const double *b, double *x, double tol, int maxit, int ell, double *resid_out);
     int idrs(const double *b, double *x, double tol, int maxit, int s, double *resid_out);
+    int sor(const double *b, double *x, double tol, int maxit, double omega, double *resid_out);
     void pmv(const double *in, double *out);  // out = M^{-1} A in (through k.tmp)
     double *ext(size_t vectors);             // k.ext grown to hold `vectors` vectors
     // host-driven pieces of those three: y = a u + b v (y may be u or v),
@@ -2060,6 +2062,57 @@ int Solve::idrs(const double *b, double *x, double tol, int maxit, int s, double
     return it;
 }
 
+// one level of an SOR sweep: the rows of a level depend below the diagonal
+// only on rows of earlier levels, which already hold this sweep's values in
+// xn; right of the diagonal the previous sweep's values (xo) are used, as in
+// the sequential sweep
+__global__ void __launch_bounds__(KB) k_sor_level(const int *__restrict__ rows, int count, const int *__restrict__ rp,
+                                                  const int *__restrict__ col, const int *__restrict__ dg,
+                                                  const double *__restrict__ a, const double *__restrict__ b,
+                                                  const double *__restrict__ xo, double *__restrict__ xn, double omega)
+{
+    const int t = blockIdx.x * KB + threadIdx.x;
+    if (t >= count) return;
+    const int r = rows[t];
+    double acc = b[r];
+    for (int q = rp[r]; q < dg[r]; q++) acc -= a[q] * xn[col[q]];
+    for (int q = dg[r] + 1; q < rp[r + 1]; q++) acc -= a[q] * xo[col[q]];
+    xn[r] = (1.0 - omega) * xo[r] + omega * acc / a[dg[r]];
+}
+
+// SOR (TNL::Solvers::Linear::SOR): forward sweeps in row order, relaxation
+// omega, the sequential sweep's result computed level by level (the ILU(0)
+// level tables of the lower triangle).  The preconditioner is not used; the
+// stopping test is ||b - A x|| / ||b|| < tol after each sweep (one iteration).
+int Solve::sor(const double *b, double *x, double tol, int maxit, double omega, double *resid_out)
+{
+    Ilu0 &I = k.ilu;
+    double *xo = k.tmp2.p, *r = k.r.p;
+    hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p, c.csr_to_sell.p,
+                       I.a0.p);
+    MHFEM_HIP(hipGetLastError());
+    set_scalars(tol, maxit);
+    double b_norm = std::sqrt(dot1(b, b));
+    if (b_norm == 0.0) b_norm = 1.0;
+    sresidual<double>(x, b, r, M_ALL);
+    double resid = std::sqrt(dot1(r, r)) / b_norm;
+    int it = 0;
+    while (resid >= tol && it < maxit && std::isfinite(resid)) {
+        copy(x, xo);
+        for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
+            const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
+            hipLaunchKernelGGL(k_sor_level, (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
+                               c.rowptr.p, c.csr_col.p, I.diag.p, I.a0.p, b, xo, x, omega);
+        }
+        MHFEM_HIP(hipGetLastError());
+        sresidual<double>(x, b, r, M_ALL);
+        resid = std::sqrt(dot1(r, r)) / b_norm;
+        it++;
+    }
+    *resid_out = resid;
+    return it;
+}
+
 // Stationary iteration x += M^{-1}(b - A x) (relaxation 1): with the diagonal
 // preconditioner, or the unit diagonal the MHFEM rows are scaled to, it is the
 // Jacobi method (TNL::Solvers::Linear::Jacobi, omega = 1)
@@ -2399,6 +2452,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
             case MHFEM_SOLVER_TFQMR: out.iterations = S.tfqmr(b, x, tol, maxit, &res); break;
             case MHFEM_SOLVER_BICGSTABL: out.iterations = S.bicgstabl(b, x, tol, maxit, c.cfg.bicgstab_ell, &res); break;
             case MHFEM_SOLVER_IDRS: out.iterations = S.idrs(b, x, tol, maxit, c.cfg.idr_s, &res); break;
+            case MHFEM_SOLVER_SOR: out.iterations = S.sor(b, x, tol, maxit, c.cfg.sor_omega, &res); break;
             default: out.iterations = S.richardson(b, x, tol, maxit, &res); break;
             }
             out.residue = res;
