@@ -125,13 +125,18 @@ enum {
  * equations instead of the level-scheduled elimination).  Same reuse rule,
  * configurations and limits as MHFEM_PREC_AMG.  (PARILUT_ISAI is not
  * provided.) */
+/* MHFEM_PREC_ILUT: TNL's "ilut" (configSetup.h:61), ILUT(p, tau) after Saad
+ * (Alg. 10.6), factorised on the host every step and applied by level-
+ * scheduled triangular solves on its own pattern; FP64 solvers, single
+ * systems, the four per-step calls, not with SOR. */
 enum {
     MHFEM_PREC_NONE = 0,
     MHFEM_PREC_DIAGONAL = 1,
     MHFEM_PREC_ILU0 = 2,
     MHFEM_PREC_AMG = 3,
     MHFEM_PREC_ILU_ISAI = 4,
-    MHFEM_PREC_PARILU_ISAI = 5
+    MHFEM_PREC_PARILU_ISAI = 5,
+    MHFEM_PREC_ILUT = 6
 };
 /* precision of the Krylov iteration.  FP32_REFINED: FP64 iterative
    refinement whose every correction is an FP32 BiCGStab solve started from
@@ -161,6 +166,9 @@ typedef struct {
     int bicgstab_ell;           /* l of BiCGStab(l) ("bicgstab-ell"), 1..8; 0: 1 */
     int idr_s;                  /* s of IDR(s) ("idr-s"), 1..8; 0: 4 */
     double sor_omega;           /* relaxation of SOR ("sor-omega"), in (0, 2); 0: 1 */
+    int ilut_p;                 /* ILUT: largest entries kept per row in L and in U ("ilut-p"); 0: 10 */
+    double ilut_threshold;      /* ILUT: drop tolerance relative to the row norm ("ilut-threshold");
+                                   0: 1e-4 */
 } mhfem_solver_config;
 
 typedef struct {

@@ -63,6 +63,8 @@ public:
         cfg_.bicgstab_ell = p.getParameter<int>("bicgstab-ell", 1);
         cfg_.idr_s = p.getParameter<int>("idr-s", 4);
         cfg_.sor_omega = p.getParameter<double>("sor-omega", 1.0);
+        cfg_.ilut_p = p.getParameter<int>("ilut-p", 10);
+        cfg_.ilut_threshold = p.getParameter<double>("ilut-threshold", 1e-4);
         if (prec == "none") cfg_.preconditioner = MHFEM_PREC_NONE;
         else if (prec == "diagonal") cfg_.preconditioner = MHFEM_PREC_DIAGONAL;
         else if (prec == "ilu0") cfg_.preconditioner = MHFEM_PREC_ILU0;
@@ -71,6 +73,7 @@ public:
         // its "ILU_ISAI" and "PARILU_ISAI" (Solver.hpp:1089-1130); PARILUT_ISAI is not built
         else if (prec == "ilu-isai" || prec == "ILU_ISAI") cfg_.preconditioner = MHFEM_PREC_ILU_ISAI;
         else if (prec == "parilu-isai" || prec == "PARILU_ISAI") cfg_.preconditioner = MHFEM_PREC_PARILU_ISAI;
+        else if (prec == "ilut") cfg_.preconditioner = MHFEM_PREC_ILUT;
         else return false;
         cfg_.convergence_residue = p.getParameter<double>("convergence-residue", 1e-6);
         cfg_.max_iterations = p.getParameter<int>("max-iterations", 100000);

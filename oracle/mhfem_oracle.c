@@ -1059,12 +1059,118 @@ void or_ilu0_apply(int nrows, const int *rp, const int *col, const double *lu, c
     }
 }
 
+/* ILUT(p, tau) (Saad, Iterative Methods, Alg. 10.6; TNL's "ilut"
+ * preconditioner): row by row, the row of A in a dense work vector w is
+ * eliminated against the finished rows of U in increasing column order,
+ * fill-in included; a multiplier w_k / u_kk below tau_i = tau ||a_i||_2 is
+ * dropped; then entries below tau_i are dropped and only the p largest in
+ * magnitude of the L part and of the U part are kept (ties: smaller column),
+ * the diagonal always (a zero pivot becomes tau_i, or 1).  Output: CSR with
+ * sorted columns, L (unit diagonal not stored) left of the diagonal entry of
+ * U.  Returns nnz, or -1 on allocation failure; *rp_out etc. malloc'ed. */
+typedef struct { int j; double v; } ilut_entry;
+static int ilut_by_mag(const void *a, const void *b)
+{
+    const ilut_entry *x = a, *y = b;
+    const double ax = fabs(x->v), ay = fabs(y->v);
+    if (ax != ay) return ax > ay ? -1 : 1;
+    return x->j - y->j;
+}
+static int ilut_by_col(const void *a, const void *b)
+{
+    return ((const ilut_entry *)a)->j - ((const ilut_entry *)b)->j;
+}
+long or_ilut_factorize(int n, const int *rp, const int *col, const double *val, int p, double tau, int **rp_out,
+                       int **col_out, double **val_out)
+{
+    long cap = (long)rp[n] * 2 + n, nnz = 0;
+    int *frp = malloc(sizeof(int) * (n + 1)), *fcol = malloc(sizeof(int) * cap), *fdg = malloc(sizeof(int) * n);
+    double *fval = malloc(sizeof(double) * cap), *w = calloc(n, sizeof(double));
+    int *mark = malloc(sizeof(int) * n), *idx = malloc(sizeof(int) * n), *done = calloc(n, sizeof(int));
+    ilut_entry *ent = malloc(sizeof(ilut_entry) * n);
+    for (int i = 0; i < n; i++) mark[i] = -1;
+    frp[0] = 0;
+    for (int i = 0; i < n; i++) {
+        int cnt = 0;
+        double nrm = 0;
+        for (int q = rp[i]; q < rp[i + 1]; q++) {
+            w[col[q]] = val[q];
+            mark[col[q]] = i;
+            idx[cnt++] = col[q];
+            nrm += val[q] * val[q];
+        }
+        if (mark[i] != i) { w[i] = 0; mark[i] = i; idx[cnt++] = i; }
+        const double ti = tau * sqrt(nrm);
+        for (;;) {
+            /* the smallest unprocessed column left of the diagonal */
+            int k = -1;
+            for (int t = 0; t < cnt; t++)
+                if (idx[t] < i && done[idx[t]] != i + 1 && (k < 0 || idx[t] < k)) k = idx[t];
+            if (k < 0) break;
+            done[k] = i + 1;
+            if (w[k] == 0.0) continue;
+            w[k] /= fval[fdg[k]];
+            if (fabs(w[k]) < ti) { w[k] = 0.0; continue; }
+            for (long q = fdg[k] + 1; q < frp[k + 1]; q++) {
+                const int j = fcol[q];
+                if (mark[j] != i) { mark[j] = i; w[j] = 0.0; idx[cnt++] = j; }
+                w[j] -= w[k] * fval[q];
+            }
+        }
+        int nl = 0, nu = 0;
+        ilut_entry *L = ent, *U;
+        for (int t = 0; t < cnt; t++) {
+            const int j = idx[t];
+            if (j < i && w[j] != 0.0 && fabs(w[j]) >= ti) L[nl++] = (ilut_entry){j, w[j]};
+        }
+        U = ent + nl;
+        for (int t = 0; t < cnt; t++) {
+            const int j = idx[t];
+            if (j > i && w[j] != 0.0 && fabs(w[j]) >= ti) U[nu++] = (ilut_entry){j, w[j]};
+        }
+        qsort(L, nl, sizeof(ilut_entry), ilut_by_mag);
+        qsort(U, nu, sizeof(ilut_entry), ilut_by_mag);
+        if (nl > p) nl = p;
+        if (nu > p) nu = p;
+        qsort(L, nl, sizeof(ilut_entry), ilut_by_col);
+        qsort(U, nu, sizeof(ilut_entry), ilut_by_col);
+        if (nnz + nl + nu + 1 > cap) {
+            cap = 2 * cap + nl + nu + 1;
+            fcol = realloc(fcol, sizeof(int) * cap);
+            fval = realloc(fval, sizeof(double) * cap);
+        }
+        for (int t = 0; t < nl; t++) { fcol[nnz] = L[t].j; fval[nnz++] = L[t].v; }
+        double d = w[i];
+        if (d == 0.0) d = ti > 0 ? ti : 1.0;
+        fdg[i] = (int)nnz;
+        fcol[nnz] = i;
+        fval[nnz++] = d;
+        for (int t = 0; t < nu; t++) { fcol[nnz] = U[t].j; fval[nnz++] = U[t].v; }
+        frp[i + 1] = (int)nnz;
+        for (int t = 0; t < cnt; t++) w[idx[t]] = 0.0;
+    }
+    free(w); free(mark); free(idx); free(done); free(ent); free(fdg);
+    *rp_out = frp;
+    *col_out = fcol;
+    *val_out = fval;
+    return nnz;
+}
+
+static int ilut_p = 10;
+static double ilut_tau = 1e-4;
+void or_set_ilut(int p, double tau)
+{
+    ilut_p = p;
+    ilut_tau = tau;
+}
+
 typedef struct {
     int n;
     const int *rp, *col;
     const double *val;
     int prec;
     double *dinv, *lu, *tmp;
+    int *frp, *fcol;  /* ILUT: the factors' own pattern */
 } krylov_op;
 
 static void prec_apply(krylov_op *op, const double *x, double *y)
@@ -1075,6 +1181,8 @@ static void prec_apply(krylov_op *op, const double *x, double *y)
         for (int r = 0; r < n; r++) y[r] = x[r] * op->dinv[r];
     } else if (op->prec == OR_PREC_ILU0) {
         or_ilu0_apply(n, op->rp, op->col, op->lu, x, y);
+    } else if (op->prec == OR_PREC_ILUT) {
+        or_ilu0_apply(n, op->frp, op->fcol, op->lu, x, y);
     } else {
         memcpy(y, x, sizeof(double) * n);
     }
@@ -1625,7 +1733,7 @@ int or_krylov(int nrows, const int *rp, const int *col, const double *val, const
               double *x, int solver, int precond, double tol, int max_iter, int gmres_restart,
               int *iterations, double *residue)
 {
-    krylov_op op = {nrows, rp, col, val, precond, NULL, NULL, NULL};
+    krylov_op op = {nrows, rp, col, val, precond, NULL, NULL, NULL, NULL, NULL};
     op.tmp = malloc(sizeof(double) * nrows);
     int rc = 0;
     if (precond == OR_PREC_DIAGONAL) {
@@ -1642,6 +1750,8 @@ int or_krylov(int nrows, const int *rp, const int *col, const double *val, const
         op.lu = malloc(sizeof(double) * nnz);
         memcpy(op.lu, val, sizeof(double) * nnz);
         if (or_ilu0_factorize(nrows, rp, col, op.lu)) rc = -1;
+    } else if (precond == OR_PREC_ILUT) {
+        if (or_ilut_factorize(nrows, rp, col, val, ilut_p, ilut_tau, &op.frp, &op.fcol, &op.lu) < 0) rc = -1;
     }
     int conv = 0;
     if (!rc) {
@@ -1654,7 +1764,7 @@ int or_krylov(int nrows, const int *rp, const int *col, const double *val, const
         else if (solver == OR_SOR) conv = sor(&op, b, x, tol, max_iter, gmres_restart / 100.0, iterations, residue);
         else conv = bicgstab(&op, b, x, tol, max_iter, iterations, residue);
     }
-    free(op.tmp); free(op.dinv); free(op.lu);
+    free(op.tmp); free(op.dinv); free(op.lu); free(op.frp); free(op.fcol);
     return rc ? -1 : conv;
 }
 

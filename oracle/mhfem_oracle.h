@@ -43,7 +43,12 @@ enum { OR_BICGSTAB = 0, OR_GMRES = 1, OR_CG = 2, OR_TFQMR = 3, OR_JACOBI = 4, OR
  * s of IDR(s) and 100 w for SOR's relaxation w; or_idr_shadow_value: entry
  * (i, k) of IDR(s)'s shadow space before orthonormalisation */
 double or_idr_shadow_value(long i, int k);
-enum { OR_PREC_NONE = 0, OR_PREC_DIAGONAL = 1, OR_PREC_ILU0 = 2 };
+enum { OR_PREC_NONE = 0, OR_PREC_DIAGONAL = 1, OR_PREC_ILU0 = 2, OR_PREC_ILUT = 3 };
+/* ILUT(p, tau) factors (CSR, sorted columns, unit-lower L left of U's
+ * diagonal; arrays malloc'ed) and the p, tau that or_krylov's ILUT uses */
+long or_ilut_factorize(int n, const int *rp, const int *col, const double *val, int p, double tau, int **rp_out,
+                       int **col_out, double **val_out);
+void or_set_ilut(int p, double tau);
 
 /* Topology only: face numbering and incidence.  Returns the number of faces
  * (>= 0) or -1 on a non-conforming mesh.  cell_faces: nc*(dim+1),

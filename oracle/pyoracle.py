@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MHFEM_ORACLE_LIB", os.path.join(_HERE, "_build", "libmhfem_oracle.so"))
 
 SOLVERS = {"bicgstab": 0, "gmres": 1, "cg": 2, "tfqmr": 3, "jacobi": 4, "bicgstabl": 5, "idrs": 6, "sor": 7}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "ilut": 3}
 
 _lib = None
 
@@ -55,6 +55,8 @@ def lib():
             "or_spmv": (None, [I, P, P, P, P, P]),
             "or_krylov": (I, [I, P, P, P, P, P, I, I, D, I, I, C.POINTER(I), C.POINTER(D)]),
             "or_local_mass_inverse": (None, [I, I, P, D, D, P]),
+            "or_ilut_factorize": (L64, [I, P, P, P, I, D, P, P, P]),
+            "or_set_ilut": (None, [I, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -109,6 +111,30 @@ def ilu0(rowptr, col, val):
     if rc:
         raise ValueError("ILU(0) breakdown")
     return lu
+
+
+def ilut(rowptr, col, val, p=10, tau=1e-4):
+    """ILUT(p, tau) factors as (rowptr, col, values) in one CSR."""
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    c = np.ascontiguousarray(col, np.int32)
+    v = np.ascontiguousarray(val, np.float64)
+    prp, pcol, pval = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nnz = lib().or_ilut_factorize(rp.size - 1, _ptr(rp), _ptr(c), _ptr(v), p, tau, C.byref(prp), C.byref(pcol),
+                                  C.byref(pval))
+    if nnz < 0:
+        raise MemoryError("ILUT")
+    n = rp.size - 1
+    out = (np.frombuffer((C.c_int * (n + 1)).from_address(prp.value), np.int32).copy(),
+           np.frombuffer((C.c_int * nnz).from_address(pcol.value), np.int32).copy(),
+           np.frombuffer((C.c_double * nnz).from_address(pval.value), np.float64).copy())
+    libc = C.CDLL(None)
+    for ptr in (prp, pcol, pval):
+        libc.free(ptr)
+    return out
+
+
+def set_ilut(p, tau):
+    lib().or_set_ilut(p, tau)
 
 
 def ilu0_apply(rowptr, col, lu, x):

@@ -26,7 +26,7 @@ def init_py_initial_condition(points, cells, alpha=-0.05, beta=-0.05, gamma=5.0)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("solver,prec", [("gmres", "diagonal"), ("cg", "diagonal"), ("tfqmr", "ILU_ISAI"),
+@pytest.mark.parametrize("solver,prec", [("gmres", "diagonal"), ("cg", "diagonal"), ("tfqmr", "ILU_ISAI"), ("idrs", "ilut"), ("sor", "none"),
                                          ("bicgstabl", "AMGX")])
 def test_config_driven_heat_example_matches_oracle(tmp_path, solver, prec):
     import pyoracle

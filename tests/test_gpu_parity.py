@@ -170,7 +170,8 @@ def test_time_steps(case):
 
 
 @pytest.mark.parametrize("solver,prec", [("bicgstab", "diagonal"), ("bicgstab", "none"), ("bicgstab", "ilu0"),
-                                         ("gmres", "diagonal"), ("gmres", "ilu0")])
+                                         ("gmres", "diagonal"), ("gmres", "ilu0"), ("bicgstab", "ilut"),
+                                         ("gmres", "ilut"), ("tfqmr", "ilut")])
 def test_krylov_iteration_parity(solver, prec):
     """Same algorithm, same x0: iteration counts agree and the iterates match."""
     print(f"\n[krylov] {solver} {prec}", flush=True)
@@ -267,7 +268,7 @@ def test_sor_iteration_parity(omega, case):
     s.close()
 
 
-@pytest.mark.parametrize("solver,prec", [("sor", "none"), ("bicgstabl", "ilu-isai"), ("idrs", "amg"),
+@pytest.mark.parametrize("solver,prec", [("sor", "none"), ("bicgstab", "ilut"), ("bicgstabl", "ilu-isai"), ("idrs", "amg"),
                                          ("cg", "diagonal"), ("tfqmr", "ilu0"), ("tfqmr", "amg"),
                                          ("cg", "amg"), ("jacobi", "ilu0")])
 def test_other_solvers_step_parity(solver, prec):
@@ -283,6 +284,18 @@ def test_other_solvers_step_parity(solver, prec):
         t += pr.dt
         assert st.converged
         assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10, k
+    s.close()
+
+
+def test_ilut_refusals():
+    pr = problems.heat_2d(s=6, steps=1)
+    with pytest.raises(capi.MHFEMError, match="FP64"):
+        capi.Solver(pr, preconditioner="ilut", precision="fp32-refined")
+    with pytest.raises(capi.MHFEMError, match="SOR"):
+        capi.Solver(pr, solver="sor", preconditioner="ilut")
+    s = capi.Solver(pr, preconditioner="ilut")
+    with pytest.raises(capi.MHFEMError, match="ILUT"):
+        s.step_graph(0.0, pr.dt)
     s.close()
 
 

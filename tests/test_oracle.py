@@ -670,3 +670,57 @@ def test_sor_by_hand_and_against_direct_solve():
                                            restart=m)
         assert conv == 1
         assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+
+
+def _ilut_dense(rp, col, val, n):
+    frp, fcol, fval = pyoracle.ilut(rp, col, val, *_ILUT)
+    F = sp.csr_matrix((fval, fcol, frp), shape=(n, n)).toarray()
+    return np.tril(F, -1) + np.eye(n), np.triu(F), frp
+
+
+_ILUT = (10, 1e-4)
+
+
+def test_ilut_without_dropping_is_the_complete_lu():
+    """tau = 0 and p >= N: nothing is dropped, so L U = A (the complete LU with
+    fill-in), checked against the closed form on a tridiagonal matrix and by
+    the product on an MHFEM system; BiCGStab then converges at once."""
+    global _ILUT
+    A = sp.csr_matrix(np.array([[4.0, 1.0, 0.0], [1.0, 4.0, 1.0], [0.0, 1.0, 4.0]]))
+    _ILUT = (3, 0.0)
+    L, U, _ = _ilut_dense(A.indptr, A.indices, A.data, 3)
+    assert np.allclose(L, [[1, 0, 0], [0.25, 1, 0], [0, 1 / 3.75, 1]], rtol=1e-15)
+    assert np.allclose(U, [[4, 1, 0], [0, 3.75, 1], [0, 0, 4 - 1 / 3.75]], rtol=1e-15)
+    pr = problems.darcy_3d(s=2)
+    o = pyoracle.Oracle(pr)
+    o.pre_iterate(0.0, pr.dt)
+    o.assemble(0.0, pr.dt)
+    rp, col, val, rhs = [np.array(a) for a in o.csr()]
+    n = rp.size - 1
+    _ILUT = (n, 0.0)
+    L, U, frp = _ilut_dense(rp, col, val, n)
+    Ad = sp.csr_matrix((val, col, rp), shape=(n, n)).toarray()
+    assert np.abs(L @ U - Ad).max() <= 1e-12 * np.abs(Ad).max()
+    assert frp[-1] > rp[-1]  # fill-in was kept
+    pyoracle.set_ilut(n, 0.0)
+    x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), "bicgstab", "ilut", 1e-10)
+    assert conv == 1 and it <= 1
+    _ILUT = (10, 1e-4)
+    pyoracle.set_ilut(10, 1e-4)
+
+
+def test_ilut_with_p0_is_the_diagonal_and_default_beats_ilu0():
+    global _ILUT
+    rp, col, val, rhs, _ = _mhfem_system("none")
+    n = rp.size - 1
+    _ILUT = (0, 1e-4)
+    L, U, _ = _ilut_dense(rp, col, val, n)
+    Ad = sp.csr_matrix((val, col, rp), shape=(n, n)).toarray()
+    assert np.array_equal(L, np.eye(n)) and np.array_equal(U, np.diag(np.diag(Ad)))
+    _ILUT = (10, 1e-4)
+    pyoracle.set_ilut(10, 1e-4)
+    xd = spla.spsolve(sp.csr_matrix((val, col, rp), shape=(n, n)).tocsc(), rhs)
+    x, conv, it, res = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), "bicgstab", "ilut", 1e-12)
+    _, _, it0, _ = pyoracle.krylov(rp, col, val, rhs, np.zeros(n), "bicgstab", "ilu0", 1e-12)
+    assert conv == 1 and np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+    assert it <= it0, (it, it0)

@@ -719,10 +719,15 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
     return guarded(ctx, [&](Ctx &c) {
         if (!cfg) throw Error{MHFEM_E_ARGUMENT, "mhfem_setup_linear_system: null config"};
         if (cfg->solver < 0 || cfg->solver > MHFEM_SOLVER_SOR) throw Error{MHFEM_E_ARGUMENT, "unknown linear solver"};
-        if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_PARILU_ISAI)
+        if (cfg->preconditioner < 0 || cfg->preconditioner > MHFEM_PREC_ILUT)
             throw Error{MHFEM_E_ARGUMENT, "unknown preconditioner"};
-        const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || is_isai(cfg->preconditioner);
-        const char *pname = cfg->preconditioner == MHFEM_PREC_AMG ? "AMG" : "ILU-ISAI";
+        const bool reused = cfg->preconditioner == MHFEM_PREC_AMG || is_isai(cfg->preconditioner) ||
+                            cfg->preconditioner == MHFEM_PREC_ILUT;
+        const char *pname = cfg->preconditioner == MHFEM_PREC_AMG    ? "AMG"
+                            : cfg->preconditioner == MHFEM_PREC_ILUT ? "ILUT"
+                                                                     : "ILU-ISAI";
+        if (cfg->preconditioner == MHFEM_PREC_ILUT && cfg->solver == MHFEM_SOLVER_SOR)
+            throw Error{MHFEM_E_ARGUMENT, "SOR uses no preconditioner: take none with it, not ILUT"};
         if (reused && cfg->precision != MHFEM_PRECISION_FP64)
             throw Error{MHFEM_E_ARGUMENT, std::string("the ") + pname + " preconditioner works with the FP64 solvers"};
         if (reused && c.h_seg_faces.size() > 2)
@@ -738,6 +743,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         if (c.cfg.idr_s <= 0) c.cfg.idr_s = 4;
         if (c.cfg.bicgstab_ell > 8 || c.cfg.idr_s > 8) throw Error{MHFEM_E_ARGUMENT, "bicgstab_ell and idr_s are at most 8"};
         if (c.cfg.sor_omega == 0.0) c.cfg.sor_omega = 1.0;
+        if (c.cfg.ilut_p <= 0) c.cfg.ilut_p = 10;
+        if (c.cfg.ilut_threshold <= 0.0) c.cfg.ilut_threshold = 1e-4;
         if (!(c.cfg.sor_omega > 0.0 && c.cfg.sor_omega < 2.0)) throw Error{MHFEM_E_ARGUMENT, "sor_omega must lie in (0, 2)"};
         // row lengths (Solver.hpp:449-474)
         const int n = c.n, fpc = c.fpc;

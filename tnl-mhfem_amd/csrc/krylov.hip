@@ -761,6 +761,10 @@ struct Ilu0 {
     DBuf<double> lu;
     DBuf<float> lu32;
     DBuf<double> a0, swp;  // ParILU: the matrix in CSR order, the other sweep buffer
+    // ILUT: factors on their own pattern (rebuilt with the factors)
+    bool own = false;
+    DBuf<int> frp, fcol;
+    std::vector<double> h_a;  // the matrix values, host copy
 };
 
 // ---------------------------------------------------------------------------
@@ -1055,6 +1059,127 @@ static int isai_tune_lanes(Ctx &c, const IsaiFactor &F, const double *x, double 
     return best;
 }
 
+// level tables of a factor pattern (sorted columns, diagonal present): the
+// rows of the lower and of the upper triangular solve bucketed by dependency
+// depth, and the diagonal positions; returns the latter
+static std::vector<int> build_levels(Ilu0 &I, int n, const int *rp, const int *col)
+{
+    std::vector<int> dg(n), lev(n), ulev(n);
+    for (int r = 0; r < n; r++)
+        for (int q = rp[r]; q < rp[r + 1]; q++)
+            if (col[q] == r) dg[r] = q;
+    int nl = 0, nu = 0;
+    for (int r = 0; r < n; r++) {
+        int l = 0;
+        for (int q = rp[r]; q < dg[r]; q++) l = std::max(l, lev[col[q]] + 1);
+        lev[r] = l;
+        nl = std::max(nl, l + 1);
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        int l = 0;
+        for (int q = dg[r] + 1; q < rp[r + 1]; q++) l = std::max(l, ulev[col[q]] + 1);
+        ulev[r] = l;
+        nu = std::max(nu, l + 1);
+    }
+    auto bucket = [n](const std::vector<int> &lv, int nlev, std::vector<int> &ptr, std::vector<int> &rows) {
+        ptr.assign(nlev + 1, 0);
+        for (int r = 0; r < n; r++) ptr[lv[r] + 1]++;
+        for (int l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
+        rows.resize(n);
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (int r = 0; r < n; r++) rows[pos[lv[r]]++] = r;
+    };
+    std::vector<int> lrows, urows;
+    bucket(lev, nl, I.h_lptr, lrows);
+    bucket(ulev, nu, I.h_uptr, urows);
+    if (!I.lrows.p) {
+        I.lrows.alloc(n);
+        I.urows.alloc(n);
+        I.diag.alloc(n);
+    }
+    MHFEM_HIP(hipMemcpy(I.lrows.p, lrows.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(I.urows.p, urows.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(I.diag.p, dg.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    return dg;
+}
+
+// ILUT(p, tau) (Saad, Iterative Methods for Sparse Linear Systems, Alg. 10.6;
+// TNL's "ilut" preconditioner), on the host: each row of A, in a dense work
+// row, is eliminated against the finished rows of U in increasing column
+// order (fill-in included); multipliers below tau_i = tau ||a_i||_2 are
+// dropped; of the rest, entries below tau_i go and only the p largest of the
+// L part and of the U part stay (ties to the smaller column), the diagonal
+// always (a zero pivot becomes tau_i, or 1).  Output: one CSR, sorted
+// columns, unit-lower L left of U's diagonal.
+static void ilut_host(int n, const int *rp, const int *col, const double *val, int p, double tau,
+                      std::vector<int> &frp, std::vector<int> &fcol, std::vector<double> &fval)
+{
+    std::vector<double> w(n, 0.0);
+    std::vector<int> mark(n, -1), seen(n, -1), fdg(n), nz;
+    std::vector<std::pair<int, double>> lo, up;
+    frp.assign(1, 0);
+    fcol.clear();
+    fval.clear();
+    fcol.reserve((size_t)rp[n] * 2);
+    fval.reserve((size_t)rp[n] * 2);
+    auto keep = [p](std::vector<std::pair<int, double>> &v) {
+        std::sort(v.begin(), v.end(), [](const auto &a, const auto &b) {
+            const double x = std::fabs(a.second), y = std::fabs(b.second);
+            return x != y ? x > y : a.first < b.first;
+        });
+        if ((int)v.size() > p) v.resize(p);
+        std::sort(v.begin(), v.end());
+    };
+    for (int i = 0; i < n; i++) {
+        nz.clear();
+        double nrm = 0.0;
+        auto touch = [&](int j) {
+            if (mark[j] != i) {
+                mark[j] = i;
+                w[j] = 0.0;
+                nz.push_back(j);
+            }
+        };
+        for (int q = rp[i]; q < rp[i + 1]; q++) {
+            touch(col[q]);
+            w[col[q]] = val[q];
+            nrm += val[q] * val[q];
+        }
+        touch(i);
+        const double ti = tau * std::sqrt(nrm);
+        for (;;) {
+            int kk = -1;
+            for (int j : nz)
+                if (j < i && seen[j] != i && (kk < 0 || j < kk)) kk = j;
+            if (kk < 0) break;
+            seen[kk] = i;
+            if (w[kk] == 0.0) continue;
+            w[kk] /= fval[fdg[kk]];
+            if (std::fabs(w[kk]) < ti) {
+                w[kk] = 0.0;
+                continue;
+            }
+            for (int q = fdg[kk] + 1; q < frp[kk + 1]; q++) {
+                touch(fcol[q]);
+                w[fcol[q]] -= w[kk] * fval[q];
+            }
+        }
+        lo.clear();
+        up.clear();
+        for (int j : nz)
+            if (j != i && w[j] != 0.0 && std::fabs(w[j]) >= ti) (j < i ? lo : up).emplace_back(j, w[j]);
+        keep(lo);
+        keep(up);
+        for (const auto &e : lo) fcol.push_back(e.first), fval.push_back(e.second);
+        fdg[i] = (int)fcol.size();
+        fcol.push_back(i);
+        fval.push_back(w[i] != 0.0 ? w[i] : (ti > 0 ? ti : 1.0));
+        for (const auto &e : up) fcol.push_back(e.first), fval.push_back(e.second);
+        if (fcol.size() > (size_t)INT32_MAX) throw Error{MHFEM_E_ARGUMENT, "ILUT factors exceed 2^31 entries"};
+        frp.push_back((int)fcol.size());
+    }
+}
+
 // patterns of L^2 and U^2 from the factors' pattern (that of the matrix):
 // row i of L^2 is the union of the rows t of L that row i of L reaches
 static void isai_patterns(Ctx &c, Krylov &k, const std::vector<int> &dg)
@@ -1108,43 +1233,11 @@ Krylov *krylov_create(Ctx &c)
         MHFEM_HIP(hipEventCreateWithFlags(&k->ev_ahead[q], hipEventDisableTiming));
     }
     build_blocks(c, *k, c.cfg.rows_per_block > 0 ? c.cfg.rows_per_block : 2048);
+    if (c.cfg.preconditioner == MHFEM_PREC_ILUT) k->ilu.a0.alloc(c.nnz);
     // level tables: ILU(0), the ISAI factors and SOR's sweeps
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0 || is_isai(c.cfg.preconditioner) || c.cfg.solver == MHFEM_SOLVER_SOR) {
         Ilu0 &I = k->ilu;
-        std::vector<int> dg(n), lev(n), ulev(n);
-        for (int r = 0; r < n; r++)
-            for (int q = c.h_rowptr[r]; q < c.h_rowptr[r + 1]; q++)
-                if (c.h_col[q] == r) dg[r] = q;
-        int nl = 0, nu = 0;
-        for (int r = 0; r < n; r++) {
-            int l = 0;
-            for (int q = c.h_rowptr[r]; q < dg[r]; q++) l = std::max(l, lev[c.h_col[q]] + 1);
-            lev[r] = l;
-            nl = std::max(nl, l + 1);
-        }
-        for (int r = n - 1; r >= 0; r--) {
-            int l = 0;
-            for (int q = dg[r] + 1; q < c.h_rowptr[r + 1]; q++) l = std::max(l, ulev[c.h_col[q]] + 1);
-            ulev[r] = l;
-            nu = std::max(nu, l + 1);
-        }
-        auto bucket = [n](const std::vector<int> &lv, int nlev, std::vector<int> &ptr, std::vector<int> &rows) {
-            ptr.assign(nlev + 1, 0);
-            for (int r = 0; r < n; r++) ptr[lv[r] + 1]++;
-            for (int l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
-            rows.resize(n);
-            std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-            for (int r = 0; r < n; r++) rows[pos[lv[r]]++] = r;
-        };
-        std::vector<int> lrows, urows;
-        bucket(lev, nl, I.h_lptr, lrows);
-        bucket(ulev, nu, I.h_uptr, urows);
-        I.lrows.alloc(n);
-        I.urows.alloc(n);
-        I.diag.alloc(n);
-        MHFEM_HIP(hipMemcpy(I.lrows.p, lrows.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        MHFEM_HIP(hipMemcpy(I.urows.p, urows.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        MHFEM_HIP(hipMemcpy(I.diag.p, dg.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        const std::vector<int> dg = build_levels(I, n, c.h_rowptr.data(), c.h_col.data());
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
         if (is_isai(c.cfg.preconditioner)) isai_patterns(c, *k, dg);
@@ -1332,7 +1425,7 @@ struct Solve {
         if constexpr (sizeof(T) == 8) lu = I.lu.p;
         else lu = I.lu32.p;
         const double *flag = (skip_when_halted && k.nseg == 1) ? k.sc.p : nullptr;
-        const int *rp = c.rowptr.p, *col = c.csr_col.p;
+        const int *rp = I.own ? I.frp.p : c.rowptr.p, *col = I.own ? I.fcol.p : c.csr_col.p;
         for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
             const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
             hipLaunchKernelGGL((k_ilu_lower<T>), (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
@@ -1375,7 +1468,7 @@ struct Solve {
     bool separate_prec() const
     {
         const int p = c.cfg.preconditioner;
-        return p == MHFEM_PREC_ILU0 || p == MHFEM_PREC_AMG || is_isai(p);
+        return p == MHFEM_PREC_ILU0 || p == MHFEM_PREC_ILUT || p == MHFEM_PREC_AMG || is_isai(p);
     }
     template <typename T>
     void separate_apply(const T *x, T *y, bool skip_when_halted)
@@ -2223,6 +2316,28 @@ static void precond_update(Ctx &c, Krylov &k)
             S.rebuilds++;
         }
     }
+    if (c.cfg.preconditioner == MHFEM_PREC_ILUT) {
+        // every step, as the reference's preconditioners (Solver.hpp:1237-1242)
+        Ilu0 &I = k.ilu;
+        hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p,
+                           c.csr_to_sell.p, I.a0.p);
+        MHFEM_HIP(hipGetLastError());
+        I.h_a.resize(c.nnz);
+        MHFEM_HIP(hipMemcpyAsync(I.h_a.data(), I.a0.p, c.nnz * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+        std::vector<int> frp, fcol;
+        std::vector<double> fval;
+        ilut_host(n, c.h_rowptr.data(), c.h_col.data(), I.h_a.data(), c.cfg.ilut_p, c.cfg.ilut_threshold, frp, fcol,
+                  fval);
+        if (I.lu.n < fval.size()) I.lu.alloc(fval.size());
+        if (I.fcol.n < fcol.size()) I.fcol.alloc(fcol.size());
+        if (!I.frp.p) I.frp.alloc(n + 1);
+        MHFEM_HIP(hipMemcpy(I.frp.p, frp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(I.fcol.p, fcol.data(), fcol.size() * sizeof(int), hipMemcpyHostToDevice));
+        MHFEM_HIP(hipMemcpy(I.lu.p, fval.data(), fval.size() * sizeof(double), hipMemcpyHostToDevice));
+        build_levels(I, n, frp.data(), fcol.data());
+        I.own = true;
+    }
     if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
         Ilu0 &I = k.ilu;
         ilu_factor(c, k);
@@ -2628,9 +2743,10 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
     Krylov &k = *c.krylov;
     if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB || c.cfg.precision != MHFEM_PRECISION_FP64)
         throw Error{MHFEM_E_ARGUMENT, "graph steps use FP64 BiCGStab"};
-    if (c.cfg.preconditioner == MHFEM_PREC_AMG || is_isai(c.cfg.preconditioner))
-        throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG or ILU-ISAI preconditioners (their reuse rule "
-                                      "runs on the host); use the four calls"};
+    if (c.cfg.preconditioner == MHFEM_PREC_AMG || is_isai(c.cfg.preconditioner) ||
+        c.cfg.preconditioner == MHFEM_PREC_ILUT)
+        throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG, ILU-ISAI or ILUT preconditioners (their set-up "
+                                      "decisions run on the host); use the four calls"};
     const int every = std::max(1, c.cfg.check_every);
     if (k.g_first && (k.g_tau != tau || k.g_iters != every)) drop_graphs(k);
     Solve S(c, k);

@@ -22,7 +22,7 @@ ARRAYS = ["Z_iF", "Z_iK", "N_ijK", "u_ijKe", "m_iK", "D_ijK", "w_iKe", "a_ijKe",
 ARRAY_ID = {name: k for k, name in enumerate(ARRAYS)}
 
 SOLVERS = {"bicgstab": 0, "gmres": 1, "cg": 2, "tfqmr": 3, "jacobi": 4, "bicgstabl": 5, "idrs": 6, "sor": 7}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3, "ilu-isai": 4, "parilu-isai": 5}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "jacobi": 1, "ilu0": 2, "amg": 3, "ilu-isai": 4, "parilu-isai": 5, "ilut": 6}
 PRECISIONS = {"fp64": 0, "fp32-refined": 1, "fp32-reliable": 2}
 
 
@@ -42,7 +42,7 @@ class SolverConfig(C.Structure):
                 ("gmres_restart", C.c_int), ("precision", C.c_int),
                 ("inner_residue", C.c_double), ("check_every", C.c_int), ("rows_per_block", C.c_int),
                 ("fp32_face_groups", C.c_int), ("bicgstab_ell", C.c_int), ("idr_s", C.c_int),
-                ("sor_omega", C.c_double)]
+                ("sor_omega", C.c_double), ("ilut_p", C.c_int), ("ilut_threshold", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -288,7 +288,7 @@ class Solver:
     def __init__(self, problem, solver=None, preconditioner=None, tol=None, precision="fp64",
                  inner_tol=1e-4, check_every=8, max_iterations=100000, gmres_restart=30,
                  setup=True, rows_per_block=0, output_directory=None, fp32_face_groups=0, bicgstab_ell=0,
-                 idr_s=0, sor_omega=0.0):
+                 idr_s=0, sor_omega=0.0, ilut_p=0, ilut_threshold=0.0):
         pr = problem
         self.problem = pr
         # where a failed solve leaves its linear system (the reference's output-directory)
@@ -320,7 +320,7 @@ class Solver:
                                    PRECONDITIONERS[preconditioner or pr.preconditioner],
                                    pr.tol if tol is None else tol, max_iterations, gmres_restart,
                                    PRECISIONS[precision], inner_tol, check_every, rows_per_block,
-                                   fp32_face_groups, bicgstab_ell, idr_s, sor_omega)
+                                   fp32_face_groups, bicgstab_ell, idr_s, sor_omega, ilut_p, ilut_threshold)
         if setup:
             self.setInitialCondition()
             self.setupLinearSystem()
