@@ -394,3 +394,32 @@ def test_unsupported_cells_rejected():
     for vpc in (5, 6, 3):
         assert not L.mhfem_create_cells(mesh, vpc, model)
         assert b"are not supported" in L.mhfem_last_error(None)
+
+
+ALL_SOLVERS = ["bicgstab", "gmres", "cg", "tfqmr", "jacobi", "bicgstabl", "idrs", "sor"]
+
+
+@pytest.mark.parametrize("solver", ALL_SOLVERS)
+def test_solvers_on_zero_rhs_and_from_the_solution(solver):
+    """Edge cases of every linear solver: a zero right-hand side from a zero
+    start needs no iteration and leaves x at zero; started from a solution
+    already within the tolerance, a solver stops at once."""
+    print(f"\n[krylov] {solver} edge cases", flush=True)
+    pr = problems.heat_2d(s=8, steps=1)
+    prec = "none" if solver == "sor" else "diagonal"
+    s = capi.Solver(pr, solver=solver, preconditioner=prec, tol=1e-10)
+    s.preIterate(0.0, pr.dt)
+    s.assembleLinearSystem(0.0, pr.dt)
+    rp, col, val, rhs = s.export_system()
+    n = rhs.size
+    x, st = s.solve_rhs(np.zeros(n), np.zeros(n))
+    assert st.converged and st.iterations == 0 and not x.any()
+    ref = capi.Solver(pr, preconditioner="ilu0", tol=1e-14)
+    ref.preIterate(0.0, pr.dt)
+    ref.assembleLinearSystem(0.0, pr.dt)
+    xs, st_ref = ref.solve_rhs(rhs, np.zeros(n))
+    ref.close()
+    x, st = s.solve_rhs(rhs, xs)
+    assert st.converged and st.iterations == 0, st.iterations
+    assert rel(x, xs) < 1e-14
+    s.close()
