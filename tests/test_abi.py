@@ -38,3 +38,32 @@ def test_create_rejects_bad_arguments():
     L = capi.lib()
     assert not L.mhfem_create(None, None)
     assert b"null" in L.mhfem_last_error(None)
+
+
+STRUCTS = {"mhfem_mesh_desc": "MeshDesc", "mhfem_model_desc": "ModelDesc", "mhfem_solver_config": "SolverConfig",
+           "mhfem_solve_stats": "SolveStats", "mhfem_timers": "Timers", "mhfem_guess_config": "GuessConfig",
+           "mhfem_guess_stats": "GuessStats"}
+
+
+def test_struct_layouts_match_the_python_mirror(tmp_path):
+    """Size and every field offset of each struct in include/mhfem.h, as the C
+    compiler lays them out, equal the ctypes mirror's."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "mhfem.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        st = getattr(capi, pyname)
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in st._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, pyname in STRUCTS.items():
+        st = getattr(capi, pyname)
+        assert got[(cname, "size")] == ctypes.sizeof(st), cname
+        for field, _ in st._fields_:
+            assert got[(cname, field)] == getattr(st, field).offset, (cname, field)
