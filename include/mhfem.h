@@ -231,6 +231,31 @@ void *mhfem_stream(mhfem_ctx *ctx); /* the hipStream_t of the context */
 int mhfem_set_boundary_conditions(mhfem_ctx *ctx, const uint8_t *tags, const double *values,
                                   const double *dirichlet_values);
 
+/* ---- boundary model: the BoundaryModel argument of the reference's Solver
+ * (Solver.h:28-31), whose getDirichletValue / getNeumannValue (i, E, time,
+ * tau) the assembly calls on the device with time = t + tau (Solver.hpp:948,
+ * BoundaryConditions.hpp:379, 414, 455).  The default is the reference's
+ * Stationary model (BoundaryModels/Stationary.h:14-37): the stored
+ * dirichlet_values[i*F+E] and values[i*F+E].  A time function per species
+ * and per kind of value makes the model time dependent:
+ *   getDirichletValue(i, E, time) = dirichlet_values[i*F+E] * f_D,i(time)
+ *   getNeumannValue(i, E, time)   = values[i*F+E] * f_N,i(time)
+ * evaluated on the device inside the assembly, for the boundary rows only
+ * (no per-step upload).  Kinds: CONSTANT f = offset; SINE f = offset +
+ * amplitude * sin(angular_frequency * t + phase); TABLE f piecewise linear
+ * through (times[k], values[k]), k < points (times strictly ascending),
+ * constant before the first and after the last point (the arrays are
+ * copied).  which = MHFEM_BC_DIRICHLET or MHFEM_BC_VALUES; f = NULL restores
+ * the stationary value of that species. */
+enum { MHFEM_TIME_CONSTANT = 0, MHFEM_TIME_SINE = 1, MHFEM_TIME_TABLE = 2 };
+typedef struct {
+    int kind;                   /* MHFEM_TIME_* */
+    double offset, amplitude, angular_frequency, phase;
+    int points;                 /* MHFEM_TIME_TABLE */
+    const double *times, *values;
+} mhfem_time_function;
+int mhfem_set_boundary_time_function(mhfem_ctx *ctx, int which, int species, const mhfem_time_function *f);
+
 /* ---- Solver::setInitialCondition (Solver.hpp:255-344) for component i
  * (BaseModel::setInitialCondition, BaseModel.h:316-326) */
 int mhfem_set_initial_condition(mhfem_ctx *ctx, int i, const double *Z_K, long count);

@@ -1,7 +1,10 @@
 // BoundaryModels.hpp -- boundary models (BoundaryModels/Stationary.h:8-38).
 // A boundary model supplies the Neumann and Dirichlet values of (i, E) at
-// time + tau; the solver evaluates it on the host before each assembly
-// (the reference evaluates it inside the assembly kernel, Solver.hpp:948).
+// time + tau.  A model with time_dependent = true is evaluated on the host
+// before each assembly (the reference evaluates it inside the assembly
+// kernel, Solver.hpp:948); a separable model (stored values times a function
+// of time per species) is evaluated on the device instead, through
+// Solver::setBoundaryTimeFunction.
 #pragma once
 
 #include <vector>

@@ -103,6 +103,17 @@ public:
 
     // boundary conditions before setInitialCondition (BoundaryConditions::init, BoundaryConditions.hpp:272-281)
     void setBoundaryConditions(const BoundaryConditionsStorage &s) { bc_ = s; }
+    // A time-dependent boundary model evaluated on the device inside the
+    // assembly (mhfem_set_boundary_time_function): the stored Dirichlet
+    // (which = MHFEM_BC_DIRICHLET) or Neumann (MHFEM_BC_VALUES) values of
+    // species i times f(time + tau).  Call after setMesh; nullptr restores
+    // the stationary values.  It needs no per-step upload, unlike a
+    // BoundaryModel with time_dependent = true, which is evaluated on the
+    // host before each assembly.
+    void setBoundaryTimeFunction(int which, int species, const mhfem_time_function *f)
+    {
+        check(mhfem_set_boundary_time_function(ctx_, which, species, f));
+    }
 
     // Solver::setInitialCondition (Solver.hpp:255-344).  The boundary
     // conditions come from "boundary-conditions-file" (TNL's binary format)

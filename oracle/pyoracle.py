@@ -230,8 +230,21 @@ class Oracle:
         lib().or_pre_iterate(self.p, t, tau)
 
     def assemble(self, t, tau):
-        if self.problem.dirichlet_at is not None:
-            self.array("bc_dirichlet")[:] = self.problem.dirichlet_at(t + tau)
+        """Solver::assembleLinearSystem; a time-dependent boundary model is
+        evaluated here on the host at time + tau (Solver.hpp:948)."""
+        pr = self.problem
+        if pr.boundary_functions:
+            from mhfem_amd import problems as _problems
+
+            F = self.nf
+            if not hasattr(self, "_bc_base"):
+                self._bc_base = (np.array(pr.bc_dirichlet, np.float64), np.array(pr.bc_values, np.float64))
+            for which, base, name in (("dirichlet", self._bc_base[0], "bc_dirichlet"),
+                                      ("neumann", self._bc_base[1], "bc_values")):
+                self.array(name)[:] = _problems.boundary_values_at(base, self.n, F, pr.boundary_functions,
+                                                                   which)(t + tau)
+        elif pr.dirichlet_at is not None:
+            self.array("bc_dirichlet")[:] = pr.dirichlet_at(t + tau)
         lib().or_assemble(self.p, t, tau)
 
     def solve(self, solver=None, preconditioner=None, tol=None, max_iter=100000, restart=30):

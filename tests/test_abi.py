@@ -42,7 +42,7 @@ def test_create_rejects_bad_arguments():
 
 STRUCTS = {"mhfem_mesh_desc": "MeshDesc", "mhfem_model_desc": "ModelDesc", "mhfem_solver_config": "SolverConfig",
            "mhfem_solve_stats": "SolveStats", "mhfem_timers": "Timers", "mhfem_guess_config": "GuessConfig",
-           "mhfem_guess_stats": "GuessStats"}
+           "mhfem_guess_stats": "GuessStats", "mhfem_time_function": "TimeFunction"}
 
 
 def test_struct_layouts_match_the_python_mirror(tmp_path):

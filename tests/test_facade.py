@@ -26,7 +26,7 @@ def test_heat_equation_example_matches_oracle():
 
     with tempfile.TemporaryDirectory() as d:
         out = os.path.join(d, "z.bin")
-        res = subprocess.run([EXAMPLE, out, "32", "20"], capture_output=True, text=True, timeout=120)
+        res = subprocess.run([EXAMPLE, out, "32", "20", "1e-13"], capture_output=True, text=True, timeout=120)
         print(res.stdout[-2000:], res.stderr[-2000:])
         assert res.returncode == 0
         assert "Total number of time steps:" in res.stdout
@@ -36,4 +36,4 @@ def test_heat_equation_example_matches_oracle():
     o.run(0.0, 20 * pr.dt, pr.dt, tol=1e-13)
     zo = o.array("Z_iK")
     assert z.size == zo.size
-    assert np.linalg.norm(z - zo) / np.linalg.norm(zo) < 1e-8
+    assert np.linalg.norm(z - zo) / np.linalg.norm(zo) < 1e-10

@@ -58,7 +58,7 @@ time-step = 0.05
 linear-solver = {solver}
 gmres-restarting-max = 30
 preconditioner = {prec}
-convergence-residue = 1e-11
+convergence-residue = 1e-13
 
 # Heat equation model parameters
 diffusivity = 1e-3
@@ -92,7 +92,7 @@ diffusivity = 1e-3
         zo = o.array("Z_iK")
         err = np.linalg.norm(cdata["Z"] - zo) / np.linalg.norm(zo)
         print(f"snapshot {step}: rel err {err:.2e}", flush=True)
-        assert err < 1e-8
+        assert err < 1e-10
         # the library's reader takes the snapshot back
         np.testing.assert_array_equal(capi.MeshFile(snap).cell_data("Z"), cdata["Z"])
     assert not (out / "data_00004.vtu").exists()

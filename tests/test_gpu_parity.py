@@ -11,7 +11,7 @@ import dataclasses
 import numpy as np
 import pytest
 
-from mhfem_amd import problems, capi
+from mhfem_amd import meshgen, problems, capi
 import pyoracle
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +59,29 @@ def _uneven_boxes(pr, seed=7):
     return dataclasses.replace(pr, points=pts)
 
 
+def _mobility(pr, seed=9):
+    """The upwinded-mobility model (do_mobility_upwind) with a random,
+    non-uniform m_iK and a boundary mobility different from it, so that the
+    choice of upwind cell changes the values.  Boundary faces get FixedValue,
+    FixedFlux or FixedFluxNoAdvection with fluxes away from zero for every
+    species: a zero flux (AdvectiveOutflow, a no-flow wall) leaves the
+    any-species inflow test (Solver.hpp:680-690) to rounding."""
+    rng = np.random.default_rng(seed)
+    n, C, F = pr.n, pr.num_cells, pr.num_faces
+    tags = np.array(pr.tags, copy=True)
+    vals = np.zeros(n * F)
+    dirv = np.zeros(n * F)
+    kinds = [problems.FIXED_VALUE, problems.FIXED_FLUX, problems.FIXED_FLUX_NO_ADVECTION]
+    for i in range(n):
+        for q, E in enumerate(np.nonzero(pr.face_cells[:, 1] < 0)[0]):
+            tags[i * F + E] = kinds[(q + i) % 3]
+            vals[i * F + E] = rng.choice([-1.0, 1.0]) * rng.uniform(0.2, 1.0)
+            dirv[i * F + E] = rng.uniform(-1, 1)
+    co = dict(pr.coeffs, m_iK=rng.uniform(0.5, 2.0, n * C))
+    return dataclasses.replace(pr, coeffs=co, tags=tags, bc_values=vals, bc_dirichlet=dirv, do_mobility_upwind=1,
+                               bc_mobility=rng.uniform(0.5, 2.0, n * F), Z_iK0=rng.uniform(0, 1, n * C))
+
+
 CASES = {
     "heat2d": lambda: problems.heat_2d(s=12, steps=3),
     "heat2d_lumped": lambda: problems.heat_2d(s=12, steps=3, lumping=1),
@@ -94,6 +117,13 @@ CASES = {
     "hex_adr_n2_implicit_upwind": lambda: dataclasses.replace(_uneven_boxes(problems.adr_hexes(s=3, n=2)),
                                                               advection=problems.IMPLICIT_UPWIND),
     "hex_adr_n3_mixed_tags": lambda: _mixed_tags(_uneven_boxes(problems.adr_hexes(s=3, n=3))),
+    # non-uniform mobility with mobility upwinding (Solver.hpp:663-727)
+    "adr3d_n2_mobility": lambda: _mobility(problems.adr_3d(s=4, steps=3, preconditioner="diagonal")),
+    "darcy3d_mobility": lambda: _mobility(problems.darcy_3d(s=5, steps=3, preconditioner="diagonal")),
+    "adr2d_n3_mobility": lambda: _mobility(problems.adr_2d(s=6, n=3, steps=3)),
+    "hex_adr_n2_mobility": lambda: _mobility(_uneven_boxes(problems.adr_hexes(s=3, n=2, steps=3))),
+    "quad_adr_n2_mobility_implicit_upwind": lambda: dataclasses.replace(
+        _mobility(_uneven_boxes(problems.adr_quads(s=6, n=2, steps=3))), advection=problems.IMPLICIT_UPWIND),
 }
 
 
@@ -167,6 +197,39 @@ def test_time_steps(case):
         assert eF < 1e-10 and eK < 1e-10, (k, eF, eK)
         if pr.do_mobility_upwind:
             assert rel(s.get_array("v_iKe"), o.array("v_iKe")) < 1e-9
+            # the upwind choices of this step's preIterate: copies of stored
+            # mobilities or means of two, so identical
+            assert np.array_equal(s.get_array("m_iE_upw"), o.array("m_iE_upw")), k
+
+
+def test_mobility_upwind_of_assigned_fluxes():
+    """K4 on the GPU from fluxes set directly (the oracle's pin of the same
+    rule): species 0 and 1 carry the face fluxes of two constant fields of
+    different directions, species 2 none; the HIP k_upwind and the oracle
+    take identical decisions on every face."""
+    pr = _mobility(problems.adr_3d(s=3, steps=1, preconditioner="diagonal", n=3))
+    n, C, F = pr.n, pr.num_cells, pr.num_faces
+    fv = meshgen.outward_face_vectors(pr.points, pr.cells)  # (C, fpc, 3): |E| n
+    u = np.array([[1.0, 0.37, 0.21], [-0.3, 0.8, -0.45], [0.0, 0.0, 0.0]])
+    v = np.einsum("id,ked->iek", u, fv).ravel()
+    s = capi.Solver(pr, tol=1e-13)
+    o = pyoracle.Oracle(pr)
+    s.set_array("v_iKe", v)
+    o.array("v_iKe")[:] = v
+    s.preIterate(0.0, pr.dt)
+    o.pre_iterate(0.0, pr.dt)
+    got, ref = s.get_array("m_iE_upw"), o.array("m_iE_upw")
+    assert np.array_equal(got, ref)
+    # the data exercises both upwind cells, the tie and boundary inflow
+    m = pr.coefficient_array("m_iK").reshape(n, C)
+    inner = pr.face_cells[:, 1] >= 0
+    K1, K2 = pr.face_cells[inner, 0], pr.face_cells[inner, 1]
+    g = got.reshape(F, n)[inner]
+    assert np.any(g[:, 0] == m[0, K1]) and np.any(g[:, 0] == m[0, K2])
+    assert np.allclose(g[:, 2], 0.5 * (m[2, K1] + m[2, K2]))
+    bnd = ~inner
+    assert np.any(got.reshape(F, n)[bnd] == pr.bc_mobility.reshape(n, F).T[bnd])
+    s.close()
 
 
 @pytest.mark.parametrize("solver,prec", [("bicgstab", "diagonal"), ("bicgstab", "none"), ("bicgstab", "ilu0"),
@@ -299,6 +362,22 @@ def test_ilut_refusals():
     s.close()
 
 
+def test_setup_refuses_combinations_that_cannot_run():
+    """Checked when the linear system is set up, before any level table or
+    preconditioner is built: SOR takes no preconditioner, batches take FP64
+    BiCGStab."""
+    pr = problems.heat_2d(s=6, steps=1)
+    for prec in ("ilu0", "amg", "ilu-isai"):
+        with pytest.raises(capi.MHFEMError, match="SOR uses no preconditioner"):
+            capi.Solver(pr, solver="sor", preconditioner=prec)
+    batch = problems.concatenate([problems.ensemble_member(4, 1, steps=1), problems.ensemble_member(5, 2, steps=1)])
+    for solver in ("gmres", "cg"):
+        with pytest.raises(capi.MHFEMError, match="batched systems"):
+            capi.Solver(batch, solver=solver)
+    with pytest.raises(capi.MHFEMError, match="batched systems"):
+        capi.Solver(batch, precision="fp32-reliable")
+
+
 def test_other_solvers_refuse_fp32():
     pr = problems.heat_2d(s=6, steps=1)
     for solver in ("cg", "tfqmr", "jacobi", "bicgstabl", "idrs", "sor"):
@@ -319,22 +398,60 @@ def test_ilu0_step_parity():
         assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10
 
 
-@pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2)])
-@pytest.mark.parametrize("prec", ["diagonal", "ilu0"])
-def test_fp32_refinement(prec, precision, inner):
-    """FP32 inner BiCGStab in FP64 refinement reaches the FP64 tolerance; the
-    error against the all-FP64 oracle stays at the level of the tolerance."""
-    pr = problems.adr_3d(s=5, steps=3, preconditioner=prec)
-    s = capi.Solver(pr, tol=1e-10, precision=precision, inner_tol=inner)
+@pytest.mark.parametrize("n,precision", [(1, "fp64"), (2, "fp64"), (3, "fp64"), (2, "fp32-reliable")])
+def test_ilu0_sweeps_on_many_chunks(n, precision):
+    """The single-launch ILU(0) triangular solves (one wavefront per chunk of
+    64 faces, waits on earlier chunks) on meshes of thousands of chunks, n =
+    1, 2, 3, FP64 and FP32: on the first step's system, BiCGStab + ILU(0) to
+    1e-10 from the same start takes the oracle's iteration count within +-2
+    and reaches its iterate; whole steps solved to 1e-13 stay within 1e-10
+    of the oracle."""
+    print(f"\n[ilu0 sweeps] n={n} {precision}", flush=True)
+    pr = problems.adr_3d(s={1: 20, 2: 16, 3: 12}[n], n=n, steps=2, preconditioner="ilu0")
+    if precision == "fp64":
+        a = capi.Solver(pr, tol=1e-10, check_every=1)
+        a.preIterate(0.0, pr.dt)
+        a.assembleLinearSystem(0.0, pr.dt)
+        rp, col, val, rhs = a.export_system()
+        x0 = a.get_array("Z_iF")
+        x, st = a.solve_rhs(rhs, x0)
+        xo, conv, it, _ = pyoracle.krylov(rp, col, val, rhs, x0, "bicgstab", "ilu0", 1e-10)
+        assert conv == 1 and st.converged
+        assert abs(st.iterations - it) <= 2, (st.iterations, it)
+        assert rel(x, xo) < 1e-7
+        a.close()
+    s = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=0.03)
     o = pyoracle.Oracle(pr)
     t = 0.0
     for k in range(pr.steps):
         st = s.step(t, pr.dt)
         o.step(t, pr.dt, tol=1e-13)
         t += pr.dt
-        assert st.converged and st.residue < 1e-10
+        assert st.converged and st.residue < 1e-13
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10, k
+        assert rel(s.get_array("Z_iK"), o.array("Z_iK")) < 1e-10, k
+    s.close()
+
+
+@pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2)])
+@pytest.mark.parametrize("prec", ["diagonal", "ilu0"])
+def test_fp32_refinement(prec, precision, inner):
+    """FP32 inner BiCGStab in FP64 refinement: the residual and the correction
+    are FP64, so solved as tightly as the oracle (1e-13) the mixed-precision
+    solution meets the same 1e-10 bar against the all-FP64 oracle as the
+    FP64 solvers do."""
+    pr = problems.adr_3d(s=5, steps=3, preconditioner=prec)
+    s = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=inner)
+    o = pyoracle.Oracle(pr)
+    t = 0.0
+    for k in range(pr.steps):
+        st = s.step(t, pr.dt)
+        o.step(t, pr.dt, tol=1e-13)
+        t += pr.dt
+        assert st.converged and st.residue < 1e-13
         assert st.outer_iterations >= 1
-        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-8
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10, k
+        assert rel(s.get_array("Z_iK"), o.array("Z_iK")) < 1e-10, k
 
 
 @pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2)])
@@ -343,8 +460,8 @@ def test_fp32_face_groups(precision, inner):
     translated residuals and corrections) reaches the FP64 tolerance and the
     oracle's solution, with iteration counts close to the natural order's."""
     pr = problems.adr_3d(s=5, steps=3, preconditioner="diagonal")
-    s = capi.Solver(pr, tol=1e-10, precision=precision, inner_tol=inner, fp32_face_groups=1)
-    nat = capi.Solver(pr, tol=1e-10, precision=precision, inner_tol=inner)
+    s = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=inner, fp32_face_groups=1)
+    nat = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=inner)
     o = pyoracle.Oracle(pr)
     t = 0.0
     for k in range(pr.steps):
@@ -352,8 +469,8 @@ def test_fp32_face_groups(precision, inner):
         sn = nat.step(t, pr.dt)
         o.step(t, pr.dt, tol=1e-13)
         t += pr.dt
-        assert st.converged and st.residue < 1e-10
-        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-8
+        assert st.converged and st.residue < 1e-13
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10
         assert abs(st.iterations - sn.iterations) <= max(3, 0.2 * sn.iterations), (st.iterations, sn.iterations)
 
 

@@ -367,10 +367,16 @@ def _mixed_reference(pr, o_pts, tau, Z_old, Z_F_old, m_upw, bc_t):
             r += 1
     assert r == N
     x = np.linalg.solve(A, b)
-    return x[nv + n * C:].reshape(n, F), x[nv:nv + n * C].reshape(n, C)
+    # the fluxes v_iKe in the reference's (i, e, K) layout
+    v = x[:nv].reshape(n, C, fpc).transpose(0, 2, 1)
+    return x[nv + n * C:].reshape(n, F), x[nv:nv + n * C].reshape(n, C), v
 
 
-def _tiny_problem(kind, seed):
+def _tiny_problem(kind, seed, mobility=False):
+    """A few-cell problem of every coefficient kind with random data and all
+    four boundary types.  ``mobility``: the upwinded mobility model
+    (do_mobility_upwind) with a random, non-uniform m_iK and boundary
+    mobility, otherwise the constant-mobility model (m_iE_upw = 1)."""
     rng = np.random.default_rng(seed)
     if kind == "tri_n1":
         pr = problems.heat_2d(s=3, steps=2)
@@ -424,6 +430,10 @@ def _tiny_problem(kind, seed):
             vals[i * F + E] = rng.uniform(-1, 1)
             dirv[i * F + E] = rng.uniform(-1, 1)
     z0 = rng.uniform(0, 1, n * C)
+    if mobility:
+        co["m_iK"] = rng.uniform(0.5, 2.0, n * C)
+        return dataclasses.replace(pr, coeffs=co, tags=tags, bc_values=vals, bc_dirichlet=dirv, Z_iK0=z0,
+                                   do_mobility_upwind=1, bc_mobility=rng.uniform(0.5, 2.0, n * F))
     return dataclasses.replace(pr, coeffs=co, tags=tags, bc_values=vals, bc_dirichlet=dirv, Z_iK0=z0,
                                do_mobility_upwind=0, bc_mobility=np.ones(n * F))
 
@@ -441,12 +451,215 @@ def test_brute_force_mixed_system(kind):
         ZF_old = o.array("Z_iF").reshape(F, n).T.copy()
         m_upw = o.array("m_iE_upw").reshape(F, n).copy()
         o.step(t, pr.dt, tol=1e-14)
-        ZF, ZK = _mixed_reference(pr, None, pr.dt, Z_old, ZF_old, m_upw, pr.bc_dirichlet)
+        ZF, ZK, _ = _mixed_reference(pr, None, pr.dt, Z_old, ZF_old, m_upw, pr.bc_dirichlet)
         got_F = o.array("Z_iF").reshape(F, n).T
         got_K = o.array("Z_iK").reshape(n, C)
         assert np.abs(got_F - ZF).max() / np.abs(ZF).max() < 1e-9, step
         assert np.abs(got_K - ZK).max() / np.abs(ZK).max() < 1e-9, step
         t += pr.dt
+
+
+def _face_average(pr, Z):
+    """K11 (Solver.hpp:319-339): a face's initial trace is the mean of its
+    cells' values (the one cell on the boundary)."""
+    K0, K1 = pr.face_cells[:, 0], pr.face_cells[:, 1]
+    return np.where(K1 < 0, Z[:, K0], 0.5 * (Z[:, K0] + Z[:, np.maximum(K1, 0)]))
+
+
+def _upwind_mobility(pr, v, m, bc_mob):
+    """Solver.hpp:663-727 applied to fluxes v[i, e, K]: on an interior face
+    the mobility of the cell the flux of species i leaves (vel = v_iK1e1 -
+    v_iK2e2 > 0: K1 = cellIndexes[0], < 0: K2, == 0: the mean); on a boundary
+    face the boundary mobility when any species flows in, else the cell's.
+    Returns (m_iE_upw as [E, i], mask of decisions taken within rounding of a
+    tie, which a test cannot hold to one side)."""
+    n, C, F = pr.n, pr.num_cells, pr.num_faces
+    cf, fc = pr.cell_faces, pr.face_cells
+    m = m.reshape(n, C)
+    bcm = np.asarray(bc_mob).reshape(n, F)
+    eps = 1e-9 * max(np.abs(v).max(), 1e-300)
+    out = np.empty((F, n))
+    near = np.zeros((F, n), bool)
+    for E in range(F):
+        K1, K2 = fc[E]
+        e1 = list(cf[K1]).index(E)
+        for i in range(n):
+            if K2 < 0:
+                vj = v[:, e1, K1]
+                out[E, i] = bcm[i, E] if np.any(vj < 0) else m[i, K1]
+                near[E, i] = bool(np.any((np.abs(vj) < eps) & (vj != 0)))
+            else:
+                e2 = list(cf[K2]).index(E)
+                vel = v[i, e1, K1] - v[i, e2, K2]
+                out[E, i] = m[i, K1] if vel > 0 else m[i, K2] if vel < 0 else 0.5 * (m[i, K1] + m[i, K2])
+                near[E, i] = vel != 0 and abs(vel) < eps
+    return out, near
+
+
+@pytest.mark.parametrize("kind", ["tri_n1", "tet_n1", "adr_n2", "adr_n3", "tri_adr_n2", "quad_n2",
+                                  "edge_n2", "hex_n2", "hex_lumped_n2"])
+def test_brute_force_fluxes_and_mobility_upwind(kind):
+    """K4 and K10 against the dense mixed system, over three steps of a
+    problem with a non-uniform mobility m_iK, a boundary mobility different
+    from it and random boundary data: the brute-force chain starts from the
+    initial cell values and the face averages, takes its mobility upwind from
+    its own fluxes of the step before (zero before the first step), and
+    solves for the fluxes v_iKe together with the traces.  The oracle's
+    m_iE_upw must equal the brute-force choice, and its velocities and
+    traces the brute-force ones."""
+    pr = _tiny_problem(kind, 11, mobility=True)
+    # boundary types whose fluxes are away from zero for every species, so
+    # that the any-species inflow test is decided by the data, not by rounding
+    # (AdvectiveOutflow rows carry a zero diffusive flux)
+    tags = np.array(pr.tags, copy=True)
+    vals = np.array(pr.bc_values, copy=True)
+    rng = np.random.default_rng(12)
+    ao = tags == problems.ADVECTIVE_OUTFLOW
+    tags[ao] = problems.FIXED_FLUX
+    flux = tags != problems.FIXED_VALUE
+    vals[flux] = rng.choice([-1.0, 1.0], flux.sum()) * rng.uniform(0.2, 1.0, flux.sum())
+    pr = dataclasses.replace(pr, tags=tags, bc_values=vals)
+    o = pyoracle.Oracle(pr)
+    n, C, F, fpc = pr.n, pr.num_cells, pr.num_faces, pr.fpc
+    m = pr.coefficient_array("m_iK").reshape(n, C)
+    Z_old = np.asarray(pr.Z_iK0).reshape(n, C)
+    ZF_old = _face_average(pr, Z_old)
+    assert np.abs(o.array("Z_iF").reshape(F, n).T - ZF_old).max() < 1e-15
+    v_prev = np.zeros((n, fpc, C))
+    bnd = pr.face_cells[:, 1] < 0
+    seen = {"K1": 0, "K2": 0, "tie": 0, "inflow_by_other": 0, "inflow": 0}
+    t = 0.0
+    for step in range(3):
+        m_upw, near = _upwind_mobility(pr, v_prev, m, pr.bc_mobility)
+        assert not near.any(), step
+        o.pre_iterate(t, pr.dt)
+        got = o.array("m_iE_upw").reshape(F, n)
+        assert np.array_equal(got, m_upw), (step, np.argwhere(got != m_upw)[:5])
+        for E in range(F):
+            K1, K2 = pr.face_cells[E]
+            for i in range(n):
+                if K2 >= 0:
+                    seen["K1" if m_upw[E, i] == m[i, K1] != m[i, K2] else "K2" if m_upw[E, i] == m[i, K2] != m[i, K1]
+                         else "tie"] += 1
+                else:
+                    e1 = list(pr.cell_faces[K1]).index(E)
+                    if np.any(v_prev[:, e1, K1] < 0):
+                        seen["inflow"] += 1
+                        seen["inflow_by_other"] += int(v_prev[i, e1, K1] >= 0)
+        o.assemble(t, pr.dt)
+        o.solve(tol=1e-14)
+        o.post_iterate(t, pr.dt)
+        ZF, ZK, v = _mixed_reference(pr, None, pr.dt, Z_old, ZF_old, m_upw, pr.bc_dirichlet)
+        got_F = o.array("Z_iF").reshape(F, n).T
+        got_K = o.array("Z_iK").reshape(n, C)
+        got_v = o.array("v_iKe").reshape(n, fpc, C)
+        assert np.abs(got_F - ZF).max() / np.abs(ZF).max() < 1e-9, step
+        assert np.abs(got_K - ZK).max() / np.abs(ZK).max() < 1e-9, step
+        assert np.abs(got_v - v).max() / np.abs(v).max() < 1e-9, step
+        v_prev, Z_old, ZF_old = v, ZK, ZF
+        t += pr.dt
+    # every branch of the rule was taken: both upwind cells, the tie of the
+    # first step, boundary inflow, and (with several species) inflow of
+    # species i decided by another species' flux
+    assert seen["K1"] > 0 and seen["K2"] > 0 and seen["tie"] > 0 and seen["inflow"] > 0, seen
+    if n > 1 and pr.dim > 1:
+        assert seen["inflow_by_other"] > 0, seen
+
+
+def _area_normals(pr):
+    """|E_e| n_{K,e} of every local face of a simplex, from the vertices:
+    face e is opposite vertex e, and the normal points away from it."""
+    P = pr.points[pr.cells]  # (C, d+1, d)
+    C, nv, d = P.shape
+    out = np.zeros((C, nv, d))
+    for e in range(nv):
+        others = [q for q in range(nv) if q != e]
+        a = P[:, others[0]]
+        if d == 2:
+            t = P[:, others[1]] - a
+            nrm = np.stack([t[:, 1], -t[:, 0]], axis=1)
+        else:
+            nrm = 0.5 * np.cross(P[:, others[1]] - a, P[:, others[2]] - a)
+        flip = np.einsum("kd,kd->k", nrm, P[:, e] - a) > 0
+        nrm[flip] *= -1
+        out[:, e] = nrm
+    return out
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_velocities_and_upwind_of_a_linear_field(mesh):
+    """K10 and K4 on the linear patch test.  Steady diffusion (N = 0) with
+    Dirichlet data from Z = g.x on the whole boundary is reproduced exactly by
+    RTN0, and so is its constant velocity: the oracle's v_iKe must equal
+    -D g . n_{K,e} |E_e| with the normals computed here from the vertices.
+    Then, with a non-uniform mobility, the next step's m_iE_upw must take on
+    every interior face the mobility of the cell the flow -D g leaves, and on
+    the boundary the boundary mobility exactly where the flow enters."""
+    pr = problems.heat_2d(s=5) if mesh == "tri" else problems.darcy_3d(s=3, preconditioner="diagonal")
+    F, C = pr.num_faces, pr.num_cells
+    g = np.array([0.7, -0.45, 0.3])[:pr.dim]
+    D = 2.5
+    fcen = meshgen.face_centroids(pr.points, pr.cells, pr.cell_faces, F)
+    co = dict(pr.coeffs, N_ijK=np.array(0.0), D_ijK=np.array(D), m_iK=np.array(1.0))
+    rng = np.random.default_rng(4)
+    pr = dataclasses.replace(pr, coeffs=co, tags=np.full(F, problems.FIXED_VALUE, np.uint8),
+                             bc_values=np.zeros(F), bc_dirichlet=0.3 + fcen @ g, Z_iK0=np.zeros(C),
+                             do_mobility_upwind=1, bc_mobility=rng.uniform(0.5, 2.0, F))
+    o = pyoracle.Oracle(pr)
+    o.step(0.0, 1.0, tol=1e-14)
+    nrm = _area_normals(pr)  # (C, fpc, d)
+    expect = -D * np.einsum("ked,d->ek", nrm, g)
+    v = o.array("v_iKe").reshape(pr.fpc, C)
+    assert np.abs(v - expect).max() < 1e-11 * np.abs(expect).max()
+    # the mobility upwind of the next step follows the flow
+    m = rng.uniform(0.5, 2.0, C)
+    o.array("m_iK")[:] = m
+    o.pre_iterate(1.0, 1.0)
+    got = o.array("m_iE_upw")
+    flow = -D * g
+    for E in range(F):
+        K1, K2 = pr.face_cells[E]
+        e1 = list(pr.cell_faces[K1]).index(E)
+        out_of_K1 = flow @ nrm[K1, e1]
+        assert abs(out_of_K1) > 1e-3 * np.linalg.norm(nrm[K1, e1])  # no face lies along the flow
+        if K2 >= 0:
+            assert got[E] == (m[K1] if out_of_K1 > 0 else m[K2]), E
+        else:
+            assert got[E] == (pr.bc_mobility[E] if out_of_K1 < 0 else m[K1]), E
+
+
+def test_mobility_upwind_of_assigned_fluxes():
+    """K4 on fluxes set directly: species 0 and 1 carry the fluxes of two
+    constant fields of different directions (v = u_i . n |E|, normals from
+    the vertices), species 2 carries none.  Interior faces take the mobility
+    of the cell each species' field leaves, the still species the mean of the
+    two cells; a boundary face takes the boundary mobility for every species
+    as soon as one species' field enters there (Solver.hpp:680-690)."""
+    pr = _tiny_problem("adr_n3", 2, mobility=True)
+    n, C, F = pr.n, pr.num_cells, pr.num_faces
+    o = pyoracle.Oracle(pr)
+    nrm = _area_normals(pr)
+    u = np.array([[1.0, 0.37, 0.21], [-0.3, 0.8, -0.45], [0.0, 0.0, 0.0]])
+    v = np.einsum("id,ked->iek", u, nrm)
+    o.array("v_iKe")[:] = v.ravel()
+    o.pre_iterate(0.0, pr.dt)
+    got = o.array("m_iE_upw").reshape(F, n)
+    m = pr.coefficient_array("m_iK").reshape(n, C)
+    bcm = pr.bc_mobility.reshape(n, F)
+    other_decides = 0
+    for E in range(F):
+        K1, K2 = pr.face_cells[E]
+        e1 = list(pr.cell_faces[K1]).index(E)
+        out = u @ nrm[K1, e1]
+        for i in range(n):
+            if K2 >= 0:
+                ref = 0.5 * (m[i, K1] + m[i, K2]) if i == 2 else (m[i, K1] if out[i] > 0 else m[i, K2])
+            else:
+                inflow = np.any(out < 0)
+                ref = bcm[i, E] if inflow else m[i, K1]
+                other_decides += int(inflow and out[i] >= 0)
+            assert got[E, i] == ref, (E, i)
+    assert other_decides > 0
 
 
 # ---------------------------------------------------------------- invariants

@@ -372,6 +372,8 @@ SchemeView Ctx::view()
     s.bc_values = arr[MHFEM_BC_VALUES].p;
     s.bc_dirichlet = arr[MHFEM_BC_DIRICHLET].p;
     s.bc_mobility = arr[MHFEM_BC_MOBILITY].p;
+    s.bc_fn = bc_timed ? bc_fn.p : nullptr;
+    s.clock = clock.p;
     s.A = SellView<double>{F, slice_ptr.p, slice_width.p, sell_col.p, sell_val.p, diag_base};
     s.rhs = arr[MHFEM_RHS].p;
     return s;
@@ -480,6 +482,12 @@ mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell
         // default boundary conditions: zero Dirichlet (Solver.hpp:281-294)
         c->h_tags.assign((size_t)c->n * c->F, MHFEM_BC_FIXED_VALUE);
         upload(c->tags, c->h_tags.data(), c->h_tags.size());
+        // stationary boundary model (BoundaryModels/Stationary.h:8-38) until
+        // mhfem_set_boundary_time_function
+        for (TimeFn &f : c->h_bc_fn) f = TimeFn{-1, 0, 0.0, 0.0, 0.0, 0.0, nullptr, nullptr};
+        upload(c->bc_fn, c->h_bc_fn, 2 * (size_t)c->n);
+        c->clock.alloc(2);
+        MHFEM_HIP(hipMemset(c->clock.p, 0, 2 * sizeof(double)));
         MHFEM_HIP(hipDeviceSynchronize());
         // fixed geometry: M^{-1} of every cell, once (packed planes, and
         // row-expanded with FPC*FPC values rounded up to an even count per cell)
@@ -620,17 +628,21 @@ int mhfem_set_array(mhfem_ctx *ctx, int which, const double *host, long count)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_set_array: measures are derived from the mesh"};
         const long full = c.array_size(which);
         if (which == MHFEM_RHS && !c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
-        if (is_coefficient(which) && count == 1) {
-            c.arr[which].alloc(1);
-            c.uniform[which] = true;
-        } else if (count == full) {
-            if (c.arr[which].n != (size_t)full) c.arr[which].alloc(full);
-            c.uniform[which] = false;
-        } else {
+        const bool make_uniform = is_coefficient(which) && count == 1;
+        if (!make_uniform && count != full)
             throw Error{MHFEM_E_ARGUMENT, "mhfem_set_array: count must be the full size" +
                                               std::string(is_coefficient(which) ? " or 1" : "") + " (expected " +
                                               std::to_string(full) + ", got " + std::to_string(count) + ")"};
+        // An allocation of the right size is kept, so that kernels queued (or
+        // captured in step graphs) keep valid pointers; a new one is made only
+        // after the work queued on the old one has finished.  Step graphs are
+        // re-captured when a pointer or a uniform mask of the scheme changes.
+        const size_t need = make_uniform ? 1 : (size_t)full;
+        if (c.arr[which].n != need || !c.arr[which].p) {
+            MHFEM_HIP(hipStreamSynchronize(c.stream));
+            c.arr[which].alloc(need);
         }
+        if (is_coefficient(which)) c.uniform[which] = make_uniform;
         MHFEM_HIP(hipMemcpyAsync(c.arr[which].p, host, count * sizeof(double), hipMemcpyHostToDevice, c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
     });
@@ -694,6 +706,53 @@ int mhfem_set_boundary_conditions(mhfem_ctx *ctx, const uint8_t *tags, const dou
     });
 }
 
+int mhfem_set_boundary_time_function(mhfem_ctx *ctx, int which, int species, const mhfem_time_function *f)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (which != MHFEM_BC_DIRICHLET && which != MHFEM_BC_VALUES)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_set_boundary_time_function: which must be MHFEM_BC_DIRICHLET or "
+                                          "MHFEM_BC_VALUES"};
+        if (species < 0 || species >= c.n)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_set_boundary_time_function: species out of range"};
+        TimeFn t{-1, 0, 0.0, 0.0, 0.0, 0.0, nullptr, nullptr};
+        if (f) {
+            if (f->kind < MHFEM_TIME_CONSTANT || f->kind > MHFEM_TIME_TABLE)
+                throw Error{MHFEM_E_ARGUMENT, "mhfem_set_boundary_time_function: unknown kind"};
+            t.kind = f->kind;
+            t.offset = f->offset;
+            t.amplitude = f->amplitude;
+            t.omega = f->angular_frequency;
+            t.phase = f->phase;
+            if (f->kind == MHFEM_TIME_TABLE) {
+                if (f->points < 1 || !f->times || !f->values)
+                    throw Error{MHFEM_E_ARGUMENT, "mhfem_set_boundary_time_function: a table needs points >= 1"};
+                for (int k = 0; k < f->points; k++)
+                    if (!std::isfinite(f->times[k]) || !std::isfinite(f->values[k]) ||
+                        (k > 0 && !(f->times[k] > f->times[k - 1])))
+                        throw Error{MHFEM_E_ARGUMENT,
+                                    "mhfem_set_boundary_time_function: table times must be finite and ascending"};
+                t.points = f->points;
+            }
+        }
+        const int slot = (which == MHFEM_BC_DIRICHLET ? 0 : c.n) + species;
+        // the work queued so far may still read the old function and table
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+        if (t.kind == MHFEM_TIME_TABLE) {
+            std::vector<double> tab(f->times, f->times + f->points);
+            tab.insert(tab.end(), f->values, f->values + f->points);
+            upload(c.bc_tab[slot], tab.data(), tab.size());
+            t.times = c.bc_tab[slot].p;
+            t.values = c.bc_tab[slot].p + f->points;
+        } else {
+            c.bc_tab[slot].release();
+        }
+        c.h_bc_fn[slot] = t;
+        MHFEM_HIP(hipMemcpy(c.bc_fn.p, c.h_bc_fn, 2 * (size_t)c.n * sizeof(TimeFn), hipMemcpyHostToDevice));
+        c.bc_timed = false;
+        for (int q = 0; q < 2 * c.n; q++) c.bc_timed |= c.h_bc_fn[q].kind >= 0;
+    });
+}
+
 int mhfem_set_initial_condition(mhfem_ctx *ctx, int i, const double *Z_K, long count)
 {
     return guarded(ctx, [&](Ctx &c) {
@@ -726,8 +785,12 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         const char *pname = cfg->preconditioner == MHFEM_PREC_AMG    ? "AMG"
                             : cfg->preconditioner == MHFEM_PREC_ILUT ? "ILUT"
                                                                      : "ILU-ISAI";
-        if (cfg->preconditioner == MHFEM_PREC_ILUT && cfg->solver == MHFEM_SOLVER_SOR)
-            throw Error{MHFEM_E_ARGUMENT, "SOR uses no preconditioner: take none with it, not ILUT"};
+        if (cfg->solver == MHFEM_SOLVER_SOR && cfg->preconditioner != MHFEM_PREC_NONE &&
+            cfg->preconditioner != MHFEM_PREC_DIAGONAL)
+            throw Error{MHFEM_E_ARGUMENT, std::string("SOR uses no preconditioner: take none with it, not ") +
+                                              (cfg->preconditioner == MHFEM_PREC_ILUT ? "ILUT" : "this one")};
+        if (c.h_seg_faces.size() > 2 && (cfg->solver != MHFEM_SOLVER_BICGSTAB || cfg->precision != MHFEM_PRECISION_FP64))
+            throw Error{MHFEM_E_ARGUMENT, "batched systems are solved with FP64 BiCGStab"};
         if (reused && cfg->precision != MHFEM_PRECISION_FP64)
             throw Error{MHFEM_E_ARGUMENT, std::string("the ") + pname + " preconditioner works with the FP64 solvers"};
         if (reused && c.h_seg_faces.size() > 2)
@@ -861,6 +924,8 @@ int mhfem_setup_linear_system(mhfem_ctx *ctx, const mhfem_solver_config *cfg)
         MHFEM_HIP(hipMemset(c.sell_val.p, 0, std::max<int64_t>(c.sell_size, 1) * sizeof(double)));
         c.arr[MHFEM_RHS].alloc(c.rows);
         MHFEM_HIP(hipMemset(c.arr[MHFEM_RHS].p, 0, c.rows * sizeof(double)));
+        // the old solver's buffers may still be read by queued work
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
         krylov_destroy(c.krylov);
         c.krylov = nullptr;
         guess_destroy(c.guess);
@@ -894,6 +959,8 @@ int mhfem_assemble(mhfem_ctx *ctx, double time, double tau)
     return guarded(ctx, [&](Ctx &c) {
         if (!c.system_ready) throw Error{MHFEM_E_STATE, "mhfem_setup_linear_system must be called first"};
         phase_begin(c, 8);
+        // the boundary model sees time + tau (Solver.hpp:948)
+        launch_set_clock(c, time + tau);
         launch_assemble(c, time, tau);
         phase_end(c, 8, c.timers.assemble);
     });

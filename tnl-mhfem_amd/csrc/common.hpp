@@ -136,6 +136,36 @@ __device__ __forceinline__ void st(T *p, const Pack<T, M> &v)
     *reinterpret_cast<Pack<T, M> *>(p) = v;
 }
 
+// Time function of a boundary model (mhfem_time_function), as the device
+// reads it: a boundary value of species i is the stored value times the
+// function at the step's new time (Solver.hpp:948 passes time + tau).
+struct TimeFn {
+    int kind;    // MHFEM_TIME_*, or -1: stationary (factor 1)
+    int points;  // MHFEM_TIME_TABLE
+    double offset, amplitude, omega, phase;
+    const double *times, *values;  // MHFEM_TIME_TABLE, device
+};
+
+__device__ inline double time_factor(const TimeFn &f, double t)
+{
+    if (f.kind == MHFEM_TIME_CONSTANT) return f.offset;
+    if (f.kind == MHFEM_TIME_SINE) return f.offset + f.amplitude * sin(f.omega * t + f.phase);
+    if (f.kind == MHFEM_TIME_TABLE) {
+        const int last = f.points - 1;
+        if (!(t > f.times[0])) return f.values[0];
+        if (!(t < f.times[last])) return f.values[last];
+        int lo = 0, hi = last;  // times[lo] < t < times[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (f.times[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const double w = (t - f.times[lo]) / (f.times[hi] - f.times[lo]);
+        return f.values[lo] + w * (f.values[hi] - f.values[lo]);
+    }
+    return 1.0;
+}
+
 // All device pointers of the scheme, passed to kernels by value.
 struct SchemeView {
     int C, F, dim, fpc, S;
@@ -160,6 +190,10 @@ struct SchemeView {
     // boundary conditions
     const uint8_t *tags;
     const double *bc_values, *bc_dirichlet, *bc_mobility;
+    // boundary model: 2n time functions (Dirichlet of species i at [i],
+    // Neumann at [n + i]), nullptr for the stationary model
+    const TimeFn *bc_fn;
+    const double *clock;  // [0]: the step's new time (time + tau), device
     // linear system
     SellView<double> A;
     double *rhs;
@@ -217,6 +251,13 @@ struct Ctx {
     DBuf<uint8_t> tags;
     std::vector<uint8_t> h_tags;
     bool bc_set = false;
+    // boundary model (mhfem_set_boundary_time_function): time functions of
+    // the Dirichlet (slot i) and Neumann (slot n + i) values, their tables
+    TimeFn h_bc_fn[6] = {};
+    DBuf<TimeFn> bc_fn;
+    DBuf<double> bc_tab[6];
+    bool bc_timed = false;
+    DBuf<double> clock;  // the step's new time, written before each assembly
     // linear system (block SELL-64 storage + CSR pattern on the host)
     int rows = 0;
     long nnz = 0;
@@ -306,6 +347,8 @@ inline void kernel_timing_end(Ctx &c) { MHFEM_HIP(hipEventRecord(c.kt_ev[c.kt_us
 
 // scheme.hip
 void launch_face_average(Ctx &c);
+// the time the boundary model is evaluated at (clock[0] = t), on the stream
+void launch_set_clock(Ctx &c, double t);
 void launch_pre_iterate(Ctx &c, double time, double tau);
 void launch_assemble(Ctx &c, double time, double tau);
 void launch_post_iterate(Ctx &c);

@@ -755,11 +755,281 @@ __global__ void __launch_bounds__(KB) k_ilu_upper(const int *__restrict__ rows, 
     y[r] = s / lu[dg[r]];
 }
 
+// ---------------------------------------------------------------------------
+// ILU(0) triangular solves without level launches.  The level-scheduled
+// solve above needs one launch per dependency level (about 900 per solve on
+// a 64^3 Kuhn mesh at n = 2), and its cost is the GPU-side latency of those
+// dependent launches.  Here each triangular solve is one launch:
+// * faces are cut into chunks of ILU_WG consecutive faces, one work-group
+//   each, one lane per face; a lane solves its face's n rows in row order,
+//   each row's entries in CSR order: the arithmetic of the row-wise solve,
+//   so the results are the same to the last bit;
+// * a work-group takes its chunk by ticket (an atomic counter), so chunks
+//   start in solve order and every chunk a lane waits on has started before:
+//   the earliest unfinished chunk is resident and waits on finished ones;
+// * a value is ready when it no longer holds a sentinel (a signalling-NaN
+//   payload that arithmetic never produces): values of the chunk's own faces
+//   are read from LDS, those of other chunks from the output vector with
+//   device-coherent loads (agent-scope atomics), written by their producer
+//   with device-coherent stores.  The value is its own flag, so no fence is
+//   needed;
+// * each wavefront repeats rounds until its lanes are done.  A lane keeps
+//   its place (row, entry, partial sum) when a value is not ready yet and
+//   resumes there, so it reads each value once and a waiting lane polls one
+//   address per round; a round without progress backs off with s_sleep;
+// * every wait is bounded in time: past ILU_WAIT_TICKS of the real-time
+//   counter (100 MHz) since the wavefront started, or once another one has
+//   given up, the remaining rows are written as zero, an error flag is set
+//   and the segment is marked DONE, so every launch ends, the queued work of
+//   the solve turns into no-ops and the host reports the error.
+// The lower solve writes the intermediate vector z and resets the output y
+// to the sentinel; the upper solve reads z, writes y and resets z to the
+// sentinel for the next apply (each row of z is read only by its own lane).
+// Each solve also resets the other's ticket counter.
+#ifndef MHFEM_ILU_WG
+#define MHFEM_ILU_WG 512
+#endif
+constexpr int ILU_WG = MHFEM_ILU_WG;
+constexpr int ILU_BACKOFF_MAX = 7;  // s_sleep(4) rounds: about 0.9 us at most
+constexpr unsigned long long ILU_WAIT_TICKS = 20000000ull;  // 0.2 s at 100 MHz
+
+template <typename T>
+struct SentinelBits;
+template <>
+struct SentinelBits<double> {
+    using U = unsigned long long;
+    static constexpr U bits = 0x7FF4A5A5A5A5A5A5ull;
+};
+template <>
+struct SentinelBits<float> {
+    using U = unsigned int;
+    static constexpr U bits = 0x7FA5A5A5u;
+};
+template <typename T>
+__device__ __forceinline__ bool is_sentinel(T v)
+{
+    return __builtin_bit_cast(typename SentinelBits<T>::U, v) == SentinelBits<T>::bits;
+}
+template <typename T>
+__device__ __forceinline__ T sentinel_value()
+{
+    return __builtin_bit_cast(T, SentinelBits<T>::bits);
+}
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void coherent_store(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T lds_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void lds_store(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+struct IluSweep {
+    int faces, nchunks;
+    const int *rp, *col, *dg;
+    const int *order;  // chunks in start order: [0, nchunks) lower solve, [nchunks, 2 nchunks) upper
+    int *ticket;  // [0]: lower solve, [1]: upper solve
+    int *err;
+    double *sc;   // the scalar slots of every segment: a failed wait marks them DONE
+    int nseg;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(KB) k_ilu_prepare(long n, T *z, int *ticket, int *err)
+{
+    const T s = sentinel_value<T>();
+    for (long i = (long)blockIdx.x * KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) z[i] = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ticket[0] = 0;
+        ticket[1] = 0;
+        *err = 0;
+    }
+}
+
+// lower (UPPER = false: in = x, out = z, reset = y) or upper (UPPER = true:
+// in = z, out = y, reset = z) triangular solve.  sc: flags of the single
+// segment (the solve is skipped once it has halted; a wait that gives up
+// marks it DONE) or nullptr.
+template <typename T, int N, bool UPPER>
+__global__ void __launch_bounds__(ILU_WG) k_ilu_sweep(IluSweep a, const T *__restrict__ lu, const T *in, T *out,
+                                                      T *reset, const double *sc)
+{
+    if (sc && halted(sc)) return;
+    __shared__ T ys[ILU_WG * N];
+    __shared__ int s_ticket;
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        s_ticket = atomicAdd(&a.ticket[UPPER ? 1 : 0], 1);
+        // the other solve's counter is free now: reset it for its next launch
+        if (s_ticket == 0) a.ticket[UPPER ? 0 : 1] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) ys[lane * N + i] = sentinel_value<T>();
+    __syncthreads();
+    // tickets past the chunk count only follow a failed wait (the counter
+    // was not reset): nothing to do
+    if (s_ticket >= a.nchunks) return;
+    const int chunk = a.order[(UPPER ? a.nchunks : 0) + s_ticket];
+    const int f0 = chunk * ILU_WG, f1 = min(a.faces, f0 + ILU_WG);
+    const int E = f0 + lane;
+    // resumable state: rows finished, next entry of the current row (-1: the
+    // row is not started), its partial sum, the finished rows' values, and
+    // the entry it waits on (its column, -1: none, and its factor value)
+    int q = E < f1 ? 0 : N;
+    int k = -1;
+    T s = T(0);
+    T yi[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) yi[i] = T(0);
+    int pc = -1;
+    T pv = T(0);
+    // a launch that follows a failed wait ends at once
+    bool give_up = coherent_load(a.err) != 0;
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    int backoff = 0;
+    while (__any(q < N)) {
+        bool progress = false;
+        bool blocked = false;
+        if (q < N && !give_up && pc >= 0) {
+            // a waiting lane polls only the value it waits on
+            const int F = pc / N, j = pc - F * N;
+            const T v = (F >= f0 && F < f1) ? lds_load(&ys[(F - f0) * N + j]) : coherent_load(out + pc);
+            if (is_sentinel(v)) {
+                blocked = true;
+            } else {
+                s -= pv * v;
+                k++;
+                pc = -1;
+                progress = true;
+            }
+        }
+        while (q < N && !give_up && !blocked) {
+            const int i = UPPER ? N - 1 - q : q;
+            const int r = E * N + i;
+            if (k < 0) {
+                k = UPPER ? a.dg[r] + 1 : a.rp[r];
+                s = in[r];
+            }
+            const int kend = UPPER ? a.rp[r + 1] : a.dg[r];
+            // entries in batches: the indices, the factor values and then
+            // the needed values of a batch are each loaded together, and
+            // consumed in CSR order up to the first one not ready
+            while (k < kend) {
+                constexpr int B = 8;
+                int cols[B];
+                T vals[B], vs[B];
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const bool in_row = k + b < kend;
+                    cols[b] = in_row ? a.col[k + b] : -1;
+                    vals[b] = in_row ? lu[k + b] : T(0);
+                }
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const int cc = cols[b];
+                    T v = T(0);
+                    if (cc >= 0) {
+                        const int F = cc / N, j = cc - F * N;
+                        if (F == E) {
+                            v = yi[0];
+                            if (N > 1 && j == 1) v = yi[N > 1 ? 1 : 0];
+                            if (N > 2 && j == 2) v = yi[N > 2 ? 2 : 0];
+                        } else if (F >= f0 && F < f1) {
+                            v = lds_load(&ys[(F - f0) * N + j]);
+                        } else {
+                            v = coherent_load(out + cc);
+                        }
+                    }
+                    vs[b] = v;
+                }
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    if (cols[b] < 0 || blocked) continue;
+                    if (is_sentinel(vs[b])) {
+                        blocked = true;
+                        pc = cols[b];
+                        pv = vals[b];
+                        continue;
+                    }
+                    s -= vals[b] * vs[b];
+                    k++;
+                    progress = true;
+                }
+                if (blocked) break;
+            }
+            if (blocked) break;
+            T val = s;
+            if (UPPER) val = s / lu[a.dg[r]];
+            if (is_sentinel(val)) val = __builtin_nan("");
+            yi[i] = val;
+            lds_store(&ys[(E - f0) * N + i], val);
+            coherent_store(out + r, val);
+            q++;
+            k = -1;
+            progress = true;
+        }
+        if (__any(progress)) {
+            backoff = 0;
+            continue;
+        }
+        // nothing moved in this wavefront.  Waits on the chunk's own faces
+        // (LDS) spin on; waits on other chunks back off, so that waiting
+        // wavefronts do not flood the memory system; give up past the limit
+        const bool waits_outside = pc >= 0 && !(pc / N >= f0 && pc / N < f1);
+        if (__any(waits_outside)) {
+            for (int b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(4);
+            backoff = min(2 * backoff + 1, ILU_BACKOFF_MAX);
+        } else {
+            // a short pause leaves the issue slots to the wavefronts that work
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!give_up && (__builtin_amdgcn_s_memrealtime() - t0 > ILU_WAIT_TICKS || coherent_load(a.err) != 0))
+            give_up = true;
+        if (give_up) {
+            if (q < N) {
+                coherent_store(a.err, 1);
+                for (int g = 0; g < a.nseg; g++) a.sc[(long)g * NSLOTS + S_DONE] = 1.0;
+                // finish the rows with zeros so that nothing waits on them
+#pragma unroll
+                for (int t = 0; t < N; t++) {
+                    if (t < q) continue;
+                    const int i = UPPER ? N - 1 - t : t;
+                    lds_store(&ys[(E - f0) * N + i], T(0));
+                    coherent_store(out + E * N + i, T(0));
+                }
+                q = N;
+            }
+        }
+    }
+    if (E < f1) {
+#pragma unroll
+        for (int i = 0; i < N; i++) reset[E * N + i] = sentinel_value<T>();
+    }
+}
+
 struct Ilu0 {
     std::vector<int> h_lptr, h_uptr;  // level pointers
     DBuf<int> lrows, urows, diag;     // rows in level order, diagonal positions
     DBuf<double> lu;
     DBuf<float> lu32;
+    // single-launch triangular solves: intermediate vector (sentinel between
+    // applies), ticket counters and the error flag
+    DBuf<double> zmid;
+    DBuf<float> zmid32;
+    DBuf<int> sweep_state;  // [0], [1]: tickets, [2]: error flag
+    DBuf<int> chunk_order;  // start order of the chunks, lower then upper solve
     DBuf<double> a0, swp;  // ParILU: the matrix in CSR order, the other sweep buffer
     // ILUT: factors on their own pattern (rebuilt with the factors)
     bool own = false;
@@ -882,10 +1152,12 @@ struct Krylov {
     DBuf<double> ext;
     // mixed precision: solves left that go straight to FP64 after a fallback
     int fp64_cooldown = 0;
-    // captured step graphs
+    // captured step graphs, and the scheme arguments they were captured with
     hipGraphExec_t g_first = nullptr, g_more = nullptr, g_post = nullptr;
     double g_tau = 0;
     int g_iters = 0;
+    SchemeView g_view{};
+    hipStream_t g_stream = nullptr;
 };
 
 static inline unsigned grid_for(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + KB - 1) / KB, MAXG)); }
@@ -1240,6 +1512,41 @@ Krylov *krylov_create(Ctx &c)
         const std::vector<int> dg = build_levels(I, n, c.h_rowptr.data(), c.h_col.data());
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
+            // single-launch triangular solves (k_ilu_sweep)
+            I.zmid.alloc(n);
+            if (c.cfg.precision != MHFEM_PRECISION_FP64) I.zmid32.alloc(n);
+            I.sweep_state.alloc(3);
+            // chunks in an order that respects their dependencies and keeps
+            // the resident work-groups on the solve's wavefront: by the
+            // depth of the chunk in the chunk dependency graph, then by index
+            const int nch = (c.F + ILU_WG - 1) / ILU_WG;
+            std::vector<int> lvl(nch, 0), ord(2 * (size_t)nch);
+            for (int pass = 0; pass < 2; pass++) {
+                std::fill(lvl.begin(), lvl.end(), 0);
+                for (int t = 0; t < nch; t++) {
+                    const int ch = pass == 0 ? t : nch - 1 - t;
+                    int l = 0;
+                    for (int E = ch * ILU_WG; E < std::min(c.F, (ch + 1) * ILU_WG); E++)
+                        for (int i = 0; i < c.n; i++) {
+                            const int r = E * c.n + i;
+                            const int q0 = pass == 0 ? c.h_rowptr[r] : dg[r] + 1;
+                            const int q1 = pass == 0 ? dg[r] : c.h_rowptr[r + 1];
+                            for (int q = q0; q < q1; q++) {
+                                const int other = c.h_col[q] / c.n / ILU_WG;
+                                if (other != ch) l = std::max(l, lvl[other] + 1);
+                            }
+                        }
+                    lvl[ch] = l;
+                }
+                int *o = ord.data() + (size_t)pass * nch;
+                std::iota(o, o + nch, 0);
+                if (pass == 1) std::reverse(o, o + nch);
+                std::stable_sort(o, o + nch, [&](int x, int y) { return lvl[x] < lvl[y]; });
+            }
+            I.chunk_order.alloc(ord.size());
+            MHFEM_HIP(hipMemcpy(I.chunk_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
         if (is_isai(c.cfg.preconditioner)) isai_patterns(c, *k, dg);
         if (c.cfg.preconditioner == MHFEM_PREC_PARILU_ISAI || c.cfg.solver == MHFEM_SOLVER_SOR) {
             I.a0.alloc(c.nnz);
@@ -1261,8 +1568,12 @@ Krylov *krylov_create(Ctx &c)
     return k;
 }
 
+// Destroys the captured step graphs once the work queued on their stream
+// (replays included) has finished.
 static void drop_graphs(Krylov &k)
 {
+    if (!k.g_first && !k.g_more && !k.g_post) return;
+    if (k.g_stream) (void)hipStreamSynchronize(k.g_stream);
     for (hipGraphExec_t *g : {&k.g_first, &k.g_more, &k.g_post})
         if (*g) {
             (void)hipGraphExecDestroy(*g);
@@ -1426,6 +1737,23 @@ struct Solve {
         else lu = I.lu32.p;
         const double *flag = (skip_when_halted && k.nseg == 1) ? k.sc.p : nullptr;
         const int *rp = I.own ? I.frp.p : c.rowptr.p, *col = I.own ? I.fcol.p : c.csr_col.p;
+        if (!I.own && I.sweep_state.p) {
+            // ILU(0): one launch per triangular solve (k_ilu_sweep)
+            T *z;
+            if constexpr (sizeof(T) == 8) z = I.zmid.p;
+            else z = I.zmid32.p;
+            const int nchunks = (c.F + ILU_WG - 1) / ILU_WG;
+            const IluSweep a{c.F, nchunks, rp, col, I.diag.p, I.chunk_order.p, I.sweep_state.p, I.sweep_state.p + 2,
+                             k.sc.p, k.nseg};
+            with_n(c.n, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                hipLaunchKernelGGL((k_ilu_sweep<T, N, false>), nchunks, ILU_WG, 0, c.stream, a, lu, x, z, y, flag);
+                hipLaunchKernelGGL((k_ilu_sweep<T, N, true>), nchunks, ILU_WG, 0, c.stream, a, lu, (const T *)z, y,
+                                   z, flag);
+            });
+            MHFEM_HIP(hipGetLastError());
+            return;
+        }
         for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
             const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
             hipLaunchKernelGGL((k_ilu_lower<T>), (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
@@ -2343,6 +2671,13 @@ static void precond_update(Ctx &c, Krylov &k)
         ilu_factor(c, k);
         if (mixed)
             hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, I.lu.p, I.lu32.p);
+        // the intermediate vectors start as sentinels, the tickets and the
+        // error flag at zero
+        hipLaunchKernelGGL((k_ilu_prepare<double>), grid_for(n), KB, 0, c.stream, (long)n, I.zmid.p,
+                           I.sweep_state.p, I.sweep_state.p + 2);
+        if (mixed)
+            hipLaunchKernelGGL((k_ilu_prepare<float>), grid_for(n), KB, 0, c.stream, (long)n, I.zmid32.p,
+                               I.sweep_state.p, I.sweep_state.p + 2);
         MHFEM_HIP(hipGetLastError());
     }
 }
@@ -2701,6 +3036,14 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
         out.residue = res;
         out.converged = res < tol;
     }
+    if (c.cfg.preconditioner == MHFEM_PREC_ILU0 && k.ilu.sweep_state.p) {
+        int err = 0;
+        MHFEM_HIP(hipMemcpyAsync(&err, k.ilu.sweep_state.p + 2, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+        if (err)
+            throw Error{MHFEM_E_STATE, "ILU(0) triangular solve: a wait for a dependency exceeded its limit (the "
+                                       "values of this solve are not valid)"};
+    }
     if (c.guess && c.guess_on_solve) guess_after_solve(c, *c.guess, x);
     if (k.amg) amg_after_solve(*k.amg, out.iterations);
     if (is_isai(c.cfg.preconditioner)) {
@@ -2748,7 +3091,14 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
         throw Error{MHFEM_E_ARGUMENT, "graph steps do not take the AMG, ILU-ISAI or ILUT preconditioners (their set-up "
                                       "decisions run on the host); use the four calls"};
     const int every = std::max(1, c.cfg.check_every);
-    if (k.g_first && (k.g_tau != tau || k.g_iters != every)) drop_graphs(k);
+    // The graphs hold the kernel arguments of their capture: tau, the batch
+    // size and every device pointer and uniform mask of the scheme.  Any
+    // change (a coefficient set uniform or full, a reallocated array, a
+    // boundary model switched on) re-captures them.  SchemeView is built
+    // zero-initialised, so its padding compares equal.
+    const SchemeView view = c.view();
+    if (k.g_first && (k.g_tau != tau || k.g_iters != every || std::memcmp(&k.g_view, &view, sizeof view) != 0))
+        drop_graphs(k);
     Solve S(c, k);
     Solve::Vecs v{k.r.p, k.r_ast.p, k.p.p, k.Ap.p, k.s.p, k.As.p, k.tmp.p, k.tmp2.p, k.dinv.p};
     const double *b = c.arr[MHFEM_RHS].p;
@@ -2776,7 +3126,12 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
         capture([&]() { launch_post_iterate(c); }, &k.g_post);
         k.g_tau = tau;
         k.g_iters = every;
+        k.g_view = view;
+        k.g_stream = c.stream;
     }
+    // the boundary model's time is read from device memory by the captured
+    // assembly, so each replay sees its own step's time + tau
+    launch_set_clock(c, time + tau);
     MHFEM_HIP(hipGraphLaunch(k.g_first, c.stream));
     const int it = S.bicg_drive<double>(b, x, v, [&]() { MHFEM_HIP(hipGraphLaunch(k.g_more, c.stream)); });
     MHFEM_HIP(hipGraphLaunch(k.g_post, c.stream));

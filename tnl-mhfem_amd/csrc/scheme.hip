@@ -810,7 +810,10 @@ __device__ __forceinline__ void boundary_row(const SchemeView &s, int q, int i)
     const long bi = (long)i * s.F + E;
     const int tag = s.tags[bi];
     if (tag == MHFEM_BC_FIXED_VALUE) {
-        s.rhs[(long)E * N + i] = s.bc_dirichlet[bi];
+        // BoundaryModel::getDirichletValue at time + tau (BoundaryConditions.hpp:379)
+        double value = s.bc_dirichlet[bi];
+        if (s.bc_fn) value *= time_factor(s.bc_fn[i], s.clock[0]);
+        s.rhs[(long)E * N + i] = value;
         return;
     }
     const int K0 = s.face_cells[2 * E];
@@ -845,7 +848,10 @@ __device__ __forceinline__ void boundary_row(const SchemeView &s, int q, int i)
         bval = 0;
         bval += inflow ? s.Z_iK[(long)i * C + K0] : r.rhs;
     } else {
-        bval = -s.bc_values[bi] * s.face_meas[E];
+        // BoundaryModel::getNeumannValue at time + tau (BoundaryConditions.hpp:414, 455)
+        double J = s.bc_values[bi];
+        if (s.bc_fn) J *= time_factor(s.bc_fn[N + i], s.clock[0]);
+        bval = -J * s.face_meas[E];
         bval += r.rhs;
     }
     s.rhs[(long)E * N + i] = bval / d;
@@ -987,6 +993,11 @@ __global__ void __launch_bounds__(WG) k_post(SchemeView s)
     }
 }
 
+__global__ void k_set_clock(double *clock, double t)
+{
+    if (threadIdx.x == 0) clock[0] = t;
+}
+
 __global__ void k_fill(double *p, size_t n, double v)
 {
     const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
@@ -1085,6 +1096,12 @@ void launch_post_iterate(Ctx &c)
 {
     SchemeView s = c.view();
     MHFEM_DISPATCH(c, k_post, blocks(c.C), s);
+}
+
+void launch_set_clock(Ctx &c, double t)
+{
+    hipLaunchKernelGGL(k_set_clock, 1, 64, 0, c.stream, c.clock.p, t);
+    MHFEM_HIP(hipGetLastError());
 }
 
 void launch_zero_double(Ctx &c, double *p, size_t n)

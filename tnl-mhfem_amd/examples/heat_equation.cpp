@@ -98,10 +98,11 @@ int main(int argc, char **argv)
     if (argc == 3 && std::string(argv[1]) == "--config") return run_config(argv[2]);
     const int s = argc > 2 ? std::atoi(argv[2]) : 64;
     const int steps = argc > 3 ? std::atoi(argv[3]) : 100;
+    const double residue = argc > 4 ? std::atof(argv[4]) : 1e-10;
     ParameterContainer p;
     p.setParameter("linear-solver", std::string("bicgstab"));
     p.setParameter("preconditioner", std::string("diagonal"));
-    p.setParameter("convergence-residue", 1e-10);
+    p.setParameter("convergence-residue", residue);
     p.setParameter("time-step", 1e-4);
     p.setParameter("initial-time", 0.0);
     p.setParameter("final-time", 1e-4 * steps);

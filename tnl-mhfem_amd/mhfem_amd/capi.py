@@ -58,6 +58,15 @@ class Timers(C.Structure):
                 ("spmv_ms", C.c_double), ("spmv_timed", C.c_longlong)]
 
 
+TIME_KINDS = {"constant": 0, "sine": 1, "table": 2}
+
+
+class TimeFunction(C.Structure):
+    _fields_ = [("kind", C.c_int), ("offset", C.c_double), ("amplitude", C.c_double),
+                ("angular_frequency", C.c_double), ("phase", C.c_double), ("points", C.c_int),
+                ("times", C.c_void_p), ("values", C.c_void_p)]
+
+
 GUESS_COMBINE = {"combined": 0, "network": 1, "history": 2}
 
 
@@ -89,6 +98,7 @@ SIGNATURES = {
     "mhfem_stream": (_P, [_P]),
     "mhfem_set_boundary_conditions": (_I, [_P, _P, _P, _P]),
     "mhfem_set_initial_condition": (_I, [_P, _I, _P, _L]),
+    "mhfem_set_boundary_time_function": (_I, [_P, _I, _I, C.POINTER(TimeFunction)]),
     "mhfem_init": (_I, [_P]),
     "mhfem_setup_linear_system": (_I, [_P, C.POINTER(SolverConfig)]),
     "mhfem_pre_iterate": (_I, [_P, _D, _D]),
@@ -366,11 +376,32 @@ class Solver:
             self.set_array(name, v.reshape(1) if v.ndim == 0 else v)
         if pr.bc_mobility is not None:
             self.set_array("bc_mobility", pr.bc_mobility)
+        for (which, i), spec in (pr.boundary_functions or {}).items():
+            self.set_boundary_time_function(which, i, spec)
         z = np.ascontiguousarray(pr.Z_iK0, np.float64).reshape(self.n, -1)
         for i in range(self.n):
             zi = np.ascontiguousarray(z[i])
             self._check(lib().mhfem_set_initial_condition(self.ctx, i, _ptr(zi), zi.size))
         self._check(lib().mhfem_init(self.ctx))
+
+    def set_boundary_time_function(self, which, species, spec):
+        """The boundary model's time function of the Dirichlet ("dirichlet") or
+        Neumann ("neumann") values of one species (mhfem_set_boundary_time_function);
+        spec None restores the stationary values."""
+        w = ARRAY_ID["bc_dirichlet"] if which == "dirichlet" else ARRAY_ID["bc_values"]
+        if spec is None:
+            self._check(lib().mhfem_set_boundary_time_function(self.ctx, w, species, None))
+            return
+        tt = np.ascontiguousarray(spec.get("times", []), np.float64)
+        vv = np.ascontiguousarray(spec.get("values", []), np.float64)
+        f = TimeFunction(TIME_KINDS[spec["kind"]], spec.get("offset", 0.0), spec.get("amplitude", 0.0),
+                         spec.get("angular_frequency", 0.0), spec.get("phase", 0.0), int(tt.size),
+                         tt.ctypes.data if tt.size else None, vv.ctypes.data if vv.size else None)
+        self._check(lib().mhfem_set_boundary_time_function(self.ctx, w, species, C.byref(f)))
+
+    def _host_boundary_values(self):
+        """a time-dependent boundary model the library does not evaluate itself"""
+        return self.problem.dirichlet_at is not None and not self.problem.boundary_functions
 
     def setupLinearSystem(self):
         self._check(lib().mhfem_setup_linear_system(self.ctx, C.byref(self.config)))
@@ -422,7 +453,7 @@ class Solver:
         self._check(lib().mhfem_pre_iterate(self.ctx, t, tau))
 
     def assembleLinearSystem(self, t, tau):
-        if self.problem.dirichlet_at is not None:
+        if self._host_boundary_values():
             self.set_array("bc_dirichlet", self.problem.dirichlet_at(t + tau))
         self._check(lib().mhfem_assemble(self.ctx, t, tau))
 
@@ -451,7 +482,7 @@ class Solver:
 
     def step_graph(self, t, tau):
         """one time step replayed from captured HIP graphs (mhfem_step)"""
-        if self.problem.dirichlet_at is not None:
+        if self._host_boundary_values():
             self.set_array("bc_dirichlet", self.problem.dirichlet_at(t + tau))
         st = SolveStats()
         self._check(lib().mhfem_step(self.ctx, t, tau, C.byref(st)))
@@ -482,7 +513,7 @@ class Solver:
 
     def solve(self, t0, t1, dt):
         """control.h::solve(problem, t0, t1, dt) (control.h:64-102)."""
-        if self.problem.dirichlet_at is None:
+        if not self._host_boundary_values():
             st = SolveStats()
             self._check(lib().mhfem_solve_interval(self.ctx, t0, t1, dt, C.byref(st)))
             return st

@@ -48,6 +48,11 @@ class Problem:
     face_cells: Optional[np.ndarray] = None
     # time-dependent Dirichlet values: f(time) -> array (i*F+E layout) or None
     dirichlet_at: Optional[Callable[[float], np.ndarray]] = None
+    # the same boundary model as device time functions (mhfem_time_function):
+    # {("dirichlet" | "neumann", species): dict(kind=..., ...)}; the stored
+    # values times f(time + tau).  When set, the library evaluates the model in
+    # the assembly; dirichlet_at then gives the same values on the host.
+    boundary_functions: dict = field(default_factory=dict)
     extra: dict = field(default_factory=dict)
 
     @property
@@ -212,22 +217,47 @@ def adr_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 50,
     fcen = meshgen.face_centroids(points, cells, cf, F)
     tags, dirichlet, x0 = _adr_boundary(n, fc, fcen)
     dirichlet_at = None
+    functions = {}
     if time_dependent:
+        # stored Z_1 = 1 at x = 0, times sin(2 pi t / 0.5)
         base = dirichlet.ravel().copy()
-        idx = np.nonzero(x0)[0]
-
-        def dirichlet_at(t, base=base, idx=idx):
-            out = base.copy()
-            out[idx] = np.sin(2 * np.pi * t / 0.5)
-            return out
-
-        dirichlet[0, x0] = 0.0
+        functions[("dirichlet", 0)] = dict(kind="sine", amplitude=1.0, angular_frequency=2 * np.pi / 0.5)
+        dirichlet_at = boundary_values_at(base, n, F, functions)
     return Problem("adr_3d", points, cells, n, coeffs, tags=tags.ravel(),
                    bc_values=np.zeros(n * F), bc_dirichlet=dirichlet.ravel(),
                    Z_iK0=np.zeros(n * C), dt=dt, steps=steps, solver="bicgstab",
                    preconditioner=preconditioner, do_mobility_upwind=1,
                    bc_mobility=np.ones(n * F), cell_faces=cf, face_cells=fc,
-                   dirichlet_at=dirichlet_at, extra=dict(seed=seed, s=s))
+                   dirichlet_at=dirichlet_at, boundary_functions=functions, extra=dict(seed=seed, s=s))
+
+
+def time_function(spec, t):
+    """A boundary time function (the library's mhfem_time_function) at t,
+    on the host."""
+    kind = spec["kind"]
+    if kind == "constant":
+        return spec.get("offset", 0.0)
+    if kind == "sine":
+        return spec.get("offset", 0.0) + spec.get("amplitude", 0.0) * np.sin(
+            spec.get("angular_frequency", 0.0) * t + spec.get("phase", 0.0))
+    if kind == "table":
+        return float(np.interp(t, spec["times"], spec["values"]))
+    raise ValueError(kind)
+
+
+def boundary_values_at(base, n, F, functions, which="dirichlet"):
+    """The values of a separable boundary model at time t as an array in the
+    i*F+E layout: the stored values of species i times its time function."""
+    base = np.asarray(base, np.float64).ravel()
+
+    def values_at(t):
+        out = base.copy()
+        for (w, i), spec in functions.items():
+            if w == which:
+                out[i * F:(i + 1) * F] *= time_function(spec, t)
+        return out
+
+    return values_at
 
 
 def adr_2d(s: int = 16, n: int = 2, dt: float = 1e-3, steps: int = 3,
