@@ -303,6 +303,21 @@ int mhfem_system_size(const mhfem_ctx *ctx, int *rows, long *nnz);
 int mhfem_export_system(mhfem_ctx *ctx, int32_t *rowptr, int32_t *col, double *val, double *rhs);
 /* solve an explicit right-hand side with the current matrix (testing aid) */
 int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats);
+/* mhfem_solve_rhs allocating nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  mhfem_solve_rhs
+ * stages rhs and x in device buffers it allocates on every call; this call
+ * stages them in a fixed device array of the library instead (2 *
+ * MHFEM_NOALLOC_MAX_ROWS doubles, 16 MiB, one per GPU the library runs on,
+ * held until the process ends).  Limits: at most MHFEM_NOALLOC_MAX_ROWS rows,
+ * and only configurations whose solve allocates nothing: BiCGStab with no,
+ * diagonal or ILU(0) preconditioning (any precision), learned guess off;
+ * anything else is refused with MHFEM_E_ARGUMENT.  Every call on one GPU
+ * uses the same array, which keeps the last call's contents: calls that
+ * overlap on one GPU (several host threads or contexts) race, and the caller
+ * must serialise them. */
+#define MHFEM_NOALLOC_MAX_ROWS 1048576
+int mhfem_solve_rhs_noalloc(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats);
 
 /* ---- learned initial guess of each solve.  No counterpart in the reference:
  * the reference starts every Krylov solve from the previous step's Z_iF

@@ -4,12 +4,19 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <unordered_map>
 
 #include "common.hpp"
 
 using namespace mhfem;
+
+// Staging of mhfem_solve_rhs_noalloc: the right-hand side and the iterate of
+// one call, in a fixed device array of the library instead of buffers
+// allocated per call.  Its name is a host-side stand-in: host code uses only
+// the device address hipGetSymbolAddress returns (one copy per GPU).
+static __device__ __align__(256) double scratch[2 * MHFEM_NOALLOC_MAX_ROWS];
 
 namespace {
 
@@ -381,6 +388,34 @@ SchemeView Ctx::view()
 
 }  // namespace mhfem
 
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on `device`: looked up on the first call
+// on that device, with that device current, then kept
+double *scratch_on(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw Error{MHFEM_E_ARGUMENT, "device index out of range"};
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        int current = 0;
+        MHFEM_HIP(hipGetDevice(&current));
+        if (current != device) MHFEM_HIP(hipSetDevice(device));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (current != device) (void)hipSetDevice(current);
+        if (e != hipSuccess)
+            throw Error{MHFEM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch[device] = static_cast<double *>(ptr);
+    }
+    return g_scratch[device];
+}
+
+}  // namespace
+
 extern "C" {
 
 mhfem_ctx *mhfem_create(const mhfem_mesh_desc *mesh, const mhfem_model_desc *model)
@@ -422,6 +457,7 @@ mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell
         c->lumping = model->mass_lumping ? 1 : 0;
         c->advection = model->advection;
         c->do_mob = model->do_mobility_upwind ? 1 : 0;
+        MHFEM_HIP(hipGetDevice(&c->device));
         MHFEM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) MHFEM_HIP(hipEventCreate(&e));
         build_topology(*c, mesh->cell_vertices);
@@ -1050,6 +1086,37 @@ int mhfem_solve_rhs(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_st
         mhfem_solve_stats st{};
         krylov_solve(c, b.p, xx.p, &st);
         MHFEM_HIP(hipMemcpyAsync(x, xx.p, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        MHFEM_HIP(hipStreamSynchronize(c.stream));
+        if (stats) *stats = st;
+    });
+}
+
+int mhfem_solve_rhs_noalloc(mhfem_ctx *ctx, const double *rhs, double *x, mhfem_solve_stats *stats)
+{
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.system_ready) throw Error{MHFEM_E_STATE, "linear system not set up"};
+        if (!rhs || !x) throw Error{MHFEM_E_ARGUMENT, "null argument"};
+        if ((long)c.rows > (long)MHFEM_NOALLOC_MAX_ROWS)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_solve_rhs_noalloc: the system has " + std::to_string(c.rows) +
+                                              " rows; the staging array holds systems of at most " +
+                                              std::to_string((long)MHFEM_NOALLOC_MAX_ROWS) + " rows"};
+        // only configurations whose solve allocates nothing: the BiCGStab
+        // vectors, the FP32 copies and the ILU(0) factors are allocated when
+        // the linear system is set up
+        const int p = c.cfg.preconditioner;
+        if (c.cfg.solver != MHFEM_SOLVER_BICGSTAB ||
+            (p != MHFEM_PREC_NONE && p != MHFEM_PREC_DIAGONAL && p != MHFEM_PREC_ILU0))
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_solve_rhs_noalloc: BiCGStab with no, diagonal or ILU(0) "
+                                          "preconditioning only"};
+        if (c.guess && c.guess_on_solve)
+            throw Error{MHFEM_E_ARGUMENT, "mhfem_solve_rhs_noalloc: the learned guess must be off"};
+        double *const base = scratch_on(c.device);
+        double *const b = base, *const xx = base + c.rows;
+        MHFEM_HIP(hipMemcpyAsync(b, rhs, c.rows * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        MHFEM_HIP(hipMemcpyAsync(xx, x, c.rows * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        mhfem_solve_stats st{};
+        krylov_solve(c, b, xx, &st);
+        MHFEM_HIP(hipMemcpyAsync(x, xx, c.rows * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         MHFEM_HIP(hipStreamSynchronize(c.stream));
         if (stats) *stats = st;
     });

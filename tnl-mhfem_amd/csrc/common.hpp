@@ -233,6 +233,7 @@ struct Ctx {
     int box = 0, nvc = 0;  // box cells (edges, rectangles) or simplices; vertices per cell
     int nv = 0, C = 0, F = 0, nbnd = 0;
     int lumping = 0, advection = 0, do_mob = 1;
+    int device = 0;  // the HIP device current at creation
     hipStream_t stream = nullptr;
     // host topology
     std::vector<int> h_cell_faces, h_face_cells;

@@ -109,6 +109,7 @@ SIGNATURES = {
     "mhfem_system_size": (_I, [_P, C.POINTER(_I), C.POINTER(_L)]),
     "mhfem_export_system": (_I, [_P, _P, _P, _P, _P]),
     "mhfem_solve_rhs": (_I, [_P, _P, _P, C.POINTER(SolveStats)]),
+    "mhfem_solve_rhs_noalloc": (_I, [_P, _P, _P, C.POINTER(SolveStats)]),
     "mhfem_benchmark_spmv": (_I, [_P, _I, _I, C.POINTER(_D)]),
     "mhfem_step": (_I, [_P, _D, _D, C.POINTER(SolveStats)]),
     "mhfem_autotune": (_I, [_P, C.POINTER(_I), C.POINTER(_D)]),
