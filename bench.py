@@ -12,19 +12,23 @@ preIterate + assembleLinearSystem + solveLinearSystem + postIterate
 workload's 50 steps: `warmup` untimed steps, then `steps` timed ones.
 
 Forms, each run from the same initial state on a fresh solver:
-  chosen     -- learned initial guess (per-face MLP trained online during the
-                first steps, combined with the last 16 increments by least
-                residual) + FP32 BiCGStab whose residual is replaced by the
-                FP64 one every 33x drop (FP64 iterative refinement with
-                reliable updates), Jacobi (the identity on the unit-diagonal
-                rows).  `value` is its steps/s.
+  chosen     -- starting vector = previous solution + the least-residual
+                combination of the last 16 increments, FP32 BiCGStab whose
+                residual is replaced by the FP64 one every 33x drop (FP64
+                iterative refinement with reliable updates), Jacobi (the
+                identity on the unit-diagonal rows).  `value` is its steps/s.
   plain      -- FP64 BiCGStab + Jacobi, previous solution as initial guess:
                 the fastest plain FP64 form on the GPU.
   plain_ilu0 -- FP64 BiCGStab + ILU(0), the reference's configured method
-                (bounded sample: its level-scheduled triangular solves make a
-                step take seconds).
-  chosen_without_network -- the chosen form with the network switched off
-                (previous increments only): what the trained network adds.
+                (bounded sample: its triangular solves make a step take
+                seconds).
+  config2_as_configured -- FP32 BiCGStab + ILU(0) in FP32 inside FP64
+                refinement (reliable updates), as configuration 2 names it.
+  config1    -- n = 1 Darcy on the same mesh and K field, FP64 BiCGStab +
+                ILU(0) (configuration 1), with the Jacobi form beside it.
+  with_network -- the chosen form plus the learned network's direction
+                (per-face MLP trained online on the first steps, switched off
+                by its rejection rule): what the trained network adds.
   fp64_history -- FP64 BiCGStab + Jacobi from the previous increments: what
                 the FP32 iteration adds, and (its distance to `plain`) the
                 spread between two FP64 solutions that both meet 1e-10.
@@ -41,13 +45,15 @@ form's iteration.  Every 8th launch during the chosen form's timed steps is
 bracketed by HIP events on the library's stream (timing level 2);
 achieved = the bytes the block-SELL product must move per launch (blocks of
 n*n values with one column index, the faces' own off-diagonal entries, x
-read once, y written once) / average launch duration.  The same product
+read once, y written once, and the operand of the dot product the kernel
+fuses, read once) / average launch duration.  The same product
 priced as an int32 CSR SpMV is reported beside it (csr_equivalent), and the
 plain FP64 form's SpMV the same way (plain_spmv_fp64).
 assembly: the same for assembleLinearSystem (one launch: interior and boundary rows),
 timed with HIP events around every assembly of the chosen form's timed steps.
-cpu_baseline: the C oracle (a restatement of the reference host path), FP64
-BiCGStab + ILU(0), on a bounded sample of one step of the same workload.
+cpu_baseline: the C oracle (a restatement of the reference host path), the
+first time step of the same workload, whole: FP64 BiCGStab + ILU(0) to 1e-10.
+Its solution is the all-FP64 reference of error_vs_oracle_first_step.
 """
 import argparse
 import ctypes as C
@@ -78,13 +84,16 @@ def spmv_bytes(rows, nnz, fp32):
     return 12 * nnz + 4 * (rows + 1) + 16 * rows
 
 
-def block_sell_bytes(n, faces, blocks_per_face, fp32):
+def block_sell_bytes(n, faces, blocks_per_face, fp32, dot_operand=False):
     """Bytes the block-SELL SpMV must move for the same product: per stored
     off-diagonal block n*n values and one column index, per face the n(n-1)
-    off-diagonal entries of its own block, x read once, y written once."""
+    off-diagonal entries of its own block, x read once, y written once; with
+    `dot_operand`, the vector the fused dot product reads (the iteration's
+    SpMV launches fuse (A x, z) into the product and read z once more)."""
     w = 4 if fp32 else 8
     rows = n * faces
-    return blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
+    return (blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
+            + (rows * w if dot_operand else 0))
 
 
 def assembly_bytes(n, fpc, cells, faces, nnz):
@@ -117,9 +126,12 @@ def run_form(pr, form, steps, warmup, timing=0):
     if form.get("guess"):
         s.configure_guess(True, **form["guess"])
     t = 0.0
-    for _ in range(warmup):
+    for w in range(warmup):
         s.step(t, pr.dt)
         t += pr.dt
+        if w == 0:
+            # the solution after the first step, for the comparison with the oracle's first step
+            form["z_first_step"] = s.get_array("Z_iF").copy()
     s.synchronize()
     if form.get("autotune"):
         form["rows_per_block"] = s.autotune()[0]
@@ -276,18 +288,39 @@ def study96(size, train_seeds, train_steps, eval_seed, eval_steps, history_depth
     return out
 
 
-def cpu_baseline(pr, iters_per_step, sample_iters):
-    """Oracle step time = preIterate + assembly + ILU(0) update +
-    iterations_per_step x (time of one FP64 BiCGStab+ILU(0) iteration) +
-    postIterate, the iteration time measured over `sample_iters` iterations."""
+def cpu_baseline(pr, sample_iters):
+    """The first time step of the workload on the C oracle, whole and timed:
+    preIterate, assembly, FP64 BiCGStab + ILU(0) to 1e-10 (the reference's
+    configured method; the ILU(0) factorization and triangular solves are
+    sequential), postIterate, on the host's cores.  Returns the record and
+    the oracle's Z_iF after the step (the all-FP64 reference the GPU forms'
+    first steps are compared with).  A one-thread figure is extrapolated from
+    `sample_iters` iterations of the same step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
+    # the GPU box's host share for one GPU is 16 cores
     threads = min(16, len(os.sched_getaffinity(0)))
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    o = pyoracle.Oracle(pr)
     L = pyoracle.lib()
+    L.or_set_threads(threads)
+    o = pyoracle.Oracle(pr)
     it, res = C.c_int(0), C.c_double(0)
+    t0 = time.perf_counter()
+    o.pre_iterate(0.0, pr.dt)
+    o.assemble(0.0, pr.dt)
+    t_pre_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    conv = L.or_solve(o.p, pyoracle.SOLVERS["bicgstab"], pyoracle.PRECONDITIONERS["ilu0"], 1e-10, 100000, 30,
+                      C.byref(it), C.byref(res))
+    t_solve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.post_iterate(0.0, pr.dt)
+    t_post = time.perf_counter() - t0
+    step_s = t_pre_asm + t_solve + t_post
+    z = o.array("Z_iF").copy()
+    iters = it.value
+    del o
 
     def solve(orc, maxit):
         t0 = time.perf_counter()
@@ -295,19 +328,7 @@ def cpu_baseline(pr, iters_per_step, sample_iters):
                    C.byref(it), C.byref(res))
         return time.perf_counter() - t0
 
-    t0 = time.perf_counter()
-    o.pre_iterate(0.0, pr.dt)
-    o.assemble(0.0, pr.dt)
-    t_pre_asm = time.perf_counter() - t0
-    t1 = solve(o, 1)                   # ILU(0) factorization + 1 iteration
-    t2 = solve(o, 1 + sample_iters)    # ILU(0) factorization + 1 + sample_iters iterations
-    t_it = (t2 - t1) / sample_iters
-    t_factor = max(0.0, t1 - t_it)
-    t0 = time.perf_counter()
-    o.post_iterate(0.0, pr.dt)
-    t_post = time.perf_counter() - t0
-    step_s = t_pre_asm + t_factor + iters_per_step * t_it + t_post
-    # the same first step on one thread (fewer sampled iterations)
+    # the same first step on one thread: the iteration time from a sample
     L.or_set_threads(1)
     o1 = pyoracle.Oracle(pr)
     t0 = time.perf_counter()
@@ -315,13 +336,13 @@ def cpu_baseline(pr, iters_per_step, sample_iters):
     o1.assemble(0.0, pr.dt)
     t1_pre_asm = time.perf_counter() - t0
     s1 = solve(o1, 1)
-    s2 = solve(o1, 1 + max(2, sample_iters // 4))
-    t1_it = (s2 - s1) / max(2, sample_iters // 4)
+    s2 = solve(o1, 1 + sample_iters)
+    t1_it = (s2 - s1) / sample_iters
     t1_factor = max(0.0, s1 - t1_it)
     t0 = time.perf_counter()
     o1.post_iterate(0.0, pr.dt)
     t1_post = time.perf_counter() - t0
-    step1_s = t1_pre_asm + t1_factor + iters_per_step * t1_it + t1_post
+    step1_s = t1_pre_asm + t1_factor + iters * t1_it + t1_post
     L.or_set_threads(threads)
     model = ""
     try:
@@ -329,22 +350,25 @@ def cpu_baseline(pr, iters_per_step, sample_iters):
             model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
     except OSError:
         pass
-    return {
+    rec = {
         "value": round(1.0 / step_s, 6),
         "unit": "time steps/s",
         "cores": threads,
         "kind": "port",
         "cpu_model": model,
+        "converged": int(conv),
+        "iterations": iters,
         "one_thread": {"value": round(1.0 / step1_s, 6), "unit": "time steps/s",
-                       "sample": (f"the same first step on 1 thread: preIterate+assembly {t1_pre_asm:.2f} s, "
-                                  f"ILU(0) update {t1_factor:.2f} s, {max(2, sample_iters // 4)} iterations at "
-                                  f"{t1_it:.3f} s each, postIterate {t1_post:.2f} s")},
-        "sample": (f"first step of the same workload on the C oracle (OpenMP, {threads} host threads; "
-                   f"ILU(0) factorization and triangular solves sequential): preIterate+assembly "
-                   f"{t_pre_asm:.2f} s, ILU(0) update {t_factor:.2f} s, {sample_iters} FP64 BiCGStab+ILU(0) "
-                   f"iterations timed at {t_it:.3f} s each and scaled to {iters_per_step:.0f} iterations/step "
-                   f"(the GPU's count for the same method), postIterate {t_post:.2f} s"),
+                       "sample": (f"the same first step on 1 thread, extrapolated: preIterate+assembly "
+                                  f"{t1_pre_asm:.2f} s, ILU(0) update {t1_factor:.2f} s, {sample_iters} "
+                                  f"iterations timed at {t1_it:.3f} s each and scaled to the {iters} iterations "
+                                  f"of the step, postIterate {t1_post:.2f} s")},
+        "sample": (f"the first time step of the same workload, whole, on the C oracle (OpenMP, {threads} host "
+                   f"threads; ILU(0) factorization and triangular solves sequential): preIterate+assembly "
+                   f"{t_pre_asm:.2f} s, ILU(0) update and {iters} FP64 BiCGStab+ILU(0) iterations to 1e-10 "
+                   f"{t_solve:.2f} s, postIterate {t_post:.2f} s"),
     }
+    return rec, z
 
 
 def main():
@@ -362,8 +386,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-ilu", action="store_true")
-    ap.add_argument("--fp32-face-groups", action="store_true",
-                    help="FP32 iteration with faces grouped by type (comparison)")
+    ap.add_argument("--natural-order", action="store_true",
+                    help="iterations in the natural face order (default: faces grouped by type, face_groups = 1)")
     ap.add_argument("--amg-steps", type=int, default=3,
                     help="timed steps of the FP64 BiCGStab + AMG form (0 skips it)")
     ap.add_argument("--isai-steps", type=int, default=3,
@@ -401,37 +425,47 @@ def main():
         return 2
 
     import numpy as np
-    from mhfem_amd import problems
+    from mhfem_amd import capi, problems
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     log(f"[bench] building the n=2 {args.size}^3 workload")
     pr = problems.adr_3d(s=args.size, seed=args.seed, preconditioner="diagonal")
     rows = pr.num_faces * pr.n
-    guess = dict(train_steps=args.train_steps, epochs=10, learning_rate=2e-3, sample_stride=8, combine="combined",
-                 reject_after=3, history_depth=args.history_depth)
-    chosen = {"name": "learned guess + FP32 BiCGStab with FP64 reliable updates (Jacobi)",
-              "solver": dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=args.delta,
-                             check_every=8, tol=1e-10, fp32_face_groups=int(args.fp32_face_groups)),
-              "guess": guess}
+    groups = 0 if args.natural_order else 1
+    history = dict(train_steps=0, combine="history", history_depth=args.history_depth)
+    network = dict(train_steps=args.train_steps, epochs=10, learning_rate=2e-3, sample_stride=8, combine="combined",
+                   reject_after=3, history_depth=args.history_depth)
+    fp32 = dict(preconditioner="diagonal", precision="fp32-reliable", inner_tol=args.delta, check_every=8, tol=1e-10,
+                face_groups=groups)
+    chosen = {"name": "previous-increment projection (least residual over the last 16 increments) + FP32 "
+                      "BiCGStab with FP64 reliable updates (Jacobi)",
+              "solver": fp32, "guess": history}
+    with_net = {"name": "the chosen form with the learned network's direction added (trained online on the "
+                        "first steps, switched off by its rejection rule)",
+                "solver": fp32, "guess": network}
     plain = {"name": "FP64 BiCGStab + Jacobi",
-             "solver": dict(preconditioner="diagonal", precision="fp64", check_every=8, tol=1e-10)}
+             "solver": dict(preconditioner="diagonal", precision="fp64", check_every=8, tol=1e-10,
+                            face_groups=groups)}
     plain_ilu = {"name": "FP64 BiCGStab + ILU(0)",
                  "solver": dict(preconditioner="ilu0", precision="fp64", check_every=4, tol=1e-10)}
+    cfg2 = {"name": "FP32 BiCGStab + ILU(0) (FP32 factors) inside FP64 refinement with reliable updates",
+            "solver": dict(preconditioner="ilu0", precision="fp32-reliable", inner_tol=args.delta, check_every=4,
+                           tol=1e-10)}
     plain_amg = {"name": "FP64 BiCGStab + AMG (PGM aggregation, block-Jacobi Richardson smoothing, V-cycle)",
                  "solver": dict(preconditioner="amg", precision="fp64", check_every=4, tol=1e-10)}
     plain_isai = {"name": "FP64 BiCGStab + ILU-ISAI (ILU(0) factors as sparse approximate inverses, power 2)",
                   "solver": dict(preconditioner="ilu-isai", precision="fp64", check_every=4, tol=1e-10)}
-    ablation = {"name": "chosen form without the network (previous increments only)",
-                "solver": chosen["solver"], "guess": dict(guess, train_steps=0, combine="history")}
     fp64_hist = {"name": "FP64 BiCGStab + Jacobi from the previous increments",
-                 "solver": plain["solver"], "guess": dict(guess, train_steps=0, combine="history")}
-    for f in (chosen, plain, ablation, fp64_hist):
+                 "solver": plain["solver"], "guess": history}
+    for f in (chosen, with_net, plain, fp64_hist):
         f["autotune"] = not args.no_autotune
 
     log(f"[bench] chosen form: {chosen['name']}")
     s, dt, its = run_form(pr, chosen, args.steps, args.warmup, timing=2)
     tm = s.timers()
     spmv_ms = tm["spmv_ms"] / max(1, tm["spmv_timed"])
-    gs = s.guess_stats()
     rp, col, _, _ = s.export_system()
     nnz = int(rp[-1])
     # stored off-diagonal blocks per face: the distinct other faces in each face's rows
@@ -443,8 +477,7 @@ def main():
     z_chosen = s.get_array("Z_iF").copy()
     s.close()
     res_chosen = summary(chosen["name"], args.steps, dt, its, chosen.get("rows_per_block"), chosen["fallback_steps"])
-    log(f"[bench] chosen: {res_chosen}; SpMV {spmv_ms*1e3:.1f} us over {tm['spmv_timed']} timed launches; "
-        f"guess {gs}")
+    log(f"[bench] chosen: {res_chosen}; SpMV {spmv_ms*1e3:.1f} us over {tm['spmv_timed']} timed launches")
 
     res_plain = z_plain = None
     spmv64_ms = None
@@ -458,20 +491,34 @@ def main():
         res_plain = summary(plain["name"], args.steps, dtp, itsp, plain.get("rows_per_block"))
         log(f"[bench] plain: {res_plain}; FP64 SpMV {spmv64_ms*1e3:.1f} us over {tmp['spmv_timed']} timed launches")
 
-    res_ilu = None
-    if not args.skip_ilu:
-        log(f"[bench] plain ILU(0) form (sample of {args.ilu_steps} steps after 1 warmup step)")
-        si, dti, itsi = run_form(pr, plain_ilu, args.ilu_steps, 1)
+    def ilu_leg(problem, form, label):
+        log(f"[bench] {label} ({args.ilu_steps} timed steps after 1 warmup step)")
+        si, dti, itsi = run_form(problem, form, args.ilu_steps, 1)
         si.close()
-        res_ilu = summary(plain_ilu["name"], args.ilu_steps, dti, itsi)
-        res_ilu["sample"] = f"steps 2-{1 + args.ilu_steps} of the trajectory"
-        log(f"[bench] plain ILU(0): {res_ilu}")
+        res = summary(form["name"], args.ilu_steps, dti, itsi, None, form["fallback_steps"])
+        res["sample"] = f"steps 2-{1 + args.ilu_steps} of the trajectory"
+        log(f"[bench] {label}: {res}")
+        return res
 
-    res_amg = None
-    if args.amg_steps > 0:
-        log(f"[bench] AMG form ({args.amg_steps} timed steps after 1 warmup step, which builds the hierarchy)")
-        from mhfem_amd import capi as _capi
-        sa = _capi.Solver(pr, **plain_amg["solver"])
+    res_ilu = res_cfg2 = res_cfg1 = None
+    if not args.skip_ilu:
+        res_ilu = ilu_leg(pr, plain_ilu, "plain FP64 ILU(0) form (configurations 1-2's preconditioner, n = 2)")
+        res_cfg2 = ilu_leg(pr, cfg2, "configuration 2 as configured: FP32 ILU(0) inside FP64 refinement")
+        log(f"[bench] building the n=1 Darcy {args.size}^3 workload (configuration 1)")
+        pr1 = problems.darcy_3d(s=args.size, seed=args.seed, preconditioner="ilu0")
+        c1_ilu = dict(plain_ilu, name="configuration 1: n = 1 Darcy, FP64 BiCGStab + ILU(0)")
+        res_cfg1 = ilu_leg(pr1, c1_ilu, "configuration 1 (n = 1 Darcy, FP64 BiCGStab + ILU(0))")
+        c1_jac = {"name": "configuration 1 with FP64 BiCGStab + Jacobi",
+                  "solver": dict(plain["solver"]), "autotune": not args.no_autotune}
+        sj, dtj, itsj = run_form(pr1, c1_jac, args.ilu_steps, 1)
+        sj.close()
+        res_cfg1["jacobi"] = summary(c1_jac["name"], args.ilu_steps, dtj, itsj, c1_jac.get("rows_per_block"))
+        res_cfg1["cells"], res_cfg1["faces"], res_cfg1["dofs"] = pr1.num_cells, pr1.num_faces, pr1.num_faces
+        del pr1
+
+    def timed_precond_leg(form, steps, label, extra):
+        log(f"[bench] {label} ({steps} timed steps after 1 warmup step, which builds it)")
+        sa = capi.Solver(pr, **form["solver"])
         t_setup = time.perf_counter()
         sa.step(0.0, pr.dt)
         sa.synchronize()
@@ -479,49 +526,40 @@ def main():
         t = pr.dt
         its0 = len(sa.iterations)
         t0 = time.perf_counter()
-        for _ in range(args.amg_steps):
+        for _ in range(steps):
             sa.step(t, pr.dt)
             t += pr.dt
         sa.synchronize()
         dta = time.perf_counter() - t0
-        levels, rebuilds = sa.amg_hierarchy()
-        res_amg = summary(plain_amg["name"], args.amg_steps, dta, sa.iterations[its0:])
-        res_amg.update(first_step_ms_with_setup=round(first_ms, 1), levels=len(levels),
-                       level_rows=[lv[0] for lv in levels[:6]], rebuilds=rebuilds)
+        res = summary(form["name"], steps, dta, sa.iterations[its0:])
+        res["first_step_ms_with_setup"] = round(first_ms, 1)
+        res.update(extra(sa))
         sa.close()
-        log(f"[bench] AMG: {res_amg}")
+        log(f"[bench] {label}: {res}")
+        return res
 
-    res_isai = None
+    res_amg = res_isai = None
+    if args.amg_steps > 0:
+        def amg_extra(sa):
+            levels, rebuilds = sa.amg_hierarchy()
+            return dict(levels=len(levels), level_rows=[lv[0] for lv in levels[:6]], rebuilds=rebuilds)
+        res_amg = timed_precond_leg(plain_amg, args.amg_steps, "AMG form", amg_extra)
     if args.isai_steps > 0:
-        log(f"[bench] ILU-ISAI form ({args.isai_steps} timed steps after 1 warmup step, which builds it)")
-        from mhfem_amd import capi as _capi
-        si = _capi.Solver(pr, **plain_isai["solver"])
-        t_setup = time.perf_counter()
-        si.step(0.0, pr.dt)
-        si.synchronize()
-        first_ms = 1e3 * (time.perf_counter() - t_setup)
-        t = pr.dt
-        its0 = len(si.iterations)
-        t0 = time.perf_counter()
-        for _ in range(args.isai_steps):
-            si.step(t, pr.dt)
-            t += pr.dt
-        si.synchronize()
-        dti = time.perf_counter() - t0
-        (lrp, _, _), (urp, _, _), rebuilds = si.isai_matrices()
-        res_isai = summary(plain_isai["name"], args.isai_steps, dti, si.iterations[its0:])
-        res_isai.update(first_step_ms_with_setup=round(first_ms, 1), nnz_lower=int(lrp[-1]),
-                        nnz_upper=int(urp[-1]), rebuilds=rebuilds)
-        si.close()
-        log(f"[bench] ILU-ISAI: {res_isai}")
+        def isai_extra(sa):
+            (lrp, _, _), (urp, _, _), rebuilds = sa.isai_matrices()
+            return dict(nnz_lower=int(lrp[-1]), nnz_upper=int(urp[-1]), rebuilds=rebuilds)
+        res_isai = timed_precond_leg(plain_isai, args.isai_steps, "ILU-ISAI form", isai_extra)
 
-    res_abl = res_h64 = z_h64 = None
+    res_net = res_h64 = z_h64 = None
+    net_stats = None
     if not args.skip_ablation:
-        log(f"[bench] ablation: {ablation['name']}")
-        sa, dta, itsa = run_form(pr, ablation, args.steps, args.warmup)
-        sa.close()
-        res_abl = summary(ablation["name"], args.steps, dta, itsa, ablation.get("rows_per_block"), ablation["fallback_steps"])
-        log(f"[bench] ablation: {res_abl}")
+        log(f"[bench] ablation: {with_net['name']}")
+        sn, dtn, itsn = run_form(pr, with_net, args.steps, args.warmup)
+        net_stats = sn.guess_stats()
+        sn.close()
+        res_net = summary(with_net["name"], args.steps, dtn, itsn, with_net.get("rows_per_block"),
+                          with_net["fallback_steps"])
+        log(f"[bench] with network: {res_net}; guess {net_stats}")
         log(f"[bench] {fp64_hist['name']}")
         sh, dth, itsh = run_form(pr, fp64_hist, args.steps, args.warmup)
         z_h64 = sh.get_array("Z_iF").copy()
@@ -531,7 +569,9 @@ def main():
 
     value = args.steps / dt
     csr = spmv_bytes(rows, nnz, True)
-    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True)
+    # the timed launches are the iteration's SpMV kernels with their fused dot product
+    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True, dot_operand=True)
+    fmt64 = block_sell_bytes(pr.n, faces, blocks_per_face, False, dot_operand=True)
     # priced on the bytes the block-SELL kernel must move; the CSR count of
     # the same product is reported beside it
     achieved = fmt / (spmv_ms * 1e-3) / 1e9
@@ -553,25 +593,28 @@ def main():
             "workload": f"n=2 advection-diffusion-reaction, {args.size}^3 cubes x 6 Kuhn tets, dt=1e-3, "
                         f"{args.warmup + args.steps}-step trajectory, 1e-10 rel. residual",
             "cells": pr.num_cells, "faces": faces, "dofs": rows, "nnz": nnz,
-            "form": chosen["name"],
+            "form": chosen["name"], "face_groups": groups,
         },
         "iterations_per_step": res_chosen["iterations_per_step"],
         "dof_per_s": round(rows * value, 1),
         "plain": res_plain,
         "plain_ilu0": res_ilu,
+        "config2_as_configured": res_cfg2,
+        "config1": res_cfg1,
         "plain_amg": res_amg,
         "plain_ilu_isai": res_isai,
-        "chosen_without_network": res_abl,
+        "with_network": res_net,
         "fp64_history": res_h64,
         "chosen_vs_plain": round(value / res_plain["steps_per_s"], 3) if res_plain else None,
         "chosen_vs_plain_ilu0": round(value / res_ilu["steps_per_s"], 3) if res_ilu else None,
-        "chosen_vs_fp64_rel_l2": (float(np.linalg.norm(z_chosen - z_plain) / np.linalg.norm(z_plain))
-                                  if z_plain is not None else None),
-        "fp64_history_vs_fp64_rel_l2": (float(np.linalg.norm(z_h64 - z_plain) / np.linalg.norm(z_plain))
-                                        if z_plain is not None and z_h64 is not None else None),
-        "guess": {"network_coefficient": round(gs["coef_network"], 4), "trained_steps": gs["trained_steps"],
-                  "used": gs["used"], "network_active_at_end": gs["network_active"],
-                  "network_gain_at_end": round(gs["network_gain"], 5)},
+        "chosen_vs_fp64_rel_l2": rel(z_chosen, z_plain) if z_plain is not None else None,
+        "fp64_history_vs_fp64_rel_l2": (rel(z_h64, z_plain) if z_plain is not None and z_h64 is not None
+                                        else None),
+        "network": None if net_stats is None else {
+            "network_coefficient_at_end": round(net_stats["coef_network"], 4),
+            "trained_steps": net_stats["trained_steps"], "network_active_at_end": net_stats["network_active"],
+            "network_gain_at_end": round(net_stats["network_gain"], 5),
+            "with_network_vs_chosen": round(res_net["steps_per_s"] / res_chosen["steps_per_s"], 3)},
         "roofline": {
             "kernel": f"k_sspmv<float, {pr.n}, ...> (block-SELL SpMV of the FP32 iteration)",
             "bound": "hbm",
@@ -590,12 +633,11 @@ def main():
         "plain_spmv_fp64": None if spmv64_ms is None else {
             "kernel": f"k_sspmv<double, {pr.n}, ...> (block-SELL SpMV of the plain FP64 iteration)",
             "avg_launch_us": round(spmv64_ms * 1e3, 2),
-            "algorithmic_bytes": int(block_sell_bytes(pr.n, faces, blocks_per_face, False)),
-            "achieved": round(block_sell_bytes(pr.n, faces, blocks_per_face, False) / (spmv64_ms * 1e-3) / 1e9, 1),
+            "algorithmic_bytes": int(fmt64),
+            "achieved": round(fmt64 / (spmv64_ms * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(block_sell_bytes(pr.n, faces, blocks_per_face, False) / (spmv64_ms * 1e-3) / 1e9
-                          / HBM_PEAK_GBS, 4),
+            "frac": round(fmt64 / (spmv64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "assembly": {
             "kernels": "k_assemble (interior and boundary rows, assembleLinearSystem)",
@@ -611,16 +653,24 @@ def main():
         },
         "ensemble": None,
         "cpu_baseline": None,
+        "error_vs_oracle_first_step": None,
     }
     if args.ensemble_steps > 0:
         log("[bench] batched ensemble")
         out["ensemble"] = ensemble_leg(args.ensemble_steps, 2)
         log(f"[bench] ensemble: {out['ensemble']}")
     if not args.skip_cpu:
-        log("[bench] cpu baseline (oracle)")
-        per_step = res_ilu["iterations_per_step"] if res_ilu else res_chosen["iterations_per_step"]
-        out["cpu_baseline"] = cpu_baseline(problems.adr_3d(s=args.size, seed=args.seed, preconditioner="ilu0"),
-                                           per_step, args.cpu_iters)
+        log("[bench] cpu baseline: the oracle's first step (FP64 BiCGStab + ILU(0) to 1e-10)")
+        out["cpu_baseline"], z_oracle = cpu_baseline(problems.adr_3d(s=args.size, seed=args.seed,
+                                                                     preconditioner="ilu0"), args.cpu_iters)
+        errs = {"note": "relative L2 of Z_iF after the first time step against the all-FP64 oracle's, both "
+                        "solved to 1e-10 (the difference of two solutions that meet that tolerance)"}
+        for name, form in (("chosen", chosen), ("plain", plain), ("plain_ilu0", plain_ilu),
+                           ("config2_as_configured", cfg2), ("with_network", with_net)):
+            if "z_first_step" in form:
+                errs[name] = rel(form["z_first_step"], z_oracle)
+        out["error_vs_oracle_first_step"] = errs
+        log(f"[bench] cpu baseline: {out['cpu_baseline']}; errors {errs}")
     print(json.dumps(out))
     return 0
 

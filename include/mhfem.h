@@ -159,8 +159,9 @@ typedef struct {
     int check_every;            /* host convergence poll period (iterations), >= 1 */
     int rows_per_block;         /* rows per work-group of the iteration kernels, 0: default
                                    (mhfem_autotune picks it by measurement) */
-    int fp32_face_groups;       /* FP32 iterations with Jacobi or no preconditioner: 1 keeps the
-                                   FP32 matrix copy and vectors in an order that groups faces of
+    int face_groups;            /* BiCGStab iterations with Jacobi or no preconditioner on a
+                                   single system (FP32 and FP64): 1 keeps a copy of the matrix
+                                   and the iteration's vectors in an order that groups faces of
                                    one type inside windows of 4096 faces, for coalesced gathers
                                    (results unchanged up to rounding); 0 (default): natural order */
     int bicgstab_ell;           /* l of BiCGStab(l) ("bicgstab-ell"), 1..8; 0: 1 */

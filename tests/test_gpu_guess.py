@@ -1,6 +1,7 @@
 """The learned initial guess on the GPU: forward pass and gradient of the
-hand-written BF16 MFMA kernels against the CPU oracle of the network, and the
-gain of the trained network over the warm start (previous Z_iF)."""
+hand-written BF16 MFMA kernels against the CPU oracle of the network, and
+what the trained network adds to the previous increments (its gain in the
+initial residual and its rejection rule)."""
 import numpy as np
 import pytest
 
@@ -51,45 +52,62 @@ def test_gradient_matches_oracle(rows):
         assert err < 1e-2, (name, err)
 
 
-def _run(pr, steps, guess=None, weights=None, train_steps=0, combine="combined", prec="diagonal"):
+def _run(pr, steps, guess=None, weights=None, train_steps=0, combine="combined", prec="diagonal", reject_after=0):
     s = capi.Solver(pr, preconditioner=prec, tol=1e-10)
     if guess:
-        s.configure_guess(True, train_steps=train_steps, epochs=30, learning_rate=2e-3, seed=3, combine=combine)
+        s.configure_guess(True, train_steps=train_steps, epochs=30, learning_rate=2e-3, seed=3, combine=combine,
+                          reject_after=reject_after)
         if weights is not None:
             s.set_guess_weights(weights)
-    its, used = [], []
+    its, stats = [], []
     t = 0.0
     for k in range(steps):
         st = s.step(t, pr.dt)
         t += pr.dt
         its.append(st.iterations)
         if guess:
-            used.append(s.guess_stats()["used"])
-    return s, its, used
+            stats.append(s.guess_stats())
+    return s, its, stats
 
 
-def test_trained_guess_cuts_iterations():
-    """Train online on one K field, evaluate frozen weights on a held-out K
-    field.  Ablation: the reference's warm start (previous Z_iF), the
-    least-residual combination of the previous increments alone (the method
-    without the network), and the same combination with the network's
-    direction added."""
+def test_trained_guess_gain_over_history():
+    """What the trained network adds to the method without it.  Train online
+    on one K field, evaluate frozen weights on a held-out field beside the
+    reference's warm start (previous Z_iF) and the least-residual
+    combination of the previous increments alone.  At every step the
+    network's direction lowers the squared initial residual beyond the
+    previous increments by `network_gain`, which is never negative (the
+    combination is least squares over a superset of the directions); its
+    size is what the network earns.  On these trajectories it is below 1 %
+    (printed), so the method's value comes from the previous increments, and
+    the rejection rule -- the network switched off after `reject_after`
+    consecutive solves with a gain below 1 % -- is replayed on the recorded
+    gains and must match the library's state at every step."""
     train = problems.darcy_3d(s=12, seed=11, steps=30, preconditioner="diagonal")
-    s, its_train, _ = _run(train, 30, guess=True, train_steps=30)
+    s, _, _ = _run(train, 30, guess=True, train_steps=30)
     w = s.guess_weights()
     test = problems.darcy_3d(s=12, seed=12, steps=20, preconditioner="diagonal")
     _, base, _ = _run(test, 20)
     _, hist, _ = _run(test, 20, guess=True, combine="history")
-    _, learned, used = _run(test, 20, guess=True, weights=w, train_steps=0)
+    _, learned, stats = _run(test, 20, guess=True, weights=w, train_steps=0)
+    # the gain is measured from the second step on (the first has no previous increment)
+    gains = [g["network_gain"] for g in stats[1:] if g["network_active"]]
     b, h, l = sum(base[3:]), sum(hist[3:]), sum(learned[3:])
-    print(f"\nwarm start {b}, previous increments {h}, with network {l}, used {sum(used)}", flush=True)
-    assert sum(used) > 0
-    assert l < b
-    # The combination never has a larger initial residual than the previous
-    # increments alone; iteration counts still vary by a few percent between
-    # nearby starting vectors, so this only asserts that the network's
-    # direction does not make the method materially worse.
-    assert l <= h * 1.05
+    print(f"\nwarm start {b}, previous increments {h}, with network {l}; network gain per step: "
+          f"mean {np.mean(gains):.4f}, max {np.max(gains):.4f}", flush=True)
+    assert sum(g["used"] for g in stats) > 0
+    assert len(gains) > 10 and min(gains) >= -1e-6
+    assert l < b and h < b
+    # the rejection rule, replayed
+    reject_after = 2
+    _, _, stats = _run(test, 20, guess=True, weights=w, train_steps=0, reject_after=reject_after)
+    fails, active = 0, True
+    for k, g in enumerate(stats):
+        assert g["network_active"] == int(active), (k, g)
+        if active and k > 0:
+            fails = fails + 1 if g["network_gain"] < 0.01 else 0
+            active = fails < reject_after
+    assert not active  # a network that adds under 1 % is rejected
 
 
 def test_deep_history_cuts_iterations():

@@ -454,13 +454,15 @@ def test_fp32_refinement(prec, precision, inner):
         assert rel(s.get_array("Z_iK"), o.array("Z_iK")) < 1e-10, k
 
 
-@pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2)])
-def test_fp32_face_groups(precision, inner):
-    """The FP32 iteration in the grouped face order (permuted FP32 matrix copy,
-    translated residuals and corrections) reaches the FP64 tolerance and the
-    oracle's solution, with iteration counts close to the natural order's."""
+@pytest.mark.parametrize("precision,inner", [("fp32-refined", 1e-4), ("fp32-reliable", 1e-2), ("fp64", 0.0)])
+def test_face_groups(precision, inner):
+    """The iteration in the grouped face order (a permuted copy of the matrix;
+    FP32: translated residuals and corrections, FP64: the right-hand side and
+    the iterate translated at the start and end of the solve) reaches the
+    tolerance and the oracle's solution, with iteration counts close to the
+    natural order's."""
     pr = problems.adr_3d(s=5, steps=3, preconditioner="diagonal")
-    s = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=inner, fp32_face_groups=1)
+    s = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=inner, face_groups=1)
     nat = capi.Solver(pr, tol=1e-13, precision=precision, inner_tol=inner)
     o = pyoracle.Oracle(pr)
     t = 0.0

@@ -583,26 +583,39 @@ __global__ void __launch_bounds__(KB) k_residual_mixed(SellView<double> A, const
 // The FP32 copy of the matrix in the gather-friendly face order: block b of
 // the permuted layout takes natural block map[b] (-1: padding, zero); the
 // within-face couplings of permuted face p are those of face perm[p].
-template <int N>
+template <int N, typename TO>
 __global__ void __launch_bounds__(KB) k_convert_permuted(int64_t blocks, const int64_t *__restrict__ map,
-                                                         const double *__restrict__ src, float *__restrict__ dst)
+                                                         const double *__restrict__ src, TO *__restrict__ dst)
 {
     constexpr int NN = N * N;
     for (int64_t b = (int64_t)blockIdx.x * KB + threadIdx.x; b < blocks; b += (int64_t)gridDim.x * KB) {
         const int64_t q = map[b];
 #pragma unroll
-        for (int t = 0; t < NN; t++) dst[b * NN + t] = q >= 0 ? (float)src[q * NN + t] : 0.0f;
+        for (int t = 0; t < NN; t++) dst[b * NN + t] = q >= 0 ? (TO)src[q * NN + t] : TO(0);
     }
 }
-template <int N>
+template <int N, typename TO>
 __global__ void __launch_bounds__(KB) k_convert_permuted_diag(int faces, const int *__restrict__ perm,
-                                                              const double *__restrict__ src, float *__restrict__ dst)
+                                                              const double *__restrict__ src, TO *__restrict__ dst)
 {
     constexpr int D = N * (N - 1);
     for (int p = blockIdx.x * KB + threadIdx.x; p < faces; p += gridDim.x * KB) {
         const int64_t e = perm[p];
 #pragma unroll
-        for (int t = 0; t < D; t++) dst[(int64_t)p * D + t] = (float)src[e * D + t];
+        for (int t = 0; t < D; t++) dst[(int64_t)p * D + t] = (TO)src[e * D + t];
+    }
+}
+
+// the n values of every face into the permuted face order (FORWARD:
+// out[p] = in[perm[p]]) or back (out[perm[p]] = in[p])
+template <int N, bool FORWARD>
+__global__ void __launch_bounds__(KB) k_permute_faces(int faces, const int *__restrict__ perm,
+                                                      const double *__restrict__ in, double *__restrict__ out)
+{
+    for (int p = blockIdx.x * KB + threadIdx.x; p < faces; p += gridDim.x * KB) {
+        const int64_t e = perm[p];
+        if (FORWARD) st<double, N>(out + (int64_t)p * N, ld<double, N>(in + e * N));
+        else st<double, N>(out + e * N, ld<double, N>(in + (int64_t)p * N));
     }
 }
 
@@ -1141,6 +1154,10 @@ struct Krylov {
     // its matrix in a face order that groups faces with the same neighbour
     // offsets (see build_fp32_order): perm[p] = natural face, iperm inverse
     bool permuted = false;
+    // the same order for the FP64 iteration: permuted FP64 matrix copy, the
+    // right-hand side and iterate of the solve in that order
+    bool permuted64 = false;
+    DBuf<double> val64p, bp, xp;
     DBuf<int> perm, iperm, slice_width32, col32;
     DBuf<int64_t> slice_ptr32, map32;
     int64_t blocks32 = 0, diag_base32 = 0;
@@ -1554,10 +1571,23 @@ Krylov *krylov_create(Ctx &c)
         }
     }
     if (c.cfg.preconditioner == MHFEM_PREC_AMG) k->amg = amg_create();
+    if (c.cfg.precision == MHFEM_PRECISION_FP64 && c.cfg.face_groups != 0 && k->nseg == 1 && c.unit_diagonal &&
+        c.cfg.solver == MHFEM_SOLVER_BICGSTAB &&
+        (c.cfg.preconditioner == MHFEM_PREC_DIAGONAL || c.cfg.preconditioner == MHFEM_PREC_NONE)) {
+        // the FP64 BiCGStab iteration in the grouped face order: a permuted
+        // copy of the matrix, the right-hand side and the iterate translated
+        // at the start and the end of each solve
+        build_fp32_order(c, *k);
+        k->permuted = false;
+        k->permuted64 = true;
+        k->val64p.alloc(k->diag_base32 + (int64_t)c.F * c.n * (c.n - 1));
+        k->bp.alloc(n);
+        k->xp.alloc(n);
+    }
     if (c.cfg.precision != MHFEM_PRECISION_FP64) {
         // the gather-friendly order needs a plain FP32 iteration: one system,
         // the implicit unit-diagonal Jacobi (or none), the block layout
-        const bool reorder = k->nseg == 1 && c.unit_diagonal && c.cfg.fp32_face_groups != 0 &&
+        const bool reorder = k->nseg == 1 && c.unit_diagonal && c.cfg.face_groups != 0 &&
                              (c.cfg.preconditioner == MHFEM_PREC_DIAGONAL || c.cfg.preconditioner == MHFEM_PREC_NONE);
         if (reorder) build_fp32_order(c, *k);
         k->val32.alloc(k->permuted ? k->diag_base32 + (int64_t)c.F * c.n * (c.n - 1) : c.sell_size);
@@ -1610,11 +1640,15 @@ struct Solve {
         sg = Seg{k.nseg, k.nblk, k.blk_seg.p, k.blk_row.p, k.seg_blk.p, k.sc.p, k.partials.p};
     }
 
+    // the FP64 BiCGStab solve runs in the grouped face order (k.permuted64)
+    bool perm64 = false;
+
     template <typename T>
     SellView<T> matrix()
     {
         if constexpr (sizeof(T) == 8)
-            return SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, c.sell_val.p, c.diag_base};
+            return perm64 ? SellView<T>{c.F, k.slice_ptr32.p, k.slice_width32.p, k.col32.p, k.val64p.p, k.diag_base32}
+                          : SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, c.sell_val.p, c.diag_base};
         else
             return k.permuted ? SellView<T>{c.F, k.slice_ptr32.p, k.slice_width32.p, k.col32.p, k.val32.p, k.diag_base32}
                               : SellView<T>{c.F, c.slice_ptr.p, c.slice_width.p, c.sell_col.p, k.val32.p, c.diag_base};
@@ -2606,13 +2640,24 @@ static void precond_update(Ctx &c, Krylov &k)
     hipLaunchKernelGGL((k_fill_t<double>), g, KB, 0, c.stream, (long)n, k.dinv.p, 1.0);
     MHFEM_HIP(hipGetLastError());
     const bool mixed = c.cfg.precision != MHFEM_PRECISION_FP64;
+    if (k.permuted64) {
+        with_n(c.n, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_convert_permuted<N, double>), grid_for(k.blocks32), KB, 0, c.stream, k.blocks32,
+                               k.map32.p, c.sell_val.p, k.val64p.p);
+            if constexpr (N > 1)
+                hipLaunchKernelGGL((k_convert_permuted_diag<N, double>), grid_for(c.F), KB, 0, c.stream, c.F,
+                                   k.perm.p, c.sell_val.p + c.diag_base, k.val64p.p + k.diag_base32);
+        });
+        MHFEM_HIP(hipGetLastError());
+    }
     if (mixed && k.permuted) {
         with_n(c.n, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_convert_permuted<N>), grid_for(k.blocks32), KB, 0, c.stream, k.blocks32, k.map32.p,
-                               c.sell_val.p, k.val32.p);
+            hipLaunchKernelGGL((k_convert_permuted<N, float>), grid_for(k.blocks32), KB, 0, c.stream, k.blocks32,
+                               k.map32.p, c.sell_val.p, k.val32.p);
             if constexpr (N > 1)
-                hipLaunchKernelGGL((k_convert_permuted_diag<N>), grid_for(c.F), KB, 0, c.stream, c.F, k.perm.p,
+                hipLaunchKernelGGL((k_convert_permuted_diag<N, float>), grid_for(c.F), KB, 0, c.stream, c.F, k.perm.p,
                                    c.sell_val.p + c.diag_base, k.val32.p + k.diag_base32);
         });
         hipLaunchKernelGGL((k_convert<float, double>), g, KB, 0, c.stream, (long)n, k.dinv.p, k.dinv32.p);
@@ -2909,7 +2954,27 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
             out.converged = res < tol;
         } else {
             Solve::Vecs v{k.r.p, k.r_ast.p, k.p.p, k.Ap.p, k.s.p, k.As.p, k.tmp.p, k.tmp2.p, k.dinv.p};
-            out.iterations = S.bicgstab<double>(b, x, v, tol, maxit);
+            if (k.permuted64) {
+                with_n(c.n, [&](auto NC) {
+                    constexpr int N = decltype(NC)::value;
+                    hipLaunchKernelGGL((k_permute_faces<N, true>), grid_for(c.F), KB, 0, c.stream, c.F, k.perm.p, b,
+                                       k.bp.p);
+                    hipLaunchKernelGGL((k_permute_faces<N, true>), grid_for(c.F), KB, 0, c.stream, c.F, k.perm.p,
+                                       (const double *)x, k.xp.p);
+                });
+                MHFEM_HIP(hipGetLastError());
+                S.perm64 = true;
+                out.iterations = S.bicgstab<double>(k.bp.p, k.xp.p, v, tol, maxit);
+                S.perm64 = false;
+                with_n(c.n, [&](auto NC) {
+                    constexpr int N = decltype(NC)::value;
+                    hipLaunchKernelGGL((k_permute_faces<N, false>), grid_for(c.F), KB, 0, c.stream, c.F, k.perm.p,
+                                       (const double *)k.xp.p, x);
+                });
+                MHFEM_HIP(hipGetLastError());
+            } else {
+                out.iterations = S.bicgstab<double>(b, x, v, tol, maxit);
+            }
             out.residue = 0;
             out.converged = 1;
             for (int s = 0; s < k.nseg; s++) {

@@ -41,7 +41,7 @@ class SolverConfig(C.Structure):
                 ("convergence_residue", C.c_double), ("max_iterations", C.c_int),
                 ("gmres_restart", C.c_int), ("precision", C.c_int),
                 ("inner_residue", C.c_double), ("check_every", C.c_int), ("rows_per_block", C.c_int),
-                ("fp32_face_groups", C.c_int), ("bicgstab_ell", C.c_int), ("idr_s", C.c_int),
+                ("face_groups", C.c_int), ("bicgstab_ell", C.c_int), ("idr_s", C.c_int),
                 ("sor_omega", C.c_double), ("ilut_p", C.c_int), ("ilut_threshold", C.c_double)]
 
 
@@ -298,7 +298,7 @@ class Solver:
 
     def __init__(self, problem, solver=None, preconditioner=None, tol=None, precision="fp64",
                  inner_tol=1e-4, check_every=8, max_iterations=100000, gmres_restart=30,
-                 setup=True, rows_per_block=0, output_directory=None, fp32_face_groups=0, bicgstab_ell=0,
+                 setup=True, rows_per_block=0, output_directory=None, face_groups=0, bicgstab_ell=0,
                  idr_s=0, sor_omega=0.0, ilut_p=0, ilut_threshold=0.0):
         pr = problem
         self.problem = pr
@@ -331,7 +331,7 @@ class Solver:
                                    PRECONDITIONERS[preconditioner or pr.preconditioner],
                                    pr.tol if tol is None else tol, max_iterations, gmres_restart,
                                    PRECISIONS[precision], inner_tol, check_every, rows_per_block,
-                                   fp32_face_groups, bicgstab_ell, idr_s, sor_omega, ilut_p, ilut_threshold)
+                                   face_groups, bicgstab_ell, idr_s, sor_omega, ilut_p, ilut_threshold)
         if setup:
             self.setInitialCondition()
             self.setupLinearSystem()
