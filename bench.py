@@ -68,7 +68,7 @@ sys.path.insert(0, os.path.join(ROOT, "tnl-mhfem_amd"))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by a plain copy's reading); the summary holds the commands
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_fp32_n2_cube64_hbm_counters.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_block_sell_n2_cube64_hbm_counters.md")
 METRIC = "time steps/s at 1e-10 rel. residual, chosen vs plain GPU form; HBM GB/s"
 
 
