@@ -35,7 +35,8 @@ template <typename Problem>
 void solve(Problem &problem, double startTime, double stopTime, double timeStep)
 {
     double t = startTime;
-    // ignore very small steps at the end, most likely caused by truncation errors
+    // the loop ends when what is left of the interval is below 1e-6 of a step:
+    // such a remainder comes from rounding in the accumulated t, not from the interval
     while (stopTime - t > timeStep * 1e-6) {
         const double currentTau = std::min(timeStep, stopTime - t);
         problem.preIterate(t, currentTau);

@@ -8,7 +8,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from mhfem_amd import problems
+from mhfem_amd import meshgen, problems
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXAMPLE = os.path.join(ROOT, "tnl-mhfem_amd", "examples", "heat_equation")
@@ -37,3 +37,41 @@ def test_heat_equation_example_matches_oracle():
     zo = o.array("Z_iK")
     assert z.size == zo.size
     assert np.linalg.norm(z - zo) / np.linalg.norm(zo) < 1e-10
+
+
+@pytest.mark.gpu
+def test_model_hook_rewrites_a_coefficient():
+    """A model hook that rewrites a coefficient on every step: the example's
+    HookedHeatModel, whose preIterate hook (BaseModel.h:248-250) uploads
+    D_K = (1 + 50 (t + tau)) (1 + 0.5 x_K) as a full array (uniform before
+    the first step).  The reference calls the hook at the end of
+    Solver::preIterate (Solver.hpp:873-875), so the new values enter the next
+    step; the oracle applies the same update at the same point."""
+    import pyoracle
+
+    s, steps = 24, 12
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "z.bin")
+        res = subprocess.run([EXAMPLE, "--hook", out, str(s), str(steps), "1e-13"], capture_output=True, text=True,
+                             timeout=120)
+        print(res.stdout[-1500:], res.stderr[-1500:])
+        assert res.returncode == 0
+        z = np.fromfile(out, np.float64)
+    pr = problems.heat_2d(s=s, steps=steps)
+    o = pyoracle.Oracle(pr)
+    cx = meshgen.cell_centroids(pr.points, pr.cells)[:, 0]
+    t = 0.0
+    for _ in range(steps):
+        o.pre_iterate(t, pr.dt)
+        o.array("D_ijK")[:] = (1.0 + 50.0 * (t + pr.dt)) * (1.0 + 0.5 * cx)
+        o.assemble(t, pr.dt)
+        o.solve(tol=1e-13)
+        o.post_iterate(t, pr.dt)
+        t += pr.dt
+    zo = o.array("Z_iK")
+    assert np.linalg.norm(z - zo) / np.linalg.norm(zo) < 1e-10
+    # the hook changes the result: the same run without it differs
+    plain = problems.heat_2d(s=s, steps=steps)
+    o2 = pyoracle.Oracle(plain)
+    o2.run(0.0, steps * plain.dt, plain.dt, tol=1e-13)
+    assert np.linalg.norm(o2.array("Z_iK") - zo) / np.linalg.norm(zo) > 1e-3

@@ -1033,7 +1033,7 @@ int mhfem_solve_interval(mhfem_ctx *ctx, double t0, double t1, double dt, mhfem_
     if (!(dt > 0)) return MHFEM_E_ARGUMENT;
     double t = t0;
     int rc = MHFEM_OK;
-    // control.h:74-101: ignore very small steps at the end
+    // control.h:74-101; the loop stops when less than 1e-6 of a step is left (rounding in t)
     while (t1 - t > dt * 1e-6) {
         const double tau = std::min(dt, t1 - t);
         if ((rc = mhfem_pre_iterate(ctx, t, tau))) return rc;

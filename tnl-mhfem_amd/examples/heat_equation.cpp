@@ -3,8 +3,9 @@
 // 2-D heat equation on the unit square, s x s squares split into triangles,
 // zero Dirichlet boundary, Gaussian initial condition, BiCGStab + diagonal.
 //
-//   heat_equation [out.bin] [s] [steps]
-// writes the final cell values Z_iK (float64) to out.bin when given.
+//   heat_equation [--hook] [out.bin] [s] [steps] [residue]
+// writes the final cell values Z_iK (float64) to out.bin when given; --hook
+// runs the model with a preIterate hook that rewrites the diffusivity.
 //
 //   heat_equation --config config.ini
 // runs the reference's file-driven example instead: the mesh (.vtk or .vtu),
@@ -16,6 +17,7 @@
 #include <cstdio>
 #include <iostream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mhfem/control.hpp"
@@ -57,6 +59,23 @@ public:
     }
 };
 
+// The same model with a hook that rewrites a coefficient every step: the
+// reference's preIterate hook (BaseModel.h:248-250), which Solver::preIterate
+// calls after the scheme's update (Solver.hpp:873-875), so the new values
+// enter the next step's update.  Here the diffusivity grows in time and
+// varies in x, D_K = (1 + 50 (t + tau)) (1 + 0.5 x_K), uploaded as a full
+// array in the D_ijK layout.
+class HookedHeatModel : public HeatEquationModel {
+public:
+    std::vector<double> centroid_x;
+    void preIterate(double time, double tau) override
+    {
+        std::vector<double> D(centroid_x.size());
+        for (size_t K = 0; K < D.size(); K++) D[K] = (1.0 + 50.0 * (time + tau)) * (1.0 + 0.5 * centroid_x[K]);
+        setCoefficient(MHFEM_D_ijK, D);
+    }
+};
+
 static SimplexMesh unit_square(int s)
 {
     SimplexMesh m;
@@ -93,12 +112,11 @@ static int run_config(const char *config)
     return 0;
 }
 
-int main(int argc, char **argv)
+// heat_equation [--hook] [out.bin] [s] [steps] [residue]: the heat example on
+// an s x s triangulated square (--hook: with HookedHeatModel's coefficient hook)
+template <typename Model>
+static int run(const char *out, int s, int steps, double residue)
 {
-    if (argc == 3 && std::string(argv[1]) == "--config") return run_config(argv[2]);
-    const int s = argc > 2 ? std::atoi(argv[2]) : 64;
-    const int steps = argc > 3 ? std::atoi(argv[3]) : 100;
-    const double residue = argc > 4 ? std::atof(argv[4]) : 1e-10;
     ParameterContainer p;
     p.setParameter("linear-solver", std::string("bicgstab"));
     p.setParameter("preconditioner", std::string("diagonal"));
@@ -110,10 +128,10 @@ int main(int argc, char **argv)
     p.setParameter("diffusivity", 1.0);
     try {
         const SimplexMesh mesh = unit_square(s);
-        Solver<HeatEquationModel> problem;
+        Solver<Model> problem;
         problem.setMesh(mesh);
         // initial condition at cell centroids
-        std::vector<double> z0(mesh.numCells());
+        std::vector<double> z0(mesh.numCells()), cxs(mesh.numCells());
         for (int K = 0; K < mesh.numCells(); K++) {
             double cx = 0, cy = 0;
             for (int k = 0; k < 3; k++) {
@@ -121,7 +139,9 @@ int main(int argc, char **argv)
                 cy += mesh.points[2 * mesh.cells[3 * K + k] + 1] / 3;
             }
             z0[K] = std::exp(-((cx - 0.5) * (cx - 0.5) + (cy - 0.5) * (cy - 0.5)) / 0.01);
+            cxs[K] = cx;
         }
+        if constexpr (std::is_same_v<Model, HookedHeatModel>) problem.getMeshDependentData().centroid_x = cxs;
         problem.getMeshDependentData().setInitialCondition(0, z0);
         if (!problem.setup(p)) throw std::runtime_error("Failed to set up the MHFEM solver.");
         if (!problem.setInitialCondition(p)) throw std::runtime_error("Failed to set up the initial condition.");
@@ -129,9 +149,9 @@ int main(int argc, char **argv)
         problem.enableTimers();
         solve(problem, p);
         problem.writeEpilog(std::cout);
-        if (argc > 1) {
+        if (out) {
             const auto &Z = problem.getMeshDependentData().lastSnapshot;
-            FILE *f = std::fopen(argv[1], "wb");
+            FILE *f = std::fopen(out, "wb");
             if (!f || std::fwrite(Z.data(), sizeof(double), Z.size(), f) != Z.size()) return 3;
             std::fclose(f);
         }
@@ -140,4 +160,16 @@ int main(int argc, char **argv)
         return 1;
     }
     return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc == 3 && std::string(argv[1]) == "--config") return run_config(argv[2]);
+    const bool hook = argc > 1 && std::string(argv[1]) == "--hook";
+    const int a = hook ? 1 : 0;
+    const char *out = argc > 1 + a ? argv[1 + a] : nullptr;
+    const int s = argc > 2 + a ? std::atoi(argv[2 + a]) : 64;
+    const int steps = argc > 3 + a ? std::atoi(argv[3 + a]) : 100;
+    const double residue = argc > 4 + a ? std::atof(argv[4 + a]) : 1e-10;
+    return hook ? run<HookedHeatModel>(out, s, steps, residue) : run<HeatEquationModel>(out, s, steps, residue);
 }
