@@ -781,30 +781,49 @@ __global__ void __launch_bounds__(KB) k_ilu_upper(const int *__restrict__ rows, 
 //   start in solve order and every chunk a lane waits on has started before:
 //   the earliest unfinished chunk is resident and waits on finished ones;
 // * a value is ready when it no longer holds a sentinel (a signalling-NaN
-//   payload that arithmetic never produces): values of the chunk's own faces
-//   are read from LDS, those of other chunks from the output vector with
-//   device-coherent loads (agent-scope atomics), written by their producer
-//   with device-coherent stores.  The value is its own flag, so no fence is
-//   needed;
+//   payload that arithmetic never produces).  Lanes read values from LDS
+//   only.  The chunk's own rows are written there by their lanes; the rows of
+//   earlier chunks that the chunk reads (its imports, listed on the host with
+//   an LDS slot for every factor entry) are copied in by one more wavefront,
+//   which polls them in the output vector with device-coherent loads
+//   (agent-scope atomics).  Producers write the output vector with
+//   device-coherent stores.  The value is its own flag, so no fence (which
+//   would write back the XCD's L2) is needed;
+// * before it waits on anything, a lane loads its face's factors and the
+//   LDS slots of its neighbour faces into registers (up to ilu_window<N>()
+//   neighbours; the rest are read from memory), so a released wait is
+//   followed only by LDS reads and arithmetic: a dependency inside the chunk
+//   costs one round of the wavefront, one in another chunk the copy
+//   wavefront's poll;
 // * each wavefront repeats rounds until its lanes are done.  A lane keeps
-//   its place (row, entry, partial sum) when a value is not ready yet and
-//   resumes there, so it reads each value once and a waiting lane polls one
-//   address per round; a round without progress backs off with s_sleep;
+//   its place (neighbour, partial sums) when a neighbour is not ready yet
+//   and resumes there, polling only that neighbour;
 // * every wait is bounded in time: past ILU_WAIT_TICKS of the real-time
 //   counter (100 MHz) since the wavefront started, or once another one has
-//   given up, the remaining rows are written as zero, an error flag is set
-//   and the segment is marked DONE, so every launch ends, the queued work of
-//   the solve turns into no-ops and the host reports the error.
+//   given up, the remaining rows (or imports) are written as zero, an error
+//   flag is set and the segment is marked DONE, so every launch ends, the
+//   queued work of the solve turns into no-ops and the host reports the error.
 // The lower solve writes the intermediate vector z and resets the output y
 // to the sentinel; the upper solve reads z, writes y and resets z to the
 // sentinel for the next apply (each row of z is read only by its own lane).
-// Each solve also resets the other's ticket counter.
+// Each solve also resets the other's ticket counter.  A matrix that is not
+// made of full n x n face blocks, or with a chunk that imports more than
+// ILU_IMPORTS values, uses the level launches.
 #ifndef MHFEM_ILU_WG
 #define MHFEM_ILU_WG 512
 #endif
 constexpr int ILU_WG = MHFEM_ILU_WG;
-constexpr int ILU_BACKOFF_MAX = 7;  // s_sleep(4) rounds: about 0.9 us at most
+constexpr int ILU_NF = 64;                   // the copy wavefront
+constexpr int ILU_TF = 16;                   // imports per copy lane
+constexpr int ILU_IMPORTS = ILU_NF * ILU_TF;
 constexpr unsigned long long ILU_WAIT_TICKS = 20000000ull;  // 0.2 s at 100 MHz
+// neighbour faces whose factors a lane holds (a face of a tetrahedral mesh
+// has at most 6 neighbours, 5 of them in the lower triangle at n = 2)
+template <int N, bool UPPER>
+constexpr int ilu_window()
+{
+    return N == 1 ? 8 : N == 2 ? (UPPER ? 6 : 5) : 4;
+}
 
 template <typename T>
 struct SentinelBits;
@@ -851,11 +870,15 @@ __device__ __forceinline__ void lds_store(T *p, T v)
 
 struct IluSweep {
     int faces, nchunks;
-    const int *rp, *col, *dg;
-    const int *order;  // chunks in start order: [0, nchunks) lower solve, [nchunks, 2 nchunks) upper
-    int *ticket;  // [0]: lower solve, [1]: upper solve
+    const int *rp, *dg;
+    const int *slot;      // per CSR entry of either triangle: its value's LDS slot
+    const int *imp_ptr;   // [0, nchunks] lower solve, [nchunks + 1, 2 nchunks + 1] upper
+    const int *imp_rows;  // the imported rows of each chunk, in the order they become ready
+    const int *lane_face; // [0, faces) lower solve, [faces, 2 faces): the face offset each lane of a chunk solves
+    const int *order;     // chunks in start order: [0, nchunks) lower solve, [nchunks, 2 nchunks) upper
+    int *ticket;          // [0]: lower solve, [1]: upper solve
     int *err;
-    double *sc;   // the scalar slots of every segment: a failed wait marks them DONE
+    double *sc;  // the scalar slots of every segment: a failed wait marks them DONE
     int nseg;
 };
 
@@ -871,159 +894,316 @@ __global__ void __launch_bounds__(KB) k_ilu_prepare(long n, T *z, int *ticket, i
     }
 }
 
+__device__ __forceinline__ void ilu_fail(const IluSweep &a)
+{
+    coherent_store(a.err, 1);
+    for (int g = 0; g < a.nseg; g++) a.sc[(long)g * NSLOTS + S_DONE] = 1.0;
+}
+
 // lower (UPPER = false: in = x, out = z, reset = y) or upper (UPPER = true:
 // in = z, out = y, reset = z) triangular solve.  sc: flags of the single
 // segment (the solve is skipped once it has halted; a wait that gives up
-// marks it DONE) or nullptr.
+// marks it DONE) or nullptr.  ILU_WG solving lanes and one copy wavefront.
 template <typename T, int N, bool UPPER>
-__global__ void __launch_bounds__(ILU_WG) k_ilu_sweep(IluSweep a, const T *__restrict__ lu, const T *in, T *out,
-                                                      T *reset, const double *sc)
+__global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(IluSweep a, const T *__restrict__ lu, const T *in, T *out,
+                                                               T *reset, const double *sc)
 {
     if (sc && halted(sc)) return;
-    __shared__ T ys[ILU_WG * N];
+    __shared__ T ys[ILU_WG * N + ILU_IMPORTS];
     __shared__ int s_ticket;
-    const int lane = threadIdx.x;
-    if (lane == 0) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
         s_ticket = atomicAdd(&a.ticket[UPPER ? 1 : 0], 1);
         // the other solve's counter is free now: reset it for its next launch
         if (s_ticket == 0) a.ticket[UPPER ? 0 : 1] = 0;
     }
-#pragma unroll
-    for (int i = 0; i < N; i++) ys[lane * N + i] = sentinel_value<T>();
+    for (int i = tid; i < ILU_WG * N + ILU_IMPORTS; i += ILU_WG + ILU_NF) ys[i] = sentinel_value<T>();
     __syncthreads();
     // tickets past the chunk count only follow a failed wait (the counter
     // was not reset): nothing to do
     if (s_ticket >= a.nchunks) return;
     const int chunk = a.order[(UPPER ? a.nchunks : 0) + s_ticket];
     const int f0 = chunk * ILU_WG, f1 = min(a.faces, f0 + ILU_WG);
-    const int E = f0 + lane;
-    // resumable state: rows finished, next entry of the current row (-1: the
-    // row is not started), its partial sum, the finished rows' values, and
-    // the entry it waits on (its column, -1: none, and its factor value)
-    int q = E < f1 ? 0 : N;
-    int k = -1;
-    T s = T(0);
-    T yi[N];
-#pragma unroll
-    for (int i = 0; i < N; i++) yi[i] = T(0);
-    int pc = -1;
-    T pv = T(0);
     // a launch that follows a failed wait ends at once
     bool give_up = coherent_load(a.err) != 0;
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-    int backoff = 0;
-    while (__any(q < N)) {
+    int idle = 0;
+
+    if (tid >= ILU_WG) {
+        // ---- the copy wavefront: imports into LDS as they become ready
+        T *imp = ys + ILU_WG * N;
+        const int j = tid - ILU_WG;
+        const int *ip = a.imp_ptr + (UPPER ? a.nchunks + 1 : 0);
+        const int i0 = ip[chunk], ni = ip[chunk + 1] - i0;
+        int rows[ILU_TF];
+        unsigned pend = 0;
+#pragma unroll
+        for (int t = 0; t < ILU_TF; t++) {
+            const int e = j + t * ILU_NF;
+            rows[t] = e < ni ? a.imp_rows[i0 + e] : 0;
+            if (e < ni) pend |= 1u << t;
+        }
+        while (__any(pend != 0)) {
+            bool progress = false;
+            if (pend != 0 && !give_up) {
+                // poll the earliest pending import; once it is there, load
+                // the lane's other pending ones together
+                const int f = __builtin_ctz(pend);
+                int row = rows[0];
+#pragma unroll
+                for (int t = 1; t < ILU_TF; t++)
+                    if (t == f) row = rows[t];
+                const T v = coherent_load(out + row);
+                if (!is_sentinel(v)) {
+                    lds_store(&imp[j + f * ILU_NF], v);
+                    pend &= pend - 1;
+                    progress = true;
+                    T w[ILU_TF];
+#pragma unroll
+                    for (int t = 0; t < ILU_TF; t++) w[t] = (pend >> t & 1u) ? coherent_load(out + rows[t]) : T(0);
+#pragma unroll
+                    for (int t = 0; t < ILU_TF; t++)
+                        if ((pend >> t & 1u) && !is_sentinel(w[t])) {
+                            lds_store(&imp[j + t * ILU_NF], w[t]);
+                            pend &= ~(1u << t);
+                        }
+                }
+            }
+            if (__any(progress)) continue;
+            // each poll is a round trip to memory already: a short pause
+            __builtin_amdgcn_s_sleep(2);
+            if (!give_up && (++idle & 15) == 0 &&
+                (__builtin_amdgcn_s_memrealtime() - t0 > ILU_WAIT_TICKS || coherent_load(a.err) != 0))
+                give_up = true;
+            if (give_up && pend != 0) {
+                // release the lanes that wait on these imports
+                ilu_fail(a);
+#pragma unroll
+                for (int t = 0; t < ILU_TF; t++)
+                    if (pend >> t & 1u) lds_store(&imp[j + t * ILU_NF], T(0));
+                pend = 0;
+            }
+        }
+        return;
+    }
+
+    // ---- a solving lane: face E's n rows.  The factors are made of full
+    // n x n blocks (checked on the host), so every row of a face lists the
+    // same neighbour faces: each neighbour's n values are read once per
+    // round and serve all rows.  The chunk's faces are handed to lanes by
+    // dependency level, so that a wavefront's lanes become ready together;
+    // face E's values live at LDS slots (E - f0) * N + i, and those of a
+    // neighbour face from its slot sb on, at sb + j.
+    // The rows that sum their neighbour terms as the values come in (lower:
+    // every row; upper: the last one, the other rows subtract the face's own
+    // values first and are summed at the end) keep their factors in
+    // registers; the upper solve's other rows keep theirs in LDS.
+    constexpr int K = ilu_window<N, UPPER>();
+    constexpr int NR = UPPER ? 1 : N;
+    constexpr int NL = UPPER ? N - 1 : 0;
+    __shared__ T upf[NL > 0 ? NL * K * N * ILU_WG : 1];
+    const int E = f0 + tid < f1 ? f0 + a.lane_face[(UPPER ? a.faces : 0) + f0 + tid] : f1;
+    const int me = (E - f0) * N;
+    bool done = E >= f1;
+    int nb = 0;          // neighbour faces
+    int nbase[N];        // row i's first neighbour entry
+    int slotF[K];        // LDS slot of each neighbour face (component 0)
+    T lf[NR][K][N];      // summing row's factor for neighbour b, component j
+    T own[N][N];         // the face's own off-diagonal factors (lower: j < i, upper: j > i)
+    T z[N], dgv[N];      // right-hand side and (upper) diagonal of each row
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        nbase[i] = 0;
+        z[i] = T(0);
+        dgv[i] = T(1);
+#pragma unroll
+        for (int j = 0; j < N; j++) own[i][j] = T(0);
+    }
+    if (!done) {
+        // a row lists its neighbours and, unless the face's rows are
+        // decoupled (a Dirichlet face), the face's own other components
+        int d[N], rpv[N + 1];
+#pragma unroll
+        for (int i = 0; i <= N; i++) rpv[i] = a.rp[E * N + i];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            d[i] = a.dg[E * N + i];
+            z[i] = in[E * N + i];
+        }
+        const int nl = d[0] - rpv[0], nu = rpv[N] - d[N - 1] - 1;
+        nb = (UPPER ? nu : nl) / N;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const bool has_own = UPPER ? rpv[i + 1] - d[i] - 1 > nu : d[i] - rpv[i] > nl;
+            nbase[i] = UPPER ? d[i] + 1 + (has_own ? N - 1 - i : 0) : rpv[i];
+            if (UPPER) dgv[i] = lu[d[i]];
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                if (!UPPER && j < i && has_own) own[i][j] = lu[rpv[i] + nl + j];
+                if (UPPER && j > i && has_own) own[i][j] = lu[d[i] + j - i];
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < K; b++) {
+        const bool has = b < nb;
+        slotF[b] = has ? a.slot[nbase[0] + b * N] : 0;
+#pragma unroll
+        for (int i = 0; i < NR; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++) lf[i][b][j] = has ? lu[nbase[UPPER ? N - 1 : i] + b * N + j] : T(0);
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++)
+                upf[((i * K + b) * N + j) * ILU_WG + tid] = has ? lu[nbase[i] + b * N + j] : T(0);
+    }
+    // wait for these loads here, once: the rounds below then have no memory
+    // wait on their path (the compiler would otherwise wait in the loop, and
+    // so also for the output stores of the previous round)
+    __builtin_amdgcn_s_waitcnt(0);
+    // partial sums: every row (lower) or the last row (upper: the other rows
+    // subtract the face's own values first, so they are summed at the end)
+    T s[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) s[i] = z[i];
+    int pos = 0;        // neighbour faces consumed
+    bool pend = false;  // waits on the neighbour face at slot ps
+    int ps = 0;
+    while (__any(!done)) {
         bool progress = false;
-        bool blocked = false;
-        if (q < N && !give_up && pc >= 0) {
-            // a waiting lane polls only the value it waits on
-            const int F = pc / N, j = pc - F * N;
-            const T v = (F >= f0 && F < f1) ? lds_load(&ys[(F - f0) * N + j]) : coherent_load(out + pc);
-            if (is_sentinel(v)) {
-                blocked = true;
-            } else {
-                s -= pv * v;
-                k++;
-                pc = -1;
+        if (!done && pend && !give_up) {
+            bool ready = true;
+#pragma unroll
+            for (int j = 0; j < N; j++) ready = ready && !is_sentinel(lds_load(&ys[ps + j]));
+            pend = !ready;
+        }
+        if (!done && !pend && !give_up) {
+            // every neighbour's values at once (past the last neighbour: slot
+            // 0, not used), then consumed in CSR order up to the first face
+            // not ready, without branches
+            T vs[K][N];
+#pragma unroll
+            for (int b = 0; b < K; b++)
+#pragma unroll
+                for (int j = 0; j < N; j++) vs[b][j] = lds_load(&ys[slotF[b] + j]);
+            const int p0 = pos;
+            bool stop = false;
+#pragma unroll
+            for (int b = 0; b < K; b++) {
+                const bool act = !stop && b >= p0 && b < nb;
+                bool ready = true;
+#pragma unroll
+                for (int j = 0; j < N; j++) ready = ready && !is_sentinel(vs[b][j]);
+                const bool take = act && ready, blk = act && !ready;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const int row = UPPER ? N - 1 : i;
+                    T t = s[row];
+#pragma unroll
+                    for (int j = 0; j < N; j++) t -= lf[i][b][j] * vs[b][j];
+                    s[row] = take ? t : s[row];
+                }
+                pos += take ? 1 : 0;
+                ps = blk ? slotF[b] : ps;
+                stop = stop || blk;
+            }
+            // neighbours past the register window, from memory
+            while (!stop && pos >= K && pos < nb) {
+                const int sb = a.slot[nbase[0] + pos * N];
+                T lk[N][N];
+#pragma unroll
+                for (int i = 0; i < N; i++)
+#pragma unroll
+                    for (int j = 0; j < N; j++) lk[i][j] = lu[nbase[i] + pos * N + j];
+                // complete these loads here, so that no later write has to
+                // wait for memory
+                __builtin_amdgcn_s_waitcnt(0);
+                T v[N];
+                bool ready = true;
+#pragma unroll
+                for (int j = 0; j < N; j++) {
+                    v[j] = lds_load(&ys[sb + j]);
+                    ready = ready && !is_sentinel(v[j]);
+                }
+                if (!ready) {
+                    stop = true;
+                    ps = sb;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; i++) {
+                        if (UPPER && i != N - 1) continue;
+#pragma unroll
+                        for (int j = 0; j < N; j++) s[i] -= lk[i][j] * v[j];
+                    }
+                    pos++;
+                }
+            }
+            pend = stop;
+            progress = progress || pos != p0;
+            if (!stop) {
+                // every neighbour is in: the face's rows, in solve order
+                T y[N];
+                if (!UPPER) {
+#pragma unroll
+                    for (int i = 0; i < N; i++) {
+                        T t = s[i];
+#pragma unroll
+                        for (int j = 0; j < N; j++)
+                            if (j < i) t -= own[i][j] * y[j];
+                        y[i] = t;
+                    }
+                } else {
+                    y[N - 1] = s[N - 1] / dgv[N - 1];
+#pragma unroll
+                    for (int i = N - 2; i >= 0; i--) {
+                        T t = z[i];
+#pragma unroll
+                        for (int j = 0; j < N; j++)
+                            if (j > i) t -= own[i][j] * y[j];
+                        // the neighbour values, all final now
+#pragma unroll
+                        for (int b = 0; b < K; b++) {
+                            if (b >= nb) break;
+#pragma unroll
+                            for (int j = 0; j < N; j++)
+                                t -= upf[((i * K + b) * N + j) * ILU_WG + tid] * lds_load(&ys[slotF[b] + j]);
+                        }
+                        for (int b = K; b < nb; b++) {
+                            const int sb = a.slot[nbase[0] + b * N];
+#pragma unroll
+                            for (int j = 0; j < N; j++) t -= lu[nbase[i] + b * N + j] * lds_load(&ys[sb + j]);
+                        }
+                        y[i] = t / dgv[i];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    T val = y[i];
+                    if (is_sentinel(val)) val = __builtin_nan("");
+                    lds_store(&ys[me + i], val);
+                    coherent_store(out + E * N + i, val);
+                }
+                done = true;
                 progress = true;
             }
         }
-        while (q < N && !give_up && !blocked) {
-            const int i = UPPER ? N - 1 - q : q;
-            const int r = E * N + i;
-            if (k < 0) {
-                k = UPPER ? a.dg[r] + 1 : a.rp[r];
-                s = in[r];
-            }
-            const int kend = UPPER ? a.rp[r + 1] : a.dg[r];
-            // entries in batches: the indices, the factor values and then
-            // the needed values of a batch are each loaded together, and
-            // consumed in CSR order up to the first one not ready
-            while (k < kend) {
-                constexpr int B = 8;
-                int cols[B];
-                T vals[B], vs[B];
-#pragma unroll
-                for (int b = 0; b < B; b++) {
-                    const bool in_row = k + b < kend;
-                    cols[b] = in_row ? a.col[k + b] : -1;
-                    vals[b] = in_row ? lu[k + b] : T(0);
-                }
-#pragma unroll
-                for (int b = 0; b < B; b++) {
-                    const int cc = cols[b];
-                    T v = T(0);
-                    if (cc >= 0) {
-                        const int F = cc / N, j = cc - F * N;
-                        if (F == E) {
-                            v = yi[0];
-                            if (N > 1 && j == 1) v = yi[N > 1 ? 1 : 0];
-                            if (N > 2 && j == 2) v = yi[N > 2 ? 2 : 0];
-                        } else if (F >= f0 && F < f1) {
-                            v = lds_load(&ys[(F - f0) * N + j]);
-                        } else {
-                            v = coherent_load(out + cc);
-                        }
-                    }
-                    vs[b] = v;
-                }
-#pragma unroll
-                for (int b = 0; b < B; b++) {
-                    if (cols[b] < 0 || blocked) continue;
-                    if (is_sentinel(vs[b])) {
-                        blocked = true;
-                        pc = cols[b];
-                        pv = vals[b];
-                        continue;
-                    }
-                    s -= vals[b] * vs[b];
-                    k++;
-                    progress = true;
-                }
-                if (blocked) break;
-            }
-            if (blocked) break;
-            T val = s;
-            if (UPPER) val = s / lu[a.dg[r]];
-            if (is_sentinel(val)) val = __builtin_nan("");
-            yi[i] = val;
-            lds_store(&ys[(E - f0) * N + i], val);
-            coherent_store(out + r, val);
-            q++;
-            k = -1;
-            progress = true;
-        }
-        if (__any(progress)) {
-            backoff = 0;
-            continue;
-        }
-        // nothing moved in this wavefront.  Waits on the chunk's own faces
-        // (LDS) spin on; waits on other chunks back off, so that waiting
-        // wavefronts do not flood the memory system; give up past the limit
-        const bool waits_outside = pc >= 0 && !(pc / N >= f0 && pc / N < f1);
-        if (__any(waits_outside)) {
-            for (int b = 0; b < backoff; b++) __builtin_amdgcn_s_sleep(4);
-            backoff = min(2 * backoff + 1, ILU_BACKOFF_MAX);
-        } else {
-            // a short pause leaves the issue slots to the wavefronts that work
-            __builtin_amdgcn_s_sleep(1);
-        }
-        if (!give_up && (__builtin_amdgcn_s_memrealtime() - t0 > ILU_WAIT_TICKS || coherent_load(a.err) != 0))
+        if (__any(progress)) continue;
+        // nothing moved in this wavefront: a short pause leaves the issue
+        // slots to the wavefronts that work; give up past the limit
+        __builtin_amdgcn_s_sleep(1);
+        if (!give_up && (++idle & 31) == 0 &&
+            (__builtin_amdgcn_s_memrealtime() - t0 > ILU_WAIT_TICKS || coherent_load(a.err) != 0))
             give_up = true;
-        if (give_up) {
-            if (q < N) {
-                coherent_store(a.err, 1);
-                for (int g = 0; g < a.nseg; g++) a.sc[(long)g * NSLOTS + S_DONE] = 1.0;
-                // finish the rows with zeros so that nothing waits on them
+        if (give_up && !done) {
+            ilu_fail(a);
+            // finish the rows with zeros so that nothing waits on them
 #pragma unroll
-                for (int t = 0; t < N; t++) {
-                    if (t < q) continue;
-                    const int i = UPPER ? N - 1 - t : t;
-                    lds_store(&ys[(E - f0) * N + i], T(0));
-                    coherent_store(out + E * N + i, T(0));
-                }
-                q = N;
+            for (int i = 0; i < N; i++) {
+                lds_store(&ys[me + i], T(0));
+                coherent_store(out + E * N + i, T(0));
             }
+            done = true;
         }
     }
     if (E < f1) {
@@ -1043,6 +1223,8 @@ struct Ilu0 {
     DBuf<float> zmid32;
     DBuf<int> sweep_state;  // [0], [1]: tickets, [2]: error flag
     DBuf<int> chunk_order;  // start order of the chunks, lower then upper solve
+    DBuf<int> slot, imp_ptr, imp_rows;  // LDS slot of every factor entry, each chunk's imports
+    DBuf<int> lane_face;                // the face each lane of a chunk solves, lower then upper
     DBuf<double> a0, swp;  // ParILU: the matrix in CSR order, the other sweep buffer
     // ILUT: factors on their own pattern (rebuilt with the factors)
     bool own = false;
@@ -1351,6 +1533,116 @@ static int isai_tune_lanes(Ctx &c, const IsaiFactor &F, const double *x, double 
 // level tables of a factor pattern (sorted columns, diagonal present): the
 // rows of the lower and of the upper triangular solve bucketed by dependency
 // depth, and the diagonal positions; returns the latter
+// Tables of the single-launch triangular solves: for each chunk and triangle
+// the faces of other chunks it reads (its imports, all n rows of each:
+// ascending for the lower solve, descending for the upper one, the order in
+// which they are written), and for every factor entry the LDS slot its value
+// is read from (the chunk's own rows first, then its imports).  False, and
+// nothing uploaded, when the pattern is not made of full n x n face blocks
+// (the solve reads a neighbour face's n values together) or a chunk imports
+// more than ILU_IMPORTS values.
+static bool ilu_sweep_tables(const Ctx &c, Ilu0 &I, const std::vector<int> &dg)
+{
+    const int n = c.n, F = c.F, nch = (F + ILU_WG - 1) / ILU_WG;
+    const int *rp = c.h_rowptr.data(), *col = c.h_col.data();
+    // every row of face E: lower part [neighbours][own j < i][diagonal],
+    // upper part [diagonal][own j > i][neighbours], the same neighbour list
+    // (full blocks of ascending faces) in every row, and the face's own
+    // components either all there or none (rows decoupled: a Dirichlet face)
+    for (int E = 0; E < F; E++) {
+        const int r0 = E * n, r1 = E * n + n - 1;
+        const int nl = dg[r0] - rp[r0], nu = rp[r1 + 1] - dg[r1] - 1;
+        if (nl % n || nu % n) return false;
+        for (int part = 0; part < 2; part++) {
+            const int *list = col + (part == 0 ? rp[r0] : dg[r1] + 1), len = part == 0 ? nl : nu;
+            for (int q = 0; q < len; q++)
+                if (list[q] % n != q % n || list[q] / n != list[q - q % n] / n || list[q] / n == E) return false;
+        }
+        const bool own = dg[r1] - rp[r1] > nl;
+        for (int i = 0; i < n; i++) {
+            const int r = E * n + i, ol = own ? i : 0, ou = own ? n - 1 - i : 0;
+            if (dg[r] - rp[r] != nl + ol || rp[r + 1] - dg[r] - 1 != ou + nu) return false;
+            for (int q = 0; q < nl; q++)
+                if (col[rp[r] + q] != col[rp[r0] + q]) return false;
+            for (int j = 0; j < ol; j++)
+                if (col[rp[r] + nl + j] != E * n + j) return false;
+            for (int j = 1; j <= ou; j++)
+                if (col[dg[r] + j] != E * n + i + j) return false;
+            for (int q = 0; q < nu; q++)
+                if (col[dg[r] + 1 + ou + q] != col[dg[r1] + 1 + q]) return false;
+        }
+    }
+    std::vector<int> slot(c.nnz, 0), iptr(2 * ((size_t)nch + 1)), irows, mark(F, -1), where(F, 0);
+    std::vector<int> list;
+    for (int pass = 0; pass < 2; pass++) {
+        int *P = iptr.data() + (size_t)pass * (nch + 1);
+        for (int ch = 0; ch < nch; ch++) {
+            const int f0 = ch * ILU_WG, f1 = std::min(F, f0 + ILU_WG), key = pass * nch + ch;
+            P[ch] = (int)irows.size();
+            list.clear();
+            for (int r = f0 * n; r < f1 * n; r++) {
+                const int q0 = pass == 0 ? rp[r] : dg[r] + 1, q1 = pass == 0 ? dg[r] : rp[r + 1];
+                for (int q = q0; q < q1; q++) {
+                    const int G = col[q] / n;
+                    if (G >= f0 && G < f1) {
+                        slot[q] = col[q] - f0 * n;
+                    } else if (mark[G] != key) {
+                        mark[G] = key;
+                        list.push_back(G);
+                    }
+                }
+            }
+            if (list.size() * n > (size_t)ILU_IMPORTS) return false;
+            if (pass == 0) std::sort(list.begin(), list.end());
+            else std::sort(list.begin(), list.end(), std::greater<int>());
+            for (size_t e = 0; e < list.size(); e++) where[list[e]] = (int)e;
+            for (int r = f0 * n; r < f1 * n; r++) {
+                const int q0 = pass == 0 ? rp[r] : dg[r] + 1, q1 = pass == 0 ? dg[r] : rp[r + 1];
+                for (int q = q0; q < q1; q++) {
+                    const int G = col[q] / n;
+                    if (G < f0 || G >= f1) slot[q] = ILU_WG * n + where[G] * n + col[q] % n;
+                }
+            }
+            for (const int G : list)
+                for (int j = 0; j < n; j++) irows.push_back(G * n + j);
+        }
+        P[nch] = (int)irows.size();
+    }
+    // faces to lanes within a chunk by their level in the face dependency
+    // graph of each triangle, then by index
+    std::vector<int> lev(F), lanes(2 * (size_t)F);
+    for (int pass = 0; pass < 2; pass++) {
+        for (int t = 0; t < F; t++) {
+            const int E = pass == 0 ? t : F - 1 - t;
+            int l = 0;
+            for (int r = E * n; r < (E + 1) * n; r++) {
+                const int q0 = pass == 0 ? c.h_rowptr[r] : dg[r] + 1, q1 = pass == 0 ? dg[r] : c.h_rowptr[r + 1];
+                for (int q = q0; q < q1; q++) {
+                    const int G = c.h_col[q] / n;
+                    if (G != E) l = std::max(l, lev[G] + 1);
+                }
+            }
+            lev[E] = l;
+        }
+        int *L = lanes.data() + (size_t)pass * F;
+        for (int ch = 0; ch < nch; ch++) {
+            const int f0 = ch * ILU_WG, f1 = std::min(F, f0 + ILU_WG);
+            std::iota(L + f0, L + f1, 0);
+            std::stable_sort(L + f0, L + f1, [&](int x, int y) { return lev[f0 + x] < lev[f0 + y]; });
+        }
+    }
+    I.lane_face.alloc(lanes.size());
+    MHFEM_HIP(hipMemcpy(I.lane_face.p, lanes.data(), lanes.size() * sizeof(int), hipMemcpyHostToDevice));
+    I.slot.alloc(slot.size());
+    I.imp_ptr.alloc(iptr.size());
+    I.imp_rows.alloc(std::max<size_t>(irows.size(), 1));
+    MHFEM_HIP(hipMemcpy(I.slot.p, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice));
+    MHFEM_HIP(hipMemcpy(I.imp_ptr.p, iptr.data(), iptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!irows.empty())
+        MHFEM_HIP(hipMemcpy(I.imp_rows.p, irows.data(), irows.size() * sizeof(int), hipMemcpyHostToDevice));
+    return true;
+}
+
 static std::vector<int> build_levels(Ilu0 &I, int n, const int *rp, const int *col)
 {
     std::vector<int> dg(n), lev(n), ulev(n);
@@ -1529,7 +1821,7 @@ Krylov *krylov_create(Ctx &c)
         const std::vector<int> dg = build_levels(I, n, c.h_rowptr.data(), c.h_col.data());
         I.lu.alloc(c.nnz);
         if (c.cfg.precision != MHFEM_PRECISION_FP64) I.lu32.alloc(c.nnz);
-        if (c.cfg.preconditioner == MHFEM_PREC_ILU0) {
+        if (c.cfg.preconditioner == MHFEM_PREC_ILU0 && ilu_sweep_tables(c, I, dg)) {
             // single-launch triangular solves (k_ilu_sweep)
             I.zmid.alloc(n);
             if (c.cfg.precision != MHFEM_PRECISION_FP64) I.zmid32.alloc(n);
@@ -1777,13 +2069,14 @@ struct Solve {
             if constexpr (sizeof(T) == 8) z = I.zmid.p;
             else z = I.zmid32.p;
             const int nchunks = (c.F + ILU_WG - 1) / ILU_WG;
-            const IluSweep a{c.F, nchunks, rp, col, I.diag.p, I.chunk_order.p, I.sweep_state.p, I.sweep_state.p + 2,
-                             k.sc.p, k.nseg};
+            const IluSweep a{c.F, nchunks, rp, I.diag.p, I.slot.p, I.imp_ptr.p, I.imp_rows.p, I.lane_face.p,
+                             I.chunk_order.p, I.sweep_state.p, I.sweep_state.p + 2, k.sc.p, k.nseg};
             with_n(c.n, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_ilu_sweep<T, N, false>), nchunks, ILU_WG, 0, c.stream, a, lu, x, z, y, flag);
-                hipLaunchKernelGGL((k_ilu_sweep<T, N, true>), nchunks, ILU_WG, 0, c.stream, a, lu, (const T *)z, y,
-                                   z, flag);
+                hipLaunchKernelGGL((k_ilu_sweep<T, N, false>), nchunks, ILU_WG + ILU_NF, 0, c.stream, a, lu, x, z, y,
+                                   flag);
+                hipLaunchKernelGGL((k_ilu_sweep<T, N, true>), nchunks, ILU_WG + ILU_NF, 0, c.stream, a, lu,
+                                   (const T *)z, y, z, flag);
             });
             MHFEM_HIP(hipGetLastError());
             return;
@@ -2717,10 +3010,11 @@ static void precond_update(Ctx &c, Krylov &k)
         if (mixed)
             hipLaunchKernelGGL((k_convert<float, double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, I.lu.p, I.lu32.p);
         // the intermediate vectors start as sentinels, the tickets and the
-        // error flag at zero
-        hipLaunchKernelGGL((k_ilu_prepare<double>), grid_for(n), KB, 0, c.stream, (long)n, I.zmid.p,
-                           I.sweep_state.p, I.sweep_state.p + 2);
-        if (mixed)
+        // error flag at zero (single-launch solves only)
+        if (I.sweep_state.p)
+            hipLaunchKernelGGL((k_ilu_prepare<double>), grid_for(n), KB, 0, c.stream, (long)n, I.zmid.p,
+                               I.sweep_state.p, I.sweep_state.p + 2);
+        if (mixed && I.sweep_state.p)
             hipLaunchKernelGGL((k_ilu_prepare<float>), grid_for(n), KB, 0, c.stream, (long)n, I.zmid32.p,
                                I.sweep_state.p, I.sweep_state.p + 2);
         MHFEM_HIP(hipGetLastError());
