@@ -382,7 +382,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--history-depth", type=int, default=16, help="previous increments in the learned guess")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default rows per work-group")
-    ap.add_argument("--ilu-steps", type=int, default=2)
+    ap.add_argument("--ilu-steps", type=int, default=4)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-ilu", action="store_true")
