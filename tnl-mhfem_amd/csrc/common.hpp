@@ -27,7 +27,7 @@ void set_global_error(const std::string &msg);
 // device scalar slots of the Krylov solver (per segment)
 enum Slot {
     S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BETA, S_RAST_SQ, S_BNORM, S_RESID, S_TOL, S_IT,
-    S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, S_TOLABS, S_DELTA, NSLOTS = 32
+    S_DONE, S_RESTART, S_MAXIT, S_D0, S_D1, S_D2, S_D3, S_TOLABS, S_DELTA, S_BFIX, S_BREAK, NSLOTS = 32
 };
 // a segment's solve has stopped (converged, failed or waiting for a restart):
 // kernels queued behind it return at once

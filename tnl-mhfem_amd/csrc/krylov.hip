@@ -37,7 +37,8 @@ namespace mhfem {
 constexpr int KB = 256;
 constexpr int MAXG = 2048;
 
-enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT, FIN_REPLACE };
+enum Fin { FIN_NONE = 0, FIN_ALPHA, FIN_OMEGA, FIN_UPDATE, FIN_BNORM, FIN_INIT, FIN_REPLACE,
+           FIN_ALPHA_MERGED, FIN_OMEGA_MERGED, FIN_REPLACE_MERGED };
 // which segments a kernel works on
 enum Mode { M_ALL = 0, M_ACTIVE = 1, M_RESTART = 2 };
 
@@ -107,6 +108,7 @@ __device__ void finalize(double *sc, int fin)
         break;
     }
     case FIN_INIT: {
+        sc[S_BREAK] = 0;
         sc[S_RHO] = sc[S_D0];
         sc[S_RAST_SQ] = sc[S_D0];
         const double res = sqrt(sc[S_D0]) / sc[S_BNORM];
@@ -132,6 +134,73 @@ __device__ void finalize(double *sc, int fin)
         sc[S_RESTART] = (rn > 0.0 && !(fabs(sc[S_RHO]) >= eps2 * sc[S_RAST_SQ])) ? 1 : 0;
         break;
     }
+    // The merged iteration (FP32 inner solves): one reduction fewer per
+    // iteration.  The convergence test on (r, r), which the x/r/p update of
+    // the previous iteration summed (D1), comes with alpha (D0 = (r*, A p)).
+    case FIN_ALPHA_MERGED: {
+        const double res = sqrt(sc[S_D1]) / sc[S_BNORM];
+        sc[S_RESID] = res;
+        if (!(res >= sc[S_TOL]) || sc[S_IT] >= sc[S_MAXIT]) {
+            sc[S_DONE] = 1;
+            break;
+        }
+        if (sc[S_BREAK] != 0.0) {
+            // the previous iteration broke down: restart from its x
+            sc[S_BREAK] = 0;
+            sc[S_RESTART] = 1;
+            break;
+        }
+        const double a = sc[S_RHO] / sc[S_D0];
+        if (!(fabs(sc[S_D0]) > 0.0) || !isfinite(a)) {
+            sc[S_RESTART] = 1;
+            sc[S_IT] += 1;
+            if (sc[S_IT] >= sc[S_MAXIT]) sc[S_DONE] = 1;
+        } else {
+            sc[S_ALPHA] = a;
+        }
+        break;
+    }
+    // omega = (As, s) / (As, As); the next rho = (r, r*) with r = s - omega As
+    // is (s, r*) - omega (As, r*) (D3, D2), so beta is known before r is
+    // formed and p is updated in the same pass as x and r
+    case FIN_OMEGA_MERGED: {
+        const double om = sc[S_D1] > 0 ? sc[S_D0] / sc[S_D1] : 0.0;
+        const double rho = sc[S_D3] - om * sc[S_D2];
+        sc[S_OMEGA] = om;
+        sc[S_IT] += 1;
+        sc[S_RHO_OLD] = sc[S_RHO];
+        sc[S_RHO] = rho;
+        if (!(fabs(rho) >= eps2 * sc[S_RAST_SQ]) || om == 0.0) {
+            // breakdown: x and r are still updated (p = r), then the next
+            // alpha step tests convergence and otherwise asks the host to
+            // recompute the residual and restart
+            sc[S_BETA] = 0;
+            sc[S_BREAK] = 1;
+        } else {
+            sc[S_BETA] = (rho / sc[S_RHO_OLD]) * (sc[S_ALPHA] / om);
+        }
+        break;
+    }
+    // reliable update in the merged iteration (D0 = D1 = (r, r), D2 = (r, r*)
+    // of the replaced residual).  p was already advanced with the recursive
+    // residual: p = r_rec + beta d, d = p_old - omega A p_old.  The
+    // continued direction is r + beta' d = r + (beta' / beta) (p - r_rec);
+    // S_BFIX is that ratio (0: the direction restarts from r)
+    case FIN_REPLACE_MERGED: {
+        const double rn = sqrt(sc[S_D0]);
+        const double beta_old = sc[S_BETA];
+        sc[S_RHO] = sc[S_D2];
+        sc[S_BETA] = (sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]);
+        const double fix = sc[S_BETA] / beta_old;
+        sc[S_BFIX] = isfinite(fix) ? fix : 0.0;
+        sc[S_BREAK] = 0;
+        sc[S_BNORM] = rn;
+        sc[S_TOL] = rn > 0 ? fmax(sc[S_DELTA], sc[S_TOLABS] / rn) : 1.0;
+        sc[S_RESID] = 1.0;
+        sc[S_DONE] = rn == 0.0 ? 1 : 0;
+        sc[S_RESTART] = (rn > 0.0 && !(fabs(sc[S_RHO]) >= eps2 * sc[S_RAST_SQ])) ? 1 : 0;
+        break;
+    }
     default: break;
     }
 }
@@ -153,7 +222,7 @@ __device__ __forceinline__ int xcd_block(int nblk)
     return k * q + (k < r ? k : r) + j;
 }
 
-template <int ND>
+template <int ND, int OFF = 0>
 __device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg, int blk)
 {
     __shared__ double sm[ND][KB / 64];
@@ -170,7 +239,7 @@ __device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg, in
             double s = sm[d][0];
 #pragma unroll
             for (int q = 1; q < KB / 64; q++) s += sm[d][q];
-            sg.partials[(long)d * sg.nblk + blk] = s;
+            sg.partials[(long)(d + OFF) * sg.nblk + blk] = s;
         }
     }
 }
@@ -314,11 +383,13 @@ static void with_n(int n, F &&f)
     if (!seg_selected(sc_, (mode))) return;                            \
     const int row0_ = (sg).blk_row[blk_], row1_ = (sg).blk_row[blk_ + 1]
 
-// y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y).  A
-// work-group's row range holds whole faces (build_blocks).
+// y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y), and
+// for ND = 4 also (y, w), (z, w).  A work-group's row range holds whole
+// faces (build_blocks).
 template <typename T, int N, bool DIAG, int ND>
 __global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
-                                              const T *__restrict__ dinv, T *__restrict__ y, const T *__restrict__ z)
+                                              const T *__restrict__ dinv, T *__restrict__ y, const T *__restrict__ z,
+                                              const T *__restrict__ w)
 {
     SEG_BLOCK_PROLOGUE(sg, mode);
     double acc[ND > 0 ? ND : 1] = {};
@@ -338,6 +409,14 @@ __global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, c
         if constexpr (ND >= 2) {
 #pragma unroll
             for (int i = 0; i < N; i++) acc[1] += (double)v.v[i] * (double)v.v[i];
+        }
+        if constexpr (ND >= 4) {
+            const Pack<T, N> zz = ld<T, N>(z + (int64_t)f * N), ww = ld<T, N>(w + (int64_t)f * N);
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                acc[2] += (double)v.v[i] * (double)ww.v[i];
+                acc[3] += (double)zz.v[i] * (double)ww.v[i];
+            }
         }
     }
     if constexpr (ND > 0) block_partial<ND>(acc, sg, blk_);
@@ -435,6 +514,46 @@ __global__ void __launch_bounds__(KB) k_bicg_p(Seg sg, const T *__restrict__ r, 
     SEG_BLOCK_PROLOGUE(sg, M_ACTIVE);
     const T beta = (T)sc_[S_BETA], omega = (T)sc_[S_OMEGA];
     for (int i = row0_ + threadIdx.x; i < row1_; i += KB) p[i] = r[i] + beta * (p[i] - omega * Ap[i]);
+}
+
+// merged iteration: x += alpha p + omega s; r = s - omega As;
+// p = r + beta (p - omega Ap); dot (r, r) into partial row 1 (read with
+// the next alpha)
+template <typename T>
+__global__ void __launch_bounds__(KB) k_bicg_xrp(Seg sg, T *__restrict__ x, T *__restrict__ p,
+                                                 const T *__restrict__ s, const T *__restrict__ As, T *__restrict__ r,
+                                                 const T *__restrict__ Ap)
+{
+    SEG_BLOCK_PROLOGUE(sg, M_ACTIVE);
+    const T alpha = (T)sc_[S_ALPHA], omega = (T)sc_[S_OMEGA], beta = (T)sc_[S_BETA];
+    double acc[1] = {0};
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) {
+        const T si = s[i], pi = p[i];
+        x[i] += alpha * pi + omega * si;
+        const T ri = si - omega * As[i];
+        r[i] = ri;
+        p[i] = ri + beta * (pi - omega * Ap[i]);
+        acc[0] += (double)ri * (double)ri;
+    }
+    block_partial<1, 1>(acc, sg, blk_);
+}
+
+// merged iteration, after a reliable update: q = p - r (before r is
+// replaced), then p = r + S_BFIX q (after)
+template <typename T>
+__global__ void __launch_bounds__(KB) k_bicg_dir(Seg sg, const T *__restrict__ p, const T *__restrict__ r,
+                                                 T *__restrict__ q)
+{
+    SEG_BLOCK_PROLOGUE(sg, M_ALL);
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) q[i] = p[i] - r[i];
+}
+template <typename T>
+__global__ void __launch_bounds__(KB) k_bicg_pfix(Seg sg, const T *__restrict__ r, T *__restrict__ p,
+                                                  const T *__restrict__ q)
+{
+    SEG_BLOCK_PROLOGUE(sg, M_ACTIVE);
+    const T f = (T)sc_[S_BFIX];
+    for (int i = row0_ + threadIdx.x; i < row1_; i += KB) p[i] = r[i] + f * q[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1390,7 +1509,7 @@ static void build_blocks(Ctx &c, Krylov &k, int rpb)
     MHFEM_HIP(hipMemcpy(k.blk_seg.p, bseg.data(), k.nblk * sizeof(int), hipMemcpyHostToDevice));
     MHFEM_HIP(hipMemcpy(k.blk_row.p, brow.data(), (k.nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
     MHFEM_HIP(hipMemcpy(k.seg_blk.p, sblk.data(), (k.nseg + 1) * sizeof(int), hipMemcpyHostToDevice));
-    k.partials.alloc((size_t)2 * k.nblk);
+    k.partials.alloc((size_t)4 * k.nblk);
 }
 
 // Face order of the FP32 iteration's matrix copy.  The natural
@@ -1948,13 +2067,13 @@ struct Solve {
 
     // launches of the matrix kernels, dispatched on the number of equations
     template <typename T, bool DIAG, int ND>
-    void sspmv(const T *x, const T *dinv, T *y, const T *z, int mode)
+    void sspmv(const T *x, const T *dinv, T *y, const T *z, int mode, const T *w = nullptr)
     {
         SellView<T> A = matrix<T>();
         const bool timed = kernel_timing_begin(c);
         with_n(c.n, [&](auto NC) {
             hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND>), k.nblk, KB, 0, c.stream, A, sg, mode, x,
-                               dinv, y, z);
+                               dinv, y, z, w);
         });
         MHFEM_HIP(hipGetLastError());
         if (timed) kernel_timing_end(c);
@@ -2040,11 +2159,19 @@ struct Solve {
         hipLaunchKernelGGL((k_finalize<ND>), k.nseg, KB, 0, c.stream, sg, fin, mode);
         MHFEM_HIP(hipGetLastError());
     }
-    template <typename T, int ND>
+    // NF: partial rows the finalize sums (ND, or more when earlier kernels
+    // left their partials in the rows past ND)
+    template <typename T, int ND, int NF = ND>
     void dots(DotArgs<T, ND> a, int fin, int mode)
     {
         hipLaunchKernelGGL((k_sdots<T, ND>), k.nblk, KB, 0, c.stream, sg, mode, a);
-        finish<ND>(fin, mode);
+        finish<NF>(fin, mode);
+    }
+    // FP32 inner solves run the merged BiCGStab iteration
+    template <typename T>
+    static constexpr bool merged()
+    {
+        return sizeof(T) == 4;
     }
 
     // The assembled MHFEM rows are scaled so that every diagonal entry is
@@ -2141,21 +2268,22 @@ struct Solve {
     }
 
     // y = M^{-1} A x on the active segments; ND fused dots: 1 -> (y, z),
-    // 2 -> (y, z), (y, y); then `fin`
-    template <typename T, int ND>
-    void pmatvec(const T *x, T *y, T *tmp, const T *dinv, const T *z, int fin)
+    // 2 -> (y, z), (y, y), 4 -> (y, z), (y, y), (y, w), (z, w); then `fin`
+    template <typename T, int ND, int NF = ND>
+    void pmatvec(const T *x, T *y, T *tmp, const T *dinv, const T *z, int fin, const T *w = nullptr)
     {
         if (separate_prec()) {
             sspmv<T, false, 0>(x, (const T *)nullptr, tmp, (const T *)nullptr, M_ACTIVE);
             separate_apply<T>(tmp, y, true);
-            if constexpr (ND == 1) dots<T, 1>(DotArgs<T, 1>{{y}, {z}}, fin, M_ACTIVE);
+            if constexpr (ND == 1) dots<T, 1, NF>(DotArgs<T, 1>{{y}, {z}}, fin, M_ACTIVE);
             if constexpr (ND == 2) dots<T, 2>(DotArgs<T, 2>{{y, y}, {z, y}}, fin, M_ACTIVE);
+            if constexpr (ND == 4) dots<T, 4>(DotArgs<T, 4>{{y, y, y, z}, {z, y, w, w}}, fin, M_ACTIVE);
         } else {
             if (jacobi_scales())
-                sspmv<T, true, ND>(x, dinv, y, z, M_ACTIVE);
+                sspmv<T, true, ND>(x, dinv, y, z, M_ACTIVE, w);
             else
-                sspmv<T, false, ND>(x, dinv, y, z, M_ACTIVE);
-            if constexpr (ND > 0) finish<ND>(fin, M_ACTIVE);
+                sspmv<T, false, ND>(x, dinv, y, z, M_ACTIVE, w);
+            if constexpr (ND > 0) finish<NF>(fin, M_ACTIVE);
         }
         MHFEM_HIP(hipGetLastError());
     }
@@ -2198,7 +2326,10 @@ struct Solve {
         presidue<T>(x, b, r, (T *)v.tmp, (const T *)v.dinv, mode, (T *)v.tmp2);
         hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, r, (T *)v.p, (T *)v.r_ast);
         MHFEM_HIP(hipGetLastError());
-        dots<T, 1>(DotArgs<T, 1>{{r}, {(T *)v.r_ast}}, FIN_INIT, mode);
+        if constexpr (merged<T>())  // (r, r) also in partial row 1, for the first merged alpha
+            dots<T, 2>(DotArgs<T, 2>{{r, r}, {(T *)v.r_ast, r}}, FIN_INIT, mode);
+        else
+            dots<T, 1>(DotArgs<T, 1>{{r}, {(T *)v.r_ast}}, FIN_INIT, mode);
     }
     template <typename T>
     void bicg_iterations(T *x, const Vecs &v, int count)
@@ -2207,12 +2338,22 @@ struct Solve {
           *tmp = (T *)v.tmp;
         const T *dinv = (const T *)v.dinv;
         for (int q = 0; q < count; q++) {
-            pmatvec<T, 1>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA);
-            hipLaunchKernelGGL((k_bicg_s<T>), k.nblk, KB, 0, c.stream, sg, r, Ap, s);
-            pmatvec<T, 2>(s, As, tmp, dinv, s, FIN_OMEGA);
-            hipLaunchKernelGGL((k_bicg_xr<T>), k.nblk, KB, 0, c.stream, sg, x, p, s, As, r, r_ast);
-            finish<2>(FIN_UPDATE, M_ACTIVE);
-            hipLaunchKernelGGL((k_bicg_p<T>), k.nblk, KB, 0, c.stream, sg, r, p, Ap);
+            if constexpr (merged<T>()) {
+                // alpha with the previous iteration's convergence test;
+                // omega and beta from the second product's four dots; x, r
+                // and p in one pass
+                pmatvec<T, 1, 2>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA_MERGED);
+                hipLaunchKernelGGL((k_bicg_s<T>), k.nblk, KB, 0, c.stream, sg, r, Ap, s);
+                pmatvec<T, 4>(s, As, tmp, dinv, s, FIN_OMEGA_MERGED, r_ast);
+                hipLaunchKernelGGL((k_bicg_xrp<T>), k.nblk, KB, 0, c.stream, sg, x, p, s, As, r, Ap);
+            } else {
+                pmatvec<T, 1>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA);
+                hipLaunchKernelGGL((k_bicg_s<T>), k.nblk, KB, 0, c.stream, sg, r, Ap, s);
+                pmatvec<T, 2>(s, As, tmp, dinv, s, FIN_OMEGA);
+                hipLaunchKernelGGL((k_bicg_xr<T>), k.nblk, KB, 0, c.stream, sg, x, p, s, As, r, r_ast);
+                finish<2>(FIN_UPDATE, M_ACTIVE);
+                hipLaunchKernelGGL((k_bicg_p<T>), k.nblk, KB, 0, c.stream, sg, r, p, Ap);
+            }
             MHFEM_HIP(hipGetLastError());
         }
     }
@@ -3331,10 +3472,16 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                     break;
                 }
                 S.add_correction(x, k.x32.p);
+                // the merged iteration has advanced p with the recursive
+                // residual: keep p - r for the continued direction
+                hipLaunchKernelGGL((k_bicg_dir<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)k.p32.p,
+                                   (const float *)r, k.s32.p);
                 S.residual_mixed(x, b, rhs32);
                 MHFEM_HIP(hipGetLastError());
                 S.prec<float>(rhs32, r, k.dinv32.p, M_ALL);
-                S.dots<float, 2>(DotArgs<float, 2>{{r, r}, {r, k.r_ast32.p}}, FIN_REPLACE, M_ALL);
+                // D0 = (r, r) (read below), (r, r) again in partial row 1 for
+                // the next merged alpha, D2 = (r, r*)
+                S.dots<float, 3>(DotArgs<float, 3>{{r, r, r}, {r, r, k.r_ast32.p}}, FIN_REPLACE_MERGED, M_ALL);
                 S.poll();
                 outer++;
                 const double prev = res;
@@ -3350,8 +3497,8 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                 }
                 hipLaunchKernelGGL(k_set_slot, 1, 64, 0, c.stream, k.sc.p, (int)S_MAXIT,
                                    (double)std::min(maxit, total + round_cap));
-                hipLaunchKernelGGL((k_bicg_p<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)r, k.p32.p,
-                                   (const float *)k.Ap32.p);
+                hipLaunchKernelGGL((k_bicg_pfix<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)r, k.p32.p,
+                                   (const float *)k.s32.p);
                 MHFEM_HIP(hipGetLastError());
             }
             break;
