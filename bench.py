@@ -84,16 +84,18 @@ def spmv_bytes(rows, nnz, fp32):
     return 12 * nnz + 4 * (rows + 1) + 16 * rows
 
 
-def block_sell_bytes(n, faces, blocks_per_face, fp32, dot_operand=False):
+def block_sell_bytes(n, faces, blocks_per_face, fp32, dot_operands=0.0):
     """Bytes the block-SELL SpMV must move for the same product: per stored
     off-diagonal block n*n values and one column index, per face the n(n-1)
-    off-diagonal entries of its own block, x read once, y written once; with
-    `dot_operand`, the vector the fused dot product reads (the iteration's
-    SpMV launches fuse (A x, z) into the product and read z once more)."""
+    off-diagonal entries of its own block, x read once, y written once, and
+    `dot_operands` vectors the fused dot products read.  The iteration's two
+    launches fuse their dots into the product: the FP64 iteration's read one
+    more vector each (r* and s); the merged FP32 iteration's read r* and
+    s, r* (1.5 on average over the alternating samples)."""
     w = 4 if fp32 else 8
     rows = n * faces
-    return (blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
-            + (rows * w if dot_operand else 0))
+    return int(blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
+               + dot_operands * rows * w)
 
 
 def assembly_bytes(n, fpc, cells, faces, nnz):
@@ -569,9 +571,10 @@ def main():
 
     value = args.steps / dt
     csr = spmv_bytes(rows, nnz, True)
-    # the timed launches are the iteration's SpMV kernels with their fused dot product
-    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True, dot_operand=True)
-    fmt64 = block_sell_bytes(pr.n, faces, blocks_per_face, False, dot_operand=True)
+    # the timed launches are the iteration's SpMV kernels with their fused
+    # dot products, sampled alternately from its two products
+    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True, dot_operands=1.5)
+    fmt64 = block_sell_bytes(pr.n, faces, blocks_per_face, False, dot_operands=1.0)
     # priced on the bytes the block-SELL kernel must move; the CSR count of
     # the same product is reported beside it
     achieved = fmt / (spmv_ms * 1e-3) / 1e9

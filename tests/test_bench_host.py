@@ -28,8 +28,9 @@ def test_spmv_byte_counts():
     faces = 3170304
     got = bench.block_sell_bytes(2, faces, 6.0, True)
     assert got == 6 * faces * (16 + 4) + faces * 2 * 4 + 2 * 2 * faces * 4
-    # the iteration's launches also read the fused dot product's operand once
-    assert bench.block_sell_bytes(2, faces, 6.0, True, dot_operand=True) == got + 2 * faces * 4
+    # the iteration's launches also read the fused dot products' operands
+    assert bench.block_sell_bytes(2, faces, 6.0, True, dot_operands=1.0) == got + 2 * faces * 4
+    assert bench.block_sell_bytes(2, faces, 6.0, True, dot_operands=1.5) == got + 3 * faces * 4
     assert bench.block_sell_bytes(2, faces, 6.0, False) == 6 * faces * (32 + 4) + faces * 2 * 8 + 2 * 2 * faces * 8
 
 

@@ -299,7 +299,8 @@ struct Ctx {
 // Sampled kernel timing: kernel_timing_begin returns true when this launch
 // is to be timed (then it has recorded the opening event), and
 // kernel_timing_end records the closing one; the durations are read out by
-// kernel_timing_flush.
+// kernel_timing_flush.  Every 9th launch is timed: an odd period, so that
+// the samples alternate between the two products of a BiCGStab iteration.
 inline void kernel_timing_flush(Ctx &c)
 {
     for (size_t q = 0; q + 1 < c.kt_used; q += 2) {
@@ -329,7 +330,7 @@ inline void kernel_timing_summary(const Ctx &c, double &ms_sum, long long &count
 }
 inline bool kernel_timing_begin(Ctx &c)
 {
-    if (!c.timing || c.timing_level < 2 || (c.kt_count++ & 7) != 0) return false;
+    if (!c.timing || c.timing_level < 2 || c.kt_count++ % 9 != 0) return false;
     if (c.kt_used + 2 > c.kt_ev.size()) {
         if (c.kt_ev.size() < 512) {
             for (int q = 0; q < 64; q++) {
