@@ -398,6 +398,45 @@ def test_ilu0_step_parity():
         assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10
 
 
+def _max_chunk_imports(rp, col, n, chunk=512):
+    """The largest number of values a chunk of `chunk` consecutive faces reads
+    from other chunks in either triangle (whole faces: n values each), as the
+    library counts them for its single-launch ILU(0) solves."""
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    mx = 0
+    for part in (col < rows, col > rows):
+        own = rows[part] // n // chunk
+        face = col[part] // n
+        ext = face // chunk != own
+        key = np.unique(own[ext].astype(np.int64) * (len(rp) - 1) + face[ext])
+        mx = max(mx, int(np.bincount(key // (len(rp) - 1)).max()) * n)
+    return mx
+
+
+@pytest.mark.parametrize("shuffle", [None, 5])
+def test_ilu0_level_launches_when_chunks_import_too_much(shuffle):
+    """With the cells in random order the faces are numbered all over the
+    mesh, a chunk of 512 faces reads more than the 1,024 values of other
+    chunks its LDS holds, and the triangular solves run as one launch per
+    dependency level; in the natural order they run as one launch each.
+    Both agree with the oracle."""
+    pr = problems.darcy_3d(s=8, steps=2, preconditioner="ilu0", shuffle_cells=shuffle)
+    o = pyoracle.Oracle(pr)
+    rp, col, _, _ = o.csr()
+    imports = _max_chunk_imports(np.asarray(rp), np.asarray(col), pr.n)
+    assert (imports > 1024) == (shuffle is not None), imports
+    print(f"\n[ilu0 chunk imports] shuffle={shuffle} max imports {imports}", flush=True)
+    s = capi.Solver(pr, tol=1e-13, check_every=1)
+    t = 0.0
+    for k in range(pr.steps):
+        s.step(t, pr.dt)
+        it, _ = o.step(t, pr.dt, tol=1e-13)
+        assert abs(s.iterations[-1] - it) <= 2
+        t += pr.dt
+        assert rel(s.get_array("Z_iF"), o.array("Z_iF")) < 1e-10
+    s.close()
+
+
 @pytest.mark.parametrize("n,precision", [(1, "fp64"), (2, "fp64"), (3, "fp64"), (2, "fp32-reliable")])
 def test_ilu0_sweeps_on_many_chunks(n, precision):
     """The single-launch ILU(0) triangular solves (one wavefront per chunk of

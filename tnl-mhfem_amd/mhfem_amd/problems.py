@@ -137,15 +137,20 @@ def _cube_lnK(s, sigma, corr_len, seed):
 
 
 def darcy_3d(s: int = 64, seed: int = 1, dt: float = 1e-3, steps: int = 20,
-             preconditioner: str = "ilu0") -> Problem:
+             preconditioner: str = "ilu0", shuffle_cells: Optional[int] = None) -> Problem:
     """3-D Darcy-type pressure on the unit cube, Kuhn tetrahedra: n=1, N=0.2,
     m=1, D=K with ln K a seeded Gaussian field (exponential covariance,
     sigma=1, correlation length 0.1, constant per cube); Z=1 at x=0, Z=0 at
-    x=1, no-flow (FixedFluxNoAdvection, J=0) elsewhere; Z0=0."""
+    x=1, no-flow (FixedFluxNoAdvection, J=0) elsewhere; Z0=0.
+    shuffle_cells: the cells (and with them the face numbering, faces being
+    numbered in order of first appearance) in a random order of that seed."""
     points, cells = meshgen.unit_cube_kuhn(s)
+    K = np.exp(_cube_lnK(s, 1.0, 0.1, seed))
+    if shuffle_cells is not None:
+        perm = np.random.default_rng(shuffle_cells).permutation(cells.shape[0])
+        cells, K = np.ascontiguousarray(cells[perm]), K[perm]
     cf, fc = _topology(points, cells)
     F, C = fc.shape[0], cells.shape[0]
-    K = np.exp(_cube_lnK(s, 1.0, 0.1, seed))
     coeffs = dict(N_ijK=np.array(0.2), u_ijKe=np.array(0.0), m_iK=np.array(1.0), D_ijK=K,
                   w_iKe=np.array(0.0), a_ijKe=np.array(0.0), r_ijK=np.array(0.0),
                   f_iK=np.array(0.0))
