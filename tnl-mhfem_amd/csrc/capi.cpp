@@ -466,15 +466,19 @@ mhfem_ctx *mhfem_create_cells(const mhfem_mesh_desc *mesh, int vertices_per_cell
         const size_t Cn = c->C, fpc = c->fpc, nvc = c->nvc;
         // measures
         std::vector<double> cm(Cn), fm(c->F, 0.0);
+        c->h_face_pos.assign(3 * (size_t)c->F, 0.0f);
         for (size_t K = 0; K < Cn; K++) {
-            double P[24];
+            double P[24], lowc[3] = {0, 0, 0};
             for (size_t k = 0; k < nvc; k++)
-                for (int d = 0; d < c->dim; d++)
+                for (int d = 0; d < c->dim; d++) {
                     P[k * c->dim + d] = mesh->points[(size_t)mesh->cell_vertices[K * nvc + k] * c->dim + d];
+                    lowc[d] = k == 0 ? P[d] : std::min(lowc[d], P[k * c->dim + d]);
+                }
             cm[K] = c->box ? box_measure(c->dim, P) : simplex_measure(c->dim, P);
             for (size_t e = 0; e < fpc; e++) {
                 const int E = c->h_cell_faces[K * fpc + e];
                 if (c->h_face_cells[2 * (size_t)E] != (int)K) continue;
+                for (int d = 0; d < 3; d++) c->h_face_pos[3 * (size_t)E + d] = (float)lowc[d];
                 double Q[12];
                 int lv[4];
                 const int t = face_vertices(c->dim, c->box, (int)e, lv);

@@ -235,8 +235,11 @@ struct Ctx {
     int lumping = 0, advection = 0, do_mob = 1;
     int device = 0;  // the HIP device current at creation
     hipStream_t stream = nullptr;
-    // host topology
+    // host topology; position of each face (the lowest corner of its first
+    // cell's bounding box, 3 floats; empty for batched instances), for the
+    // ILU(0) solves' chunks
     std::vector<int> h_cell_faces, h_face_cells;
+    std::vector<float> h_face_pos;
     // segments (batched ensembles): cells, faces and rows of each instance
     std::vector<int> h_seg_cells, h_seg_faces, h_seg_rows;
     // device topology
