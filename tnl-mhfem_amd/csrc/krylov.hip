@@ -944,6 +944,23 @@ constexpr int ilu_window()
 {
     return N == 1 ? 8 : N == 2 ? (UPPER ? 6 : 5) : 4;
 }
+// LDS a work-group holds at least: one chunk per CU (160 KiB).  More
+// resident chunks only add wavefronts that poll and share the SIMDs with the
+// ones on the critical path.  Per step at 64^3 (plate chunks): n = 2 FP64
+// 714 ms with two chunks per CU, 634 ms with one; FP32 662 and 570 ms (924 ms
+// with the three its registers allowed); n = 1 516 and 413 ms.  With one per
+// CU, chunks of 384, 768 or 960 faces were slower than 512 (704, 672, 656 ms).
+#ifndef MHFEM_ILU_MIN_LDS
+#define MHFEM_ILU_MIN_LDS 83000
+#endif
+template <typename T, int N, bool UPPER>
+constexpr int ilu_lds_values()
+{
+    constexpr int used = (ILU_WG * N + ILU_IMPORTS) * (int)sizeof(T) +
+                         (UPPER ? (N - 1) * ilu_window<N, UPPER>() * N * ILU_WG * (int)sizeof(T) : 0);
+    constexpr int pad = used < MHFEM_ILU_MIN_LDS ? (MHFEM_ILU_MIN_LDS - used) / (int)sizeof(T) : 0;
+    return ILU_WG * N + ILU_IMPORTS + pad;
+}
 
 template <typename T>
 struct SentinelBits;
@@ -1030,7 +1047,9 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
                                                                T *reset, const double *sc)
 {
     if (sc && halted(sc)) return;
-    __shared__ T ys[ILU_WG * N + ILU_IMPORTS];
+    // the values (and, past them, the padding that holds the work-groups
+    // per CU to one; one store keeps it from being taken out)
+    __shared__ T ys[ilu_lds_values<T, N, UPPER>()];
     __shared__ int s_ticket;
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -1039,6 +1058,8 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
         if (s_ticket == 0) a.ticket[UPPER ? 0 : 1] = 0;
     }
     for (int i = tid; i < ILU_WG * N + ILU_IMPORTS; i += ILU_WG + ILU_NF) ys[i] = sentinel_value<T>();
+    if (ilu_lds_values<T, N, UPPER>() > ILU_WG * N + ILU_IMPORTS && tid == 0)
+        ys[ilu_lds_values<T, N, UPPER>() - 1] = T(0);
     __syncthreads();
     // tickets past the chunk count only follow a failed wait (the counter
     // was not reset): nothing to do
