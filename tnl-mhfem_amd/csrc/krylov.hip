@@ -824,6 +824,120 @@ __global__ void __launch_bounds__(KB) k_ilu_factor(const int *__restrict__ rows,
     }
 }
 
+// The same IKJ factorization of a level's rows with FG lanes per row: lane l
+// holds the row's entries l, l + FG, ... (FS slots), all pivots' diagonal
+// values and the first FG entries of their rows' upper parts are loaded up
+// front (the pivot columns are the row's own lower entries, and those rows
+// lie in earlier levels), and each update a_rj -= piv * a_cj goes to the lane
+// that holds column j, found by comparing the columns passed round the group.
+// Every entry receives the same updates in the same order as in
+// k_ilu_factor, so the factors are the same to the last bit.  Pivots past
+// ILU_FP, or pivot rows with more than FG upper entries, are loaded when
+// they are reached.
+constexpr int ILU_FG = 16;
+constexpr int ILU_FP = 12;
+template <int FS>
+__device__ __forceinline__ void ilu_factor_pivot(int l, int cp, double pd, int uc, double uv, int u1, int ufirst,
+                                                 const int *__restrict__ col, const double *a, int (&mc)[FS],
+                                                 double (&mv)[FS], int p)
+{
+    // the pivot value, as updated by the earlier pivots, from its lane
+    double vk = 0.0;
+#pragma unroll
+    for (int t = 0; t < FS; t++)
+        if (t == p / ILU_FG) vk = mv[t];
+    vk = __shfl(vk, p % ILU_FG, ILU_FG);
+    const double piv = vk / pd;
+    if (l == p % ILU_FG) {
+#pragma unroll
+        for (int t = 0; t < FS; t++)
+            if (t == p / ILU_FG) mv[t] = piv;
+    }
+    // the pivot row's upper entries, ILU_FG at a time
+    for (int ub = ufirst;; ub += ILU_FG) {
+#pragma unroll
+        for (int src = 0; src < ILU_FG; src++) {
+            const int c2 = __shfl(uc, src, ILU_FG);
+            const double v2 = __shfl(uv, src, ILU_FG);
+#pragma unroll
+            for (int t = 0; t < FS; t++)
+                if (mc[t] == c2) mv[t] -= piv * v2;
+        }
+        if (ub + ILU_FG >= u1) break;
+        const int q = ub + ILU_FG + l;
+        uc = q < u1 ? col[q] : -2;
+        uv = q < u1 ? a[q] : 0.0;
+    }
+    (void)cp;
+}
+
+template <int FS>
+__global__ void __launch_bounds__(KB) k_ilu_factor_rows(const int *__restrict__ rows, int count,
+                                                        const int *__restrict__ rp, const int *__restrict__ col,
+                                                        const int *__restrict__ dg, double *__restrict__ a)
+{
+    const int t = blockIdx.x * KB + threadIdx.x;
+    const int g = t / ILU_FG, l = t % ILU_FG;
+    if (g >= count) return;  // whole groups: KB is a multiple of ILU_FG
+    const int r = rows[g], b0 = rp[r], b1 = rp[r + 1], np = dg[r] - b0;
+    int mc[FS];
+    double mv[FS];
+#pragma unroll
+    for (int s = 0; s < FS; s++) {
+        const int q = b0 + s * ILU_FG + l;
+        mc[s] = q < b1 ? col[q] : -1;  // -1: no entry; a pivot row's padding is -2
+        mv[s] = q < b1 ? a[q] : 0.0;
+    }
+    // prefetch: each pivot's column, diagonal value, end of row and the first
+    // ILU_FG entries of its upper part (one per lane)
+    int pc[ILU_FP], pu1[ILU_FP], puc[ILU_FP];
+    double pd[ILU_FP], puv[ILU_FP];
+#pragma unroll
+    for (int p = 0; p < ILU_FP; p++) {
+        pc[p] = 0;
+        if (p < np) {
+            int cp = 0;
+#pragma unroll
+            for (int s = 0; s < FS; s++)
+                if (s == p / ILU_FG) cp = mc[s];
+            pc[p] = __shfl(cp, p % ILU_FG, ILU_FG);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < ILU_FP; p++) {
+        pd[p] = 1.0;
+        pu1[p] = 0;
+        puc[p] = -2;
+        puv[p] = 0.0;
+        if (p < np) {
+            const int dc = dg[pc[p]];
+            pu1[p] = rp[pc[p] + 1];
+            pd[p] = a[dc];
+            const int q = dc + 1 + l;
+            puc[p] = q < pu1[p] ? col[q] : -2;
+            puv[p] = q < pu1[p] ? a[q] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < ILU_FP; p++)
+        if (p < np) ilu_factor_pivot<FS>(l, pc[p], pd[p], puc[p], puv[p], pu1[p], dg[pc[p]] + 1, col, a, mc, mv, p);
+    // pivots past the prefetched ones
+    for (int p = ILU_FP; p < np; p++) {
+        int cp = 0;
+#pragma unroll
+        for (int s = 0; s < FS; s++)
+            if (s == p / ILU_FG) cp = mc[s];
+        cp = __shfl(cp, p % ILU_FG, ILU_FG);
+        const int dc = dg[cp], u1 = rp[cp + 1], q = dc + 1 + l;
+        ilu_factor_pivot<FS>(l, cp, a[dc], q < u1 ? col[q] : -2, q < u1 ? a[q] : 0.0, u1, dc + 1, col, a, mc, mv, p);
+    }
+#pragma unroll
+    for (int s = 0; s < FS; s++) {
+        const int q = b0 + s * ILU_FG + l;
+        if (q < b1) a[q] = mv[s];
+    }
+}
+
 // ParILU (Chow & Patel 2015): the ILU(0) equations on the matrix pattern,
 //   l_ij u_jj + sum_{k<j} l_ik u_kj = a_ij (i > j),  u_ij + sum_{k<i} l_ik u_kj = a_ij (i <= j),
 // solved by synchronous fixed-point sweeps: every entry of a sweep is
@@ -904,8 +1018,8 @@ __global__ void __launch_bounds__(KB) k_ilu_upper(const int *__restrict__ rows, 
 //   payload that arithmetic never produces).  Lanes read values from LDS
 //   only.  The chunk's own rows are written there by their lanes; the rows of
 //   earlier chunks that the chunk reads (its imports, listed on the host with
-//   an LDS slot for every factor entry) are copied in by one more wavefront,
-//   which polls them in the output vector with device-coherent loads
+//   an LDS slot for every factor entry) are copied in by two more
+//   wavefronts, which poll them in the output vector with device-coherent loads
 //   (agent-scope atomics).  Producers write the output vector with
 //   device-coherent stores.  The value is its own flag, so no fence (which
 //   would write back the XCD's L2) is needed;
@@ -913,7 +1027,7 @@ __global__ void __launch_bounds__(KB) k_ilu_upper(const int *__restrict__ rows, 
 //   LDS slots of its neighbour faces into registers (up to ilu_window<N>()
 //   neighbours; the rest are read from memory), so a released wait is
 //   followed only by LDS reads and arithmetic: a dependency inside the chunk
-//   costs one round of the wavefront, one in another chunk the copy
+//   costs one round of the wavefront, one in another chunk a copy
 //   wavefront's poll;
 // * each wavefront repeats rounds until its lanes are done.  A lane keeps
 //   its place (neighbour, partial sums) when a neighbour is not ready yet
@@ -933,8 +1047,8 @@ __global__ void __launch_bounds__(KB) k_ilu_upper(const int *__restrict__ rows, 
 #define MHFEM_ILU_WG 512
 #endif
 constexpr int ILU_WG = MHFEM_ILU_WG;
-constexpr int ILU_NF = 64;                   // the copy wavefront
-constexpr int ILU_TF = 16;                   // imports per copy lane
+constexpr int ILU_NF = 128;                  // the copy wavefronts
+constexpr int ILU_TF = 8;                    // imports per copy lane
 constexpr int ILU_IMPORTS = ILU_NF * ILU_TF;
 constexpr unsigned long long ILU_WAIT_TICKS = 20000000ull;  // 0.2 s at 100 MHz
 // neighbour faces whose factors a lane holds (a face of a tetrahedral mesh
@@ -956,8 +1070,7 @@ constexpr int ilu_window()
 template <typename T, int N, bool UPPER>
 constexpr int ilu_lds_values()
 {
-    constexpr int used = (ILU_WG * N + ILU_IMPORTS) * (int)sizeof(T) +
-                         (UPPER ? (N - 1) * ilu_window<N, UPPER>() * N * ILU_WG * (int)sizeof(T) : 0);
+    constexpr int used = (ILU_WG * N + ILU_IMPORTS) * (int)sizeof(T);
     constexpr int pad = used < MHFEM_ILU_MIN_LDS ? (MHFEM_ILU_MIN_LDS - used) / (int)sizeof(T) : 0;
     return ILU_WG * N + ILU_IMPORTS + pad;
 }
@@ -1041,9 +1154,9 @@ __device__ __forceinline__ void ilu_fail(const IluSweep &a)
 // lower (UPPER = false: in = x, out = z, reset = y) or upper (UPPER = true:
 // in = z, out = y, reset = z) triangular solve.  sc: flags of the single
 // segment (the solve is skipped once it has halted; a wait that gives up
-// marks it DONE) or nullptr.  ILU_WG solving lanes and one copy wavefront.
+// marks it DONE) or nullptr.  ILU_WG solving lanes and ILU_NF copying ones.
 template <typename T, int N, bool UPPER>
-__global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(IluSweep a, const T *__restrict__ lu, const T *in, T *out,
+__global__ void __launch_bounds__(ILU_WG + ILU_NF, 3) k_ilu_sweep(IluSweep a, const T *__restrict__ lu, const T *in, T *out,
                                                                T *reset, const double *sc)
 {
     if (sc && halted(sc)) return;
@@ -1072,7 +1185,7 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
     int idle = 0;
 
     if (tid >= ILU_WG) {
-        // ---- the copy wavefront: imports into LDS as they become ready
+        // ---- the copy wavefronts: imports into LDS as they become ready
         T *imp = ys + ILU_WG * N;
         const int j = tid - ILU_WG;
         const int *ip = a.imp_ptr + (UPPER ? a.nchunks + 1 : 0);
@@ -1136,21 +1249,20 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
     // dependency level, so that a wavefront's lanes become ready together;
     // lane t's face keeps its values at LDS slots t * N + i, and those of a
     // neighbour face from its slot sb on, at sb + j.
-    // The rows that sum their neighbour terms as the values come in (lower:
-    // every row; upper: the last one, the other rows subtract the face's own
-    // values first and are summed at the end) keep their factors in
-    // registers; the upper solve's other rows keep theirs in LDS.
+    // The factors stay in registers (one work-group per CU leaves room).
+    // Rows sum their neighbour terms as the values come in, except the upper
+    // solve's rows before the last, which subtract the face's own values
+    // first: they are summed at the end, from the neighbour values the lane
+    // keeps as it takes them.
     constexpr int K = ilu_window<N, UPPER>();
-    constexpr int NR = UPPER ? 1 : N;
-    constexpr int NL = UPPER ? N - 1 : 0;
-    __shared__ T upf[NL > 0 ? NL * K * N * ILU_WG : 1];
     bool done = tid >= nface;
     const int E = done ? 0 : a.chunk_face[(UPPER ? a.faces : 0) + c0 + tid];
     const int me = tid * N;
     int nb = 0;          // neighbour faces
     int nbase[N];        // row i's first neighbour entry
     int slotF[K];        // LDS slot of each neighbour face (component 0)
-    T lf[NR][K][N];      // summing row's factor for neighbour b, component j
+    T lf[N][K][N];       // row i's factor for neighbour b, component j
+    T nvs[UPPER ? K : 1][N];  // upper: the neighbour values taken
     T own[N][N];         // the face's own off-diagonal factors (lower: j < i, upper: j > i)
     T z[N], dgv[N];      // right-hand side and (upper) diagonal of each row
 #pragma unroll
@@ -1191,15 +1303,14 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
         const bool has = b < nb;
         slotF[b] = has ? a.slot[nbase[0] + b * N] : 0;
 #pragma unroll
-        for (int i = 0; i < NR; i++)
+        for (int i = 0; i < N; i++)
 #pragma unroll
-            for (int j = 0; j < N; j++) lf[i][b][j] = has ? lu[nbase[UPPER ? N - 1 : i] + b * N + j] : T(0);
-#pragma unroll
-        for (int i = 0; i < NL; i++)
-#pragma unroll
-            for (int j = 0; j < N; j++)
-                upf[((i * K + b) * N + j) * ILU_WG + tid] = has ? lu[nbase[i] + b * N + j] : T(0);
+            for (int j = 0; j < N; j++) lf[i][b][j] = has ? lu[nbase[i] + b * N + j] : T(0);
     }
+#pragma unroll
+    for (int b = 0; b < (UPPER ? K : 1); b++)
+#pragma unroll
+        for (int j = 0; j < N; j++) nvs[b][j] = T(0);
     // wait for these loads here, once: the rounds below then have no memory
     // wait on their path (the compiler would otherwise wait in the loop, and
     // so also for the output stores of the previous round)
@@ -1239,12 +1350,16 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
                 for (int j = 0; j < N; j++) ready = ready && !is_sentinel(vs[b][j]);
                 const bool take = act && ready, blk = act && !ready;
 #pragma unroll
-                for (int i = 0; i < NR; i++) {
-                    const int row = UPPER ? N - 1 : i;
-                    T t = s[row];
+                for (int i = 0; i < N; i++) {
+                    if (UPPER && i != N - 1) continue;
+                    T t = s[i];
 #pragma unroll
                     for (int j = 0; j < N; j++) t -= lf[i][b][j] * vs[b][j];
-                    s[row] = take ? t : s[row];
+                    s[i] = take ? t : s[i];
+                }
+                if (UPPER) {
+#pragma unroll
+                    for (int j = 0; j < N; j++) nvs[UPPER ? b : 0][j] = take ? vs[b][j] : nvs[UPPER ? b : 0][j];
                 }
                 pos += take ? 1 : 0;
                 ps = blk ? slotF[b] : ps;
@@ -1303,13 +1418,12 @@ __global__ void __launch_bounds__(ILU_WG + ILU_NF, N < 3 ? 5 : 3) k_ilu_sweep(Il
 #pragma unroll
                         for (int j = 0; j < N; j++)
                             if (j > i) t -= own[i][j] * y[j];
-                        // the neighbour values, all final now
+                        // the neighbour values, taken in the rounds
 #pragma unroll
                         for (int b = 0; b < K; b++) {
                             if (b >= nb) break;
 #pragma unroll
-                            for (int j = 0; j < N; j++)
-                                t -= upf[((i * K + b) * N + j) * ILU_WG + tid] * lds_load(&ys[slotF[b] + j]);
+                            for (int j = 0; j < N; j++) t -= lf[i][b][j] * nvs[UPPER ? b : 0][j];
                         }
                         for (int b = K; b < nb; b++) {
                             const int sb = a.slot[nbase[0] + b * N];
@@ -1368,6 +1482,7 @@ struct Ilu0 {
     DBuf<int> slot, imp_ptr, imp_rows;  // LDS slot of every factor entry, each chunk's imports
     DBuf<int> chunk_ptr, chunk_face;    // each chunk's faces in lane order, lower then upper solve
     int nchunks = 0;
+    int factor_slots = 0;  // row entries per lane of k_ilu_factor_rows (0: rows too long, k_ilu_factor)
     DBuf<double> a0, swp;  // ParILU: the matrix in CSR order, the other sweep buffer
     // ILUT: factors on their own pattern (rebuilt with the factors)
     bool own = false;
@@ -3212,10 +3327,36 @@ static void ilu_factor(Ctx &c, Krylov &k)
     Ilu0 &I = k.ilu;
     hipLaunchKernelGGL((k_gather<double>), grid_for(c.nnz), KB, 0, c.stream, c.nnz, c.sell_val.p, c.csr_to_sell.p,
                        I.lu.p);
+    if (I.factor_slots == 0) {
+        int longest = 0;
+        for (int r = 0; r < k.n; r++) longest = std::max(longest, c.h_rowptr[r + 1] - c.h_rowptr[r]);
+        I.factor_slots = longest <= 4 * ILU_FG ? (longest + ILU_FG - 1) / ILU_FG : -1;
+    }
     for (size_t l = 0; l + 1 < I.h_lptr.size(); l++) {
         const int cnt = I.h_lptr[l + 1] - I.h_lptr[l];
-        hipLaunchKernelGGL(k_ilu_factor, (cnt + KB - 1) / KB, KB, 0, c.stream, I.lrows.p + I.h_lptr[l], cnt,
-                           c.rowptr.p, c.csr_col.p, I.diag.p, I.lu.p);
+        const int *rows = I.lrows.p + I.h_lptr[l];
+        const unsigned groups = (unsigned)(((long)cnt * ILU_FG + KB - 1) / KB);
+        switch (I.factor_slots) {
+        case 1:
+            hipLaunchKernelGGL((k_ilu_factor_rows<1>), groups, KB, 0, c.stream, rows, cnt, c.rowptr.p, c.csr_col.p,
+                               I.diag.p, I.lu.p);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_ilu_factor_rows<2>), groups, KB, 0, c.stream, rows, cnt, c.rowptr.p, c.csr_col.p,
+                               I.diag.p, I.lu.p);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_ilu_factor_rows<3>), groups, KB, 0, c.stream, rows, cnt, c.rowptr.p, c.csr_col.p,
+                               I.diag.p, I.lu.p);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_ilu_factor_rows<4>), groups, KB, 0, c.stream, rows, cnt, c.rowptr.p, c.csr_col.p,
+                               I.diag.p, I.lu.p);
+            break;
+        default:
+            hipLaunchKernelGGL(k_ilu_factor, (cnt + KB - 1) / KB, KB, 0, c.stream, rows, cnt, c.rowptr.p,
+                               c.csr_col.p, I.diag.p, I.lu.p);
+        }
     }
     MHFEM_HIP(hipGetLastError());
 }
