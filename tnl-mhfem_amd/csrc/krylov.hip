@@ -50,6 +50,10 @@ struct Seg {
     const int *blk_seg, *blk_row, *seg_blk;
     double *sc;        // [nseg][NSLOTS]
     double *partials;  // [ND][nblk]
+    // a launch over nact work-groups that take the blocks active[0..nact);
+    // nullptr: nact == nblk work-groups take every block
+    const int *active;
+    int nact;
 };
 
 
@@ -377,7 +381,8 @@ static void with_n(int n, F &&f)
 }
 
 #define SEG_BLOCK_PROLOGUE(sg, mode)                                   \
-    const int blk_ = xcd_block((sg).nblk);                             \
+    const int blk_ = (sg).active ? (sg).active[xcd_block((sg).nact)]   \
+                                 : xcd_block((sg).nblk);               \
     const int seg_ = (sg).blk_seg[blk_];                               \
     double *sc_ = (sg).sc + (long)seg_ * NSLOTS;                       \
     if (!seg_selected(sc_, (mode))) return;                            \
@@ -1579,6 +1584,8 @@ struct Krylov {
     // segmented decomposition
     int nseg = 1, rows_per_block = 2048;
     DBuf<int> blk_seg, blk_row, seg_blk;
+    std::vector<int> h_seg_blk;  // host copy of seg_blk
+    DBuf<int> active;            // nblk: the blocks of the running segments
     int nblk = 0;
     DBuf<double> partials;
     DBuf<double> sc;
@@ -1649,6 +1656,8 @@ static void build_blocks(Ctx &c, Krylov &k, int rpb)
     MHFEM_HIP(hipMemcpy(k.blk_row.p, brow.data(), (k.nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
     MHFEM_HIP(hipMemcpy(k.seg_blk.p, sblk.data(), (k.nseg + 1) * sizeof(int), hipMemcpyHostToDevice));
     k.partials.alloc((size_t)4 * k.nblk);
+    k.active.alloc(k.nblk);
+    k.h_seg_blk = sblk;
 }
 
 // Face order of the FP32 iteration's matrix copy.  The natural
@@ -2335,8 +2344,67 @@ struct Solve {
 
     Solve(Ctx &c_, Krylov &k_) : c(c_), k(k_), n(k_.n), g(grid_for(k_.n))
     {
-        sg = Seg{k.nseg, k.nblk, k.blk_seg.p, k.blk_row.p, k.seg_blk.p, k.sc.p, k.partials.p};
+        sg = Seg{k.nseg, k.nblk, k.blk_seg.p, k.blk_row.p, k.seg_blk.p, k.sc.p, k.partials.p, nullptr, k.nblk};
     }
+
+    // The segments a launch covers: M_ACTIVE work goes over sg, which may
+    // be narrowed to the blocks of the segments still running (Narrowed);
+    // every other mode takes all blocks.
+    Seg seg(int mode) const
+    {
+        if (mode == M_ACTIVE) return sg;
+        Seg s = sg;
+        s.active = nullptr;
+        s.nact = s.nblk;
+        return s;
+    }
+    void widen()
+    {
+        sg.active = nullptr;
+        sg.nact = sg.nblk;
+    }
+
+    // Narrowing of the M_ACTIVE launches of one eager solve (bicg_drive)
+    // to the blocks of the segments that the latest host copy of the flags
+    // (k.h_sc) shows running, once those are at most 3/5 of the work-groups
+    // launched now: the tail of an ensemble whose segments converge at
+    // different iteration counts then launches its few running segments'
+    // blocks instead of every block only to see most of them return.
+    // A segment halted in that copy does not run again before a restart,
+    // which widen()s first; one that halted after the copy is still
+    // launched and returns on its own flags.  The list goes to k.active in
+    // stream order, after the kernels queued with the previous list; each
+    // host list stays alive until the solve ends (the stream has read it
+    // by then).  A graph capture never narrows: the drive is eager only.
+    struct Narrowed {
+        Solve &S;
+        const bool on;  // off where `more` replays a captured graph
+        std::vector<std::vector<int>> lists;
+        Narrowed(Solve &s, bool on_) : S(s), on(on_) {}
+        ~Narrowed()
+        {
+            // an exception can leave copies from the host lists queued
+            if (!lists.empty()) (void)hipStreamSynchronize(S.c.stream);
+            S.widen();
+        }
+        void update()
+        {
+            const Krylov &k = S.k;
+            if (!on || k.nseg < 2 || S.any_restart()) return;
+            std::vector<int> a;
+            for (int s = 0; s < k.nseg; s++) {
+                if (S.seg_sc(s)[S_DONE] != 0.0) continue;
+                for (int b = k.h_seg_blk[s]; b < k.h_seg_blk[s + 1]; b++) a.push_back(b);
+            }
+            if (a.empty() || 5 * (long)a.size() > 3 * (long)S.sg.nact) return;
+            lists.push_back(std::move(a));
+            const std::vector<int> &l = lists.back();
+            MHFEM_HIP(hipMemcpyAsync(k.active.p, l.data(), l.size() * sizeof(int), hipMemcpyHostToDevice,
+                                     S.c.stream));
+            S.sg.active = k.active.p;
+            S.sg.nact = (int)l.size();
+        }
+    };
 
     // the FP64 BiCGStab solve runs in the grouped face order (k.permuted64)
     bool perm64 = false;
@@ -2359,7 +2427,7 @@ struct Solve {
         SellView<T> A = matrix<T>();
         const bool timed = kernel_timing_begin(c);
         with_n(c.n, [&](auto NC) {
-            hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND>), k.nblk, KB, 0, c.stream, A, sg, mode, x,
+            hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND>), seg(mode).nact, KB, 0, c.stream, A, seg(mode), mode, x,
                                dinv, y, z, w);
         });
         MHFEM_HIP(hipGetLastError());
@@ -2370,7 +2438,7 @@ struct Solve {
     {
         SellView<T> A = matrix<T>();
         with_n(c.n, [&](auto NC) {
-            hipLaunchKernelGGL((k_sresidual<T, decltype(NC)::value>), k.nblk, KB, 0, c.stream, A, sg, mode, x, b, y);
+            hipLaunchKernelGGL((k_sresidual<T, decltype(NC)::value>), seg(mode).nact, KB, 0, c.stream, A, seg(mode), mode, x, b, y);
         });
         MHFEM_HIP(hipGetLastError());
     }
@@ -2451,7 +2519,7 @@ struct Solve {
     template <typename T, int ND, int NF = ND>
     void dots(DotArgs<T, ND> a, int fin, int mode)
     {
-        hipLaunchKernelGGL((k_sdots<T, ND>), k.nblk, KB, 0, c.stream, sg, mode, a);
+        hipLaunchKernelGGL((k_sdots<T, ND>), seg(mode).nact, KB, 0, c.stream, seg(mode), mode, a);
         finish<NF>(fin, mode);
     }
     // FP32 inner solves with the preconditioner fused into the products
@@ -2553,7 +2621,7 @@ struct Solve {
     void prec(const T *x, T *y, const T *dinv, int mode)
     {
         if (separate_prec()) separate_apply<T>(x, y, false);
-        else hipLaunchKernelGGL((k_sscale<T>), k.nblk, KB, 0, c.stream, sg, mode, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
+        else hipLaunchKernelGGL((k_sscale<T>), seg(mode).nact, KB, 0, c.stream, seg(mode), mode, x, jacobi_scales() ? dinv : (const T *)nullptr, y);
         MHFEM_HIP(hipGetLastError());
     }
 
@@ -2587,7 +2655,7 @@ struct Solve {
         if (separate_prec() && mode != M_ALL) {
             if (!scratch) throw Error{MHFEM_E_STATE, "restart with ILU(0), ILU-ISAI or AMG needs a scratch vector"};
             separate_apply<T>(tmp, scratch, false);
-            hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, (const T *)scratch, r, (T *)nullptr);
+            hipLaunchKernelGGL((k_scopy2<T>), seg(mode).nact, KB, 0, c.stream, seg(mode), mode, (const T *)scratch, r, (T *)nullptr);
             MHFEM_HIP(hipGetLastError());
             return;
         }
@@ -2614,7 +2682,7 @@ struct Solve {
     {
         T *r = (T *)v.r;
         presidue<T>(x, b, r, (T *)v.tmp, (const T *)v.dinv, mode, (T *)v.tmp2);
-        hipLaunchKernelGGL((k_scopy2<T>), k.nblk, KB, 0, c.stream, sg, mode, r, (T *)v.p, (T *)v.r_ast);
+        hipLaunchKernelGGL((k_scopy2<T>), seg(mode).nact, KB, 0, c.stream, seg(mode), mode, r, (T *)v.p, (T *)v.r_ast);
         MHFEM_HIP(hipGetLastError());
         if (merged<T>())  // (r, r) also in partial row 1, for the first merged alpha
             dots<T, 2>(DotArgs<T, 2>{{r, r}, {(T *)v.r_ast, r}}, FIN_INIT, mode);
@@ -2633,16 +2701,16 @@ struct Solve {
                 // omega and beta from the second product's four dots; x, r
                 // and p in one pass
                 pmatvec<T, 1, 2>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA_MERGED);
-                hipLaunchKernelGGL((k_bicg_s<T>), k.nblk, KB, 0, c.stream, sg, r, Ap, s);
+                hipLaunchKernelGGL((k_bicg_s<T>), sg.nact, KB, 0, c.stream, sg, r, Ap, s);
                 pmatvec<T, 4>(s, As, tmp, dinv, s, FIN_OMEGA_MERGED, r_ast);
-                hipLaunchKernelGGL((k_bicg_xrp<T>), k.nblk, KB, 0, c.stream, sg, x, p, s, As, r, Ap);
+                hipLaunchKernelGGL((k_bicg_xrp<T>), sg.nact, KB, 0, c.stream, sg, x, p, s, As, r, Ap);
             } else {
                 pmatvec<T, 1>(p, Ap, tmp, dinv, r_ast, FIN_ALPHA);
-                hipLaunchKernelGGL((k_bicg_s<T>), k.nblk, KB, 0, c.stream, sg, r, Ap, s);
+                hipLaunchKernelGGL((k_bicg_s<T>), sg.nact, KB, 0, c.stream, sg, r, Ap, s);
                 pmatvec<T, 2>(s, As, tmp, dinv, s, FIN_OMEGA);
-                hipLaunchKernelGGL((k_bicg_xr<T>), k.nblk, KB, 0, c.stream, sg, x, p, s, As, r, r_ast);
+                hipLaunchKernelGGL((k_bicg_xr<T>), sg.nact, KB, 0, c.stream, sg, x, p, s, As, r, r_ast);
                 finish<2>(FIN_UPDATE, M_ACTIVE);
-                hipLaunchKernelGGL((k_bicg_p<T>), k.nblk, KB, 0, c.stream, sg, r, p, Ap);
+                hipLaunchKernelGGL((k_bicg_p<T>), sg.nact, KB, 0, c.stream, sg, r, p, Ap);
             }
             MHFEM_HIP(hipGetLastError());
         }
@@ -2664,8 +2732,9 @@ struct Solve {
     // the host waits for each batch instead: a batch queued after convergence
     // would cost its launches (the kernels of a finished segment return at
     // once) and short solves would lose more than they gain.
-    int bicg_drive(const T *b, T *x, const Vecs &v, More &&more)
+    int bicg_drive(const T *b, T *x, const Vecs &v, More &&more, bool narrow = true)
     {
+        Narrowed narrowed(*this, narrow);
         const size_t bytes = (size_t)k.nseg * NSLOTS * sizeof(double);
         auto snap = [&](int q) {
             MHFEM_HIP(hipMemcpyAsync(k.h_sc_ahead[q], k.sc.p, bytes, hipMemcpyDeviceToHost, c.stream));
@@ -2697,6 +2766,7 @@ struct Solve {
         for (;;) {
             if (all_done()) break;
             if (any_restart()) {
+                widen();
                 bicg_init<T>(b, x, v, M_RESTART);
                 std::fill(prev.begin(), prev.end(), 0.0);
                 poll();
@@ -2710,11 +2780,13 @@ struct Solve {
                     snap(q ^ 1);
                     remember();
                     take(q);
+                    narrowed.update();
                     if (all_done() || any_restart() || !(batches_left() > 2.0)) break;
                 }
             }
             remember();
             poll();  // the batch still queued ends; the latest flags decide
+            narrowed.update();
         }
         int it = 0;
         for (int s = 0; s < k.nseg; s++) it = std::max(it, (int)seg_sc(s)[S_IT]);
@@ -3792,7 +3864,7 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                 // residual: keep p - r for the continued direction
                 const bool merged = S.merged<float>();
                 if (merged)
-                    hipLaunchKernelGGL((k_bicg_dir<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)k.p32.p,
+                    hipLaunchKernelGGL((k_bicg_dir<float>), k.nblk, KB, 0, c.stream, S.seg(M_ALL), (const float *)k.p32.p,
                                        (const float *)r, k.s32.p);
                 S.residual_mixed(x, b, rhs32);
                 MHFEM_HIP(hipGetLastError());
@@ -3819,10 +3891,10 @@ void krylov_solve(Ctx &c, const double *b, double *x, mhfem_solve_stats *st)
                 hipLaunchKernelGGL(k_set_slot, 1, 64, 0, c.stream, k.sc.p, (int)S_MAXIT,
                                    (double)std::min(maxit, total + round_cap));
                 if (merged)
-                    hipLaunchKernelGGL((k_bicg_pfix<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)r,
+                    hipLaunchKernelGGL((k_bicg_pfix<float>), S.sg.nact, KB, 0, c.stream, S.sg, (const float *)r,
                                        k.p32.p, (const float *)k.s32.p);
                 else
-                    hipLaunchKernelGGL((k_bicg_p<float>), k.nblk, KB, 0, c.stream, S.sg, (const float *)r, k.p32.p,
+                    hipLaunchKernelGGL((k_bicg_p<float>), S.sg.nact, KB, 0, c.stream, S.sg, (const float *)r, k.p32.p,
                                        (const float *)k.Ap32.p);
                 MHFEM_HIP(hipGetLastError());
             }
@@ -3964,7 +4036,7 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
     // assembly, so each replay sees its own step's time + tau
     launch_set_clock(c, time + tau);
     MHFEM_HIP(hipGraphLaunch(k.g_first, c.stream));
-    const int it = S.bicg_drive<double>(b, x, v, [&]() { MHFEM_HIP(hipGraphLaunch(k.g_more, c.stream)); });
+    const int it = S.bicg_drive<double>(b, x, v, [&]() { MHFEM_HIP(hipGraphLaunch(k.g_more, c.stream)); }, false);
     MHFEM_HIP(hipGraphLaunch(k.g_post, c.stream));
     mhfem_solve_stats out{};
     out.iterations = it;
