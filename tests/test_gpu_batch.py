@@ -54,30 +54,44 @@ def test_batch_iterations_per_instance():
     assert len(set(its.tolist())) > 1  # instances really stop independently
 
 
-@pytest.mark.parametrize("check_every", [1, 3])
-def test_batch_eager_step_equals_graph_step(check_every):
-    """The eager step of a batch narrows its launches to the blocks of the
-    instances still iterating once the others have converged (the graph
-    replays keep every block).  Skipping the halted instances' blocks changes
-    no arithmetic: per-instance iteration counts and the solutions equal the
-    graph step's bit for bit, step after step."""
+@pytest.mark.parametrize("check_every,rows_per_block", [(1, 128), (3, 0)])
+def test_batch_narrowed_steps_equal_single_runs(check_every, rows_per_block):
+    """Once some instances of a batch have converged, its steps launch only
+    the blocks of the instances still iterating (the graph step replays its
+    graph until then and iterates eagerly over the narrowed grids after).
+    Skipping the halted instances' blocks changes no arithmetic: a segment's
+    work-groups start at its first row, so its products and dot products are
+    summed as in a context of its own.  Each instance's iteration count and
+    solution equal its own single-instance run bit for bit, for the eager
+    and the graph step, step after step (128 rows per work-group: several
+    work-groups per instance; 0: the default, one per instance here)."""
     members = _members([4, 12, 6, 16, 3, 9, 5])
     batch = problems.concatenate(members)
-    a = capi.Solver(batch, tol=1e-12, check_every=check_every)
-    b = capi.Solver(batch, tol=1e-12, check_every=check_every)
+    kw = dict(tol=1e-12, check_every=check_every, rows_per_block=rows_per_block)
+    a = capi.Solver(batch, **kw)
+    b = capi.Solver(batch, **kw)
+    singles = [capi.Solver(m, **kw) for m in members]
+    co, fo = a.batch_info()
     t = 0.0
     for k in range(3):
-        print(f"[batch] eager/graph step {k}", flush=True)
+        print(f"[batch] narrowed step {k}", flush=True)
         a.step(t, batch.dt)
         b.step_graph(t, batch.dt)
+        its1 = np.array([s.step(t, batch.dt).iterations for s in singles])
         t += batch.dt
-        ia, ra = a.batch_stats()
-        ib, rb = b.batch_stats()
-        assert len(set(ia.tolist())) > 1, ia  # instances stop at different counts
-        assert np.array_equal(ia, ib), (k, ia, ib)
-        assert np.array_equal(ra, rb), k
-        assert np.array_equal(a.get_array("Z_iF"), b.get_array("Z_iF")), k
-        assert np.array_equal(a.get_array("Z_iK"), b.get_array("Z_iK")), k
+        ia, _ = a.batch_stats()
+        ib, _ = b.batch_stats()
+        assert len(set(its1.tolist())) > 1, its1  # instances stop at different counts
+        assert np.array_equal(ia, its1), (k, ia, its1)
+        assert np.array_equal(ib, its1), (k, ib, its1)
+        za, zb = a.get_array("Z_iF"), b.get_array("Z_iF")
+        ka, kb = a.get_array("Z_iK"), b.get_array("Z_iK")
+        for j, s in enumerate(singles):
+            z1, k1 = s.get_array("Z_iF"), s.get_array("Z_iK")
+            assert np.array_equal(za[fo[j]:fo[j + 1]], z1), (k, j)
+            assert np.array_equal(zb[fo[j]:fo[j + 1]], z1), (k, j)
+            assert np.array_equal(ka[co[j]:co[j + 1]], k1), (k, j)
+            assert np.array_equal(kb[co[j]:co[j + 1]], k1), (k, j)
 
 
 @pytest.mark.parametrize("prec", ["diagonal", "ilu0"])

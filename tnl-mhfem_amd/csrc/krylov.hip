@@ -2375,12 +2375,12 @@ struct Solve {
     // launched and returns on its own flags.  The list goes to k.active in
     // stream order, after the kernels queued with the previous list; each
     // host list stays alive until the solve ends (the stream has read it
-    // by then).  A graph capture never narrows: the drive is eager only.
+    // by then).  A captured graph keeps the grids of its capture: the graph
+    // step replays its graph only while the launches are not narrowed.
     struct Narrowed {
         Solve &S;
-        const bool on;  // off where `more` replays a captured graph
         std::vector<std::vector<int>> lists;
-        Narrowed(Solve &s, bool on_) : S(s), on(on_) {}
+        explicit Narrowed(Solve &s) : S(s) {}
         ~Narrowed()
         {
             // an exception can leave copies from the host lists queued
@@ -2390,7 +2390,7 @@ struct Solve {
         void update()
         {
             const Krylov &k = S.k;
-            if (!on || k.nseg < 2 || S.any_restart()) return;
+            if (k.nseg < 2 || S.any_restart()) return;
             std::vector<int> a;
             for (int s = 0; s < k.nseg; s++) {
                 if (S.seg_sc(s)[S_DONE] != 0.0) continue;
@@ -2732,9 +2732,9 @@ struct Solve {
     // the host waits for each batch instead: a batch queued after convergence
     // would cost its launches (the kernels of a finished segment return at
     // once) and short solves would lose more than they gain.
-    int bicg_drive(const T *b, T *x, const Vecs &v, More &&more, bool narrow = true)
+    int bicg_drive(const T *b, T *x, const Vecs &v, More &&more)
     {
-        Narrowed narrowed(*this, narrow);
+        Narrowed narrowed(*this);
         const size_t bytes = (size_t)k.nseg * NSLOTS * sizeof(double);
         auto snap = [&](int q) {
             MHFEM_HIP(hipMemcpyAsync(k.h_sc_ahead[q], k.sc.p, bytes, hipMemcpyDeviceToHost, c.stream));
@@ -4036,7 +4036,13 @@ void krylov_graph_step(Ctx &c, double time, double tau, mhfem_solve_stats *st)
     // assembly, so each replay sees its own step's time + tau
     launch_set_clock(c, time + tau);
     MHFEM_HIP(hipGraphLaunch(k.g_first, c.stream));
-    const int it = S.bicg_drive<double>(b, x, v, [&]() { MHFEM_HIP(hipGraphLaunch(k.g_more, c.stream)); }, false);
+    // once the launches are narrowed to the instances still iterating, the
+    // same iterations go eagerly over the narrowed grids (the kernels and
+    // arguments of the capture but for the grid, so the same results)
+    const int it = S.bicg_drive<double>(b, x, v, [&]() {
+        if (S.sg.active) S.bicg_iterations<double>(x, v, every);
+        else MHFEM_HIP(hipGraphLaunch(k.g_more, c.stream));
+    });
     MHFEM_HIP(hipGraphLaunch(k.g_post, c.stream));
     mhfem_solve_stats out{};
     out.iterations = it;
