@@ -187,7 +187,7 @@ typedef struct {
     double pre_iterate, update_b_R_Q, upwind, assemble, preconditioner, linear_solver,
         post_iterate;
     long long iterations, time_steps;
-    /* timing level 2: every 8th SpMV launch of the Krylov iteration is
+    /* timing level 2: every 9th SpMV launch of the Krylov iteration is
        bracketed by HIP events on the context's stream */
     double spmv_ms;       /* summed duration of the timed SpMV launches that did the
                              product (launches queued after convergence return at once
