@@ -226,10 +226,10 @@ __device__ __forceinline__ int xcd_block(int nblk)
     return k * q + (k < r ? k : r) + j;
 }
 
-template <int ND, int OFF = 0>
+template <int ND, int OFF = 0, int BS = KB>
 __device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg, int blk)
 {
-    __shared__ double sm[ND][KB / 64];
+    __shared__ double sm[ND][BS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 #pragma unroll
     for (int d = 0; d < ND; d++) {
@@ -242,14 +242,21 @@ __device__ __forceinline__ void block_partial(double (&v)[ND], const Seg &sg, in
         for (int d = 0; d < ND; d++) {
             double s = sm[d][0];
 #pragma unroll
-            for (int q = 1; q < KB / 64; q++) s += sm[d][q];
+            for (int q = 1; q < BS / 64; q++) s += sm[d][q];
             sg.partials[(long)(d + OFF) * sg.nblk + blk] = s;
         }
     }
 }
 
 // Sum the partials of each selected segment into sc[S_D0 + d] and evaluate
-// the scalar recurrence `fin`.  One work-group per segment.
+// the scalar recurrence `fin`.  One work-group per segment.  Thread tid
+// sums the partials b0 + tid + m KB of each row in four chains, m going to
+// chain m mod 4 while a whole group of four is left and to chain 0 after
+// (a fixed order: the sum is the same for a given decomposition).  The
+// loads of eight values of every row are issued at once, clamped to the
+// segment's last partial (adding zero), with no branch between them, so a
+// single-segment system with thousands of work-groups reads its partials
+// in a few rounds of memory latency.
 template <int ND>
 __global__ void __launch_bounds__(KB) k_finalize(Seg sg, int fin, int mode)
 {
@@ -259,22 +266,34 @@ __global__ void __launch_bounds__(KB) k_finalize(Seg sg, int fin, int mode)
     __shared__ double sm[ND][KB / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int b0 = sg.seg_blk[s], b1 = sg.seg_blk[s + 1];
+    const int first = b0 + tid;
+    const int M = first < b1 ? (b1 - first + KB - 1) / KB : 0;  // partials of this thread
+    const int full = M & ~3;
+    double a[ND][4];
+#pragma unroll
+    for (int d = 0; d < ND; d++) a[d][0] = a[d][1] = a[d][2] = a[d][3] = 0.0;
+    for (int q = 0; q < M; q += 8) {
+        double v[ND][8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int i = min(first + (q + j) * KB, b1 - 1);
+#pragma unroll
+            for (int d = 0; d < ND; d++) v[d][j] = sg.partials[(long)d * sg.nblk + i];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int m = q + j;
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+                a[d][j & 3] += m < full ? v[d][j] : 0.0;
+                a[d][0] += m >= full && m < M ? v[d][j] : 0.0;
+            }
+        }
+    }
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-        // four independent loads in flight per thread (a fixed order: the
-        // sum is the same for a given decomposition)
-        const double *P = sg.partials + (long)d * sg.nblk;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        int b = b0 + tid;
-        for (; b + 3 * KB < b1; b += 4 * KB) {
-            a0 += P[b];
-            a1 += P[b + KB];
-            a2 += P[b + 2 * KB];
-            a3 += P[b + 3 * KB];
-        }
-        for (; b < b1; b += KB) a0 += P[b];
-        double a = wave_sum((a0 + a1) + (a2 + a3));
-        if (lane == 0) sm[d][wid] = a;
+        const double w = wave_sum((a[d][0] + a[d][1]) + (a[d][2] + a[d][3]));
+        if (lane == 0) sm[d][wid] = w;
     }
     __syncthreads();
     if (tid == 0) {
@@ -391,14 +410,16 @@ static void with_n(int n, F &&f)
 // y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y), and
 // for ND = 4 also (y, w), (z, w).  A work-group's row range holds whole
 // faces (build_blocks).
-template <typename T, int N, bool DIAG, int ND>
-__global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
+// BS threads per work-group (sspmv() takes one thread per face of a
+// work-group's rows, up to 1024).
+template <typename T, int N, bool DIAG, int ND, int BS>
+__global__ void __launch_bounds__(BS) k_sspmv(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
                                               const T *__restrict__ dinv, T *__restrict__ y, const T *__restrict__ z,
                                               const T *__restrict__ w)
 {
     SEG_BLOCK_PROLOGUE(sg, mode);
     double acc[ND > 0 ? ND : 1] = {};
-    for (int f = row0_ / N + threadIdx.x; f < row1_ / N; f += KB) {
+    for (int f = row0_ / N + threadIdx.x; f < row1_ / N; f += BS) {
         Pack<T, N> v = sell_face<N>(A, x, f);
         if constexpr (DIAG) {
             const Pack<T, N> d = ld<T, N>(dinv + (int64_t)f * N);
@@ -424,7 +445,7 @@ __global__ void __launch_bounds__(KB) k_sspmv(SellView<T> A, Seg sg, int mode, c
             }
         }
     }
-    if constexpr (ND > 0) block_partial<ND>(acc, sg, blk_);
+    if constexpr (ND > 0) block_partial<ND, 0, BS>(acc, sg, blk_);
 }
 
 // y = b - A x on the selected segments
@@ -2426,9 +2447,19 @@ struct Solve {
     {
         SellView<T> A = matrix<T>();
         const bool timed = kernel_timing_begin(c);
+        // one thread per face of a work-group's rows (up to 1024 threads):
+        // the product's gathers are in flight for all of them at once instead
+        // of in turns of KB faces
+        const int fpb = k.rows_per_block / c.n;
         with_n(c.n, [&](auto NC) {
-            hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND>), seg(mode).nact, KB, 0, c.stream, A, seg(mode), mode, x,
-                               dinv, y, z, w);
+            auto go = [&](auto BSC) {
+                constexpr int BS = decltype(BSC)::value;
+                hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND, BS>), seg(mode).nact, BS, 0, c.stream,
+                                   A, seg(mode), mode, x, dinv, y, z, w);
+            };
+            if (fpb >= 1024) go(std::integral_constant<int, 1024>{});
+            else if (fpb >= 512) go(std::integral_constant<int, 512>{});
+            else go(std::integral_constant<int, KB>{});
         });
         MHFEM_HIP(hipGetLastError());
         if (timed) kernel_timing_end(c);
