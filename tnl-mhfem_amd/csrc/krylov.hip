@@ -410,8 +410,7 @@ static void with_n(int n, F &&f)
 // y = D (A x) (D = dinv or identity); ND fused dots: (y, z), (y, y), and
 // for ND = 4 also (y, w), (z, w).  A work-group's row range holds whole
 // faces (build_blocks).
-// BS threads per work-group (sspmv() takes one thread per face of a
-// work-group's rows, up to 1024).
+// BS threads per work-group (256, or 512 for the FP32 product; sspmv()).
 template <typename T, int N, bool DIAG, int ND, int BS>
 __global__ void __launch_bounds__(BS) k_sspmv(SellView<T> A, Seg sg, int mode, const T *__restrict__ x,
                                               const T *__restrict__ dinv, T *__restrict__ y, const T *__restrict__ z,
@@ -2447,9 +2446,11 @@ struct Solve {
     {
         SellView<T> A = matrix<T>();
         const bool timed = kernel_timing_begin(c);
-        // one thread per face of a work-group's rows (up to 1024 threads):
-        // the product's gathers are in flight for all of them at once instead
-        // of in turns of KB faces
+        // Threads per work-group, measured at n = 2 on the 64^3 Kuhn mesh:
+        // the FP32 product runs best with 512 (one face per thread at 1024
+        // rows per work-group: 89-91 us, against 92 us with 256 threads at 512
+        // rows and 91-98 us with 1024 threads at 2048 rows); the FP64 product
+        // with 256 (186 us at 512 rows; 213 us with 1024 threads at 2048).
         const int fpb = k.rows_per_block / c.n;
         with_n(c.n, [&](auto NC) {
             auto go = [&](auto BSC) {
@@ -2457,8 +2458,7 @@ struct Solve {
                 hipLaunchKernelGGL((k_sspmv<T, decltype(NC)::value, DIAG, ND, BS>), seg(mode).nact, BS, 0, c.stream,
                                    A, seg(mode), mode, x, dinv, y, z, w);
             };
-            if (fpb >= 1024) go(std::integral_constant<int, 1024>{});
-            else if (fpb >= 512) go(std::integral_constant<int, 512>{});
+            if (sizeof(T) == 4 && fpb >= 512) go(std::integral_constant<int, 512>{});
             else go(std::integral_constant<int, KB>{});
         });
         MHFEM_HIP(hipGetLastError());
