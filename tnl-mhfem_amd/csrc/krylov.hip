@@ -623,15 +623,20 @@ __global__ void __launch_bounds__(KB) k_spmv_plain(SellView<T> A, const T *__res
 // AH = A H for the stored increments of the learned guess.  H and AH keep
 // the HIST_SLOTS slots of a row together (FP64, 128 bytes per row): the
 // gather of a column face fetches the values of every slot from two cache
-// lines, where one array per slot would touch a line per slot.  The G4
-// groups of 4 slots (those in use) are taken GP groups per sweep over the
-// face's blocks (the blocks come from the caches after the first sweep).
-template <int N, int G4, int GP>
+// lines, where one array per slot would touch a line per slot.  L lanes
+// share a face, lane g taking the slots 4g..4g+3 (L = the groups of 4 slots
+// in use): the L lanes read the face's blocks at the same addresses (one
+// request per wavefront) and its rows' slots as one contiguous run, and a
+// lane holds 4 slots of accumulators instead of 16 (about 60 VGPRs instead
+// of 130, so 8 waves per SIMD instead of 3).  Each slot's sum runs in the
+// same order as with one lane per face.
+template <int N, int L>
 __global__ void __launch_bounds__(KB) k_spmm_hist(SellView<double> A, const double *__restrict__ H,
                                                   double *__restrict__ AH)
 {
-    constexpr int PF = 8, SW = 4 * GP;  // slots per sweep
-    const int f = xcd_block(gridDim.x) * KB + threadIdx.x;
+    constexpr int PF = 8;
+    const int64_t gid = (int64_t)xcd_block(gridDim.x) * KB + threadIdx.x;
+    const int f = (int)(gid / L), g0 = (int)(gid % L);  // face, and this lane's group of 4 slots
     if (f >= A.faces) return;
     const int64_t base = A.slice_ptr[f >> 6] + (f & 63);
     const int w = A.slice_width[f >> 6];
@@ -639,71 +644,67 @@ __global__ void __launch_bounds__(KB) k_spmm_hist(SellView<double> A, const doub
 #pragma unroll
     for (int k = 0; k < PF; k++)
         if (k < w) col[k] = A.col[base + (int64_t)k * 64];
-#pragma unroll 1
-    for (int g0 = 0; g0 < G4; g0 += GP) {
-        auto load_rows = [&](int face, double (&v)[N][SW]) {
+    auto load_rows = [&](int face, double (&v)[N][4]) {
 #pragma unroll
-            for (int j = 0; j < N; j++) {
-                const double *p = H + ((int64_t)face * N + j) * HIST_SLOTS + 4 * g0;
-#pragma unroll
-                for (int t = 0; t < SW; t += 2) {
-                    const Pack<double, 2> a = ld<double, 2>(p + t);
-                    v[j][t] = a.v[0];
-                    v[j][t + 1] = a.v[1];
-                }
-            }
-        };
-        double y[N][SW];
-        {
-            // the unit diagonal and the stored entries of the diagonal block
-            double xe[N][SW];
-            load_rows(f, xe);
-#pragma unroll
-            for (int i = 0; i < N; i++)
-#pragma unroll
-                for (int s = 0; s < SW; s++) y[i][s] = xe[i][s];
-            if constexpr (N > 1) {
-                const Pack<double, N *(N - 1)> dg = ld<double, N *(N - 1)>(A.val + A.diag_base + (int64_t)f * N * (N - 1));
-#pragma unroll
-                for (int i = 0; i < N; i++)
-#pragma unroll
-                    for (int j = 0; j < N; j++)
-                        if (j != i) {
-                            const double a = dg.v[i * (N - 1) + (j < i ? j : j - 1)];
-#pragma unroll
-                            for (int s = 0; s < SW; s++) y[i][s] += a * xe[j][s];
-                        }
-            }
+        for (int j = 0; j < N; j++) {
+            const double *p = H + ((int64_t)face * N + j) * HIST_SLOTS + 4 * g0;
+            const Pack<double, 2> a = ld<double, 2>(p), b = ld<double, 2>(p + 2);
+            v[j][0] = a.v[0];
+            v[j][1] = a.v[1];
+            v[j][2] = b.v[0];
+            v[j][3] = b.v[1];
         }
-        auto block = [&](int64_t q, int c) {
-            const Pack<double, N * N> b = ld<double, N * N>(A.val + q * (N * N));
-            double xc[N][SW];
-            load_rows(c, xc);
+    };
+    double y[N][4];
+    {
+        // the unit diagonal and the stored entries of the diagonal block
+        double xe[N][4];
+        load_rows(f, xe);
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) y[i][s] = xe[i][s];
+        if constexpr (N > 1) {
+            const Pack<double, N *(N - 1)> dg = ld<double, N *(N - 1)>(A.val + A.diag_base + (int64_t)f * N * (N - 1));
 #pragma unroll
             for (int i = 0; i < N; i++)
 #pragma unroll
                 for (int j = 0; j < N; j++)
+                    if (j != i) {
+                        const double a = dg.v[i * (N - 1) + (j < i ? j : j - 1)];
 #pragma unroll
-                    for (int s = 0; s < SW; s++) y[i][s] += b.v[i * N + j] * xc[j][s];
-        };
-#pragma unroll
-        for (int k = 0; k < PF; k++)
-            if (k < w) block(base + (int64_t)k * 64, col[k]);
-        for (int k = PF; k < w; k++) {
-            const int64_t q = base + (int64_t)k * 64;
-            block(q, A.col[q]);
+                        for (int s = 0; s < 4; s++) y[i][s] += a * xe[j][s];
+                    }
         }
+    }
+    auto block = [&](int64_t q, int c) {
+        const Pack<double, N * N> b = ld<double, N * N>(A.val + q * (N * N));
+        double xc[N][4];
+        load_rows(c, xc);
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            double *o = AH + ((int64_t)f * N + i) * HIST_SLOTS + 4 * g0;
+        for (int i = 0; i < N; i++)
 #pragma unroll
-            for (int t = 0; t < SW; t += 2) {
-                Pack<double, 2> a;
-                a.v[0] = y[i][t];
-                a.v[1] = y[i][t + 1];
-                st<double, 2>(o + t, a);
-            }
-        }
+            for (int j = 0; j < N; j++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) y[i][s] += b.v[i * N + j] * xc[j][s];
+    };
+#pragma unroll
+    for (int k = 0; k < PF; k++)
+        if (k < w) block(base + (int64_t)k * 64, col[k]);
+    for (int k = PF; k < w; k++) {
+        const int64_t q = base + (int64_t)k * 64;
+        block(q, A.col[q]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double *o = AH + ((int64_t)f * N + i) * HIST_SLOTS + 4 * g0;
+        Pack<double, 2> a, b;
+        a.v[0] = y[i][0];
+        a.v[1] = y[i][1];
+        b.v[0] = y[i][2];
+        b.v[1] = y[i][3];
+        st<double, 2>(o, a);
+        st<double, 2>(o + 2, b);
     }
 }
 
@@ -3631,17 +3632,15 @@ void spmm_hist(Ctx &c, const double *H, double *AH, int slots)
     Krylov &k = *c.krylov;
     Solve S(c, k);
     SellView<double> A = S.matrix<double>();
-    const unsigned grid = (unsigned)((c.F + KB - 1) / KB);
     const int g4 = (slots + 3) / 4;
+    const unsigned grid = (unsigned)(((int64_t)c.F * g4 + KB - 1) / KB);
     with_n(c.n, [&](auto NC) {
         constexpr int NN = decltype(NC)::value;
-        // every slot in one sweep where the registers allow (8 slots per sweep at n = 3)
-        constexpr int GPMAX = NN <= 2 ? 4 : 2;
         switch (g4) {
-        case 1: hipLaunchKernelGGL((k_spmm_hist<NN, 1, 1>), grid, KB, 0, c.stream, A, H, AH); break;
-        case 2: hipLaunchKernelGGL((k_spmm_hist<NN, 2, 2>), grid, KB, 0, c.stream, A, H, AH); break;
-        case 3: hipLaunchKernelGGL((k_spmm_hist<NN, 3, (GPMAX < 3 ? 1 : 3)>), grid, KB, 0, c.stream, A, H, AH); break;
-        default: hipLaunchKernelGGL((k_spmm_hist<NN, 4, GPMAX>), grid, KB, 0, c.stream, A, H, AH); break;
+        case 1: hipLaunchKernelGGL((k_spmm_hist<NN, 1>), grid, KB, 0, c.stream, A, H, AH); break;
+        case 2: hipLaunchKernelGGL((k_spmm_hist<NN, 2>), grid, KB, 0, c.stream, A, H, AH); break;
+        case 3: hipLaunchKernelGGL((k_spmm_hist<NN, 3>), grid, KB, 0, c.stream, A, H, AH); break;
+        default: hipLaunchKernelGGL((k_spmm_hist<NN, 4>), grid, KB, 0, c.stream, A, H, AH); break;
         }
     });
     MHFEM_HIP(hipGetLastError());
