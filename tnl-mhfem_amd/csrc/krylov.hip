@@ -735,8 +735,16 @@ __global__ void __launch_bounds__(KB) k_convert_permuted(int64_t blocks, const i
     constexpr int NN = N * N;
     for (int64_t b = (int64_t)blockIdx.x * KB + threadIdx.x; b < blocks; b += (int64_t)gridDim.x * KB) {
         const int64_t q = map[b];
+        Pack<TO, NN> o;
+        if (q >= 0) {
+            const Pack<double, NN> s = ld<double, NN>(src + q * NN);
 #pragma unroll
-        for (int t = 0; t < NN; t++) dst[b * NN + t] = q >= 0 ? (TO)src[q * NN + t] : TO(0);
+            for (int t = 0; t < NN; t++) o.v[t] = (TO)s.v[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < NN; t++) o.v[t] = TO(0);
+        }
+        st<TO, NN>(dst + b * NN, o);
     }
 }
 template <int N, typename TO>
