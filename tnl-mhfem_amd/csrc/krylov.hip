@@ -2404,8 +2404,8 @@ struct Solve {
     // which widen()s first; one that halted after the copy is still
     // launched and returns on its own flags.  The list goes to k.active in
     // stream order, after the kernels queued with the previous list; each
-    // host list stays alive until the solve ends (the stream has read it
-    // by then).  A captured graph keeps the grids of its capture: the graph
+    // host list stays alive until the stream has read it (a synchronising
+    // flag read, or the end of the solve).  A captured graph keeps the grids of its capture: the graph
     // step replays its graph only while the launches are not narrowed.
     struct Narrowed {
         Solve &S;
@@ -2417,6 +2417,8 @@ struct Solve {
             if (!lists.empty()) (void)hipStreamSynchronize(S.c.stream);
             S.widen();
         }
+        // after a synchronising flag read: every copy queued so far is done
+        void synchronized() { lists.clear(); }
         void update()
         {
             const Krylov &k = S.k;
@@ -2827,6 +2829,7 @@ struct Solve {
             }
             remember();
             poll();  // the batch still queued ends; the latest flags decide
+            narrowed.synchronized();
             narrowed.update();
         }
         int it = 0;
