@@ -333,17 +333,24 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 // loads, which leave the caches to the gathered x (whole time step of the
 // mixed-precision solve 36.9 ms; 37.9 ms at PF = 6, 38.7 ms at PF = 8,
 // 39.6 ms at PF = 3); FP64 with PF = 8 and ordinary loads (isolated product
-// 169 us; 183 us with streaming loads).
+// 169 us; 183 us with streaming loads).  The FP32 product also loads the
+// column indices of blocks PF and PF+1 up front; those two blocks then wait
+// for one load each (value and gather issued together) instead of an index
+// and then a gather, one block at a time (the empty asm keeps the compiler
+// from hoisting their loads above the first blocks' products, which would
+// hold both sets of registers: 46-52 VGPRs, 8 waves per SIMD; 2 % more time
+// steps per second at n = 2 on 64^3).
 template <int N, typename T>
 __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
 {
     constexpr bool fp32 = sizeof(T) == 4;
     constexpr int PF = N <= 2 ? (fp32 ? 4 : 8) : 3;
+    constexpr int P2 = fp32 && N <= 2 ? 2 : 0;
     const int64_t base = A.slice_ptr[E >> 6] + (E & 63);
     const int w = A.slice_width[E >> 6];
-    int col[PF];
+    int col[PF + P2];
 #pragma unroll
-    for (int k = 0; k < PF; k++)
+    for (int k = 0; k < PF + P2; k++)
         if (k < w) {
             if constexpr (fp32) col[k] = __builtin_nontemporal_load(A.col + base + (int64_t)k * 64);
             else col[k] = A.col[base + (int64_t)k * 64];
@@ -375,7 +382,20 @@ __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *_
 #pragma unroll
                 for (int j = 0; j < N; j++) y.v[i] += b[k].v[i * N + j] * xf[k].v[j];
         }
-    for (int k = PF; k < w; k++) {
+#pragma unroll
+    for (int k = PF; k < PF + P2; k++) {
+        asm volatile("" ::: "memory");
+        if (k < w) {
+            const int64_t q = base + (int64_t)k * 64;
+            const Pack<T, N * N> bk = ld_stream<T, N * N>(A.val + q * (N * N));
+            const Pack<T, N> xk = ld<T, N>(x + (int64_t)col[k] * N);
+#pragma unroll
+            for (int i = 0; i < N; i++)
+#pragma unroll
+                for (int j = 0; j < N; j++) y.v[i] += bk.v[i * N + j] * xk.v[j];
+        }
+    }
+    for (int k = PF + P2; k < w; k++) {
         const int64_t q = base + (int64_t)k * 64;
         const Pack<T, N * N> bk = ld<T, N * N>(A.val + q * (N * N));
         const Pack<T, N> xk = ld<T, N>(x + (int64_t)A.col[q] * N);
