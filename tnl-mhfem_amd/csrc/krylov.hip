@@ -334,12 +334,13 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 // mixed-precision solve 36.9 ms; 37.9 ms at PF = 6, 38.7 ms at PF = 8,
 // 39.6 ms at PF = 3); FP64 with PF = 8 and ordinary loads (isolated product
 // 169 us; 183 us with streaming loads).  The FP32 product also loads the
-// column indices of blocks PF and PF+1 up front; those two blocks then wait
-// for one load each (value and gather issued together) instead of an index
-// and then a gather, one block at a time (the empty asm keeps the compiler
-// from hoisting their loads above the first blocks' products, which would
-// hold both sets of registers: 46-52 VGPRs, 8 waves per SIMD; 2 % more time
-// steps per second at n = 2 on 64^3).
+// column indices of blocks PF and PF+1 up front, and those two blocks' values
+// and gathers as one second batch after the first blocks' products (the
+// empty asm keeps the compiler from hoisting them above those products):
+// 58-64 VGPRs, 8 waves per SIMD, and one round of loads for the two instead
+// of an index and then a gather for each in turn (3 % more time steps per
+// second at n = 2 on 64^3; issuing the second batch with the first, 56-62
+// VGPRs, was 2 % slower).
 template <int N, typename T>
 __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
 {
@@ -382,18 +383,24 @@ __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *_
 #pragma unroll
                 for (int j = 0; j < N; j++) y.v[i] += b[k].v[i * N + j] * xf[k].v[j];
         }
-#pragma unroll
-    for (int k = PF; k < PF + P2; k++) {
+    if constexpr (P2 > 0) {
         asm volatile("" ::: "memory");
-        if (k < w) {
-            const int64_t q = base + (int64_t)k * 64;
-            const Pack<T, N * N> bk = ld_stream<T, N * N>(A.val + q * (N * N));
-            const Pack<T, N> xk = ld<T, N>(x + (int64_t)col[k] * N);
+        Pack<T, N * N> b2[P2 > 0 ? P2 : 1];
+        Pack<T, N> x2[P2 > 0 ? P2 : 1];
 #pragma unroll
-            for (int i = 0; i < N; i++)
+        for (int k = 0; k < P2; k++)
+            if (PF + k < w) {
+                b2[k] = ld_stream<T, N * N>(A.val + (base + (int64_t)(PF + k) * 64) * (N * N));
+                x2[k] = ld<T, N>(x + (int64_t)col[PF + k] * N);
+            }
 #pragma unroll
-                for (int j = 0; j < N; j++) y.v[i] += bk.v[i * N + j] * xk.v[j];
-        }
+        for (int k = 0; k < P2; k++)
+            if (PF + k < w) {
+#pragma unroll
+                for (int i = 0; i < N; i++)
+#pragma unroll
+                    for (int j = 0; j < N; j++) y.v[i] += b2[k].v[i * N + j] * x2[k].v[j];
+            }
     }
     for (int k = PF + P2; k < w; k++) {
         const int64_t q = base + (int64_t)k * 64;
