@@ -88,10 +88,11 @@ def block_sell_bytes(n, faces, blocks_per_face, fp32, dot_operands=0.0):
     """Bytes the block-SELL SpMV must move for the same product: per stored
     off-diagonal block n*n values and one column index, per face the n(n-1)
     off-diagonal entries of its own block, x read once, y written once, and
-    `dot_operands` vectors the fused dot products read.  The iteration's two
-    launches fuse their dots into the product: the FP64 iteration's read one
-    more vector each (r* and s); the merged FP32 iteration's read r* and
-    s, r* (1.5 on average over the alternating samples)."""
+    `dot_operands` vectors the fused dot products read beyond x.  The
+    iteration's two launches fuse their dots into the product; the second
+    product's operand s is its own input x, read once: the FP64 iteration's
+    read r* and nothing more (0.5 on average over the alternating samples),
+    the merged FP32 iteration's r* in both (1)."""
     w = 4 if fp32 else 8
     rows = n * faces
     return int(blocks_per_face * faces * (n * n * w + 4) + faces * n * (n - 1) * w + 2 * rows * w
@@ -573,8 +574,8 @@ def main():
     csr = spmv_bytes(rows, nnz, True)
     # the timed launches are the iteration's SpMV kernels with their fused
     # dot products, sampled alternately from its two products
-    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True, dot_operands=1.5)
-    fmt64 = block_sell_bytes(pr.n, faces, blocks_per_face, False, dot_operands=1.0)
+    fmt = block_sell_bytes(pr.n, faces, blocks_per_face, True, dot_operands=1.0)
+    fmt64 = block_sell_bytes(pr.n, faces, blocks_per_face, False, dot_operands=0.5)
     # priced on the bytes the block-SELL kernel must move; the CSR count of
     # the same product is reported beside it
     achieved = fmt / (spmv_ms * 1e-3) / 1e9

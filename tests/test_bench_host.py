@@ -30,7 +30,8 @@ def test_spmv_byte_counts():
     assert got == 6 * faces * (16 + 4) + faces * 2 * 4 + 2 * 2 * faces * 4
     # the iteration's launches also read the fused dot products' operands
     assert bench.block_sell_bytes(2, faces, 6.0, True, dot_operands=1.0) == got + 2 * faces * 4
-    assert bench.block_sell_bytes(2, faces, 6.0, True, dot_operands=1.5) == got + 3 * faces * 4
+    assert bench.block_sell_bytes(2, faces, 6.0, False, dot_operands=0.5) == (
+        bench.block_sell_bytes(2, faces, 6.0, False) + faces * 8)
     assert bench.block_sell_bytes(2, faces, 6.0, False) == 6 * faces * (32 + 4) + faces * 2 * 8 + 2 * 2 * faces * 8
 
 

@@ -342,7 +342,8 @@ __global__ void k_init_scalars(double *sc, int nseg, double tol, double maxit)
 // second at n = 2 on 64^3; issuing the second batch with the first, 56-62
 // VGPRs, was 2 % slower).
 template <int N, typename T>
-__device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
+__device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E,
+                                                Pack<T, N> &xe_out)
 {
     constexpr bool fp32 = sizeof(T) == 4;
     constexpr int PF = N <= 2 ? (fp32 ? 4 : 8) : 3;
@@ -357,6 +358,7 @@ __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *_
             else col[k] = A.col[base + (int64_t)k * 64];
         }
     const Pack<T, N> xe = ld<T, N>(x + (int64_t)E * N);
+    xe_out = xe;
     Pack<T, N * N> b[PF];
     Pack<T, N> xf[PF];
 #pragma unroll
@@ -414,6 +416,14 @@ __device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *_
     return y;
 }
 
+// the same without the face's own x values
+template <int N, typename T>
+__device__ __forceinline__ Pack<T, N> sell_face(const SellView<T> &A, const T *__restrict__ x, int E)
+{
+    Pack<T, N> xe;
+    return sell_face<N>(A, x, E, xe);
+}
+
 // launch helper: f(std::integral_constant<int, n>) for the supported n
 template <typename F>
 static void with_n(int n, F &&f)
@@ -445,16 +455,21 @@ __global__ void __launch_bounds__(BS) k_sspmv(SellView<T> A, Seg sg, int mode, c
 {
     SEG_BLOCK_PROLOGUE(sg, mode);
     double acc[ND > 0 ? ND : 1] = {};
+    // the iteration's second product dots with its own input (z == x): the
+    // face's x values loaded for the product serve as z
+    const bool zx = z == x;
     for (int f = row0_ / N + threadIdx.x; f < row1_ / N; f += BS) {
-        Pack<T, N> v = sell_face<N>(A, x, f);
+        Pack<T, N> xe;
+        Pack<T, N> v = sell_face<N>(A, x, f, xe);
         if constexpr (DIAG) {
             const Pack<T, N> d = ld<T, N>(dinv + (int64_t)f * N);
 #pragma unroll
             for (int i = 0; i < N; i++) v.v[i] *= d.v[i];
         }
         st<T, N>(y + (int64_t)f * N, v);
+        Pack<T, N> zz;
         if constexpr (ND >= 1) {
-            const Pack<T, N> zz = ld<T, N>(z + (int64_t)f * N);
+            zz = zx ? xe : ld<T, N>(z + (int64_t)f * N);
 #pragma unroll
             for (int i = 0; i < N; i++) acc[0] += (double)v.v[i] * (double)zz.v[i];
         }
@@ -463,7 +478,7 @@ __global__ void __launch_bounds__(BS) k_sspmv(SellView<T> A, Seg sg, int mode, c
             for (int i = 0; i < N; i++) acc[1] += (double)v.v[i] * (double)v.v[i];
         }
         if constexpr (ND >= 4) {
-            const Pack<T, N> zz = ld<T, N>(z + (int64_t)f * N), ww = ld<T, N>(w + (int64_t)f * N);
+            const Pack<T, N> ww = ld<T, N>(w + (int64_t)f * N);
 #pragma unroll
             for (int i = 0; i < N; i++) {
                 acc[2] += (double)v.v[i] * (double)ww.v[i];
